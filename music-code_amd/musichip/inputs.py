@@ -1,0 +1,426 @@
+"""Readers for the reference's input files: ctrlfile sections, .atm/.mol
+files, the atom-atom and species-species forcefield files.
+
+Formats follow the reference readers:
+  ctrlfile sections located by tag string     general.F90:55-113, utils filesrchstr
+  atom file  (Atom_Name:, Atom_Symbol:, ...)   atom.F90
+  molecule file (Coord_Info: Listed ...)       molecule.F90:830-926 (principal axes)
+  simulation cell section                      simcell.F90:77-225
+  atom_atom_file LJ lines                      lj.F90:88-262 (lj_init), :900-981 (mix)
+  spc_spc_file lines                           ssdriver.F90:81-253
+  mapmaker section                             mapmaker.F90:69-312
+Only the subset needed by the GCMC energy hot path is read: LJ / EWALD
+pair models, rigid molecules, orthorhombic cells.
+"""
+import os
+import re
+
+import numpy as np
+
+from . import constants as C
+
+
+def _strip(line):
+    return line.split("#", 1)[0].strip()
+
+
+def _lines(path):
+    with open(path, errors="replace") as f:
+        return f.read().splitlines()
+
+
+def find_section(lines, tag):
+    """Index of the line after the first line containing tag (filesrchstr)."""
+    for i, l in enumerate(lines):
+        if tag.lower() in l.lower():
+            return i + 1
+    raise KeyError("section '%s' not found" % tag)
+
+
+def _nonblank(lines, i):
+    while i < len(lines) and _strip(lines[i]) == "":
+        i += 1
+    return i
+
+
+def _ints(s):
+    return [int(x) for x in re.split(r"[,\s]+", _strip(s)) if x]
+
+
+def _floats(s):
+    return [float(x) for x in re.split(r"[,\s]+", _strip(s)) if x]
+
+
+def find_file(name, envvar, base):
+    """Resolve a reference input file: $ENVVAR/name, base/name, then the
+    shared potentials tree next to base."""
+    cands = []
+    d = os.environ.get(envvar)
+    if d:
+        cands.append(os.path.join(d, name))
+    cands.append(os.path.join(base, name))
+    sub = {"ATOMSDIR": "atoms", "MOLSDIR": "molecules", "PMAPDIR": "pmaps",
+           "EMAPDIR": "pmaps"}.get(envvar, "")
+    p = base
+    for _ in range(3):
+        cands.append(os.path.join(p, "potentials", sub, name))
+        p = os.path.dirname(p)
+    for c in cands:
+        if os.path.exists(c):
+            return c
+    raise FileNotFoundError("%s (looked in %s)" % (name, cands))
+
+
+# ----------------------------------------------------------------- atoms
+def read_atom(path):
+    d = {"sscharge": 0.0, "mass": 0.0}
+    for l in _lines(path):
+        s = _strip(l)
+        if ":" not in s:
+            continue
+        k, v = [x.strip() for x in s.split(":", 1)]
+        if k == "Atom_Name":
+            d["name"] = v
+        elif k == "Atom_Symbol":
+            d["symbol"] = v
+        elif k == "Atom_Mass":
+            d["mass"] = float(v)
+        elif k == "Atom_SS_Charge":
+            d["sscharge"] = float(v)
+    return d
+
+
+# ------------------------------------------------------------- molecules
+def read_molecule(path):
+    """Rigid 'Listed Cartesian' molecule file.  Returns dict with names,
+    xyz (as listed), charges, transform flag and optional fundcell."""
+    L = _lines(path)
+    mol = {"fundcell": None}
+    i = 0
+    while i < len(L):
+        s = _strip(L[i])
+        if s.startswith("Molecule_Name:"):
+            mol["name"] = s.split(":", 1)[1].split()[0]
+        elif s.startswith("Coord_Info:"):
+            info = s.split(":", 1)[1].split()
+            mol["coordinfo"] = info
+            mol["transform"] = not any(w.lower() == "notransform" for w in info)
+            i += 1
+            n = _ints(L[i])[0]
+            names, xyz, q = [], [], []
+            for j in range(n):
+                f = _strip(L[i + 1 + j]).split()
+                xyz.append([float(f[1]), float(f[2]), float(f[3])])
+                names.append(f[4])
+                q.append(float(f[5]) if len(f) > 5 else 0.0)
+            mol["atoms"] = names
+            mol["xyz"] = np.array(xyz, dtype=np.float64)
+            mol["q"] = np.array(q, dtype=np.float64)
+            i += n
+        elif s.startswith("Fundcell_Info:"):
+            ell = _floats(L[i + 1])
+            ang = _floats(L[i + 2])
+            org = _floats(L[i + 3])
+            eff = _floats(L[i + 4])
+            mol["fundcell"] = dict(ell=np.array(ell), angles=np.array(ang),
+                                   origin=np.array(org), eff=np.array(eff))
+            i += 4
+        i += 1
+    return mol
+
+
+def principal_axes(xyz, masses):
+    """molecule_genprincipleaxes (molecule.F90:830-926): shift to the centre
+    of mass computed with INTEGER-truncated masses (molecule.F90:833) and
+    rotate onto the principal axes.  Eigenvectors come from numpy (the
+    reference uses EISPACK tred2/tql2); axis order ascending eigenvalue."""
+    m = np.floor(np.asarray(masses, dtype=np.float64))
+    com = (xyz * m[:, None]).sum(0) / m.sum()
+    r = xyz - com
+    if len(xyz) == 1:
+        return r
+    I = np.zeros((3, 3))
+    for mi, v in zip(m, r):
+        I += mi * (np.dot(v, v) * np.eye(3) - np.outer(v, v))
+    w, V = np.linalg.eigh(I)
+    out = r @ V
+    out[np.abs(out) < 1e-12] = 0.0
+    return out
+
+
+# ------------------------------------------------------------- ctrlfile
+def read_ctrlfile(path):
+    """Parse the ctrlfile sections used by music_mapmaker / music_gcmc."""
+    L = _lines(path)
+    base = os.path.dirname(os.path.abspath(path))
+    c = {"path": path, "base": base}
+    i = find_section(L, "General Information")
+    g = [l for l in L[i:i + 12]]
+    vals = [_strip(x) for x in g]
+    c["general"] = dict(desc=vals[0], niterations=int(vals[1].split()[0]),
+                        iprint=int(vals[2].split()[0]), icrash=int(vals[3].split()[0]),
+                        iconfig=int(vals[4].split()[0]), simstart=int(vals[5].split()[0]),
+                        iseed=int(vals[6].split()[0]))
+    # atom types
+    i = find_section(L, "Atomic Types")
+    n = _ints(L[i])[0]
+    i += 1
+    atoms = []
+    for _ in range(n):
+        i = _nonblank(L, i)
+        name = _strip(L[i]).split()[0]
+        fname = _strip(L[i + 1]).split()[0]
+        atoms.append(dict(name=name, file=fname))
+        i += 2
+    c["atoms"] = atoms
+    i = find_section(L, "Molecule Types")
+    n = _ints(L[i])[0]
+    i += 1
+    mols = []
+    for _ in range(n):
+        i = _nonblank(L, i)
+        name = _strip(L[i]).split()[0]
+        fname = _strip(L[i + 1]).split()[0]
+        mols.append(dict(name=name, file=fname))
+        i += 2
+    c["molecules"] = mols
+    # simulation cell
+    i = find_section(L, "Simulation Cell Information")
+    first = _strip(L[i]).split()[0]
+    if first.upper() == "NULL":
+        ell = _floats(L[i + 1])
+        ang = _floats(L[i + 2])
+        org = _floats(L[i + 3])
+        eff = _floats(L[i + 4])
+        nxyz = _ints(L[i + 5])
+        pbc = _ints(L[i + 6])
+        c["simcell"] = dict(fundcell_source=None,
+                            fund=dict(ell=np.array(ell), angles=np.array(ang),
+                                      origin=np.array(org), eff=np.array(eff)),
+                            nxyz=nxyz, pbc=pbc)
+    else:
+        nxyz = _ints(L[i + 1])
+        pbc = _ints(L[i + 2])
+        c["simcell"] = dict(fundcell_source=first, fund=None, nxyz=nxyz, pbc=pbc)
+    # forcefield
+    i = find_section(L, "Forcefield Information")
+    ff = [_strip(x).split()[0] for x in L[i:i + 5]]
+    c["forcefield"] = dict(kind=ff[0], storage=ff[1], atom_atom_file=ff[2],
+                           spc_spc_file=ff[3], intra_file=ff[4])
+    # optional sections
+    try:
+        i = find_section(L, "Mapmaker Information")
+        n = _ints(L[i])[0]
+        i += 1
+        maps = []
+        for _ in range(n):
+            i = _nonblank(L, i)
+            mapsorb = _strip(L[i]).split()[0]
+            probe = _strip(L[i + 1]).split()[0]
+            inter = _strip(L[i + 2]).split()
+            grid = float(_strip(L[i + 3]).split()[0])
+            hicut = float(_strip(L[i + 4]).split()[0])
+            fname = _strip(L[i + 5]).split()[0]
+            maps.append(dict(mapsorb=mapsorb, probe=probe, int=inter[0], model=inter[1],
+                             gridspace=grid, hipotcut=hicut, filename=fname))
+            i += 6
+        c["mapmaker"] = maps
+    except KeyError:
+        c["mapmaker"] = []
+    try:
+        c["gcmc"] = _read_gcmc_section(L)
+    except KeyError:
+        c["gcmc"] = None
+    return c
+
+
+def _read_gcmc_section(L):
+    i = find_section(L, "GCMC Information")
+    v = [_strip(x) for x in L[i:i + 6]]
+    g = dict(niterations=int(v[0].split()[0]), tk=float(v[1].split()[0]),
+             npressures=int(v[3].split()[0]), blocksize=int(v[4].split()[0]),
+             nspc=int(v[5].split()[0]))
+    i += 6
+    spcs = []
+    for _ in range(g["nspc"]):
+        while i < len(L) and (_strip(L[i]) == "" or set(_strip(L[i])) <= set("-")):
+            i += 1
+        name = _strip(L[i]).split()[0]
+        fug = _floats(L[i + 1].replace("-", " ") if False else L[i + 1])
+        smap = _strip(L[i + 2]).split()[0]
+        nmoves = _ints(L[i + 3])[0]
+        wts = _floats(L[i + 4])[:nmoves]
+        i += 5
+        moves = []
+        for m in range(nmoves):
+            mt = _strip(L[i]).split()[0].upper()
+            i += 1
+            mv = dict(type=mt, weight=wts[m])
+            if mt in ("RTRANSLATE", "RROTATE"):
+                p = _floats(L[i])
+                mv["delta"] = p[0]
+                mv["adjust"] = int(p[1]) if len(p) > 1 else 0
+                i += 1
+            moves.append(mv)
+        spcs.append(dict(name=name, fugacity=fug, smap=smap, moves=moves))
+    g["species"] = spcs
+    return g
+
+
+# ------------------------------------------------------------ forcefield
+def _kv(tok):
+    if "@" in tok:
+        k, v = tok.split("@", 1)
+        return k.upper(), v
+    return tok.upper(), None
+
+
+def read_atom_atom(path):
+    """atom_atom_file -> {(a1,a2): dict(model, sig, eps, hicut2, locut2, off)}
+    following lj_init (lj.F90:140-262) defaults: locut = 1e-3, hicut = 13."""
+    out = {}
+    for l in _lines(path):
+        s = _strip(l)
+        f = s.split()
+        if len(f) < 3:
+            continue
+        a1, a2, model = f[0], f[1], f[2].upper()
+        d = dict(model=model, off=False, line=s)
+        if model == "LJ":
+            d.update(sig=None, eps=None, locut=C.LOW_DIST_CUTOFF,
+                     locut2=C.LOW_DIST_CUTOFF * C.LOW_DIST_CUTOFF,
+                     hicut=C.HIGH_DIST_CUTOFF, hicut2=C.HIGH_DIST_CUTOFF * C.HIGH_DIST_CUTOFF,
+                     mix=False)
+            for tok in f[3:]:
+                k, v = _kv(tok)
+                if k == "OFF":
+                    d.update(off=True, hicut=0.0, hicut2=0.0)
+                elif k == "SIG":
+                    if "MIX" in v.upper():
+                        d["mix"] = True
+                    else:
+                        d["sig"] = float(v)
+                elif k == "EPS":
+                    if "MIX" not in v.upper():
+                        d["eps"] = float(v)
+                elif k in ("HICUT", "TRUNC"):
+                    d["hicut"] = float(v)
+                    d["hicut2"] = d["hicut"] ** 2
+                elif k in ("LOCUT", "LOWCUT"):
+                    d["locut"] = float(v)
+                    d["locut2"] = d["locut"] ** 2
+            if not d["off"] and d["sig"] is not None and d["eps"] is not None:
+                d["A"], d["B"] = lj_AB(d["sig"], d["eps"])
+        out[(a1, a2)] = d
+        out[(a2, a1)] = d
+    return out
+
+
+def lj_AB(sig, eps):
+    """lj_calc_AB (lj.F90:860-869): B = 4 eps sig^6 kB, A = B sig^6."""
+    s6 = sig ** 6
+    B = 4.0 * eps * s6 * C.KCALMOLE_KB
+    return B * s6, B
+
+
+def read_spc_spc(path):
+    """spc_spc_file -> {(s1,s2,'NCOUL'|'COUL'): dict(kind, tokens)}."""
+    out = {}
+    for l in _lines(path):
+        s = _strip(l)
+        f = s.split()
+        if len(f) < 3 or f[2].upper() not in ("NCOUL", "COUL"):
+            continue
+        s1, s2, ic = f[0], f[1], f[2].upper()
+        kind = f[3].upper() if len(f) > 3 else "OFF"
+        d = dict(kind=kind, tokens=[t.upper() for t in f[3:]], line=s)
+        if kind == "SUM" and "EWALD" in d["tokens"]:
+            e = dict(kmax=None, kappa=None, locut=None, sfactor=False)
+            for t in f[3:]:
+                k, v = _kv(t)
+                if k == "KMAX":
+                    e["kmax"] = int(v)
+                elif k == "KAPPA":
+                    e["kappa"] = float(v)
+                elif k == "LOCUT":
+                    e["locut"] = float(v)
+                elif k == "SFACTOR":
+                    e["sfactor"] = True
+            d["ewald"] = e
+        out[(s1, s2, ic)] = d
+        out[(s2, s1, ic)] = d
+    return out
+
+
+# ---------------------------------------------------------------- system
+def load_system(ctrl_path):
+    """Read a ctrlfile and everything it references.  Returns a dict with
+    atom types, molecules (with replicated framework coordinates), the
+    simulation cell (fund/full edges), forcefield tables and map jobs."""
+    c = read_ctrlfile(ctrl_path)
+    base = c["base"]
+    atypes = []
+    for a in c["atoms"]:
+        d = read_atom(find_file(a["file"], "ATOMSDIR", base))
+        d["name"] = a["name"]
+        atypes.append(d)
+    aidx = {a["name"]: i for i, a in enumerate(atypes)}
+    mols = {}
+    for m in c["molecules"]:
+        d = read_molecule(find_file(m["file"], "MOLSDIR", base))
+        d["name"] = m["name"]
+        d["type"] = np.array([aidx[n] for n in d["atoms"]], dtype=np.int32)
+        masses = [atypes[t]["mass"] for t in d["type"]]
+        d["mass"] = float(np.sum(masses))
+        if d["transform"] and len(d["xyz"]) > 1:
+            d["ref"] = principal_axes(d["xyz"], masses)
+        else:
+            d["ref"] = d["xyz"].copy()
+        mols[m["name"]] = d
+    sc = c["simcell"]
+    if sc["fundcell_source"]:
+        fc = mols[sc["fundcell_source"]]["fundcell"]
+    else:
+        fc = sc["fund"]
+    if np.any(np.abs(np.asarray(fc["angles"]) - 90.0) > 1e-9):
+        raise NotImplementedError("only orthorhombic cells are supported")
+    fund = np.asarray(fc["ell"], dtype=np.float64)
+    nxyz = np.asarray(sc["nxyz"], dtype=np.int64)
+    full = fund * nxyz
+    # replicate the framework (simcell.F90:200-216 -> molecule_expand,
+    # molecule.F90:637-700): copies along x, then y, then z
+    if sc["fundcell_source"]:
+        fw = mols[sc["fundcell_source"]]
+        xyz, typ, q = fw["xyz"], fw["type"], fw["q"]
+        for ax in range(3):
+            n = int(nxyz[ax])
+            off = np.zeros(3)
+            off[ax] = fund[ax]
+            xyz = np.concatenate([xyz + off * i for i in range(n)])
+            typ = np.concatenate([typ] * n)
+            q = np.concatenate([q] * n)
+        fw["full_xyz"], fw["full_type"], fw["full_q"] = xyz, typ, q
+    ff = c["forcefield"]
+    aa = read_atom_atom(find_file(ff["atom_atom_file"], "NONE", base))
+    ss = read_spc_spc(find_file(ff["spc_spc_file"], "NONE", base))
+    return dict(ctrl=c, atypes=atypes, aidx=aidx, mols=mols, fund_ell=fund, full_ell=full,
+                nxyz=nxyz, aa=aa, ss=ss, seed=c["general"]["iseed"])
+
+
+def lj_table_for_probe(system, probe_type, ntypes=None):
+    """Per framework-atom-type LJ parameters against one probe atom type:
+    arrays A, B, hicut2, locut2 and an 'on' mask (OFF pairs -> on=0)."""
+    at = system["atypes"]
+    nt = len(at) if ntypes is None else ntypes
+    A = np.zeros(nt)
+    B = np.zeros(nt)
+    hc2 = np.zeros(nt)
+    lc2 = np.zeros(nt)
+    on = np.zeros(nt, dtype=np.uint8)
+    pn = at[probe_type]["name"]
+    for t in range(nt):
+        d = system["aa"].get((at[t]["name"], pn))
+        if d is None or d["model"] != "LJ" or d["off"]:
+            continue
+        A[t], B[t], hc2[t], lc2[t], on[t] = d["A"], d["B"], d["hicut2"], d["locut2"], 1
+    return A, B, hc2, lc2, on

@@ -1,0 +1,227 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The product path (music-code_amd/) never
+imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_ll = ctypes.c_longlong
+_vp = ctypes.c_void_p
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp) if a is not None else None
+
+
+def lib(native=False):
+    """Load (building once if absent) liboracle.so."""
+    global _LIB
+    if _LIB is not None and not native:
+        return _LIB
+    name = "liboracle_native.so" if native else "liboracle.so"
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        subprocess.check_call(["make", "-s", "-C", HERE, "native" if native else "all"])
+    L = ctypes.CDLL(path)
+    L.mo_rranf_n.argtypes = [ctypes.c_int32, _i, _vp, _vp, _vp]
+    L.mo_erfc.argtypes = [_d]
+    L.mo_erfc.restype = _d
+    L.mo_lj_node.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.mo_lj_node.restype = _i
+    L.mo_ewald_sfactor.argtypes = [_i, _vp, _vp, _vp, _d, _i, _d, _i, _vp, _vp, _vp, _vp, _vp]
+    L.mo_ewald_sfactor.restype = _i
+    L.mo_ewald_node.argtypes = [_vp, _i, _vp, _vp, _d, _d, _i, _vp, _vp, _vp, _vp, _vp]
+    L.mo_ewald_node.restype = _i
+    L.mo_build_nodes.argtypes = [_i, _vp, _vp, _vp, _ll, _ll, _vp]
+    L.mo_compact.argtypes = [_vp, _vp, _d, _vp, _vp, _vp]
+    L.mo_compact.restype = _ll
+    L.mo_map_write.argtypes = [ctypes.c_char_p, _i, _vp, ctypes.c_float, _vp, _ll, _vp, _vp]
+    L.mo_map_write.restype = _i
+    L.mo_map_read_sizes.argtypes = [ctypes.c_char_p, _vp, _vp, _vp, _vp]
+    L.mo_map_read_sizes.restype = _i
+    L.mo_map_read.argtypes = [ctypes.c_char_p, _vp, _vp]
+    L.mo_map_read.restype = _i
+    L.mo_pothermite.argtypes = [_vp, _vp, _vp, _vp, _ll, _vp, _vp]
+    L.mo_pothermite.restype = _i
+    L.mo_forcehermite.argtypes = [_vp, _vp, _vp, _vp, _ll, _vp, _vp]
+    L.mo_forcehermite.restype = _i
+    L.mo_maptouc.argtypes = [_vp, _vp, _vp]
+    if not native:
+        _LIB = L
+    return L
+
+
+def rranf(seed, n):
+    out = np.zeros(n)
+    ix = ctypes.c_int32()
+    iy = ctypes.c_int32()
+    lib().mo_rranf_n(seed, n, _p(out), ctypes.byref(ix), ctypes.byref(iy))
+    return out, ix.value, iy.value
+
+
+def erfc(x):
+    return np.array([lib().mo_erfc(float(v)) for v in np.atleast_1d(x)])
+
+
+class _Framework(ctypes.Structure):
+    _fields_ = [("natoms", _i), ("xyz", _vp), ("type", _vp), ("q", _vp),
+                ("A", _vp), ("B", _vp), ("hicut2", _vp), ("locut2", _vp),
+                ("bA", _vp), ("brho", _vp), ("bC", _vp), ("bD", _vp),
+                ("charge1", _d), ("kappa", _d), ("nk", _i),
+                ("k", _vp), ("s", _vp), ("si", _vp), ("full_ell", _d * 3)]
+
+
+def ewald_sfactor(xyz, q, ell, kappa_param, kmax, locut, nkmax=50000):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    ell = np.ascontiguousarray(ell, dtype=np.float64)
+    ncand = (2 * kmax + 1) ** 3
+    k = np.zeros((ncand, 3))
+    s = np.zeros(ncand)
+    si = np.zeros(ncand)
+    kidx = np.zeros((ncand, 3), dtype=np.int32)
+    kap = ctypes.c_double()
+    nk = lib().mo_ewald_sfactor(len(q), _p(xyz), _p(q), _p(ell), kappa_param, kmax, locut,
+                                nkmax, _p(k), _p(s), _p(si), _p(kidx), ctypes.byref(kap))
+    if nk < 0:
+        raise RuntimeError("too many k vectors")
+    return dict(nk=nk, k=k[:nk].copy(), s=s[:nk].copy(), si=si[:nk].copy(),
+                kidx=kidx[:nk].copy(), kappa=kap.value)
+
+
+def build_nodes(kind, fw, npnts, step, m0=0, m1=None):
+    """Evaluate grid nodes m0..m1 (config_makegrid order) -> [n, 8] FP64.
+
+    fw: dict with xyz, type, q, A, B, hicut2, locut2 (per type), charge1,
+    kappa, k, s, si, full_ell (and bA, brho, bC, bD for Buckingham)."""
+    npnts = np.ascontiguousarray(npnts, dtype=np.int32)
+    step = np.ascontiguousarray(step, dtype=np.float64)
+    if m1 is None:
+        m1 = int(np.prod(npnts.astype(np.int64)))
+    keep = {}
+
+    def arr(name, dt=np.float64):
+        v = fw.get(name)
+        if v is None:
+            return None
+        a = np.ascontiguousarray(v, dtype=dt)
+        keep[name] = a
+        return _p(a)
+
+    st = _Framework()
+    st.natoms = len(fw["xyz"])
+    st.xyz = arr("xyz")
+    st.type = arr("type", np.int32)
+    st.q = arr("q")
+    st.A = arr("A")
+    st.B = arr("B")
+    st.hicut2 = arr("hicut2")
+    st.locut2 = arr("locut2")
+    st.bA = arr("bA")
+    st.brho = arr("brho")
+    st.bC = arr("bC")
+    st.bD = arr("bD")
+    st.charge1 = float(fw.get("charge1", 1.0))
+    st.kappa = float(fw.get("kappa", 0.0))
+    st.nk = int(len(fw["s"])) if fw.get("s") is not None else 0
+    st.k = arr("k")
+    st.s = arr("s")
+    st.si = arr("si")
+    for i in range(3):
+        st.full_ell[i] = float(fw["full_ell"][i])
+    out = np.zeros((m1 - m0, 8))
+    lib().mo_build_nodes(kind, ctypes.byref(st), _p(npnts), _p(step), m0, m1, _p(out))
+    return out
+
+
+def compact(npnts, r, cutoff):
+    npnts = np.ascontiguousarray(npnts, dtype=np.int32)
+    r = np.ascontiguousarray(r, dtype=np.float32)
+    n = int(np.prod(npnts.astype(np.int64)))
+    ptr = np.zeros(n, dtype=np.int32)
+    src = np.zeros(n, dtype=np.int64)
+    counts = np.zeros(4, dtype=np.int64)
+    nacc = lib().mo_compact(_p(npnts), _p(r), float(cutoff), _p(ptr), _p(src), _p(counts))
+    return ptr, src[:nacc].copy(), counts
+
+
+class _ZAtom(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 26), ("symbol", ctypes.c_char * 3),
+                ("charge", ctypes.c_float), ("A12", ctypes.c_float), ("B6", ctypes.c_float),
+                ("C2", ctypes.c_float), ("D0", ctypes.c_float), ("rcuthi", ctypes.c_float),
+                ("rcutlo", ctypes.c_float)]
+
+
+def map_write(path, zatoms, pcut, npnts, ptr, vals8):
+    """zatoms: list of dicts(name, symbol, charge, A12, B6, C2, D0, rcuthi, rcutlo)."""
+    arr = (_ZAtom * max(1, len(zatoms)))()
+    for i, z in enumerate(zatoms):
+        arr[i].name = z["name"].encode()[:25]
+        arr[i].symbol = z["symbol"].encode()[:2]
+        for f in ("charge", "A12", "B6", "C2", "D0", "rcuthi", "rcutlo"):
+            setattr(arr[i], f, float(z.get(f, 0.0)))
+    npnts = np.ascontiguousarray(npnts, dtype=np.int32)
+    ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+    vals8 = np.ascontiguousarray(vals8, dtype=np.float32)
+    nf = vals8.shape[1]
+    rc = lib().mo_map_write(path.encode(), len(zatoms), ctypes.cast(arr, _vp), float(pcut),
+                            _p(npnts), nf, _p(ptr), _p(vals8))
+    if rc:
+        raise IOError("mo_map_write failed")
+
+
+def map_read(path):
+    np3 = np.zeros(3, dtype=np.int32)
+    nf = ctypes.c_longlong()
+    pc = ctypes.c_float()
+    nz = ctypes.c_int()
+    if lib().mo_map_read_sizes(path.encode(), _p(np3), ctypes.byref(nf), ctypes.byref(pc),
+                               ctypes.byref(nz)):
+        raise IOError("bad map header " + path)
+    ptr = np.zeros(int(np.prod(np3.astype(np.int64))), dtype=np.int32)
+    vals = np.zeros((8, nf.value), dtype=np.float32)
+    if lib().mo_map_read(path.encode(), _p(ptr), _p(vals)):
+        raise IOError("bad map body " + path)
+    return dict(npnts=np3, nfsize=nf.value, pcut=pc.value, ptr=ptr, vals=vals)
+
+
+def pothermite(npnts, step, ptr, vals8, coords, fund_ell=None, force=False):
+    """Interpolate at coords [n,3]; if fund_ell given, coords are first folded
+    with simcell_maptouc.  Returns (ok[n], V[n]) or (ok, p[n,4]) if force."""
+    npnts = np.ascontiguousarray(npnts, dtype=np.int32)
+    step = np.ascontiguousarray(step, dtype=np.float64)
+    ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+    vals8 = np.ascontiguousarray(vals8, dtype=np.float32)
+    coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
+    n = len(coords)
+    ok = np.zeros(n, dtype=bool)
+    out = np.zeros((n, 4)) if force else np.zeros(n)
+    L = lib()
+    fe = np.ascontiguousarray(fund_ell, dtype=np.float64) if fund_ell is not None else None
+    c = np.zeros(3)
+    for i in range(n):
+        if fe is not None:
+            L.mo_maptouc(_p(fe), _p(coords[i]), _p(c))
+        else:
+            c[:] = coords[i]
+        if force:
+            p = np.zeros(4)
+            ok[i] = bool(L.mo_forcehermite(_p(npnts), _p(step), _p(ptr), _p(vals8),
+                                           vals8.shape[1], _p(c), _p(p)))
+            out[i] = p
+        else:
+            v = ctypes.c_double()
+            ok[i] = bool(L.mo_pothermite(_p(npnts), _p(step), _p(ptr), _p(vals8),
+                                         vals8.shape[1], _p(c), ctypes.byref(v)))
+            out[i] = v.value
+    return ok, out

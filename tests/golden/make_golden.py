@@ -1,0 +1,202 @@
+"""Generate tests/golden/ref_vectors.json from oracle/_ref.
+
+oracle/_ref/ref_harness is the reference's own Fortran modules (random,
+utils, lj, simcell, ewald, pmap, interpolate) compiled with flang from the
+reference source tree (oracle/Makefile.ref).  This script feeds it seeded
+inputs and stores inputs + outputs as small JSON vectors so that the CPU
+oracle and the HIP library can be checked without the reference present.
+
+Run in the build container (needs /root/reference and flang):
+    make -f oracle/Makefile.ref && python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "music-code_amd"))
+from musichip import inputs  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def run(lines, env=None, cwd=None):
+    r = subprocess.run([HARNESS], input="\n".join(lines) + "\n", capture_output=True,
+                       text=True, env=env, cwd=cwd)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr[-2000:])
+    return r.stdout, r.stderr
+
+
+def f17(*v):
+    return " ".join("%.17g" % float(x) for x in v)
+
+
+def parse_rows(out):
+    rows = []
+    for l in out.splitlines():
+        f = l.split()
+        if not f or f[0] not in ("T", "F"):
+            continue
+        rows.append([f[0] == "T"] + [float(x) for x in f[1:]])
+    return rows
+
+
+def gen_rng():
+    res = {}
+    for seed in (22111971, 1892134, 1001, 3000, 3063, 4999):
+        out, _ = run(["RNG", "%d %d" % (seed, 20000)])
+        v = [float(x) for x in out.split()]
+        res[str(seed)] = dict(first=v[:64], tail_index=19990, tail=v[19990:20000])
+    return res
+
+
+def gen_erfc():
+    xs = list(np.linspace(0.0, 8.0, 41)) + [1e-8, 0.335, 1.23456789, 3.3]
+    out, _ = run(["ERFC", str(len(xs))] + ["%.17g" % x for x in xs])
+    return dict(x=xs, y=[float(v) for v in out.split()])
+
+
+def gen_ljnode():
+    s = inputs.load_system(os.path.join(GOLD, "test012-pmapgen", "test012.ctr"))
+    fw = s["mols"]["Lattice"]
+    probe = s["mols"]["Probe"]
+    A, B, hc2, lc2, on = inputs.lj_table_for_probe(s, int(probe["type"][0]))
+    rng = np.random.default_rng(12012)
+    nodes = rng.uniform(0, 1, (160, 3)) * s["fund_ell"]
+    # a few nodes on top of / very near atoms (locut failures, partial sums)
+    near = fw["full_xyz"][[0, 5, 17, 40]] + rng.normal(0, 0.03, (4, 3))
+    nodes = np.concatenate([nodes, near, fw["full_xyz"][[3, 9]]])
+    lines = ["LJNODE", f17(*s["full_ell"]), str(len(fw["full_xyz"]))]
+    for x, t in zip(fw["full_xyz"], fw["full_type"]):
+        lines.append(f17(*x) + " %d" % t)
+    lines.append(str(len(A)))
+    for t in range(len(A)):
+        lines.append(f17(A[t], B[t], hc2[t], lc2[t]))
+    lines.append(str(len(nodes)))
+    lines += [f17(*p) for p in nodes]
+    out, _ = run(lines)
+    rows = parse_rows(out)
+    return dict(ctrl="test012-pmapgen/test012.ctr", nodes=nodes.tolist(),
+                A=A.tolist(), B=B.tolist(), hicut2=hc2.tolist(), locut2=lc2.tolist(),
+                ok=[r[0] for r in rows], vals=[r[1:] for r in rows])
+
+
+def _ewald_case(xyz, q, ell, line, charge1, nodes):
+    lines = ["EWALD", line, f17(*ell), str(len(q))]
+    for x, qq in zip(xyz, q):
+        lines.append(f17(*x, qq))
+    lines.append(f17(charge1))
+    lines.append(str(len(nodes)))
+    lines += [f17(*p) for p in nodes]
+    out, err = run(lines)
+    nk = None
+    for l in err.splitlines():
+        if "Total number of K vectors" in l:
+            nk = int(l.split()[-1])
+    rows = parse_rows(out)
+    return nk, rows
+
+
+def gen_ewald():
+    s = inputs.load_system(os.path.join(GOLD, "test013-emapgen", "test013.ctr"))
+    fw = s["mols"]["Lattice"]
+    probe = s["mols"]["Probe"]
+    e = s["ss"][("Probe", "Lattice", "COUL")]
+    rng = np.random.default_rng(13013)
+    nodes = rng.uniform(0, 1, (60, 3)) * s["fund_ell"]
+    nodes = np.concatenate([nodes, fw["full_xyz"][[2]] + 1e-4])
+    nk, rows = _ewald_case(fw["full_xyz"], fw["full_q"], s["full_ell"],
+                           e["line"].split(None, 3)[3], probe["q"][0], nodes)
+    case1 = dict(ctrl="test013-emapgen/test013.ctr", nodes=nodes.tolist(), nk=nk,
+                 ok=[r[0] for r in rows], vals=[r[1:] for r in rows])
+    # a denser synthetic charged box with kmax 9 (the charged-map config's kmax)
+    rng = np.random.default_rng(2002)
+    L = np.array([18.0, 18.0, 18.0])
+    n = 150
+    xyz = rng.uniform(0, 1, (n, 3)) * L
+    q = rng.uniform(-1, 1, n)
+    q -= q.mean()
+    nodes = rng.uniform(0, 1, (40, 3)) * L
+    line = "EWALD SFACTOR KMAX@9 KAPPA@4.5 LOCUT@1e-10"
+    nk2, rows2 = _ewald_case(xyz, q, L, line, 1.0, nodes)
+    case2 = dict(ell=L.tolist(), xyz=xyz.tolist(), q=q.tolist(), kappa_param=4.5, kmax=9,
+                 locut=1e-10, charge1=1.0, nodes=nodes.tolist(), nk=nk2,
+                 ok=[r[0] for r in rows2], vals=[r[1:] for r in rows2])
+    return dict(test013=case1, synth=case2)
+
+
+def gen_compact_and_hermite():
+    rng = np.random.default_rng(777)
+    npnts = [13, 11, 9]
+    step = [0.5, 0.55, 0.6]
+    n = int(np.prod(npnts))
+    # smooth field with a few low wells so that neighbours get pulled in
+    m = np.arange(n)
+    x = (m % npnts[0]) * step[0]
+    y = ((m // npnts[0]) % npnts[1]) * step[1]
+    z = (m // (npnts[0] * npnts[1])) * step[2]
+    V = 40.0 * np.cos(0.9 * x) * np.cos(0.8 * y) * np.cos(0.7 * z) + 5.0 * np.sin(0.3 * x + 0.2 * z)
+    vals = np.zeros((n, 8))
+    vals[:, 0] = V
+    vals[:, 1:] = rng.normal(0, 3, (n, 7))
+    cutoff = 10.0
+    zat = [dict(name="Latom", symbol="L", charge=0.5, A12=6569285.0, B6=2563.017, C2=0.0,
+                D0=0.0, rcuthi=5.9, rcutlo=0.1)]
+    with tempfile.TemporaryDirectory() as td:
+        fname = os.path.join(td, "t.map")
+        lines = ["COMPACT", fname, "%d %d %d" % tuple(npnts), f17(cutoff), f17(41.84), "1"]
+        lines.append("%-25s %-2s" % (zat[0]["name"], zat[0]["symbol"]))
+        z0 = zat[0]
+        lines.append(f17(z0["charge"], z0["A12"], z0["B6"], z0["C2"], z0["D0"], z0["rcuthi"],
+                         z0["rcutlo"]))
+        lines += [f17(*v) for v in vals]
+        out, err = run(lines)
+        counts = []
+        for l in err.splitlines():
+            if "Accepted" in l or "Rejected" in l:
+                counts.append(int(l.split()[-1]))
+        data = bytearray(open(fname, "rb").read())
+        # record 6 is zeoatom_types, an uninitialized local written by
+        # mapheader_makeOldHeader (mapheader.F90:183-186,204): its 44 bytes
+        # are stack junk, so they are zeroed before hashing
+        assert data[173:177] == (44).to_bytes(4, "little")
+        data[177:221] = bytes(44)
+        data = bytes(data)
+        # Hermite through the reference's pmap_int on the same file
+        eff = [step[i] * (npnts[i] - 1) for i in range(3)]
+        pts = rng.uniform(-0.5, 1.5, (80, 3)) * np.array(eff)
+        env = dict(os.environ, PMAPDIR=td + "/")
+        hl = ["HERMITE", "t.map", f17(*eff), str(len(pts))] + [f17(*p) for p in pts]
+        hout, _ = run(hl, env=env, cwd=td)
+        hrows = parse_rows(hout)
+    return dict(compact=dict(npnts=npnts, cutoff=cutoff, pcut=41.84, zatoms=zat,
+                             vals=vals.tolist(), counts=counts,
+                             file_sha256=hashlib.sha256(data).hexdigest(),
+                             file_size=len(data)),
+                hermite=dict(npnts=npnts, eff=eff, points=pts.tolist(),
+                             ok=[r[0] for r in hrows], V=[r[1] for r in hrows]))
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        raise SystemExit("build oracle/_ref first: make -f oracle/Makefile.ref")
+    g = dict(source="oracle/_ref/ref_harness (reference Fortran modules compiled with flang)")
+    g["rng"] = gen_rng()
+    g["erfc"] = gen_erfc()
+    g["ljnode"] = gen_ljnode()
+    g["ewald"] = gen_ewald()
+    g.update(gen_compact_and_hermite())
+    with open(os.path.join(GOLD, "ref_vectors.json"), "w") as f:
+        json.dump(g, f)
+    print("wrote", os.path.join(GOLD, "ref_vectors.json"))
+
+
+if __name__ == "__main__":
+    main()
