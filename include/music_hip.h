@@ -1,0 +1,130 @@
+/*
+ * music_hip.h -- C-ABI of libmusic_hip.so, the MI355X-native GCMC energy
+ * hot path of Music (map build, map interpolation, sorbate-sorbate energies,
+ * Ewald structure-factor deltas, device-resident GCMC).
+ *
+ * Plain C types only (pointers + sizes).  Every function returns an int
+ * status: MH_OK (0) or a positive MH_E* code; mh_last_error() gives the
+ * message of the last failure on the calling thread.  Nothing here aborts
+ * the process.  One context per HIP device; a context is used by one host
+ * thread at a time.
+ *
+ * Units: Angstrom, kcal/mol (maps and energies), elementary charge, K.
+ * Cells are orthorhombic with the origin at 0 (the reference's fast path,
+ * fundcell.F90:640-709); other cells are rejected with MH_EINVAL.
+ * Indices passed in are 0-based; Fortran callers subtract 1.
+ */
+#ifndef MUSIC_HIP_H
+#define MUSIC_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MH_OK = 0,
+    MH_EINVAL = 1,    /* bad argument / unsupported input */
+    MH_ENOMEM = 2,    /* device allocation failed */
+    MH_EHIP = 3,      /* HIP runtime error */
+    MH_EIO = 4,       /* map file I/O */
+    MH_ESTATE = 5,    /* call order (e.g. map build before framework) */
+    MH_ECAPACITY = 6  /* a fixed capacity (k vectors, molecules) exceeded */
+};
+
+enum { MH_MAP_LJ = 1, MH_MAP_EWALD = 2, MH_MAP_BUCK = 3 };
+/* MH_PREC_FP64: FP64 everywhere, follows the reference arithmetic.
+ * MH_PREC_FP32: FP32 pair arithmetic, FP32 partial sums per atom tile,
+ *               FP64 Kahan accumulation across tiles; Ewald reciprocal part
+ *               stays FP64. */
+enum { MH_PREC_FP64 = 0, MH_PREC_FP32 = 1 };
+
+typedef struct mh_ctx mh_ctx;
+typedef struct mh_map mh_map;
+
+const char *mh_last_error(void);
+const char *mh_version(void);
+
+/* Replaces the process-wide module state the reference initialises in
+ * interact_init / forcefield_init (interact.F90:121-287). */
+int mh_ctx_create(int device, mh_ctx **out);
+int mh_ctx_destroy(mh_ctx *ctx);
+
+/* The fixed framework species (config_init "Fixed", config.F90:179-) after
+ * replication to the simulation cell (simcell.F90:200-216).
+ * xyz[natoms][3], type[natoms] (atom-type index), q[natoms].
+ * fund_ell: fundamental cell edges (grid and map folding, simcell_geteff
+ * with fund=.True.); full_ell: simulation cell edges (minimum image,
+ * Ewald, insertion volume). */
+int mh_framework_upload(mh_ctx *ctx, int natoms, const double *xyz, const int *type,
+                        const double *q, const double fund_ell[3], const double full_ell[3]);
+
+/* LJ parameters of the probe atom against each framework atom type
+ * (lj_init / lj_calc_AB, lj.F90:88-262, 860-869): A, B in kcal/mol A^12/A^6,
+ * squared cutoffs.  on[t] = 0 marks an OFF pair (the reference's map build
+ * stops on it, pairmodel.F90:903-907; here such atoms contribute nothing). */
+int mh_ff_set_lj(mh_ctx *ctx, int ntypes, const double *A, const double *B,
+                 const double *hicut2, const double *locut2, const uint8_t *on);
+
+/* Buckingham A exp(-r/rho) - C r^-6 - D r^-8 per framework type (the
+ * reference's buck_snglint, buck.F90:292-358, stops before evaluating). */
+int mh_ff_set_buck(mh_ctx *ctx, int ntypes, const double *A, const double *rho,
+                   const double *C, const double *D, const double *hicut2,
+                   const double *locut2);
+
+/* Ewald parameters and framework structure factor (ewald_init,
+ * ewald_initCalc -> ewald_sfactor, ewald.F90:99-216, 558-681).  kappa =
+ * kappa_param / min(full_ell).  Returns the number of accepted k vectors. */
+int mh_ewald_set(mh_ctx *ctx, double kappa_param, int kmax, double locut, int *nk_out);
+/* copy out k[nk][3], s[nk], si[nk] (any pointer may be NULL) */
+int mh_ewald_get(mh_ctx *ctx, double *k, double *s, double *si, double *kappa);
+
+/* Map build: mapmaker_makemap (mapmaker.F90:422-598) = grid (config_makegrid,
+ * config.F90:735-808) + per-node V and 7 derivatives + FP32 store
+ * (pmap_addpoint, pmap.F90:449-500) + compaction (pmap_compact,
+ * pmap.F90:509-656) with cutoff hipotcut_kJ / 4.184.
+ * probe_q is the probe charge for Ewald maps (charge1 of
+ * ewald_getInteractionHOT).  keep_full != 0 keeps the uncompacted FP32 and
+ * FP64 (FP64 precision only) node values for mh_map_get_full. */
+int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q, double gridspace,
+                 double hipotcut_kJ, int keep_full, mh_map **out);
+int mh_map_destroy(mh_map *map);
+
+/* npnts[3] (nodes per axis), nfsize (tabulated points), counts[4] =
+ * accepted below cutoff, accepted neighbours, total, rejected (pmap.F90:
+ * 651-654), step[3] (grid spacing). */
+int mh_map_info(const mh_map *map, int npnts[3], int64_t *nfsize, int64_t counts[4],
+                double step[3]);
+/* Compacted map in the file layout: ptr[nx][ny][nz] (z fastest, Fortran
+ * ptr(k,j,i)), 1-based, 0 = untabulated; vals8[8][nfsize] SoA order
+ * r, rx, ry, rz, rxy, rxz, ryz, rxyz (interpolate.F90:30-36). */
+int mh_map_get(const mh_map *map, int32_t *ptr, float *vals8);
+/* Uncompacted node values in config_makegrid order (x fastest), [m][8]
+ * (requires keep_full).  f64 may be NULL; for FP32 builds it receives the
+ * FP64 accumulators. */
+int mh_map_get_full(const mh_map *map, float *f32, double *f64);
+
+/* Map file (mapheader_makeOldHeader + interpolate_writecore,
+ * mapheader.F90:178-232, interpolate.F90:922-938; Fortran unformatted
+ * sequential, 4-byte record markers).  zatoms: nzatoms records of
+ * name[25] symbol[2] (space padded, not NUL terminated) and 7 floats
+ * charge, A12, B6, C2, D0, rcuthi, rcutlo.  pcut_kJ goes to pcut_off. */
+int mh_map_write(const mh_map *map, const char *path, int nzatoms, const char *names25,
+                 const char *symbols2, const float *zparams7, float pcut_kJ);
+/* pmap_readmap (pmap.F90:321-358): read a map file onto the device.
+ * fund_ell gives the grid spacing eff/(nbr-1) (mapheader.F90:140-146). */
+int mh_map_load(mh_ctx *ctx, const char *path, const double fund_ell[3], mh_map **out);
+/* Upload a map held by the caller (same layout as mh_map_get). */
+int mh_map_from_host(mh_ctx *ctx, const int npnts[3], int64_t nfsize, const int32_t *ptr,
+                     const float *vals8, const double fund_ell[3], mh_map **out);
+
+/* Map interpolation at n points (maps_int -> pmap_int -> interpolate_int ->
+ * simcell_maptouc + pothermite, maps.F90:87-110, interpolate.F90:104-158,
+ * 603-874).  xyz[n][3] host arrays; e[n] = V * factor; ok[n] = 0 where a
+ * corner is untabulated (the move fails, mapflag = .False.). */
+int mh_map_interp(mh_ctx *ctx, const mh_map *map, int n, const double *xyz, double factor,
+                  double *e, uint8_t *ok);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,276 @@
+// libmusic_hip: context, framework/forcefield upload, Ewald structure factor.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "mh_internal.h"
+
+namespace mh {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+int DBuf::alloc(size_t bytes) {
+    release();
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        p = nullptr;
+        set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        return MH_ENOMEM;
+    }
+    n = bytes;
+    return MH_OK;
+}
+
+void DBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+}
+
+// Reference constants (defaults.F90:33-35): truncated pi.
+constexpr double kPi = 3.1415927;
+constexpr double kTwoPi = 2.0 * kPi;
+
+// ewald_sfactor / ewald_sfSum (ewald.F90:558-681): one lane per candidate
+// k = (x, y, z) in the reference loop order; the structure-factor sums run
+// over the atoms in their given order, exactly like the reference loop.
+__global__ void k_sfactor(int kmax, int natoms, const double *__restrict__ xyz,
+                          const double *__restrict__ q, double ialpha, double ibeta,
+                          double igamma, double ikapPiSq, double iVolPi, double locut,
+                          uint8_t *__restrict__ keep, double *__restrict__ outs,
+                          double *__restrict__ outsi, double *__restrict__ outk) {
+    const int side = 2 * kmax + 1;
+    const int ncand = side * side * side;
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncand) return;
+    int x = c / (side * side) - kmax;
+    int y = (c / side) % side - kmax;
+    int z = c % side - kmax;
+    keep[c] = 0;
+    if (x == 0 && y == 0 && z == 0) return;
+    double kalpha = (double)x * ialpha, kbeta = (double)y * ibeta, kgamma = (double)z * igamma;
+    double ka2pi = kalpha * kTwoPi, kb2pi = kbeta * kTwoPi, kg2pi = kgamma * kTwoPi;
+    double ksqx = kalpha * kalpha, ksqy = kbeta * kbeta, ksqz = kgamma * kgamma;
+    double ksq = __dadd_rn(__dadd_rn(ksqx, ksqy), ksqz);
+    double factor = __ddiv_rn(__dmul_rn(iVolPi, exp(-ikapPiSq * ksq)), ksq);
+    if (!(factor > locut)) return;
+    double sum = 0.0, sumi = 0.0;
+    for (int i = 0; i < natoms; i++) {
+        double rdotk = __dadd_rn(__dadd_rn(__dmul_rn(ka2pi, xyz[3 * i]), __dmul_rn(kb2pi, xyz[3 * i + 1])),
+                                 __dmul_rn(kg2pi, xyz[3 * i + 2]));
+        double sn, cs;
+        sincos(rdotk, &sn, &cs);
+        sum = __dadd_rn(sum, __dmul_rn(cs, q[i]));
+        sumi = __dadd_rn(sumi, __dmul_rn(sn, q[i]));
+    }
+    double smag = __dmul_rn(__dmul_rn(factor, factor), __dadd_rn(__dmul_rn(sum, sum), __dmul_rn(sumi, sumi)));
+    if (smag <= locut * 1.0e-3) return;
+    keep[c] = 1;
+    outs[c] = sum * factor;
+    outsi[c] = sumi * factor;
+    outk[3 * c] = ka2pi;
+    outk[3 * c + 1] = kb2pi;
+    outk[3 * c + 2] = kg2pi;
+}
+
+}  // namespace mh
+
+using namespace mh;
+
+extern "C" {
+
+const char *mh_last_error(void) { return g_err.c_str(); }
+const char *mh_version(void) { return "music_hip 0.1 gfx950"; }
+
+int mh_ctx_create(int device, mh_ctx **out) {
+    MH_CHECK(out, MH_EINVAL, "mh_ctx_create: out is NULL");
+    int ndev = 0;
+    MH_HIP(hipGetDeviceCount(&ndev));
+    MH_CHECK(device >= 0 && device < ndev, MH_EINVAL, "mh_ctx_create: no such device");
+    MH_HIP(hipSetDevice(device));
+    mh_ctx *c = new mh_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        return MH_EHIP;
+    }
+    *out = c;
+    return MH_OK;
+}
+
+int mh_ctx_destroy(mh_ctx *ctx) {
+    if (!ctx) return MH_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MH_OK;
+}
+
+static int check_cell(const double e[3]) {
+    for (int i = 0; i < 3; i++)
+        if (!(e[i] > 0.0) || !std::isfinite(e[i])) return 0;
+    return 1;
+}
+
+int mh_framework_upload(mh_ctx *ctx, int natoms, const double *xyz, const int *type,
+                        const double *q, const double fund_ell[3], const double full_ell[3]) {
+    MH_CHECK(ctx, MH_EINVAL, "mh_framework_upload: ctx is NULL");
+    MH_CHECK(natoms > 0 && xyz && type && fund_ell && full_ell, MH_EINVAL,
+             "mh_framework_upload: bad arguments");
+    MH_CHECK(check_cell(fund_ell) && check_cell(full_ell), MH_EINVAL,
+             "mh_framework_upload: cell edges must be positive");
+    MH_HIP(hipSetDevice(ctx->device));
+    ctx->natoms = natoms;
+    ctx->hxyz.assign(xyz, xyz + 3 * (size_t)natoms);
+    ctx->htype.assign(type, type + natoms);
+    ctx->hq.assign(natoms, 0.0);
+    if (q) ctx->hq.assign(q, q + natoms);
+    ctx->max_type = -1;
+    for (int i = 0; i < natoms; i++) {
+        MH_CHECK(type[i] >= 0 && type[i] < kMaxTypes, MH_EINVAL,
+                 "mh_framework_upload: atom type out of range [0,64)");
+        if (type[i] > ctx->max_type) ctx->max_type = type[i];
+    }
+    for (int i = 0; i < 3; i++) {
+        ctx->fund_ell[i] = fund_ell[i];
+        ctx->full_ell[i] = full_ell[i];
+    }
+    int rc;
+    if ((rc = ctx->xyz.alloc(sizeof(double) * 3 * natoms))) return rc;
+    if ((rc = ctx->type.alloc(sizeof(int) * natoms))) return rc;
+    if ((rc = ctx->q.alloc(sizeof(double) * natoms))) return rc;
+    MH_HIP(hipMemcpy(ctx->xyz.p, ctx->hxyz.data(), sizeof(double) * 3 * natoms, hipMemcpyHostToDevice));
+    MH_HIP(hipMemcpy(ctx->type.p, type, sizeof(int) * natoms, hipMemcpyHostToDevice));
+    MH_HIP(hipMemcpy(ctx->q.p, ctx->hq.data(), sizeof(double) * natoms, hipMemcpyHostToDevice));
+    ctx->ew.set = false;
+    return MH_OK;
+}
+
+int mh_ff_set_lj(mh_ctx *ctx, int ntypes, const double *A, const double *B,
+                 const double *hicut2, const double *locut2, const uint8_t *on) {
+    MH_CHECK(ctx && A && B && hicut2 && locut2, MH_EINVAL, "mh_ff_set_lj: bad arguments");
+    MH_CHECK(ntypes > 0 && ntypes <= kMaxTypes, MH_EINVAL, "mh_ff_set_lj: ntypes out of range");
+    LJTable &t = ctx->lj;
+    t.ntypes = ntypes;
+    for (int i = 0; i < ntypes; i++) {
+        t.A[i] = A[i];
+        t.B[i] = B[i];
+        t.hicut2[i] = hicut2[i];
+        t.locut2[i] = locut2[i];
+        t.on[i] = on ? on[i] : 1;
+        if (!t.on[i]) t.A[i] = t.B[i] = t.hicut2[i] = t.locut2[i] = 0.0;
+    }
+    ctx->lj_set = true;
+    return MH_OK;
+}
+
+int mh_ff_set_buck(mh_ctx *ctx, int ntypes, const double *A, const double *rho,
+                   const double *C, const double *D, const double *hicut2,
+                   const double *locut2) {
+    MH_CHECK(ctx && A && rho && C && D && hicut2 && locut2, MH_EINVAL,
+             "mh_ff_set_buck: bad arguments");
+    MH_CHECK(ntypes > 0 && ntypes <= kMaxTypes, MH_EINVAL, "mh_ff_set_buck: ntypes out of range");
+    BuckTable &t = ctx->buck;
+    t.ntypes = ntypes;
+    for (int i = 0; i < ntypes; i++) {
+        MH_CHECK(rho[i] > 0.0 || A[i] == 0.0, MH_EINVAL, "mh_ff_set_buck: rho must be > 0");
+        t.A[i] = A[i];
+        t.rho[i] = rho[i] > 0.0 ? rho[i] : 1.0;
+        t.C[i] = C[i];
+        t.D[i] = D[i];
+        t.hicut2[i] = hicut2[i];
+        t.locut2[i] = locut2[i];
+    }
+    ctx->buck_set = true;
+    return MH_OK;
+}
+
+int mh_ewald_set(mh_ctx *ctx, double kappa_param, int kmax, double locut, int *nk_out) {
+    MH_CHECK(ctx, MH_EINVAL, "mh_ewald_set: ctx is NULL");
+    MH_CHECK(ctx->natoms > 0, MH_ESTATE, "mh_ewald_set: upload the framework first");
+    MH_CHECK(kmax >= 1 && kmax <= 40, MH_EINVAL, "mh_ewald_set: kmax out of range [1,40]");
+    MH_CHECK(kappa_param > 0.0, MH_EINVAL, "mh_ewald_set: kappa must be > 0");
+    MH_HIP(hipSetDevice(ctx->device));
+    Ewald &ew = ctx->ew;
+    const double *ell = ctx->full_ell;
+    double mn = ell[0] < ell[1] ? ell[0] : ell[1];
+    mn = mn < ell[2] ? mn : ell[2];
+    ew.kappa_param = kappa_param;
+    ew.kappa = kappa_param / mn;
+    ew.kmax = kmax;
+    ew.locut = locut;
+    double ialpha = 1.0 / ell[0], ibeta = 1.0 / ell[1], igamma = 1.0 / ell[2];
+    double ikap = 1.0 / ew.kappa;
+    double pk = kPi * ikap;
+    double ikapPiSq = pk * pk;
+    double iVolPi = ialpha * ibeta * igamma / kPi;
+    const int side = 2 * kmax + 1;
+    const int ncand = side * side * side;
+    DBuf keep, cs, csi, ck;
+    int rc;
+    if ((rc = keep.alloc(ncand))) return rc;
+    if ((rc = cs.alloc(sizeof(double) * ncand))) return rc;
+    if ((rc = csi.alloc(sizeof(double) * ncand))) return rc;
+    if ((rc = ck.alloc(sizeof(double) * 3 * ncand))) return rc;
+    int bs = 64;
+    hipLaunchKernelGGL(k_sfactor, dim3((ncand + bs - 1) / bs), dim3(bs), 0, ctx->stream, kmax,
+                       ctx->natoms, ctx->xyz.as<double>(), ctx->q.as<double>(), ialpha, ibeta,
+                       igamma, ikapPiSq, iVolPi, locut, keep.as<uint8_t>(), cs.as<double>(),
+                       csi.as<double>(), ck.as<double>());
+    MH_HIP(hipGetLastError());
+    std::vector<uint8_t> hkeep(ncand);
+    std::vector<double> hs(ncand), hsi(ncand), hk(3 * (size_t)ncand);
+    MH_HIP(hipMemcpyAsync(hkeep.data(), keep.p, ncand, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipMemcpyAsync(hs.data(), cs.p, sizeof(double) * ncand, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipMemcpyAsync(hsi.data(), csi.p, sizeof(double) * ncand, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipMemcpyAsync(hk.data(), ck.p, sizeof(double) * 3 * ncand, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    // ordered compaction of the accepted vectors (reference order x,y,z)
+    ew.hk.clear();
+    ew.hs.clear();
+    ew.hsi.clear();
+    std::vector<double> grid(2 * (size_t)ncand, 0.0);  // complex c = s + i si per candidate
+    for (int c = 0; c < ncand; c++) {
+        if (!hkeep[c]) continue;
+        ew.hs.push_back(hs[c]);
+        ew.hsi.push_back(hsi[c]);
+        ew.hk.push_back(hk[3 * c]);
+        ew.hk.push_back(hk[3 * c + 1]);
+        ew.hk.push_back(hk[3 * c + 2]);
+        grid[2 * c] = hs[c];
+        grid[2 * c + 1] = hsi[c];
+    }
+    ew.nk = (int)ew.hs.size();
+    MH_CHECK(ew.nk <= 50000, MH_ECAPACITY,
+             "mh_ewald_set: more than nkmax = 50000 k vectors (ewald.F90:134)");
+    if ((rc = ew.k.alloc(sizeof(double) * 3 * (ew.nk + 1)))) return rc;
+    if ((rc = ew.s.alloc(sizeof(double) * (ew.nk + 1)))) return rc;
+    if ((rc = ew.si.alloc(sizeof(double) * (ew.nk + 1)))) return rc;
+    if ((rc = ew.ccoef.alloc(sizeof(double) * 2 * ncand))) return rc;
+    if (ew.nk) {
+        MH_HIP(hipMemcpy(ew.k.p, ew.hk.data(), sizeof(double) * 3 * ew.nk, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(ew.s.p, ew.hs.data(), sizeof(double) * ew.nk, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(ew.si.p, ew.hsi.data(), sizeof(double) * ew.nk, hipMemcpyHostToDevice));
+    }
+    MH_HIP(hipMemcpy(ew.ccoef.p, grid.data(), sizeof(double) * 2 * ncand, hipMemcpyHostToDevice));
+    ew.set = true;
+    if (nk_out) *nk_out = ew.nk;
+    return MH_OK;
+}
+
+int mh_ewald_get(mh_ctx *ctx, double *k, double *s, double *si, double *kappa) {
+    MH_CHECK(ctx && ctx->ew.set, MH_ESTATE, "mh_ewald_get: call mh_ewald_set first");
+    const Ewald &ew = ctx->ew;
+    if (k) std::memcpy(k, ew.hk.data(), sizeof(double) * 3 * ew.nk);
+    if (s) std::memcpy(s, ew.hs.data(), sizeof(double) * ew.nk);
+    if (si) std::memcpy(si, ew.hsi.data(), sizeof(double) * ew.nk);
+    if (kappa) *kappa = ew.kappa;
+    return MH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// Internal declarations of libmusic_hip (not part of the C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/music_hip.h"
+
+namespace mh {
+
+// thread-local last error message, returned by mh_last_error()
+void set_error(const std::string &msg);
+
+#define MH_HIP(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            mh::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return MH_EHIP;                                                   \
+        }                                                                     \
+    } while (0)
+
+#define MH_CHECK(cond, code, msg)  \
+    do {                                 \
+        if (!(cond)) {                   \
+            mh::set_error(msg);          \
+            return (code);               \
+        }                                \
+    } while (0)
+
+// device buffer owned by the library (hipMalloc / hipFree)
+struct DBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    ~DBuf() { release(); }
+    int alloc(size_t bytes);
+    void release();
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+constexpr int kMaxTypes = 64;
+
+struct LJTable {
+    int ntypes = 0;
+    double A[kMaxTypes], B[kMaxTypes], hicut2[kMaxTypes], locut2[kMaxTypes];
+    uint8_t on[kMaxTypes];
+};
+
+struct BuckTable {
+    int ntypes = 0;
+    double A[kMaxTypes], rho[kMaxTypes], C[kMaxTypes], D[kMaxTypes];
+    double hicut2[kMaxTypes], locut2[kMaxTypes];
+};
+
+struct Ewald {
+    bool set = false;
+    double kappa_param = 0, kappa = 0, locut = 0;
+    int kmax = 0, nk = 0;
+    DBuf k, s, si;        // accepted vectors, reference order
+    DBuf ccoef;           // complex coefficient grid c(mx,my,mz) = s + i si, 0 if not kept
+    std::vector<double> hk, hs, hsi;
+};
+
+}  // namespace mh
+
+struct mh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // framework
+    int natoms = 0;
+    double fund_ell[3] = {0, 0, 0}, full_ell[3] = {0, 0, 0};
+    mh::DBuf xyz;   // double [natoms][3]
+    mh::DBuf type;  // int [natoms]
+    mh::DBuf q;     // double [natoms]
+    std::vector<double> hxyz, hq;
+    std::vector<int> htype;
+    int max_type = -1;
+    mh::LJTable lj;
+    bool lj_set = false;
+    mh::BuckTable buck;
+    bool buck_set = false;
+    mh::Ewald ew;
+};
+
+struct mh_map {
+    mh_ctx *ctx = nullptr;
+    int npnts[3] = {0, 0, 0};
+    double step[3] = {0, 0, 0};
+    double fund_ell[3] = {0, 0, 0};
+    int64_t nfsize = 0;
+    int64_t counts[4] = {0, 0, 0, 0};
+    mh::DBuf ptr;    // int32 [nx][ny][nz] z fastest, 1-based
+    mh::DBuf vals;   // float [nfsize][8] (AoS records, 32 B)
+    mh::DBuf full32; // float [nodes][8] when kept
+    mh::DBuf full64; // double [nodes][8] when kept
+};

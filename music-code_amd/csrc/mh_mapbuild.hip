@@ -1,0 +1,1027 @@
+// libmusic_hip: potential-map build kernels (LJ, Buckingham, Ewald real and
+// reciprocal space) and the deterministic parallel compaction.
+//
+// Layout in HBM:
+//   framework atoms        double [natoms][3] + int type + double q
+//   full map (FP32)        float  [nodes][8]   node m = x + nx*(y + ny*z)
+//   full map (FP64, opt.)  double [nodes][8]
+//   Ewald reciprocal part  double [nodes][8]   (separable 3-stage DFT)
+//   compact map            float  [nfsize][8]  32-byte records
+//   ptr                    int32  [nx][ny][nz] z fastest, 1-based
+//
+// One workgroup of 256 lanes owns an 8x8x4 brick of grid nodes (one lane per
+// node).  It culls the framework atoms against the brick's bounding box
+// (lower bound of the minimum-image distance), keeps the survivors in LDS in
+// their original order and streams them to every lane.
+#include <cmath>
+#include <cstring>
+
+#include "mh_internal.h"
+
+namespace mh {
+
+constexpr int BX = 8, BY = 8, BZ = 4, NT = BX * BY * BZ;
+constexpr int SEG = 1024;  // culled atoms held in LDS per pass
+
+constexpr double kPiR = 3.1415927;  // defaults.F90:33
+constexpr double kTokcal = 332.064;                       // ewald.F90:60
+constexpr double kVeryLrg = 1.0e10;                       // mapmaker.F90:57
+constexpr double kLowCut2 = 1.0e-3 * 1.0e-3;              // ewald.F90:361-362
+
+struct PairTab {
+    int ntypes;
+    double A[kMaxTypes], B[kMaxTypes], h2[kMaxTypes], l2[kMaxTypes];
+    double bA[kMaxTypes], brho[kMaxTypes], bC[kMaxTypes], bD[kMaxTypes];
+};
+
+struct GridP {
+    int nx, ny, nz;
+    double step[3];
+    double full[3];
+    int natoms;
+    double cull2;    // atoms farther than this from the brick are skipped
+    double r2safe;   // FP32 path: pairs closer than this are evaluated in FP64
+    double kappa, charge1;
+    float rc2f;      // FP32 Ewald real-space cutoff^2
+};
+
+__device__ __forceinline__ double fold1(double d, double L) {
+    // fundcell_multMinImage orthorhombic branch (fundcell.F90:676-690)
+    double h = 0.5 * L;
+    if (d > h) d = d - L;
+    else if (d < -h) d = d + L;
+    return d;
+}
+
+// lower bound of |d + kL| over d in [lo, hi], k in {-1,0,1}
+__device__ __forceinline__ double ival_lb(double lo, double hi, double L) {
+    double best = 1e300;
+#pragma unroll
+    for (int k = -1; k <= 1; k++) {
+        double a = lo + k * L, b = hi + k * L;
+        double d = (a > 0.0) ? a : ((b < 0.0) ? -b : 0.0);
+        best = d < best ? d : best;
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------- pairs
+// lj_snglint with hot (lj.F90:652-733).  Returns 2 evaluated, 1 beyond
+// hicut, 0 below locut.
+__device__ __forceinline__ int lj_pair64(double A, double B, double h2, double l2, double sx,
+                                         double sy, double sz, double &pot, double *h) {
+    double r2 = __dadd_rn(__dadd_rn(__dmul_rn(sx, sx), __dmul_rn(sy, sy)), __dmul_rn(sz, sz));
+    if (r2 > h2) return 1;
+    if (r2 < l2) return 0;
+    double r2i = 1.0 / r2;
+    double r6i = __dmul_rn(__dmul_rn(r2i, r2i), r2i);
+    double r12i = __dmul_rn(r6i, r6i);
+    double nrg = __dsub_rn(__dmul_rn(A, r12i), __dmul_rn(B, r6i));
+    pot = __dadd_rn(pot, nrg);
+    double fac1 = __dmul_rn(__dsub_rn(__dmul_rn(__dmul_rn(12.0, A), r12i), __dmul_rn(__dmul_rn(6.0, B), r6i)), r2i);
+    h[0] = __dadd_rn(h[0], -__dmul_rn(sx, fac1));
+    h[1] = __dadd_rn(h[1], -__dmul_rn(sy, fac1));
+    h[2] = __dadd_rn(h[2], -__dmul_rn(sz, fac1));
+    double r10i = __dmul_rn(r12i, r2);
+    double r16i = __dmul_rn(r10i, r6i);
+    fac1 = __dsub_rn(__dmul_rn(__dmul_rn(168.0, A), r16i), __dmul_rn(__dmul_rn(48.0, B), r10i));
+    h[3] = __dadd_rn(h[3], __dmul_rn(__dmul_rn(fac1, sx), sy));
+    h[4] = __dadd_rn(h[4], __dmul_rn(__dmul_rn(fac1, sx), sz));
+    h[5] = __dadd_rn(h[5], __dmul_rn(__dmul_rn(fac1, sy), sz));
+    fac1 = __dmul_rn(__dadd_rn(__dmul_rn(__dmul_rn(-2688.0, A), r16i), __dmul_rn(__dmul_rn(480.0, B), r10i)), r2i);
+    h[6] = __dadd_rn(h[6], __dmul_rn(__dmul_rn(__dmul_rn(fac1, sx), sy), sz));
+    return 2;
+}
+
+// Buckingham closed form, radial derivatives f1 = V'/r, g2 = f1'/r, g3 = g2'/r
+__device__ __forceinline__ int buck_pair64(double A, double rho, double C, double D, double h2,
+                                           double l2, double sx, double sy, double sz,
+                                           double &pot, double *h) {
+    double r2 = sx * sx + sy * sy + sz * sz;
+    if (r2 > h2) return 1;
+    if (r2 < l2) return 0;
+    double r = sqrt(r2);
+    double ri = 1.0 / r, r2i = ri * ri, r6i = r2i * r2i * r2i, r8i = r6i * r2i;
+    double e = A * exp(-r / rho), ir = 1.0 / rho;
+    double v = e - C * r6i - D * r8i;
+    double f1 = -e * ir * ri + 6.0 * C * r8i + 8.0 * D * r8i * r2i;
+    double g2 = e * ir * (ir * r2i + r2i * ri) - 48.0 * C * r8i * r2i - 80.0 * D * r8i * r2i * r2i;
+    double g3 = -e * ir * (ir * ir * r2i * ri + 3.0 * ir * r2i * r2i + 3.0 * r2i * r2i * ri) +
+                480.0 * C * r6i * r6i + 960.0 * D * r8i * r6i;
+    pot += v;
+    h[0] += sx * f1;
+    h[1] += sy * f1;
+    h[2] += sz * f1;
+    h[3] += g2 * sx * sy;
+    h[4] += g2 * sx * sz;
+    h[5] += g2 * sy * sz;
+    h[6] += g3 * sx * sy * sz;
+    return 2;
+}
+
+// A&S 7.1.26 erfc with single-precision coefficients (utils.F90:2090-2101)
+__device__ __forceinline__ double erfc_ref64(double x) {
+    const double a1 = (double)0.254829592f, a2 = (double)-0.284496736f, a3 = (double)1.421413741f,
+                 a4 = (double)-1.453152027f, a5 = (double)1.061405429f, p = (double)0.3275911f;
+    double t = 1.0 / __dadd_rn(1.0, __dmul_rn(p, x));
+    double xsq = __dmul_rn(x, x);
+    double tp = __dmul_rn(t, __dadd_rn(a1, __dmul_rn(t, __dadd_rn(a2, __dmul_rn(t, __dadd_rn(a3, __dmul_rn(t, __dadd_rn(a4, __dmul_rn(t, a5)))))))));
+    return __dmul_rn(tp, exp(-xsq));
+}
+
+// ewald_rwaldHOT per atom (ewald.F90:874-916); h[0..6] = hot(2..8)
+__device__ __forceinline__ void ewald_real_pair64(double kappa, double charge1, double q2,
+                                                  double sx, double sy, double sz, double d2,
+                                                  double &pot, double *h) {
+    const double risqrtpi = 0.5641896;
+    double dist = sqrt(d2);
+    double kr = kappa * dist;
+    double erfkr = erfc_ref64(kr);
+    double ri1 = 1.0 / dist;
+    pot = __dadd_rn(pot, __dmul_rn(__dmul_rn(__dmul_rn(charge1, q2), erfkr), ri1));
+    double ri2 = ri1 * ri1, ri3 = ri2 * ri1;
+    double ek2r2 = exp(-(kr * kr));
+    double kappa3 = kappa * kappa * kappa;
+    double temp = __dsub_rn(-(ri3 * erfkr), __dmul_rn(__dmul_rn(__dmul_rn(2.0 * kappa, ri2), risqrtpi), ek2r2));
+    double qt = __dmul_rn(q2, temp);
+    h[0] = __dadd_rn(h[0], __dmul_rn(__dmul_rn(sx, qt), charge1));
+    h[1] = __dadd_rn(h[1], __dmul_rn(__dmul_rn(sy, qt), charge1));
+    h[2] = __dadd_rn(h[2], __dmul_rn(__dmul_rn(sz, qt), charge1));
+    double ri4 = ri3 * ri1, ri5 = ri4 * ri1;
+    temp = __dadd_rn(__dadd_rn(__dmul_rn(3.0 * ri5, erfkr),
+                               __dmul_rn(__dmul_rn(__dmul_rn(6.0 * kappa, risqrtpi), ri4), ek2r2)),
+                     __dmul_rn(__dmul_rn(__dmul_rn(4.0 * kappa3, risqrtpi), ri2), ek2r2));
+    double cq = __dmul_rn(charge1, q2);
+    h[3] = __dadd_rn(h[3], __dmul_rn(__dmul_rn(__dmul_rn(cq, sx), sy), temp));
+    h[4] = __dadd_rn(h[4], __dmul_rn(__dmul_rn(__dmul_rn(cq, sx), sz), temp));
+    h[5] = __dadd_rn(h[5], __dmul_rn(__dmul_rn(__dmul_rn(cq, sy), sz), temp));
+    double ri6 = ri5 * ri1, ri7 = ri6 * ri1;
+    temp = __dsub_rn(__dsub_rn(__dsub_rn(-(15.0 * ri7 * erfkr),
+                                         __dmul_rn(__dmul_rn(__dmul_rn(30.0 * kappa, risqrtpi), ri6), ek2r2)),
+                               __dmul_rn(__dmul_rn(__dmul_rn(20.0 * kappa3, risqrtpi), ri4), ek2r2)),
+                     __dmul_rn(__dmul_rn(__dmul_rn(8.0 * kappa3 * kappa * kappa, risqrtpi), ri2), ek2r2));
+    h[6] = __dadd_rn(h[6], __dmul_rn(__dmul_rn(__dmul_rn(__dmul_rn(cq, sx), sy), sz), temp));
+}
+
+// FP32 Ewald real-space pair: same expression tree, single precision
+__device__ __forceinline__ void ewald_real_pair32(float kappa, float qq, float sx, float sy,
+                                                  float sz, float r2, float &pot, float *h) {
+#pragma clang fp contract(fast)
+    const float risqrtpi = 0.5641896f;
+    const float a1 = 0.254829592f, a2 = -0.284496736f, a3 = 1.421413741f, a4 = -1.453152027f,
+                a5 = 1.061405429f, p = 0.3275911f;
+    float ri1 = __frsqrt_rn(r2);
+    float dist = r2 * ri1;
+    float kr = kappa * dist;
+    float t = __frcp_rn(1.0f + p * kr);
+    float ek = __expf(-kr * kr);
+    float erfkr = t * (a1 + t * (a2 + t * (a3 + t * (a4 + t * a5)))) * ek;
+    float ri2 = ri1 * ri1, ri3 = ri2 * ri1, ri4 = ri3 * ri1, ri5 = ri4 * ri1;
+    float ri6 = ri5 * ri1, ri7 = ri6 * ri1;
+    float k3 = kappa * kappa * kappa;
+    float ekr = risqrtpi * ek;
+    pot += qq * erfkr * ri1;
+    float t1 = qq * (-ri3 * erfkr - 2.0f * kappa * ri2 * ekr);
+    h[0] += sx * t1;
+    h[1] += sy * t1;
+    h[2] += sz * t1;
+    float t2 = qq * (3.0f * ri5 * erfkr + (6.0f * kappa * ri4 + 4.0f * k3 * ri2) * ekr);
+    h[3] += t2 * sx * sy;
+    h[4] += t2 * sx * sz;
+    h[5] += t2 * sy * sz;
+    float t3 = qq * (-15.0f * ri7 * erfkr -
+                     (30.0f * kappa * ri6 + 20.0f * k3 * ri4 + 8.0f * k3 * kappa * kappa * ri2) * ekr);
+    h[6] += t3 * sx * sy * sz;
+}
+
+// FP32 LJ pair with derivatives
+__device__ __forceinline__ void lj_pair32(float A, float B, float sx, float sy, float sz, float r2,
+                                          float &pot, float *h) {
+#pragma clang fp contract(fast)
+    float r2i = __frcp_rn(r2);
+    float r6i = r2i * r2i * r2i;
+    float r12i = r6i * r6i;
+    pot += A * r12i - B * r6i;
+    float fac1 = (12.0f * A * r12i - 6.0f * B * r6i) * r2i;
+    h[0] -= sx * fac1;
+    h[1] -= sy * fac1;
+    h[2] -= sz * fac1;
+    float r10i = r12i * r2;
+    float r16i = r10i * r6i;
+    float f2 = 168.0f * A * r16i - 48.0f * B * r10i;
+    h[3] += f2 * sx * sy;
+    h[4] += f2 * sx * sz;
+    h[5] += f2 * sy * sz;
+    float f3 = (-2688.0f * A * r16i + 480.0f * B * r10i) * r2i;
+    h[6] += f3 * sx * sy * sz;
+}
+
+__device__ __forceinline__ void kahan_add(double &s, double &c, double v) {
+    double y = v - c;
+    double t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+enum { K_LJ = 1, K_EWALD = 2, K_BUCK = 3 };
+
+// ------------------------------------------------------- node kernels
+// FP64 node kernel: reference arithmetic, atoms visited in their order
+// (culled atoms are beyond every cutoff of the brick and contribute +0).
+// Ewald: real space only; the reciprocal part is added from recip[].
+template <int KIND>
+__global__ void __launch_bounds__(NT) k_nodes64(GridP g, const PairTab *__restrict__ tab_g,
+                                                 const double *__restrict__ xyz,
+                                                 const int *__restrict__ typ,
+                                                 const double *__restrict__ qv,
+                                                 const double *__restrict__ recip,
+                                                 float *__restrict__ out32,
+                                                 double *__restrict__ out64) {
+    __shared__ double s_x[SEG], s_y[SEG], s_z[SEG], s_q[SEG];
+    __shared__ int s_t[SEG];
+    __shared__ int s_wtot[NT / 64];
+    __shared__ PairTab tab;
+    const int tid = threadIdx.x;
+    if (KIND != K_EWALD) {
+        for (int i = tid; i < (int)(sizeof(PairTab) / sizeof(double)); i += NT)
+            reinterpret_cast<double *>(&tab)[i] = reinterpret_cast<const double *>(tab_g)[i];
+    }
+    const int nbx = (g.nx + BX - 1) / BX, nby = (g.ny + BY - 1) / BY;
+    const int b = blockIdx.x;
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int x = bx * BX + (tid % BX), y = by * BY + ((tid / BX) % BY), z = bz * BZ + tid / (BX * BY);
+    const bool valid = (x < g.nx) && (y < g.ny) && (z < g.nz);
+    const double px = (double)x * g.step[0], py = (double)y * g.step[1], pz = (double)z * g.step[2];
+    // brick bounding box
+    const double lo[3] = {(double)(bx * BX) * g.step[0], (double)(by * BY) * g.step[1], (double)(bz * BZ) * g.step[2]};
+    const double hi[3] = {(double)min(bx * BX + BX - 1, g.nx - 1) * g.step[0],
+                          (double)min(by * BY + BY - 1, g.ny - 1) * g.step[1],
+                          (double)min(bz * BZ + BZ - 1, g.nz - 1) * g.step[2]};
+    __syncthreads();
+    double pot = 0.0, h[7] = {0, 0, 0, 0, 0, 0, 0};
+    bool failed = false;
+    for (int seg0 = 0; seg0 < g.natoms; seg0 += SEG) {
+        const int segn = min(SEG, g.natoms - seg0);
+        int cnt = 0;
+        for (int c0 = 0; c0 < segn; c0 += NT) {
+            int a = seg0 + c0 + tid;
+            bool keep = false;
+            double ax = 0, ay = 0, az = 0;
+            if (c0 + tid < segn) {
+                ax = xyz[3 * a];
+                ay = xyz[3 * a + 1];
+                az = xyz[3 * a + 2];
+                double lx = ival_lb(lo[0] - ax, hi[0] - ax, g.full[0]);
+                double ly = ival_lb(lo[1] - ay, hi[1] - ay, g.full[1]);
+                double lz = ival_lb(lo[2] - az, hi[2] - az, g.full[2]);
+                keep = (lx * lx + ly * ly + lz * lz) <= g.cull2;
+            }
+            unsigned long long msk = __ballot(keep);
+            const int lane = tid & 63, wave = tid >> 6;
+            if (lane == 0) s_wtot[wave] = __popcll(msk);
+            __syncthreads();
+            int off = cnt, add = 0;
+            for (int w = 0; w < NT / 64; w++) {
+                if (w < wave) off += s_wtot[w];
+                add += s_wtot[w];
+            }
+            if (keep) {
+                int pos = off + __popcll(msk & ((1ull << lane) - 1ull));
+                s_x[pos] = ax;
+                s_y[pos] = ay;
+                s_z[pos] = az;
+                s_t[pos] = typ[a];
+                s_q[pos] = (KIND == K_EWALD) ? qv[a] : 0.0;
+            }
+            cnt += add;
+            __syncthreads();
+        }
+        if (valid && !failed) {
+            for (int j = 0; j < cnt; j++) {
+                double sx = fold1(px - s_x[j], g.full[0]);
+                double sy = fold1(py - s_y[j], g.full[1]);
+                double sz = fold1(pz - s_z[j], g.full[2]);
+                if (KIND == K_LJ) {
+                    int t = s_t[j];
+                    double hh[7] = {0, 0, 0, 0, 0, 0, 0};
+                    int rc = lj_pair64(tab.A[t], tab.B[t], tab.h2[t], tab.l2[t], sx, sy, sz, pot, hh);
+                    for (int i = 0; i < 7; i++) h[i] = __dadd_rn(h[i], hh[i]);
+                    if (rc == 0) { failed = true; break; }
+                } else if (KIND == K_BUCK) {
+                    int t = s_t[j];
+                    int rc = buck_pair64(tab.bA[t], tab.brho[t], tab.bC[t], tab.bD[t], tab.h2[t],
+                                         tab.l2[t], sx, sy, sz, pot, h);
+                    if (rc == 0) { failed = true; break; }
+                } else {
+                    double d2 = __dadd_rn(__dadd_rn(__dmul_rn(sx, sx), __dmul_rn(sy, sy)), __dmul_rn(sz, sz));
+                    if (d2 < kLowCut2) { failed = true; break; }
+                    ewald_real_pair64(g.kappa, g.charge1, s_q[j], sx, sy, sz, d2, pot, h);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+    const long long m = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
+    double o[8];
+    if (KIND == K_EWALD) {
+        if (failed) {
+            o[0] = kVeryLrg;
+            for (int i = 1; i < 8; i++) o[i] = 0.0;
+        } else {
+            const double *rp = recip + 8 * m;
+            o[0] = __dmul_rn(__dadd_rn(pot, rp[0]), kTokcal);
+            for (int i = 0; i < 7; i++) o[i + 1] = __dmul_rn(__dadd_rn(h[i], rp[i + 1]), kTokcal);
+        }
+    } else {
+        o[0] = failed ? kVeryLrg : pot;
+        for (int i = 0; i < 7; i++) o[i + 1] = h[i];
+    }
+    float4 *d32 = reinterpret_cast<float4 *>(out32 + 8 * m);
+    d32[0] = make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
+    d32[1] = make_float4((float)o[4], (float)o[5], (float)o[6], (float)o[7]);
+    if (out64) {
+        double2 *d64 = reinterpret_cast<double2 *>(out64 + 8 * m);
+        for (int i = 0; i < 4; i++) d64[i] = make_double2(o[2 * i], o[2 * i + 1]);
+    }
+}
+
+// FP32 node kernel (LJ or Ewald real space): separations relative to the
+// brick centre in FP32, FP32 pair terms summed in FP32 over tiles of 32
+// atoms, FP64 Kahan sums across tiles.  Pairs closer than sqrt(r2safe) are
+// evaluated with the FP64 routine (LJ locut / Ewald low-cut decisions and
+// the steep near-atom terms stay in FP64).
+template <int KIND>
+__global__ void __launch_bounds__(NT) k_nodes32(GridP g, const PairTab *__restrict__ tab_g,
+                                                 const double *__restrict__ xyz,
+                                                 const int *__restrict__ typ,
+                                                 const double *__restrict__ qv,
+                                                 const double *__restrict__ recip,
+                                                 float *__restrict__ out32,
+                                                 double *__restrict__ out64) {
+    __shared__ float s_x[SEG], s_y[SEG], s_z[SEG], s_p0[SEG], s_p1[SEG];
+    __shared__ int s_a[SEG];
+    __shared__ int s_wtot[NT / 64];
+    __shared__ PairTab tab;
+    const int tid = threadIdx.x;
+    if (KIND == K_LJ) {
+        for (int i = tid; i < (int)(sizeof(PairTab) / sizeof(double)); i += NT)
+            reinterpret_cast<double *>(&tab)[i] = reinterpret_cast<const double *>(tab_g)[i];
+    }
+    const int nbx = (g.nx + BX - 1) / BX, nby = (g.ny + BY - 1) / BY;
+    const int b = blockIdx.x;
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int x = bx * BX + (tid % BX), y = by * BY + ((tid / BX) % BY), z = bz * BZ + tid / (BX * BY);
+    const bool valid = (x < g.nx) && (y < g.ny) && (z < g.nz);
+    const double px = (double)x * g.step[0], py = (double)y * g.step[1], pz = (double)z * g.step[2];
+    const double lo[3] = {(double)(bx * BX) * g.step[0], (double)(by * BY) * g.step[1], (double)(bz * BZ) * g.step[2]};
+    const double hi[3] = {(double)min(bx * BX + BX - 1, g.nx - 1) * g.step[0],
+                          (double)min(by * BY + BY - 1, g.ny - 1) * g.step[1],
+                          (double)min(bz * BZ + BZ - 1, g.nz - 1) * g.step[2]};
+    const double cen[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
+    const float dx0 = (float)(px - cen[0]), dy0 = (float)(py - cen[1]), dz0 = (float)(pz - cen[2]);
+    const float r2safe = (float)g.r2safe;
+    const float kappaf = (float)g.kappa;
+    const float c1f = (float)g.charge1;
+    __syncthreads();
+    double S[8] = {0, 0, 0, 0, 0, 0, 0, 0}, Cc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double pot64 = 0.0, h64[7] = {0, 0, 0, 0, 0, 0, 0};
+    bool failed = false;
+    for (int seg0 = 0; seg0 < g.natoms; seg0 += SEG) {
+        const int segn = min(SEG, g.natoms - seg0);
+        int cnt = 0;
+        for (int c0 = 0; c0 < segn; c0 += NT) {
+            int a = seg0 + c0 + tid;
+            bool keep = false;
+            double ax = 0, ay = 0, az = 0;
+            if (c0 + tid < segn) {
+                ax = xyz[3 * a];
+                ay = xyz[3 * a + 1];
+                az = xyz[3 * a + 2];
+                double lx = ival_lb(lo[0] - ax, hi[0] - ax, g.full[0]);
+                double ly = ival_lb(lo[1] - ay, hi[1] - ay, g.full[1]);
+                double lz = ival_lb(lo[2] - az, hi[2] - az, g.full[2]);
+                keep = (lx * lx + ly * ly + lz * lz) <= g.cull2;
+            }
+            unsigned long long m = __ballot(keep);
+            const int lane = tid & 63, wave = tid >> 6;
+            if (lane == 0) s_wtot[wave] = __popcll(m);
+            __syncthreads();
+            int off = cnt, add = 0;
+            for (int w = 0; w < NT / 64; w++) {
+                if (w < wave) off += s_wtot[w];
+                add += s_wtot[w];
+            }
+            if (keep) {
+                int pos = off + __popcll(m & ((1ull << lane) - 1ull));
+                s_x[pos] = (float)fold1(cen[0] - ax, g.full[0]);
+                s_y[pos] = (float)fold1(cen[1] - ay, g.full[1]);
+                s_z[pos] = (float)fold1(cen[2] - az, g.full[2]);
+                s_a[pos] = a;
+                if (KIND == K_LJ) {
+                    int t = typ[a];
+                    s_p0[pos] = (float)tab.A[t];
+                    s_p1[pos] = __int_as_float(t);
+                } else {
+                    s_p0[pos] = (float)(g.charge1 * qv[a]);
+                    s_p1[pos] = 0.0f;
+                }
+            }
+            cnt += add;
+            __syncthreads();
+        }
+        if (valid && !failed) {
+            for (int j0 = 0; j0 < cnt && !failed; j0 += 32) {
+                float pf = 0.0f, hf[7] = {0, 0, 0, 0, 0, 0, 0};
+                const int j1 = min(cnt, j0 + 32);
+                for (int j = j0; j < j1; j++) {
+                    float sx = dx0 + s_x[j], sy = dy0 + s_y[j], sz = dz0 + s_z[j];
+                    float r2 = sx * sx + sy * sy + sz * sz;
+                    if (KIND == K_LJ) {
+                        int t = __float_as_int(s_p1[j]);
+                        const float h2f = (float)tab.h2[t];
+                        const float band = h2f * 1.0e-6f;
+                        if (r2 > h2f + band) continue;
+                        // near-atom pairs and pairs within rounding of hicut:
+                        // FP64 separation and reference cutoff decisions
+                        if (r2 < r2safe || r2 > h2f - band) {
+                            int a = s_a[j];
+                            double ex = fold1(px - xyz[3 * a], g.full[0]);
+                            double ey = fold1(py - xyz[3 * a + 1], g.full[1]);
+                            double ez = fold1(pz - xyz[3 * a + 2], g.full[2]);
+                            int rc = lj_pair64(tab.A[t], tab.B[t], tab.h2[t], tab.l2[t], ex, ey, ez, pot64, h64);
+                            if (rc == 0) { failed = true; break; }
+                            continue;
+                        }
+                        lj_pair32(s_p0[j], (float)tab.B[t], sx, sy, sz, r2, pf, hf);
+                    } else {
+                        if (r2 > g.rc2f) continue;
+                        if (r2 < r2safe) {
+                            int a = s_a[j];
+                            double ex = fold1(px - xyz[3 * a], g.full[0]);
+                            double ey = fold1(py - xyz[3 * a + 1], g.full[1]);
+                            double ez = fold1(pz - xyz[3 * a + 2], g.full[2]);
+                            double d2 = ex * ex + ey * ey + ez * ez;
+                            if (d2 < kLowCut2) { failed = true; break; }
+                            ewald_real_pair64(g.kappa, g.charge1, qv[a], ex, ey, ez, d2, pot64, h64);
+                            continue;
+                        }
+                        ewald_real_pair32(kappaf, s_p0[j], sx, sy, sz, r2, pf, hf);
+                    }
+                }
+                kahan_add(S[0], Cc[0], (double)pf);
+                for (int i = 0; i < 7; i++) kahan_add(S[i + 1], Cc[i + 1], (double)hf[i]);
+            }
+        }
+        __syncthreads();
+    }
+    (void)c1f;
+    if (!valid) return;
+    const long long m = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
+    double o[8];
+    kahan_add(S[0], Cc[0], pot64);
+    for (int i = 0; i < 7; i++) kahan_add(S[i + 1], Cc[i + 1], h64[i]);
+    if (KIND == K_EWALD) {
+        if (failed) {
+            o[0] = kVeryLrg;
+            for (int i = 1; i < 8; i++) o[i] = 0.0;
+        } else {
+            const double *rp = recip + 8 * m;
+            for (int i = 0; i < 8; i++) o[i] = (S[i] + rp[i]) * kTokcal;
+        }
+    } else {
+        o[0] = failed ? kVeryLrg : S[0];
+        for (int i = 1; i < 8; i++) o[i] = S[i];
+    }
+    float4 *d32 = reinterpret_cast<float4 *>(out32 + 8 * m);
+    d32[0] = make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
+    d32[1] = make_float4((float)o[4], (float)o[5], (float)o[6], (float)o[7]);
+    if (out64) {
+        double2 *d64 = reinterpret_cast<double2 *>(out64 + 8 * m);
+        for (int i = 0; i < 4; i++) d64[i] = make_double2(o[2 * i], o[2 * i + 1]);
+    }
+}
+
+// ------------------------------------------------ Ewald reciprocal part
+// V(r) = Re sum_k c_k e^{ik.r}, with c_k = s_k + i si_k (ewald_skwaldHOT,
+// ewald.F90:966-1005: realPart = s cos - si sin); the derivative terms use
+// imagPart = s sin - si cos = Im(conj(c_k) e^{ik.r}).  k lies on the lattice
+// 2 pi (mx/Lx, my/Ly, mz/Lz) and the nodes on the grid lattice, so every
+// output is a separable sum evaluated axis by axis:
+//   stage Z: Zc_w(mx,my,z) = sum_mz c kz^w e^{i kz z}, Zb_w likewise with conj(c)
+//   stage Y+X (per (y,z) line): Y(mx) = sum_my ky^v e^{i ky y} Z(mx,my),
+//                               out(x) = sum_mx kx^u e^{i kx x} Y(mx)
+// Table E[axis][m][n] = e^{i k(m) n step}, FP64.
+__global__ void k_recip_z(int side, int nz, const double2 *__restrict__ cgrid,
+                          const double2 *__restrict__ Ez, const double *__restrict__ kzv,
+                          double2 *__restrict__ Zout) {
+    // one lane per (mx, my, z); Zout[((mx*side+my)*nz + z)*4 + w]
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long n = (long long)side * side * nz;
+    if (i >= n) return;
+    int z = (int)(i % nz);
+    int mxy = (int)(i / nz);
+    double2 c0 = make_double2(0, 0), c1 = c0, b0 = c0, b1 = c0;
+    for (int mz = 0; mz < side; mz++) {
+        double2 c = cgrid[(long long)mxy * side + mz];
+        double2 e = Ez[(long long)mz * nz + z];
+        double kz = kzv[mz];
+        // c e
+        double re = c.x * e.x - c.y * e.y, im = c.x * e.y + c.y * e.x;
+        // conj(c) e
+        double bre = c.x * e.x + c.y * e.y, bim = c.x * e.y - c.y * e.x;
+        c0.x += re; c0.y += im;
+        c1.x += kz * re; c1.y += kz * im;
+        b0.x += bre; b0.y += bim;
+        b1.x += kz * bre; b1.y += kz * bim;
+    }
+    double2 *o = Zout + i * 4;
+    o[0] = c0; o[1] = c1; o[2] = b0; o[3] = b1;
+}
+
+constexpr int MAXSIDE = 81;
+
+__global__ void __launch_bounds__(256) k_recip_xy(int side, int nx, int ny, int nz,
+                                                  const double2 *__restrict__ Zin,
+                                                  const double2 *__restrict__ Ex,
+                                                  const double2 *__restrict__ Ey,
+                                                  const double *__restrict__ kxv,
+                                                  const double *__restrict__ kyv, double charge1,
+                                                  double *__restrict__ recip) {
+    // block per (y, z) line
+    __shared__ double2 Y[MAXSIDE * 8];
+    const int line = blockIdx.x;
+    const int y = line % ny, z = line / ny;
+    for (int t = threadIdx.x; t < side * 8; t += blockDim.x) {
+        int mx = t / 8, o = t % 8;
+        // o: 0 Yc00, 1 Yc10, 2 Yc01, 3 Yc11, 4 Yb00, 5 Yb10, 6 Yb01, 7 Yb11
+        int zsel = (o < 4) ? ((o & 2) ? 1 : 0) : ((o & 2) ? 3 : 2);
+        bool kyw = (o & 1);
+        double2 acc = make_double2(0, 0);
+        for (int my = 0; my < side; my++) {
+            double2 zz = Zin[(((long long)mx * side + my) * nz + z) * 4 + zsel];
+            double2 e = Ey[(long long)my * ny + y];
+            double re = zz.x * e.x - zz.y * e.y, im = zz.x * e.y + zz.y * e.x;
+            if (kyw) { re *= kyv[my]; im *= kyv[my]; }
+            acc.x += re;
+            acc.y += im;
+        }
+        Y[t] = acc;
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+        double V = 0, h2 = 0, h3 = 0, h4 = 0, h5 = 0, h6 = 0, h7 = 0, h8 = 0;
+        for (int mx = 0; mx < side; mx++) {
+            double2 e = Ex[(long long)mx * nx + x];
+            double kx = kxv[mx];
+            const double2 *Yp = &Y[mx * 8];
+            // Re(Y e) and Im(Y e)
+#define RE(q) (Yp[q].x * e.x - Yp[q].y * e.y)
+#define IM(q) (Yp[q].x * e.y + Yp[q].y * e.x)
+            V += RE(0);
+            h5 += kx * RE(1);
+            h6 += kx * RE(2);
+            h7 += RE(3);
+            h2 += kx * IM(4);
+            h3 += IM(5);
+            h4 += IM(6);
+            h8 += kx * IM(7);
+#undef RE
+#undef IM
+        }
+        long long m = (long long)x + (long long)nx * ((long long)y + (long long)ny * z);
+        double *o = recip + 8 * m;
+        o[0] = V * charge1;
+        o[1] = -h2 * charge1;
+        o[2] = -h3 * charge1;
+        o[3] = -h4 * charge1;
+        o[4] = -h5 * charge1;
+        o[5] = -h6 * charge1;
+        o[6] = -h7 * charge1;
+        o[7] = -h8 * charge1;
+    }
+}
+
+// ---------------------------------------------------------- compaction
+// pmap_compact (pmap.F90:509-656) made order-independent: a node above the
+// cutoff is numbered by the first below-cutoff node (in the reference's
+// scan order i(x) outer, k(z) inner) that has it as a wrapped neighbour;
+// each below-cutoff node takes 1 + (number of neighbours it owns) indices,
+// assigned by an exclusive prefix sum in scan order.  Scan index
+// s = (i*ny + j)*nz + k is also the ptr layout index.
+__device__ __forceinline__ int wrapi(int a, int n) { return a >= n ? 0 : (a < 0 ? n - 1 : a); }
+
+__global__ void k_below(long long n, int nx, int ny, int nz, const float *__restrict__ full,
+                        float cut, uint8_t *__restrict__ below) {
+    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int k = (int)(s % nz), j = (int)((s / nz) % ny), i = (int)(s / ((long long)nz * ny));
+    long long m = (long long)i + (long long)nx * ((long long)j + (long long)ny * k);
+    below[s] = full[8 * m] < cut ? 1 : 0;
+}
+
+__global__ void k_owner(long long n, int nx, int ny, int nz, const uint8_t *__restrict__ below,
+                        int64_t *__restrict__ owner) {
+    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int k = (int)(s % nz), j = (int)((s / nz) % ny), i = (int)(s / ((long long)nz * ny));
+    long long best = -1;
+    if (!below[s]) {
+        for (int ii = -1; ii <= 1; ii++)
+            for (int jj = -1; jj <= 1; jj++)
+                for (int kk = -1; kk <= 1; kk++) {
+                    if (!ii && !jj && !kk) continue;
+                    long long t = ((long long)wrapi(i + ii, nx) * ny + wrapi(j + jj, ny)) * nz + wrapi(k + kk, nz);
+                    if (below[t] && (best < 0 || t < best)) best = t;
+                }
+    }
+    owner[s] = best;
+}
+
+__global__ void k_count(long long n, int nx, int ny, int nz, const uint8_t *__restrict__ below,
+                        const int64_t *__restrict__ owner, int64_t *__restrict__ cnt,
+                        unsigned long long *__restrict__ tallies) {
+    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long c = 0;
+    unsigned long long tb = 0, tn = 0, tr = 0;
+    if (s < n) {
+        int k = (int)(s % nz), j = (int)((s / nz) % ny), i = (int)(s / ((long long)nz * ny));
+        if (below[s]) {
+            c = 1;
+            tb = 1;
+            for (int ii = -1; ii <= 1; ii++)
+                for (int jj = -1; jj <= 1; jj++)
+                    for (int kk = -1; kk <= 1; kk++) {
+                        if (!ii && !jj && !kk) continue;
+                        long long t = ((long long)wrapi(i + ii, nx) * ny + wrapi(j + jj, ny)) * nz + wrapi(k + kk, nz);
+                        if (!below[t] && owner[t] == s) c++;
+                    }
+        } else {
+            long long o = owner[s];
+            if (o >= 0) tn = 1;
+            if (o < 0 || o > s) tr = 1;  // still unassigned when the scan reaches s
+        }
+        cnt[s] = c;
+    }
+    // wave reductions then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        tb += __shfl_down(tb, off);
+        tn += __shfl_down(tn, off);
+        tr += __shfl_down(tr, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&tallies[0], tb);
+        atomicAdd(&tallies[1], tn);
+        atomicAdd(&tallies[2], tr);
+    }
+}
+
+// exclusive scan, three phases: block sums, scan of block sums, add back
+constexpr int SCAN_B = 1024;
+__global__ void __launch_bounds__(SCAN_B) k_scan_blocks(long long n, const int64_t *__restrict__ in,
+                                                         int64_t *__restrict__ out,
+                                                         int64_t *__restrict__ bsum) {
+    __shared__ int64_t sh[SCAN_B];
+    long long i = (long long)blockIdx.x * SCAN_B + threadIdx.x;
+    int64_t v = (i < n) ? in[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < SCAN_B; off <<= 1) {
+        int64_t t = (threadIdx.x >= (unsigned)off) ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (i < n) out[i] = sh[threadIdx.x] - v;
+    if (threadIdx.x == SCAN_B - 1) bsum[blockIdx.x] = sh[SCAN_B - 1];
+}
+
+__global__ void k_scan_add(long long n, int64_t *__restrict__ out, const int64_t *__restrict__ boff) {
+    long long i = (long long)blockIdx.x * SCAN_B + threadIdx.x;
+    if (i < n) out[i] += boff[blockIdx.x];
+}
+
+__global__ void k_assign(long long n, int nx, int ny, int nz, const uint8_t *__restrict__ below,
+                         const int64_t *__restrict__ owner, const int64_t *__restrict__ base,
+                         const float *__restrict__ full, int32_t *__restrict__ ptr,
+                         float *__restrict__ cvals) {
+    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int k = (int)(s % nz), j = (int)((s / nz) % ny), i = (int)(s / ((long long)nz * ny));
+    if (below[s]) {
+        long long idx = base[s];
+        long long m = (long long)i + (long long)nx * ((long long)j + (long long)ny * k);
+        ptr[s] = (int32_t)(idx + 1);
+        const float4 *src = reinterpret_cast<const float4 *>(full + 8 * m);
+        float4 *dst = reinterpret_cast<float4 *>(cvals + 8 * idx);
+        dst[0] = src[0];
+        dst[1] = src[1];
+        for (int ii = -1; ii <= 1; ii++)
+            for (int jj = -1; jj <= 1; jj++)
+                for (int kk = -1; kk <= 1; kk++) {
+                    if (!ii && !jj && !kk) continue;
+                    int a = wrapi(i + ii, nx), bb = wrapi(j + jj, ny), c = wrapi(k + kk, nz);
+                    long long t = ((long long)a * ny + bb) * nz + c;
+                    if (!below[t] && owner[t] == s) {
+                        idx++;
+                        ptr[t] = (int32_t)(idx + 1);
+                        long long mt = (long long)a + (long long)nx * ((long long)bb + (long long)ny * c);
+                        const float4 *s2 = reinterpret_cast<const float4 *>(full + 8 * mt);
+                        float4 *d2 = reinterpret_cast<float4 *>(cvals + 8 * idx);
+                        d2[0] = s2[0];
+                        d2[1] = s2[1];
+                    }
+                }
+    } else if (owner[s] < 0) {
+        ptr[s] = 0;
+    }
+}
+
+}  // namespace mh
+
+using namespace mh;
+
+// ------------------------------------------------------------ host side
+static int scan_exclusive(hipStream_t st, long long n, const int64_t *in, int64_t *out,
+                          int64_t *total) {
+    long long nb = (n + SCAN_B - 1) / SCAN_B;
+    DBuf bsum, boff;
+    int rc;
+    if ((rc = bsum.alloc(sizeof(int64_t) * (nb + 1)))) return rc;
+    if ((rc = boff.alloc(sizeof(int64_t) * (nb + 1)))) return rc;
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(SCAN_B), 0, st, n, in, out,
+                       bsum.as<int64_t>());
+    MH_HIP(hipGetLastError());
+    if (nb > 1) {
+        int64_t sub = 0;
+        if ((rc = scan_exclusive(st, nb, bsum.as<int64_t>(), boff.as<int64_t>(), &sub))) return rc;
+        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(SCAN_B), 0, st, n, out,
+                           boff.as<int64_t>());
+        MH_HIP(hipGetLastError());
+    }
+    // total = last exclusive value + last input
+    int64_t lastx = 0, lasti = 0;
+    MH_HIP(hipMemcpyAsync(&lastx, out + (n - 1), sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    MH_HIP(hipMemcpyAsync(&lasti, in + (n - 1), sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    MH_HIP(hipStreamSynchronize(st));
+    *total = lastx + lasti;
+    return MH_OK;
+}
+
+static void grid_dims(const double eff[3], double gridspace, int np[3], double step[3]) {
+    // config_makegrid (config.F90:765-786)
+    for (int i = 0; i < 3; i++) {
+        np[i] = (int)(eff[i] / gridspace) + 1;
+        step[i] = eff[i] / (double)(np[i] - 1);
+    }
+}
+
+// separable reciprocal tables and stages; writes recip[nodes][8]
+static int build_recip(mh_ctx *ctx, const int np[3], const double step[3], double charge1,
+                       DBuf &recip) {
+    const Ewald &ew = ctx->ew;
+    const int K = ew.kmax, side = 2 * K + 1;
+    MH_CHECK(side <= MAXSIDE, MH_EINVAL, "Ewald kmax too large for the reciprocal kernel");
+    const long long nodes = (long long)np[0] * np[1] * np[2];
+    int rc;
+    if ((rc = recip.alloc(sizeof(double) * 8 * nodes))) return rc;
+    // per-axis k values (ewald.F90:599-619: k = (m * (1/L)) * twopi) and
+    // tables E[m][n] = exp(i k(m) * n * step)
+    std::vector<double> kv[3];
+    std::vector<double> E[3];
+    for (int a = 0; a < 3; a++) {
+        double il = 1.0 / ctx->full_ell[a];
+        kv[a].resize(side);
+        E[a].resize(2 * (size_t)side * np[a]);
+        for (int mm = 0; mm < side; mm++) {
+            double k = ((double)(mm - K) * il) * (2.0 * kPiR);
+            kv[a][mm] = k;
+            for (int n = 0; n < np[a]; n++) {
+                double arg = k * ((double)n * step[a]);
+                E[a][2 * ((size_t)mm * np[a] + n)] = std::cos(arg);
+                E[a][2 * ((size_t)mm * np[a] + n) + 1] = std::sin(arg);
+            }
+        }
+    }
+    DBuf dE[3], dk[3], Z;
+    for (int a = 0; a < 3; a++) {
+        if ((rc = dE[a].alloc(sizeof(double) * E[a].size()))) return rc;
+        if ((rc = dk[a].alloc(sizeof(double) * side))) return rc;
+        MH_HIP(hipMemcpyAsync(dE[a].p, E[a].data(), sizeof(double) * E[a].size(), hipMemcpyHostToDevice, ctx->stream));
+        MH_HIP(hipMemcpyAsync(dk[a].p, kv[a].data(), sizeof(double) * side, hipMemcpyHostToDevice, ctx->stream));
+    }
+    long long nzs = (long long)side * side * np[2];
+    if ((rc = Z.alloc(sizeof(double2) * 4 * nzs))) return rc;
+    hipLaunchKernelGGL(k_recip_z, dim3((unsigned)((nzs + 255) / 256)), dim3(256), 0, ctx->stream,
+                       side, np[2], ew.ccoef.as<double2>(), dE[2].as<double2>(), dk[2].as<double>(),
+                       Z.as<double2>());
+    MH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_recip_xy, dim3((unsigned)((long long)np[1] * np[2])), dim3(256), 0,
+                       ctx->stream, side, np[0], np[1], np[2], Z.as<double2>(), dE[0].as<double2>(),
+                       dE[1].as<double2>(), dk[0].as<double>(), dk[1].as<double>(), charge1,
+                       recip.as<double>());
+    MH_HIP(hipGetLastError());
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    return MH_OK;
+}
+
+// erfc(x) of A&S 7.1.26 bounded by 1.5e-7 + exp(-x^2) * t-polynomial; the
+// FP32 real-space cutoff is the distance where erfc(kappa r)/r < 1e-9 r^-1
+static double real_space_cutoff(double kappa) {
+    // smallest x with erfc(x) <= 1e-9 is ~4.32
+    return 4.32 / kappa;
+}
+
+extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q,
+                            double gridspace, double hipotcut_kJ, int keep_full, mh_map **out) {
+    MH_CHECK(ctx && out, MH_EINVAL, "mh_map_build: bad arguments");
+    MH_CHECK(ctx->natoms > 0, MH_ESTATE, "mh_map_build: upload the framework first");
+    MH_CHECK(kind == MH_MAP_LJ || kind == MH_MAP_EWALD || kind == MH_MAP_BUCK, MH_EINVAL,
+             "mh_map_build: unknown map kind");
+    MH_CHECK(precision == MH_PREC_FP64 || precision == MH_PREC_FP32, MH_EINVAL,
+             "mh_map_build: unknown precision");
+    MH_CHECK(gridspace > 0.0, MH_EINVAL, "mh_map_build: gridspace must be > 0");
+    MH_CHECK(kind != MH_MAP_LJ || ctx->lj_set, MH_ESTATE, "mh_map_build: call mh_ff_set_lj first");
+    MH_CHECK(kind != MH_MAP_BUCK || ctx->buck_set, MH_ESTATE, "mh_map_build: call mh_ff_set_buck first");
+    MH_CHECK(kind != MH_MAP_EWALD || ctx->ew.set, MH_ESTATE, "mh_map_build: call mh_ewald_set first");
+    const LJTable &lj = ctx->lj;
+    MH_CHECK(kind != MH_MAP_LJ || ctx->max_type < lj.ntypes, MH_EINVAL,
+             "mh_map_build: framework atom type without LJ parameters");
+    MH_CHECK(kind != MH_MAP_BUCK || ctx->max_type < ctx->buck.ntypes, MH_EINVAL,
+             "mh_map_build: framework atom type without Buckingham parameters");
+    MH_HIP(hipSetDevice(ctx->device));
+    int np[3];
+    double step[3];
+    grid_dims(ctx->fund_ell, gridspace, np, step);
+    MH_CHECK(np[0] >= 3 && np[1] >= 3 && np[2] >= 3, MH_EINVAL,
+             "mh_map_build: at least 3 grid nodes per axis are required");
+    const long long nodes = (long long)np[0] * np[1] * np[2];
+    MH_CHECK(nodes < (1LL << 31), MH_ECAPACITY, "mh_map_build: more than 2^31 grid nodes");
+    mh_map *map = new mh_map();
+    map->ctx = ctx;
+    for (int i = 0; i < 3; i++) {
+        map->npnts[i] = np[i];
+        map->step[i] = step[i];
+        map->fund_ell[i] = ctx->fund_ell[i];
+    }
+    int rc = MH_OK;
+    auto fail = [&](int code) {
+        delete map;
+        return code;
+    };
+    DBuf full32, full64, recip, dtab;
+    if ((rc = full32.alloc(sizeof(float) * 8 * nodes))) return fail(rc);
+    bool want64 = keep_full && true;
+    if (want64 && (rc = full64.alloc(sizeof(double) * 8 * nodes))) return fail(rc);
+    // pair tables
+    PairTab tab;
+    std::memset(&tab, 0, sizeof tab);
+    double maxh2 = 0.0, maxl2 = 0.0;
+    if (kind == MH_MAP_LJ) {
+        tab.ntypes = lj.ntypes;
+        for (int t = 0; t < lj.ntypes; t++) {
+            tab.A[t] = lj.A[t];
+            tab.B[t] = lj.B[t];
+            tab.h2[t] = lj.hicut2[t];
+            tab.l2[t] = lj.locut2[t];
+            if (lj.on[t]) {
+                maxh2 = fmax(maxh2, lj.hicut2[t]);
+                maxl2 = fmax(maxl2, lj.locut2[t]);
+            }
+        }
+    } else if (kind == MH_MAP_BUCK) {
+        const BuckTable &bk = ctx->buck;
+        tab.ntypes = bk.ntypes;
+        for (int t = 0; t < bk.ntypes; t++) {
+            tab.bA[t] = bk.A[t];
+            tab.brho[t] = bk.rho[t];
+            tab.bC[t] = bk.C[t];
+            tab.bD[t] = bk.D[t];
+            tab.h2[t] = bk.hicut2[t];
+            tab.l2[t] = bk.locut2[t];
+            maxh2 = fmax(maxh2, bk.hicut2[t]);
+        }
+    }
+    if ((rc = dtab.alloc(sizeof(PairTab)))) return fail(rc);
+    if (hipMemcpyAsync(dtab.p, &tab, sizeof tab, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        return fail(MH_EHIP);
+    GridP g;
+    g.nx = np[0];
+    g.ny = np[1];
+    g.nz = np[2];
+    for (int i = 0; i < 3; i++) {
+        g.step[i] = step[i];
+        g.full[i] = ctx->full_ell[i];
+    }
+    g.natoms = ctx->natoms;
+    g.kappa = ctx->ew.kappa;
+    g.charge1 = probe_q;
+    g.r2safe = 0.0;
+    g.rc2f = 0.0f;
+    if (kind == MH_MAP_EWALD) {
+        if ((rc = build_recip(ctx, np, step, probe_q, recip))) return fail(rc);
+        if (precision == MH_PREC_FP32) {
+            double rc_real = real_space_cutoff(ctx->ew.kappa);
+            g.cull2 = rc_real * rc_real * (1.0 + 1e-6);
+            g.rc2f = (float)(rc_real * rc_real);
+            g.r2safe = 0.25;  // pairs within 0.5 A evaluated in FP64
+        } else {
+            g.cull2 = 1e300;  // the reference sums every atom (ewald.F90:874)
+        }
+    } else {
+        g.cull2 = maxh2 * (1.0 + 1e-9) + 1e-9;
+        g.r2safe = fmax(2.25, 4.0 * maxl2);
+    }
+    const int nbx = (np[0] + BX - 1) / BX, nby = (np[1] + BY - 1) / BY, nbz = (np[2] + BZ - 1) / BZ;
+    const long long nblk = (long long)nbx * nby * nbz;
+    const bool fp32 = (precision == MH_PREC_FP32) && kind != MH_MAP_BUCK;
+    const PairTab *dt = dtab.as<PairTab>();
+    double *o64 = want64 ? full64.as<double>() : nullptr;
+    if (kind == MH_MAP_LJ) {
+        if (fp32)
+            hipLaunchKernelGGL(k_nodes32<K_LJ>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
+                               ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
+                               (const double *)nullptr, full32.as<float>(), o64);
+        else
+            hipLaunchKernelGGL(k_nodes64<K_LJ>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
+                               ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
+                               (const double *)nullptr, full32.as<float>(), o64);
+    } else if (kind == MH_MAP_BUCK) {
+        hipLaunchKernelGGL(k_nodes64<K_BUCK>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
+                           ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
+                           (const double *)nullptr, full32.as<float>(), o64);
+    } else {
+        if (fp32)
+            hipLaunchKernelGGL(k_nodes32<K_EWALD>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
+                               ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
+                               recip.as<double>(), full32.as<float>(), o64);
+        else
+            hipLaunchKernelGGL(k_nodes64<K_EWALD>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
+                               ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
+                               recip.as<double>(), full32.as<float>(), o64);
+    }
+    if (hipGetLastError() != hipSuccess) {
+        set_error("mh_map_build: node kernel launch failed");
+        return fail(MH_EHIP);
+    }
+    recip.release();
+    // compaction
+    DBuf below, owner, cnt, base, tallies;
+    if ((rc = below.alloc(nodes))) return fail(rc);
+    if ((rc = owner.alloc(sizeof(int64_t) * nodes))) return fail(rc);
+    if ((rc = cnt.alloc(sizeof(int64_t) * nodes))) return fail(rc);
+    if ((rc = base.alloc(sizeof(int64_t) * nodes))) return fail(rc);
+    if ((rc = tallies.alloc(sizeof(unsigned long long) * 4))) return fail(rc);
+    if ((rc = map->ptr.alloc(sizeof(int32_t) * nodes))) return fail(rc);
+    if (hipMemsetAsync(tallies.p, 0, sizeof(unsigned long long) * 4, ctx->stream) != hipSuccess ||
+        hipMemsetAsync(map->ptr.p, 0, sizeof(int32_t) * nodes, ctx->stream) != hipSuccess)
+        return fail(MH_EHIP);
+    float cut = (float)(hipotcut_kJ / 4.184);  // mapmaker.F90:562, Real(cutoff,RSgl)
+    unsigned gb = (unsigned)((nodes + 255) / 256);
+    hipLaunchKernelGGL(k_below, dim3(gb), dim3(256), 0, ctx->stream, nodes, np[0], np[1], np[2],
+                       full32.as<float>(), cut, below.as<uint8_t>());
+    hipLaunchKernelGGL(k_owner, dim3(gb), dim3(256), 0, ctx->stream, nodes, np[0], np[1], np[2],
+                       below.as<uint8_t>(), owner.as<int64_t>());
+    hipLaunchKernelGGL(k_count, dim3(gb), dim3(256), 0, ctx->stream, nodes, np[0], np[1], np[2],
+                       below.as<uint8_t>(), owner.as<int64_t>(), cnt.as<int64_t>(),
+                       tallies.as<unsigned long long>());
+    if (hipGetLastError() != hipSuccess) {
+        set_error("mh_map_build: compaction kernel launch failed");
+        return fail(MH_EHIP);
+    }
+    int64_t total = 0;
+    if ((rc = scan_exclusive(ctx->stream, nodes, cnt.as<int64_t>(), base.as<int64_t>(), &total)))
+        return fail(rc);
+    MH_CHECK(total < (1LL << 31), MH_ECAPACITY, "mh_map_build: more than 2^31 tabulated points");
+    map->nfsize = total;
+    if ((rc = map->vals.alloc(sizeof(float) * 8 * (total + 1)))) return fail(rc);
+    hipLaunchKernelGGL(k_assign, dim3(gb), dim3(256), 0, ctx->stream, nodes, np[0], np[1], np[2],
+                       below.as<uint8_t>(), owner.as<int64_t>(), base.as<int64_t>(),
+                       full32.as<float>(), map->ptr.as<int32_t>(), map->vals.as<float>());
+    if (hipGetLastError() != hipSuccess) {
+        set_error("mh_map_build: assign kernel launch failed");
+        return fail(MH_EHIP);
+    }
+    unsigned long long t[4];
+    if (hipMemcpyAsync(t, tallies.p, sizeof t, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        set_error("mh_map_build: synchronisation failed");
+        return fail(MH_EHIP);
+    }
+    map->counts[0] = (int64_t)t[0];
+    map->counts[1] = (int64_t)t[1];
+    map->counts[2] = total;
+    map->counts[3] = (int64_t)t[2];
+    if (keep_full) {
+        map->full32.p = full32.p;
+        map->full32.n = full32.n;
+        full32.p = nullptr;
+        if (want64) {
+            map->full64.p = full64.p;
+            map->full64.n = full64.n;
+            full64.p = nullptr;
+        }
+    }
+    *out = map;
+    return MH_OK;
+}

@@ -1,0 +1,327 @@
+// libmusic_hip: map object access, map file I/O and map interpolation.
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "mh_internal.h"
+
+namespace mh {
+
+// tricubic Hermite at one point (interpolate_int + pothermite,
+// interpolate.F90:104-158, 603-874); the 64-term sum runs corner-major in the
+// reference order, FP32 table values widened to FP64.
+__device__ __forceinline__ double h0f(double x) { return x * x * (3.0 - 2.0 * x); }
+__device__ __forceinline__ double h1f(double x, double ds) { return ds * x * x * (x - 1.0); }
+
+__device__ bool hermite_point(const int32_t *__restrict__ ptr, const float *__restrict__ vals,
+                              int nx, int ny, int nz, double sx, double sy, double sz,
+                              double ex, double ey, double ez, double rx, double ry, double rz,
+                              double &p) {
+    // simcell_maptouc (simcell.F90:634-668), orthorhombic, origin 0
+    double c[3] = {rx, ry, rz};
+    const double ell[3] = {ex, ey, ez};
+    const double st[3] = {sx, sy, sz};
+    const int nb[3] = {nx, ny, nz};
+    int idx[3], idp[3];
+    double tuv[3];
+    for (int d = 0; d < 3; d++) {
+        double f = c[d] * (1.0 / ell[d]);
+        f = (f >= 0.0) ? fmod(f, 1.0) : fmod(f, 1.0) + 1.0;
+        c[d] = f * ell[d];
+        idx[d] = (int)(c[d] / st[d] + 1.0);  // mapheader_getIndices, Int()
+        if (idx[d] < 1 || idx[d] > nb[d]) {
+            p = 0.0;
+            return false;
+        }
+        tuv[d] = (c[d] - (double)(idx[d] - 1) * st[d]) / st[d];
+        idp[d] = (idx[d] == nb[d]) ? 1 : idx[d] + 1;
+    }
+#define P(i, j, k) ptr[(((long long)(i) - 1) * ny + ((j) - 1)) * nz + ((k) - 1)]
+    const int ii = idx[0], jj = idx[1], kk = idx[2], ip = idp[0], jp = idp[1], kp = idp[2];
+    int32_t mi[8] = {P(ii, jj, kk), P(ip, jj, kk), P(ii, jp, kk), P(ii, jj, kp),
+                     P(ip, jp, kk), P(ip, jj, kp), P(ii, jp, kp), P(ip, jp, kp)};
+#undef P
+    for (int i = 0; i < 8; i++)
+        if (mi[i] == 0) {
+            p = 0.0;
+            return false;
+        }
+    double hx[2][2], hy[2][2], hz[2][2];
+    hx[0][0] = h0f(1 - tuv[0]); hx[1][0] = h0f(tuv[0]);
+    hx[0][1] = h1f(1 - tuv[0], -sx); hx[1][1] = h1f(tuv[0], sx);
+    hy[0][0] = h0f(1 - tuv[1]); hy[1][0] = h0f(tuv[1]);
+    hy[0][1] = h1f(1 - tuv[1], -sy); hy[1][1] = h1f(tuv[1], sy);
+    hz[0][0] = h0f(1 - tuv[2]); hz[1][0] = h0f(tuv[2]);
+    hz[0][1] = h1f(1 - tuv[2], -sz); hz[1][1] = h1f(tuv[2], sz);
+    // corner pattern and value-slot pattern (interpolate.F90:725-835)
+    const int CX = 0xB2, CY = 0xD4, CZ = 0xE8;  // bit i: upper node along axis for corner i
+    double sum = 0.0;
+    for (int i = 0; i < 8; i++) {
+        const float4 *r = reinterpret_cast<const float4 *>(vals + 8 * (long long)(mi[i] - 1));
+        float4 a = r[0], b = r[1];
+        const float f[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        const int cx = (CX >> i) & 1, cy = (CY >> i) & 1, cz = (CZ >> i) & 1;
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const int dx = (CX >> s) & 1, dy = (CY >> s) & 1, dz = (CZ >> s) & 1;
+            sum = __dadd_rn(sum, __dmul_rn(__dmul_rn(__dmul_rn((double)f[s], hx[cx][dx]), hy[cy][dy]), hz[cz][dz]));
+        }
+    }
+    p = sum;
+    return true;
+}
+
+__global__ void k_interp(int n, const double *__restrict__ xyz, const int32_t *__restrict__ ptr,
+                         const float *__restrict__ vals, int nx, int ny, int nz, double sx,
+                         double sy, double sz, double ex, double ey, double ez, double factor,
+                         double *__restrict__ e, uint8_t *__restrict__ ok) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double p;
+    bool good = hermite_point(ptr, vals, nx, ny, nz, sx, sy, sz, ex, ey, ez, xyz[3 * i],
+                              xyz[3 * i + 1], xyz[3 * i + 2], p);
+    e[i] = good ? p * factor : 0.0;
+    ok[i] = good ? 1 : 0;
+}
+
+// SoA [8][nf] <-> AoS [nf][8]
+__global__ void k_aos_to_soa(long long nf, const float *__restrict__ aos, float *__restrict__ soa) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    for (int a = 0; a < 8; a++) soa[a * nf + i] = aos[8 * i + a];
+}
+
+__global__ void k_soa_to_aos(long long nf, const float *__restrict__ soa, float *__restrict__ aos) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    for (int a = 0; a < 8; a++) aos[8 * i + a] = soa[a * nf + i];
+}
+
+// Fortran unformatted sequential record I/O (4-byte markers)
+static bool rec_write(FILE *f, const void *buf, size_t n) {
+    if (n > 0x7fffffffULL) return false;
+    uint32_t m = (uint32_t)n;
+    return fwrite(&m, 4, 1, f) == 1 && (n == 0 || fwrite(buf, 1, n, f) == n) && fwrite(&m, 4, 1, f) == 1;
+}
+
+static bool rec_read(FILE *f, void *buf, size_t cap, size_t *got) {
+    uint32_t m1, m2;
+    if (fread(&m1, 4, 1, f) != 1 || m1 > cap) return false;
+    if (m1 && fread(buf, 1, m1, f) != m1) return false;
+    if (fread(&m2, 4, 1, f) != 1 || m1 != m2) return false;
+    if (got) *got = m1;
+    return true;
+}
+
+static void padcpy(char *dst, const char *src, size_t n) {
+    size_t l = 0;
+    while (l < n && src[l]) l++;
+    memset(dst, ' ', n);
+    memcpy(dst, src, l);
+}
+
+}  // namespace mh
+
+using namespace mh;
+
+extern "C" {
+
+int mh_map_destroy(mh_map *map) {
+    if (!map) return MH_OK;
+    if (map->ctx) (void)hipSetDevice(map->ctx->device);
+    delete map;
+    return MH_OK;
+}
+
+int mh_map_info(const mh_map *map, int npnts[3], int64_t *nfsize, int64_t counts[4], double step[3]) {
+    MH_CHECK(map, MH_EINVAL, "mh_map_info: map is NULL");
+    for (int i = 0; i < 3; i++) {
+        if (npnts) npnts[i] = map->npnts[i];
+        if (step) step[i] = map->step[i];
+    }
+    if (nfsize) *nfsize = map->nfsize;
+    if (counts)
+        for (int i = 0; i < 4; i++) counts[i] = map->counts[i];
+    return MH_OK;
+}
+
+int mh_map_get(const mh_map *map, int32_t *ptr, float *vals8) {
+    MH_CHECK(map, MH_EINVAL, "mh_map_get: map is NULL");
+    mh_ctx *ctx = map->ctx;
+    MH_HIP(hipSetDevice(ctx->device));
+    const long long nodes = (long long)map->npnts[0] * map->npnts[1] * map->npnts[2];
+    if (ptr) MH_HIP(hipMemcpy(ptr, map->ptr.p, sizeof(int32_t) * nodes, hipMemcpyDeviceToHost));
+    if (vals8 && map->nfsize > 0) {
+        DBuf soa;
+        int rc;
+        if ((rc = soa.alloc(sizeof(float) * 8 * map->nfsize))) return rc;
+        hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((map->nfsize + 255) / 256)), dim3(256), 0,
+                           ctx->stream, (long long)map->nfsize, map->vals.as<float>(), soa.as<float>());
+        MH_HIP(hipGetLastError());
+        MH_HIP(hipStreamSynchronize(ctx->stream));
+        MH_HIP(hipMemcpy(vals8, soa.p, sizeof(float) * 8 * map->nfsize, hipMemcpyDeviceToHost));
+    }
+    return MH_OK;
+}
+
+int mh_map_get_full(const mh_map *map, float *f32, double *f64) {
+    MH_CHECK(map, MH_EINVAL, "mh_map_get_full: map is NULL");
+    MH_CHECK(map->full32.p, MH_ESTATE, "mh_map_get_full: map was built without keep_full");
+    MH_HIP(hipSetDevice(map->ctx->device));
+    const long long nodes = (long long)map->npnts[0] * map->npnts[1] * map->npnts[2];
+    if (f32) MH_HIP(hipMemcpy(f32, map->full32.p, sizeof(float) * 8 * nodes, hipMemcpyDeviceToHost));
+    if (f64) {
+        MH_CHECK(map->full64.p, MH_ESTATE, "mh_map_get_full: no FP64 node values kept");
+        MH_HIP(hipMemcpy(f64, map->full64.p, sizeof(double) * 8 * nodes, hipMemcpyDeviceToHost));
+    }
+    return MH_OK;
+}
+
+int mh_map_write(const mh_map *map, const char *path, int nzatoms, const char *names25,
+                 const char *symbols2, const float *zparams7, float pcut_kJ) {
+    MH_CHECK(map && path, MH_EINVAL, "mh_map_write: bad arguments");
+    MH_CHECK(nzatoms >= 0 && (nzatoms == 0 || (names25 && symbols2 && zparams7)), MH_EINVAL,
+             "mh_map_write: bad zatom arrays");
+    const long long nodes = (long long)map->npnts[0] * map->npnts[1] * map->npnts[2];
+    std::vector<int32_t> ptr(nodes);
+    std::vector<float> v8(8 * (size_t)map->nfsize);
+    int rc = mh_map_get(map, ptr.data(), v8.data());
+    if (rc) return rc;
+    FILE *f = fopen(path, "wb");
+    MH_CHECK(f, MH_EIO, std::string("mh_map_write: cannot open ") + path);
+    bool ok = true;
+    char s40[40], s60[60], s25[25], s27[27];
+    float rj = 0.0f;
+    int32_t ij = 0, zt[11] = {0};
+    padcpy(s40, "file contains header", 40);
+    padcpy(s60, "Old type header 2003", 60);
+    padcpy(s25, "probe", 25);
+    ok = ok && rec_write(f, s40, 40) && rec_write(f, s60, 60) && rec_write(f, s25, 25) &&
+         rec_write(f, &rj, 4);
+    int32_t nz = nzatoms;
+    ok = ok && rec_write(f, &nz, 4) && rec_write(f, zt, sizeof zt);
+    for (int i = 0; i < nzatoms && ok; i++) {
+        memcpy(s27, names25 + 25 * i, 25);
+        memcpy(s27 + 25, symbols2 + 2 * i, 2);
+        const float *zp = zparams7 + 7 * i;
+        ok = rec_write(f, s27, 27) && rec_write(f, zp, 4) && rec_write(f, zp + 1, 16) &&
+             rec_write(f, zp + 5, 8);
+    }
+    int32_t nb[3] = {map->npnts[0], map->npnts[1], map->npnts[2]};
+    int32_t nf = (int32_t)map->nfsize;
+    ok = ok && rec_write(f, &ij, 4) && rec_write(f, &pcut_kJ, 4) && rec_write(f, &rj, 4) &&
+         rec_write(f, &rj, 4) && rec_write(f, &ij, 4) && rec_write(f, &rj, 4) &&
+         rec_write(f, nb, 12) && rec_write(f, &nf, 4) && rec_write(f, ptr.data(), 4 * (size_t)nodes);
+    for (int a = 0; a < 8 && ok; a++)
+        ok = rec_write(f, v8.data() + (size_t)a * map->nfsize, 4 * (size_t)map->nfsize);
+    if (fclose(f) != 0) ok = false;
+    MH_CHECK(ok, MH_EIO, std::string("mh_map_write: write failed for ") + path);
+    return MH_OK;
+}
+
+static int upload_map(mh_ctx *ctx, const int np[3], int64_t nf, const int32_t *ptr, const float *v8,
+                      const double fund_ell[3], mh_map **out) {
+    MH_CHECK(np[0] >= 2 && np[1] >= 2 && np[2] >= 2, MH_EINVAL, "map: bad grid size");
+    MH_CHECK(nf >= 0 && nf < (1LL << 31), MH_EINVAL, "map: bad nfsize");
+    const long long nodes = (long long)np[0] * np[1] * np[2];
+    for (long long i = 0; i < nodes; i++)
+        MH_CHECK(ptr[i] >= 0 && ptr[i] <= nf, MH_EINVAL, "map: ptr entry out of range");
+    MH_HIP(hipSetDevice(ctx->device));
+    mh_map *m = new mh_map();
+    m->ctx = ctx;
+    for (int i = 0; i < 3; i++) {
+        m->npnts[i] = np[i];
+        m->fund_ell[i] = fund_ell[i];
+        m->step[i] = fund_ell[i] / ((double)np[i] - 1.0);  // mapheader.F90:140-146
+    }
+    m->nfsize = nf;
+    m->counts[2] = nf;
+    int rc;
+    DBuf soa;
+    if ((rc = m->ptr.alloc(sizeof(int32_t) * nodes)) || (rc = m->vals.alloc(sizeof(float) * 8 * (nf + 1))) ||
+        (rc = soa.alloc(sizeof(float) * 8 * (nf + 1)))) {
+        delete m;
+        return rc;
+    }
+    if (hipMemcpy(m->ptr.p, ptr, sizeof(int32_t) * nodes, hipMemcpyHostToDevice) != hipSuccess ||
+        (nf && hipMemcpy(soa.p, v8, sizeof(float) * 8 * nf, hipMemcpyHostToDevice) != hipSuccess)) {
+        delete m;
+        set_error("map upload failed");
+        return MH_EHIP;
+    }
+    if (nf) {
+        hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (long long)nf, soa.as<float>(), m->vals.as<float>());
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            delete m;
+            set_error("map upload: transpose failed");
+            return MH_EHIP;
+        }
+    }
+    *out = m;
+    return MH_OK;
+}
+
+int mh_map_from_host(mh_ctx *ctx, const int npnts[3], int64_t nfsize, const int32_t *ptr,
+                     const float *vals8, const double fund_ell[3], mh_map **out) {
+    MH_CHECK(ctx && npnts && ptr && (vals8 || nfsize == 0) && fund_ell && out, MH_EINVAL,
+             "mh_map_from_host: bad arguments");
+    return upload_map(ctx, npnts, nfsize, ptr, vals8, fund_ell, out);
+}
+
+int mh_map_load(mh_ctx *ctx, const char *path, const double fund_ell[3], mh_map **out) {
+    MH_CHECK(ctx && path && fund_ell && out, MH_EINVAL, "mh_map_load: bad arguments");
+    FILE *f = fopen(path, "rb");
+    MH_CHECK(f, MH_EIO, std::string("mh_map_load: cannot open ") + path);
+    char buf[128];
+    size_t n = 0;
+    bool ok = rec_read(f, buf, sizeof buf, &n) && n == 40 && strncmp(buf, "file contains header", 20) == 0;
+    for (int r = 0; r < 3 && ok; r++) ok = rec_read(f, buf, sizeof buf, &n);
+    int32_t nz = 0;
+    ok = ok && rec_read(f, &nz, 4, &n) && nz >= 0 && nz < 100000 && rec_read(f, buf, sizeof buf, &n);
+    for (int i = 0; i < nz && ok; i++)
+        for (int r = 0; r < 4 && ok; r++) ok = rec_read(f, buf, sizeof buf, &n);
+    for (int r = 0; r < 6 && ok; r++) ok = rec_read(f, buf, sizeof buf, &n);
+    int32_t nb[3] = {0, 0, 0}, nf = 0;
+    ok = ok && rec_read(f, nb, 12, &n) && n == 12 && rec_read(f, &nf, 4, &n);
+    ok = ok && nb[0] > 1 && nb[1] > 1 && nb[2] > 1 && nf >= 0;
+    std::vector<int32_t> ptr;
+    std::vector<float> v8;
+    if (ok) {
+        long long nodes = (long long)nb[0] * nb[1] * nb[2];
+        ptr.resize(nodes);
+        v8.resize(8 * (size_t)nf);
+        ok = rec_read(f, ptr.data(), 4 * (size_t)nodes, &n) && n == 4 * (size_t)nodes;
+        for (int a = 0; a < 8 && ok; a++)
+            ok = rec_read(f, v8.data() + (size_t)a * nf, 4 * (size_t)nf, &n) && n == 4 * (size_t)nf;
+    }
+    fclose(f);
+    MH_CHECK(ok, MH_EIO, std::string("mh_map_load: not a map file or truncated: ") + path);
+    int np[3] = {nb[0], nb[1], nb[2]};
+    return upload_map(ctx, np, nf, ptr.data(), v8.data(), fund_ell, out);
+}
+
+int mh_map_interp(mh_ctx *ctx, const mh_map *map, int n, const double *xyz, double factor,
+                  double *e, uint8_t *ok) {
+    MH_CHECK(ctx && map && (n == 0 || (xyz && e && ok)), MH_EINVAL, "mh_map_interp: bad arguments");
+    MH_CHECK(n >= 0, MH_EINVAL, "mh_map_interp: n < 0");
+    if (n == 0) return MH_OK;
+    MH_HIP(hipSetDevice(ctx->device));
+    DBuf dx, de, dok;
+    int rc;
+    if ((rc = dx.alloc(sizeof(double) * 3 * n)) || (rc = de.alloc(sizeof(double) * n)) ||
+        (rc = dok.alloc(n)))
+        return rc;
+    MH_HIP(hipMemcpyAsync(dx.p, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_interp, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, dx.as<double>(),
+                       map->ptr.as<int32_t>(), map->vals.as<float>(), map->npnts[0], map->npnts[1],
+                       map->npnts[2], map->step[0], map->step[1], map->step[2], map->fund_ell[0],
+                       map->fund_ell[1], map->fund_ell[2], factor, de.as<double>(), dok.as<uint8_t>());
+    MH_HIP(hipGetLastError());
+    MH_HIP(hipMemcpyAsync(e, de.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipMemcpyAsync(ok, dok.p, n, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    return MH_OK;
+}
+
+}  // extern "C"

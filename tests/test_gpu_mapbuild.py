@@ -1,0 +1,221 @@
+"""GPU map build (libmusic_hip) against the CPU oracle and the reference's
+golden values.  FP64 mode: node values within 1e-10 relative, integer
+grid/compaction bit-exact.  FP32 mode: error against FP64 stated and bounded."""
+import hashlib
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from musichip import hip as H
+from musichip import inputs, synth
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TOL64 = 1e-10
+
+
+def rel_close(got, ref, tol, what):
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    scale = np.max(np.abs(ref), axis=0, keepdims=True)
+    err = np.abs(got - ref) / (np.abs(ref) + 1e-6 * scale + 1e-300)
+    worst = float(np.max(err)) if err.size else 0.0
+    print("%s: max scaled rel err %.3e" % (what, worst), flush=True)
+    assert worst <= tol, what
+
+
+def _system(name):
+    return inputs.load_system(os.path.join(GOLD, name))
+
+
+def _grid(s, job):
+    npn = (s["fund_ell"] / job["gridspace"]).astype(int) + 1
+    return npn, s["fund_ell"] / (npn - 1)
+
+
+@pytest.fixture(scope="module")
+def t012():
+    s = _system("test012-pmapgen/test012.ctr")
+    fw = s["mols"]["Lattice"]
+    A, B, h2, l2, on = inputs.lj_table_for_probe(s, int(s["mols"]["Probe"]["type"][0]))
+    job = s["ctrl"]["mapmaker"][0]
+    npn, step = _grid(s, job)
+    ofw = dict(xyz=fw["full_xyz"], type=fw["full_type"], A=A, B=B, hicut2=h2, locut2=l2,
+               full_ell=s["full_ell"])
+    ref = O.build_nodes(1, ofw, npn, step)
+    return dict(s=s, fw=fw, lj=(A, B, h2, l2, on), job=job, npn=npn, step=step, ref=ref)
+
+
+def test_lj_map_fp64_matches_oracle_and_goldens(hip, t012):
+    s, fw, job = t012["s"], t012["fw"], t012["job"]
+    print("case: test012 LJ map FP64", flush=True)
+    hip.upload_framework(fw["full_xyz"], fw["full_type"], fw["full_q"], s["fund_ell"], s["full_ell"])
+    hip.set_lj(*t012["lj"])
+    m = hip.build_map(H.MH_MAP_LJ, H.MH_PREC_FP64, job["gridspace"], job["hipotcut"], keep_full=True)
+    assert list(m.npnts) == [50, 50, 30]
+    assert list(m.counts) == [28904, 20544, 49448, 29627]
+    f32, f64 = m.get_full()
+    rel_close(f64, t012["ref"], TOL64, "test012 LJ nodes FP64")
+    assert np.array_equal(f32, t012["ref"].astype(np.float32)) or \
+        np.max(np.abs(f32.view(np.int32).astype(np.int64) - t012["ref"].astype(np.float32).view(np.int32))) <= 1
+    ptr_o, src_o, counts_o = O.compact(t012["npn"], t012["ref"][:, 0].astype(np.float32),
+                                       job["hipotcut"] / 4.184)
+    ptr, vals = m.get()
+    assert np.array_equal(ptr, ptr_o)
+    np.testing.assert_array_equal(vals, f32[src_o].T)
+    m.close()
+
+
+def test_lj_map_file_bytes_match_oracle_writer(hip, t012):
+    s, fw, job = t012["s"], t012["fw"], t012["job"]
+    hip.upload_framework(fw["full_xyz"], fw["full_type"], fw["full_q"], s["fund_ell"], s["full_ell"])
+    hip.set_lj(*t012["lj"])
+    m = hip.build_map(H.MH_MAP_LJ, H.MH_PREC_FP64, job["gridspace"], job["hipotcut"])
+    ptr, vals = m.get()
+    zat = [dict(name="Latom", symbol="L", charge=1.0, A12=t012["lj"][0][1], B6=t012["lj"][1][1],
+                rcuthi=5.9, rcutlo=0.1)]
+    with tempfile.TemporaryDirectory() as td:
+        a, b = os.path.join(td, "gpu.map"), os.path.join(td, "ora.map")
+        m.write(a, zat, job["hipotcut"])
+        O.map_write(b, zat, job["hipotcut"], m.npnts, ptr, vals)
+        da, db = open(a, "rb").read(), open(b, "rb").read()
+        assert hashlib.sha256(da).hexdigest() == hashlib.sha256(db).hexdigest()
+        m2 = hip.load_map(a, s["fund_ell"])
+        p2, v2 = m2.get()
+        assert np.array_equal(p2, ptr) and np.array_equal(v2, vals)
+
+
+def test_lj_map_fp32_error_vs_fp64(hip, t012):
+    s, fw, job = t012["s"], t012["fw"], t012["job"]
+    hip.upload_framework(fw["full_xyz"], fw["full_type"], fw["full_q"], s["fund_ell"], s["full_ell"])
+    hip.set_lj(*t012["lj"])
+    m = hip.build_map(H.MH_MAP_LJ, H.MH_PREC_FP32, job["gridspace"], job["hipotcut"], keep_full=True)
+    f32, acc = m.get_full()
+    ref = t012["ref"]
+    live = ref[:, 0] < job["hipotcut"] / 4.184  # tabulated region
+    err = np.abs(acc[live, 0] - ref[live, 0]) / np.maximum(np.abs(ref[live, 0]), 1e-3)
+    print("FP32 LJ V max rel err (tabulated nodes): %.3e" % err.max(), flush=True)
+    assert err.max() < 1e-5
+    assert list(m.counts) == [28904, 20544, 49448, 29627]
+
+
+@pytest.fixture(scope="module")
+def t013():
+    s = _system("test013-emapgen/test013.ctr")
+    fw = s["mols"]["Lattice"]
+    e = s["ss"][("Probe", "Lattice", "COUL")]["ewald"]
+    sf = O.ewald_sfactor(fw["full_xyz"], fw["full_q"], s["full_ell"], e["kappa"], e["kmax"], e["locut"])
+    job = s["ctrl"]["mapmaker"][0]
+    npn, step = _grid(s, job)
+    d = dict(xyz=fw["full_xyz"], q=fw["full_q"], charge1=1.0, kappa=sf["kappa"], k=sf["k"],
+             s=sf["s"], si=sf["si"], full_ell=s["full_ell"])
+    ref = O.build_nodes(2, d, npn, step)
+    return dict(s=s, fw=fw, e=e, sf=sf, job=job, npn=npn, step=step, ref=ref)
+
+
+def test_ewald_map_fp64_matches_oracle(hip, t013):
+    s, fw, e, job = t013["s"], t013["fw"], t013["e"], t013["job"]
+    print("case: test013 Ewald map FP64", flush=True)
+    hip.upload_framework(fw["full_xyz"], fw["full_type"], fw["full_q"], s["fund_ell"], s["full_ell"])
+    nk = hip.set_ewald(e["kappa"], e["kmax"], e["locut"])
+    assert nk == 158
+    kk = hip.ewald_get(nk)
+    np.testing.assert_allclose(kk["k"], t013["sf"]["k"], rtol=0, atol=0)
+    np.testing.assert_allclose(kk["s"], t013["sf"]["s"], rtol=1e-12, atol=1e-15)
+    m = hip.build_map(H.MH_MAP_EWALD, H.MH_PREC_FP64, job["gridspace"], job["hipotcut"],
+                      probe_q=1.0, keep_full=True)
+    assert list(m.counts) == [12240, 1360, 13600, 10200]
+    f32, f64 = m.get_full()
+    rel_close(f64, t013["ref"], TOL64, "test013 Ewald nodes FP64")
+    ptr_o, _, _ = O.compact(t013["npn"], t013["ref"][:, 0].astype(np.float32), job["hipotcut"] / 4.184)
+    ptr, _ = m.get()
+    assert np.array_equal(ptr, ptr_o)
+
+
+def test_ewald_map_fp32_error_vs_fp64(hip, t013):
+    s, fw, e, job = t013["s"], t013["fw"], t013["e"], t013["job"]
+    hip.upload_framework(fw["full_xyz"], fw["full_type"], fw["full_q"], s["fund_ell"], s["full_ell"])
+    hip.set_ewald(e["kappa"], e["kmax"], e["locut"])
+    m = hip.build_map(H.MH_MAP_EWALD, H.MH_PREC_FP32, job["gridspace"], job["hipotcut"],
+                      probe_q=1.0, keep_full=True)
+    f32, acc = m.get_full()
+    ref = t013["ref"]
+    live = ref[:, 0] < job["hipotcut"] / 4.184
+    scale = np.max(np.abs(ref[live, 0]))
+    err = np.abs(acc[live, 0] - ref[live, 0]) / np.maximum(np.abs(ref[live, 0]), 1e-3 * scale)
+    print("FP32 Ewald V max rel err (tabulated nodes): %.3e" % err.max(), flush=True)
+    assert err.max() < 1e-5
+
+
+def test_buck_map_fp64_matches_oracle(hip):
+    rng = np.random.default_rng(5)
+    L = np.array([14.0, 15.0, 13.0])
+    xyz = rng.uniform(0, 1, (60, 3)) * L
+    typ = rng.integers(0, 3, 60).astype(np.int32)
+    A = np.array([20000.0, 30000.0, 25000.0])
+    rho = np.array([0.30, 0.28, 0.33])
+    Cc = np.array([300.0, 250.0, 400.0])
+    D = np.array([0.0, 50.0, 10.0])
+    h2 = np.full(3, 6.5 ** 2)
+    l2 = np.full(3, 0.5 ** 2)
+    hip.upload_framework(xyz, typ, None, L, L)
+    hip.set_buck(A, rho, Cc, D, h2, l2)
+    m = hip.build_map(H.MH_MAP_BUCK, H.MH_PREC_FP64, 0.5, 100.0, keep_full=True)
+    npn = (L / 0.5).astype(int) + 1
+    step = L / (npn - 1)
+    ref = O.build_nodes(3, dict(xyz=xyz, type=typ, bA=A, brho=rho, bC=Cc, bD=D, hicut2=h2, locut2=l2,
+                                full_ell=L), npn, step)
+    _, f64 = m.get_full()
+    rel_close(f64, ref, 1e-12, "Buckingham nodes")
+
+
+def test_interp_matches_oracle_hermite(hip, t012):
+    s, fw, job = t012["s"], t012["fw"], t012["job"]
+    hip.upload_framework(fw["full_xyz"], fw["full_type"], fw["full_q"], s["fund_ell"], s["full_ell"])
+    hip.set_lj(*t012["lj"])
+    m = hip.build_map(H.MH_MAP_LJ, H.MH_PREC_FP64, job["gridspace"], job["hipotcut"])
+    ptr, vals = m.get()
+    rng = np.random.default_rng(99)
+    pts = rng.uniform(-0.3, 1.3, (4000, 3)) * s["full_ell"]
+    ok, e = m.interp(pts, 1.0)
+    ok_o, e_o = O.pothermite(m.npnts, m.step, ptr, vals, pts, fund_ell=s["fund_ell"])
+    assert np.array_equal(ok, ok_o)
+    assert ok.sum() > 1000
+    np.testing.assert_array_equal(e[ok], e_o[ok])
+
+
+def test_small_charged_config_fp64_and_fp32(hip):
+    # a reduced version of the charged-framework configuration
+    rng = np.random.default_rng(2002)
+    L = np.array([20.0, 20.0, 20.0])
+    xyz = rng.uniform(0, 1, (300, 3)) * L
+    q = rng.uniform(-1, 1, 300)
+    q -= q.mean()
+    typ = rng.integers(0, synth.NTYPES_FW, 300).astype(np.int32)
+    eps, sig = rng.uniform(20, 80, synth.NTYPES_FW), rng.uniform(2.5, 3.8, synth.NTYPES_FW)
+    hip.upload_framework(xyz, typ, q, L, L)
+    nk = hip.set_ewald(5.0, 9, 1e-10)
+    sf = O.ewald_sfactor(xyz, q, L, 5.0, 9, 1e-10)
+    assert nk == sf["nk"]
+    m = hip.build_map(H.MH_MAP_EWALD, H.MH_PREC_FP64, 0.5, 100.0, probe_q=0.7, keep_full=True)
+    npn = (L / 0.5).astype(int) + 1
+    step = L / (npn - 1)
+    ref = O.build_nodes(2, dict(xyz=xyz, q=q, charge1=0.7, kappa=sf["kappa"], k=sf["k"], s=sf["s"],
+                                si=sf["si"], full_ell=L), npn, step)
+    _, f64 = m.get_full()
+    rel_close(f64, ref, TOL64, "charged box Ewald FP64")
+    A, B, h2, l2 = synth.lb_table(eps, sig, 27.0, 2.80, 9.5)
+    hip.set_lj(A, B, h2, l2)
+    m2 = hip.build_map(H.MH_MAP_LJ, H.MH_PREC_FP64, 0.5, 100.0, keep_full=True)
+    ref2 = O.build_nodes(1, dict(xyz=xyz, type=typ, A=A, B=B, hicut2=h2, locut2=l2, full_ell=L), npn, step)
+    _, g64 = m2.get_full()
+    rel_close(g64, ref2, TOL64, "charged box LJ FP64")
+    m3 = hip.build_map(H.MH_MAP_LJ, H.MH_PREC_FP32, 0.5, 100.0, keep_full=True)
+    _, g32 = m3.get_full()
+    live = ref2[:, 0] < 100.0 / 4.184
+    err = np.abs(g32[live, 0] - ref2[live, 0]) / np.maximum(np.abs(ref2[live, 0]), 1e-3)
+    print("charged box FP32 LJ V max rel err: %.3e" % err.max(), flush=True)
+    assert err.max() < 1e-5
