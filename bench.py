@@ -1,0 +1,204 @@
+"""Benchmark: potential-map build of the charged-framework configuration on
+one MI355X (libmusic_hip), with the CPU oracle timed beside it.
+
+Workload (one step): the three maps a 3-site charged sorbate needs in a
+50 A charged framework -- LJ pmap for the C probe, LJ pmap for the O probe
+and the Ewald emap -- each at 0.15 A (334^3 = 37,259,704 nodes), computed in
+FP32 pair arithmetic with FP64 Kahan accumulation, compacted on the device.
+A map point is one grid node with all 8 values computed.
+
+Prints ONE JSON line.  python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "music-code_amd"))
+
+METRIC = "GCMC trial moves/s + map pts/s, FP32-acc vs FP64; p50/p99 request latency"
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, dense
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def setup(ctx, cfg):
+    from musichip import synth
+    ctx.upload_framework(cfg["xyz"], cfg["type"], cfg["q"], cfg["fund_ell"], cfg["full_ell"])
+    nk = ctx.set_ewald(cfg["kappa_param"], cfg["kmax"], cfg["ewald_locut"])
+    tabs = []
+    for p in cfg["probes"]:
+        tabs.append(synth.lb_table(cfg["fw_eps"], cfg["fw_sig"], p["eps"], p["sig"], cfg["hicut"]))
+    return nk, tabs
+
+
+def one_step(ctx, H, cfg, tabs, prec, keep=False):
+    maps = []
+    for tab, p in zip(tabs, cfg["probes"]):
+        ctx.set_lj(*tab)
+        maps.append(ctx.build_map(H.MH_MAP_LJ, prec, cfg["gridspace"], cfg["hipotcut"], keep_full=keep))
+    maps.append(ctx.build_map(H.MH_MAP_EWALD, prec, cfg["gridspace"], cfg["hipotcut"], probe_q=1.0,
+                              keep_full=keep))
+    return maps
+
+
+def cpu_baseline(cfg, tabs, seconds=15.0):
+    """The oracle (-O3 -march=native, one thread) on a bounded sample of the
+    same nodes: the first nodes of each of the three maps."""
+    from oracle import oracle as O
+    L = O.lib(native=True)
+    O._LIB = L  # build_nodes uses the loaded library
+    npn = (cfg["fund_ell"] / cfg["gridspace"]).astype(int) + 1
+    step = cfg["fund_ell"] / (npn - 1)
+    sf = O.ewald_sfactor(cfg["xyz"], cfg["q"], cfg["full_ell"], cfg["kappa_param"], cfg["kmax"],
+                         cfg["ewald_locut"])
+    fws = []
+    for tab in tabs:
+        A, B, h2, l2 = tab
+        fws.append((1, dict(xyz=cfg["xyz"], type=cfg["type"], A=A, B=B, hicut2=h2, locut2=l2,
+                            full_ell=cfg["full_ell"])))
+    fws.append((2, dict(xyz=cfg["xyz"], q=cfg["q"], charge1=1.0, kappa=sf["kappa"], k=sf["k"],
+                        s=sf["s"], si=sf["si"], full_ell=cfg["full_ell"])))
+    per_node = []
+    for kind, fw in fws:
+        t = time.perf_counter()
+        O.build_nodes(kind, fw, npn, step, 0, 64)
+        per_node.append((time.perf_counter() - t) / 64)
+    budget = seconds / 3.0
+    total_t, total_n = 0.0, 0
+    sample = []
+    for (kind, fw), pn in zip(fws, per_node):
+        n = int(max(64, min(200000, budget / max(pn, 1e-9))))
+        t = time.perf_counter()
+        O.build_nodes(kind, fw, npn, step, 0, n)
+        dt = time.perf_counter() - t
+        total_t += dt
+        total_n += n
+        sample.append(n)
+    # one map point = one node of one map with its 8 values
+    return dict(value=total_n / total_t, unit="map pts/s", cores=1, kind="port",
+                sample="first %s nodes of the C/O pmaps and the emap (FP64 restatement, "
+                       "one thread, -O3 -march=native, %.1f s)" % (sample, total_t))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the map build of one framework runs on one "
+              "device (multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+    import torch  # noqa: F401  (device memory/stream plumbing only)
+    from musichip import hip as H
+    from musichip import synth
+
+    cfg = synth.config2()
+    ctx = H.Context(0)
+    nk, tabs = setup(ctx, cfg)
+    npn = (cfg["fund_ell"] / cfg["gridspace"]).astype(int) + 1
+    nodes = int(np.prod(npn))
+    log("config2: %d atoms, nk=%d, grid %s = %d nodes" % (len(cfg["xyz"]), nk, list(npn), nodes))
+    for w in range(args.warmup):
+        for m in one_step(ctx, H, cfg, tabs, H.MH_PREC_FP32):
+            m.close()
+        log("warmup %d done" % w)
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    counts = None
+    for s in range(args.steps):
+        maps = one_step(ctx, H, cfg, tabs, H.MH_PREC_FP32)
+        counts = [list(map(int, m.counts)) for m in maps]
+        for m in maps:
+            m.close()
+        log("step %d done" % s)
+    t1 = time.perf_counter()
+    ktimes = ctx.timings()
+    ctx.set_timing(False)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = 3.0 * nodes / ((t1 - t0) / args.steps)
+    log("FP32-acc: %.1f ms/step, %.3e map pts/s; kernel ms: %s" % (ms_per_step, value, ktimes))
+
+    # dominant kernel roofline: Ewald real space in FP32 (documented cutoff)
+    from musichip.hip import real_space_cutoff_default
+    rc = real_space_cutoff_default(cfg["kappa_param"] / np.min(cfg["full_ell"]))
+    vol = float(np.prod(cfg["full_ell"]))
+    pairs_per_node = len(cfg["xyz"]) * (4.0 / 3.0) * np.pi * rc ** 3 / vol
+    flops_launch = 85.0 * pairs_per_node * nodes
+    kname = "nodes32_ewald_real"
+    k_ms = ktimes[kname][0] / ktimes[kname][1]
+    achieved = flops_launch / (k_ms * 1e-3) / 1e12
+    roofline = dict(bound="valu", achieved=round(achieved, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=round(achieved / FP32_PEAK_TFLOPS, 4), traffic=None, kernel=kname,
+                    kernel_ms=round(k_ms, 3), algorithmic="85 FLOP x %.1f pairs/node (r < %.2f A) x %d nodes"
+                    % (pairs_per_node, rc, nodes))
+
+    # FP64 mode: timing + error of the FP32 build against it
+    fp64 = None
+    errs = {}
+    if args.fp64_steps > 0:
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        for s in range(args.fp64_steps):
+            m64 = one_step(ctx, H, cfg, tabs, H.MH_PREC_FP64, keep=False)
+            for m in m64:
+                m.close()
+        t1 = time.perf_counter()
+        fp64 = 3.0 * nodes / ((t1 - t0) / args.fp64_steps)
+        k64 = ctx.timings()
+        ctx.set_timing(False)
+        log("FP64: %.3e map pts/s; kernel ms: %s" % (fp64, k64))
+        # error of FP32 vs FP64 on each map (tabulated region, |V| floor 1 kcal/mol)
+        names = ["pmap_C", "pmap_O", "emap"]
+        for i, (tab, nm) in enumerate(zip(tabs + [None], names)):
+            if tab is not None:
+                ctx.set_lj(*tab)
+                a = ctx.build_map(H.MH_MAP_LJ, H.MH_PREC_FP32, cfg["gridspace"], cfg["hipotcut"], keep_full=True)
+                b = ctx.build_map(H.MH_MAP_LJ, H.MH_PREC_FP64, cfg["gridspace"], cfg["hipotcut"], keep_full=True)
+            else:
+                a = ctx.build_map(H.MH_MAP_EWALD, H.MH_PREC_FP32, cfg["gridspace"], cfg["hipotcut"], keep_full=True)
+                b = ctx.build_map(H.MH_MAP_EWALD, H.MH_PREC_FP64, cfg["gridspace"], cfg["hipotcut"], keep_full=True)
+            _, va = a.get_full(True)
+            _, vb = b.get_full(True)
+            live = vb[:, 0] < cfg["hipotcut"] / 4.184
+            d = np.abs(va[live, 0] - vb[live, 0])
+            errs[nm] = dict(max_rel_err_V=float(np.max(d / np.maximum(np.abs(vb[live, 0]), 1.0))),
+                            max_abs_err_V_kcal=float(d.max()),
+                            same_compaction=bool(list(a.counts) == list(b.counts)))
+            a.close()
+            b.close()
+            del va, vb
+            log("FP32 vs FP64 %s: %s" % (nm, errs[nm]))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, tabs, args.cpu_seconds)
+        log("CPU baseline: %s" % cpu)
+    out = dict(metric=METRIC, value=value, unit="map pts/s", n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=ms_per_step, higher_is_better=True, scaling="weak",
+               vs_baseline=None, dtype="f32 (f64 Kahan accumulation)", data="synthetic",
+               config=dict(workload="charged-framework map build: 50 A cubic cell, 4096 atoms "
+                                    "(seed 2002), pmap C + pmap O + Ewald emap at 0.15 A (334^3 nodes "
+                                    "each), alpha 0.25/A, kmax 9",
+                           nodes_per_map=nodes, maps_per_step=3, nk=nk, compaction_counts=counts),
+               fp64_map_pts_per_s=fp64, fp32_vs_fp64=errs, kernel_ms_per_step={
+                   k: v[0] / args.steps for k, v in ktimes.items()},
+               gcmc_trial_moves_per_s=None, p50_latency_ms=None, p99_latency_ms=None,
+               roofline=roofline, cpu_baseline=cpu)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -48,6 +48,11 @@ const char *mh_version(void);
  * interact_init / forcefield_init (interact.F90:121-287). */
 int mh_ctx_create(int device, mh_ctx **out);
 int mh_ctx_destroy(mh_ctx *ctx);
+/* Per-kernel timing with HIP events on the context's stream (no
+ * counterpart in the reference, whose timers are wtime.F90 CPU clocks).
+ * mh_ctx_get_timing writes a JSON object {"kernel": [ms_total, launches]}. */
+int mh_ctx_set_timing(mh_ctx *ctx, int on);
+int mh_ctx_get_timing(mh_ctx *ctx, char *buf, int len);
 
 /* The fixed framework species (config_init "Fixed", config.F90:179-) after
  * replication to the simulation cell (simcell.F90:200-216).

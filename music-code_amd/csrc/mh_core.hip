@@ -1,7 +1,7 @@
 // libmusic_hip: context, framework/forcefield upload, Ewald structure factor.
 #include <cmath>
 #include <cstring>
-#include <mutex>
+#include <cstdio>
 
 #include "mh_internal.h"
 
@@ -260,6 +260,30 @@ int mh_ewald_set(mh_ctx *ctx, double kappa_param, int kmax, double locut, int *n
     MH_HIP(hipMemcpy(ew.ccoef.p, grid.data(), sizeof(double) * 2 * ncand, hipMemcpyHostToDevice));
     ew.set = true;
     if (nk_out) *nk_out = ew.nk;
+    return MH_OK;
+}
+
+int mh_ctx_set_timing(mh_ctx *ctx, int on) {
+    MH_CHECK(ctx, MH_EINVAL, "mh_ctx_set_timing: ctx is NULL");
+    ctx->timing = on != 0;
+    ctx->ktimes.clear();
+    return MH_OK;
+}
+
+int mh_ctx_get_timing(mh_ctx *ctx, char *buf, int len) {
+    MH_CHECK(ctx && buf && len > 0, MH_EINVAL, "mh_ctx_get_timing: bad arguments");
+    std::string js = "{";
+    bool first = true;
+    for (auto &kv : ctx->ktimes) {
+        char tmp[256];
+        snprintf(tmp, sizeof tmp, "%s\"%s\": [%.6f, %lld]", first ? "" : ", ", kv.first.c_str(),
+                 kv.second.first, kv.second.second);
+        js += tmp;
+        first = false;
+    }
+    js += "}";
+    MH_CHECK((int)js.size() < len, MH_ECAPACITY, "mh_ctx_get_timing: buffer too small");
+    memcpy(buf, js.c_str(), js.size() + 1);
     return MH_OK;
 }
 

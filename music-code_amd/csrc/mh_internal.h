@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -81,7 +82,40 @@ struct mh_ctx {
     mh::BuckTable buck;
     bool buck_set = false;
     mh::Ewald ew;
+    // optional per-kernel timing with HIP events on the context stream
+    bool timing = false;
+    std::map<std::string, std::pair<double, long long>> ktimes;  // name -> (ms, launches)
 };
+
+namespace mh {
+// Records events around a region on ctx->stream when timing is enabled.
+struct KTimer {
+    mh_ctx *ctx;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    KTimer(mh_ctx *c, const char *n) : ctx(c), name(n) {
+        if (ctx->timing) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, ctx->stream);
+        }
+    }
+    void stop() {
+        if (!ctx->timing || !a) return;
+        (void)hipEventRecord(b, ctx->stream);
+        (void)hipEventSynchronize(b);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        auto &t = ctx->ktimes[name];
+        t.first += ms;
+        t.second += 1;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        a = b = nullptr;
+    }
+    ~KTimer() { stop(); }
+};
+}  // namespace mh
 
 struct mh_map {
     mh_ctx *ctx = nullptr;

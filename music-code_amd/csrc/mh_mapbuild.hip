@@ -810,6 +810,7 @@ static int build_recip(mh_ctx *ctx, const int np[3], const double step[3], doubl
         MH_HIP(hipMemcpyAsync(dE[a].p, E[a].data(), sizeof(double) * E[a].size(), hipMemcpyHostToDevice, ctx->stream));
         MH_HIP(hipMemcpyAsync(dk[a].p, kv[a].data(), sizeof(double) * side, hipMemcpyHostToDevice, ctx->stream));
     }
+    KTimer trec(ctx, "ewald_recip");
     long long nzs = (long long)side * side * np[2];
     if ((rc = Z.alloc(sizeof(double2) * 4 * nzs))) return rc;
     hipLaunchKernelGGL(k_recip_z, dim3((unsigned)((nzs + 255) / 256)), dim3(256), 0, ctx->stream,
@@ -937,6 +938,10 @@ extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q
     const bool fp32 = (precision == MH_PREC_FP32) && kind != MH_MAP_BUCK;
     const PairTab *dt = dtab.as<PairTab>();
     double *o64 = want64 ? full64.as<double>() : nullptr;
+    static const char *knames[2][4] = {
+        {"", "nodes64_lj", "nodes64_ewald_real", "nodes64_buck"},
+        {"", "nodes32_lj", "nodes32_ewald_real", "nodes64_buck"}};
+    KTimer tnode(ctx, knames[fp32 ? 1 : 0][kind]);
     if (kind == MH_MAP_LJ) {
         if (fp32)
             hipLaunchKernelGGL(k_nodes32<K_LJ>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
@@ -964,7 +969,9 @@ extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q
         set_error("mh_map_build: node kernel launch failed");
         return fail(MH_EHIP);
     }
+    tnode.stop();
     recip.release();
+    KTimer tcomp(ctx, "compaction");
     // compaction
     DBuf below, owner, cnt, base, tallies;
     if ((rc = below.alloc(nodes))) return fail(rc);
@@ -1008,6 +1015,7 @@ extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q
         set_error("mh_map_build: synchronisation failed");
         return fail(MH_EHIP);
     }
+    tcomp.stop();
     map->counts[0] = (int64_t)t[0];
     map->counts[1] = (int64_t)t[1];
     map->counts[2] = total;

@@ -22,6 +22,12 @@ _i64 = ctypes.c_int64
 _LIB = None
 
 
+def real_space_cutoff_default(kappa):
+    """FP32 Ewald real-space cutoff used by the library: erfc(kappa r) <= 1e-9
+    at kappa r = 4.32 (mirrors real_space_cutoff in csrc/mh_mapbuild.hip)."""
+    return 4.32 / kappa
+
+
 class MusicHipError(RuntimeError):
     pass
 
@@ -42,6 +48,8 @@ def lib():
     sig = {
         "mh_ctx_create": [_i, ctypes.POINTER(_vp)],
         "mh_ctx_destroy": [_vp],
+        "mh_ctx_set_timing": [_vp, _i],
+        "mh_ctx_get_timing": [_vp, ctypes.c_char_p, _i],
         "mh_framework_upload": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
         "mh_ff_set_lj": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
         "mh_ff_set_buck": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
@@ -134,6 +142,15 @@ class Context:
         h = _vp()
         check(lib().mh_ctx_create(device, ctypes.byref(h)), "mh_ctx_create")
         self.h = h
+
+    def set_timing(self, on=True):
+        check(lib().mh_ctx_set_timing(self.h, 1 if on else 0), "mh_ctx_set_timing")
+
+    def timings(self):
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        check(lib().mh_ctx_get_timing(self.h, buf, len(buf)), "mh_ctx_get_timing")
+        return json.loads(buf.value.decode())
 
     def upload_framework(self, xyz, types, q, fund_ell, full_ell):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)
