@@ -44,6 +44,7 @@ def one_step(ctx, H, cfg, tabs, prec, keep=False):
     for tab, p in zip(tabs, cfg["probes"]):
         ctx.set_lj(*tab)
         maps.append(ctx.build_map(H.MH_MAP_LJ, prec, cfg["gridspace"], cfg["hipotcut"], keep_full=keep))
+        log("  map %s built" % p["name"])
     maps.append(ctx.build_map(H.MH_MAP_EWALD, prec, cfg["gridspace"], cfg["hipotcut"], probe_q=1.0,
                               keep_full=keep))
     return maps
@@ -118,11 +119,14 @@ def main():
     ctx.set_timing(True)
     t0 = time.perf_counter()
     counts = None
+    last32 = None
     for s in range(args.steps):
         maps = one_step(ctx, H, cfg, tabs, H.MH_PREC_FP32)
         counts = [list(map(int, m.counts)) for m in maps]
-        for m in maps:
-            m.close()
+        if last32:
+            for m in last32:
+                m.close()
+        last32 = maps
         log("step %d done" % s)
     t1 = time.perf_counter()
     ktimes = ctx.timings()
@@ -142,45 +146,46 @@ def main():
     achieved = flops_launch / (k_ms * 1e-3) / 1e12
     roofline = dict(bound="valu", achieved=round(achieved, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=round(achieved / FP32_PEAK_TFLOPS, 4), traffic=None, kernel=kname,
-                    kernel_ms=round(k_ms, 3), algorithmic="85 FLOP x %.1f pairs/node (r < %.2f A) x %d nodes"
-                    % (pairs_per_node, rc, nodes))
+                    kernel_ms=round(k_ms, 3),
+                    algorithmic="85 FLOP x %.1f pairs/node (r < %.2f A) x %d nodes" % (pairs_per_node, rc, nodes))
 
-    # FP64 mode: timing + error of the FP32 build against it
+    # FP64 mode: timing, then the stored FP32 values of the FP32-acc maps
+    # against those of the FP64 maps (same compaction required)
     fp64 = None
     errs = {}
     if args.fp64_steps > 0:
         ctx.set_timing(True)
         t0 = time.perf_counter()
+        m64 = None
         for s in range(args.fp64_steps):
-            m64 = one_step(ctx, H, cfg, tabs, H.MH_PREC_FP64, keep=False)
-            for m in m64:
-                m.close()
+            if m64:
+                for m in m64:
+                    m.close()
+            m64 = one_step(ctx, H, cfg, tabs, H.MH_PREC_FP64)
         t1 = time.perf_counter()
         fp64 = 3.0 * nodes / ((t1 - t0) / args.fp64_steps)
         k64 = ctx.timings()
         ctx.set_timing(False)
         log("FP64: %.3e map pts/s; kernel ms: %s" % (fp64, k64))
-        # error of FP32 vs FP64 on each map (tabulated region, |V| floor 1 kcal/mol)
-        names = ["pmap_C", "pmap_O", "emap"]
-        for i, (tab, nm) in enumerate(zip(tabs + [None], names)):
-            if tab is not None:
-                ctx.set_lj(*tab)
-                a = ctx.build_map(H.MH_MAP_LJ, H.MH_PREC_FP32, cfg["gridspace"], cfg["hipotcut"], keep_full=True)
-                b = ctx.build_map(H.MH_MAP_LJ, H.MH_PREC_FP64, cfg["gridspace"], cfg["hipotcut"], keep_full=True)
-            else:
-                a = ctx.build_map(H.MH_MAP_EWALD, H.MH_PREC_FP32, cfg["gridspace"], cfg["hipotcut"], keep_full=True)
-                b = ctx.build_map(H.MH_MAP_EWALD, H.MH_PREC_FP64, cfg["gridspace"], cfg["hipotcut"], keep_full=True)
-            _, va = a.get_full(True)
-            _, vb = b.get_full(True)
-            live = vb[:, 0] < cfg["hipotcut"] / 4.184
-            d = np.abs(va[live, 0] - vb[live, 0])
-            errs[nm] = dict(max_rel_err_V=float(np.max(d / np.maximum(np.abs(vb[live, 0]), 1.0))),
-                            max_abs_err_V_kcal=float(d.max()),
-                            same_compaction=bool(list(a.counts) == list(b.counts)))
-            a.close()
-            b.close()
-            del va, vb
-            log("FP32 vs FP64 %s: %s" % (nm, errs[nm]))
+        for nm, a, b in zip(["pmap_C", "pmap_O", "emap"], last32, m64):
+            pa, va = a.get()
+            pb, vb = b.get()
+            both = (pa > 0) & (pb > 0)
+            e = dict(same_compaction=bool(np.array_equal(pa, pb)),
+                     nodes_tabulated_differently=int(np.count_nonzero((pa > 0) != (pb > 0))))
+            ia, ib = pa[both] - 1, pb[both] - 1
+            A = va[:, ia].astype(np.float64)
+            B = vb[:, ib].astype(np.float64)
+            d = np.abs(A - B)
+            e["max_rel_err_V"] = float(np.max(d[0] / np.maximum(np.abs(B[0]), 1.0)))
+            e["max_abs_err_V_kcal"] = float(d[0].max())
+            # derivatives: relative error with the same floor of 1 (kcal/mol/A^n)
+            e["max_rel_err_derivs"] = float(np.max(d[1:] / np.maximum(np.abs(B[1:]), 1.0)))
+            errs[nm] = e
+            log("FP32 vs FP64 %s: %s" % (nm, e))
+            del pa, va, pb, vb, A, B, d
+        for m in m64 + last32:
+            m.close()
 
     cpu = None
     if not args.no_cpu:

@@ -29,6 +29,22 @@ void DBuf::release() {
     n = 0;
 }
 
+void *ws_get(mh_ctx *ctx, const std::string &name, size_t bytes, int *rc) {
+    auto &slot = ctx->ws[name];
+    if (!slot) slot.reset(new DBuf());
+    if (slot->n < bytes) {
+        (void)hipStreamSynchronize(ctx->stream);  // no kernel may still use the old buffer
+        size_t grow = bytes + bytes / 8;
+        int r = slot->alloc(grow);
+        if (r) {
+            *rc = r;
+            return nullptr;
+        }
+    }
+    *rc = MH_OK;
+    return slot->p;
+}
+
 // Reference constants (defaults.F90:33-35): truncated pi.
 constexpr double kPi = 3.1415927;
 constexpr double kTwoPi = 2.0 * kPi;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -82,12 +83,17 @@ struct mh_ctx {
     mh::BuckTable buck;
     bool buck_set = false;
     mh::Ewald ew;
+    // scratch buffers reused across calls (grown on demand, freed with ctx)
+    std::map<std::string, std::unique_ptr<mh::DBuf>> ws;
     // optional per-kernel timing with HIP events on the context stream
     bool timing = false;
     std::map<std::string, std::pair<double, long long>> ktimes;  // name -> (ms, launches)
 };
 
 namespace mh {
+// Scratch buffer 'name' of at least 'bytes' bytes owned by the context.
+void *ws_get(mh_ctx *ctx, const std::string &name, size_t bytes, int *rc);
+
 // Records events around a region on ctx->stream when timing is enabled.
 struct KTimer {
     mh_ctx *ctx;

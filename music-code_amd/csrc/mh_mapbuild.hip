@@ -43,6 +43,7 @@ struct GridP {
     double r2safe;   // FP32 path: pairs closer than this are evaluated in FP64
     double kappa, charge1;
     float rc2f;      // FP32 Ewald real-space cutoff^2
+    int refold;      // FP32 path: fold each separation again (cutoff near L/2)
 };
 
 __device__ __forceinline__ double fold1(double d, double L) {
@@ -163,9 +164,10 @@ __device__ __forceinline__ void ewald_real_pair64(double kappa, double charge1, 
     h[6] = __dadd_rn(h[6], __dmul_rn(__dmul_rn(__dmul_rn(__dmul_rn(cq, sx), sy), sz), temp));
 }
 
-// FP32 Ewald real-space pair: same expression tree, single precision
-__device__ __forceinline__ void ewald_real_pair32(float kappa, float qq, float sx, float sy,
-                                                  float sz, float r2, float &pot, float *h) {
+// FP32 Ewald real-space pair: same expression tree in single precision,
+// returns the 8 terms (t[0] potential, t[1..7] derivatives)
+__device__ __forceinline__ void ewald_real_term32(float kappa, float qq, float sx, float sy,
+                                                  float sz, float r2, float *t) {
 #pragma clang fp contract(fast)
     const float risqrtpi = 0.5641896f;
     const float a1 = 0.254829592f, a2 = -0.284496736f, a3 = 1.421413741f, a4 = -1.453152027f,
@@ -173,25 +175,25 @@ __device__ __forceinline__ void ewald_real_pair32(float kappa, float qq, float s
     float ri1 = __frsqrt_rn(r2);
     float dist = r2 * ri1;
     float kr = kappa * dist;
-    float t = __frcp_rn(1.0f + p * kr);
+    float tt = __frcp_rn(1.0f + p * kr);
     float ek = __expf(-kr * kr);
-    float erfkr = t * (a1 + t * (a2 + t * (a3 + t * (a4 + t * a5)))) * ek;
+    float erfkr = tt * (a1 + tt * (a2 + tt * (a3 + tt * (a4 + tt * a5)))) * ek;
     float ri2 = ri1 * ri1, ri3 = ri2 * ri1, ri4 = ri3 * ri1, ri5 = ri4 * ri1;
     float ri6 = ri5 * ri1, ri7 = ri6 * ri1;
     float k3 = kappa * kappa * kappa;
     float ekr = risqrtpi * ek;
-    pot += qq * erfkr * ri1;
+    t[0] = qq * erfkr * ri1;
     float t1 = qq * (-ri3 * erfkr - 2.0f * kappa * ri2 * ekr);
-    h[0] += sx * t1;
-    h[1] += sy * t1;
-    h[2] += sz * t1;
+    t[1] = sx * t1;
+    t[2] = sy * t1;
+    t[3] = sz * t1;
     float t2 = qq * (3.0f * ri5 * erfkr + (6.0f * kappa * ri4 + 4.0f * k3 * ri2) * ekr);
-    h[3] += t2 * sx * sy;
-    h[4] += t2 * sx * sz;
-    h[5] += t2 * sy * sz;
+    t[4] = t2 * sx * sy;
+    t[5] = t2 * sx * sz;
+    t[6] = t2 * sy * sz;
     float t3 = qq * (-15.0f * ri7 * erfkr -
                      (30.0f * kappa * ri6 + 20.0f * k3 * ri4 + 8.0f * k3 * kappa * kappa * ri2) * ekr);
-    h[6] += t3 * sx * sy * sz;
+    t[7] = t3 * sx * sy * sz;
 }
 
 // FP32 LJ pair with derivatives
@@ -346,11 +348,22 @@ __global__ void __launch_bounds__(NT) k_nodes64(GridP g, const PairTab *__restri
     }
 }
 
-// FP32 node kernel (LJ or Ewald real space): separations relative to the
-// brick centre in FP32, FP32 pair terms summed in FP32 over tiles of 32
-// atoms, FP64 Kahan sums across tiles.  Pairs closer than sqrt(r2safe) are
-// evaluated with the FP64 routine (LJ locut / Ewald low-cut decisions and
-// the steep near-atom terms stay in FP64).
+// FP32 node kernel (LJ or Ewald real space).  Separations are formed
+// relative to the brick centre c: sep = (p - c) + fold(c - a) with fold(c - a)
+// held as a hi+lo float pair, so the FP32 separation keeps ~1e-8 A absolute
+// accuracy even next to an atom.  When a cutoff plus the brick half-diagonal
+// reaches half the cell, each component is folded again (same single fold as
+// the reference).  LJ: FP32 pair terms summed in FP32 over tiles of 32 atoms,
+// FP64 Kahan sums across tiles.  Ewald: every FP32 term is added to FP64
+// sums.  Pairs closer than sqrt(r2safe) and LJ pairs within rounding of
+// hicut are evaluated with the FP64 routine (reference cutoff decisions).
+__device__ __forceinline__ float foldf(float d, float L) {
+    float h = 0.5f * L;
+    d = (d > h) ? d - L : d;
+    d = (d < -h) ? d + L : d;
+    return d;
+}
+
 template <int KIND>
 __global__ void __launch_bounds__(NT) k_nodes32(GridP g, const PairTab *__restrict__ tab_g,
                                                  const double *__restrict__ xyz,
@@ -359,14 +372,18 @@ __global__ void __launch_bounds__(NT) k_nodes32(GridP g, const PairTab *__restri
                                                  const double *__restrict__ recip,
                                                  float *__restrict__ out32,
                                                  double *__restrict__ out64) {
-    __shared__ float s_x[SEG], s_y[SEG], s_z[SEG], s_p0[SEG], s_p1[SEG];
-    __shared__ int s_a[SEG];
+    __shared__ float s_x[SEG], s_y[SEG], s_z[SEG], s_xl[SEG], s_yl[SEG], s_zl[SEG];
+    __shared__ float s_p0[SEG], s_p1[SEG];
+    __shared__ int s_a[SEG], s_t[SEG];
     __shared__ int s_wtot[NT / 64];
+    __shared__ float s_h2f[kMaxTypes];
     __shared__ PairTab tab;
     const int tid = threadIdx.x;
     if (KIND == K_LJ) {
         for (int i = tid; i < (int)(sizeof(PairTab) / sizeof(double)); i += NT)
             reinterpret_cast<double *>(&tab)[i] = reinterpret_cast<const double *>(tab_g)[i];
+        __syncthreads();
+        if (tid < kMaxTypes) s_h2f[tid] = (float)tab.h2[tid];
     }
     const int nbx = (g.nx + BX - 1) / BX, nby = (g.ny + BY - 1) / BY;
     const int b = blockIdx.x;
@@ -382,7 +399,8 @@ __global__ void __launch_bounds__(NT) k_nodes32(GridP g, const PairTab *__restri
     const float dx0 = (float)(px - cen[0]), dy0 = (float)(py - cen[1]), dz0 = (float)(pz - cen[2]);
     const float r2safe = (float)g.r2safe;
     const float kappaf = (float)g.kappa;
-    const float c1f = (float)g.charge1;
+    const float Lx = (float)g.full[0], Ly = (float)g.full[1], Lz = (float)g.full[2];
+    const bool refold = g.refold != 0;
     __syncthreads();
     double S[8] = {0, 0, 0, 0, 0, 0, 0, 0}, Cc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double pot64 = 0.0, h64[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -414,15 +432,24 @@ __global__ void __launch_bounds__(NT) k_nodes32(GridP g, const PairTab *__restri
             }
             if (keep) {
                 int pos = off + __popcll(m & ((1ull << lane) - 1ull));
-                s_x[pos] = (float)fold1(cen[0] - ax, g.full[0]);
-                s_y[pos] = (float)fold1(cen[1] - ay, g.full[1]);
-                s_z[pos] = (float)fold1(cen[2] - az, g.full[2]);
+                double fx = fold1(cen[0] - ax, g.full[0]);
+                double fy = fold1(cen[1] - ay, g.full[1]);
+                double fz = fold1(cen[2] - az, g.full[2]);
+                float hx = (float)fx, hy = (float)fy, hz = (float)fz;
+                s_x[pos] = hx;
+                s_y[pos] = hy;
+                s_z[pos] = hz;
+                s_xl[pos] = (float)(fx - (double)hx);
+                s_yl[pos] = (float)(fy - (double)hy);
+                s_zl[pos] = (float)(fz - (double)hz);
                 s_a[pos] = a;
                 if (KIND == K_LJ) {
                     int t = typ[a];
+                    s_t[pos] = t;
                     s_p0[pos] = (float)tab.A[t];
-                    s_p1[pos] = __int_as_float(t);
+                    s_p1[pos] = (float)tab.B[t];
                 } else {
+                    s_t[pos] = 0;
                     s_p0[pos] = (float)(g.charge1 * qv[a]);
                     s_p1[pos] = 0.0f;
                 }
@@ -435,11 +462,18 @@ __global__ void __launch_bounds__(NT) k_nodes32(GridP g, const PairTab *__restri
                 float pf = 0.0f, hf[7] = {0, 0, 0, 0, 0, 0, 0};
                 const int j1 = min(cnt, j0 + 32);
                 for (int j = j0; j < j1; j++) {
-                    float sx = dx0 + s_x[j], sy = dy0 + s_y[j], sz = dz0 + s_z[j];
+                    float sx = (dx0 + s_xl[j]) + s_x[j];
+                    float sy = (dy0 + s_yl[j]) + s_y[j];
+                    float sz = (dz0 + s_zl[j]) + s_z[j];
+                    if (refold) {
+                        sx = foldf(sx, Lx);
+                        sy = foldf(sy, Ly);
+                        sz = foldf(sz, Lz);
+                    }
                     float r2 = sx * sx + sy * sy + sz * sz;
                     if (KIND == K_LJ) {
-                        int t = __float_as_int(s_p1[j]);
-                        const float h2f = (float)tab.h2[t];
+                        const int t = s_t[j];
+                        const float h2f = s_h2f[t];
                         const float band = h2f * 1.0e-6f;
                         if (r2 > h2f + band) continue;
                         // near-atom pairs and pairs within rounding of hicut:
@@ -453,7 +487,7 @@ __global__ void __launch_bounds__(NT) k_nodes32(GridP g, const PairTab *__restri
                             if (rc == 0) { failed = true; break; }
                             continue;
                         }
-                        lj_pair32(s_p0[j], (float)tab.B[t], sx, sy, sz, r2, pf, hf);
+                        lj_pair32(s_p0[j], s_p1[j], sx, sy, sz, r2, pf, hf);
                     } else {
                         if (r2 > g.rc2f) continue;
                         if (r2 < r2safe) {
@@ -466,16 +500,20 @@ __global__ void __launch_bounds__(NT) k_nodes32(GridP g, const PairTab *__restri
                             ewald_real_pair64(g.kappa, g.charge1, qv[a], ex, ey, ez, d2, pot64, h64);
                             continue;
                         }
-                        ewald_real_pair32(kappaf, s_p0[j], sx, sy, sz, r2, pf, hf);
+                        // potential terms cancel strongly: each FP32 term goes
+                        // straight into the FP64 sum; derivatives use the tile
+                        float tt[8];
+                        ewald_real_term32(kappaf, s_p0[j], sx, sy, sz, r2, tt);
+                        S[0] += (double)tt[0];
+                        for (int i = 0; i < 7; i++) hf[i] += tt[i + 1];
                     }
                 }
-                kahan_add(S[0], Cc[0], (double)pf);
+                if (KIND == K_LJ) kahan_add(S[0], Cc[0], (double)pf);
                 for (int i = 0; i < 7; i++) kahan_add(S[i + 1], Cc[i + 1], (double)hf[i]);
             }
         }
         __syncthreads();
     }
-    (void)c1f;
     if (!valid) return;
     const long long m = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
     double o[8];
@@ -621,7 +659,7 @@ __global__ void k_below(long long n, int nx, int ny, int nz, const float *__rest
 }
 
 __global__ void k_owner(long long n, int nx, int ny, int nz, const uint8_t *__restrict__ below,
-                        int64_t *__restrict__ owner) {
+                        int32_t *__restrict__ owner) {
     long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     int k = (int)(s % nz), j = (int)((s / nz) % ny), i = (int)(s / ((long long)nz * ny));
@@ -635,14 +673,17 @@ __global__ void k_owner(long long n, int nx, int ny, int nz, const uint8_t *__re
                     if (below[t] && (best < 0 || t < best)) best = t;
                 }
     }
-    owner[s] = best;
+    owner[s] = (int32_t)best;
 }
 
-__global__ void k_count(long long n, int nx, int ny, int nz, const uint8_t *__restrict__ below,
-                        const int64_t *__restrict__ owner, int64_t *__restrict__ cnt,
-                        unsigned long long *__restrict__ tallies) {
+__global__ void __launch_bounds__(256) k_count(long long n, int nx, int ny, int nz,
+                                               const uint8_t *__restrict__ below,
+                                               const int32_t *__restrict__ owner,
+                                               int32_t *__restrict__ cnt,
+                                               unsigned long long *__restrict__ partial) {
+    __shared__ unsigned long long sred[3][4];
     long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long c = 0;
+    int c = 0;
     unsigned long long tb = 0, tn = 0, tr = 0;
     if (s < n) {
         int k = (int)(s % nz), j = (int)((s / nz) % ny), i = (int)(s / ((long long)nz * ny));
@@ -663,31 +704,54 @@ __global__ void k_count(long long n, int nx, int ny, int nz, const uint8_t *__re
         }
         cnt[s] = c;
     }
-    // wave reductions then one atomic per wave
+    // block reduction, one partial per block (no global atomics)
     for (int off = 32; off > 0; off >>= 1) {
         tb += __shfl_down(tb, off);
         tn += __shfl_down(tn, off);
         tr += __shfl_down(tr, off);
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&tallies[0], tb);
-        atomicAdd(&tallies[1], tn);
-        atomicAdd(&tallies[2], tr);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        sred[0][wave] = tb;
+        sred[1][wave] = tn;
+        sred[2][wave] = tr;
     }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        unsigned long long v = 0;
+        for (int w = 0; w < 4; w++) v += sred[threadIdx.x][w];
+        partial[3 * (long long)blockIdx.x + threadIdx.x] = v;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_tally(long long nblk, const unsigned long long *__restrict__ partial,
+                                                unsigned long long *__restrict__ tallies) {
+    __shared__ unsigned long long sh[3][1024];
+    unsigned long long v[3] = {0, 0, 0};
+    for (long long b = threadIdx.x; b < nblk; b += 1024)
+        for (int q = 0; q < 3; q++) v[q] += partial[3 * b + q];
+    for (int q = 0; q < 3; q++) sh[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (threadIdx.x < (unsigned)off)
+            for (int q = 0; q < 3; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) tallies[threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // exclusive scan, three phases: block sums, scan of block sums, add back
 constexpr int SCAN_B = 1024;
-__global__ void __launch_bounds__(SCAN_B) k_scan_blocks(long long n, const int64_t *__restrict__ in,
-                                                         int64_t *__restrict__ out,
-                                                         int64_t *__restrict__ bsum) {
-    __shared__ int64_t sh[SCAN_B];
+__global__ void __launch_bounds__(SCAN_B) k_scan_blocks(long long n, const int32_t *__restrict__ in,
+                                                         int32_t *__restrict__ out,
+                                                         int32_t *__restrict__ bsum) {
+    __shared__ int32_t sh[SCAN_B];
     long long i = (long long)blockIdx.x * SCAN_B + threadIdx.x;
-    int64_t v = (i < n) ? in[i] : 0;
+    int32_t v = (i < n) ? in[i] : 0;
     sh[threadIdx.x] = v;
     __syncthreads();
     for (int off = 1; off < SCAN_B; off <<= 1) {
-        int64_t t = (threadIdx.x >= (unsigned)off) ? sh[threadIdx.x - off] : 0;
+        int32_t t = (threadIdx.x >= (unsigned)off) ? sh[threadIdx.x - off] : 0;
         __syncthreads();
         sh[threadIdx.x] += t;
         __syncthreads();
@@ -696,13 +760,13 @@ __global__ void __launch_bounds__(SCAN_B) k_scan_blocks(long long n, const int64
     if (threadIdx.x == SCAN_B - 1) bsum[blockIdx.x] = sh[SCAN_B - 1];
 }
 
-__global__ void k_scan_add(long long n, int64_t *__restrict__ out, const int64_t *__restrict__ boff) {
+__global__ void k_scan_add(long long n, int32_t *__restrict__ out, const int32_t *__restrict__ boff) {
     long long i = (long long)blockIdx.x * SCAN_B + threadIdx.x;
     if (i < n) out[i] += boff[blockIdx.x];
 }
 
 __global__ void k_assign(long long n, int nx, int ny, int nz, const uint8_t *__restrict__ below,
-                         const int64_t *__restrict__ owner, const int64_t *__restrict__ base,
+                         const int32_t *__restrict__ owner, const int32_t *__restrict__ base,
                          const float *__restrict__ full, int32_t *__restrict__ ptr,
                          float *__restrict__ cvals) {
     long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -742,29 +806,32 @@ __global__ void k_assign(long long n, int nx, int ny, int nz, const uint8_t *__r
 using namespace mh;
 
 // ------------------------------------------------------------ host side
-static int scan_exclusive(hipStream_t st, long long n, const int64_t *in, int64_t *out,
+static int scan_exclusive(mh_ctx *ctx, int level, long long n, const int32_t *in, int32_t *out,
                           int64_t *total) {
+    hipStream_t st = ctx->stream;
     long long nb = (n + SCAN_B - 1) / SCAN_B;
-    DBuf bsum, boff;
     int rc;
-    if ((rc = bsum.alloc(sizeof(int64_t) * (nb + 1)))) return rc;
-    if ((rc = boff.alloc(sizeof(int64_t) * (nb + 1)))) return rc;
-    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(SCAN_B), 0, st, n, in, out,
-                       bsum.as<int64_t>());
+    std::string lv = std::to_string(level);
+    int32_t *bsum = (int32_t *)ws_get(ctx, "scan_bsum" + lv, sizeof(int32_t) * (nb + 1), &rc);
+    if (rc) return rc;
+    int32_t *boff = (int32_t *)ws_get(ctx, "scan_boff" + lv, sizeof(int32_t) * (nb + 1), &rc);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(SCAN_B), 0, st, n, in, out, bsum);
     MH_HIP(hipGetLastError());
     if (nb > 1) {
         int64_t sub = 0;
-        if ((rc = scan_exclusive(st, nb, bsum.as<int64_t>(), boff.as<int64_t>(), &sub))) return rc;
-        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(SCAN_B), 0, st, n, out,
-                           boff.as<int64_t>());
+        if ((rc = scan_exclusive(ctx, level + 1, nb, bsum, boff, &sub))) return rc;
+        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(SCAN_B), 0, st, n, out, boff);
         MH_HIP(hipGetLastError());
     }
-    // total = last exclusive value + last input
-    int64_t lastx = 0, lasti = 0;
-    MH_HIP(hipMemcpyAsync(&lastx, out + (n - 1), sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    MH_HIP(hipMemcpyAsync(&lasti, in + (n - 1), sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    MH_HIP(hipStreamSynchronize(st));
-    *total = lastx + lasti;
+    if (level == 0) {
+        // total = last exclusive value + last input
+        int32_t lastx = 0, lasti = 0;
+        MH_HIP(hipMemcpyAsync(&lastx, out + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        MH_HIP(hipMemcpyAsync(&lasti, in + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        MH_HIP(hipStreamSynchronize(st));
+        *total = (int64_t)lastx + lasti;
+    }
     return MH_OK;
 }
 
@@ -778,13 +845,15 @@ static void grid_dims(const double eff[3], double gridspace, int np[3], double s
 
 // separable reciprocal tables and stages; writes recip[nodes][8]
 static int build_recip(mh_ctx *ctx, const int np[3], const double step[3], double charge1,
-                       DBuf &recip) {
+                       double **recip_out) {
     const Ewald &ew = ctx->ew;
     const int K = ew.kmax, side = 2 * K + 1;
     MH_CHECK(side <= MAXSIDE, MH_EINVAL, "Ewald kmax too large for the reciprocal kernel");
     const long long nodes = (long long)np[0] * np[1] * np[2];
     int rc;
-    if ((rc = recip.alloc(sizeof(double) * 8 * nodes))) return rc;
+    double *recip = (double *)ws_get(ctx, "recip", sizeof(double) * 8 * nodes, &rc);
+    if (rc) return rc;
+    *recip_out = recip;
     // per-axis k values (ewald.F90:599-619: k = (m * (1/L)) * twopi) and
     // tables E[m][n] = exp(i k(m) * n * step)
     std::vector<double> kv[3];
@@ -803,26 +872,26 @@ static int build_recip(mh_ctx *ctx, const int np[3], const double step[3], doubl
             }
         }
     }
-    DBuf dE[3], dk[3], Z;
+    double *dE[3], *dk[3];
     for (int a = 0; a < 3; a++) {
-        if ((rc = dE[a].alloc(sizeof(double) * E[a].size()))) return rc;
-        if ((rc = dk[a].alloc(sizeof(double) * side))) return rc;
-        MH_HIP(hipMemcpyAsync(dE[a].p, E[a].data(), sizeof(double) * E[a].size(), hipMemcpyHostToDevice, ctx->stream));
-        MH_HIP(hipMemcpyAsync(dk[a].p, kv[a].data(), sizeof(double) * side, hipMemcpyHostToDevice, ctx->stream));
+        dE[a] = (double *)ws_get(ctx, "recipE" + std::to_string(a), sizeof(double) * E[a].size(), &rc);
+        if (rc) return rc;
+        dk[a] = (double *)ws_get(ctx, "recipk" + std::to_string(a), sizeof(double) * side, &rc);
+        if (rc) return rc;
+        MH_HIP(hipMemcpyAsync(dE[a], E[a].data(), sizeof(double) * E[a].size(), hipMemcpyHostToDevice, ctx->stream));
+        MH_HIP(hipMemcpyAsync(dk[a], kv[a].data(), sizeof(double) * side, hipMemcpyHostToDevice, ctx->stream));
     }
     KTimer trec(ctx, "ewald_recip");
     long long nzs = (long long)side * side * np[2];
-    if ((rc = Z.alloc(sizeof(double2) * 4 * nzs))) return rc;
+    double2 *Z = (double2 *)ws_get(ctx, "recipZ", sizeof(double2) * 4 * nzs, &rc);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_recip_z, dim3((unsigned)((nzs + 255) / 256)), dim3(256), 0, ctx->stream,
-                       side, np[2], ew.ccoef.as<double2>(), dE[2].as<double2>(), dk[2].as<double>(),
-                       Z.as<double2>());
+                       side, np[2], ew.ccoef.as<double2>(), (const double2 *)dE[2], dk[2], Z);
     MH_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_recip_xy, dim3((unsigned)((long long)np[1] * np[2])), dim3(256), 0,
-                       ctx->stream, side, np[0], np[1], np[2], Z.as<double2>(), dE[0].as<double2>(),
-                       dE[1].as<double2>(), dk[0].as<double>(), dk[1].as<double>(), charge1,
-                       recip.as<double>());
+                       ctx->stream, side, np[0], np[1], np[2], Z, (const double2 *)dE[0],
+                       (const double2 *)dE[1], dk[0], dk[1], charge1, recip);
     MH_HIP(hipGetLastError());
-    MH_HIP(hipStreamSynchronize(ctx->stream));
     return MH_OK;
 }
 
@@ -870,10 +939,21 @@ extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q
         delete map;
         return code;
     };
-    DBuf full32, full64, recip, dtab;
-    if ((rc = full32.alloc(sizeof(float) * 8 * nodes))) return fail(rc);
-    bool want64 = keep_full && true;
-    if (want64 && (rc = full64.alloc(sizeof(double) * 8 * nodes))) return fail(rc);
+    // node values: owned by the map when kept, else context scratch
+    DBuf full32k, full64k;
+    float *full32 = nullptr;
+    double *full64 = nullptr;
+    double *recip = nullptr;
+    const bool want64 = keep_full != 0;
+    if (keep_full) {
+        if ((rc = full32k.alloc(sizeof(float) * 8 * nodes))) return fail(rc);
+        if ((rc = full64k.alloc(sizeof(double) * 8 * nodes))) return fail(rc);
+        full32 = full32k.as<float>();
+        full64 = full64k.as<double>();
+    } else {
+        full32 = (float *)ws_get(ctx, "full32", sizeof(float) * 8 * nodes, &rc);
+        if (rc) return fail(rc);
+    }
     // pair tables
     PairTab tab;
     std::memset(&tab, 0, sizeof tab);
@@ -903,8 +983,9 @@ extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q
             maxh2 = fmax(maxh2, bk.hicut2[t]);
         }
     }
-    if ((rc = dtab.alloc(sizeof(PairTab)))) return fail(rc);
-    if (hipMemcpyAsync(dtab.p, &tab, sizeof tab, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    PairTab *dt = (PairTab *)ws_get(ctx, "pairtab", sizeof(PairTab), &rc);
+    if (rc) return fail(rc);
+    if (hipMemcpyAsync(dt, &tab, sizeof tab, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
         return fail(MH_EHIP);
     GridP g;
     g.nx = np[0];
@@ -919,13 +1000,14 @@ extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q
     g.charge1 = probe_q;
     g.r2safe = 0.0;
     g.rc2f = 0.0f;
+    g.refold = 0;
     if (kind == MH_MAP_EWALD) {
-        if ((rc = build_recip(ctx, np, step, probe_q, recip))) return fail(rc);
+        if ((rc = build_recip(ctx, np, step, probe_q, &recip))) return fail(rc);
         if (precision == MH_PREC_FP32) {
             double rc_real = real_space_cutoff(ctx->ew.kappa);
             g.cull2 = rc_real * rc_real * (1.0 + 1e-6);
             g.rc2f = (float)(rc_real * rc_real);
-            g.r2safe = 0.25;  // pairs within 0.5 A evaluated in FP64
+            g.r2safe = 9.0;  // the large near terms (r < 3 A) are evaluated in FP64
         } else {
             g.cull2 = 1e300;  // the reference sums every atom (ewald.F90:874)
         }
@@ -933,11 +1015,19 @@ extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q
         g.cull2 = maxh2 * (1.0 + 1e-9) + 1e-9;
         g.r2safe = fmax(2.25, 4.0 * maxl2);
     }
+    {
+        // brick half-diagonal; fold(c - a) + (p - c) equals the reference's
+        // single fold of p - a whenever cutoff + half-diagonal < L/2
+        double hd = 0.5 * sqrt(pow((BX - 1) * step[0], 2) + pow((BY - 1) * step[1], 2) +
+                               pow((BZ - 1) * step[2], 2));
+        double reach = sqrt(fmin(g.cull2, 1e200)) + hd;
+        double half = 0.5 * fmin(ctx->full_ell[0], fmin(ctx->full_ell[1], ctx->full_ell[2]));
+        g.refold = (reach >= half) ? 1 : 0;
+    }
     const int nbx = (np[0] + BX - 1) / BX, nby = (np[1] + BY - 1) / BY, nbz = (np[2] + BZ - 1) / BZ;
     const long long nblk = (long long)nbx * nby * nbz;
     const bool fp32 = (precision == MH_PREC_FP32) && kind != MH_MAP_BUCK;
-    const PairTab *dt = dtab.as<PairTab>();
-    double *o64 = want64 ? full64.as<double>() : nullptr;
+    double *o64 = want64 ? full64 : nullptr;
     static const char *knames[2][4] = {
         {"", "nodes64_lj", "nodes64_ewald_real", "nodes64_buck"},
         {"", "nodes32_lj", "nodes32_ewald_real", "nodes64_buck"}};
@@ -946,71 +1036,76 @@ extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q
         if (fp32)
             hipLaunchKernelGGL(k_nodes32<K_LJ>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
                                ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
-                               (const double *)nullptr, full32.as<float>(), o64);
+                               (const double *)nullptr, full32, o64);
         else
             hipLaunchKernelGGL(k_nodes64<K_LJ>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
                                ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
-                               (const double *)nullptr, full32.as<float>(), o64);
+                               (const double *)nullptr, full32, o64);
     } else if (kind == MH_MAP_BUCK) {
         hipLaunchKernelGGL(k_nodes64<K_BUCK>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
                            ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
-                           (const double *)nullptr, full32.as<float>(), o64);
+                           (const double *)nullptr, full32, o64);
     } else {
         if (fp32)
             hipLaunchKernelGGL(k_nodes32<K_EWALD>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
                                ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
-                               recip.as<double>(), full32.as<float>(), o64);
+                               recip, full32, o64);
         else
             hipLaunchKernelGGL(k_nodes64<K_EWALD>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
                                ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
-                               recip.as<double>(), full32.as<float>(), o64);
+                               recip, full32, o64);
     }
     if (hipGetLastError() != hipSuccess) {
         set_error("mh_map_build: node kernel launch failed");
         return fail(MH_EHIP);
     }
     tnode.stop();
-    recip.release();
     KTimer tcomp(ctx, "compaction");
     // compaction
-    DBuf below, owner, cnt, base, tallies;
-    if ((rc = below.alloc(nodes))) return fail(rc);
-    if ((rc = owner.alloc(sizeof(int64_t) * nodes))) return fail(rc);
-    if ((rc = cnt.alloc(sizeof(int64_t) * nodes))) return fail(rc);
-    if ((rc = base.alloc(sizeof(int64_t) * nodes))) return fail(rc);
-    if ((rc = tallies.alloc(sizeof(unsigned long long) * 4))) return fail(rc);
+    uint8_t *below = (uint8_t *)ws_get(ctx, "below", nodes, &rc);
+    if (rc) return fail(rc);
+    int32_t *owner = (int32_t *)ws_get(ctx, "owner", sizeof(int32_t) * nodes, &rc);
+    if (rc) return fail(rc);
+    int32_t *cnt = (int32_t *)ws_get(ctx, "cnt", sizeof(int32_t) * nodes, &rc);
+    if (rc) return fail(rc);
+    int32_t *base = (int32_t *)ws_get(ctx, "base", sizeof(int32_t) * nodes, &rc);
+    if (rc) return fail(rc);
+    unsigned long long *tallies = (unsigned long long *)ws_get(ctx, "tallies", 64, &rc);
+    if (rc) return fail(rc);
     if ((rc = map->ptr.alloc(sizeof(int32_t) * nodes))) return fail(rc);
-    if (hipMemsetAsync(tallies.p, 0, sizeof(unsigned long long) * 4, ctx->stream) != hipSuccess ||
+    if (hipMemsetAsync(tallies, 0, sizeof(unsigned long long) * 4, ctx->stream) != hipSuccess ||
         hipMemsetAsync(map->ptr.p, 0, sizeof(int32_t) * nodes, ctx->stream) != hipSuccess)
         return fail(MH_EHIP);
     float cut = (float)(hipotcut_kJ / 4.184);  // mapmaker.F90:562, Real(cutoff,RSgl)
     unsigned gb = (unsigned)((nodes + 255) / 256);
     hipLaunchKernelGGL(k_below, dim3(gb), dim3(256), 0, ctx->stream, nodes, np[0], np[1], np[2],
-                       full32.as<float>(), cut, below.as<uint8_t>());
+                       full32, cut, below);
     hipLaunchKernelGGL(k_owner, dim3(gb), dim3(256), 0, ctx->stream, nodes, np[0], np[1], np[2],
-                       below.as<uint8_t>(), owner.as<int64_t>());
+                       below, owner);
+    unsigned long long *partial = (unsigned long long *)ws_get(ctx, "cpartial", sizeof(unsigned long long) * 3 * gb, &rc);
+    if (rc) return fail(rc);
     hipLaunchKernelGGL(k_count, dim3(gb), dim3(256), 0, ctx->stream, nodes, np[0], np[1], np[2],
-                       below.as<uint8_t>(), owner.as<int64_t>(), cnt.as<int64_t>(),
-                       tallies.as<unsigned long long>());
+                       below, owner, cnt, partial);
+    hipLaunchKernelGGL(k_tally, dim3(1), dim3(1024), 0, ctx->stream, (long long)gb, partial, tallies);
     if (hipGetLastError() != hipSuccess) {
         set_error("mh_map_build: compaction kernel launch failed");
         return fail(MH_EHIP);
     }
     int64_t total = 0;
-    if ((rc = scan_exclusive(ctx->stream, nodes, cnt.as<int64_t>(), base.as<int64_t>(), &total)))
+    if ((rc = scan_exclusive(ctx, 0, nodes, cnt, base, &total)))
         return fail(rc);
     MH_CHECK(total < (1LL << 31), MH_ECAPACITY, "mh_map_build: more than 2^31 tabulated points");
     map->nfsize = total;
     if ((rc = map->vals.alloc(sizeof(float) * 8 * (total + 1)))) return fail(rc);
     hipLaunchKernelGGL(k_assign, dim3(gb), dim3(256), 0, ctx->stream, nodes, np[0], np[1], np[2],
-                       below.as<uint8_t>(), owner.as<int64_t>(), base.as<int64_t>(),
-                       full32.as<float>(), map->ptr.as<int32_t>(), map->vals.as<float>());
+                       below, owner, base,
+                       full32, map->ptr.as<int32_t>(), map->vals.as<float>());
     if (hipGetLastError() != hipSuccess) {
         set_error("mh_map_build: assign kernel launch failed");
         return fail(MH_EHIP);
     }
     unsigned long long t[4];
-    if (hipMemcpyAsync(t, tallies.p, sizeof t, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+    if (hipMemcpyAsync(t, tallies, sizeof t, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
         set_error("mh_map_build: synchronisation failed");
         return fail(MH_EHIP);
@@ -1021,14 +1116,12 @@ extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q
     map->counts[2] = total;
     map->counts[3] = (int64_t)t[2];
     if (keep_full) {
-        map->full32.p = full32.p;
-        map->full32.n = full32.n;
-        full32.p = nullptr;
-        if (want64) {
-            map->full64.p = full64.p;
-            map->full64.n = full64.n;
-            full64.p = nullptr;
-        }
+        map->full32.p = full32k.p;
+        map->full32.n = full32k.n;
+        full32k.p = nullptr;
+        map->full64.p = full64k.p;
+        map->full64.n = full64k.n;
+        full64k.p = nullptr;
     }
     *out = map;
     return MH_OK;
