@@ -1,5 +1,497 @@
 /*
- * music_gcmc_oracle.c -- CPU FP64 restatement of the GCMC move loop
- * (test infrastructure only; see music_oracle.h).
+ * music_gcmc_oracle.c -- CPU FP64 restatement of the GCMC move loop for
+ * rigid sorbates (RINSERT / RDELETE / RTRANSLATE / RROTATE) with energies
+ * from potential maps, sorbate-sorbate LJ with minimum image and (a new
+ * capability the reference lacks) Ewald electrostatics with structure-factor
+ * deltas.  TEST INFRASTRUCTURE ONLY: checker for the HIP GCMC kernel.
+ *
+ * Step structure and RNG draw order follow:
+ *   gcmc_dosim            gcmc.F90:240-250  (species draw every step)
+ *   gcmcmoves_move        gcmcmoves.F90:270-364 (move draw, molecule draw)
+ *   mcmoveset_pickmove    mcmoveset.F90:314-335
+ *   mcmoves_insert        mcmoves.F90:476-566 (no draw when the energy fails)
+ *   mcmoves_delete        mcmoves.F90:578-647
+ *   mcmoves_perturb       mcmoves.F90:352-463
+ *   mcmoves_attempt       mcmoves.F90:290-335 (clamp +-30, boltz > rranf())
+ *   rigidmoves_rinsert    rigidmoves.F90:1313-1367 (3 draws, +3 if multi-site)
+ *   rigidmoves_randomtranslate / randomrotate  rigidmoves.F90:691-864
+ *   rigidmoves_rotperturb rigidmoves.F90:1675-1720
+ *   rigidcoords_toxyz     rigidcoords.F90:254-276, matrix_b2feuler matrix.F90:383-403
+ *   simcell_pbc           simcell.F90:521-547
+ *   config_delmol         config.F90:1401-1413 (last molecule moves into the slot)
+ * Molecule energy = map terms (interpolate_int per site, factor q for the
+ * emap) + LJ pairs with every other molecule (lj_snglint, nderivs = 0) +
+ * Ewald terms.  Failure (LJ locut, untabulated map corner, U > max_nrg)
+ * rejects the move without an RNG draw.
+ *
+ * Ewald (no reference implementation; ssdriver.F90:489-498 stops):
+ *   real space  sum q_a q_b erfc(kappa r)/r, r < rc (A&S erfc, utils.F90)
+ *   reciprocal  E = sum_{k in half space} w_k |S_k|^2,
+ *               w_k = (4 pi / V) exp(-k^2 / 4 kappa^2) / k^2,
+ *               k = twopi (mx/Lx, my/Ly, mz/Lz), |m_i| <= kmax
+ *   self        -kappa/sqrt(pi) sum q_a^2 per molecule
+ *   intra       -sum_{a<b} q_a q_b erf(kappa r_ab)/r_ab per molecule
+ * all times 332.064 (ewald.F90:60).  Pinned by closed forms in
+ * tests/test_gcmc_oracle.py (delta = full recompute difference; total
+ * Ewald of a small neutral system against a direct lattice sum).
  */
-#include "music_oracle.h"
+#include "music_gcmc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define TWOPI_R (2.0 * 3.1415927)
+
+/* matrix_b2feuler (matrix.F90:383-403), rigidcoords_toxyz */
+static void toxyz(const mo_gcmc_sys *s, const double com[3], const double ang[3], double *rp)
+{
+    if (s->ns == 1) {
+        rp[0] = com[0];
+        rp[1] = com[1];
+        rp[2] = com[2];
+        return;
+    }
+    double phi = ang[0], theta = ang[1], psi = ang[2];
+    double sp = sin(phi), cp = cos(phi), st = sin(theta), ct = cos(theta), ss = sin(psi), cs = cos(psi);
+    double m[3][3];
+    m[0][0] = cs * cp - ct * sp * ss;
+    m[0][1] = -ss * cp - ct * sp * cs;
+    m[0][2] = st * sp;
+    m[1][0] = cs * sp + ct * cp * ss;
+    m[1][1] = -ss * sp + ct * cp * cs;
+    m[1][2] = -st * cp;
+    m[2][0] = ss * st;
+    m[2][1] = cs * st;
+    m[2][2] = ct;
+    for (int a = 0; a < s->ns; a++) {
+        const double *v = s->ref[a];
+        for (int i = 0; i < 3; i++) {
+            /* matrix * vector: row dot (loop order j = 1..3) */
+            double d = 0.0;
+            d = d + m[i][0] * v[0];
+            d = d + m[i][1] * v[1];
+            d = d + m[i][2] * v[2];
+            rp[3 * a + i] = com[i] + d;
+        }
+    }
+}
+
+/* simcell_pbcarray (simcell.F90:521-547) */
+static void pbc(const mo_gcmc_sys *s, const double *rp, double *r)
+{
+    for (int a = 0; a < s->ns; a++)
+        for (int i = 0; i < 3; i++) {
+            double c = rp[3 * a + i];
+            double idx = floor(c / s->full_ell[i]);
+            r[3 * a + i] = c - idx * s->full_ell[i];
+        }
+}
+
+static double erfc_as(double x)
+{
+    return mo_erfc(x);
+}
+
+/* ---------------------------------------------------------------- Ewald */
+void mo_gcmc_kvecs(const mo_gcmc_sys *s, int *nk, double *kv, double *w)
+{
+    int K = s->kmax, n = 0;
+    double V = s->full_ell[0] * s->full_ell[1] * s->full_ell[2];
+    for (int mx = 0; mx <= K; mx++)
+        for (int my = -K; my <= K; my++)
+            for (int mz = -K; mz <= K; mz++) {
+                if (mx == 0 && (my < 0 || (my == 0 && mz <= 0))) continue;
+                double kx = ((double)mx / s->full_ell[0]) * TWOPI_R;
+                double ky = ((double)my / s->full_ell[1]) * TWOPI_R;
+                double kz = ((double)mz / s->full_ell[2]) * TWOPI_R;
+                double k2 = kx * kx + ky * ky + kz * kz;
+                if (kv) {
+                    kv[3 * n] = kx;
+                    kv[3 * n + 1] = ky;
+                    kv[3 * n + 2] = kz;
+                }
+                if (w) w[n] = (4.0 * M_PI / V) * exp(-k2 / (4.0 * s->kappa * s->kappa)) / k2 * MO_TOKCAL;
+                n++;
+            }
+    *nk = n;
+}
+
+/* structure factor contribution of one molecule: sre/sim[k] = sum_a q_a e^{ik.r_a} */
+static void mol_sk(const mo_gcmc_sys *s, const mo_gcmc_state *st, const double *r, double *sre, double *sim)
+{
+    for (int k = 0; k < st->nk; k++) {
+        double re = 0.0, im = 0.0;
+        const double *kv = st->kv + 3 * k;
+        for (int a = 0; a < s->ns; a++) {
+            if (s->sq[a] == 0.0) continue;
+            double arg = kv[0] * r[3 * a] + kv[1] * r[3 * a + 1] + kv[2] * r[3 * a + 2];
+            re += s->sq[a] * cos(arg);
+            im += s->sq[a] * sin(arg);
+        }
+        sre[k] = re;
+        sim[k] = im;
+    }
+}
+
+/* E_recip(S + sgn*add) - E_recip(S) (+ optional second change) */
+static double drecip(const mo_gcmc_state *st, const double *are, const double *aim, double asgn,
+                     const double *bre, const double *bim, double bsgn)
+{
+    double d = 0.0;
+    for (int k = 0; k < st->nk; k++) {
+        double re = st->Sre[k], im = st->Sim[k];
+        double nre = re, nim = im;
+        if (are) {
+            nre += asgn * are[k];
+            nim += asgn * aim[k];
+        }
+        if (bre) {
+            nre += bsgn * bre[k];
+            nim += bsgn * bim[k];
+        }
+        d += st->wk[k] * ((nre * nre + nim * nim) - (re * re + im * im));
+    }
+    return d;
+}
+
+double mo_gcmc_mol_const(const mo_gcmc_sys *s)
+{
+    /* self + intramolecular exclusion of one rigid molecule */
+    if (!s->ewald) return 0.0;
+    double e = 0.0;
+    for (int a = 0; a < s->ns; a++) e -= s->kappa / sqrt(M_PI) * s->sq[a] * s->sq[a];
+    for (int a = 0; a < s->ns; a++)
+        for (int b = a + 1; b < s->ns; b++) {
+            double d[3];
+            for (int i = 0; i < 3; i++) d[i] = s->ref[a][i] - s->ref[b][i];
+            double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            e -= s->sq[a] * s->sq[b] * (1.0 - erfc_as(s->kappa * r)) / r;
+        }
+    return e * MO_TOKCAL;
+}
+
+/* ---------------------------------------------------- molecule energy */
+/* Energy of a molecule with wrapped site coordinates r against the
+ * framework maps and every molecule except 'skip'.  Returns 0 on failure. */
+static int mol_energy(const mo_gcmc_sys *s, const mo_gcmc_state *st, const double *r, int skip,
+                      double *umap, double *upair)
+{
+    double um = 0.0;
+    for (int a = 0; a < s->ns; a++) {
+        double c[3];
+        mo_maptouc(s->fund_ell, r + 3 * a, c);
+        int pm = s->site_pmap[a];
+        if (pm >= 0) {
+            double p;
+            if (!mo_pothermite(s->map_npnts[pm], s->map_step[pm], s->map_ptr[pm], s->map_vals[pm],
+                               s->map_nf[pm], c, &p))
+                return 0;
+            um = um + p;
+        }
+        int em = s->site_emap[a];
+        if (em >= 0) {
+            double p;
+            if (!mo_pothermite(s->map_npnts[em], s->map_step[em], s->map_ptr[em], s->map_vals[em],
+                               s->map_nf[em], c, &p))
+                return 0;
+            um = um + p * s->sq[a];
+        }
+    }
+    double up = 0.0;
+    for (int m = 0; m < st->N; m++) {
+        if (m == skip) continue;
+        const double *ro = st->r + 3 * s->ns * m;
+        for (int a = 0; a < s->ns; a++)
+            for (int b = 0; b < s->ns; b++) {
+                double sep[3];
+                mo_minimage(s->full_ell, r + 3 * a, ro + 3 * b, sep);
+                double r2 = 0.0;
+                r2 = r2 + sep[0] * sep[0];
+                r2 = r2 + sep[1] * sep[1];
+                r2 = r2 + sep[2] * sep[2];
+                int ta = s->stype[a], tb = s->stype[b];
+                int t = ta * s->nst + tb;
+                if (s->lj_on[t]) {
+                    if (r2 <= s->lj_h2[t]) {
+                        if (r2 < s->lj_l2[t]) return 0;
+                        double r2i = 1.0 / r2;
+                        double r6i = r2i * r2i * r2i;
+                        double r12i = r6i * r6i;
+                        up = up + (s->lj_A[t] * r12i - s->lj_B[t] * r6i);
+                    }
+                }
+                if (s->ewald && r2 < s->rc2_real && s->sq[a] != 0.0 && s->sq[b] != 0.0) {
+                    double rr = sqrt(r2);
+                    up = up + s->sq[a] * s->sq[b] * erfc_as(s->kappa * rr) / rr * MO_TOKCAL;
+                }
+            }
+    }
+    *umap = um;
+    *upair = up;
+    return 1;
+}
+
+static void record(mo_gcmc_state *st, int step, int mtype, int acc, double de)
+{
+    if (!st->trace || step >= st->trace_cap) return;
+    mo_gcmc_trace *t = st->trace + step;
+    t->move = mtype;
+    t->accepted = acc;
+    t->N = st->N;
+    t->dE = de;
+}
+
+/* mcmoves_attempt (mcmoves.F90:290-335) */
+static int attempt(const mo_gcmc_sys *s, mo_gcmc_state *st, double deltaE, double bias, int has_x,
+                   double addxtra, int has_p, double prefactor)
+{
+    double rti = 1.0 / (MO_KCALMOLE_KB * s->T);
+    double u = -rti * deltaE;
+    u = u + log(bias);
+    if (has_x) u = u + addxtra;
+    if (u > MO_MAX_EXP) u = MO_MAX_EXP;
+    if (u < -MO_MAX_EXP) u = -MO_MAX_EXP;
+    double boltz = exp(u);
+    if (has_p) boltz = boltz * prefactor;
+    return boltz > mo_rranf(&st->rng);
+}
+
+int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
+{
+    const int ns = s->ns;
+    double rnew[24], rpnew[24], sre_new[1], sim_new[1];
+    (void)sre_new;
+    (void)sim_new;
+    double *ak = NULL, *bk = NULL, *ck = NULL, *dk = NULL;
+    if (s->ewald) {
+        ak = malloc(sizeof(double) * st->nk);
+        bk = malloc(sizeof(double) * st->nk);
+        ck = malloc(sizeof(double) * st->nk);
+        dk = malloc(sizeof(double) * st->nk);
+    }
+    const double mconst = mo_gcmc_mol_const(s);
+    for (long long it = 0; it < nsteps; it++) {
+        int step = (int)st->nsteps;
+        (void)(int)(mo_rranf(&st->rng) * 1.0); /* species draw, nspc = 1 (gcmc.F90:242) */
+        double prob = mo_rranf(&st->rng);
+        int mv = 0;
+        for (int i = 0; i < s->nmoves; i++) {
+            if (prob > s->cumwt[i]) mv++;
+            else break;
+        }
+        if (mv >= s->nmoves) mv = s->nmoves - 1;
+        int mtype = s->move_type[mv];
+        int N = st->N;
+        if (mtype == MO_MV_INSERT) {
+            st->att[0]++;
+            int accepted = 0;
+            double de = 0.0;
+            if (N >= st->maxN) return -3; /* capacity of the molecule arrays */
+            {
+                double com[3], ang[3] = {0.0, 0.0, 0.0};
+                for (int i = 0; i < 3; i++) com[i] = s->full_ell[i] * mo_rranf(&st->rng);
+                if (ns != 1) {
+                    double costheta = 1.0 - mo_rranf(&st->rng) * 2.0;
+                    ang[1] = acos(costheta);
+                    ang[0] = MO_PI - mo_rranf(&st->rng) * MO_TWOPI;
+                    ang[2] = MO_PI - mo_rranf(&st->rng) * MO_TWOPI;
+                }
+                toxyz(s, com, ang, rpnew);
+                pbc(s, rpnew, rnew);
+                double um, up;
+                int ok = mol_energy(s, st, rnew, -1, &um, &up);
+                double U = 0.0;
+                if (ok) {
+                    U = um + up;
+                    if (s->ewald) {
+                        mol_sk(s, st, rnew, ak, bk);
+                        U = U + drecip(st, ak, bk, 1.0, NULL, NULL, 0.0) + mconst;
+                    }
+                    if (U > s->max_nrg) ok = 0;
+                }
+                if (ok) {
+                    double bias = 1.0 / s->insdelratio;
+                    accepted = attempt(s, st, U, bias, 1, st->B, 1, 1.0 / (double)(float)(N + 1));
+                    if (accepted) {
+                        memcpy(st->com + 3 * N, com, sizeof com);
+                        memcpy(st->ang + 3 * N, ang, sizeof ang);
+                        memcpy(st->rp + 3 * ns * N, rpnew, sizeof(double) * 3 * ns);
+                        memcpy(st->r + 3 * ns * N, rnew, sizeof(double) * 3 * ns);
+                        if (s->ewald)
+                            for (int k = 0; k < st->nk; k++) {
+                                st->Sre[k] += ak[k];
+                                st->Sim[k] += bk[k];
+                            }
+                        st->N = N + 1;
+                        st->energy += U;
+                        st->acc[0]++;
+                        de = U;
+                    }
+                }
+            }
+            record(st, step, mtype, accepted, de);
+        } else if (mtype == MO_MV_DELETE) {
+            int accepted = 0;
+            double de = 0.0;
+            if (N != 0) {
+                int mol = (int)(mo_rranf(&st->rng) * N);
+                st->att[1]++;
+                const double *r = st->r + 3 * ns * mol;
+                double um, up;
+                int ok = mol_energy(s, st, r, mol, &um, &up);
+                double U = um + up;
+                if (ok && s->ewald) {
+                    mol_sk(s, st, r, ak, bk);
+                    U = U - drecip(st, ak, bk, -1.0, NULL, NULL, 0.0) + mconst;
+                }
+                if (!ok) return -1; /* mcmoves.F90:599-603: an existing molecule must evaluate */
+                double bias = 1.0 * s->insdelratio;
+                accepted = attempt(s, st, -U, bias, 1, -st->B, 1, 1.0 * N);
+                if (accepted) {
+                    int last = N - 1;
+                    memmove(st->com + 3 * mol, st->com + 3 * last, sizeof(double) * 3);
+                    memmove(st->ang + 3 * mol, st->ang + 3 * last, sizeof(double) * 3);
+                    memmove(st->rp + 3 * ns * mol, st->rp + 3 * ns * last, sizeof(double) * 3 * ns);
+                    memmove(st->r + 3 * ns * mol, st->r + 3 * ns * last, sizeof(double) * 3 * ns);
+                    if (s->ewald)
+                        for (int k = 0; k < st->nk; k++) {
+                            st->Sre[k] -= ak[k];
+                            st->Sim[k] -= bk[k];
+                        }
+                    st->N = N - 1;
+                    st->energy -= U;
+                    st->acc[1]++;
+                    de = -U;
+                }
+            }
+            record(st, step, mtype, accepted, de);
+        } else {
+            /* TRANSLATE / ROTATE */
+            int accepted = 0;
+            double de = 0.0;
+            int idx = (mtype == MO_MV_TRANSLATE) ? 2 : 3;
+            if (N != 0) {
+                int mol = (int)(mo_rranf(&st->rng) * N);
+                st->att[idx]++;
+                const double *rold = st->r + 3 * ns * mol;
+                double um0, up0;
+                if (!mol_energy(s, st, rold, mol, &um0, &up0)) return -2;
+                double com[3], ang[3];
+                memcpy(com, st->com + 3 * mol, sizeof com);
+                memcpy(ang, st->ang + 3 * mol, sizeof ang);
+                if (mtype == MO_MV_TRANSLATE) {
+                    double dv[3];
+                    for (int i = 0; i < 3; i++) dv[i] = ((mo_rranf(&st->rng) * 2.0) - 1.0) * s->dtrans;
+                    for (int i = 0; i < 3; i++) com[i] = com[i] + dv[i];
+                } else {
+                    double dphi = ((mo_rranf(&st->rng) * 2.0) - 1.0) * s->drot;
+                    double dpsi = ((mo_rranf(&st->rng) * 2.0) - 1.0) * s->drot;
+                    double dcos = ((mo_rranf(&st->rng) * 2.0) - 1.0) * s->drot;
+                    if (ns != 1) {
+                        double phi = ang[0] + dphi;
+                        double psi = ang[2] + dpsi;
+                        double costheta = cos(ang[1]) + dcos;
+                        ang[0] = phi - round(phi / MO_TWOPI) * MO_TWOPI;
+                        ang[2] = psi - round(psi / MO_TWOPI) * MO_TWOPI;
+                        costheta = costheta - round(costheta / 2.0) * 2.0;
+                        ang[1] = acos(costheta);
+                    }
+                }
+                toxyz(s, com, ang, rpnew);
+                pbc(s, rpnew, rnew);
+                double um1, up1;
+                int ok = mol_energy(s, st, rnew, mol, &um1, &up1);
+                double dU = 0.0;
+                if (ok) {
+                    double U1 = um1 + up1, U0 = um0 + up0;
+                    if (U1 > s->max_nrg) ok = 0;
+                    dU = U1 - U0;
+                    if (ok && s->ewald) {
+                        mol_sk(s, st, rold, ck, dk);
+                        mol_sk(s, st, rnew, ak, bk);
+                        dU = dU + drecip(st, ak, bk, 1.0, ck, dk, -1.0);
+                    }
+                }
+                if (ok) {
+                    accepted = attempt(s, st, dU, 1.0, 0, 0.0, 0, 1.0);
+                    if (accepted) {
+                        memcpy(st->com + 3 * mol, com, sizeof com);
+                        memcpy(st->ang + 3 * mol, ang, sizeof ang);
+                        memcpy(st->rp + 3 * ns * mol, rpnew, sizeof(double) * 3 * ns);
+                        memcpy(st->r + 3 * ns * mol, rnew, sizeof(double) * 3 * ns);
+                        if (s->ewald)
+                            for (int k = 0; k < st->nk; k++) {
+                                st->Sre[k] += ak[k] - ck[k];
+                                st->Sim[k] += bk[k] - dk[k];
+                            }
+                        st->energy += dU;
+                        st->acc[idx]++;
+                        de = dU;
+                    }
+                }
+            }
+            record(st, step, mtype, accepted, de);
+        }
+        st->sumN += st->N;
+        st->nsteps++;
+    }
+    free(ak);
+    free(bk);
+    free(ck);
+    free(dk);
+    return 0;
+}
+
+/* full-system energy (checks the running total, interact_checknrgs) */
+double mo_gcmc_total_energy(const mo_gcmc_sys *s, const mo_gcmc_state *st)
+{
+    double e = 0.0;
+    mo_gcmc_state tmp = *st;
+    /* map terms and pair terms, each pair once */
+    for (int m = 0; m < st->N; m++) {
+        tmp.N = m; /* molecules 0..m-1 only */
+        double um, up;
+        mol_energy(s, &tmp, st->r + 3 * s->ns * m, -1, &um, &up);
+        e += um + up;
+    }
+    if (s->ewald) {
+        double er = 0.0;
+        for (int k = 0; k < st->nk; k++) er += st->wk[k] * (st->Sre[k] * st->Sre[k] + st->Sim[k] * st->Sim[k]);
+        e += er + st->N * mo_gcmc_mol_const(s);
+    }
+    return e;
+}
+
+void mo_gcmc_recompute_sk(const mo_gcmc_sys *s, mo_gcmc_state *st)
+{
+    for (int k = 0; k < st->nk; k++) st->Sre[k] = st->Sim[k] = 0.0;
+    if (!s->ewald) return;
+    double *a = malloc(sizeof(double) * st->nk), *b = malloc(sizeof(double) * st->nk);
+    for (int m = 0; m < st->N; m++) {
+        mol_sk(s, st, st->r + 3 * s->ns * m, a, b);
+        for (int k = 0; k < st->nk; k++) {
+            st->Sre[k] += a[k];
+            st->Sim[k] += b[k];
+        }
+    }
+    free(a);
+    free(b);
+}
+
+/* place molecule m at com/ang (for seeded initial loadings) */
+void mo_gcmc_set_mol(const mo_gcmc_sys *s, mo_gcmc_state *st, int m, const double com[3], const double ang[3])
+{
+    memcpy(st->com + 3 * m, com, sizeof(double) * 3);
+    memcpy(st->ang + 3 * m, ang, sizeof(double) * 3);
+    toxyz(s, com, ang, st->rp + 3 * s->ns * m);
+    pbc(s, st->rp + 3 * s->ns * m, st->r + 3 * s->ns * m);
+}
+
+/* gcmc_genfuglist (gcmc.F90:303-319): B = ln(f 1e3 V 1e-30 N_A / (R T)) - ln(Zig) */
+double mo_adams_B(double fug_kPa, double volume_A3, double T, double Zig)
+{
+    double sivolume = volume_A3 * 1.0e-30 * MO_NAV;
+    double B = log(fug_kPa * 1.0e3 * sivolume / (MO_RGAS * T));
+    return B - log(Zig);
+}

@@ -1,0 +1,70 @@
+/*
+ * music_gcmc_oracle.h -- GCMC move-loop oracle (test infrastructure only).
+ */
+#ifndef MUSIC_GCMC_ORACLE_H
+#define MUSIC_GCMC_ORACLE_H
+#include "music_oracle.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { MO_MV_INSERT = 0, MO_MV_DELETE = 1, MO_MV_TRANSLATE = 2, MO_MV_ROTATE = 3 };
+#define MO_MAXS 8
+#define MO_MAXMAPS 8
+
+typedef struct {
+    double full_ell[3], fund_ell[3];
+    int ns;
+    double ref[MO_MAXS][3];
+    int stype[MO_MAXS];
+    double sq[MO_MAXS];
+    int site_pmap[MO_MAXS], site_emap[MO_MAXS];
+    int nmaps;
+    const int32_t *map_ptr[MO_MAXMAPS];
+    const float *map_vals[MO_MAXMAPS]; /* SoA [8][nf] as in the map file */
+    int map_npnts[MO_MAXMAPS][3];
+    double map_step[MO_MAXMAPS][3];
+    long long map_nf[MO_MAXMAPS];
+    int nst;                                  /* sorbate site types */
+    double lj_A[64], lj_B[64], lj_h2[64], lj_l2[64];
+    int lj_on[64];                            /* [nst][nst] */
+    int ewald, kmax;
+    double kappa, rc2_real;
+    double T;
+    int nmoves;
+    int move_type[4];
+    double cumwt[4];
+    double insdelratio, dtrans, drot, max_nrg;
+} mo_gcmc_sys;
+
+typedef struct {
+    int move, accepted, N;
+    double dE;
+} mo_gcmc_trace;
+
+typedef struct {
+    int N, maxN;
+    double *com, *ang, *rp, *r;  /* [maxN][3], [maxN][3], [maxN][ns][3] x2 */
+    int nk;
+    double *kv, *wk, *Sre, *Sim; /* [nk][3], [nk], [nk], [nk] */
+    double energy, B;
+    mo_rng rng;
+    long long att[4], acc[4];
+    double sumN;
+    long long nsteps;
+    mo_gcmc_trace *trace;
+    int trace_cap;
+} mo_gcmc_state;
+
+void mo_gcmc_kvecs(const mo_gcmc_sys *s, int *nk, double *kv, double *w);
+double mo_gcmc_mol_const(const mo_gcmc_sys *s);
+int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps);
+double mo_gcmc_total_energy(const mo_gcmc_sys *s, const mo_gcmc_state *st);
+void mo_gcmc_recompute_sk(const mo_gcmc_sys *s, mo_gcmc_state *st);
+void mo_gcmc_set_mol(const mo_gcmc_sys *s, mo_gcmc_state *st, int m, const double com[3], const double ang[3]);
+double mo_adams_B(double fug_kPa, double volume_A3, double T, double Zig);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -22,6 +22,7 @@
  *      exact reproduction of tricubic polynomials, finite differences.
  */
 #include "music_oracle.h"
+#include "music_gcmc_oracle.h"
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>

@@ -225,3 +225,175 @@ def pothermite(npnts, step, ptr, vals8, coords, fund_ell=None, force=False):
                                          vals8.shape[1], _p(c), ctypes.byref(v)))
             out[i] = v.value
     return ok, out
+
+
+# ------------------------------------------------------------------ GCMC
+MV_INSERT, MV_DELETE, MV_TRANSLATE, MV_ROTATE = 0, 1, 2, 3
+MOVE_NAMES = {"RINSERT": 0, "RDELETE": 1, "RTRANSLATE": 2, "RROTATE": 3,
+              "INSERT": 0, "DELETE": 1, "TRANSLATE": 2, "ROTATE": 3}
+
+
+class GcmcSys(ctypes.Structure):
+    _fields_ = [("full_ell", _d * 3), ("fund_ell", _d * 3), ("ns", _i),
+                ("ref", (_d * 3) * 8), ("stype", _i * 8), ("sq", _d * 8),
+                ("site_pmap", _i * 8), ("site_emap", _i * 8), ("nmaps", _i),
+                ("map_ptr", _vp * 8), ("map_vals", _vp * 8), ("map_npnts", (_i * 3) * 8),
+                ("map_step", (_d * 3) * 8), ("map_nf", _ll * 8),
+                ("nst", _i), ("lj_A", _d * 64), ("lj_B", _d * 64), ("lj_h2", _d * 64),
+                ("lj_l2", _d * 64), ("lj_on", _i * 64),
+                ("ewald", _i), ("kmax", _i), ("kappa", _d), ("rc2_real", _d), ("T", _d),
+                ("nmoves", _i), ("move_type", _i * 4), ("cumwt", _d * 4),
+                ("insdelratio", _d), ("dtrans", _d), ("drot", _d), ("max_nrg", _d)]
+
+
+class GcmcTrace(ctypes.Structure):
+    _fields_ = [("move", _i), ("accepted", _i), ("N", _i), ("dE", _d)]
+
+
+class _Rng(ctypes.Structure):
+    _fields_ = [("ix", ctypes.c_int32), ("iy", ctypes.c_int32), ("iseed", ctypes.c_int32),
+                ("first", ctypes.c_int32)]
+
+
+class GcmcState(ctypes.Structure):
+    _fields_ = [("N", _i), ("maxN", _i), ("com", _vp), ("ang", _vp), ("rp", _vp), ("r", _vp),
+                ("nk", _i), ("kv", _vp), ("wk", _vp), ("Sre", _vp), ("Sim", _vp),
+                ("energy", _d), ("B", _d), ("rng", _Rng), ("att", _ll * 4), ("acc", _ll * 4),
+                ("sumN", _d), ("nsteps", _ll), ("trace", _vp), ("trace_cap", _i)]
+
+
+def _gcmc_lib():
+    L = lib()
+    if not hasattr(L, "_gcmc_sig"):
+        L.mo_gcmc_kvecs.argtypes = [_vp, _vp, _vp, _vp]
+        L.mo_gcmc_mol_const.argtypes = [_vp]
+        L.mo_gcmc_mol_const.restype = _d
+        L.mo_gcmc_run.argtypes = [_vp, _vp, _ll]
+        L.mo_gcmc_run.restype = _i
+        L.mo_gcmc_total_energy.argtypes = [_vp, _vp]
+        L.mo_gcmc_total_energy.restype = _d
+        L.mo_gcmc_recompute_sk.argtypes = [_vp, _vp]
+        L.mo_gcmc_set_mol.argtypes = [_vp, _vp, _i, _vp, _vp]
+        L.mo_adams_B.argtypes = [_d, _d, _d, _d]
+        L.mo_adams_B.restype = _d
+        L.mo_rng_init.argtypes = [_vp, ctypes.c_int32]
+        L._gcmc_sig = True
+    return L
+
+
+def adams_B(fug_kPa, volume, T, Zig=1.0):
+    return _gcmc_lib().mo_adams_B(fug_kPa, volume, T, Zig)
+
+
+class Gcmc:
+    """One GCMC chain on the CPU oracle.  spec: dict with full_ell, fund_ell,
+    ref [ns,3], stype [ns], sq [ns], nst, lj (A, B, h2, l2, on: [nst*nst]),
+    maps: list of (npnts, step, ptr, vals8) and site_pmap/site_emap,
+    ewald (kappa, kmax, rc), T, moves [(type, weight)], dtrans, drot."""
+
+    def __init__(self, spec, fugacity_kPa, seed, maxN=2048, trace=0):
+        L = _gcmc_lib()
+        s = GcmcSys()
+        self.keep = []
+        for i in range(3):
+            s.full_ell[i] = spec["full_ell"][i]
+            s.fund_ell[i] = spec["fund_ell"][i]
+        ref = np.asarray(spec["ref"], dtype=np.float64).reshape(-1, 3)
+        s.ns = ns = len(ref)
+        for a in range(ns):
+            for i in range(3):
+                s.ref[a][i] = ref[a][i]
+            s.stype[a] = int(spec["stype"][a])
+            s.sq[a] = float(spec["sq"][a])
+            s.site_pmap[a] = int(spec.get("site_pmap", [-1] * ns)[a])
+            s.site_emap[a] = int(spec.get("site_emap", [-1] * ns)[a])
+        maps = spec.get("maps", [])
+        s.nmaps = len(maps)
+        for j, (npn, step, ptr, vals) in enumerate(maps):
+            ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+            vals = np.ascontiguousarray(vals, dtype=np.float32)
+            self.keep += [ptr, vals]
+            s.map_ptr[j] = ptr.ctypes.data
+            s.map_vals[j] = vals.ctypes.data
+            s.map_nf[j] = vals.shape[1]
+            for i in range(3):
+                s.map_npnts[j][i] = int(npn[i])
+                s.map_step[j][i] = float(step[i])
+        s.nst = spec["nst"]
+        A, B, h2, l2, on = spec["lj"]
+        for t in range(s.nst * s.nst):
+            s.lj_A[t], s.lj_B[t], s.lj_h2[t], s.lj_l2[t], s.lj_on[t] = A[t], B[t], h2[t], l2[t], int(on[t])
+        ew = spec.get("ewald")
+        if ew:
+            s.ewald, s.kappa, s.kmax, s.rc2_real = 1, ew["kappa"], ew["kmax"], ew["rc"] ** 2
+        s.T = spec["T"]
+        mv = spec["moves"]
+        s.nmoves = len(mv)
+        wsum = sum(w for _, w in mv)
+        c = 0.0
+        wi = wd = None
+        for i, (t, w) in enumerate(mv):
+            s.move_type[i] = MOVE_NAMES[t] if isinstance(t, str) else int(t)
+            c = c + w / wsum
+            s.cumwt[i] = c
+            if s.move_type[i] == 0:
+                wi = w / wsum
+            if s.move_type[i] == 1:
+                wd = w / wsum
+        s.insdelratio = (wi / wd) if (wi and wd) else 1.0
+        s.dtrans = spec.get("dtrans", 0.0)
+        s.drot = spec.get("drot", 0.0)
+        s.max_nrg = spec.get("max_nrg", 1.0e10)
+        self.sys = s
+        st = GcmcState()
+        st.maxN = maxN
+        self.com = np.zeros((maxN, 3))
+        self.ang = np.zeros((maxN, 3))
+        self.rp = np.zeros((maxN, ns, 3))
+        self.r = np.zeros((maxN, ns, 3))
+        st.com, st.ang, st.rp, st.r = [a.ctypes.data for a in (self.com, self.ang, self.rp, self.r)]
+        nk = ctypes.c_int()
+        L.mo_gcmc_kvecs(ctypes.byref(s), ctypes.byref(nk), None, None)
+        n = nk.value if ew else 0
+        self.kv = np.zeros((max(n, 1), 3))
+        self.wk = np.zeros(max(n, 1))
+        self.Sre = np.zeros(max(n, 1))
+        self.Sim = np.zeros(max(n, 1))
+        if ew:
+            L.mo_gcmc_kvecs(ctypes.byref(s), ctypes.byref(nk), self.kv.ctypes.data, self.wk.ctypes.data)
+        st.nk = n
+        st.kv, st.wk, st.Sre, st.Sim = [a.ctypes.data for a in (self.kv, self.wk, self.Sre, self.Sim)]
+        vol = float(np.prod(spec["full_ell"]))
+        st.B = adams_B(fugacity_kPa, vol, spec["T"])
+        L.mo_rng_init(ctypes.byref(st.rng), int(seed))
+        self.trace = (GcmcTrace * max(trace, 1))()
+        st.trace = ctypes.cast(self.trace, _vp) if trace else None
+        st.trace_cap = trace
+        self.st = st
+
+    def set_config(self, com, ang=None):
+        L = _gcmc_lib()
+        com = np.asarray(com, dtype=np.float64).reshape(-1, 3)
+        ang = np.zeros_like(com) if ang is None else np.asarray(ang, dtype=np.float64).reshape(-1, 3)
+        for m in range(len(com)):
+            L.mo_gcmc_set_mol(ctypes.byref(self.sys), ctypes.byref(self.st), m,
+                              np.ascontiguousarray(com[m]).ctypes.data,
+                              np.ascontiguousarray(ang[m]).ctypes.data)
+        self.st.N = len(com)
+        L.mo_gcmc_recompute_sk(ctypes.byref(self.sys), ctypes.byref(self.st))
+        self.st.energy = self.total_energy()
+
+    def run(self, nsteps):
+        rc = _gcmc_lib().mo_gcmc_run(ctypes.byref(self.sys), ctypes.byref(self.st), nsteps)
+        if rc:
+            raise RuntimeError("mo_gcmc_run failed %d" % rc)
+
+    def total_energy(self):
+        return _gcmc_lib().mo_gcmc_total_energy(ctypes.byref(self.sys), ctypes.byref(self.st))
+
+    def mol_const(self):
+        return _gcmc_lib().mo_gcmc_mol_const(ctypes.byref(self.sys))
+
+    def trace_array(self, n=None):
+        n = self.st.trace_cap if n is None else n
+        return np.array([(t.move, t.accepted, t.N, t.dE) for t in self.trace[:n]])
