@@ -129,6 +129,83 @@ int mh_map_from_host(mh_ctx *ctx, const int npnts[3], int64_t nfsize, const int3
 int mh_map_interp(mh_ctx *ctx, const mh_map *map, int n, const double *xyz, double factor,
                   double *e, uint8_t *ok);
 
+/* ------------------------------------------------------------------ GCMC
+ * Device-resident grand-canonical MC of one rigid sorbate species in a
+ * fixed framework (gcmc_dosim, gcmc.F90:224-258), many independent replicas
+ * (state points) per batch, one workgroup per replica, one persistent kernel
+ * launch per mh_gcmc_run call.  Moves RINSERT / RDELETE / RTRANSLATE /
+ * RROTATE (rigidmoves.F90:691-1570) with the reference's RNG (rranf_NR,
+ * random.F90:121-152, one stream per replica) and draw order, acceptance
+ * mcmoves_attempt (mcmoves.F90:290-335).  Molecule energies: framework maps
+ * (interpolate_int, factor q for emaps) + sorbate-sorbate LJ with minimum
+ * image (ssbasic_msint, lj_snglint) + optional Ewald electrostatics with
+ * structure-factor deltas (real space + reciprocal + self + intramolecular
+ * exclusion; no reference counterpart: the reference stops,
+ * ssdriver.F90:489-498).  Failures (LJ locut, untabulated map corner,
+ * U > max_nrg) reject the move without an RNG draw. */
+enum { MH_MV_INSERT = 0, MH_MV_DELETE = 1, MH_MV_TRANSLATE = 2, MH_MV_ROTATE = 3 };
+#define MH_MAX_SITES 8
+
+typedef struct {
+    int nsites;                          /* 1..8 */
+    double ref[MH_MAX_SITES][3];         /* body-frame site coordinates */
+    int site_type[MH_MAX_SITES];         /* index into the sorbate LJ table */
+    double site_q[MH_MAX_SITES];         /* e */
+    const mh_map *site_pmap[MH_MAX_SITES]; /* LJ map of each site or NULL */
+    const mh_map *site_emap[MH_MAX_SITES]; /* electrostatic map or NULL (factor q) */
+    int ntypes;                          /* sorbate site types, 1..8 */
+    double lj_A[64], lj_B[64], lj_hicut2[64], lj_locut2[64]; /* [ntypes][ntypes] */
+    uint8_t lj_on[64];
+    int ewald;                           /* 1: Ewald sorbate-sorbate electrostatics */
+    double ewald_kappa;                  /* 1/A */
+    int ewald_kmax;                      /* |m_i| <= kmax, half space */
+    double ewald_rcut;                   /* real-space cutoff, A */
+    double temperature;                  /* K */
+    int nmoves;
+    int move_type[4];                    /* MH_MV_* */
+    double move_weight[4];
+    double delta_trans, delta_rot;       /* fixed displacements (A, rad) */
+    double max_nrg;                      /* kcal/mol (1e10 in the reference) */
+    int precision;                       /* MH_PREC_FP64 or MH_PREC_FP32 */
+} mh_gcmc_params;
+
+typedef struct mh_gcmc mh_gcmc;
+
+/* nrep replicas; fugacity per replica (kPa, ideal gas: Adams B from
+ * gcmc_genfuglist, gcmc.F90:303-319); seeds per replica (iseed of
+ * random_init); maxN = molecule capacity per replica. */
+int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double *fugacity_kPa,
+                   const int32_t *seeds, int maxN, mh_gcmc **out);
+int mh_gcmc_destroy(mh_gcmc *g);
+/* Threads per replica (64..1024, power of 2); chosen by the autotuner. */
+int mh_gcmc_set_threads(mh_gcmc *g, int threads);
+/* Place n molecules (com[n][3], Euler angles phi, theta, psi [n][3]) in a
+ * replica; recomputes its structure factor and energy. */
+int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const double *ang);
+/* nsteps MC steps on every replica in one kernel launch. */
+int mh_gcmc_run(mh_gcmc *g, int nsteps);
+/* Per replica: N, running energy (kcal/mol), sum of N over steps, steps,
+ * att/acc [nrep][4] by move type, RNG state (for checkpoints).  Any
+ * pointer may be NULL. */
+int mh_gcmc_get_state(mh_gcmc *g, int32_t *N, double *energy, double *sumN, int64_t *nsteps,
+                      int64_t *att, int64_t *acc, int32_t *rng_ix, int32_t *rng_iy);
+/* Restore a checkpoint: N and RNG state per replica (the configuration via
+ * mh_gcmc_set_config).  The reference does not save the RNG state. */
+int mh_gcmc_set_rng(mh_gcmc *g, int rep, int32_t ix, int32_t iy);
+int mh_gcmc_get_config(mh_gcmc *g, int rep, int *n, double *com, double *ang, double *r);
+/* Full FP64 recompute of every replica's energy (interact_checknrgs); with
+ * refresh != 0 the structure factors and running energies are reset to it. */
+int mh_gcmc_total_energy(mh_gcmc *g, double *e, int refresh);
+/* Per-step trace (move, accepted, N after, energy change) of every replica,
+ * capacity steps from the next run on. */
+int mh_gcmc_set_trace(mh_gcmc *g, int capacity);
+int mh_gcmc_get_trace(mh_gcmc *g, int rep, int *move, int *accepted, int *N, double *dE, int *ntrace);
+/* Insertion energies of n trial molecules (com, Euler angles) into the
+ * current configuration of replica rep (subinteract_int of a new molecule),
+ * in the batch precision; ok = 0 where the energy evaluation fails. */
+int mh_trial_energy(mh_gcmc *g, int rep, int n, const double *com, const double *ang, double *e,
+                    uint8_t *ok);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,12 @@ void DBuf::release() {
     n = 0;
 }
 
+void ctx_retain(mh_ctx *ctx) { ctx->refs++; }
+
+void ctx_release(mh_ctx *ctx) {
+    if (--ctx->refs == 0 && ctx->closing) mh_ctx_destroy(ctx);
+}
+
 void *ws_get(mh_ctx *ctx, const std::string &name, size_t bytes, int *rc) {
     auto &slot = ctx->ws[name];
     if (!slot) slot.reset(new DBuf());
@@ -120,10 +126,14 @@ int mh_ctx_create(int device, mh_ctx **out) {
 
 int mh_ctx_destroy(mh_ctx *ctx) {
     if (!ctx) return MH_OK;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    ctx->closing = true;
+    if (ctx->refs == 0) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->ws.clear();
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    }
     return MH_OK;
 }
 

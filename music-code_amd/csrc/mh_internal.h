@@ -68,6 +68,10 @@ struct Ewald {
 
 struct mh_ctx {
     int device = 0;
+    // maps and GCMC batches keep their context alive: mh_ctx_destroy frees
+    // it only when the last of them is destroyed
+    int refs = 0;
+    bool closing = false;
     hipStream_t stream = nullptr;
     // framework
     int natoms = 0;
@@ -91,6 +95,8 @@ struct mh_ctx {
 };
 
 namespace mh {
+void ctx_retain(mh_ctx *ctx);
+void ctx_release(mh_ctx *ctx);
 // Scratch buffer 'name' of at least 'bytes' bytes owned by the context.
 void *ws_get(mh_ctx *ctx, const std::string &name, size_t bytes, int *rc);
 
@@ -125,6 +131,7 @@ struct KTimer {
 
 struct mh_map {
     mh_ctx *ctx = nullptr;
+    bool retained = false;
     int npnts[3] = {0, 0, 0};
     double step[3] = {0, 0, 0};
     double fund_ell[3] = {0, 0, 0};

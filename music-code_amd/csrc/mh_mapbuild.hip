@@ -1123,6 +1123,8 @@ extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q
         map->full64.n = full64k.n;
         full64k.p = nullptr;
     }
+    map->retained = true;
+    ctx_retain(ctx);
     *out = map;
     return MH_OK;
 }

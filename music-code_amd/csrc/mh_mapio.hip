@@ -3,73 +3,10 @@
 #include <cstring>
 #include <vector>
 
+#include "mh_device.h"
 #include "mh_internal.h"
 
 namespace mh {
-
-// tricubic Hermite at one point (interpolate_int + pothermite,
-// interpolate.F90:104-158, 603-874); the 64-term sum runs corner-major in the
-// reference order, FP32 table values widened to FP64.
-__device__ __forceinline__ double h0f(double x) { return x * x * (3.0 - 2.0 * x); }
-__device__ __forceinline__ double h1f(double x, double ds) { return ds * x * x * (x - 1.0); }
-
-__device__ bool hermite_point(const int32_t *__restrict__ ptr, const float *__restrict__ vals,
-                              int nx, int ny, int nz, double sx, double sy, double sz,
-                              double ex, double ey, double ez, double rx, double ry, double rz,
-                              double &p) {
-    // simcell_maptouc (simcell.F90:634-668), orthorhombic, origin 0
-    double c[3] = {rx, ry, rz};
-    const double ell[3] = {ex, ey, ez};
-    const double st[3] = {sx, sy, sz};
-    const int nb[3] = {nx, ny, nz};
-    int idx[3], idp[3];
-    double tuv[3];
-    for (int d = 0; d < 3; d++) {
-        double f = c[d] * (1.0 / ell[d]);
-        f = (f >= 0.0) ? fmod(f, 1.0) : fmod(f, 1.0) + 1.0;
-        c[d] = f * ell[d];
-        idx[d] = (int)(c[d] / st[d] + 1.0);  // mapheader_getIndices, Int()
-        if (idx[d] < 1 || idx[d] > nb[d]) {
-            p = 0.0;
-            return false;
-        }
-        tuv[d] = (c[d] - (double)(idx[d] - 1) * st[d]) / st[d];
-        idp[d] = (idx[d] == nb[d]) ? 1 : idx[d] + 1;
-    }
-#define P(i, j, k) ptr[(((long long)(i) - 1) * ny + ((j) - 1)) * nz + ((k) - 1)]
-    const int ii = idx[0], jj = idx[1], kk = idx[2], ip = idp[0], jp = idp[1], kp = idp[2];
-    int32_t mi[8] = {P(ii, jj, kk), P(ip, jj, kk), P(ii, jp, kk), P(ii, jj, kp),
-                     P(ip, jp, kk), P(ip, jj, kp), P(ii, jp, kp), P(ip, jp, kp)};
-#undef P
-    for (int i = 0; i < 8; i++)
-        if (mi[i] == 0) {
-            p = 0.0;
-            return false;
-        }
-    double hx[2][2], hy[2][2], hz[2][2];
-    hx[0][0] = h0f(1 - tuv[0]); hx[1][0] = h0f(tuv[0]);
-    hx[0][1] = h1f(1 - tuv[0], -sx); hx[1][1] = h1f(tuv[0], sx);
-    hy[0][0] = h0f(1 - tuv[1]); hy[1][0] = h0f(tuv[1]);
-    hy[0][1] = h1f(1 - tuv[1], -sy); hy[1][1] = h1f(tuv[1], sy);
-    hz[0][0] = h0f(1 - tuv[2]); hz[1][0] = h0f(tuv[2]);
-    hz[0][1] = h1f(1 - tuv[2], -sz); hz[1][1] = h1f(tuv[2], sz);
-    // corner pattern and value-slot pattern (interpolate.F90:725-835)
-    const int CX = 0xB2, CY = 0xD4, CZ = 0xE8;  // bit i: upper node along axis for corner i
-    double sum = 0.0;
-    for (int i = 0; i < 8; i++) {
-        const float4 *r = reinterpret_cast<const float4 *>(vals + 8 * (long long)(mi[i] - 1));
-        float4 a = r[0], b = r[1];
-        const float f[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        const int cx = (CX >> i) & 1, cy = (CY >> i) & 1, cz = (CZ >> i) & 1;
-#pragma unroll
-        for (int s = 0; s < 8; s++) {
-            const int dx = (CX >> s) & 1, dy = (CY >> s) & 1, dz = (CZ >> s) & 1;
-            sum = __dadd_rn(sum, __dmul_rn(__dmul_rn(__dmul_rn((double)f[s], hx[cx][dx]), hy[cy][dy]), hz[cz][dz]));
-        }
-    }
-    p = sum;
-    return true;
-}
 
 __global__ void k_interp(int n, const double *__restrict__ xyz, const int32_t *__restrict__ ptr,
                          const float *__restrict__ vals, int nx, int ny, int nz, double sx,
@@ -128,8 +65,11 @@ extern "C" {
 
 int mh_map_destroy(mh_map *map) {
     if (!map) return MH_OK;
-    if (map->ctx) (void)hipSetDevice(map->ctx->device);
+    mh_ctx *ctx = map->ctx;
+    bool ret = map->retained;
+    if (ctx) (void)hipSetDevice(ctx->device);
     delete map;
+    if (ctx && ret) ctx_release(ctx);
     return MH_OK;
 }
 
@@ -258,6 +198,8 @@ static int upload_map(mh_ctx *ctx, const int np[3], int64_t nf, const int32_t *p
             return MH_EHIP;
         }
     }
+    m->retained = true;
+    ctx_retain(ctx);
     *out = m;
     return MH_OK;
 }

@@ -1,0 +1,203 @@
+"""Host side of the device-resident GCMC (mh_gcmc_* in include/music_hip.h):
+builds mh_gcmc_params, runs batches of replicas and reads statistics.
+
+Mirrors the reference's GCMC driver pieces: move set and weights
+(mcmoveset.F90:280-335), Adams B from fugacity (gcmc.F90:266-347), block /
+cumulative loading statistics (stats.F90:152-195, gcmcmoves.F90:371-382).
+"""
+import ctypes
+
+import numpy as np
+
+from . import constants as C
+from . import hip as H
+
+MH_MV_INSERT, MH_MV_DELETE, MH_MV_TRANSLATE, MH_MV_ROTATE = 0, 1, 2, 3
+MOVES = {"RINSERT": 0, "RDELETE": 1, "RTRANSLATE": 2, "RROTATE": 3,
+         "INSERT": 0, "DELETE": 1, "TRANSLATE": 2, "ROTATE": 3}
+
+_d, _i, _vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+
+
+class GcmcParams(ctypes.Structure):
+    _fields_ = [("nsites", _i), ("ref", (_d * 3) * 8), ("site_type", _i * 8), ("site_q", _d * 8),
+                ("site_pmap", _vp * 8), ("site_emap", _vp * 8), ("ntypes", _i),
+                ("lj_A", _d * 64), ("lj_B", _d * 64), ("lj_hicut2", _d * 64), ("lj_locut2", _d * 64),
+                ("lj_on", ctypes.c_uint8 * 64), ("ewald", _i), ("ewald_kappa", _d), ("ewald_kmax", _i),
+                ("ewald_rcut", _d), ("temperature", _d), ("nmoves", _i), ("move_type", _i * 4),
+                ("move_weight", _d * 4), ("delta_trans", _d), ("delta_rot", _d), ("max_nrg", _d),
+                ("precision", _i)]
+
+
+def _sig():
+    L = H.lib()
+    if getattr(L, "_gcmc_sig", False):
+        return L
+    P = ctypes.POINTER
+    sig = {
+        "mh_gcmc_create": [_vp, _vp, _i, _vp, _vp, _i, P(_vp)],
+        "mh_gcmc_destroy": [_vp],
+        "mh_gcmc_set_threads": [_vp, _i],
+        "mh_gcmc_set_config": [_vp, _i, _i, _vp, _vp],
+        "mh_gcmc_run": [_vp, _i],
+        "mh_gcmc_get_state": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+        "mh_gcmc_set_rng": [_vp, _i, ctypes.c_int32, ctypes.c_int32],
+        "mh_gcmc_get_config": [_vp, _i, P(_i), _vp, _vp, _vp],
+        "mh_gcmc_total_energy": [_vp, _vp, _i],
+        "mh_gcmc_set_trace": [_vp, _i],
+        "mh_gcmc_get_trace": [_vp, _i, _vp, _vp, _vp, _vp, P(_i)],
+        "mh_trial_energy": [_vp, _i, _i, _vp, _vp, _vp, _vp],
+    }
+    for n, a in sig.items():
+        f = getattr(L, n)
+        f.argtypes = a
+        f.restype = _i
+    L._gcmc_sig = True
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp) if a is not None else None
+
+
+def make_params(spec, maps=None, precision=H.MH_PREC_FP64):
+    """spec as for the oracle's Gcmc (ref, stype, sq, nst, lj, ewald, T,
+    moves, dtrans, drot); maps: dict site index -> (pmap Map or None,
+    emap Map or None)."""
+    p = GcmcParams()
+    ref = np.asarray(spec["ref"], dtype=np.float64).reshape(-1, 3)
+    p.nsites = len(ref)
+    for a in range(len(ref)):
+        for i in range(3):
+            p.ref[a][i] = ref[a][i]
+        p.site_type[a] = int(spec["stype"][a])
+        p.site_q[a] = float(spec["sq"][a])
+        if maps and a in maps:
+            pm, em = maps[a]
+            p.site_pmap[a] = pm.h.value if pm is not None else None
+            p.site_emap[a] = em.h.value if em is not None else None
+    p.ntypes = spec["nst"]
+    A, B, h2, l2, on = spec["lj"]
+    for t in range(p.ntypes * p.ntypes):
+        p.lj_A[t], p.lj_B[t], p.lj_hicut2[t], p.lj_locut2[t] = A[t], B[t], h2[t], l2[t]
+        p.lj_on[t] = 1 if on[t] else 0
+    ew = spec.get("ewald")
+    if ew:
+        p.ewald, p.ewald_kappa, p.ewald_kmax, p.ewald_rcut = 1, ew["kappa"], ew["kmax"], ew["rc"]
+    p.temperature = spec["T"]
+    p.nmoves = len(spec["moves"])
+    for i, (t, w) in enumerate(spec["moves"]):
+        p.move_type[i] = MOVES[t] if isinstance(t, str) else int(t)
+        p.move_weight[i] = w
+    p.delta_trans = spec.get("dtrans", 0.0)
+    p.delta_rot = spec.get("drot", 0.0)
+    p.max_nrg = spec.get("max_nrg", C.VERY_LRG_NRG)
+    p.precision = precision
+    return p
+
+
+class GcmcBatch:
+    """nrep replicas advanced together on the device (one workgroup each)."""
+
+    def __init__(self, ctx, spec, fugacities, seeds, maxN=1024, maps=None,
+                 precision=H.MH_PREC_FP64, threads=None):
+        L = _sig()
+        self.ctx = ctx
+        self.params = make_params(spec, maps, precision)
+        self.nrep = len(fugacities)
+        self.ns = self.params.nsites
+        self.maxN = maxN
+        f = np.ascontiguousarray(fugacities, dtype=np.float64)
+        s = np.ascontiguousarray(seeds, dtype=np.int32)
+        h = _vp()
+        H.check(L.mh_gcmc_create(ctx.h, ctypes.byref(self.params), self.nrep, _p(f), _p(s), maxN,
+                                 ctypes.byref(h)), "mh_gcmc_create")
+        self.h = h
+        H._LIVE["gcmc"].add(self)
+        self.maps = maps  # keep the map objects alive
+        if threads:
+            self.set_threads(threads)
+
+    def set_threads(self, n):
+        H.check(_sig().mh_gcmc_set_threads(self.h, int(n)), "mh_gcmc_set_threads")
+
+    def set_config(self, rep, com, ang=None):
+        com = np.ascontiguousarray(com, dtype=np.float64).reshape(-1, 3)
+        ang = None if ang is None else np.ascontiguousarray(ang, dtype=np.float64).reshape(-1, 3)
+        H.check(_sig().mh_gcmc_set_config(self.h, rep, len(com), _p(com), _p(ang)), "mh_gcmc_set_config")
+
+    def run(self, nsteps):
+        H.check(_sig().mh_gcmc_run(self.h, int(nsteps)), "mh_gcmc_run")
+
+    def state(self):
+        R = self.nrep
+        N = np.zeros(R, dtype=np.int32)
+        e = np.zeros(R)
+        sN = np.zeros(R)
+        ns = np.zeros(R, dtype=np.int64)
+        att = np.zeros((R, 4), dtype=np.int64)
+        acc = np.zeros((R, 4), dtype=np.int64)
+        ix = np.zeros(R, dtype=np.int32)
+        iy = np.zeros(R, dtype=np.int32)
+        H.check(_sig().mh_gcmc_get_state(self.h, _p(N), _p(e), _p(sN), _p(ns), _p(att), _p(acc), _p(ix),
+                                         _p(iy)), "mh_gcmc_get_state")
+        return dict(N=N, energy=e, sumN=sN, nsteps=ns, att=att, acc=acc, ix=ix, iy=iy)
+
+    def set_rng(self, rep, ix, iy):
+        H.check(_sig().mh_gcmc_set_rng(self.h, rep, int(ix), int(iy)), "mh_gcmc_set_rng")
+
+    def config(self, rep):
+        n = ctypes.c_int()
+        com = np.zeros((self.maxN, 3))
+        ang = np.zeros((self.maxN, 3))
+        r = np.zeros((self.maxN, self.ns, 3))
+        H.check(_sig().mh_gcmc_get_config(self.h, rep, ctypes.byref(n), _p(com), _p(ang), _p(r)),
+                "mh_gcmc_get_config")
+        k = n.value
+        return com[:k].copy(), ang[:k].copy(), r[:k].copy()
+
+    def total_energy(self, refresh=False):
+        e = np.zeros(self.nrep)
+        H.check(_sig().mh_gcmc_total_energy(self.h, _p(e), 1 if refresh else 0), "mh_gcmc_total_energy")
+        return e
+
+    def set_trace(self, cap):
+        H.check(_sig().mh_gcmc_set_trace(self.h, int(cap)), "mh_gcmc_set_trace")
+
+    def trace(self, rep):
+        n = ctypes.c_int()
+        _sig().mh_gcmc_get_trace(self.h, rep, None, None, None, None, ctypes.byref(n))
+        c = n.value
+        mv = np.zeros(c, dtype=np.int32)
+        ac = np.zeros(c, dtype=np.int32)
+        N = np.zeros(c, dtype=np.int32)
+        dE = np.zeros(c)
+        H.check(_sig().mh_gcmc_get_trace(self.h, rep, _p(mv), _p(ac), _p(N), _p(dE), ctypes.byref(n)),
+                "mh_gcmc_get_trace")
+        return np.stack([mv, ac, N, dE], 1)
+
+    def trial_energy(self, rep, com, ang=None):
+        com = np.ascontiguousarray(com, dtype=np.float64).reshape(-1, 3)
+        ang = np.zeros_like(com) if ang is None else np.ascontiguousarray(ang, dtype=np.float64).reshape(-1, 3)
+        n = len(com)
+        e = np.zeros(n)
+        ok = np.zeros(n, dtype=np.uint8)
+        H.check(_sig().mh_trial_energy(self.h, rep, n, _p(com), _p(ang), _p(e), _p(ok)), "mh_trial_energy")
+        return ok.astype(bool), e
+
+    def close(self):
+        if self.h:
+            _sig().mh_gcmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def adams_B(fug_kPa, volume, T, Zig=1.0):
+    """gcmc_genfuglist (gcmc.F90:303-319)."""
+    sivolume = volume * 1.0e-30 * C.NAV
+    return np.log(fug_kPa * 1.0e3 * sivolume / (C.RGAS * T)) - np.log(Zig)

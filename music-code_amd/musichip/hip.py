@@ -4,8 +4,10 @@ The library must be built (music-code_amd/Makefile, __graft_entry__.build()).
 There is no CPU fallback: every compute call goes to the HIP device and a
 missing library or device raises.
 """
+import atexit
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -20,6 +22,19 @@ _i = ctypes.c_int
 _d = ctypes.c_double
 _i64 = ctypes.c_int64
 _LIB = None
+# live handles, released children first at interpreter exit (before the HIP
+# runtime itself is torn down)
+_LIVE = {"gcmc": weakref.WeakSet(), "map": weakref.WeakSet(), "ctx": weakref.WeakSet()}
+
+
+@atexit.register
+def _release_all():
+    for kind in ("gcmc", "map", "ctx"):
+        for obj in list(_LIVE[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
 
 
 def real_space_cutoff_default(kappa):
@@ -86,6 +101,7 @@ class Map:
     def __init__(self, ctx, handle):
         self.ctx = ctx
         self.h = handle
+        _LIVE["map"].add(self)
         npn = np.zeros(3, dtype=np.int32)
         nf = ctypes.c_int64()
         counts = np.zeros(4, dtype=np.int64)
@@ -142,6 +158,7 @@ class Context:
         h = _vp()
         check(lib().mh_ctx_create(device, ctypes.byref(h)), "mh_ctx_create")
         self.h = h
+        _LIVE["ctx"].add(self)
 
     def set_timing(self, on=True):
         check(lib().mh_ctx_set_timing(self.h, 1 if on else 0), "mh_ctx_set_timing")
