@@ -89,6 +89,80 @@ def cpu_baseline(cfg, tabs, seconds=15.0):
                        "one thread, -O3 -march=native, %.1f s)" % (sample, total_t))
 
 
+def gcmc_leg(H, fp32_steps, fp64_steps, equil_steps, n_single=8):
+    """Batched isotherm: 64 CO2-like state points advanced together in FP32
+    pair arithmetic (FP64 accumulation) against the plain FP64 kernel run one
+    state point at a time on the same equilibrated configurations."""
+    from musichip import gcmc as G
+    from musichip import workloads as W
+    ctx = H.Context(0)
+    cs = W.co2_isotherm_system(ctx)
+    nrep = len(cs["fugacities"])
+    b = G.GcmcBatch(ctx, cs["spec"], cs["fugacities"], cs["seeds"], maxN=1024, maps=cs["maps"],
+                    precision=H.MH_PREC_FP32, threads=256)
+    t = time.perf_counter()
+    b.run(equil_steps)
+    log("gcmc: equilibrated %d steps x %d state points in %.2f s" % (equil_steps, nrep, time.perf_counter() - t))
+    s0 = b.state()
+    cfgs = [b.config(r) for r in range(nrep)]
+    t = time.perf_counter()
+    b.run(fp32_steps)
+    dt32 = time.perf_counter() - t
+    s1 = b.state()
+    att32 = int((s1["att"] - s0["att"]).sum())
+    rate32 = nrep * fp32_steps / dt32
+    log("gcmc FP32 batched: %.3e steps/s (%.3e attempted moves/s)" % (rate32, att32 / dt32))
+    # the same state points, FP64, one state point per launch
+    picks = np.linspace(0, nrep - 1, n_single).round().astype(int)
+    tot_t, tot_steps, tot_att = 0.0, 0, 0
+    for r in picks:
+        one = G.GcmcBatch(ctx, cs["spec"], cs["fugacities"][r:r + 1], cs["seeds"][r:r + 1], maxN=1024,
+                          maps=cs["maps"], precision=H.MH_PREC_FP64, threads=256)
+        one.set_config(0, cfgs[r][0], cfgs[r][1])
+        a0 = one.state()["att"].sum()
+        t = time.perf_counter()
+        one.run(fp64_steps)
+        tot_t += time.perf_counter() - t
+        tot_steps += fp64_steps
+        tot_att += int(one.state()["att"].sum() - a0)
+        one.close()
+    rate64 = tot_steps / tot_t
+    log("gcmc FP64 single state point: %.3e steps/s" % rate64)
+    # FP32 vs FP64 trial energies on the equilibrated configurations
+    rng = np.random.default_rng(11)
+    L = np.asarray(cs["spec"]["full_ell"])
+    worst = 0.0
+    b64 = G.GcmcBatch(ctx, cs["spec"], cs["fugacities"], cs["seeds"], maxN=1024, maps=cs["maps"],
+                      precision=H.MH_PREC_FP64)
+    for r in picks:
+        b64.set_config(r, cfgs[r][0], cfgs[r][1])
+        tc = rng.uniform(0, 1, (256, 3)) * L
+        ta = np.stack([rng.uniform(-np.pi, np.pi, 256), np.arccos(rng.uniform(-1, 1, 256)),
+                       rng.uniform(-np.pi, np.pi, 256)], 1)
+        ok64, e64 = b64.trial_energy(r, tc, ta)
+        ok32, e32 = b.trial_energy(r, tc, ta)
+        both = ok64 & ok32
+        if both.any():
+            worst = max(worst, float(np.max(np.abs(e32[both] - e64[both]) / np.maximum(np.abs(e64[both]), 1.0))))
+    b64.close()
+    out = dict(workload="CO2-like 3-site sorbate, 64 state points 1-1e4 kPa, 300 K, C1 framework "
+                        "(30 A, 768 atoms), pmaps C/O at 0.2 A, sorbate LJ 12.8 A + Ewald kappa 0.25/A "
+                        "kmax 7 (nk %d half-space), moves 1:1:1:1" % (((2 * 7 + 1) ** 3 - 1) // 2),
+               loadings=[int(x) for x in s1["N"]], fp32_batched_steps_per_s=rate32,
+               fp32_batched_attempted_moves_per_s=att32 / dt32,
+               fp64_single_state_point_steps_per_s=rate64,
+               fp64_single_state_point_attempted_moves_per_s=tot_att / tot_t,
+               speedup_batched_fp32_vs_fp64_single=rate32 / rate64,
+               fp32_vs_fp64_trial_energy_max_rel_err=worst,
+               steps=dict(equil=equil_steps, fp32=fp32_steps, fp64_per_state_point=fp64_steps,
+                          fp64_state_points=[int(x) for x in picks]))
+    b.close()
+    for m in cs["map_objs"]:
+        m.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +171,10 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--gcmc-steps", type=int, default=4000)
+    ap.add_argument("--gcmc-fp64-steps", type=int, default=1000)
+    ap.add_argument("--gcmc-equil", type=int, default=30000)
+    ap.add_argument("--no-gcmc", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the map build of one framework runs on one "
@@ -187,6 +265,10 @@ def main():
         for m in m64 + last32:
             m.close()
 
+    gcmc = None
+    if not args.no_gcmc:
+        gcmc = gcmc_leg(H, args.gcmc_steps, args.gcmc_fp64_steps, args.gcmc_equil)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, tabs, args.cpu_seconds)
@@ -200,7 +282,8 @@ def main():
                            nodes_per_map=nodes, maps_per_step=3, nk=nk, compaction_counts=counts),
                fp64_map_pts_per_s=fp64, fp32_vs_fp64=errs, kernel_ms_per_step={
                    k: v[0] / args.steps for k, v in ktimes.items()},
-               gcmc_trial_moves_per_s=None, p50_latency_ms=None, p99_latency_ms=None,
+               gcmc=gcmc, gcmc_trial_moves_per_s=gcmc and gcmc["fp32_batched_steps_per_s"],
+               p50_latency_ms=None, p99_latency_ms=None,
                roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(out))
 

@@ -6,9 +6,19 @@
 // copy of the replica's RNG state, N, running energy and counters, so the
 // reference's serial draw order (gcmc.F90:240-250, gcmcmoves.F90:270-364,
 // mcmoves.F90:290-647, rigidmoves.F90:691-1570) is reproduced without
-// communication; the energy of the trial is computed by all lanes in
-// parallel (maps: one lane per site; pairs: lanes over the other molecules;
-// Ewald: lanes over k vectors) and reduced in LDS in a fixed order.
+// communication; the energy of a trial is computed by all lanes in parallel
+// and reduced in a fixed order:
+//   * maps: one lane per (position set, site, cell corner), the 8 corner
+//     terms of the tricubic Hermite sum reduced by lane shuffles; the map is
+//     read from an uncompacted node-major copy (one HBM round trip);
+//   * pairs: lanes over (other molecule, its site, trial site) items;
+//   * Ewald reciprocal space: lane t owns the k vectors t, t + NT, ...; their
+//     running structure factor S(k), weight and the trial delta dS(k) stay in
+//     registers for the whole launch (KPL = k vectors per lane), the per-site
+//     e^{i m 2 pi x / L} tables are built in LDS by one lane per
+//     (set, site, axis) with a complex recurrence.
+// The replica's molecules (site positions, centres of mass, Euler angles) are
+// held in LDS for the launch when they fit (LDSMOL), else read from HBM.
 //
 // Layout in HBM (per replica r, stride by replica):
 //   com[r][maxN][3], ang[r][maxN][3], pos[r][maxN][ns][3]  (FP64)
@@ -29,8 +39,7 @@ constexpr double kTwoPiR = 2.0 * 3.1415927;  // defaults.F90:35
 constexpr double kTokcalG = 332.064;
 
 struct MapRef {
-    const int32_t *ptr;
-    const float *vals;
+    const float *dense;   // [nx][ny][nz][8], slot 0 NaN where untabulated
     int n[3];
     double step[3];
     double ell[3];
@@ -56,7 +65,11 @@ struct GDev {
     int mtype[4];
     double cumwt[4];
     double insdel, dtrans, drot, maxnrg, rti, molconst;
+    double inv_ns, inv_nss;  // 1/ns, 1/ns^2 for the pair-item decomposition
     int prec, maxN;
+    // e^{i (kx x + ky y)} table per (set, site) in dynamic LDS at byte
+    // offset exy_off: [2][ns][kmax + 1][2 kmax + 1] (double2, float2 in FP32)
+    int use_exy, exy_off;
 };
 
 struct Rep {
@@ -99,109 +112,179 @@ __device__ __forceinline__ double rranf(Rng &r) {
     return __dmul_rn(am, (double)v);
 }
 
-// rigidcoords_toxyz + matrix_b2feuler + simcell_pbc
+// simcell_pbc of one coordinate (simcell.F90:521-547)
+__device__ __forceinline__ double wrap_full(double x, double L) {
+    return __dsub_rn(x, __dmul_rn(floor(x / L), L));
+}
+
+// row i of matrix_b2feuler (matrix.F90:383-403) from sin/cos of phi, theta, psi
+__device__ __forceinline__ void euler_row(int i, double sp, double cp, double st, double ct, double ss,
+                                          double cs, double m[3]) {
+    if (i == 0) {
+        m[0] = __dsub_rn(__dmul_rn(cs, cp), __dmul_rn(__dmul_rn(ct, sp), ss));
+        m[1] = __dsub_rn(__dmul_rn(-ss, cp), __dmul_rn(__dmul_rn(ct, sp), cs));
+        m[2] = __dmul_rn(st, sp);
+    } else if (i == 1) {
+        m[0] = __dadd_rn(__dmul_rn(cs, sp), __dmul_rn(__dmul_rn(ct, cp), ss));
+        m[1] = __dadd_rn(__dmul_rn(-ss, sp), __dmul_rn(__dmul_rn(ct, cp), cs));
+        m[2] = __dmul_rn(-st, cp);
+    } else {
+        m[0] = __dmul_rn(ss, st);
+        m[1] = __dmul_rn(cs, st);
+        m[2] = ct;
+    }
+}
+
+// coordinate i of site a: rigidcoords_toxyz (rigidcoords.F90:254-276) + pbc
+__device__ __forceinline__ double site_coord(const GDev &P, int a, int i, const double com[3], const double m[3]) {
+    double d = 0.0;
+    d = __dadd_rn(d, __dmul_rn(m[0], P.ref[a][0]));
+    d = __dadd_rn(d, __dmul_rn(m[1], P.ref[a][1]));
+    d = __dadd_rn(d, __dmul_rn(m[2], P.ref[a][2]));
+    return wrap_full(__dadd_rn(com[i], d), P.full[i]);
+}
+
+// all site coordinates of one molecule by one lane (host-side paths)
 __device__ void site_coords(const GDev &P, const double com[3], const double ang[3], double *out) {
     if (P.ns == 1) {
-        for (int i = 0; i < 3; i++) out[i] = __dsub_rn(com[i], __dmul_rn(floor(com[i] / P.full[i]), P.full[i]));
+        for (int i = 0; i < 3; i++) out[i] = wrap_full(com[i], P.full[i]);
         return;
     }
     double sp, cp, st, ct, ss, cs;
     sincos(ang[0], &sp, &cp);
     sincos(ang[1], &st, &ct);
     sincos(ang[2], &ss, &cs);
-    double m[3][3];
-    m[0][0] = __dsub_rn(__dmul_rn(cs, cp), __dmul_rn(__dmul_rn(ct, sp), ss));
-    m[0][1] = __dsub_rn(__dmul_rn(-ss, cp), __dmul_rn(__dmul_rn(ct, sp), cs));
-    m[0][2] = __dmul_rn(st, sp);
-    m[1][0] = __dadd_rn(__dmul_rn(cs, sp), __dmul_rn(__dmul_rn(ct, cp), ss));
-    m[1][1] = __dadd_rn(__dmul_rn(-ss, sp), __dmul_rn(__dmul_rn(ct, cp), cs));
-    m[1][2] = __dmul_rn(-st, cp);
-    m[2][0] = __dmul_rn(ss, st);
-    m[2][1] = __dmul_rn(cs, st);
-    m[2][2] = ct;
-    for (int a = 0; a < P.ns; a++)
-        for (int i = 0; i < 3; i++) {
-            double d = 0.0;
-            d = __dadd_rn(d, __dmul_rn(m[i][0], P.ref[a][0]));
-            d = __dadd_rn(d, __dmul_rn(m[i][1], P.ref[a][1]));
-            d = __dadd_rn(d, __dmul_rn(m[i][2], P.ref[a][2]));
-            double rp = __dadd_rn(com[i], d);
-            out[3 * a + i] = __dsub_rn(rp, __dmul_rn(floor(rp / P.full[i]), P.full[i]));
-        }
+    for (int i = 0; i < 3; i++) {
+        double m[3];
+        euler_row(i, sp, cp, st, ct, ss, cs, m);
+        for (int a = 0; a < P.ns; a++) out[3 * a + i] = site_coord(P, a, i, com, m);
+    }
 }
 
-// map energy of one site (maps_int -> interpolate_int); returns false when
-// a corner is untabulated
+// simcell_maptouc + mapheader_getIndices along one axis (simcell.F90:634-668,
+// mapheader.F90:277-299): 0-based lower / upper node (upper wraps to 0 at the
+// last node, interpolate.F90:640-660) and the fractional offset t.
+__device__ __forceinline__ bool cell_axis(double c, double ell, double st, int nb, int &i0, int &i1, double &t) {
+    double f = c * (1.0 / ell);
+    double fr = f - trunc(f);  // == fmod(f, 1.0), exact
+    f = (f >= 0.0) ? fr : fr + 1.0;
+    c = f * ell;
+    const int idx = (int)(c / st + 1.0);
+    if (idx < 1 || idx > nb) return false;
+    t = (c - (double)(idx - 1) * st) / st;
+    i0 = idx - 1;
+    i1 = (idx == nb) ? 0 : idx;
+    return true;
+}
+
+// corner and value-slot patterns of pothermite (interpolate.F90:725-835):
+// bit i = upper node along the axis for corner i / derivative order for slot i
+constexpr int kCX = 0xB2, kCY = 0xD4, kCZ = 0xE8;
+
+// Hermite contribution of one cell corner: sum over the 8 stored values of
+// f[s] hx hy hz (FP64 products in the reference order, or FP32).
+template <int PREC>
+__device__ __forceinline__ bool corner_term(const MapRef &m, const double *r, int corner, double &part) {
+    int i0[3], i1[3];
+    double t[3];
+    for (int d = 0; d < 3; d++)
+        if (!cell_axis(r[d], m.ell[d], m.step[d], m.n[d], i0[d], i1[d], t[d])) return false;
+    const int cx = (kCX >> corner) & 1, cy = (kCY >> corner) & 1, cz = (kCZ >> corner) & 1;
+    const long long node = ((long long)(cx ? i1[0] : i0[0]) * m.n[1] + (cy ? i1[1] : i0[1])) * m.n[2] +
+                           (cz ? i1[2] : i0[2]);
+    const float4 *rec = reinterpret_cast<const float4 *>(m.dense + 8 * node);
+    const float4 a = rec[0], b = rec[1];
+    if (a.x != a.x) return false;  // untabulated node
+    const float f[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    if (PREC == MH_PREC_FP64) {
+        double hx[2], hy[2], hz[2];
+        hx[0] = cx ? h0f(t[0]) : h0f(1 - t[0]);
+        hx[1] = cx ? h1f(t[0], m.step[0]) : h1f(1 - t[0], -m.step[0]);
+        hy[0] = cy ? h0f(t[1]) : h0f(1 - t[1]);
+        hy[1] = cy ? h1f(t[1], m.step[1]) : h1f(1 - t[1], -m.step[1]);
+        hz[0] = cz ? h0f(t[2]) : h0f(1 - t[2]);
+        hz[1] = cz ? h1f(t[2], m.step[2]) : h1f(1 - t[2], -m.step[2]);
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            s = __dadd_rn(s, __dmul_rn(__dmul_rn(__dmul_rn((double)f[k], hx[(kCX >> k) & 1]), hy[(kCY >> k) & 1]),
+                                       hz[(kCZ >> k) & 1]));
+        part = s;
+    } else {
+#pragma clang fp contract(fast)
+        float hx[2], hy[2], hz[2];
+        const float tx = cx ? (float)t[0] : (float)(1 - t[0]);
+        const float ty = cy ? (float)t[1] : (float)(1 - t[1]);
+        const float tz = cz ? (float)t[2] : (float)(1 - t[2]);
+        hx[0] = tx * tx * (3.0f - 2.0f * tx);
+        hx[1] = (cx ? (float)m.step[0] : -(float)m.step[0]) * tx * tx * (tx - 1.0f);
+        hy[0] = ty * ty * (3.0f - 2.0f * ty);
+        hy[1] = (cy ? (float)m.step[1] : -(float)m.step[1]) * ty * ty * (ty - 1.0f);
+        hz[0] = tz * tz * (3.0f - 2.0f * tz);
+        hz[1] = (cz ? (float)m.step[2] : -(float)m.step[2]) * tz * tz * (tz - 1.0f);
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) s += f[k] * hx[(kCX >> k) & 1] * hy[(kCY >> k) & 1] * hz[(kCZ >> k) & 1];
+        part = s;
+    }
+    return true;
+}
+
+// map energy of one site by one lane (totals and trial energies off the hot
+// loop): the 8 corner terms in corner order
 __device__ bool site_map_energy(const GDev &P, int a, const double *r, double &e) {
     e = 0.0;
-    double p;
-    if (P.has_p[a]) {
-        const MapRef &m = P.pm[a];
-        if (!hermite_point(m.ptr, m.vals, m.n[0], m.n[1], m.n[2], m.step[0], m.step[1], m.step[2],
-                           m.ell[0], m.ell[1], m.ell[2], r[0], r[1], r[2], p))
-            return false;
-        e = __dadd_rn(e, p);
+    double pm = 0.0, em = 0.0;
+    for (int w = 0; w < 2; w++) {
+        if (!(w == 0 ? P.has_p[a] : P.has_e[a])) continue;
+        const MapRef &m = w == 0 ? P.pm[a] : P.em[a];
+        double s = 0.0;
+        for (int c = 0; c < 8; c++) {
+            double part;
+            if (!corner_term<MH_PREC_FP64>(m, r, c, part)) return false;
+            s = __dadd_rn(s, part);
+        }
+        (w == 0 ? pm : em) = s;
     }
-    if (P.has_e[a]) {
-        const MapRef &m = P.em[a];
-        if (!hermite_point(m.ptr, m.vals, m.n[0], m.n[1], m.n[2], m.step[0], m.step[1], m.step[2],
-                           m.ell[0], m.ell[1], m.ell[2], r[0], r[1], r[2], p))
-            return false;
-        e = __dadd_rn(e, __dmul_rn(p, P.q[a]));
-    }
+    if (P.has_p[a]) e = __dadd_rn(e, pm);
+    if (P.has_e[a]) e = __dadd_rn(e, __dmul_rn(em, P.q[a]));
     return true;
 }
 
 // pair energy of a site pair (lj_snglint without derivatives + Ewald real
 // space).  Returns false on an LJ locut violation.
-template <int PREC>
-__device__ __forceinline__ bool pair_energy(const GDev &P, int ta, int tb, double qa, double qb,
-                                            double sx, double sy, double sz, double &e) {
-    const int t = ta * P.nst + tb;
-    if (PREC == MH_PREC_FP64) {
-        double r2 = __dadd_rn(__dadd_rn(__dmul_rn(sx, sx), __dmul_rn(sy, sy)), __dmul_rn(sz, sz));
-        if (P.on[t] && !(r2 > P.h2[t])) {
-            if (r2 < P.l2[t]) return false;
-            double r2i = 1.0 / r2;
-            double r6i = __dmul_rn(__dmul_rn(r2i, r2i), r2i);
-            double r12i = __dmul_rn(r6i, r6i);
-            e = __dadd_rn(e, __dsub_rn(__dmul_rn(P.A[t], r12i), __dmul_rn(P.B[t], r6i)));
-        }
-        if (P.ewald && r2 < P.rc2 && qa != 0.0 && qb != 0.0) {
-            double rr = sqrt(r2);
-            e = __dadd_rn(e, __dmul_rn(__ddiv_rn(__dmul_rn(__dmul_rn(qa, qb), erfc_as64(P.kappa * rr)), rr), kTokcalG));
-        }
-    } else {
-#pragma clang fp contract(fast)
-        float fx = (float)sx, fy = (float)sy, fz = (float)sz;
-        float r2 = fx * fx + fy * fy + fz * fz;
-        if (P.on[t] && !(r2 > P.h2f[t])) {
-            if (r2 < P.l2f[t]) return false;
-            float r2i = __frcp_rn(r2);
-            float r6i = r2i * r2i * r2i;
-            e += (double)(P.Af[t] * (r6i * r6i) - P.Bf[t] * r6i);
-        }
-        if (P.ewald && r2 < (float)P.rc2 && qa != 0.0 && qb != 0.0) {
-            float ri = __frsqrt_rn(r2);
-            float rr = r2 * ri;
-            e += (double)((float)(qa * qb * kTokcalG) * erfc_as32((float)P.kappa * rr) * ri);
-        }
+__device__ __forceinline__ bool pair_energy64(const GDev &P, int t, double qa, double qb, double sx, double sy,
+                                              double sz, double &e) {
+    double r2 = __dadd_rn(__dadd_rn(__dmul_rn(sx, sx), __dmul_rn(sy, sy)), __dmul_rn(sz, sz));
+    if (P.on[t] && !(r2 > P.h2[t])) {
+        if (r2 < P.l2[t]) return false;
+        double r2i = 1.0 / r2;
+        double r6i = __dmul_rn(__dmul_rn(r2i, r2i), r2i);
+        double r12i = __dmul_rn(r6i, r6i);
+        e = __dadd_rn(e, __dsub_rn(__dmul_rn(P.A[t], r12i), __dmul_rn(P.B[t], r6i)));
+    }
+    if (P.ewald && r2 < P.rc2 && qa != 0.0 && qb != 0.0) {
+        double rr = sqrt(r2);
+        e = __dadd_rn(e, __dmul_rn(__ddiv_rn(__dmul_rn(__dmul_rn(qa, qb), erfc_as64(P.kappa * rr)), rr), kTokcalG));
     }
     return true;
 }
 
-// e^{i m 2pi x / L} for m = 0..kmax; the k vector of index (mx,my,mz) uses
-// E_x[mx] E_y[|my|] E_z[|mz|] with conjugates for negative m
-template <int PREC>
-__device__ __forceinline__ double2 expk(const double2 *Ex, const double2 *Ey, const double2 *Ez, int mx,
-                                        int my, int mz) {
-    double2 a = Ex[mx];
-    double2 b = Ey[my < 0 ? -my : my];
-    if (my < 0) b.y = -b.y;
-    double2 c = Ez[mz < 0 ? -mz : mz];
-    if (mz < 0) c.y = -c.y;
-    double2 ab = make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-    return make_double2(ab.x * c.x - ab.y * c.y, ab.x * c.y + ab.y * c.x);
+__device__ __forceinline__ bool pair_energy32(const GDev &P, int t, float qq, float kappa, float rc2, float sx,
+                                              float sy, float sz, float &e) {
+#pragma clang fp contract(fast)
+    const float r2 = sx * sx + sy * sy + sz * sz;
+    if (P.on[t] && !(r2 > P.h2f[t])) {
+        if (r2 < P.l2f[t]) return false;
+        const float r2i = __frcp_rn(r2);
+        const float r6i = r2i * r2i * r2i;
+        e += P.Af[t] * (r6i * r6i) - P.Bf[t] * r6i;
+    }
+    if (qq != 0.0f && r2 < rc2) {
+        const float ri = __frsqrt_rn(r2);
+        e += qq * erfc_as32(kappa * (r2 * ri)) * ri;
+    }
+    return true;
 }
 
 // block reduction of NV doubles per lane, fixed order (lane tree then waves)
@@ -224,103 +307,311 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *red /*[NT/64]
 
 struct Shared {
     double posA[GS * 3], posB[GS * 3];   // trial (A) and old (B) site coordinates
-    double comA[3], angA[3];
+    double sc[3][2];                     // sin, cos of the trial Euler angles
     double mapv[2][GS];
     int mapok[2][GS];
-    double2 E[2][GS][3][GKMAX + 1];      // per position set, site, axis: e^{i m theta}
+    double2 E[2][GS][3][GKMAX + 1];      // per position set, site, axis: e^{i m 2 pi x / L}
+    float2 Ef[2][GS][3][GKMAX + 1];      // FP32 copy
     double red[16 * 6];
 };
 
+// k vectors owned by one lane, kept in registers for the launch
+template <int KPL>
+struct KReg {
+    double2 S[KPL > 0 ? KPL : 1];
+    double2 d[KPL > 0 ? KPL : 1];
+    double w[KPL > 0 ? KPL : 1];
+    int m[KPL > 0 ? KPL : 1];  // packed (mx, my, mz) bytes
+};
+
+__device__ __forceinline__ int kpack(const int8_t *ki) {
+    return (int)(uint8_t)ki[0] | ((int)(uint8_t)ki[1] << 8) | ((int)(uint8_t)ki[2] << 16);
+}
+
+// k vector as (index into the e^{i(kx x + ky y)} table, |mz|, mz < 0)
+__device__ __forceinline__ int kpack_xy(const int8_t *ki, int kmax) {
+    const int mz = ki[2];
+    return (ki[0] * (2 * kmax + 1) + ki[1] + kmax) | ((mz < 0 ? -mz : mz) << 10) | ((mz < 0) << 15);
+}
+
+template <typename T2>
+__device__ __forceinline__ T2 expk(const T2 *Ex, const T2 *Ey, const T2 *Ez, int mx, int my, int mz) {
+    T2 a = Ex[mx];
+    T2 b = Ey[my < 0 ? -my : my];
+    if (my < 0) b.y = -b.y;
+    T2 c = Ez[mz < 0 ? -mz : mz];
+    if (mz < 0) c.y = -c.y;
+    T2 ab;
+    ab.x = a.x * b.x - a.y * b.y;
+    ab.y = a.x * b.y + a.y * b.x;
+    T2 o;
+    o.x = ab.x * c.x - ab.y * c.y;
+    o.y = ab.x * c.y + ab.y * c.x;
+    return o;
+}
+
+// delta of the structure factor at one k for the present position sets
+template <int PREC>
+__device__ __forceinline__ double2 k_delta(const GDev &P, const Shared &sh, int mx, int my, int mz, bool hasA,
+                                           bool hasB, double asign) {
+    if (PREC == MH_PREC_FP64) {
+        double2 d = make_double2(0.0, 0.0);
+        for (int a = 0; a < P.ns; a++) {
+            if (P.q[a] == 0.0) continue;
+            if (hasA) {
+                double2 e = expk<double2>(sh.E[0][a][0], sh.E[0][a][1], sh.E[0][a][2], mx, my, mz);
+                d.x += asign * P.q[a] * e.x;
+                d.y += asign * P.q[a] * e.y;
+            }
+            if (hasB) {
+                double2 e = expk<double2>(sh.E[1][a][0], sh.E[1][a][1], sh.E[1][a][2], mx, my, mz);
+                d.x -= P.q[a] * e.x;
+                d.y -= P.q[a] * e.y;
+            }
+        }
+        return d;
+    } else {
+#pragma clang fp contract(fast)
+        float dx = 0.0f, dy = 0.0f;
+        for (int a = 0; a < P.ns; a++) {
+            if (P.q[a] == 0.0) continue;
+            const float qa = (float)P.q[a];
+            if (hasA) {
+                float2 e = expk<float2>(sh.Ef[0][a][0], sh.Ef[0][a][1], sh.Ef[0][a][2], mx, my, mz);
+                dx += (float)asign * qa * e.x;
+                dy += (float)asign * qa * e.y;
+            }
+            if (hasB) {
+                float2 e = expk<float2>(sh.Ef[1][a][0], sh.Ef[1][a][1], sh.Ef[1][a][2], mx, my, mz);
+                dx -= qa * e.x;
+                dy -= qa * e.y;
+            }
+        }
+        return make_double2((double)dx, (double)dy);
+    }
+}
+
 // Energy terms of a move: set A (+) and set B (-) of one molecule against
-// every other molecule (skip = its slot), reciprocal delta with dS written to
-// dS[k].  modeA/modeB: 1 when the set is present.  rsign: +1 add A / -1 remove A.
-template <int NT, int PREC>
-__device__ void move_energy(const GDev &P, Shared &sh, const double *__restrict__ pos, int N, int skip,
-                            bool hasA, bool hasB, double asign, const double2 *__restrict__ S,
-                            double2 *__restrict__ dS, double out[6]) {
+// every other molecule (skip = its slot) and the reciprocal-space delta,
+// whose dS(k) is left in K.d (registers) or dSg (KPL == 0).
+// out: map A, pairs A, map B, pairs B, reciprocal delta, ok(A).
+template <int NT, int PREC, int KPL>
+__device__ __forceinline__ void move_energy(const GDev &P, Shared &sh, const double *__restrict__ pos, int N,
+                                            int skip, bool hasA, bool hasB, double asign, KReg<KPL> &K,
+                                            const double2 *__restrict__ Sg, double2 *__restrict__ dSg,
+                                            void *exy, double out[6]) {
     const int tid = threadIdx.x;
     const int ns = P.ns;
-    // maps (one lane per site and set)
-    if (tid < 2 * GS) {
-        int set = tid / GS, a = tid % GS;
-        if (a < ns && ((set == 0 && hasA) || (set == 1 && hasB))) {
-            double e;
-            bool ok = site_map_energy(P, a, set == 0 ? &sh.posA[3 * a] : &sh.posB[3 * a], e);
+    // maps: item l = (set * ns + site) * 8 + corner; the 8 corner lanes of a
+    // site are an aligned lane group
+    for (int l = tid; l < 16 * ns; l += NT) {
+        const int set = l / (8 * ns), a = (l >> 3) % ns, corner = l & 7;
+        const bool present = set == 0 ? hasA : hasB;
+        const double *r = set == 0 ? &sh.posA[3 * a] : &sh.posB[3 * a];
+        double pp = 0.0, pe = 0.0;
+        int ok = 1;
+        if (present) {
+            if (P.has_p[a] && !corner_term<PREC>(P.pm[a], r, corner, pp)) ok = 0;
+            if (P.has_e[a] && !corner_term<PREC>(P.em[a], r, corner, pe)) ok = 0;
+        }
+        for (int off = 1; off < 8; off <<= 1) {
+            pp += __shfl_xor(pp, off);
+            pe += __shfl_xor(pe, off);
+            ok &= __shfl_xor(ok, off);
+        }
+        if (corner == 0 && present) {
+            double e = 0.0;
+            if (P.has_p[a]) e = __dadd_rn(e, pp);
+            if (P.has_e[a]) e = __dadd_rn(e, __dmul_rn(pe, P.q[a]));
             sh.mapv[set][a] = e;
-            sh.mapok[set][a] = ok ? 1 : 0;
+            sh.mapok[set][a] = ok;
         }
     }
-    // e^{i m theta} tables for the Ewald delta
+    // e^{i m 2 pi x / L} tables: one lane per (set, site, axis), recurrence in m
     if (P.ewald) {
-        const int ntab = 2 * GS * 3 * (P.kmax + 1);
-        for (int i = tid; i < ntab; i += NT) {
-            int m = i % (P.kmax + 1);
-            int ax = (i / (P.kmax + 1)) % 3;
-            int a = (i / (3 * (P.kmax + 1))) % GS;
-            int set = i / (GS * 3 * (P.kmax + 1));
-            if (a >= ns) continue;
-            const double *pp = set == 0 ? sh.posA : sh.posB;
-            double th = ((double)m / P.full[ax]) * kTwoPiR * pp[3 * a + ax];
-            double s_, c_;
-            if (PREC == MH_PREC_FP64) sincos(th, &s_, &c_);
-            else {
-                float sf, cf;
-                __sincosf((float)th, &sf, &cf);
-                s_ = sf;
-                c_ = cf;
+        const int e0 = NT >= 256 ? 128 : (NT == 128 ? 64 : 0);
+        const int l = tid - e0;
+        if (l >= 0 && l < 6 * ns) {
+            const int set = l / (3 * ns), a = (l / 3) % ns, ax = l % 3;
+            const bool present = set == 0 ? hasA : hasB;
+            if (present && P.q[a] != 0.0) {
+                const double *pp = set == 0 ? sh.posA : sh.posB;
+                const double th = (1.0 / P.full[ax]) * kTwoPiR * pp[3 * a + ax];
+                double s1, c1;
+                sincos(th, &s1, &c1);
+                double cr = 1.0, ci = 0.0;
+                for (int m = 0; m <= P.kmax; m++) {
+                    sh.E[set][a][ax][m] = make_double2(cr, ci);
+                    sh.Ef[set][a][ax][m] = make_float2((float)cr, (float)ci);
+                    const double nr = cr * c1 - ci * s1, ni = cr * s1 + ci * c1;
+                    cr = nr;
+                    ci = ni;
+                }
             }
-            sh.E[set][a][ax][m] = make_double2(c_, s_);
         }
     }
     __syncthreads();
-    // pairs: lanes over the other molecules
+    if (KPL > 0 && P.ewald && P.use_exy) {  // the table lives in the persistent kernel's LDS
+        const int W = 2 * P.kmax + 1, per = (P.kmax + 1) * W;
+        for (int i = tid; i < 2 * ns * per; i += NT) {
+            const int sa = i / per, r = i - sa * per;
+            const int set = sa / ns, a = sa - set * ns;
+            const int mx = r / W, my = r - mx * W - P.kmax;
+            if (!(set == 0 ? hasA : hasB) || P.q[a] == 0.0) continue;
+            const double2 ex = sh.E[set][a][0][mx];
+            double2 ey = sh.E[set][a][1][my < 0 ? -my : my];
+            if (my < 0) ey.y = -ey.y;
+            const double2 o = make_double2(ex.x * ey.x - ex.y * ey.y, ex.x * ey.y + ex.y * ey.x);
+            if (PREC == MH_PREC_FP64) reinterpret_cast<double2 *>(exy)[i] = o;
+            else reinterpret_cast<float2 *>(exy)[i] = make_float2((float)o.x, (float)o.y);
+        }
+        __syncthreads();
+    }
+    // pairs: items (other molecule j, its site b, trial site a)
     double eA = 0.0, eB = 0.0, failA = 0.0;
-    for (int j = tid; j < N; j += NT) {
+    const int nss = ns * ns;
+    const int items = N * nss;
+    float eAf = 0.0f, eBf = 0.0f;
+    const float kap32 = (float)P.kappa, rc32 = (float)P.rc2;
+    for (int w = tid; w < items; w += NT) {
+        const int j = (int)(((double)w + 0.5) * P.inv_nss);
         if (j == skip) continue;
-        const double *pj = pos + (long long)j * ns * 3;
-        for (int b = 0; b < ns; b++) {
-            const double xb = pj[3 * b], yb = pj[3 * b + 1], zb = pj[3 * b + 2];
-            const int tb = P.st[b];
-            const double qb = P.q[b];
-            for (int a = 0; a < ns; a++) {
-                if (hasA) {
-                    double sx = fold64(sh.posA[3 * a] - xb, P.full[0]);
-                    double sy = fold64(sh.posA[3 * a + 1] - yb, P.full[1]);
-                    double sz = fold64(sh.posA[3 * a + 2] - zb, P.full[2]);
-                    if (!pair_energy<PREC>(P, P.st[a], tb, P.q[a], qb, sx, sy, sz, eA)) failA = 1.0;
-                }
-                if (hasB) {
-                    double sx = fold64(sh.posB[3 * a] - xb, P.full[0]);
-                    double sy = fold64(sh.posB[3 * a + 1] - yb, P.full[1]);
-                    double sz = fold64(sh.posB[3 * a + 2] - zb, P.full[2]);
-                    pair_energy<PREC>(P, P.st[a], tb, P.q[a], qb, sx, sy, sz, eB);
-                }
+        const int r = w - j * nss;
+        const int b = (int)(((double)r + 0.5) * P.inv_ns);
+        const int a = r - b * ns;
+        const double *pj = pos + ((long long)j * ns + b) * 3;
+        const double xb = pj[0], yb = pj[1], zb = pj[2];
+        const int t = P.st[a] * P.nst + P.st[b];
+        if (PREC == MH_PREC_FP64) {
+            if (hasA) {
+                double sx = fold64(sh.posA[3 * a] - xb, P.full[0]);
+                double sy = fold64(sh.posA[3 * a + 1] - yb, P.full[1]);
+                double sz = fold64(sh.posA[3 * a + 2] - zb, P.full[2]);
+                if (!pair_energy64(P, t, P.q[a], P.q[b], sx, sy, sz, eA)) failA = 1.0;
+            }
+            if (hasB) {
+                double sx = fold64(sh.posB[3 * a] - xb, P.full[0]);
+                double sy = fold64(sh.posB[3 * a + 1] - yb, P.full[1]);
+                double sz = fold64(sh.posB[3 * a + 2] - zb, P.full[2]);
+                pair_energy64(P, t, P.q[a], P.q[b], sx, sy, sz, eB);
+            }
+        } else {
+            const float qq = P.ewald ? (float)(P.q[a] * P.q[b] * kTokcalG) : 0.0f;
+            if (hasA) {
+                float sx = (float)fold64(sh.posA[3 * a] - xb, P.full[0]);
+                float sy = (float)fold64(sh.posA[3 * a + 1] - yb, P.full[1]);
+                float sz = (float)fold64(sh.posA[3 * a + 2] - zb, P.full[2]);
+                if (!pair_energy32(P, t, qq, kap32, rc32, sx, sy, sz, eAf)) failA = 1.0;
+            }
+            if (hasB) {
+                float sx = (float)fold64(sh.posB[3 * a] - xb, P.full[0]);
+                float sy = (float)fold64(sh.posB[3 * a + 1] - yb, P.full[1]);
+                float sz = (float)fold64(sh.posB[3 * a + 2] - zb, P.full[2]);
+                pair_energy32(P, t, qq, kap32, rc32, sx, sy, sz, eBf);
             }
         }
     }
-    // reciprocal delta: lanes over k
+    if (PREC != MH_PREC_FP64) {
+        eA = (double)eAf;
+        eB = (double)eBf;
+    }
+    // reciprocal-space delta over the lane's k vectors
     double eK = 0.0;
     if (P.ewald) {
-        for (int k = tid; k < P.nk; k += NT) {
-            const int8_t *ki = P.kidx + 4 * k;
-            double2 d = make_double2(0.0, 0.0);
-            for (int a = 0; a < ns; a++) {
-                if (P.q[a] == 0.0) continue;
-                if (hasA) {
-                    double2 e = expk<PREC>(sh.E[0][a][0], sh.E[0][a][1], sh.E[0][a][2], ki[0], ki[1], ki[2]);
-                    d.x += asign * P.q[a] * e.x;
-                    d.y += asign * P.q[a] * e.y;
+        if constexpr (KPL > 0) {
+          if (P.use_exy) {
+            // site-major: the table reads of all KPL k vectors issue together
+            const int W = 2 * P.kmax + 1, per = (P.kmax + 1) * W;
+            if (PREC == MH_PREC_FP64) {
+                double2 dd[KPL];
+#pragma unroll
+                for (int j = 0; j < KPL; j++) dd[j] = make_double2(0.0, 0.0);
+                for (int a = 0; a < ns; a++) {
+                    if (P.q[a] == 0.0) continue;
+                    for (int set = 0; set < 2; set++) {
+                        if (!(set == 0 ? hasA : hasB)) continue;
+                        const double sc = set == 0 ? asign * P.q[a] : -P.q[a];
+                        const double2 *xy = reinterpret_cast<const double2 *>(exy) + (set * ns + a) * per;
+                        const double2 *ez = sh.E[set][a][2];
+#pragma unroll
+                        for (int j = 0; j < KPL; j++) {
+                            const int pk = K.m[j];
+                            const double2 u = xy[pk & 1023];
+                            double2 v = ez[(pk >> 10) & 31];
+                            if (pk & 0x8000) v.y = -v.y;
+                            const double ex = u.x * v.x - u.y * v.y, ey = u.x * v.y + u.y * v.x;
+                            dd[j].x += sc * ex;
+                            dd[j].y += sc * ey;
+                        }
+                    }
                 }
-                if (hasB) {
-                    double2 e = expk<PREC>(sh.E[1][a][0], sh.E[1][a][1], sh.E[1][a][2], ki[0], ki[1], ki[2]);
-                    d.x -= P.q[a] * e.x;
-                    d.y -= P.q[a] * e.y;
+#pragma unroll
+                for (int j = 0; j < KPL; j++) {
+                    const double2 s = K.S[j], d = dd[j];
+                    const double nre = s.x + d.x, nim = s.y + d.y;
+                    eK += K.w[j] * ((nre * nre + nim * nim) - (s.x * s.x + s.y * s.y));
+                    K.d[j] = d;
+                }
+            } else {
+#pragma clang fp contract(fast)
+                float2 dd[KPL];
+#pragma unroll
+                for (int j = 0; j < KPL; j++) dd[j] = make_float2(0.0f, 0.0f);
+                for (int a = 0; a < ns; a++) {
+                    if (P.q[a] == 0.0) continue;
+                    for (int set = 0; set < 2; set++) {
+                        if (!(set == 0 ? hasA : hasB)) continue;
+                        const float sc = (float)(set == 0 ? asign * P.q[a] : -P.q[a]);
+                        const float2 *xy = reinterpret_cast<const float2 *>(exy) + (set * ns + a) * per;
+                        const float2 *ez = sh.Ef[set][a][2];
+#pragma unroll
+                        for (int j = 0; j < KPL; j++) {
+                            const int pk = K.m[j];
+                            const float2 u = xy[pk & 1023];
+                            float2 v = ez[(pk >> 10) & 31];
+                            if (pk & 0x8000) v.y = -v.y;
+                            dd[j].x += sc * (u.x * v.x - u.y * v.y);
+                            dd[j].y += sc * (u.x * v.y + u.y * v.x);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < KPL; j++) {
+                    const double2 s = K.S[j];
+                    const double dx = dd[j].x, dy = dd[j].y;
+                    eK += K.w[j] * (2.0 * (s.x * dx + s.y * dy) + (dx * dx + dy * dy));
+                    K.d[j] = make_double2(dx, dy);
                 }
             }
-            double2 s = S[k];
-            double nre = s.x + d.x, nim = s.y + d.y;
-            eK += P.wk[k] * ((nre * nre + nim * nim) - (s.x * s.x + s.y * s.y));
-            dS[k] = d;
+          } else {
+#pragma unroll
+            for (int j = 0; j < KPL; j++) {
+                const int k = tid + j * NT;
+                if (k < P.nk) {
+                    const int pk = K.m[j];
+                    const int mx = (int8_t)(pk & 0xff), my = (int8_t)((pk >> 8) & 0xff),
+                              mz = (int8_t)((pk >> 16) & 0xff);
+                    const double2 d = k_delta<PREC>(P, sh, mx, my, mz, hasA, hasB, asign);
+                    const double2 s = K.S[j];
+                    if (PREC == MH_PREC_FP64) {
+                        const double nre = s.x + d.x, nim = s.y + d.y;
+                        eK += K.w[j] * ((nre * nre + nim * nim) - (s.x * s.x + s.y * s.y));
+                    } else {
+                        eK += K.w[j] * (2.0 * (s.x * d.x + s.y * d.y) + (d.x * d.x + d.y * d.y));
+                    }
+                    K.d[j] = d;
+                }
+            }
+          }
+        } else {
+            for (int k = tid; k < P.nk; k += NT) {
+                const int8_t *ki = P.kidx + 4 * k;
+                const double2 d = k_delta<PREC>(P, sh, ki[0], ki[1], ki[2], hasA, hasB, asign);
+                const double2 s = Sg[k];
+                const double nre = s.x + d.x, nim = s.y + d.y;
+                eK += P.wk[k] * ((nre * nre + nim * nim) - (s.x * s.x + s.y * s.y));
+                dSg[k] = d;
+            }
         }
     }
     double v[4] = {eA, eB, eK, failA};
@@ -342,8 +633,32 @@ __device__ void move_energy(const GDev &P, Shared &sh, const double *__restrict_
     out[5] = (okA && v[3] == 0.0) ? 1.0 : 0.0;
 }
 
-template <int NT, int PREC>
+// S(k) += dS(k) for the lane's k vectors after an accepted move
+template <int NT, int KPL>
+__device__ __forceinline__ void sk_accept(const GDev &P, KReg<KPL> &K, double2 *__restrict__ Sg,
+                                          const double2 *__restrict__ dSg) {
+    const int tid = threadIdx.x;
+    if constexpr (KPL > 0) {
+#pragma unroll
+        for (int j = 0; j < KPL; j++) {
+            if (tid + j * NT < P.nk) {
+                K.S[j].x += K.d[j].x;
+                K.S[j].y += K.d[j].y;
+            }
+        }
+    } else {
+        for (int k = tid; k < P.nk; k += NT) {
+            double2 s = Sg[k], d = dSg[k];
+            Sg[k] = make_double2(s.x + d.x, s.y + d.y);
+        }
+    }
+}
+
+// The persistent GCMC loop.  LDSMOL: molecules held in dynamic LDS
+// ([maxN][ns][3] sites, [maxN][3] com, [maxN][3] angles, FP64).
+template <int NT, int PREC, int KPL, bool LDSMOL>
 __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Pg, Rep R, int nsteps) {
+    extern __shared__ double dyn[];
     __shared__ Shared sh;
     __shared__ GDev P;
     const int rep = blockIdx.x;
@@ -363,11 +678,46 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Pg, Rep R,
         acc[i] = R.acc[4 * rep + i];
     }
     const double B = R.B[rep];
-    double *com = R.com + (long long)rep * P.maxN * 3;
-    double *ang = R.ang + (long long)rep * P.maxN * 3;
-    double *pos = R.pos + (long long)rep * P.maxN * ns * 3;
+    double *gcom = R.com + (long long)rep * P.maxN * 3;
+    double *gang = R.ang + (long long)rep * P.maxN * 3;
+    double *gpos = R.pos + (long long)rep * P.maxN * ns * 3;
     double2 *S = R.S + (long long)rep * P.nk;
     double2 *dS = R.dS + (long long)rep * P.nk;
+    double *pos, *com, *ang;
+    void *exy = reinterpret_cast<char *>(dyn) + P.exy_off;
+    if constexpr (LDSMOL) {
+        pos = dyn;
+        com = dyn + (long long)P.maxN * ns * 3;
+        ang = com + (long long)P.maxN * 3;
+        for (int i = tid; i < N * ns * 3; i += NT) pos[i] = gpos[i];
+        for (int i = tid; i < N * 3; i += NT) {
+            com[i] = gcom[i];
+            ang[i] = gang[i];
+        }
+    } else {
+        pos = gpos;
+        com = gcom;
+        ang = gang;
+    }
+    KReg<KPL> K;
+    if constexpr (KPL > 0) {
+#pragma unroll
+        for (int j = 0; j < KPL; j++) {
+            const int k = tid + j * NT;
+            if (k < P.nk) {
+                K.S[j] = S[k];
+                K.w[j] = P.wk[k];
+                K.m[j] = P.use_exy ? kpack_xy(P.kidx + 4 * k, P.kmax) : kpack(P.kidx + 4 * k);
+            } else {
+                // padding k: weight 0, table entry (0, 0, 0)
+                K.S[j] = make_double2(0.0, 0.0);
+                K.w[j] = 0.0;
+                K.m[j] = P.use_exy ? P.kmax : 0;
+            }
+            K.d[j] = make_double2(0.0, 0.0);
+        }
+    }
+    __syncthreads();
     int err = 0;
     for (int it = 0; it < nsteps; it++) {
         (void)rranf(rng);  // species (nspc = 1), gcmc.F90:242
@@ -433,21 +783,36 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Pg, Rep R,
         if (doit) {
             const bool hasA = (mtype != MH_MV_DELETE);
             const bool hasB = (mtype != MH_MV_INSERT);
-            if (tid == 0) {
-                if (hasA) site_coords(P, tcom, tang, sh.posA);
-                if (hasB)
-                    for (int i = 0; i < 3 * ns; i++) sh.posB[i] = pos[(long long)mol * ns * 3 + i];
+            // ---- site coordinates: set B = the molecule as stored; set A =
+            // the trial (or, for a deletion, the stored molecule)
+            const int n3 = 3 * ns;
+            if (hasB && tid >= 64 - n3 && tid < 64) sh.posB[tid - (64 - n3)] = pos[(long long)mol * n3 + tid - (64 - n3)];
+            if (mtype == MH_MV_DELETE) {
+                if (tid < n3) sh.posA[tid] = pos[(long long)mol * n3 + tid];
+            } else if (ns == 1) {
+                if (tid < 3) sh.posA[tid] = wrap_full(tcom[tid], P.full[tid]);
+            } else {
+                if (tid < 3) {
+                    double s_, c_;
+                    sincos(tang[tid], &s_, &c_);
+                    sh.sc[tid][0] = s_;
+                    sh.sc[tid][1] = c_;
+                }
+                __syncthreads();
+                if (tid < n3) {
+                    const int a = tid / 3, i = tid - 3 * a;
+                    double m[3];
+                    euler_row(i, sh.sc[0][0], sh.sc[0][1], sh.sc[1][0], sh.sc[1][1], sh.sc[2][0], sh.sc[2][1], m);
+                    sh.posA[tid] = site_coord(P, a, i, tcom, m);
+                }
             }
             __syncthreads();
             double o[6];
             if (mtype == MH_MV_DELETE) {
                 // the molecule's energy in the current system: its sites as set A
-                if (tid == 0)
-                    for (int i = 0; i < 3 * ns; i++) sh.posA[i] = sh.posB[i];
-                __syncthreads();
-                move_energy<NT, PREC>(P, sh, pos, N, mol, true, false, -1.0, S, dS, o);
+                move_energy<NT, PREC, KPL>(P, sh, pos, N, mol, true, false, -1.0, K, S, dS, exy, o);
             } else {
-                move_energy<NT, PREC>(P, sh, pos, N, mol, hasA, hasB, 1.0, S, dS, o);
+                move_energy<NT, PREC, KPL>(P, sh, pos, N, mol, hasA, hasB, 1.0, K, S, dS, exy, o);
             }
             const double mA = o[0], pA = o[1], mB = o[2], pB = o[3], eK = o[4];
             bool ok = o[5] != 0.0;
@@ -464,18 +829,12 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Pg, Rep R,
                     double boltz = exp(u) * (1.0 / (double)(float)(N + 1));
                     accepted = boltz > rranf(rng);
                     if (accepted) {
-                        if (tid == 0) {
-                            for (int i = 0; i < 3; i++) {
-                                com[3 * N + i] = tcom[i];
-                                ang[3 * N + i] = tang[i];
-                            }
-                            for (int i = 0; i < 3 * ns; i++) pos[(long long)N * ns * 3 + i] = sh.posA[i];
+                        if (tid < 3) {
+                            com[3 * N + tid] = tcom[tid];
+                            ang[3 * N + tid] = tang[tid];
                         }
-                        if (P.ewald)
-                            for (int k = tid; k < P.nk; k += NT) {
-                                double2 s = S[k], d = dS[k];
-                                S[k] = make_double2(s.x + d.x, s.y + d.y);
-                            }
+                        if (tid < n3) pos[(long long)N * n3 + tid] = sh.posA[tid];
+                        if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                         N++;
                         energy += U;
                         acc[0]++;
@@ -498,20 +857,13 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Pg, Rep R,
                 accepted = boltz > rranf(rng);
                 if (accepted) {
                     const int last = N - 1;
-                    __syncthreads();  // all lanes done reading pos before the move
-                    if (tid == 0) {
-                        for (int i = 0; i < 3; i++) {
-                            com[3 * mol + i] = com[3 * last + i];
-                            ang[3 * mol + i] = ang[3 * last + i];
-                        }
-                        for (int i = 0; i < 3 * ns; i++)
-                            pos[(long long)mol * ns * 3 + i] = pos[(long long)last * ns * 3 + i];
+                    // every lane finished reading the molecules (block_sum barrier)
+                    if (tid < 3) {
+                        com[3 * mol + tid] = com[3 * last + tid];
+                        ang[3 * mol + tid] = ang[3 * last + tid];
                     }
-                    if (P.ewald)
-                        for (int k = tid; k < P.nk; k += NT) {
-                            double2 s = S[k], d = dS[k];
-                            S[k] = make_double2(s.x + d.x, s.y + d.y);
-                        }
+                    if (tid < n3) pos[(long long)mol * n3 + tid] = pos[(long long)last * n3 + tid];
+                    if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                     N--;
                     energy -= U;
                     acc[1]++;
@@ -529,18 +881,12 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Pg, Rep R,
                     double boltz = exp(u);
                     accepted = boltz > rranf(rng);
                     if (accepted) {
-                        if (tid == 0) {
-                            for (int i = 0; i < 3; i++) {
-                                com[3 * mol + i] = tcom[i];
-                                ang[3 * mol + i] = tang[i];
-                            }
-                            for (int i = 0; i < 3 * ns; i++) pos[(long long)mol * ns * 3 + i] = sh.posA[i];
+                        if (tid < 3) {
+                            com[3 * mol + tid] = tcom[tid];
+                            ang[3 * mol + tid] = tang[tid];
                         }
-                        if (P.ewald)
-                            for (int k = tid; k < P.nk; k += NT) {
-                                double2 s = S[k], d = dS[k];
-                                S[k] = make_double2(s.x + d.x, s.y + d.y);
-                            }
+                        if (tid < n3) pos[(long long)mol * n3 + tid] = sh.posA[tid];
+                        if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                         energy += dU;
                         acc[mtype]++;
                         de = dU;
@@ -561,6 +907,21 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Pg, Rep R,
         sumN += N;
         nst++;
         __syncthreads();
+    }
+    // write the replica back
+    if constexpr (LDSMOL) {
+        for (int i = tid; i < N * ns * 3; i += NT) gpos[i] = pos[i];
+        for (int i = tid; i < N * 3; i += NT) {
+            gcom[i] = com[i];
+            gang[i] = ang[i];
+        }
+    }
+    if constexpr (KPL > 0) {
+#pragma unroll
+        for (int j = 0; j < KPL; j++) {
+            const int k = tid + j * NT;
+            if (k < P.nk) S[k] = K.S[j];
+        }
     }
     if (tid == 0) {
         R.err[rep] = err;
@@ -599,7 +960,8 @@ __global__ void __launch_bounds__(NT) k_trial(const GDev *__restrict__ Pg, Rep R
     if (tid == 0) site_coords(P, tcom + 3 * t, tang + 3 * t, sh.posA);
     __syncthreads();
     double o[6];
-    move_energy<NT, PREC>(P, sh, pos, N, -1, true, false, 1.0, S, dSbuf + (long long)t * P.nk, o);
+    KReg<0> K;
+    move_energy<NT, PREC, 0>(P, sh, pos, N, -1, true, false, 1.0, K, S, dSbuf + (long long)t * P.nk, nullptr, o);
     if (tid == 0) {
         double U = __dadd_rn(o[0], o[1]);
         if (P.ewald) U = U + o[4] + P.molconst;
@@ -662,7 +1024,7 @@ __global__ void __launch_bounds__(NT) k_total(const GDev *__restrict__ Pg, Rep R
                     double sx = fold64(pm[3 * a] - pj[3 * b], P.full[0]);
                     double sy = fold64(pm[3 * a + 1] - pj[3 * b + 1], P.full[1]);
                     double sz = fold64(pm[3 * a + 2] - pj[3 * b + 2], P.full[2]);
-                    if (!pair_energy<MH_PREC_FP64>(P, P.st[a], P.st[b], P.q[a], P.q[b], sx, sy, sz, e)) bad = 1.0;
+                    if (!pair_energy64(P, P.st[a] * P.nst + P.st[b], P.q[a], P.q[b], sx, sy, sz, e)) bad = 1.0;
                 }
         }
     }
@@ -676,6 +1038,24 @@ __global__ void __launch_bounds__(NT) k_total(const GDev *__restrict__ Pg, Rep R
     if (tid == 0) out[rep] = (v[1] != 0.0) ? NAN : v[0] + (P.ewald ? N * P.molconst : 0.0);
 }
 
+// uncompacted copy of a map: out[n][8] = vals[ptr[n] - 1][8], V = NaN where
+// ptr[n] == 0
+__global__ void k_densify(const int32_t *__restrict__ ptr, const float *__restrict__ vals, long long nodes,
+                          float *__restrict__ out) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nodes) return;
+    const int p = ptr[n];
+    float4 a = make_float4(NAN, 0.0f, 0.0f, 0.0f), b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (p > 0) {
+        const float4 *r = reinterpret_cast<const float4 *>(vals + 8 * (long long)(p - 1));
+        a = r[0];
+        b = r[1];
+    }
+    float4 *o = reinterpret_cast<float4 *>(out + 8 * n);
+    o[0] = a;
+    o[1] = b;
+}
+
 }  // namespace mh
 
 using namespace mh;
@@ -683,6 +1063,9 @@ using namespace mh;
 struct mh_gcmc {
     mh_ctx *ctx = nullptr;
     int nrep = 0, maxN = 0, ns = 0, nk = 0, prec = 0, threads = 256;
+    int kpl = 0;            // k vectors per lane held in registers (0: S(k) in HBM)
+    bool ldsmol = false;    // molecules held in LDS during a launch
+    size_t lds_bytes = 0;   // dynamic LDS of the launch
     GDev host;
     DBuf dP, wk, kidx;
     DBuf N, err, ix, iy, first, iseed, B, energy, sumN, nsteps, att, acc;
@@ -726,18 +1109,99 @@ static double erfc_as_host(double x) {
     return t * (a1 + t * (a2 + t * (a3 + t * (a4 + t * a5)))) * std::exp(-x * x);
 }
 
+// register tiles of k vectors per lane compiled for each workgroup size
+// (larger tiles spill at that size); 0 = S(k) kept in HBM
+static bool kpl_allowed(int threads, int k) {
+    switch (threads) {
+    case 64: return k == 8 || k == 16;
+    case 128: return k == 4 || k == 8 || k == 16;
+    case 256: return k == 2 || k == 4 || k == 8 || k == 16;
+    case 512: return k == 2 || k == 4;
+    default: return false;
+    }
+}
+
+// smallest register tile of k vectors per lane that holds nk at NT lanes
+static int pick_kpl(int nk, int threads) {
+    if (nk == 0) return 0;
+    const int need = (nk + threads - 1) / threads;
+    for (int k : {2, 4, 8, 16})
+        if (need <= k && kpl_allowed(threads, k)) return k;
+    return 0;
+}
+
+template <int PREC, int NT, int KPL>
+static void launch_lds(mh_gcmc *g, const GDev *P, const Rep &r, int nsteps) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_gcmc<NT, PREC, KPL, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)g->lds_bytes);
+    hipLaunchKernelGGL((k_gcmc<NT, PREC, KPL, true>), dim3(g->nrep), dim3(NT), g->lds_bytes, g->ctx->stream, P, r,
+                       nsteps);
+}
+
+template <int PREC, int NT>
+static void launch_nt(mh_gcmc *g, const GDev *P, const Rep &r, int nsteps) {
+    if (!g->ldsmol) {
+        hipLaunchKernelGGL((k_gcmc<NT, PREC, 0, false>), dim3(g->nrep), dim3(NT), 0, g->ctx->stream, P, r, nsteps);
+        return;
+    }
+    // only the tiles kpl_allowed() admits for NT are instantiated
+    constexpr bool k2 = NT == 256 || NT == 512, k4 = NT >= 128 && NT <= 512, k8 = NT <= 256, k16 = NT <= 256;
+    switch (g->kpl) {
+    case 2: if constexpr (k2) launch_lds<PREC, NT, 2>(g, P, r, nsteps); break;
+    case 4: if constexpr (k4) launch_lds<PREC, NT, 4>(g, P, r, nsteps); break;
+    case 8: if constexpr (k8) launch_lds<PREC, NT, 8>(g, P, r, nsteps); break;
+    case 16: if constexpr (k16) launch_lds<PREC, NT, 16>(g, P, r, nsteps); break;
+    default: launch_lds<PREC, NT, 0>(g, P, r, nsteps); break;
+    }
+}
+
 template <int PREC>
 static void launch_gcmc(mh_gcmc *g, int nsteps) {
     Rep r = g->rep();
     const GDev *P = g->dP.as<GDev>();
-    dim3 grid(g->nrep);
     switch (g->threads) {
-    case 64: hipLaunchKernelGGL((k_gcmc<64, PREC>), grid, dim3(64), 0, g->ctx->stream, P, r, nsteps); break;
-    case 128: hipLaunchKernelGGL((k_gcmc<128, PREC>), grid, dim3(128), 0, g->ctx->stream, P, r, nsteps); break;
-    case 512: hipLaunchKernelGGL((k_gcmc<512, PREC>), grid, dim3(512), 0, g->ctx->stream, P, r, nsteps); break;
-    case 1024: hipLaunchKernelGGL((k_gcmc<1024, PREC>), grid, dim3(1024), 0, g->ctx->stream, P, r, nsteps); break;
-    default: hipLaunchKernelGGL((k_gcmc<256, PREC>), grid, dim3(256), 0, g->ctx->stream, P, r, nsteps); break;
+    case 64: launch_nt<PREC, 64>(g, P, r, nsteps); break;
+    case 128: launch_nt<PREC, 128>(g, P, r, nsteps); break;
+    case 512: launch_nt<PREC, 512>(g, P, r, nsteps); break;
+    case 1024: launch_nt<PREC, 1024>(g, P, r, nsteps); break;
+    default: launch_nt<PREC, 256>(g, P, r, nsteps); break;
     }
+}
+
+// dense copy of a map for the GCMC lookups, built once per map
+static int map_dense(mh_ctx *ctx, mh_map *m, const float **out) {
+    const long long nodes = (long long)m->npnts[0] * m->npnts[1] * m->npnts[2];
+    if (!m->dense.p) {
+        int rc;
+        if ((rc = m->dense.alloc(sizeof(float) * 8 * nodes))) return rc;
+        hipLaunchKernelGGL(k_densify, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, ctx->stream,
+                           m->ptr.as<int32_t>(), m->vals.as<float>(), nodes, m->dense.as<float>());
+        MH_HIP(hipGetLastError());
+        MH_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    *out = m->dense.as<float>();
+    return MH_OK;
+}
+
+// LDS of the persistent kernel: molecules in dynamic LDS when they fit
+static void plan_lds(mh_gcmc *g) {
+    int dev_max = 65536;
+    (void)hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, g->ctx->device);
+    // gfx950: 160 KB of LDS per workgroup (one replica per CU)
+    const long long budget = (long long)dev_max - (long long)sizeof(Shared) - (long long)sizeof(GDev) - 1024;
+    const long long need = (long long)g->maxN * (3 * g->ns + 6) * (long long)sizeof(double);
+    g->ldsmol = need <= budget;
+    g->lds_bytes = g->ldsmol ? (size_t)need : 0;
+    g->kpl = g->ldsmol ? pick_kpl(g->nk, g->threads) : 0;
+    // e^{i(kx x + ky y)} tables behind the molecules when they fit too
+    GDev &H = g->host;
+    const long long K = H.kmax, ent = (g->prec == MH_PREC_FP64) ? 16 : 8;
+    const long long exy = H.ewald ? 2LL * g->ns * (K + 1) * (2 * K + 1) * ent : 0;
+    const long long off = (g->lds_bytes + 255) / 256 * 256;
+    H.use_exy = (g->kpl > 0 && exy > 0 && off + exy <= budget) ? 1 : 0;
+    H.exy_off = H.use_exy ? (int)off : 0;
+    if (H.use_exy) g->lds_bytes = (size_t)(off + exy);
+    (void)hipMemcpy(g->dP.p, &H, sizeof(GDev), hipMemcpyHostToDevice);
 }
 
 extern "C" {
@@ -783,8 +1247,11 @@ int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double 
             if (!maps[w]) continue;
             MapRef &m = w == 0 ? H.pm[a] : H.em[a];
             (w == 0 ? H.has_p : H.has_e)[a] = 1;
-            m.ptr = maps[w]->ptr.as<int32_t>();
-            m.vals = maps[w]->vals.as<float>();
+            int rcd = map_dense(ctx, const_cast<mh_map *>(maps[w]), &m.dense);
+            if (rcd) {
+                delete g;
+                return rcd;
+            }
             for (int i = 0; i < 3; i++) {
                 m.n[i] = maps[w]->npnts[i];
                 m.step[i] = maps[w]->step[i];
@@ -863,6 +1330,8 @@ int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double 
     H.rti = 1.0 / ((8.314e-3 / 4.184) * p->temperature);
     H.prec = p->precision;
     H.maxN = maxN;
+    H.inv_ns = 1.0 / H.ns;
+    H.inv_nss = 1.0 / (H.ns * H.ns);
     int rc;
     auto fail = [&](int code) {
         delete g;
@@ -911,6 +1380,7 @@ int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double 
         hipMemcpy(g->acc.p, zl.data(), 32 * R, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(g->S.p, 0, sizeof(double2) * R * (H.nk + 1)) != hipSuccess)
         return fail(MH_EHIP);
+    plan_lds(g);
     ctx_retain(ctx);
     *out = g;
     return MH_OK;
@@ -931,6 +1401,7 @@ int mh_gcmc_set_threads(mh_gcmc *g, int threads) {
     MH_CHECK(threads == 64 || threads == 128 || threads == 256 || threads == 512 || threads == 1024,
              MH_EINVAL, "mh_gcmc_set_threads: threads must be 64, 128, 256, 512 or 1024");
     g->threads = threads;
+    plan_lds(g);
     return MH_OK;
 }
 

@@ -141,4 +141,8 @@ struct mh_map {
     mh::DBuf vals;   // float [nfsize][8] (AoS records, 32 B)
     mh::DBuf full32; // float [nodes][8] when kept
     mh::DBuf full64; // double [nodes][8] when kept
+    // uncompacted node-major copy [nodes][8] for the GCMC lookups (one HBM
+    // round trip per corner instead of ptr then value); V = NaN where the
+    // node is untabulated.  Built on first use.
+    mh::DBuf dense;
 };

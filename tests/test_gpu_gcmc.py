@@ -114,27 +114,39 @@ def _oracle_for(spec, maps, fug, seed):
     return O.Gcmc(sp, fug, seed, trace=4000)
 
 
+# kernel variants: (workgroup size, maxN) -> molecules in LDS with the k
+# vectors in registers (tiles of 8, 16, 4) or S(k) in HBM (1024 lanes), and
+# molecules in HBM (maxN too large for LDS)
+VARIANTS = [(256, 512), (128, 512), (512, 512), (1024, 512), (256, 20000)]
+
+
 def test_gpu_fp64_trajectory_matches_oracle_with_maps_and_ewald(hip):
-    print("case: CO2-like sorbate, maps + Ewald, FP64 trajectory parity", flush=True)
     spec, maps, L = small_charged_system(hip)
     fugs = [300.0, 3000.0]
     seeds = [3001, 3002]
-    b = G.GcmcBatch(hip, spec, fugs, seeds, maxN=512, maps=maps)
-    b.set_trace(3000)
-    b.run(3000)
-    s = b.state()
+    ref = []
     for r in range(2):
         g = _oracle_for(spec, maps, fugs[r], seeds[r])
         g.run(3000)
-        to = g.trace_array(3000)
-        tg = b.trace(r)
-        assert np.array_equal(tg[:, :3].astype(int), to[:, :3].astype(int)), r
-        scale = np.maximum(np.abs(to[:, 3]), 1.0)
-        assert np.max(np.abs(tg[:, 3] - to[:, 3]) / scale) < 1e-10
-        assert g.st.N == s["N"][r] and g.st.N > 5
-        assert abs(g.st.energy - s["energy"][r]) < 1e-9 * max(1.0, abs(g.st.energy))
-    tot = b.total_energy()
-    assert np.allclose(tot, s["energy"], rtol=1e-10, atol=1e-9)
+        ref.append((g.trace_array(3000), g.st.N, g.st.energy))
+    for threads, maxN in VARIANTS:
+        print("case: CO2-like sorbate, maps + Ewald, FP64 trajectory parity, threads %d maxN %d"
+              % (threads, maxN), flush=True)
+        b = G.GcmcBatch(hip, spec, fugs, seeds, maxN=maxN, maps=maps, threads=threads)
+        b.set_trace(3000)
+        b.run(3000)
+        s = b.state()
+        for r in range(2):
+            to, No, eo = ref[r]
+            tg = b.trace(r)
+            assert np.array_equal(tg[:, :3].astype(int), to[:, :3].astype(int)), (threads, maxN, r)
+            scale = np.maximum(np.abs(to[:, 3]), 1.0)
+            assert np.max(np.abs(tg[:, 3] - to[:, 3]) / scale) < 1e-10
+            assert No == s["N"][r] and No > 5
+            assert abs(eo - s["energy"][r]) < 1e-9 * max(1.0, abs(eo))
+        tot = b.total_energy()
+        assert np.allclose(tot, s["energy"], rtol=1e-10, atol=1e-9)
+        b.close()
 
 
 def test_gpu_fp32_trial_energies_vs_fp64(hip):
