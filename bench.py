@@ -25,6 +25,21 @@ METRIC = "GCMC trial moves/s + map pts/s, FP32-acc vs FP64; p50/p99 request late
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, dense
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "map_ewald_real_fp32_charged_50A_hbm_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the Ewald real-space map kernel from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated)."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -89,36 +104,62 @@ def cpu_baseline(cfg, tabs, seconds=15.0):
                        "one thread, -O3 -march=native, %.1f s)" % (sample, total_t))
 
 
-def gcmc_leg(H, fp32_steps, fp64_steps, equil_steps, n_single=8):
-    """Batched isotherm: 64 CO2-like state points advanced together in FP32
-    pair arithmetic (FP64 accumulation) against the plain FP64 kernel run one
-    state point at a time on the same equilibrated configurations."""
+def gcmc_leg(H, fp32_steps, fp64_steps, equil_steps, chains=4, n_single=8):
+    """Batched isotherm: the 64 CO2-like state points advanced together in
+    FP32 pair arithmetic (FP64 accumulation), one chain per state point and
+    `chains` independent chains per state point (seeds 3000 + i + 1000 c),
+    against the plain FP64 kernel run one state point at a time on the same
+    equilibrated configurations."""
     from musichip import gcmc as G
     from musichip import workloads as W
     ctx = H.Context(0)
     cs = W.co2_isotherm_system(ctx)
-    nrep = len(cs["fugacities"])
-    b = G.GcmcBatch(ctx, cs["spec"], cs["fugacities"], cs["seeds"], maxN=1024, maps=cs["maps"],
-                    precision=H.MH_PREC_FP32, threads=256)
-    t = time.perf_counter()
-    b.run(equil_steps)
-    log("gcmc: equilibrated %d steps x %d state points in %.2f s" % (equil_steps, nrep, time.perf_counter() - t))
-    s0 = b.state()
-    cfgs = [b.config(r) for r in range(nrep)]
-    t = time.perf_counter()
-    b.run(fp32_steps)
-    dt32 = time.perf_counter() - t
-    s1 = b.state()
-    att32 = int((s1["att"] - s0["att"]).sum())
-    rate32 = nrep * fp32_steps / dt32
-    log("gcmc FP32 batched: %.3e steps/s (%.3e attempted moves/s)" % (rate32, att32 / dt32))
+    npt = len(cs["fugacities"])
+
+    def batch(nchains, prec):
+        fug = np.tile(cs["fugacities"], nchains)
+        seeds = [cs["seeds"][i] + 1000 * c for c in range(nchains) for i in range(npt)]
+        return G.GcmcBatch(ctx, cs["spec"], fug, seeds, maxN=1024, maps=cs["maps"], precision=prec, threads=256)
+
+    def timed(b, steps):
+        s0 = b.state()
+        t = time.perf_counter()
+        b.run(steps)
+        dt = time.perf_counter() - t
+        s1 = b.state()
+        return b.nrep * steps / dt, int((s1["att"] - s0["att"]).sum()) / dt, s1
+
+    out = dict(workload="CO2-like 3-site sorbate, 64 state points 1-1e4 kPa, 300 K, C1 framework "
+                        "(30 A, 768 atoms), pmaps C/O at 0.2 A, sorbate LJ 12.8 A + Ewald kappa 0.25/A "
+                        "kmax 7 (nk %d half-space), moves 1:1:1:1" % (((2 * 7 + 1) ** 3 - 1) // 2))
+    cfgs = None
+    for nch in (1, chains):
+        b = batch(nch, H.MH_PREC_FP32)
+        t = time.perf_counter()
+        b.run(equil_steps)
+        log("gcmc: %d chains equilibrated %d steps in %.2f s" % (b.nrep, equil_steps, time.perf_counter() - t))
+        tune = b.autotune(200)
+        rate, arate, s1 = timed(b, fp32_steps)
+        log("gcmc FP32 batched, %d chains (threads %d, xy tables %d): %.3e steps/s" % (b.nrep, tune[0], tune[1], rate))
+        key = "fp32_batched_%dchains" % b.nrep
+        out[key + "_steps_per_s"] = rate
+        out[key + "_attempted_moves_per_s"] = arate
+        out[key + "_tuned"] = dict(threads=tune[0], xy_tables=tune[1])
+        if nch == 1:
+            out["loadings"] = [int(x) for x in s1["N"]]
+            cfgs = [b.config(r) for r in range(npt)]
+            b32 = b
+        else:
+            b.close()
     # the same state points, FP64, one state point per launch
-    picks = np.linspace(0, nrep - 1, n_single).round().astype(int)
+    picks = np.linspace(0, npt - 1, n_single).round().astype(int)
     tot_t, tot_steps, tot_att = 0.0, 0, 0
+    tuned64 = []
     for r in picks:
         one = G.GcmcBatch(ctx, cs["spec"], cs["fugacities"][r:r + 1], cs["seeds"][r:r + 1], maxN=1024,
                           maps=cs["maps"], precision=H.MH_PREC_FP64, threads=256)
         one.set_config(0, cfgs[r][0], cfgs[r][1])
+        tuned64.append(one.autotune(100)[0])
         a0 = one.state()["att"].sum()
         t = time.perf_counter()
         one.run(fp64_steps)
@@ -128,6 +169,13 @@ def gcmc_leg(H, fp32_steps, fp64_steps, equil_steps, n_single=8):
         one.close()
     rate64 = tot_steps / tot_t
     log("gcmc FP64 single state point: %.3e steps/s" % rate64)
+    out["fp64_single_state_point_steps_per_s"] = rate64
+    out["fp64_single_state_point_attempted_moves_per_s"] = tot_att / tot_t
+    out["fp64_single_state_point_threads"] = tuned64
+    for nch in (1, chains):
+        k = "fp32_batched_%dchains_steps_per_s" % (npt * nch)
+        out["speedup_%dchains_vs_fp64_single" % (npt * nch)] = out[k] / rate64
+    out["fp32_batched_steps_per_s"] = out["fp32_batched_%dchains_steps_per_s" % (npt * chains)]
     # FP32 vs FP64 trial energies on the equilibrated configurations
     rng = np.random.default_rng(11)
     L = np.asarray(cs["spec"]["full_ell"])
@@ -140,23 +188,15 @@ def gcmc_leg(H, fp32_steps, fp64_steps, equil_steps, n_single=8):
         ta = np.stack([rng.uniform(-np.pi, np.pi, 256), np.arccos(rng.uniform(-1, 1, 256)),
                        rng.uniform(-np.pi, np.pi, 256)], 1)
         ok64, e64 = b64.trial_energy(r, tc, ta)
-        ok32, e32 = b.trial_energy(r, tc, ta)
+        ok32, e32 = b32.trial_energy(r, tc, ta)
         both = ok64 & ok32
         if both.any():
             worst = max(worst, float(np.max(np.abs(e32[both] - e64[both]) / np.maximum(np.abs(e64[both]), 1.0))))
+    out["fp32_vs_fp64_trial_energy_max_rel_err"] = worst
+    out["steps"] = dict(equil=equil_steps, fp32=fp32_steps, fp64_per_state_point=fp64_steps,
+                        fp64_state_points=[int(x) for x in picks], chains_per_state_point=chains)
     b64.close()
-    out = dict(workload="CO2-like 3-site sorbate, 64 state points 1-1e4 kPa, 300 K, C1 framework "
-                        "(30 A, 768 atoms), pmaps C/O at 0.2 A, sorbate LJ 12.8 A + Ewald kappa 0.25/A "
-                        "kmax 7 (nk %d half-space), moves 1:1:1:1" % (((2 * 7 + 1) ** 3 - 1) // 2),
-               loadings=[int(x) for x in s1["N"]], fp32_batched_steps_per_s=rate32,
-               fp32_batched_attempted_moves_per_s=att32 / dt32,
-               fp64_single_state_point_steps_per_s=rate64,
-               fp64_single_state_point_attempted_moves_per_s=tot_att / tot_t,
-               speedup_batched_fp32_vs_fp64_single=rate32 / rate64,
-               fp32_vs_fp64_trial_energy_max_rel_err=worst,
-               steps=dict(equil=equil_steps, fp32=fp32_steps, fp64_per_state_point=fp64_steps,
-                          fp64_state_points=[int(x) for x in picks]))
-    b.close()
+    b32.close()
     for m in cs["map_objs"]:
         m.close()
     ctx.close()
@@ -175,12 +215,24 @@ def main():
     ap.add_argument("--gcmc-fp64-steps", type=int, default=1000)
     ap.add_argument("--gcmc-equil", type=int, default=30000)
     ap.add_argument("--no-gcmc", action="store_true")
+    ap.add_argument("--gcmc-chains", type=int, default=4)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the map build of one framework runs on one "
               "device (multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
         sys.exit(2)
     import torch  # noqa: F401  (device memory/stream plumbing only)
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib << 18  # float32 elements
+        x = torch.ones(n, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(copy_probe_bytes=4 * n, launches=5)))
+        return
     from musichip import hip as H
     from musichip import synth
 
@@ -223,7 +275,7 @@ def main():
     k_ms = ktimes[kname][0] / ktimes[kname][1]
     achieved = flops_launch / (k_ms * 1e-3) / 1e12
     roofline = dict(bound="valu", achieved=round(achieved, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / FP32_PEAK_TFLOPS, 4), traffic=None, kernel=kname,
+                    frac=round(achieved / FP32_PEAK_TFLOPS, 4), traffic=measured_traffic(), kernel=kname,
                     kernel_ms=round(k_ms, 3),
                     algorithmic="85 FLOP x %.1f pairs/node (r < %.2f A) x %d nodes" % (pairs_per_node, rc, nodes))
 
@@ -267,7 +319,7 @@ def main():
 
     gcmc = None
     if not args.no_gcmc:
-        gcmc = gcmc_leg(H, args.gcmc_steps, args.gcmc_fp64_steps, args.gcmc_equil)
+        gcmc = gcmc_leg(H, args.gcmc_steps, args.gcmc_fp64_steps, args.gcmc_equil, args.gcmc_chains)
 
     cpu = None
     if not args.no_cpu:

@@ -179,6 +179,13 @@ int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double 
 int mh_gcmc_destroy(mh_gcmc *g);
 /* Threads per replica (64..1024, power of 2); chosen by the autotuner. */
 int mh_gcmc_set_threads(mh_gcmc *g, int threads);
+/* Autotuner (no reference counterpart: the reference is serial): times
+ * probe_steps on every replica for each candidate workgroup size (64, 128,
+ * 256, 512) and reciprocal-space table mode from a snapshot of the batch,
+ * restores the snapshot and keeps the fastest.  threads_out / exy_out may be
+ * NULL.  Results are identical for every candidate (FP64: bit-identical
+ * moves; only the schedule differs). */
+int mh_gcmc_autotune(mh_gcmc *g, int probe_steps, int *threads_out, int *exy_out);
 /* Place n molecules (com[n][3], Euler angles phi, theta, psi [n][3]) in a
  * replica; recomputes its structure factor and energy. */
 int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const double *ang);

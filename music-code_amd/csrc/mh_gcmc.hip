@@ -1066,6 +1066,7 @@ struct mh_gcmc {
     int kpl = 0;            // k vectors per lane held in registers (0: S(k) in HBM)
     bool ldsmol = false;    // molecules held in LDS during a launch
     size_t lds_bytes = 0;   // dynamic LDS of the launch
+    bool exy_off = false;   // autotuner: e^{i(kx x + ky y)} tables disabled
     GDev host;
     DBuf dP, wk, kidx;
     DBuf N, err, ix, iy, first, iseed, B, energy, sumN, nsteps, att, acc;
@@ -1198,7 +1199,7 @@ static void plan_lds(mh_gcmc *g) {
     const long long K = H.kmax, ent = (g->prec == MH_PREC_FP64) ? 16 : 8;
     const long long exy = H.ewald ? 2LL * g->ns * (K + 1) * (2 * K + 1) * ent : 0;
     const long long off = (g->lds_bytes + 255) / 256 * 256;
-    H.use_exy = (g->kpl > 0 && exy > 0 && off + exy <= budget) ? 1 : 0;
+    H.use_exy = (!g->exy_off && g->kpl > 0 && exy > 0 && off + exy <= budget) ? 1 : 0;
     H.exy_off = H.use_exy ? (int)off : 0;
     if (H.use_exy) g->lds_bytes = (size_t)(off + exy);
     (void)hipMemcpy(g->dP.p, &H, sizeof(GDev), hipMemcpyHostToDevice);
@@ -1595,6 +1596,79 @@ int mh_trial_energy(mh_gcmc *g, int rep, int n, const double *com, const double 
     MH_HIP(hipStreamSynchronize(g->ctx->stream));
     MH_HIP(hipMemcpy(e, de.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     MH_HIP(hipMemcpy(ok, dok.p, n, hipMemcpyDeviceToHost));
+    return MH_OK;
+}
+
+int mh_gcmc_autotune(mh_gcmc *g, int probe_steps, int *threads_out, int *exy_out) {
+    MH_CHECK(g && probe_steps > 0, MH_EINVAL, "mh_gcmc_autotune: bad arguments");
+    MH_HIP(hipSetDevice(g->ctx->device));
+    hipStream_t st = g->ctx->stream;
+    // snapshot of every buffer a run modifies
+    DBuf *bufs[] = {&g->N, &g->err, &g->ix, &g->iy, &g->first, &g->iseed, &g->energy, &g->sumN,
+                    &g->nsteps, &g->att, &g->acc, &g->com, &g->ang, &g->pos, &g->S};
+    const int nb = (int)(sizeof(bufs) / sizeof(bufs[0]));
+    std::vector<DBuf> snap(nb);
+    int rc;
+    for (int i = 0; i < nb; i++) {
+        if ((rc = snap[i].alloc(bufs[i]->n))) return rc;
+        MH_HIP(hipMemcpyAsync(snap[i].p, bufs[i]->p, bufs[i]->n, hipMemcpyDeviceToDevice, st));
+    }
+    const int tr_cap = g->tr_cap;
+    g->tr_cap = 0;  // no trace writes while probing
+    auto restore = [&]() -> int {
+        for (int i = 0; i < nb; i++)
+            MH_HIP(hipMemcpyAsync(bufs[i]->p, snap[i].p, bufs[i]->n, hipMemcpyDeviceToDevice, st));
+        MH_HIP(hipStreamSynchronize(st));
+        return MH_OK;
+    };
+    hipEvent_t e0, e1;
+    MH_HIP(hipEventCreate(&e0));
+    MH_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_t = g->threads;
+    bool best_exy_off = g->exy_off;
+    for (int t : {64, 128, 256, 512}) {
+        for (int xo = 0; xo < 2; xo++) {
+            g->threads = t;
+            g->exy_off = xo != 0;
+            plan_lds(g);
+            if (xo && !g->host.ewald) continue;
+            if ((rc = restore())) break;
+            for (int rep = 0; rep < 2; rep++) {  // first launch warms the code object
+                (void)hipEventRecord(e0, st);
+                if (g->prec == MH_PREC_FP32) launch_gcmc<MH_PREC_FP32>(g, probe_steps);
+                else launch_gcmc<MH_PREC_FP64>(g, probe_steps);
+                (void)hipEventRecord(e1, st);
+                if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                    rc = MH_EHIP;
+                    break;
+                }
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (rep == 1 && ms < best) {
+                    best = ms;
+                    best_t = t;
+                    best_exy_off = g->exy_off;
+                }
+            }
+            if (rc) break;
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g->tr_cap = tr_cap;
+    g->threads = best_t;
+    g->exy_off = best_exy_off;
+    plan_lds(g);
+    int rc2 = restore();
+    if (rc) {
+        set_error("mh_gcmc_autotune: a probe launch failed");
+        return rc;
+    }
+    if (rc2) return rc2;
+    if (threads_out) *threads_out = best_t;
+    if (exy_out) *exy_out = g->host.use_exy;
     return MH_OK;
 }
 

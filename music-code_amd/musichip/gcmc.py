@@ -47,6 +47,7 @@ def _sig():
         "mh_gcmc_set_trace": [_vp, _i],
         "mh_gcmc_get_trace": [_vp, _i, _vp, _vp, _vp, _vp, P(_i)],
         "mh_trial_energy": [_vp, _i, _i, _vp, _vp, _vp, _vp],
+        "mh_gcmc_autotune": [_vp, _i, P(_i), P(_i)],
     }
     for n, a in sig.items():
         f = getattr(L, n)
@@ -120,6 +121,13 @@ class GcmcBatch:
 
     def set_threads(self, n):
         H.check(_sig().mh_gcmc_set_threads(self.h, int(n)), "mh_gcmc_set_threads")
+
+    def autotune(self, probe_steps=200):
+        """Fastest (threads, table mode) for this batch; the state is unchanged."""
+        t, x = ctypes.c_int(), ctypes.c_int()
+        H.check(_sig().mh_gcmc_autotune(self.h, int(probe_steps), ctypes.byref(t), ctypes.byref(x)),
+                "mh_gcmc_autotune")
+        return t.value, x.value
 
     def set_config(self, rep, com, ang=None):
         com = np.ascontiguousarray(com, dtype=np.float64).reshape(-1, 3)
