@@ -203,6 +203,33 @@ def gcmc_leg(H, fp32_steps, fp64_steps, equil_steps, chains=4, n_single=8):
     return out
 
 
+def serving_leg(n=512, steps=10000, util=0.7, ncal=64):
+    """Screening service: seeded requests (map at 0.2 A + 10k-step methane
+    GCMC at 1/10/50 bar) with Poisson arrivals (seed 4999) at `util` of the
+    capacity measured on a separate calibration set (all arriving at once)."""
+    from musichip import serve
+    cal = [serve.screening_request(100000 + r, steps) for r in range(ncal)]
+    srv = serve.Server()
+    _, cst = srv.run(cal, np.zeros(ncal))
+    srv.close()
+    rate = util * cst["requests_per_s"]
+    log("serving: capacity %.1f requests/s (%d requests at once); arrival rate %.1f/s" %
+        (cst["requests_per_s"], ncal, rate))
+    reqs = [serve.screening_request(r, steps) for r in range(n)]
+    arr = serve.poisson_arrivals(n, rate)
+    srv = serve.Server()
+    res, st = srv.run(reqs, arr)
+    srv.close()
+    st["arrival_rate_per_s"] = rate
+    st["capacity_requests_per_s"] = cst["requests_per_s"]
+    st["target_utilisation"] = util
+    st["steps_per_request"] = steps * len(serve.FUGACITIES_KPA)
+    st["failed_replicas"] = int(sum(int((r["err"] != 0).sum()) for r in res))
+    log("serving: p50 %.1f ms, p99 %.1f ms, %.1f requests/s" % (st["p50_latency_ms"], st["p99_latency_ms"],
+                                                             st["requests_per_s"]))
+    return st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +243,8 @@ def main():
     ap.add_argument("--gcmc-equil", type=int, default=30000)
     ap.add_argument("--no-gcmc", action="store_true")
     ap.add_argument("--gcmc-chains", type=int, default=4)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -321,6 +350,10 @@ def main():
     if not args.no_gcmc:
         gcmc = gcmc_leg(H, args.gcmc_steps, args.gcmc_fp64_steps, args.gcmc_equil, args.gcmc_chains)
 
+    serving = None
+    if not args.no_serve:
+        serving = serving_leg(args.serve_requests)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, tabs, args.cpu_seconds)
@@ -335,7 +368,8 @@ def main():
                fp64_map_pts_per_s=fp64, fp32_vs_fp64=errs, kernel_ms_per_step={
                    k: v[0] / args.steps for k, v in ktimes.items()},
                gcmc=gcmc, gcmc_trial_moves_per_s=gcmc and gcmc["fp32_batched_steps_per_s"],
-               p50_latency_ms=None, p99_latency_ms=None,
+               p50_latency_ms=serving and serving["p50_latency_ms"],
+               p99_latency_ms=serving and serving["p99_latency_ms"], serving=serving,
                roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(out))
 

@@ -213,6 +213,39 @@ int mh_gcmc_get_trace(mh_gcmc *g, int rep, int *move, int *accepted, int *N, dou
 int mh_trial_energy(mh_gcmc *g, int rep, int n, const double *com, const double *ang, double *e,
                     uint8_t *ok);
 
+/* ------------------------------------------------------------ slot pools
+ * Continuous batching of GCMC requests (no counterpart in the reference,
+ * which runs one simulation per process).  A pool is an mh_gcmc handle
+ * with nslots replica slots and up to nsys systems (sorbate model + maps +
+ * cell), each slot carrying its own system and step budget.  One
+ * mh_pool_launch advances every occupied slot by min(nsteps, its remaining
+ * budget) in one persistent kernel; requests are admitted into free slots
+ * and released between launches, so work of different requests shares the
+ * device.  A slot's trajectory depends only on its own system, seed and
+ * budget: results are identical whether a request runs alone or with
+ * others.  mh_gcmc_get_config / set_config / total_energy / trial_energy
+ * take slot indices. */
+int mh_pool_create(mh_ctx *ctx, int nslots, int nsys, int maxN, int kmax_cap, int max_sites, int precision,
+                   mh_gcmc **out);
+/* Admit a request: nrep replicas of one system (params, its cell edges;
+ * params->precision is ignored, the pool's applies), each running nsteps
+ * steps.  ticket receives the request's id.  MH_ECAPACITY when no system
+ * entry or not enough free slots are left (admit again after a release). */
+int mh_pool_admit(mh_gcmc *g, const mh_gcmc_params *p, const double fund_ell[3], const double full_ell[3],
+                  int nrep, const double *fugacity_kPa, const int32_t *seeds, int nsteps, int *ticket);
+/* Enqueue one launch of up to nsteps steps per occupied slot (asynchronous). */
+int mh_pool_launch(mh_gcmc *g, int nsteps);
+/* Wait for the launches enqueued so far. */
+int mh_pool_wait(mh_gcmc *g);
+/* Per replica of a ticket (admit order): slot index, remaining steps,
+ * error flag (2 = reached maxN, 3 = evaluation failure of an existing
+ * molecule), N, energy, sum of N over steps, steps done, att/acc [nrep][4].
+ * nrep receives the replica count; any other pointer may be NULL. */
+int mh_pool_status(mh_gcmc *g, int ticket, int *nrep, int *slots, int *remaining, int *err, int32_t *N,
+                   double *energy, double *sumN, int64_t *nsteps, int64_t *att, int64_t *acc);
+/* Free the ticket's slots and system entry. */
+int mh_pool_release(mh_gcmc *g, int ticket);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,6 +83,11 @@ struct Rep {
     double *tr_dE;
     int tr_cap;
     long long tr_base;        // step index at which the trace starts
+    // slot pools: system (GDev table index) of each replica, -1 = free slot;
+    // remaining step budget per replica.  NULL for a single-system batch.
+    const int *sys;
+    int *budget;
+    long long kstride;        // S(k) entries per replica
 };
 
 struct Rng {
@@ -657,12 +662,19 @@ __device__ __forceinline__ void sk_accept(const GDev &P, KReg<KPL> &K, double2 *
 // The persistent GCMC loop.  LDSMOL: molecules held in dynamic LDS
 // ([maxN][ns][3] sites, [maxN][3] com, [maxN][3] angles, FP64).
 template <int NT, int PREC, int KPL, bool LDSMOL>
-__global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Pg, Rep R, int nsteps) {
+__global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep R, int nsteps) {
     extern __shared__ double dyn[];
     __shared__ Shared sh;
     __shared__ GDev P;
     const int rep = blockIdx.x;
     const int tid = threadIdx.x;
+    int nrun = nsteps;
+    if (R.sys) {
+        if (R.sys[rep] < 0) return;  // free slot
+        nrun = min(nsteps, R.budget[rep]);
+        if (nrun <= 0) return;
+    }
+    const GDev *Pg = Ptab + (R.sys ? R.sys[rep] : 0);
     for (int i = tid; i < (int)(sizeof(GDev) / 4); i += NT)
         reinterpret_cast<int *>(&P)[i] = reinterpret_cast<const int *>(Pg)[i];
     __syncthreads();
@@ -681,8 +693,8 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Pg, Rep R,
     double *gcom = R.com + (long long)rep * P.maxN * 3;
     double *gang = R.ang + (long long)rep * P.maxN * 3;
     double *gpos = R.pos + (long long)rep * P.maxN * ns * 3;
-    double2 *S = R.S + (long long)rep * P.nk;
-    double2 *dS = R.dS + (long long)rep * P.nk;
+    double2 *S = R.S + (long long)rep * R.kstride;
+    double2 *dS = R.dS + (long long)rep * R.kstride;
     double *pos, *com, *ang;
     void *exy = reinterpret_cast<char *>(dyn) + P.exy_off;
     if constexpr (LDSMOL) {
@@ -718,8 +730,8 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Pg, Rep R,
         }
     }
     __syncthreads();
-    int err = 0;
-    for (int it = 0; it < nsteps; it++) {
+    int err = 0, done = 0;
+    for (int it = 0; it < nrun; it++) {
         (void)rranf(rng);  // species (nspc = 1), gcmc.F90:242
         const double prob = rranf(rng);
         int mv = 0;
@@ -906,6 +918,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Pg, Rep R,
         }
         sumN += N;
         nst++;
+        done++;
         __syncthreads();
     }
     // write the replica back
@@ -924,6 +937,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Pg, Rep R,
         }
     }
     if (tid == 0) {
+        if (R.budget) R.budget[rep] = err ? 0 : R.budget[rep] - done;
         R.err[rep] = err;
         R.ix[rep] = rng.ix;
         R.iy[rep] = rng.iy;
@@ -942,13 +956,14 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Pg, Rep R,
 
 // Insertion energies of trial molecules (one workgroup per trial)
 template <int NT, int PREC>
-__global__ void __launch_bounds__(NT) k_trial(const GDev *__restrict__ Pg, Rep R, int rep, int n,
+__global__ void __launch_bounds__(NT) k_trial(const GDev *__restrict__ Ptab, Rep R, int rep, int n,
                                                const double *__restrict__ tcom, const double *__restrict__ tang,
                                                double2 *__restrict__ dSbuf, double *__restrict__ e,
                                                uint8_t *__restrict__ okout) {
     __shared__ Shared sh;
     __shared__ GDev P;
     const int tid = threadIdx.x;
+    const GDev *Pg = Ptab + (R.sys ? R.sys[rep] : 0);
     for (int i = tid; i < (int)(sizeof(GDev) / 4); i += NT)
         reinterpret_cast<int *>(&P)[i] = reinterpret_cast<const int *>(Pg)[i];
     __syncthreads();
@@ -956,7 +971,7 @@ __global__ void __launch_bounds__(NT) k_trial(const GDev *__restrict__ Pg, Rep R
     if (t >= n) return;
     const int N = R.N[rep];
     const double *pos = R.pos + (long long)rep * P.maxN * P.ns * 3;
-    const double2 *S = R.S + (long long)rep * P.nk;
+    const double2 *S = R.S + (long long)rep * R.kstride;
     if (tid == 0) site_coords(P, tcom + 3 * t, tang + 3 * t, sh.posA);
     __syncthreads();
     double o[6];
@@ -972,9 +987,10 @@ __global__ void __launch_bounds__(NT) k_trial(const GDev *__restrict__ Pg, Rep R
 }
 
 // S(k) of every replica from the coordinates (FP64), and total energies
-__global__ void k_sk_recompute(const GDev *__restrict__ Pg, Rep R) {
-    const GDev &P = *Pg;
+__global__ void k_sk_recompute(const GDev *__restrict__ Ptab, Rep R) {
     const int rep = blockIdx.y;
+    if (R.sys && R.sys[rep] < 0) return;
+    const GDev &P = Ptab[R.sys ? R.sys[rep] : 0];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P.nk) return;
     const int N = R.N[rep];
@@ -993,16 +1009,21 @@ __global__ void k_sk_recompute(const GDev *__restrict__ Pg, Rep R) {
             re += P.q[a] * c_;
             im += P.q[a] * s_;
         }
-    R.S[(long long)rep * P.nk + k] = make_double2(re, im);
+    R.S[(long long)rep * R.kstride + k] = make_double2(re, im);
 }
 
 // total energy per replica: sum over molecules m of (maps + pairs with j < m)
 // + reciprocal + N * molconst; one workgroup per replica
 template <int NT>
-__global__ void __launch_bounds__(NT) k_total(const GDev *__restrict__ Pg, Rep R, double *__restrict__ out) {
+__global__ void __launch_bounds__(NT) k_total(const GDev *__restrict__ Ptab, Rep R, double *__restrict__ out) {
     __shared__ GDev P;
     __shared__ double red[NT / 64 * 2];
     const int tid = threadIdx.x;
+    if (R.sys && R.sys[blockIdx.x] < 0) {
+        if (tid == 0) out[blockIdx.x] = 0.0;
+        return;
+    }
+    const GDev *Pg = Ptab + (R.sys ? R.sys[blockIdx.x] : 0);
     for (int i = tid; i < (int)(sizeof(GDev) / 4); i += NT)
         reinterpret_cast<int *>(&P)[i] = reinterpret_cast<const int *>(Pg)[i];
     __syncthreads();
@@ -1030,7 +1051,7 @@ __global__ void __launch_bounds__(NT) k_total(const GDev *__restrict__ Pg, Rep R
     }
     if (P.ewald)
         for (int k = tid; k < P.nk; k += NT) {
-            double2 s = R.S[(long long)rep * P.nk + k];
+            double2 s = R.S[(long long)rep * R.kstride + k];
             e += P.wk[k] * (s.x * s.x + s.y * s.y);
         }
     double v[2] = {e, bad};
@@ -1060,15 +1081,32 @@ __global__ void k_densify(const int32_t *__restrict__ ptr, const float *__restri
 
 using namespace mh;
 
+
+// One molecular system of a batch: sorbate model, maps, cell, k vectors.
+// A plain batch has one system; a slot pool holds one per admitted request.
+struct SysHost {
+    bool used = false;
+    GDev H;
+    DBuf wk, kidx;
+    double volume = 0.0;
+    std::vector<int> slots;  // replicas (slots) of this system, admit order
+};
+
 struct mh_gcmc {
     mh_ctx *ctx = nullptr;
-    int nrep = 0, maxN = 0, ns = 0, nk = 0, prec = 0, threads = 256;
+    int nrep = 0, maxN = 0, prec = 0, threads = 256;
+    int ns = 1;             // max sites over the systems (LDS layout)
+    int nk = 0;             // max k vectors over the systems
+    long long kstride = 0;  // S(k) entries per replica
     int kpl = 0;            // k vectors per lane held in registers (0: S(k) in HBM)
     bool ldsmol = false;    // molecules held in LDS during a launch
     size_t lds_bytes = 0;   // dynamic LDS of the launch
     bool exy_off = false;   // autotuner: e^{i(kx x + ky y)} tables disabled
-    GDev host;
-    DBuf dP, wk, kidx;
+    bool pool = false;      // slot pool (per-replica system and step budget)
+    std::vector<SysHost> sys;
+    std::vector<int> slot_sys;  // host mirror of sysidx (pool)
+    DBuf dP;                    // GDev[sys.size()]
+    DBuf sysidx, budget;        // pool only
     DBuf N, err, ix, iy, first, iseed, B, energy, sumN, nsteps, att, acc;
     DBuf com, ang, pos, S, dS;
     DBuf tr_move, tr_acc, tr_N, tr_dE;
@@ -1099,8 +1137,12 @@ struct mh_gcmc {
         r.tr_dE = tr_dE.as<double>();
         r.tr_cap = tr_cap;
         r.tr_base = tr_base;
+        r.sys = pool ? sysidx.as<int>() : nullptr;
+        r.budget = pool ? budget.as<int>() : nullptr;
+        r.kstride = kstride;
         return r;
     }
+    const GDev &sys_of(int rep) const { return sys[pool ? slot_sys[rep] : 0].H; }
 };
 
 static double erfc_as_host(double x) {
@@ -1169,6 +1211,11 @@ static void launch_gcmc(mh_gcmc *g, int nsteps) {
     }
 }
 
+static void launch_any(mh_gcmc *g, int nsteps) {
+    if (g->prec == MH_PREC_FP32) launch_gcmc<MH_PREC_FP32>(g, nsteps);
+    else launch_gcmc<MH_PREC_FP64>(g, nsteps);
+}
+
 // dense copy of a map for the GCMC lookups, built once per map
 static int map_dense(mh_ctx *ctx, mh_map *m, const float **out) {
     const long long nodes = (long long)m->npnts[0] * m->npnts[1] * m->npnts[2];
@@ -1184,79 +1231,80 @@ static int map_dense(mh_ctx *ctx, mh_map *m, const float **out) {
     return MH_OK;
 }
 
-// LDS of the persistent kernel: molecules in dynamic LDS when they fit
-static void plan_lds(mh_gcmc *g) {
+// Launch plan over the live systems: molecules in dynamic LDS when they fit
+// (gfx950: 160 KB per workgroup, one replica per CU), the k-vector register
+// tile, and per system the e^{i(kx x + ky y)} tables behind the molecules;
+// uploads the GDev table.
+static int plan(mh_gcmc *g) {
     int dev_max = 65536;
     (void)hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, g->ctx->device);
-    // gfx950: 160 KB of LDS per workgroup (one replica per CU)
     const long long budget = (long long)dev_max - (long long)sizeof(Shared) - (long long)sizeof(GDev) - 1024;
+    int ns = 1, nk = 0;
+    for (auto &s : g->sys)
+        if (s.used) {
+            ns = std::max(ns, s.H.ns);
+            nk = std::max(nk, s.H.nk);
+        }
+    g->ns = std::max(g->ns, ns);
+    g->nk = nk;
     const long long need = (long long)g->maxN * (3 * g->ns + 6) * (long long)sizeof(double);
     g->ldsmol = need <= budget;
     g->lds_bytes = g->ldsmol ? (size_t)need : 0;
-    g->kpl = g->ldsmol ? pick_kpl(g->nk, g->threads) : 0;
-    // e^{i(kx x + ky y)} tables behind the molecules when they fit too
-    GDev &H = g->host;
-    const long long K = H.kmax, ent = (g->prec == MH_PREC_FP64) ? 16 : 8;
-    const long long exy = H.ewald ? 2LL * g->ns * (K + 1) * (2 * K + 1) * ent : 0;
-    const long long off = (g->lds_bytes + 255) / 256 * 256;
-    H.use_exy = (!g->exy_off && g->kpl > 0 && exy > 0 && off + exy <= budget) ? 1 : 0;
-    H.exy_off = H.use_exy ? (int)off : 0;
-    if (H.use_exy) g->lds_bytes = (size_t)(off + exy);
-    (void)hipMemcpy(g->dP.p, &H, sizeof(GDev), hipMemcpyHostToDevice);
+    g->kpl = g->ldsmol ? pick_kpl(nk, g->threads) : 0;
+    const long long off = (g->lds_bytes + 255) / 256 * 256, ent = (g->prec == MH_PREC_FP64) ? 16 : 8;
+    long long extra = 0;
+    std::vector<GDev> tab(g->sys.size());
+    for (size_t i = 0; i < g->sys.size(); i++) {
+        GDev &H = g->sys[i].H;
+        if (g->sys[i].used) {
+            const long long K = H.kmax;
+            const long long exy = H.ewald ? 2LL * H.ns * (K + 1) * (2 * K + 1) * ent : 0;
+            H.use_exy = (!g->exy_off && g->kpl > 0 && exy > 0 && off + exy <= budget) ? 1 : 0;
+            H.exy_off = H.use_exy ? (int)off : 0;
+            if (H.use_exy) extra = std::max(extra, exy);
+            tab[i] = H;
+        } else {
+            std::memset(&tab[i], 0, sizeof(GDev));
+        }
+    }
+    if (extra) g->lds_bytes = (size_t)(off + extra);
+    MH_HIP(hipMemcpy(g->dP.p, tab.data(), sizeof(GDev) * tab.size(), hipMemcpyHostToDevice));
+    return MH_OK;
 }
 
-extern "C" {
-
-int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double *fugacity_kPa,
-                   const int32_t *seeds, int maxN, mh_gcmc **out) {
-    MH_CHECK(ctx && p && fugacity_kPa && seeds && out, MH_EINVAL, "mh_gcmc_create: bad arguments");
-    MH_CHECK(nrep >= 1 && nrep <= 65535, MH_EINVAL, "mh_gcmc_create: nrep out of range");
-    MH_CHECK(p->nsites >= 1 && p->nsites <= GS, MH_EINVAL, "mh_gcmc_create: nsites out of range [1,8]");
-    MH_CHECK(p->ntypes >= 1 && p->ntypes <= 8, MH_EINVAL, "mh_gcmc_create: ntypes out of range [1,8]");
-    MH_CHECK(maxN >= 1 && maxN <= (1 << 20), MH_EINVAL, "mh_gcmc_create: maxN out of range");
-    MH_CHECK(p->nmoves >= 1 && p->nmoves <= 4, MH_EINVAL, "mh_gcmc_create: nmoves out of range [1,4]");
-    MH_CHECK(p->temperature > 0.0, MH_EINVAL, "mh_gcmc_create: temperature must be > 0");
-    MH_CHECK(p->precision == MH_PREC_FP64 || p->precision == MH_PREC_FP32, MH_EINVAL,
-             "mh_gcmc_create: bad precision");
-    MH_CHECK(ctx->fund_ell[0] > 0.0, MH_ESTATE, "mh_gcmc_create: upload the framework (cell) first");
+// Sorbate model, maps, cell and k vectors of one system.
+static int build_system(mh_ctx *ctx, const mh_gcmc_params *p, int maxN, const double fund[3], const double full[3],
+                        SysHost &S, const char *who) {
+    std::string w(who);
+    MH_CHECK(p->nsites >= 1 && p->nsites <= GS, MH_EINVAL, w + ": nsites out of range [1,8]");
+    MH_CHECK(p->ntypes >= 1 && p->ntypes <= 8, MH_EINVAL, w + ": ntypes out of range [1,8]");
+    MH_CHECK(p->nmoves >= 1 && p->nmoves <= 4, MH_EINVAL, w + ": nmoves out of range [1,4]");
+    MH_CHECK(p->temperature > 0.0, MH_EINVAL, w + ": temperature must be > 0");
+    MH_CHECK(fund[0] > 0.0 && fund[1] > 0.0 && fund[2] > 0.0 && full[0] > 0.0 && full[1] > 0.0 && full[2] > 0.0,
+             MH_ESTATE, w + ": cell edges must be > 0 (upload the framework first)");
     if (p->ewald) {
-        MH_CHECK(p->ewald_kmax >= 1 && p->ewald_kmax <= GKMAX, MH_EINVAL,
-                 "mh_gcmc_create: ewald_kmax out of range [1,15]");
-        MH_CHECK(p->ewald_kappa > 0.0, MH_EINVAL, "mh_gcmc_create: ewald_kappa must be > 0");
+        MH_CHECK(p->ewald_kmax >= 1 && p->ewald_kmax <= GKMAX, MH_EINVAL, w + ": ewald_kmax out of range [1,15]");
+        MH_CHECK(p->ewald_kappa > 0.0, MH_EINVAL, w + ": ewald_kappa must be > 0");
     }
-    MH_HIP(hipSetDevice(ctx->device));
-    mh_gcmc *g = new mh_gcmc();
-    g->ctx = ctx;
-    g->nrep = nrep;
-    g->maxN = maxN;
-    g->ns = p->nsites;
-    g->prec = p->precision;
-    GDev &H = g->host;
+    GDev &H = S.H;
     std::memset(&H, 0, sizeof H);
     H.ns = p->nsites;
     for (int a = 0; a < H.ns; a++) {
         for (int i = 0; i < 3; i++) H.ref[a][i] = p->ref[a][i];
         H.st[a] = p->site_type[a];
         H.q[a] = p->site_q[a];
-        if (p->site_type[a] < 0 || p->site_type[a] >= p->ntypes) {
-            delete g;
-            set_error("mh_gcmc_create: site_type out of range");
-            return MH_EINVAL;
-        }
+        MH_CHECK(p->site_type[a] >= 0 && p->site_type[a] < p->ntypes, MH_EINVAL, w + ": site_type out of range");
         const mh_map *maps[2] = {p->site_pmap[a], p->site_emap[a]};
-        for (int w = 0; w < 2; w++) {
-            if (!maps[w]) continue;
-            MapRef &m = w == 0 ? H.pm[a] : H.em[a];
-            (w == 0 ? H.has_p : H.has_e)[a] = 1;
-            int rcd = map_dense(ctx, const_cast<mh_map *>(maps[w]), &m.dense);
-            if (rcd) {
-                delete g;
-                return rcd;
-            }
+        for (int k = 0; k < 2; k++) {
+            if (!maps[k]) continue;
+            MapRef &m = k == 0 ? H.pm[a] : H.em[a];
+            (k == 0 ? H.has_p : H.has_e)[a] = 1;
+            int rc = map_dense(ctx, const_cast<mh_map *>(maps[k]), &m.dense);
+            if (rc) return rc;
             for (int i = 0; i < 3; i++) {
-                m.n[i] = maps[w]->npnts[i];
-                m.step[i] = maps[w]->step[i];
-                m.ell[i] = ctx->fund_ell[i];
+                m.n[i] = maps[k]->npnts[i];
+                m.step[i] = maps[k]->step[i];
+                m.ell[i] = fund[i];
             }
         }
     }
@@ -1273,17 +1321,18 @@ int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double 
         H.on[t] = p->lj_on[t] ? 1 : 0;
     }
     for (int i = 0; i < 3; i++) {
-        H.full[i] = ctx->full_ell[i];
-        H.fund[i] = ctx->fund_ell[i];
+        H.full[i] = full[i];
+        H.fund[i] = fund[i];
     }
     H.ewald = p->ewald ? 1 : 0;
     H.kmax = p->ewald ? p->ewald_kmax : 0;
     H.kappa = p->ewald_kappa;
     H.rc2 = p->ewald_rcut * p->ewald_rcut;
     // k vectors of the half space (mx > 0, or mx = 0 & my > 0, or mx = my = 0 & mz > 0)
-    std::vector<double> w;
+    std::vector<double> wv;
     std::vector<int8_t> ki;
     const double V = H.full[0] * H.full[1] * H.full[2];
+    S.volume = V;
     if (H.ewald) {
         int K = H.kmax;
         for (int mx = 0; mx <= K; mx++)
@@ -1293,7 +1342,7 @@ int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double 
                     double kx = ((double)mx / H.full[0]) * kTwoPiR, ky = ((double)my / H.full[1]) * kTwoPiR,
                            kz = ((double)mz / H.full[2]) * kTwoPiR;
                     double k2 = kx * kx + ky * ky + kz * kz;
-                    w.push_back((4.0 * M_PI / V) * std::exp(-k2 / (4.0 * H.kappa * H.kappa)) / k2 * kTokcalG);
+                    wv.push_back((4.0 * M_PI / V) * std::exp(-k2 / (4.0 * H.kappa * H.kappa)) / k2 * kTokcalG);
                     ki.push_back((int8_t)mx);
                     ki.push_back((int8_t)my);
                     ki.push_back((int8_t)mz);
@@ -1310,13 +1359,13 @@ int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double 
             }
         H.molconst = mc * kTokcalG;
     }
-    H.nk = (int)w.size();
-    g->nk = H.nk;
+    H.nk = (int)wv.size();
     H.nmoves = p->nmoves;
     double wsum = 0.0, wi = 0.0, wd = 0.0;
     for (int i = 0; i < p->nmoves; i++) wsum += p->move_weight[i];
     double c = 0.0;
     for (int i = 0; i < p->nmoves; i++) {
+        MH_CHECK(p->move_type[i] >= 0 && p->move_type[i] <= 3, MH_EINVAL, w + ": bad move type");
         H.mtype[i] = p->move_type[i];
         double wn = p->move_weight[i] / wsum;
         c = c + wn;
@@ -1334,56 +1383,95 @@ int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double 
     H.inv_ns = 1.0 / H.ns;
     H.inv_nss = 1.0 / (H.ns * H.ns);
     int rc;
-    auto fail = [&](int code) {
-        delete g;
-        return code;
-    };
-    if ((rc = g->wk.alloc(sizeof(double) * (H.nk + 1))) || (rc = g->kidx.alloc(4 * (H.nk + 1))))
-        return fail(rc);
+    if ((rc = S.wk.alloc(sizeof(double) * (H.nk + 1))) || (rc = S.kidx.alloc(4 * (H.nk + 1)))) return rc;
     if (H.nk) {
-        if (hipMemcpy(g->wk.p, w.data(), sizeof(double) * H.nk, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(g->kidx.p, ki.data(), 4 * H.nk, hipMemcpyHostToDevice) != hipSuccess)
-            return fail(MH_EHIP);
+        MH_HIP(hipMemcpy(S.wk.p, wv.data(), sizeof(double) * H.nk, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(S.kidx.p, ki.data(), 4 * H.nk, hipMemcpyHostToDevice));
     }
-    H.wk = g->wk.as<double>();
-    H.kidx = g->kidx.as<int8_t>();
-    if ((rc = g->dP.alloc(sizeof(GDev)))) return fail(rc);
-    if (hipMemcpy(g->dP.p, &H, sizeof(GDev), hipMemcpyHostToDevice) != hipSuccess) return fail(MH_EHIP);
-    const long long R = nrep;
+    H.wk = S.wk.as<double>();
+    H.kidx = S.kidx.as<int8_t>();
+    S.used = true;
+    return MH_OK;
+}
+
+// replica state buffers for nrep replicas
+static int alloc_replicas(mh_gcmc *g) {
+    const long long R = g->nrep, maxN = g->maxN;
+    int rc;
     if ((rc = g->N.alloc(4 * R)) || (rc = g->err.alloc(4 * R)) || (rc = g->ix.alloc(4 * R)) ||
         (rc = g->iy.alloc(4 * R)) || (rc = g->first.alloc(4 * R)) || (rc = g->iseed.alloc(4 * R)) ||
         (rc = g->B.alloc(8 * R)) || (rc = g->energy.alloc(8 * R)) || (rc = g->sumN.alloc(8 * R)) ||
         (rc = g->nsteps.alloc(8 * R)) || (rc = g->att.alloc(32 * R)) || (rc = g->acc.alloc(32 * R)) ||
         (rc = g->com.alloc(sizeof(double) * 3 * R * maxN)) || (rc = g->ang.alloc(sizeof(double) * 3 * R * maxN)) ||
         (rc = g->pos.alloc(sizeof(double) * 3 * R * maxN * g->ns)) ||
-        (rc = g->S.alloc(sizeof(double2) * R * (H.nk + 1))) || (rc = g->dS.alloc(sizeof(double2) * R * (H.nk + 1))) ||
-        (rc = g->tr_move.alloc(4)) || (rc = g->tr_acc.alloc(4)) || (rc = g->tr_N.alloc(4)) || (rc = g->tr_dE.alloc(8)))
-        return fail(rc);
-    std::vector<int32_t> zi(R, 0), hiseed(seeds, seeds + R), ones(R, 1);
-    std::vector<double> hB(R), zd(R, 0.0);
-    std::vector<long long> zl(4 * R, 0);
-    const double sivolume = V * 1.0e-30 * 6.0221367e23;
-    for (int i = 0; i < nrep; i++) {
-        MH_CHECK(fugacity_kPa[i] > 0.0, MH_EINVAL, "mh_gcmc_create: fugacity must be > 0");
-        hB[i] = std::log(fugacity_kPa[i] * 1.0e3 * sivolume / (8.314 * p->temperature)) - std::log(1.0);
+        (rc = g->S.alloc(sizeof(double2) * (R * g->kstride + 1))) ||
+        (rc = g->dS.alloc(sizeof(double2) * (R * g->kstride + 1))) || (rc = g->tr_move.alloc(4)) ||
+        (rc = g->tr_acc.alloc(4)) || (rc = g->tr_N.alloc(4)) || (rc = g->tr_dE.alloc(8)))
+        return rc;
+    MH_HIP(hipMemset(g->S.p, 0, g->S.n));
+    MH_HIP(hipMemset(g->err.p, 0, g->err.n));
+    MH_HIP(hipMemset(g->N.p, 0, g->N.n));
+    return MH_OK;
+}
+
+// fresh chain state of replicas [r0, r0 + n): N = 0, RNG from seeds, B from
+// the fugacity and the system volume (gcmc_genfuglist, gcmc.F90:303-319)
+static int init_replicas(mh_gcmc *g, const std::vector<int> &reps, const double *fug, const int32_t *seeds,
+                         double volume, double T) {
+    const double sivolume = volume * 1.0e-30 * 6.0221367e23;
+    for (size_t i = 0; i < reps.size(); i++) {
+        MH_CHECK(fug[i] > 0.0, MH_EINVAL, "gcmc: fugacity must be > 0");
+        const int r = reps[i];
+        const int32_t zero = 0, one = 1;
+        const double hB = std::log(fug[i] * 1.0e3 * sivolume / (8.314 * T)) - std::log(1.0), zd = 0.0;
+        const long long zl4[4] = {0, 0, 0, 0}, zl = 0;
+        MH_HIP(hipMemcpy(g->N.as<int>() + r, &zero, 4, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->err.as<int>() + r, &zero, 4, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->ix.as<int32_t>() + r, &zero, 4, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->iy.as<int32_t>() + r, &zero, 4, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->first.as<int32_t>() + r, &one, 4, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->iseed.as<int32_t>() + r, &seeds[i], 4, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->B.as<double>() + r, &hB, 8, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->energy.as<double>() + r, &zd, 8, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->sumN.as<double>() + r, &zd, 8, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->nsteps.as<long long>() + r, &zl, 8, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->att.as<long long>() + 4 * r, zl4, 32, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->acc.as<long long>() + 4 * r, zl4, 32, hipMemcpyHostToDevice));
+        if (g->kstride)
+            MH_HIP(hipMemset(g->S.as<double2>() + (long long)r * g->kstride, 0, sizeof(double2) * g->kstride));
     }
-    if (hipMemcpy(g->N.p, zi.data(), 4 * R, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g->err.p, zi.data(), 4 * R, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g->ix.p, zi.data(), 4 * R, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g->iy.p, zi.data(), 4 * R, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g->first.p, ones.data(), 4 * R, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g->iseed.p, hiseed.data(), 4 * R, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g->B.p, hB.data(), 8 * R, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g->energy.p, zd.data(), 8 * R, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g->sumN.p, zd.data(), 8 * R, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g->nsteps.p, zl.data(), 8 * R, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g->att.p, zl.data(), 32 * R, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g->acc.p, zl.data(), 32 * R, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(g->S.p, 0, sizeof(double2) * R * (H.nk + 1)) != hipSuccess)
-        return fail(MH_EHIP);
-    plan_lds(g);
+    return MH_OK;
+}
+
+extern "C" {
+
+int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double *fugacity_kPa,
+                   const int32_t *seeds, int maxN, mh_gcmc **out) {
+    MH_CHECK(ctx && p && fugacity_kPa && seeds && out, MH_EINVAL, "mh_gcmc_create: bad arguments");
+    MH_CHECK(nrep >= 1 && nrep <= 65535, MH_EINVAL, "mh_gcmc_create: nrep out of range");
+    MH_CHECK(maxN >= 1 && maxN <= (1 << 20), MH_EINVAL, "mh_gcmc_create: maxN out of range");
+    MH_CHECK(p->precision == MH_PREC_FP64 || p->precision == MH_PREC_FP32, MH_EINVAL,
+             "mh_gcmc_create: bad precision");
+    MH_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<mh_gcmc> g(new mh_gcmc());
+    g->ctx = ctx;
+    g->nrep = nrep;
+    g->maxN = maxN;
+    g->prec = p->precision;
+    g->sys.resize(1);
+    int rc;
+    if ((rc = build_system(ctx, p, maxN, ctx->fund_ell, ctx->full_ell, g->sys[0], "mh_gcmc_create"))) return rc;
+    g->ns = p->nsites;
+    g->nk = g->sys[0].H.nk;
+    g->kstride = g->nk;
+    if ((rc = g->dP.alloc(sizeof(GDev)))) return rc;
+    if ((rc = alloc_replicas(g.get()))) return rc;
+    std::vector<int> reps(nrep);
+    for (int i = 0; i < nrep; i++) reps[i] = i;
+    if ((rc = init_replicas(g.get(), reps, fugacity_kPa, seeds, g->sys[0].volume, p->temperature))) return rc;
+    if ((rc = plan(g.get()))) return rc;
     ctx_retain(ctx);
-    *out = g;
+    *out = g.release();
     return MH_OK;
 }
 
@@ -1401,18 +1489,157 @@ int mh_gcmc_set_threads(mh_gcmc *g, int threads) {
     MH_CHECK(g, MH_EINVAL, "mh_gcmc_set_threads: g is NULL");
     MH_CHECK(threads == 64 || threads == 128 || threads == 256 || threads == 512 || threads == 1024,
              MH_EINVAL, "mh_gcmc_set_threads: threads must be 64, 128, 256, 512 or 1024");
+    MH_HIP(hipSetDevice(g->ctx->device));
+    MH_HIP(hipStreamSynchronize(g->ctx->stream));
     g->threads = threads;
-    plan_lds(g);
+    return plan(g);
+}
+
+/* ---------------------------------------------------------- slot pools */
+
+int mh_pool_create(mh_ctx *ctx, int nslots, int nsys, int maxN, int kmax_cap, int max_sites, int precision,
+                   mh_gcmc **out) {
+    MH_CHECK(ctx && out, MH_EINVAL, "mh_pool_create: bad arguments");
+    MH_CHECK(nslots >= 1 && nslots <= 65535 && nsys >= 1 && nsys <= nslots, MH_EINVAL,
+             "mh_pool_create: need 1 <= nsys <= nslots <= 65535");
+    MH_CHECK(maxN >= 1 && maxN <= (1 << 20), MH_EINVAL, "mh_pool_create: maxN out of range");
+    MH_CHECK(kmax_cap >= 0 && kmax_cap <= GKMAX, MH_EINVAL, "mh_pool_create: kmax_cap out of range [0,15]");
+    MH_CHECK(max_sites >= 1 && max_sites <= GS, MH_EINVAL, "mh_pool_create: max_sites out of range [1,8]");
+    MH_CHECK(precision == MH_PREC_FP64 || precision == MH_PREC_FP32, MH_EINVAL, "mh_pool_create: bad precision");
+    MH_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<mh_gcmc> g(new mh_gcmc());
+    g->ctx = ctx;
+    g->pool = true;
+    g->nrep = nslots;
+    g->maxN = maxN;
+    g->prec = precision;
+    g->ns = max_sites;
+    const long long K = kmax_cap;
+    g->kstride = K ? ((2 * K + 1) * (2 * K + 1) * (2 * K + 1) - 1) / 2 : 0;
+    g->sys.resize(nsys);
+    g->slot_sys.assign(nslots, -1);
+    int rc;
+    if ((rc = g->dP.alloc(sizeof(GDev) * nsys))) return rc;
+    if ((rc = alloc_replicas(g.get()))) return rc;
+    if ((rc = g->sysidx.alloc(4 * (size_t)nslots)) || (rc = g->budget.alloc(4 * (size_t)nslots))) return rc;
+    std::vector<int> neg(nslots, -1);
+    MH_HIP(hipMemcpy(g->sysidx.p, neg.data(), 4 * (size_t)nslots, hipMemcpyHostToDevice));
+    MH_HIP(hipMemset(g->budget.p, 0, 4 * (size_t)nslots));
+    if ((rc = plan(g.get()))) return rc;
+    ctx_retain(ctx);
+    *out = g.release();
     return MH_OK;
+}
+
+int mh_pool_admit(mh_gcmc *g, const mh_gcmc_params *p, const double fund_ell[3], const double full_ell[3],
+                  int nrep, const double *fugacity_kPa, const int32_t *seeds, int nsteps, int *ticket) {
+    MH_CHECK(g && g->pool && p && fund_ell && full_ell && fugacity_kPa && seeds && ticket, MH_EINVAL,
+             "mh_pool_admit: bad arguments");
+    MH_CHECK(nrep >= 1 && nsteps >= 0, MH_EINVAL, "mh_pool_admit: bad nrep / nsteps");
+    MH_CHECK(p->nsites <= g->ns, MH_EINVAL, "mh_pool_admit: more sites than the pool's max_sites");
+    MH_HIP(hipSetDevice(g->ctx->device));
+    MH_HIP(hipStreamSynchronize(g->ctx->stream));  // no launch in flight while slots change
+    int s = -1;
+    for (size_t i = 0; i < g->sys.size(); i++)
+        if (!g->sys[i].used) {
+            s = (int)i;
+            break;
+        }
+    std::vector<int> reps;
+    for (int i = 0; i < g->nrep && (int)reps.size() < nrep; i++)
+        if (g->slot_sys[i] < 0) reps.push_back(i);
+    if (s < 0 || (int)reps.size() < nrep) {
+        set_error("mh_pool_admit: no free system or slots (retry after a release)");
+        return MH_ECAPACITY;
+    }
+    SysHost &S = g->sys[s];
+    mh_gcmc_params q = *p;
+    q.precision = g->prec;
+    int rc = build_system(g->ctx, &q, g->maxN, fund_ell, full_ell, S, "mh_pool_admit");
+    if (!rc && S.H.nk > g->kstride) {
+        set_error("mh_pool_admit: more k vectors than the pool's kmax_cap");
+        rc = MH_ECAPACITY;
+    }
+    if (rc) {
+        S = SysHost();
+        return rc;
+    }
+    if ((rc = init_replicas(g, reps, fugacity_kPa, seeds, S.volume, p->temperature))) {
+        S = SysHost();
+        return rc;
+    }
+    S.slots = reps;
+    std::vector<int> bud(nrep, nsteps);
+    for (int i = 0; i < nrep; i++) {
+        g->slot_sys[reps[i]] = s;
+        MH_HIP(hipMemcpy(g->sysidx.as<int>() + reps[i], &s, 4, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->budget.as<int>() + reps[i], &bud[i], 4, hipMemcpyHostToDevice));
+    }
+    if ((rc = plan(g))) return rc;
+    *ticket = s;
+    return MH_OK;
+}
+
+int mh_pool_launch(mh_gcmc *g, int nsteps) {
+    MH_CHECK(g && g->pool && nsteps > 0, MH_EINVAL, "mh_pool_launch: bad arguments");
+    MH_HIP(hipSetDevice(g->ctx->device));
+    launch_any(g, nsteps);
+    MH_HIP(hipGetLastError());
+    return MH_OK;
+}
+
+int mh_pool_wait(mh_gcmc *g) {
+    MH_CHECK(g && g->pool, MH_EINVAL, "mh_pool_wait: bad arguments");
+    MH_HIP(hipSetDevice(g->ctx->device));
+    MH_HIP(hipStreamSynchronize(g->ctx->stream));
+    return MH_OK;
+}
+
+int mh_pool_status(mh_gcmc *g, int ticket, int *nrep, int *slots, int *remaining, int *err, int32_t *N,
+                   double *energy, double *sumN, int64_t *nsteps, int64_t *att, int64_t *acc) {
+    MH_CHECK(g && g->pool && ticket >= 0 && ticket < (int)g->sys.size() && g->sys[ticket].used && nrep,
+             MH_EINVAL, "mh_pool_status: bad ticket");
+    MH_HIP(hipSetDevice(g->ctx->device));
+    const std::vector<int> &reps = g->sys[ticket].slots;
+    *nrep = (int)reps.size();
+    for (size_t i = 0; i < reps.size(); i++) {
+        const int r = reps[i];
+        if (slots) slots[i] = r;
+        if (remaining) MH_HIP(hipMemcpy(remaining + i, g->budget.as<int>() + r, 4, hipMemcpyDeviceToHost));
+        if (err) MH_HIP(hipMemcpy(err + i, g->err.as<int>() + r, 4, hipMemcpyDeviceToHost));
+        if (N) MH_HIP(hipMemcpy(N + i, g->N.as<int>() + r, 4, hipMemcpyDeviceToHost));
+        if (energy) MH_HIP(hipMemcpy(energy + i, g->energy.as<double>() + r, 8, hipMemcpyDeviceToHost));
+        if (sumN) MH_HIP(hipMemcpy(sumN + i, g->sumN.as<double>() + r, 8, hipMemcpyDeviceToHost));
+        if (nsteps) MH_HIP(hipMemcpy(nsteps + i, g->nsteps.as<long long>() + r, 8, hipMemcpyDeviceToHost));
+        if (att) MH_HIP(hipMemcpy(att + 4 * i, g->att.as<long long>() + 4 * r, 32, hipMemcpyDeviceToHost));
+        if (acc) MH_HIP(hipMemcpy(acc + 4 * i, g->acc.as<long long>() + 4 * r, 32, hipMemcpyDeviceToHost));
+    }
+    return MH_OK;
+}
+
+int mh_pool_release(mh_gcmc *g, int ticket) {
+    MH_CHECK(g && g->pool && ticket >= 0 && ticket < (int)g->sys.size() && g->sys[ticket].used, MH_EINVAL,
+             "mh_pool_release: bad ticket");
+    MH_HIP(hipSetDevice(g->ctx->device));
+    MH_HIP(hipStreamSynchronize(g->ctx->stream));
+    const int neg = -1, zero = 0;
+    for (int r : g->sys[ticket].slots) {
+        g->slot_sys[r] = -1;
+        MH_HIP(hipMemcpy(g->sysidx.as<int>() + r, &neg, 4, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->budget.as<int>() + r, &zero, 4, hipMemcpyHostToDevice));
+    }
+    g->sys[ticket] = SysHost();
+    return plan(g);
 }
 
 int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const double *ang) {
     MH_CHECK(g && rep >= 0 && rep < g->nrep, MH_EINVAL, "mh_gcmc_set_config: bad replica");
     MH_CHECK(n >= 0 && n <= g->maxN && (n == 0 || com), MH_EINVAL, "mh_gcmc_set_config: bad n");
     MH_HIP(hipSetDevice(g->ctx->device));
-    const int ns = g->ns;
+    MH_CHECK(!g->pool || g->slot_sys[rep] >= 0, MH_EINVAL, "mh_gcmc_set_config: free slot");
+    const GDev &H = g->sys_of(rep);
+    const int ns = H.ns;
     std::vector<double> hc(3 * (size_t)n), ha(3 * (size_t)n, 0.0), hp(3 * (size_t)n * ns);
-    const GDev &H = g->host;
     for (int m = 0; m < n; m++) {
         for (int i = 0; i < 3; i++) {
             hc[3 * m + i] = com[3 * m + i];
@@ -1438,7 +1665,7 @@ int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const doub
                 hp[(3 * m * ns) + 3 * a + i] = rp - std::floor(rp / H.full[i]) * H.full[i];
             }
     }
-    const long long off3 = (long long)rep * g->maxN * 3, offp = (long long)rep * g->maxN * ns * 3;
+    const long long off3 = (long long)rep * g->maxN * 3, offp = (long long)rep * g->maxN * g->ns * 3;
     if (n) {
         MH_HIP(hipMemcpy(g->com.as<double>() + off3, hc.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->ang.as<double>() + off3, ha.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
@@ -1456,8 +1683,7 @@ int mh_gcmc_run(mh_gcmc *g, int nsteps) {
     if (nsteps == 0) return MH_OK;
     MH_HIP(hipSetDevice(g->ctx->device));
     KTimer t(g->ctx, g->prec == MH_PREC_FP32 ? "gcmc32" : "gcmc64");
-    if (g->prec == MH_PREC_FP32) launch_gcmc<MH_PREC_FP32>(g, nsteps);
-    else launch_gcmc<MH_PREC_FP64>(g, nsteps);
+    launch_any(g, nsteps);
     MH_HIP(hipGetLastError());
     MH_HIP(hipStreamSynchronize(g->ctx->stream));
     t.stop();
@@ -1506,7 +1732,9 @@ int mh_gcmc_get_config(mh_gcmc *g, int rep, int *n, double *com, double *ang, do
     const long long off3 = (long long)rep * g->maxN * 3, offp = (long long)rep * g->maxN * g->ns * 3;
     if (hn && com) MH_HIP(hipMemcpy(com, g->com.as<double>() + off3, sizeof(double) * 3 * hn, hipMemcpyDeviceToHost));
     if (hn && ang) MH_HIP(hipMemcpy(ang, g->ang.as<double>() + off3, sizeof(double) * 3 * hn, hipMemcpyDeviceToHost));
-    if (hn && r) MH_HIP(hipMemcpy(r, g->pos.as<double>() + offp, sizeof(double) * 3 * hn * g->ns, hipMemcpyDeviceToHost));
+    if (hn && r)
+        MH_HIP(hipMemcpy(r, g->pos.as<double>() + offp, sizeof(double) * 3 * hn * g->sys_of(rep).ns,
+                         hipMemcpyDeviceToHost));
     return MH_OK;
 }
 
@@ -1518,8 +1746,9 @@ int mh_gcmc_total_energy(mh_gcmc *g, double *e, int refresh) {
     int rc;
     const size_t R = g->nrep;
     if (g->nk && !refresh) {
-        if ((rc = save.alloc(sizeof(double2) * R * (g->nk + 1)))) return rc;
-        MH_HIP(hipMemcpyAsync(save.p, g->S.p, sizeof(double2) * R * g->nk, hipMemcpyDeviceToDevice, g->ctx->stream));
+        if ((rc = save.alloc(sizeof(double2) * (R * g->kstride + 1)))) return rc;
+        MH_HIP(hipMemcpyAsync(save.p, g->S.p, sizeof(double2) * R * g->kstride, hipMemcpyDeviceToDevice,
+                              g->ctx->stream));
     }
     if (g->nk) {
         hipLaunchKernelGGL(k_sk_recompute, dim3((g->nk + 127) / 128, g->nrep), dim3(128), 0, g->ctx->stream,
@@ -1532,7 +1761,8 @@ int mh_gcmc_total_energy(mh_gcmc *g, double *e, int refresh) {
     MH_HIP(hipGetLastError());
     MH_HIP(hipMemcpyAsync(e, out.p, 8 * R, hipMemcpyDeviceToHost, g->ctx->stream));
     if (g->nk && !refresh)
-        MH_HIP(hipMemcpyAsync(g->S.p, save.p, sizeof(double2) * R * g->nk, hipMemcpyDeviceToDevice, g->ctx->stream));
+        MH_HIP(hipMemcpyAsync(g->S.p, save.p, sizeof(double2) * R * g->kstride, hipMemcpyDeviceToDevice,
+                              g->ctx->stream));
     if (refresh) MH_HIP(hipMemcpyAsync(g->energy.p, out.p, 8 * R, hipMemcpyDeviceToDevice, g->ctx->stream));
     MH_HIP(hipStreamSynchronize(g->ctx->stream));
     return MH_OK;
@@ -1604,9 +1834,10 @@ int mh_gcmc_autotune(mh_gcmc *g, int probe_steps, int *threads_out, int *exy_out
     MH_HIP(hipSetDevice(g->ctx->device));
     hipStream_t st = g->ctx->stream;
     // snapshot of every buffer a run modifies
-    DBuf *bufs[] = {&g->N, &g->err, &g->ix, &g->iy, &g->first, &g->iseed, &g->energy, &g->sumN,
-                    &g->nsteps, &g->att, &g->acc, &g->com, &g->ang, &g->pos, &g->S};
-    const int nb = (int)(sizeof(bufs) / sizeof(bufs[0]));
+    std::vector<DBuf *> bufs = {&g->N, &g->err, &g->ix, &g->iy, &g->first, &g->iseed, &g->energy, &g->sumN,
+                                &g->nsteps, &g->att, &g->acc, &g->com, &g->ang, &g->pos, &g->S};
+    if (g->pool) bufs.push_back(&g->budget);
+    const int nb = (int)bufs.size();
     std::vector<DBuf> snap(nb);
     int rc;
     for (int i = 0; i < nb; i++) {
@@ -1631,13 +1862,12 @@ int mh_gcmc_autotune(mh_gcmc *g, int probe_steps, int *threads_out, int *exy_out
         for (int xo = 0; xo < 2; xo++) {
             g->threads = t;
             g->exy_off = xo != 0;
-            plan_lds(g);
-            if (xo && !g->host.ewald) continue;
+            if ((rc = plan(g))) break;
+            if (xo && g->nk == 0) continue;
             if ((rc = restore())) break;
             for (int rep = 0; rep < 2; rep++) {  // first launch warms the code object
                 (void)hipEventRecord(e0, st);
-                if (g->prec == MH_PREC_FP32) launch_gcmc<MH_PREC_FP32>(g, probe_steps);
-                else launch_gcmc<MH_PREC_FP64>(g, probe_steps);
+                launch_any(g, probe_steps);
                 (void)hipEventRecord(e1, st);
                 if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
                     rc = MH_EHIP;
@@ -1660,7 +1890,7 @@ int mh_gcmc_autotune(mh_gcmc *g, int probe_steps, int *threads_out, int *exy_out
     g->tr_cap = tr_cap;
     g->threads = best_t;
     g->exy_off = best_exy_off;
-    plan_lds(g);
+    (void)plan(g);
     int rc2 = restore();
     if (rc) {
         set_error("mh_gcmc_autotune: a probe launch failed");
@@ -1668,7 +1898,11 @@ int mh_gcmc_autotune(mh_gcmc *g, int probe_steps, int *threads_out, int *exy_out
     }
     if (rc2) return rc2;
     if (threads_out) *threads_out = best_t;
-    if (exy_out) *exy_out = g->host.use_exy;
+    if (exy_out) {
+        *exy_out = 0;
+        for (auto &s : g->sys)
+            if (s.used && s.H.use_exy) *exy_out = 1;
+    }
     return MH_OK;
 }
 

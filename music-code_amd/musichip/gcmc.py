@@ -48,6 +48,12 @@ def _sig():
         "mh_gcmc_get_trace": [_vp, _i, _vp, _vp, _vp, _vp, P(_i)],
         "mh_trial_energy": [_vp, _i, _i, _vp, _vp, _vp, _vp],
         "mh_gcmc_autotune": [_vp, _i, P(_i), P(_i)],
+        "mh_pool_create": [_vp, _i, _i, _i, _i, _i, _i, P(_vp)],
+        "mh_pool_admit": [_vp, _vp, _vp, _vp, _i, _vp, _vp, _i, P(_i)],
+        "mh_pool_launch": [_vp, _i],
+        "mh_pool_wait": [_vp],
+        "mh_pool_status": [_vp, _i, P(_i), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+        "mh_pool_release": [_vp, _i],
     }
     for n, a in sig.items():
         f = getattr(L, n)
@@ -203,6 +209,68 @@ class GcmcBatch:
             self.close()
         except Exception:
             pass
+
+
+class Pool(GcmcBatch):
+    """Slot pool for continuous batching (mh_pool_*): requests of different
+    systems share one persistent launch; admitted and released between
+    launches."""
+
+    def __init__(self, ctx, nslots, nsys, maxN=256, kmax_cap=0, max_sites=1, precision=H.MH_PREC_FP32,
+                 threads=None):
+        L = _sig()
+        self.ctx = ctx
+        self.nrep = nslots
+        self.ns = max_sites
+        self.maxN = maxN
+        self.keep = {}  # ticket -> (params, maps) kept alive while admitted
+        h = _vp()
+        H.check(L.mh_pool_create(ctx.h, nslots, nsys, maxN, kmax_cap, max_sites, precision, ctypes.byref(h)),
+                "mh_pool_create")
+        self.h = h
+        H._LIVE["gcmc"].add(self)
+        if threads:
+            self.set_threads(threads)
+
+    def admit(self, spec, maps, fund_ell, full_ell, fugacities, seeds, nsteps):
+        """Ticket of the admitted request, or None when the pool is full."""
+        params = make_params(spec, maps, H.MH_PREC_FP32)
+        f = np.ascontiguousarray(fugacities, dtype=np.float64)
+        sd = np.ascontiguousarray(seeds, dtype=np.int32)
+        fe = np.ascontiguousarray(fund_ell, dtype=np.float64)
+        fu = np.ascontiguousarray(full_ell, dtype=np.float64)
+        t = ctypes.c_int()
+        rc = _sig().mh_pool_admit(self.h, ctypes.byref(params), _p(fe), _p(fu), len(f), _p(f), _p(sd),
+                                  int(nsteps), ctypes.byref(t))
+        if rc == 6:  # MH_ECAPACITY
+            return None
+        H.check(rc, "mh_pool_admit")
+        self.keep[t.value] = (params, maps)
+        return t.value
+
+    def launch(self, nsteps):
+        H.check(_sig().mh_pool_launch(self.h, int(nsteps)), "mh_pool_launch")
+
+    def wait(self):
+        H.check(_sig().mh_pool_wait(self.h), "mh_pool_wait")
+
+    def status(self, ticket):
+        n = ctypes.c_int()
+        cap = self.nrep
+        sl, rem, err, N = (np.zeros(cap, dtype=np.int32) for _ in range(4))
+        e, sN = np.zeros(cap), np.zeros(cap)
+        ns = np.zeros(cap, dtype=np.int64)
+        att = np.zeros((cap, 4), dtype=np.int64)
+        acc = np.zeros((cap, 4), dtype=np.int64)
+        H.check(_sig().mh_pool_status(self.h, int(ticket), ctypes.byref(n), _p(sl), _p(rem), _p(err), _p(N), _p(e),
+                                      _p(sN), _p(ns), _p(att), _p(acc)), "mh_pool_status")
+        k = n.value
+        return dict(slots=sl[:k], remaining=rem[:k], err=err[:k], N=N[:k], energy=e[:k], sumN=sN[:k],
+                    nsteps=ns[:k], att=att[:k], acc=acc[:k])
+
+    def release(self, ticket):
+        H.check(_sig().mh_pool_release(self.h, int(ticket)), "mh_pool_release")
+        self.keep.pop(ticket, None)
 
 
 def adams_B(fug_kPa, volume, T, Zig=1.0):
