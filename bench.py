@@ -230,6 +230,58 @@ def serving_leg(n=512, steps=10000, util=0.7, ncal=64):
     return st
 
 
+def hmc_leg(moves=20, nrep=256):
+    """HMC bursts of flexible n-butane: 256 replicas, 50-200 chains each in
+    the 40 A framework, one move = 10 NVE velocity-Verlet steps of 1 fs with
+    Boltzmann velocities at 300 K; FP32 pair/map arithmetic with FP64
+    integration against the FP64 kernel."""
+    from musichip import hip as H
+    from musichip import hmc as HM
+    from musichip import workloads as W
+    ctx = H.Context(0)
+    w = W.butane_hmc_system(ctx, nrep=nrep, log=log)
+    nch = np.array([len(c) for c in w["configs"]])
+    out = dict(workload="flexible UA n-butane, %d replicas, 50-200 chains (placed %d..%d), 40 A framework "
+                        "(768 atoms, 10 A cells), CH3/CH2 maps at 0.2 A, LJ 12.8 A, 10 VV steps x 1 fs, 300 K" %
+                        (nrep, nch.min(), nch.max()), chains=[int(x) for x in nch])
+    for prec, tag in ((H.MH_PREC_FP32, "fp32"), (H.MH_PREC_FP64, "fp64")):
+        b = HM.HmcBatch(ctx, w["spec"], w["maps"], [5200 + i for i in range(nrep)], maxN=int(nch.max()),
+                        precision=prec)
+        for r, c in enumerate(w["configs"]):
+            b.set_config(r, c)
+        b.run(2)
+        s0 = b.state()
+        t = time.perf_counter()
+        b.run(moves)
+        dt = time.perf_counter() - t
+        s1 = b.state()
+        att = int((s1["att"] - s0["att"]).sum())
+        acc = int((s1["acc"] - s0["acc"]).sum())
+        out[tag + "_moves_per_s"] = att / dt
+        out[tag + "_md_site_steps_per_s"] = float(nch.sum()) * 4 * 10 * moves / dt
+        out[tag + "_acceptance"] = acc / max(att, 1)
+        out[tag + "_failed_moves"] = int((s1["fail"] - s0["fail"]).sum())
+        log("hmc %s: %.3e moves/s, acceptance %.3f" % (tag, att / dt, acc / max(att, 1)))
+        if tag == "fp32":
+            e32 = [b.energy_forces(r) for r in range(0, nrep, 32)]
+            keep = [b.config(r)[0] for r in range(0, nrep, 32)]
+        else:
+            errs_e, errs_f = [], []
+            for k, r in enumerate(range(0, nrep, 32)):
+                b.set_config(r, keep[k])
+                e64, F64, _ = b.energy_forces(r)
+                errs_e.append(abs(e32[k][0] - e64) / max(1.0, abs(e64)))
+                errs_f.append(np.abs(e32[k][1] - F64).max() / np.abs(F64).max())
+            out["fp32_vs_fp64_energy_max_rel_err"] = float(max(errs_e))
+            out["fp32_vs_fp64_force_max_rel_err"] = float(max(errs_f))
+        b.close()
+    out["speedup_fp32_vs_fp64"] = out["fp32_moves_per_s"] / out["fp64_moves_per_s"]
+    for m in w["maps"].values():
+        m.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,6 +296,8 @@ def main():
     ap.add_argument("--no-gcmc", action="store_true")
     ap.add_argument("--gcmc-chains", type=int, default=4)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-hmc", action="store_true")
+    ap.add_argument("--hmc-moves", type=int, default=20)
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (HBM counter calibration)")
@@ -350,6 +404,10 @@ def main():
     if not args.no_gcmc:
         gcmc = gcmc_leg(H, args.gcmc_steps, args.gcmc_fp64_steps, args.gcmc_equil, args.gcmc_chains)
 
+    hmc = None
+    if not args.no_hmc:
+        hmc = hmc_leg(args.hmc_moves)
+
     serving = None
     if not args.no_serve:
         serving = serving_leg(args.serve_requests)
@@ -369,7 +427,7 @@ def main():
                    k: v[0] / args.steps for k, v in ktimes.items()},
                gcmc=gcmc, gcmc_trial_moves_per_s=gcmc and gcmc["fp32_batched_steps_per_s"],
                p50_latency_ms=serving and serving["p50_latency_ms"],
-               p99_latency_ms=serving and serving["p99_latency_ms"], serving=serving,
+               p99_latency_ms=serving and serving["p99_latency_ms"], serving=serving, hmc=hmc,
                roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(out))
 

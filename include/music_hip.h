@@ -213,6 +213,65 @@ int mh_gcmc_get_trace(mh_gcmc *g, int rep, int *move, int *accepted, int *N, dou
 int mh_trial_energy(mh_gcmc *g, int rep, int n, const double *com, const double *ang, double *e,
                     uint8_t *ok);
 
+/* ------------------------------------------------------------------- HMC
+ * Hybrid MC bursts of flexible sorbates in a fixed framework
+ * (hybridmoves_integrate_boltz, hybridmoves.F90:186-241), many replicas per
+ * batch, one workgroup per replica: Gaussian velocities per atom with the
+ * reference RNG and Box-Muller order (hybridmoves_boltzVel,
+ * hybridmoves.F90:247-307; random_gaussian, random.F90:197-217), md_steps
+ * NVE velocity-Verlet steps (velverlet_nveIntegrate, velverlet.F90:148-288),
+ * Metropolis on dU + dKE (mcmoves_perturb, mcmoves.F90:371-426).  Energy:
+ * framework maps with forces (forcehermite, interpolate.F90:189-584),
+ * sorbate-sorbate LJ with minimum image (lj_snglint), harmonic bonds
+ * (harmonic.F90:130-160), harmonic angles (harmonica.F90:100-160),
+ * cosine-series torsions (cosexpansion.F90:88-196, polymer convention).
+ * Units: A, ps, amu; energies kcal/mol; bond k kcal/mol/A^2, angle k
+ * kcal/mol/rad^2 (U = k/2 (x - x0)^2), torsion U = sum_n c[n] cos^n. */
+typedef struct {
+    int nsites;                          /* 1..8 sites per molecule */
+    int site_type[MH_MAX_SITES];         /* index into the LJ table */
+    double site_mass[MH_MAX_SITES];      /* amu */
+    const mh_map *site_map[MH_MAX_SITES];/* LJ map per site or NULL */
+    int ntypes;
+    double lj_A[64], lj_B[64], lj_hicut2[64], lj_locut2[64]; /* [ntypes][ntypes] */
+    uint8_t lj_on[64];
+    int nbonds;
+    int bond[16][2];
+    double bond_k[16], bond_r0[16];
+    int nangles;
+    int angle[16][3];                    /* i-j-k, j the apex */
+    double angle_k[16], angle_theta0[16];/* rad */
+    int ntorsions;
+    int torsion[16][4];
+    double torsion_c[16][6];
+    double temperature;                  /* K */
+    double dt_ps;                        /* time step, ps */
+    int md_steps;                        /* VV steps per move */
+    double max_nrg_md;                   /* kcal/mol, energy check per step (1000 in the reference) */
+    double max_nrg_start;                /* kcal/mol, start configuration (1e6) */
+    int precision;                       /* MH_PREC_FP64 or MH_PREC_FP32 (FP32 pair and map arithmetic) */
+} mh_hmc_params;
+
+typedef struct mh_hmc mh_hmc;
+
+int mh_hmc_create(mh_ctx *ctx, const mh_hmc_params *p, int nrep, const int32_t *seeds, int maxN, mh_hmc **out);
+int mh_hmc_destroy(mh_hmc *h);
+int mh_hmc_set_threads(mh_hmc *h, int threads);
+/* n molecules of a replica, principal (unwrapped) site coordinates
+ * rp[n][nsites][3]; the simulation-cell coordinates are their pbc images. */
+int mh_hmc_set_config(mh_hmc *h, int rep, int n, const double *rp);
+/* nmoves HMC moves on every replica in one launch. */
+int mh_hmc_run(mh_hmc *h, int nmoves);
+/* Per replica: N, potential energy of the current configuration, attempted,
+ * accepted and failed (energy evaluation) moves, dH of the last move
+ * (NaN if it failed), RNG state.  Any pointer may be NULL. */
+int mh_hmc_get_state(mh_hmc *h, int32_t *N, double *energy, int64_t *att, int64_t *acc, int64_t *fail,
+                     double *last_dH, int32_t *rng_ix, int32_t *rng_iy);
+int mh_hmc_get_config(mh_hmc *h, int rep, int *n, double *rp, double *r);
+/* Energy and forces F[n][nsites][3] (kcal/mol/A) of a replica's current
+ * configuration in the batch precision; ok = 0 on a failed evaluation. */
+int mh_hmc_energy_forces(mh_hmc *h, int rep, double *e, double *F, int *ok);
+
 /* ------------------------------------------------------------ slot pools
  * Continuous batching of GCMC requests (no counterpart in the reference,
  * which runs one simulation per process).  A pool is an mh_gcmc handle

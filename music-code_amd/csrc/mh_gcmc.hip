@@ -30,6 +30,7 @@
 
 #include "mh_device.h"
 #include "mh_internal.h"
+#include "mh_mc_common.h"
 
 namespace mh {
 
@@ -38,12 +39,6 @@ constexpr int GKMAX = 15;          // max kmax for the sorbate Ewald
 constexpr double kTwoPiR = 2.0 * 3.1415927;  // defaults.F90:35
 constexpr double kTokcalG = 332.064;
 
-struct MapRef {
-    const float *dense;   // [nx][ny][nz][8], slot 0 NaN where untabulated
-    int n[3];
-    double step[3];
-    double ell[3];
-};
 
 struct GDev {
     int ns;
@@ -90,37 +85,8 @@ struct Rep {
     long long kstride;        // S(k) entries per replica
 };
 
-struct Rng {
-    int32_t ix, iy, first, iseed;
-};
 
-// rranf_NR (random.F90:121-152); am = (1 - 2^-24) 2^-31 (single-precision
-// Nearest(1.0,-1.0)/CONST_IM, exact)
-__device__ __forceinline__ double rranf(Rng &r) {
-    if (r.first) {
-        int32_t a = r.iseed < 0 ? -r.iseed : r.iseed;
-        r.first = 0;
-        r.iy = (int32_t)((888889999 ^ a) | 1);
-        r.ix = (int32_t)(777755555 ^ a);
-        r.iseed = a + 1;
-    }
-    uint32_t x = (uint32_t)r.ix;
-    x ^= x << 13;
-    x ^= x >> 17;
-    x ^= x << 5;
-    r.ix = (int32_t)x;
-    int32_t k = r.iy / 127773;
-    r.iy = 16807 * (r.iy - k * 127773) - 2836 * k;
-    if (r.iy < 0) r.iy += 2147483647;
-    int32_t v = (int32_t)((2147483647 & (r.ix ^ r.iy)) | 1);
-    const double am = (double)(0.99999994039535522461f / 2147483648.0f);
-    return __dmul_rn(am, (double)v);
-}
 
-// simcell_pbc of one coordinate (simcell.F90:521-547)
-__device__ __forceinline__ double wrap_full(double x, double L) {
-    return __dsub_rn(x, __dmul_rn(floor(x / L), L));
-}
 
 // row i of matrix_b2feuler (matrix.F90:383-403) from sin/cos of phi, theta, psi
 __device__ __forceinline__ void euler_row(int i, double sp, double cp, double st, double ct, double ss,
@@ -166,25 +132,7 @@ __device__ void site_coords(const GDev &P, const double com[3], const double ang
     }
 }
 
-// simcell_maptouc + mapheader_getIndices along one axis (simcell.F90:634-668,
-// mapheader.F90:277-299): 0-based lower / upper node (upper wraps to 0 at the
-// last node, interpolate.F90:640-660) and the fractional offset t.
-__device__ __forceinline__ bool cell_axis(double c, double ell, double st, int nb, int &i0, int &i1, double &t) {
-    double f = c * (1.0 / ell);
-    double fr = f - trunc(f);  // == fmod(f, 1.0), exact
-    f = (f >= 0.0) ? fr : fr + 1.0;
-    c = f * ell;
-    const int idx = (int)(c / st + 1.0);
-    if (idx < 1 || idx > nb) return false;
-    t = (c - (double)(idx - 1) * st) / st;
-    i0 = idx - 1;
-    i1 = (idx == nb) ? 0 : idx;
-    return true;
-}
 
-// corner and value-slot patterns of pothermite (interpolate.F90:725-835):
-// bit i = upper node along the axis for corner i / derivative order for slot i
-constexpr int kCX = 0xB2, kCY = 0xD4, kCZ = 0xE8;
 
 // Hermite contribution of one cell corner: sum over the 8 stored values of
 // f[s] hx hy hz (FP64 products in the reference order, or FP32).
@@ -292,23 +240,6 @@ __device__ __forceinline__ bool pair_energy32(const GDev &P, int t, float qq, fl
     return true;
 }
 
-// block reduction of NV doubles per lane, fixed order (lane tree then waves)
-template <int NT, int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double *red /*[NT/64][NV]*/) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int i = 0; i < NV; i++)
-        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off);
-    if (lane == 0)
-        for (int i = 0; i < NV; i++) red[wave * NV + i] = v[i];
-    __syncthreads();
-    for (int i = 0; i < NV; i++) {
-        double s = 0.0;
-        for (int w = 0; w < NT / 64; w++) s += red[w * NV + i];
-        v[i] = s;
-    }
-    __syncthreads();
-}
 
 struct Shared {
     double posA[GS * 3], posB[GS * 3];   // trial (A) and old (B) site coordinates
@@ -1217,7 +1148,7 @@ static void launch_any(mh_gcmc *g, int nsteps) {
 }
 
 // dense copy of a map for the GCMC lookups, built once per map
-static int map_dense(mh_ctx *ctx, mh_map *m, const float **out) {
+int mh::map_dense(mh_ctx *ctx, mh_map *m, const float **out) {
     const long long nodes = (long long)m->npnts[0] * m->npnts[1] * m->npnts[2];
     if (!m->dense.p) {
         int rc;

@@ -96,6 +96,9 @@ struct mh_ctx {
 
 namespace mh {
 void ctx_retain(mh_ctx *ctx);
+// uncompacted node-major copy of a map for the Monte Carlo lookups
+// (built once per map, kept with it)
+int map_dense(mh_ctx *ctx, mh_map *m, const float **out);
 void ctx_release(mh_ctx *ctx);
 // Scratch buffer 'name' of at least 'bytes' bytes owned by the context.
 void *ws_get(mh_ctx *ctx, const std::string &name, size_t bytes, int *rc);

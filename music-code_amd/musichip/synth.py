@@ -90,3 +90,82 @@ def co2_like():
     return dict(name="CO2", ref=np.array([[0.0, 0.0, -1.16], [0.0, 0.0, 0.0], [0.0, 0.0, 1.16]]),
                 types=["O", "C", "O"], q=np.array([-0.35, 0.70, -0.35]),
                 lj={"C": (27.0, 2.80), "O": (79.0, 3.05)}, mass=44.0)
+
+
+def butane_ua():
+    """4-site united-atom n-butane (TraPPE-UA): CH3-CH2-CH2-CH3, LJ CH3
+    98 K / 3.75 A, CH2 46 K / 3.95 A; harmonic bonds r0 = 1.54 A with
+    k/kB = 452,900 K/A^2, harmonic angle theta0 = 114 deg with
+    k/kB = 62,500 K/rad^2 (U = k/2 (x - x0)^2); torsion
+    c1 [1 + cos phi] + c2 [1 - cos 2 phi] + c3 [1 + cos 3 phi] with
+    c1, c2, c3 = 355.03, -68.19, 791.32 K (phi = 180 deg trans), rewritten
+    as sum_n c[n] x^n in the reference's polymer convention x = -cos phi
+    (cosexpansion.F90:142).  Energies in kcal/mol (k_B = 8.314e-3 / 4.184)."""
+    kb = C.KCALMOLE_KB
+    c1, c2, c3 = 355.03, -68.19, 791.32
+    tc = [(c1 + 2 * c2 + c3) * kb, -(c1 - 3 * c3) * kb, -2 * c2 * kb, -4 * c3 * kb, 0.0, 0.0]
+    th0 = np.radians(114.0)
+    b = 1.54
+    d1 = np.array([1.0, 0.0, 0.0])
+    d2 = np.array([np.cos(np.pi - th0), np.sin(np.pi - th0), 0.0])
+    chain = np.array([np.zeros(3), b * d1, b * d1 + b * d2, 2 * b * d1 + b * d2])
+    chain -= chain.mean(0)
+    return dict(name="butane", types=["CH3", "CH2"], stype=[0, 1, 1, 0], mass=[15.035, 14.027, 14.027, 15.035],
+                lj={"CH3": (98.0, 3.75), "CH2": (46.0, 3.95)},
+                bonds=[(0, 1, 452900.0 * kb, b), (1, 2, 452900.0 * kb, b), (2, 3, 452900.0 * kb, b)],
+                angles=[(0, 1, 2, 62500.0 * kb, th0), (1, 2, 3, 62500.0 * kb, th0)],
+                torsions=[(0, 1, 2, 3, tc)], chain=chain)
+
+
+def config5(seed=5000):
+    """Neutral 40 A cubic framework of the methane recipe with 10 A cells
+    (4 x 4 x 4 cells, 768 atoms): open enough for 50-200 n-butane chains
+    (the 8 A-cell variant with 1500 atoms holds about 55)."""
+    rng = np.random.default_rng(seed)
+    xyz = lattice_faces(4, 10.0, rng)
+    eps, sig, typ = fw_types(rng, len(xyz))
+    L = np.array([40.0, 40.0, 40.0])
+    return dict(name="C5", xyz=xyz, type=typ, q=np.zeros(len(xyz)), fw_eps=eps, fw_sig=sig, fund_ell=L,
+                full_ell=L.copy(), hicut=12.8, gridspace=0.2, hipotcut=100.0)
+
+
+def random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    a, b, c, d = q
+    return np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+def place_chains(model, n, L, rng, sites_ok, min_dist=3.0, max_tries=200000):
+    """Principal coordinates [n][ns][3] of n rigid all-trans chains at random
+    positions / orientations: every site passes sites_ok (framework check,
+    called on [k][3] cell positions -> bool[k]) and is at least min_dist
+    (minimum image) from the sites already placed."""
+    L = np.asarray(L, dtype=np.float64)
+    chain = np.asarray(model["chain"])
+    placed = []
+    flat = np.zeros((0, 3))
+    tries = 0
+    while len(placed) < n and tries < max_tries:
+        batch = 256
+        tries += batch
+        cand = []
+        for _ in range(batch):
+            com = rng.uniform(0.0, 1.0, 3) * L
+            cand.append(chain @ random_rotation(rng).T + com)
+        cand = np.array(cand)
+        ok = sites_ok(np.mod(cand, L).reshape(-1, 3)).reshape(batch, -1).all(1)
+        for c, good in zip(cand, ok):
+            if not good or len(placed) >= n:
+                continue
+            if len(flat):
+                d = np.mod(c[:, None, :] - flat[None, :, :] + 0.5 * L, L) - 0.5 * L
+                if (np.einsum("ijk,ijk->ij", d, d) < min_dist * min_dist).any():
+                    continue
+            placed.append(c)
+            flat = np.concatenate([flat, c])
+    if len(placed) < n:
+        raise RuntimeError("placed only %d of %d chains" % (len(placed), n))
+    return np.array(placed)

@@ -397,3 +397,150 @@ class Gcmc:
     def trace_array(self, n=None):
         n = self.st.trace_cap if n is None else n
         return np.array([(t.move, t.accepted, t.N, t.dE) for t in self.trace[:n]])
+
+
+# ---------------------------------------------------------------- HMC
+class HmcSys(ctypes.Structure):
+    _fields_ = [("ns", _i), ("stype", _i * 8), ("mass", _d * 8), ("site_map", _i * 8), ("nmaps", _i),
+                ("map_ptr", _vp * 8), ("map_vals", _vp * 8), ("map_npnts", (_i * 3) * 8),
+                ("map_step", (_d * 3) * 8), ("map_nf", _ll * 8), ("full_ell", _d * 3), ("fund_ell", _d * 3),
+                ("nst", _i), ("lj_A", _d * 64), ("lj_B", _d * 64), ("lj_h2", _d * 64), ("lj_l2", _d * 64),
+                ("lj_on", _i * 64), ("nb", _i), ("bi", (_i * 2) * 16), ("bk", _d * 16), ("br0", _d * 16),
+                ("na", _i), ("ai", (_i * 3) * 16), ("ak", _d * 16), ("at0", _d * 16), ("nt", _i),
+                ("ti", (_i * 4) * 16), ("tc", (_d * 6) * 16), ("T", _d), ("dt", _d), ("nmd", _i),
+                ("max_nrg_md", _d), ("max_nrg0", _d)]
+
+
+class MoRng(ctypes.Structure):
+    _fields_ = [("ix", ctypes.c_int32), ("iy", ctypes.c_int32), ("iseed", ctypes.c_int32),
+                ("first", ctypes.c_int32)]
+
+
+class HmcState(ctypes.Structure):
+    _fields_ = [("N", _i), ("maxN", _i), ("rp", _vp), ("r", _vp), ("v", _vp), ("a", _vp), ("rng", MoRng),
+                ("g_have", _i), ("g_z2", _d), ("energy", _d), ("att", _ll), ("acc", _ll), ("fail", _ll),
+                ("last_dH", _d)]
+
+
+def _hmc_lib():
+    L = lib()
+    if getattr(L, "_hmc_sig", False):
+        return L
+    L.mo_hmc_energy_forces.argtypes = [_vp, _i, _vp, _vp, _vp, ctypes.POINTER(_i)]
+    L.mo_hmc_energy_forces.restype = _d
+    L.mo_hmc_gaussian.argtypes = [_vp, _d, _d]
+    L.mo_hmc_gaussian.restype = _d
+    L.mo_hmc_move.argtypes = [_vp, _vp]
+    L.mo_hmc_move.restype = _i
+    L.mo_hmc_nve.argtypes = [_vp, _i, _vp, _vp, _vp, _i, _vp]
+    L.mo_hmc_nve.restype = _i
+    L._hmc_sig = True
+    return L
+
+
+def hmc_sys(spec):
+    """mo_hmc_sys from an HMC spec (keys: stype, mass, site_map, maps
+    [(npnts, step, ptr, vals8)], full_ell, fund_ell, nst, lj (A, B, h2, l2,
+    on), bonds [(i, j, k, r0)], angles [(i, j, l, k, theta0)], torsions
+    [(i, j, k, l, c[6])], T, dt, nmd, max_nrg_md, max_nrg0)."""
+    s = HmcSys()
+    keep = []
+    s.ns = ns = len(spec["stype"])
+    for a in range(ns):
+        s.stype[a] = int(spec["stype"][a])
+        s.mass[a] = float(spec["mass"][a])
+        s.site_map[a] = int(spec["site_map"][a])
+    s.nmaps = len(spec["maps"])
+    for j, (npn, step, ptr, vals) in enumerate(spec["maps"]):
+        ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+        vals = np.ascontiguousarray(vals, dtype=np.float32)
+        keep += [ptr, vals]
+        s.map_ptr[j] = ptr.ctypes.data
+        s.map_vals[j] = vals.ctypes.data
+        s.map_nf[j] = vals.shape[1]
+        for i in range(3):
+            s.map_npnts[j][i] = int(npn[i])
+            s.map_step[j][i] = float(step[i])
+    for i in range(3):
+        s.full_ell[i] = spec["full_ell"][i]
+        s.fund_ell[i] = spec["fund_ell"][i]
+    s.nst = spec["nst"]
+    A, B, h2, l2, on = spec["lj"]
+    for t in range(s.nst * s.nst):
+        s.lj_A[t], s.lj_B[t], s.lj_h2[t], s.lj_l2[t], s.lj_on[t] = A[t], B[t], h2[t], l2[t], int(on[t])
+    s.nb = len(spec["bonds"])
+    for q, (i, j, k, r0) in enumerate(spec["bonds"]):
+        s.bi[q][0], s.bi[q][1], s.bk[q], s.br0[q] = i, j, k, r0
+    s.na = len(spec["angles"])
+    for q, (i, j, l, k, t0) in enumerate(spec["angles"]):
+        s.ai[q][0], s.ai[q][1], s.ai[q][2], s.ak[q], s.at0[q] = i, j, l, k, t0
+    s.nt = len(spec["torsions"])
+    for q, (i, j, k, l, c) in enumerate(spec["torsions"]):
+        s.ti[q][0], s.ti[q][1], s.ti[q][2], s.ti[q][3] = i, j, k, l
+        for n in range(6):
+            s.tc[q][n] = c[n]
+    s.T, s.dt, s.nmd = spec["T"], spec["dt"], spec["nmd"]
+    s.max_nrg_md = spec.get("max_nrg_md", 1000.0)
+    s.max_nrg0 = spec.get("max_nrg0", 1.0e6)
+    return s, keep
+
+
+class Hmc:
+    """One HMC chain of flexible molecules on the CPU oracle."""
+
+    def __init__(self, spec, seed, maxN=256):
+        L = _hmc_lib()
+        self.sys, self.keep = hmc_sys(spec)
+        ns = self.sys.ns
+        self.ns = ns
+        self.rp = np.zeros((maxN, ns, 3))
+        self.r = np.zeros((maxN, ns, 3))
+        self.v = np.zeros((maxN, ns, 3))
+        self.a = np.zeros((maxN, ns, 3))
+        st = HmcState()
+        st.maxN = maxN
+        st.rp, st.r, st.v, st.a = [x.ctypes.data for x in (self.rp, self.r, self.v, self.a)]
+        L.mo_rng_init(ctypes.byref(st.rng), int(seed))
+        self.st = st
+        self.full = np.asarray(spec["full_ell"], dtype=np.float64)
+
+    def set_config(self, rp):
+        rp = np.asarray(rp, dtype=np.float64).reshape(-1, self.ns, 3)
+        n = len(rp)
+        self.rp[:n] = rp
+        self.r[:n] = rp - np.floor(rp / self.full) * self.full
+        self.st.N = n
+        e, _, ok = self.energy_forces()
+        self.st.energy = e
+        return e, ok
+
+    def energy_forces(self, rp=None, r=None):
+        L = _hmc_lib()
+        n = self.st.N if rp is None else len(rp)
+        rp = self.rp[:n] if rp is None else np.ascontiguousarray(rp, dtype=np.float64)
+        r = self.r[:n] if r is None else np.ascontiguousarray(r, dtype=np.float64)
+        rp = np.ascontiguousarray(rp)
+        r = np.ascontiguousarray(r)
+        F = np.zeros((max(n, 1), self.ns, 3))
+        ok = _i()
+        e = L.mo_hmc_energy_forces(ctypes.byref(self.sys), n, rp.ctypes.data, r.ctypes.data, F.ctypes.data,
+                                   ctypes.byref(ok))
+        return e, F[:n], bool(ok.value)
+
+    def move(self):
+        return _hmc_lib().mo_hmc_move(ctypes.byref(self.sys), ctypes.byref(self.st))
+
+    def gaussian(self, sigma=1.0):
+        return _hmc_lib().mo_hmc_gaussian(ctypes.byref(self.st), 0.0, sigma)
+
+    def nve(self, v, nsteps):
+        """Plain NVE trajectory from velocities v [N][ns][3]; total energy
+        (kcal/mol) after each step."""
+        n = self.st.N
+        rp = np.ascontiguousarray(self.rp[:n].copy())
+        r = np.ascontiguousarray(self.r[:n].copy())
+        v = np.ascontiguousarray(np.asarray(v, dtype=np.float64).copy())
+        et = np.zeros(nsteps + 1)
+        ok = _hmc_lib().mo_hmc_nve(ctypes.byref(self.sys), n, rp.ctypes.data, r.ctypes.data, v.ctypes.data, nsteps,
+                                   et.ctypes.data)
+        return bool(ok), et
