@@ -80,11 +80,15 @@ static __device__ __forceinline__ double erfc_as64(double x) {
     return __dmul_rn(tp, exp(-xsq));
 }
 
+// hardware reciprocal (v_rcp_f32, within 1 ulp) for the FP32 pair paths; the
+// correctly rounded __frcp_rn expands to a ten-instruction division
+static __device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
+
 static __device__ __forceinline__ float erfc_as32(float x) {
 #pragma clang fp contract(fast)
     const float a1 = 0.254829592f, a2 = -0.284496736f, a3 = 1.421413741f, a4 = -1.453152027f,
                 a5 = 1.061405429f, p = 0.3275911f;
-    float t = __frcp_rn(1.0f + p * x);
+    float t = rcp_fast(1.0f + p * x);
     return t * (a1 + t * (a2 + t * (a3 + t * (a4 + t * a5)))) * __expf(-x * x);
 }
 

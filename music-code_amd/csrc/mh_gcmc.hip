@@ -229,7 +229,7 @@ __device__ __forceinline__ bool pair_energy32(const GDev &P, int t, float qq, fl
     const float r2 = sx * sx + sy * sy + sz * sz;
     if (P.on[t] && !(r2 > P.h2f[t])) {
         if (r2 < P.l2f[t]) return false;
-        const float r2i = __frcp_rn(r2);
+        const float r2i = rcp_fast(r2);
         const float r6i = r2i * r2i * r2i;
         e += P.Af[t] * (r6i * r6i) - P.Bf[t] * r6i;
     }

@@ -324,7 +324,7 @@ __device__ void hmc_forces(const HDev &P, const double *rp, const double *r, int
                     bad = 1.0;
                     continue;
                 }
-                const float r2i = __frcp_rn(r2), r6i = r2i * r2i * r2i, r12i = r6i * r6i;
+                const float r2i = rcp_fast(r2), r6i = r2i * r2i * r2i, r12i = r6i * r6i;
                 elf += P.Af[t] * r12i - P.Bf[t] * r6i;
                 const float fm = (12.0f * P.Af[t] * r12i - 6.0f * P.Bf[t] * r6i) * r2i;
                 fxf += x * fm;

@@ -16,12 +16,13 @@
 #include <cmath>
 #include <cstring>
 
+#include "mh_device.h"
 #include "mh_internal.h"
 
 namespace mh {
 
 constexpr int BX = 8, BY = 8, BZ = 4, NT = BX * BY * BZ;
-constexpr int SEG = 1024;  // culled atoms held in LDS per pass
+constexpr int SEG = 512;   // atoms examined (and culled into LDS) per pass
 
 constexpr double kPiR = 3.1415927;  // defaults.F90:33
 constexpr double kTokcal = 332.064;                       // ewald.F90:60
@@ -166,41 +167,49 @@ __device__ __forceinline__ void ewald_real_pair64(double kappa, double charge1, 
 
 // FP32 Ewald real-space pair: same expression tree in single precision,
 // returns the 8 terms (t[0] potential, t[1..7] derivatives)
-__device__ __forceinline__ void ewald_real_term32(float kappa, float qq, float sx, float sy,
-                                                  float sz, float r2, float *t) {
+// FP32 Ewald real-space pair (ewald_rwaldHOT terms, ewald.F90:875-916) with
+// u = erfc(kr)/r and g = (2 kappa / sqrt(pi)) e^{-k^2 r^2}:
+//   V = q u, (1/r) dV/dr = -q (u + g) / r^2,
+//   d2 terms  q (3 (u + g) + 2 k^2 r^2 g) / r^4,
+//   d3 term  -q (15 (u + g) + (10 k^2 r^2 + 4 k^4 r^4) g) / r^6;
+// the potential goes straight into the FP64 sum (terms cancel strongly),
+// the 7 derivatives accumulate into the FP32 tile sums with FMAs.
+__device__ __forceinline__ void ewald_real_acc32(float kappa, float qq, float sx, float sy, float sz, float r2,
+                                                 double &S0, float *hf) {
 #pragma clang fp contract(fast)
-    const float risqrtpi = 0.5641896f;
+    const float twoksp = 2.0f * 0.5641896f * kappa;
     const float a1 = 0.254829592f, a2 = -0.284496736f, a3 = 1.421413741f, a4 = -1.453152027f,
                 a5 = 1.061405429f, p = 0.3275911f;
-    float ri1 = __frsqrt_rn(r2);
-    float dist = r2 * ri1;
-    float kr = kappa * dist;
-    float tt = __frcp_rn(1.0f + p * kr);
-    float ek = __expf(-kr * kr);
-    float erfkr = tt * (a1 + tt * (a2 + tt * (a3 + tt * (a4 + tt * a5)))) * ek;
-    float ri2 = ri1 * ri1, ri3 = ri2 * ri1, ri4 = ri3 * ri1, ri5 = ri4 * ri1;
-    float ri6 = ri5 * ri1, ri7 = ri6 * ri1;
-    float k3 = kappa * kappa * kappa;
-    float ekr = risqrtpi * ek;
-    t[0] = qq * erfkr * ri1;
-    float t1 = qq * (-ri3 * erfkr - 2.0f * kappa * ri2 * ekr);
-    t[1] = sx * t1;
-    t[2] = sy * t1;
-    t[3] = sz * t1;
-    float t2 = qq * (3.0f * ri5 * erfkr + (6.0f * kappa * ri4 + 4.0f * k3 * ri2) * ekr);
-    t[4] = t2 * sx * sy;
-    t[5] = t2 * sx * sz;
-    t[6] = t2 * sy * sz;
-    float t3 = qq * (-15.0f * ri7 * erfkr -
-                     (30.0f * kappa * ri6 + 20.0f * k3 * ri4 + 8.0f * k3 * kappa * kappa * ri2) * ekr);
-    t[7] = t3 * sx * sy * sz;
+    const float ri1 = __frsqrt_rn(r2);
+    const float kr = kappa * (r2 * ri1);
+    const float tt = rcp_fast(1.0f + p * kr);
+    const float ek = __expf(-kr * kr);
+    const float qek = qq * ek;
+    const float u = tt * (a1 + tt * (a2 + tt * (a3 + tt * (a4 + tt * a5)))) * qek * ri1;  // q erfc(kr)/r
+    const float g = twoksp * qek;
+    const float w = ri1 * ri1;
+    const float k2r2 = kr * kr;
+    const float s1 = u + g;
+    S0 += (double)u;
+    const float t1 = -s1 * w;
+    hf[0] += sx * t1;
+    hf[1] += sy * t1;
+    hf[2] += sz * t1;
+    const float w2 = w * w;
+    const float t2 = (3.0f * s1 + 2.0f * k2r2 * g) * w2;
+    const float sxt2 = sx * t2;
+    hf[3] += sxt2 * sy;
+    hf[4] += sxt2 * sz;
+    hf[5] += t2 * sy * sz;
+    const float t3 = -(15.0f * s1 + (10.0f + 4.0f * k2r2) * k2r2 * g) * (w2 * w);
+    hf[6] += t3 * sx * sy * sz;
 }
 
 // FP32 LJ pair with derivatives
 __device__ __forceinline__ void lj_pair32(float A, float B, float sx, float sy, float sz, float r2,
                                           float &pot, float *h) {
 #pragma clang fp contract(fast)
-    float r2i = __frcp_rn(r2);
+    float r2i = rcp_fast(r2);
     float r6i = r2i * r2i * r2i;
     float r12i = r6i * r6i;
     pot += A * r12i - B * r6i;
@@ -364,8 +373,36 @@ __device__ __forceinline__ float foldf(float d, float L) {
     return d;
 }
 
+// Near pairs of the FP32 kernels in FP64 (reference decisions), out of line
+// so the rare path does not hold registers in the pair loop.
+struct Near {
+    double v[8];
+    int ok;
+};
+
+__device__ __noinline__ Near near_lj64(double A, double B, double h2, double l2, double ex, double ey, double ez) {
+    Near o;
+    double pot = 0.0, h[7] = {0, 0, 0, 0, 0, 0, 0};
+    o.ok = lj_pair64(A, B, h2, l2, ex, ey, ez, pot, h) != 0;
+    o.v[0] = pot;
+    for (int i = 0; i < 7; i++) o.v[i + 1] = h[i];
+    return o;
+}
+
+__device__ __noinline__ Near near_ewald64(double kappa, double charge1, double q, double ex, double ey, double ez) {
+    Near o;
+    double pot = 0.0, h[7] = {0, 0, 0, 0, 0, 0, 0};
+    const double d2 = ex * ex + ey * ey + ez * ez;
+    o.ok = !(d2 < kLowCut2);
+    if (o.ok) ewald_real_pair64(kappa, charge1, q, ex, ey, ez, d2, pot, h);
+    o.v[0] = pot;
+    for (int i = 0; i < 7; i++) o.v[i + 1] = h[i];
+    return o;
+}
+
+// at most 128 VGPRs: four waves per SIMD hide the transcendental and LDS latency
 template <int KIND>
-__global__ void __launch_bounds__(NT) k_nodes32(GridP g, const PairTab *__restrict__ tab_g,
+__global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__restrict__ tab_g,
                                                  const double *__restrict__ xyz,
                                                  const int *__restrict__ typ,
                                                  const double *__restrict__ qv,
@@ -402,8 +439,8 @@ __global__ void __launch_bounds__(NT) k_nodes32(GridP g, const PairTab *__restri
     const float Lx = (float)g.full[0], Ly = (float)g.full[1], Lz = (float)g.full[2];
     const bool refold = g.refold != 0;
     __syncthreads();
-    double S[8] = {0, 0, 0, 0, 0, 0, 0, 0}, Cc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    double pot64 = 0.0, h64[7] = {0, 0, 0, 0, 0, 0, 0};
+    // FP64 node sums: FP32 tile partials (32 atoms) and FP64 near pairs
+    double S[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool failed = false;
     for (int seg0 = 0; seg0 < g.natoms; seg0 += SEG) {
         const int segn = min(SEG, g.natoms - seg0);
@@ -462,9 +499,18 @@ __global__ void __launch_bounds__(NT) k_nodes32(GridP g, const PairTab *__restri
                 float pf = 0.0f, hf[7] = {0, 0, 0, 0, 0, 0, 0};
                 const int j1 = min(cnt, j0 + 32);
                 for (int j = j0; j < j1; j++) {
-                    float sx = (dx0 + s_xl[j]) + s_x[j];
-                    float sy = (dy0 + s_yl[j]) + s_y[j];
-                    float sz = (dz0 + s_zl[j]) + s_z[j];
+                    // LJ: hi/lo atom offsets (steep repulsion); Ewald pairs
+                    // beyond the FP64 radius are smooth: hi part only
+                    float sx, sy, sz;
+                    if (KIND == K_LJ) {
+                        sx = (dx0 + s_xl[j]) + s_x[j];
+                        sy = (dy0 + s_yl[j]) + s_y[j];
+                        sz = (dz0 + s_zl[j]) + s_z[j];
+                    } else {
+                        sx = dx0 + s_x[j];
+                        sy = dy0 + s_y[j];
+                        sz = dz0 + s_z[j];
+                    }
                     if (refold) {
                         sx = foldf(sx, Lx);
                         sy = foldf(sy, Ly);
@@ -483,8 +529,9 @@ __global__ void __launch_bounds__(NT) k_nodes32(GridP g, const PairTab *__restri
                             double ex = fold1(px - xyz[3 * a], g.full[0]);
                             double ey = fold1(py - xyz[3 * a + 1], g.full[1]);
                             double ez = fold1(pz - xyz[3 * a + 2], g.full[2]);
-                            int rc = lj_pair64(tab.A[t], tab.B[t], tab.h2[t], tab.l2[t], ex, ey, ez, pot64, h64);
-                            if (rc == 0) { failed = true; break; }
+                            const Near o = near_lj64(tab.A[t], tab.B[t], tab.h2[t], tab.l2[t], ex, ey, ez);
+                            if (!o.ok) { failed = true; break; }
+                            for (int i = 0; i < 8; i++) S[i] += o.v[i];
                             continue;
                         }
                         lj_pair32(s_p0[j], s_p1[j], sx, sy, sz, r2, pf, hf);
@@ -495,21 +542,16 @@ __global__ void __launch_bounds__(NT) k_nodes32(GridP g, const PairTab *__restri
                             double ex = fold1(px - xyz[3 * a], g.full[0]);
                             double ey = fold1(py - xyz[3 * a + 1], g.full[1]);
                             double ez = fold1(pz - xyz[3 * a + 2], g.full[2]);
-                            double d2 = ex * ex + ey * ey + ez * ez;
-                            if (d2 < kLowCut2) { failed = true; break; }
-                            ewald_real_pair64(g.kappa, g.charge1, qv[a], ex, ey, ez, d2, pot64, h64);
+                            const Near o = near_ewald64(g.kappa, g.charge1, qv[a], ex, ey, ez);
+                            if (!o.ok) { failed = true; break; }
+                            for (int i = 0; i < 8; i++) S[i] += o.v[i];
                             continue;
                         }
-                        // potential terms cancel strongly: each FP32 term goes
-                        // straight into the FP64 sum; derivatives use the tile
-                        float tt[8];
-                        ewald_real_term32(kappaf, s_p0[j], sx, sy, sz, r2, tt);
-                        S[0] += (double)tt[0];
-                        for (int i = 0; i < 7; i++) hf[i] += tt[i + 1];
+                        ewald_real_acc32(kappaf, s_p0[j], sx, sy, sz, r2, S[0], hf);
                     }
                 }
-                if (KIND == K_LJ) kahan_add(S[0], Cc[0], (double)pf);
-                for (int i = 0; i < 7; i++) kahan_add(S[i + 1], Cc[i + 1], (double)hf[i]);
+                if (KIND == K_LJ) S[0] += (double)pf;
+                for (int i = 0; i < 7; i++) S[i + 1] += (double)hf[i];
             }
         }
         __syncthreads();
@@ -517,8 +559,6 @@ __global__ void __launch_bounds__(NT) k_nodes32(GridP g, const PairTab *__restri
     if (!valid) return;
     const long long m = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
     double o[8];
-    kahan_add(S[0], Cc[0], pot64);
-    for (int i = 0; i < 7; i++) kahan_add(S[i + 1], Cc[i + 1], h64[i]);
     if (KIND == K_EWALD) {
         if (failed) {
             o[0] = kVeryLrg;
