@@ -380,7 +380,7 @@ struct Near {
     int ok;
 };
 
-__device__ __noinline__ Near near_lj64(double A, double B, double h2, double l2, double ex, double ey, double ez) {
+__device__ __forceinline__ Near near_lj64(double A, double B, double h2, double l2, double ex, double ey, double ez) {
     Near o;
     double pot = 0.0, h[7] = {0, 0, 0, 0, 0, 0, 0};
     o.ok = lj_pair64(A, B, h2, l2, ex, ey, ez, pot, h) != 0;
@@ -389,7 +389,7 @@ __device__ __noinline__ Near near_lj64(double A, double B, double h2, double l2,
     return o;
 }
 
-__device__ __noinline__ Near near_ewald64(double kappa, double charge1, double q, double ex, double ey, double ez) {
+__device__ __forceinline__ Near near_ewald64(double kappa, double charge1, double q, double ex, double ey, double ez) {
     Near o;
     double pot = 0.0, h[7] = {0, 0, 0, 0, 0, 0, 0};
     const double d2 = ex * ex + ey * ey + ez * ez;
@@ -398,6 +398,14 @@ __device__ __noinline__ Near near_ewald64(double kappa, double charge1, double q
     o.v[0] = pot;
     for (int i = 0; i < 7; i++) o.v[i + 1] = h[i];
     return o;
+}
+
+// a workgroup-uniform double kept in SGPRs (frees VGPRs in the pair loop)
+__device__ __forceinline__ double uniform_double(double v) {
+    const unsigned long long u = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return __longlong_as_double(((unsigned long long)hi << 32) | lo);
 }
 
 // at most 128 VGPRs: four waves per SIMD hide the transcendental and LDS latency
@@ -413,6 +421,9 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
     __shared__ float s_p0[SEG], s_p1[SEG];
     __shared__ int s_a[SEG], s_t[SEG];
     __shared__ int s_wtot[NT / 64];
+    // FP64 sums of the 7 derivatives, one lane-private column per node
+    // (added once per 32-atom tile; keeps them out of the VGPR budget)
+    __shared__ double s_acc[7][NT];
     __shared__ float s_h2f[kMaxTypes];
     __shared__ PairTab tab;
     const int tid = threadIdx.x;
@@ -428,19 +439,26 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
     const int x = bx * BX + (tid % BX), y = by * BY + ((tid / BX) % BY), z = bz * BZ + tid / (BX * BY);
     const bool valid = (x < g.nx) && (y < g.ny) && (z < g.nz);
     const double px = (double)x * g.step[0], py = (double)y * g.step[1], pz = (double)z * g.step[2];
-    const double lo[3] = {(double)(bx * BX) * g.step[0], (double)(by * BY) * g.step[1], (double)(bz * BZ) * g.step[2]};
-    const double hi[3] = {(double)min(bx * BX + BX - 1, g.nx - 1) * g.step[0],
-                          (double)min(by * BY + BY - 1, g.ny - 1) * g.step[1],
-                          (double)min(bz * BZ + BZ - 1, g.nz - 1) * g.step[2]};
-    const double cen[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
+    double lo[3] = {(double)(bx * BX) * g.step[0], (double)(by * BY) * g.step[1], (double)(bz * BZ) * g.step[2]};
+    double hi[3] = {(double)min(bx * BX + BX - 1, g.nx - 1) * g.step[0],
+                    (double)min(by * BY + BY - 1, g.ny - 1) * g.step[1],
+                    (double)min(bz * BZ + BZ - 1, g.nz - 1) * g.step[2]};
+    double cen[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
+    for (int d = 0; d < 3; d++) {  // brick bounds are workgroup-uniform: SGPRs
+        lo[d] = uniform_double(lo[d]);
+        hi[d] = uniform_double(hi[d]);
+        cen[d] = uniform_double(cen[d]);
+    }
     const float dx0 = (float)(px - cen[0]), dy0 = (float)(py - cen[1]), dz0 = (float)(pz - cen[2]);
     const float r2safe = (float)g.r2safe;
     const float kappaf = (float)g.kappa;
     const float Lx = (float)g.full[0], Ly = (float)g.full[1], Lz = (float)g.full[2];
     const bool refold = g.refold != 0;
     __syncthreads();
-    // FP64 node sums: FP32 tile partials (32 atoms) and FP64 near pairs
-    double S[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // FP64 node sums: FP32 tile partials (32 atoms) and FP64 near pairs;
+    // the potential in a register, the derivatives in s_acc
+    double S0 = 0.0;
+    for (int i = 0; i < 7; i++) s_acc[i][tid] = 0.0;
     bool failed = false;
     for (int seg0 = 0; seg0 < g.natoms; seg0 += SEG) {
         const int segn = min(SEG, g.natoms - seg0);
@@ -526,12 +544,14 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
                         // FP64 separation and reference cutoff decisions
                         if (r2 < r2safe || r2 > h2f - band) {
                             int a = s_a[j];
-                            double ex = fold1(px - xyz[3 * a], g.full[0]);
-                            double ey = fold1(py - xyz[3 * a + 1], g.full[1]);
-                            double ez = fold1(pz - xyz[3 * a + 2], g.full[2]);
+                            // node position recomputed here (not kept live in the loop)
+                            double ex = fold1((double)x * g.step[0] - xyz[3 * a], g.full[0]);
+                            double ey = fold1((double)y * g.step[1] - xyz[3 * a + 1], g.full[1]);
+                            double ez = fold1((double)z * g.step[2] - xyz[3 * a + 2], g.full[2]);
                             const Near o = near_lj64(tab.A[t], tab.B[t], tab.h2[t], tab.l2[t], ex, ey, ez);
                             if (!o.ok) { failed = true; break; }
-                            for (int i = 0; i < 8; i++) S[i] += o.v[i];
+                            S0 += o.v[0];
+                            for (int i = 0; i < 7; i++) s_acc[i][tid] += o.v[i + 1];
                             continue;
                         }
                         lj_pair32(s_p0[j], s_p1[j], sx, sy, sz, r2, pf, hf);
@@ -539,26 +559,30 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
                         if (r2 > g.rc2f) continue;
                         if (r2 < r2safe) {
                             int a = s_a[j];
-                            double ex = fold1(px - xyz[3 * a], g.full[0]);
-                            double ey = fold1(py - xyz[3 * a + 1], g.full[1]);
-                            double ez = fold1(pz - xyz[3 * a + 2], g.full[2]);
+                            // node position recomputed here (not kept live in the loop)
+                            double ex = fold1((double)x * g.step[0] - xyz[3 * a], g.full[0]);
+                            double ey = fold1((double)y * g.step[1] - xyz[3 * a + 1], g.full[1]);
+                            double ez = fold1((double)z * g.step[2] - xyz[3 * a + 2], g.full[2]);
                             const Near o = near_ewald64(g.kappa, g.charge1, qv[a], ex, ey, ez);
                             if (!o.ok) { failed = true; break; }
-                            for (int i = 0; i < 8; i++) S[i] += o.v[i];
+                            S0 += o.v[0];
+                            for (int i = 0; i < 7; i++) s_acc[i][tid] += o.v[i + 1];
                             continue;
                         }
-                        ewald_real_acc32(kappaf, s_p0[j], sx, sy, sz, r2, S[0], hf);
+                        ewald_real_acc32(kappaf, s_p0[j], sx, sy, sz, r2, S0, hf);
                     }
                 }
-                if (KIND == K_LJ) S[0] += (double)pf;
-                for (int i = 0; i < 7; i++) S[i + 1] += (double)hf[i];
+                if (KIND == K_LJ) S0 += (double)pf;
+                for (int i = 0; i < 7; i++) s_acc[i][tid] += (double)hf[i];
             }
         }
         __syncthreads();
     }
     if (!valid) return;
     const long long m = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
-    double o[8];
+    double o[8], S[8];
+    S[0] = S0;
+    for (int i = 0; i < 7; i++) S[i + 1] = s_acc[i][tid];
     if (KIND == K_EWALD) {
         if (failed) {
             o[0] = kVeryLrg;
