@@ -52,3 +52,30 @@ def test_fortran_driver_builds_test012_map(hip):
         da, db = open(fmap, "rb").read(), open(pmap, "rb").read()
         assert hashlib.sha256(da).hexdigest() == hashlib.sha256(db).hexdigest()
         m.close()
+
+
+MAPMAKER = os.path.join(os.path.dirname(HERE), "music-code_amd", "bin", "music_mapmaker")
+
+
+@pytest.mark.parametrize("case", ["test012-pmapgen/test012.ctr", "test013-emapgen/test013.ctr"])
+def test_fortran_music_mapmaker_matches_python_host(hip, case):
+    """music_mapmaker (Fortran, ctrlfile in, map files out) against the
+    Python host's mapmaker on the same ctrlfile: identical map files."""
+    from musichip import mapmaker
+    print("case: music_mapmaker %s" % case, flush=True)
+    ctrl = os.path.join(GOLD, case)
+    with tempfile.TemporaryDirectory() as tf, tempfile.TemporaryDirectory() as tp:
+        out = subprocess.run([MAPMAKER, ctrl], cwd=tf, capture_output=True, text=True, timeout=300)
+        print(out.stdout, out.stderr, flush=True)
+        assert out.returncode == 0
+        res = mapmaker.make_maps(ctrl, ctx=hip, precision=H.MH_PREC_FP64, outdir=tp)
+        assert res
+        for job, m, _ in res:
+            name = os.path.basename(job["path"])
+            fa, fb = os.path.join(tf, name), job["path"]
+            assert os.path.exists(fa), name
+            da, db = open(fa, "rb").read(), open(fb, "rb").read()
+            assert hashlib.sha256(da).hexdigest() == hashlib.sha256(db).hexdigest(), name
+            assert ("%12d" * 4) % tuple(m.counts) in out.stdout.replace("\n", " ") or \
+                all(str(c) in out.stdout for c in m.counts)
+            m.close()
