@@ -196,6 +196,12 @@ int mh_gcmc_run(mh_gcmc *g, int nsteps);
  * pointer may be NULL. */
 int mh_gcmc_get_state(mh_gcmc *g, int32_t *N, double *energy, double *sumN, int64_t *nsteps,
                       int64_t *att, int64_t *acc, int32_t *rng_ix, int32_t *rng_iy);
+/* Per replica: sums over the steps run of N and N^2, from which a host forms
+ * the running averages of stats_update (stats.F90:152-195): cumulative mean
+ * sumN / steps, standard deviation sqrt(sumN2 / steps - mean^2), and the
+ * block mean from the difference of two readings.  Either pointer may be
+ * NULL. */
+int mh_gcmc_get_moments(mh_gcmc *g, double *sumN, double *sumN2);
 /* Restore a checkpoint: N and RNG state per replica (the configuration via
  * mh_gcmc_set_config).  The reference does not save the RNG state. */
 int mh_gcmc_set_rng(mh_gcmc *g, int rep, int32_t ix, int32_t iy);

@@ -23,7 +23,7 @@
 // Layout in HBM (per replica r, stride by replica):
 //   com[r][maxN][3], ang[r][maxN][3], pos[r][maxN][ns][3]  (FP64)
 //   S[r][nk] (double2 running structure factor), dS[r][nk] scratch
-//   scalars: N, energy, sumN, nsteps, att[4], acc[4], rng ix/iy/first/iseed
+//   scalars: N, energy, sumN, sumN2, nsteps, att[4], acc[4], rng ix/iy/first/iseed
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -70,7 +70,7 @@ struct GDev {
 struct Rep {
     int *N, *err;
     int32_t *ix, *iy, *first, *iseed;
-    double *B, *energy, *sumN;
+    double *B, *energy, *sumN, *sumN2;
     long long *nsteps, *att, *acc;
     double *com, *ang, *pos;  // [rep][maxN][3], [rep][maxN][3], [rep][maxN][ns][3]
     double2 *S, *dS;          // [rep][nk]
@@ -613,7 +613,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
     const int ns = P.ns;
     Rng rng{R.ix[rep], R.iy[rep], R.first[rep], R.iseed[rep]};
     int N = R.N[rep];
-    double energy = R.energy[rep], sumN = R.sumN[rep];
+    double energy = R.energy[rep], sumN = R.sumN[rep], sumN2 = R.sumN2[rep];
     long long nst = R.nsteps[rep];
     long long att[4], acc[4];
     for (int i = 0; i < 4; i++) {
@@ -848,6 +848,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
             }
         }
         sumN += N;
+        sumN2 += (double)N * N;
         nst++;
         done++;
         __syncthreads();
@@ -877,6 +878,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
         R.N[rep] = N;
         R.energy[rep] = energy;
         R.sumN[rep] = sumN;
+        R.sumN2[rep] = sumN2;
         R.nsteps[rep] = nst;
         for (int i = 0; i < 4; i++) {
             R.att[4 * rep + i] = att[i];
@@ -1038,7 +1040,7 @@ struct mh_gcmc {
     std::vector<int> slot_sys;  // host mirror of sysidx (pool)
     DBuf dP;                    // GDev[sys.size()]
     DBuf sysidx, budget;        // pool only
-    DBuf N, err, ix, iy, first, iseed, B, energy, sumN, nsteps, att, acc;
+    DBuf N, err, ix, iy, first, iseed, B, energy, sumN, sumN2, nsteps, att, acc;
     DBuf com, ang, pos, S, dS;
     DBuf tr_move, tr_acc, tr_N, tr_dE;
     int tr_cap = 0;
@@ -1054,6 +1056,7 @@ struct mh_gcmc {
         r.B = B.as<double>();
         r.energy = energy.as<double>();
         r.sumN = sumN.as<double>();
+        r.sumN2 = sumN2.as<double>();
         r.nsteps = nsteps.as<long long>();
         r.att = att.as<long long>();
         r.acc = acc.as<long long>();
@@ -1332,7 +1335,8 @@ static int alloc_replicas(mh_gcmc *g) {
     if ((rc = g->N.alloc(4 * R)) || (rc = g->err.alloc(4 * R)) || (rc = g->ix.alloc(4 * R)) ||
         (rc = g->iy.alloc(4 * R)) || (rc = g->first.alloc(4 * R)) || (rc = g->iseed.alloc(4 * R)) ||
         (rc = g->B.alloc(8 * R)) || (rc = g->energy.alloc(8 * R)) || (rc = g->sumN.alloc(8 * R)) ||
-        (rc = g->nsteps.alloc(8 * R)) || (rc = g->att.alloc(32 * R)) || (rc = g->acc.alloc(32 * R)) ||
+        (rc = g->sumN2.alloc(8 * R)) || (rc = g->nsteps.alloc(8 * R)) || (rc = g->att.alloc(32 * R)) ||
+        (rc = g->acc.alloc(32 * R)) ||
         (rc = g->com.alloc(sizeof(double) * 3 * R * maxN)) || (rc = g->ang.alloc(sizeof(double) * 3 * R * maxN)) ||
         (rc = g->pos.alloc(sizeof(double) * 3 * R * maxN * g->ns)) ||
         (rc = g->S.alloc(sizeof(double2) * (R * g->kstride + 1))) ||
@@ -1365,6 +1369,7 @@ static int init_replicas(mh_gcmc *g, const std::vector<int> &reps, const double 
         MH_HIP(hipMemcpy(g->B.as<double>() + r, &hB, 8, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->energy.as<double>() + r, &zd, 8, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->sumN.as<double>() + r, &zd, 8, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->sumN2.as<double>() + r, &zd, 8, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->nsteps.as<long long>() + r, &zl, 8, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->att.as<long long>() + 4 * r, zl4, 32, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->acc.as<long long>() + 4 * r, zl4, 32, hipMemcpyHostToDevice));
@@ -1644,6 +1649,15 @@ int mh_gcmc_get_state(mh_gcmc *g, int32_t *N, double *energy, double *sumN, int6
     return MH_OK;
 }
 
+int mh_gcmc_get_moments(mh_gcmc *g, double *sumN, double *sumN2) {
+    MH_CHECK(g, MH_EINVAL, "mh_gcmc_get_moments: g is NULL");
+    MH_HIP(hipSetDevice(g->ctx->device));
+    const size_t R = g->nrep;
+    if (sumN) MH_HIP(hipMemcpy(sumN, g->sumN.p, 8 * R, hipMemcpyDeviceToHost));
+    if (sumN2) MH_HIP(hipMemcpy(sumN2, g->sumN2.p, 8 * R, hipMemcpyDeviceToHost));
+    return MH_OK;
+}
+
 int mh_gcmc_set_rng(mh_gcmc *g, int rep, int32_t ix, int32_t iy) {
     MH_CHECK(g && rep >= 0 && rep < g->nrep, MH_EINVAL, "mh_gcmc_set_rng: bad replica");
     MH_HIP(hipSetDevice(g->ctx->device));
@@ -1766,7 +1780,7 @@ int mh_gcmc_autotune(mh_gcmc *g, int probe_steps, int *threads_out, int *exy_out
     hipStream_t st = g->ctx->stream;
     // snapshot of every buffer a run modifies
     std::vector<DBuf *> bufs = {&g->N, &g->err, &g->ix, &g->iy, &g->first, &g->iseed, &g->energy, &g->sumN,
-                                &g->nsteps, &g->att, &g->acc, &g->com, &g->ang, &g->pos, &g->S};
+                                &g->sumN2, &g->nsteps, &g->att, &g->acc, &g->com, &g->ang, &g->pos, &g->S};
     if (g->pool) bufs.push_back(&g->budget);
     const int nb = (int)bufs.size();
     std::vector<DBuf> snap(nb);

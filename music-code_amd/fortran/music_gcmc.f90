@@ -1,0 +1,638 @@
+! music_gcmc -- GCMC driver over libmusic_hip (usage: music_gcmc ctrlfile
+! [fp32] [chain] [maxn=N] [check]), the drop-in for the reference's
+! drivers/music_gcmc.F90:79-257 for one rigid species with the rigid
+! insert / delete / translate / rotate moves.  It reads the ctrlfile's
+! 'General Information' (iterations, print interval, seed; general.F90),
+! atom and molecule types, simulation cell, forcefield and 'GCMC
+! Information' sections (gcmc_init, gcmc.F90:100-200; gcmcmoves_init and
+! gcmcmoves_getpressures, gcmcmoves.F90:96-260, 430-646), and from the
+! species-species file the species-framework maps (NCOUL MAP ...
+! Atom@PMAP@file, COUL MAP ... all@EMAP@file; ssmap.F90:217-348), the
+! species-species LJ (NCOUL BASIC LJ, atom-atom pairs with LBMIX /
+! JORGMIX mixing) and optional Ewald electrostatics (COUL SUM EWALD
+! KAPPA@k KMAX@n [RCUT@r]; the reference stops there, ssdriver.F90:489-498).
+!
+! State points: by default every pressure of the list runs at once, one
+! replica per point in one device batch, each an independent chain that
+! starts empty with seed iseed + p - 1 (point 1 is the reference's run).
+! With 'chain' the points run one after another as in the reference
+! (music_gcmc.F90:169-245): the configuration and the random-number stream
+! carry over from one point to the next and the statistics restart
+! (gcmcmoves_beginsim, gcmcmoves.F90:706-718).
+!
+! Output follows the reference's: every iprint iterations the acceptance
+! counts per move and the number of molecules (instantaneous, block
+! average, cumulative average, standard deviation: stats_update,
+! stats.F90:152-195, formed from the device's sums of N and N^2), and at the
+! end of each point the recomputed and stored energies (kJ/mol), their
+! deviation (interact_checknrgs) and the loading per unit cell.  With
+! 'check' it prints the parsed parameters and stops before any device call.
+program music_gcmc
+  use, intrinsic :: iso_c_binding
+  use music_hip_mod
+  use music_inputs
+  implicit none
+  type(sim_system) :: sy
+  type(mh_gcmc_params), target :: P
+  character(len=LL) :: ctrl, arg, line
+  character(len=NL) :: spname, mvname(4)
+  integer :: i, j, k, a, niter, iprint, iseed, nmicro, npts, bsize, nspc, isp, nmv, nsite, nst, np
+  integer :: maxn, nunit
+  logical :: chain, check_only
+  real(c_double) :: T, wts(4), kappa_param, rcut
+  real(c_double), allocatable :: press(:)
+  integer :: stypes(MH_MAX_SITES)
+  character(len=LL) :: pmapf(MH_MAX_SITES), emapf(MH_MAX_SITES)
+  type(c_ptr) :: ctx, g
+  type(c_ptr), allocatable :: maps(:)
+  character(len=LL), allocatable :: mapnames(:)
+  integer :: nmaps
+  real(c_double) :: ref(3, MH_MAX_SITES)
+  integer(8) :: t0, t1, rate
+  real(c_double) :: secs
+
+  call get_command_argument(1, ctrl)
+  if (len_trim(ctrl) == 0) then
+     write(*, '(a)') 'usage: music_gcmc ctrlfile [fp32] [chain] [maxn=N] [check]'
+     stop 1
+  end if
+  P%precision = MH_PREC_FP64
+  chain = .false.
+  check_only = .false.
+  maxn = 512
+  do i = 2, command_argument_count()
+     call get_command_argument(i, arg)
+     if (trim(lower(arg)) == 'fp32') then
+        P%precision = MH_PREC_FP32
+     else if (trim(lower(arg)) == 'chain') then
+        chain = .true.
+     else if (trim(lower(arg)) == 'check') then
+        check_only = .true.
+     else if (index(lower(arg), 'maxn=') == 1) then
+        read(arg(6:), *) maxn
+     else
+        call fail('unknown argument ' // trim(arg))
+     end if
+  end do
+  call read_system(ctrl, sy)
+
+  ! ---- General Information: description, iterations, print interval,
+  ! crash and config intervals, first simulation number, seed
+  i = find_section(sy%t, 'General Information')
+  call need(i, 'General Information')
+  read(strip(sy%t%line(i + 1)), *) niter
+  read(strip(sy%t%line(i + 2)), *) iprint
+  read(strip(sy%t%line(i + 6)), *) iseed
+
+  ! ---- GCMC Information
+  i = find_section(sy%t, 'GCMC Information')
+  call need(i, 'GCMC Information')
+  read(strip(sy%t%line(i)), *) nmicro
+  read(strip(sy%t%line(i + 1)), *) T
+  if (index(lower(sy%t%line(i + 2)), 'ideal') == 0 .and. index(lower(sy%t%line(i + 2)), 'null') == 0) &
+       call fail('only the ideal-gas equation of state is offered (fugacity = pressure)')
+  read(strip(sy%t%line(i + 3)), *) npts
+  read(strip(sy%t%line(i + 4)), *) bsize
+  read(strip(sy%t%line(i + 5)), *) nspc
+  if (nspc /= 1) call fail('one GCMC species per run is offered')
+  i = i + 6
+  do while (i <= sy%t%n)
+     line = strip(sy%t%line(i))
+     if (len_trim(line) > 0 .and. verify(trim(line), '-') /= 0) exit
+     i = i + 1
+  end do
+  spname = word(strip(sy%t%line(i)), 1)
+  isp = mol_index(sy, spname)
+  i = i + 1
+  allocate(press(npts))
+  call read_pressures(i)
+  i = i + 1          ! sitemap line
+  read(strip(sy%t%line(i + 1)), *) nmv
+  if (nmv < 1 .or. nmv > 4) call fail('1 to 4 move types are offered')
+  read(strip(sy%t%line(i + 2)), *) (wts(k), k = 1, nmv)
+  i = i + 3
+  P%delta_trans = 0.0d0
+  P%delta_rot = 0.0d0
+  do k = 1, nmv
+     mvname(k) = upper(word(strip(sy%t%line(i)), 1))
+     i = i + 1
+     select case (trim(mvname(k)))
+     case ('RINSERT', 'INSERT')
+        P%move_type(k) = MH_MV_INSERT
+     case ('RDELETE', 'DELETE')
+        P%move_type(k) = MH_MV_DELETE
+     case ('RTRANSLATE', 'TRANSLATE')
+        P%move_type(k) = MH_MV_TRANSLATE
+        call read_delta(i, P%delta_trans)
+        i = i + 1
+     case ('RROTATE', 'ROTATE')
+        P%move_type(k) = MH_MV_ROTATE
+        call read_delta(i, P%delta_rot)
+        i = i + 1
+     case default
+        call fail('move type ' // trim(mvname(k)) // ' is not offered (rigid moves only)')
+     end select
+     P%move_weight(k) = wts(k)
+  end do
+  P%nmoves = nmv
+  P%temperature = T
+  P%max_nrg = 1.0d10
+
+  ! ---- the species: sites, body-frame coordinates, types, charges
+  associate (m => sy%mols(isp))
+    nsite = m%natoms
+    if (nsite > MH_MAX_SITES) call fail('at most 8 sites per molecule')
+    call principal_axes(m, sy%atoms, ref(:, 1:nsite))
+    P%nsites = nsite
+    nst = 0
+    do a = 1, nsite
+       P%ref(:, a) = ref(:, a)
+       P%site_q(a) = m%q(a)
+       k = 0
+       do j = 1, nst
+          if (stypes(j) == m%atype(a)) k = j
+       end do
+       if (k == 0) then
+          nst = nst + 1
+          stypes(nst) = m%atype(a)
+          k = nst
+       end if
+       P%site_type(a) = k - 1
+    end do
+  end associate
+  P%ntypes = nst
+  P%ewald = 0
+  P%lj_on = 0_c_int8_t
+  P%lj_A = 0.0d0
+  P%lj_B = 0.0d0
+  P%lj_hicut2 = 0.0d0
+  P%lj_locut2 = 0.0d0
+  do a = 1, MH_MAX_SITES
+     P%site_pmap(a) = c_null_ptr
+     P%site_emap(a) = c_null_ptr
+  end do
+  pmapf = ''
+  emapf = ''
+  call read_species_file()
+  nunit = sy%nxyz(1) * sy%nxyz(2) * sy%nxyz(3)
+
+  write(*, '(a)') 'Welcome to MuSiC (GCMC implementation, libmusic_hip)'
+  write(*, '(2a)') 'Reading from control file ', trim(ctrl)
+  write(*, '(a, i0, a, i0, a, i0, a, i0, a, i0)') '  iterations ', niter, ' x ', nmicro, ', print every ', iprint, &
+       ', block ', bsize, ', seed ', iseed
+  write(*, '(a, i0, a, f9.3, a)') '  species ' // trim(spname) // ': ', nsite, ' sites, T = ', T, ' K'
+  write(*, '(a, 3f10.4, a, 3i3)') '  simulation cell (A):', sy%full, ', unit cells', sy%nxyz
+  do k = 1, npts
+     write(*, '(a, i0, a, es24.16)') '  state point ', k, ': fugacity (kPa) ', press(k)
+  end do
+  do k = 1, nmv
+     write(*, '(a, i0, 2a, f8.4)') '  move ', k, ': ', trim(mvname(k)) // ' weight', P%move_weight(k)
+  end do
+  write(*, '(a, 2f10.5)') '  delta translate / rotate:', P%delta_trans, P%delta_rot
+  do a = 1, nsite
+     write(*, '(a, i0, a, 3es24.16, a, i0, a, f8.4, 4a)') '  site ', a, ':', P%ref(:, a), ' type ', &
+          P%site_type(a), ' q ', P%site_q(a), ' pmap ', trim(basename(pmapf(a))), ' emap ', trim(basename(emapf(a)))
+  end do
+  do k = 1, nst * nst
+     if (P%lj_on(k) /= 0) write(*, '(a, i0, a, 4es24.16)') '  lj ', k - 1, ':', P%lj_A(k), P%lj_B(k), &
+          P%lj_hicut2(k), P%lj_locut2(k)
+  end do
+  if (P%ewald /= 0) write(*, '(a, es24.16, a, i0, a, es24.16)') '  ewald: kappa ', P%ewald_kappa, ' kmax ', &
+       P%ewald_kmax, ' rcut ', P%ewald_rcut
+  if (check_only) go to 900
+
+  ! ---- device: the simulation cell (and framework, for reference), maps
+  call check(mh_ctx_create(0_c_int, ctx), 'mh_ctx_create')
+  if (sy%ntot > 0) then
+     call check(mh_framework_upload(ctx, int(sy%ntot, c_int), sy%xyz, sy%typ, sy%q, sy%fund, sy%full), &
+          'mh_framework_upload')
+  else
+     ! a NULL cell: one placeholder atom defines the cell; no map reads it
+     sy%xyz(:, 1) = 0.0d0
+     sy%typ(1) = 0
+     sy%q(1) = 0.0d0
+     call check(mh_framework_upload(ctx, 1_c_int, sy%xyz, sy%typ, sy%q, sy%fund, sy%full), &
+          'mh_framework_upload')
+  end if
+  allocate(maps(2 * MH_MAX_SITES), mapnames(2 * MH_MAX_SITES))
+  nmaps = 0
+  do a = 1, nsite
+     if (len_trim(pmapf(a)) > 0) P%site_pmap(a) = load_map(pmapf(a))
+     if (len_trim(emapf(a)) > 0) P%site_emap(a) = load_map(emapf(a))
+  end do
+  np = 1
+  call system_clock(t0, rate)
+  if (chain) then
+     do np = 1, npts
+        call run_points(np, np, np > 1)
+     end do
+  else
+     call run_points(1, npts, .false.)
+  end if
+  call system_clock(t1)
+  secs = real(t1 - t0, c_double) / real(rate, c_double)
+  write(*, '(a, f10.3, a, es10.3, a)') 'Main Program   Time: ', secs, ' sec   (', &
+       real(npts, c_double) * niter * nmicro / max(secs, 1.0d-9), ' steps/s over all state points)'
+  do k = 1, nmaps
+     call check(mh_map_destroy(maps(k)), 'mh_map_destroy')
+  end do
+  call check(mh_ctx_destroy(ctx), 'mh_ctx_destroy')
+900 continue
+
+contains
+
+  ! state points p0..p1 as replicas of one batch; carry: take the previous
+  ! point's configuration and random-number state (held in g) first
+  subroutine run_points(q0, q1, carry)
+    integer, intent(in) :: q0, q1
+    logical, intent(in) :: carry
+    integer :: nr, r, e, nev, it, n
+    integer(c_int32_t), allocatable :: seeds(:), Ncur(:), ix(:), iy(:)
+    real(c_double), allocatable :: energy(:), sN(:), sN2(:), snap(:,:), etot(:)
+    integer(c_int64_t), allocatable :: nsteps(:), att(:,:), acc(:,:)
+    integer(8), allocatable :: ev(:), blk(:)
+    integer(8) :: s, sprev, total
+    real(c_double), allocatable :: com(:,:), ang(:,:), rr(:,:)
+    integer(c_int) :: ncarry
+    type(c_ptr) :: gnew
+    nr = q1 - q0 + 1
+    allocate(seeds(nr), Ncur(nr), ix(nr), iy(nr), energy(nr), sN(nr), sN2(nr), etot(nr), nsteps(nr))
+    allocate(att(4, nr), acc(4, nr))
+    do r = 1, nr
+       seeds(r) = int(iseed + (q0 + r - 1) - 1, c_int32_t)
+    end do
+    if (chain) seeds(1) = int(iseed, c_int32_t)
+    call check(mh_gcmc_create(ctx, P, int(nr, c_int), press(q0:q1), seeds, int(maxn, c_int), gnew), &
+         'mh_gcmc_create')
+    if (carry) then
+       allocate(com(3, maxn), ang(3, maxn), rr(3, maxn * nsite))
+       call check(mh_gcmc_get_config(g, 0_c_int, ncarry, com, ang, rr), 'mh_gcmc_get_config')
+       call check(mh_gcmc_get_state(g, Ncur, energy, sN, nsteps, att, acc, ix, iy), 'mh_gcmc_get_state')
+       call check(mh_gcmc_set_config(gnew, 0_c_int, ncarry, com, ang), 'mh_gcmc_set_config')
+       call check(mh_gcmc_set_rng(gnew, 0_c_int, ix(1), iy(1)), 'mh_gcmc_set_rng')
+       call check(mh_gcmc_destroy(g), 'mh_gcmc_destroy')
+    end if
+    g = gnew
+
+    do r = 1, nr
+       write(*, '(a)') '------------------------------------------------------------'
+       write(*, '(a, i0, a, i0, 3x, a, i0, a)') 'Beginning GCMC simulation number ', q0 + r - 1, ' of ', npts, &
+            '(', niter, ' steps)'
+       write(*, '(a)') '------------------------------------------------------------'
+       write(*, '(a)') '    Species Name: ' // trim(spname)
+       write(*, '(a)') '      Press.(kPa)   Fugacity'
+       write(*, '(4x, 2f11.3)') press(q0 + r - 1), press(q0 + r - 1)
+    end do
+    ! events: every print step and the start of its statistics block
+    ! (a block holds the last ((s - 1) mod bsize) + 1 steps)
+    nev = 0
+    allocate(ev(2 * (niter / max(iprint, 1) + 2)), blk(niter / max(iprint, 1) + 2))
+    n = 0
+    do it = iprint, niter, iprint
+       s = int(it, 8) * nmicro
+       n = n + 1
+       blk(n) = s - (mod(s - 1, int(bsize, 8)) + 1)
+       nev = nev + 1
+       ev(nev) = blk(n)
+       nev = nev + 1
+       ev(nev) = s
+    end do
+    total = int(niter, 8) * nmicro
+    allocate(snap(nr, 0:0))
+    sprev = 0
+    it = 0
+    e = 1
+    do while (.true.)
+       ! next event at or after sprev
+       s = total
+       do k = 1, nev
+          if (ev(k) > sprev .and. ev(k) < s) s = ev(k)
+       end do
+       if (s > sprev) call run_steps(s - sprev)
+       sprev = s
+       do k = 1, n
+          if (int(k * iprint, 8) * nmicro == s) then
+             if (blk(k) == 0) snap(:, 0) = 0.0d0
+             call show_stats(k * iprint, s, s - blk(k), snap(:, 0))
+          end if
+       end do
+       ! a block start (after the print: the next block may start at this
+       ! print step): keep sum N for the block average
+       do k = 1, n
+          if (blk(k) == s .and. s > 0) then
+             call check(mh_gcmc_get_moments(g, sN, sN2), 'mh_gcmc_get_moments')
+             snap(:, 0) = sN
+          end if
+       end do
+       if (s >= total) exit
+    end do
+    ! end of the points: recomputed against stored energies
+    call check(mh_gcmc_get_state(g, Ncur, energy, sN, nsteps, att, acc, ix, iy), 'mh_gcmc_get_state')
+    call check(mh_gcmc_total_energy(g, etot, 0_c_int), 'mh_gcmc_total_energy')
+    do r = 1, nr
+       write(*, '(a)') ' ________ ACTUAL ENERGIES    _____________'
+       write(*, '(a, f16.3)') '    Energy: ', etot(r) * 4.184d0
+       write(*, '(a)') ' ________ STORED ENERGIES    _____________'
+       write(*, '(a, f16.3)') '    Energy: ', energy(r) * 4.184d0
+       write(*, '(a)') '------------------------------------------------------------'
+       write(*, '(a, i0, 4x, a, es10.3)') 'Ending GCMC simulation number ', q0 + r - 1, 'Nrg dev check: ', &
+            abs(etot(r) - energy(r)) * 4.184d0
+       write(*, '(2x, a, a, f10.1, 4x, a, f8.2)') trim(spname), ':  P (kPa):', press(q0 + r - 1), &
+            'loading (molec/uc):', sN(r) / real(max(nsteps(r), 1_c_int64_t), c_double) / nunit
+       write(*, '(a)') '------------------------------------------------------------'
+    end do
+    if (.not. chain .or. q1 == npts) call check(mh_gcmc_destroy(g), 'mh_gcmc_destroy')
+  end subroutine run_points
+
+  ! nsteps GCMC steps on every replica, in launches of at most 2^30 steps
+  subroutine run_steps(ns)
+    integer(8), intent(in) :: ns
+    integer(8) :: left
+    integer(c_int) :: rc
+    left = ns
+    do while (left > 0)
+       rc = mh_gcmc_run(g, int(min(left, 1073741824_8), c_int))
+       if (rc == MH_ECAPACITY) call fail('a state point reached maxn molecules; run again with a larger maxn=N')
+       call check(rc, 'mh_gcmc_run')
+       left = left - min(left, 1073741824_8)
+    end do
+  end subroutine run_steps
+
+  ! the reference's statistics block (gcmc_displaystats) per replica
+  subroutine show_stats(it, s, nblk, sblk)
+    integer, intent(in) :: it
+    integer(8), intent(in) :: s, nblk
+    real(c_double), intent(in) :: sblk(:)
+    integer :: r, nr, k, mt
+    integer(c_int32_t), allocatable :: Nc(:), ix(:), iy(:)
+    real(c_double), allocatable :: en(:), sN(:), sN2(:)
+    integer(c_int64_t), allocatable :: nst(:), att(:,:), acc(:,:)
+    real(c_double) :: cum, dev, blkavg, ratio
+    character(len=32) :: lab
+    nr = size(sblk)
+    allocate(Nc(nr), ix(nr), iy(nr), en(nr), sN(nr), sN2(nr), nst(nr), att(4, nr), acc(4, nr))
+    call check(mh_gcmc_get_state(g, Nc, en, sN, nst, att, acc, ix, iy), 'mh_gcmc_get_state')
+    call check(mh_gcmc_get_moments(g, sN, sN2), 'mh_gcmc_get_moments')
+    do r = 1, nr
+       write(*, '(2x, a, i0, a, i0, a, i0)') 'Macro-iteration number ', it, ' (', s, ' micro) of simulation ', &
+            merge(np, r, chain)
+       write(*, '(2x, a)') '------------------------------------------------------------'
+       write(*, '(2x, a)') 'The GCMC Stats:'
+       write(*, '(4x, a, f10.1, a)') 'Species Name: ' // trim(spname) // '    Pressure:', &
+            press(merge(np, r, chain)), ' kPa'
+       do k = 1, nmv
+          mt = P%move_type(k) + 1
+          ratio = 0.0d0
+          if (att(mt, r) > 0) ratio = real(acc(mt, r), c_double) / real(att(mt, r), c_double)
+          lab = trim(basic_tag(P%move_type(k))) // ' (' // trim(mvname(k)) // ')'
+          if (P%move_type(k) == MH_MV_TRANSLATE) then
+             write(*, '(6x, a26, a, f5.3, 3x, a, i0, a, i0, a, f6.3)') lab, 'Acc. Ratio: ', ratio, '(', acc(mt, r), &
+                  ' of ', att(mt, r), ')  Max disp: ', P%delta_trans
+          else if (P%move_type(k) == MH_MV_ROTATE) then
+             write(*, '(6x, a26, a, f5.3, 3x, a, i0, a, i0, a, f6.3)') lab, 'Acc. Ratio: ', ratio, '(', acc(mt, r), &
+                  ' of ', att(mt, r), ')  Max disp: ', P%delta_rot
+          else
+             write(*, '(6x, a26, a, f5.3, 3x, a, i0, a, i0, a)') lab, 'Acc. Ratio: ', ratio, '(', acc(mt, r), &
+                  ' of ', att(mt, r), ')'
+          end if
+       end do
+       cum = sN(r) / real(s, c_double)
+       dev = sN2(r) / real(s, c_double) - cum * cum
+       blkavg = (sN(r) - sblk(r)) / real(nblk, c_double)
+       write(*, '(6x, a, 4f7.2)') 'Number of Molecules (ins, blk, cum, stdd)', real(Nc(r), c_double), blkavg, &
+            cum, sqrt(max(dev, 0.0d0))
+       write(*, '(a)') ' '
+    end do
+  end subroutine show_stats
+
+  function basic_tag(mt) result(o)
+    integer(c_int), intent(in) :: mt
+    character(len=9) :: o
+    select case (mt)
+    case (MH_MV_INSERT)
+       o = 'INSERT'
+    case (MH_MV_DELETE)
+       o = 'DELETE'
+    case (MH_MV_TRANSLATE)
+       o = 'TRANSLATE'
+    case default
+       o = 'ROTATE'
+    end select
+  end function basic_tag
+
+  ! gcmcmoves_getpressures (gcmcmoves.F90:430-570): 'p' for one point,
+  ! 'p1, p2' for a generated list (gcmcmoves_genpressures: geometric steps
+  ! when the range spans more than two decades, else linear), or npts
+  ! comma-separated values, continued on up to two more lines ending '+'
+  subroutine read_pressures(ii)
+    integer, intent(inout) :: ii
+    integer :: nf, q, nl
+    real(c_double) :: pincr
+    character(len=LL) :: f(256)
+    nf = 0
+    nl = 0
+    do
+       line = strip(sy%t%line(ii))
+       call split_commas(line, f, nf)
+       nl = nl + 1
+       if (trim(adjustl(f(nf))) /= '+' .or. nl == 3) exit
+       nf = nf - 1
+       ii = ii + 1
+    end do
+    if (trim(adjustl(f(nf))) == '+') nf = nf - 1
+    if (npts == 1 .and. nf > 1) call fail('one state point but more than one pressure given')
+    if (nf == 1 .and. npts == 1) then
+       read(word(f(1), 1), *, err=90) press(1)
+    else if (nf == 2) then
+       read(word(f(1), 1), *, err=90) press(1)
+       read(word(f(2), 1), *, err=90) pincr
+       if (abs(log10(pincr / press(1))) > 2.0d0) then
+          pincr = exp(log(pincr / press(1)) / (npts - 1.0d0))
+          do q = 1, npts - 1
+             press(q + 1) = press(1) * pincr**q
+          end do
+       else
+          pincr = (pincr - press(1)) / (npts - 1.0d0)
+          do q = 1, npts - 1
+             press(q + 1) = press(1) + pincr * q
+          end do
+       end if
+    else if (nf == npts) then
+       do q = 1, npts
+          read(word(f(q), 1), *, err=90) press(q)
+       end do
+    else
+       call fail('could not parse the pressure line (pressure files are not read)')
+    end if
+    return
+90  call fail('could not parse the pressure line (pressure files are not read)')
+  end subroutine read_pressures
+
+  ! fields of a line split at commas, appended to f(nf+1:)
+  subroutine split_commas(s, f, nf)
+    character(len=*), intent(in) :: s
+    character(len=LL), intent(inout) :: f(:)
+    integer, intent(inout) :: nf
+    integer :: st, c
+    st = 1
+    do c = 1, len_trim(s) + 1
+       if (c > len_trim(s) .or. s(min(c, len(s)):min(c, len(s))) == ',') then
+          nf = nf + 1
+          f(nf) = adjustl(s(st:c - 1))
+          st = c + 1
+       end if
+    end do
+  end subroutine split_commas
+
+  ! 'delta, adjust' of a translate / rotate move; adjustable steps are not
+  ! offered
+  subroutine read_delta(ii, d)
+    integer, intent(in) :: ii
+    real(c_double), intent(out) :: d
+    integer :: adj
+    line = strip(sy%t%line(ii))
+    read(word(line, 1), *) d
+    adj = 0
+    if (nwords(line) > 1) read(word(line, 2), *) adj
+    if (adj /= 0) call fail('adjustable move sizes are not offered (adjust flag must be 0)')
+  end subroutine read_delta
+
+  ! species-species file entries of the GCMC species
+  subroutine read_species_file()
+    type(text_file) :: ss
+    character(len=LL) :: s, kind, tok, other
+    integer :: l, wv, k1, k2, b, c
+    logical :: self, any_lj
+    call read_text(find_file(sy%ssfile, 'NONE', '', sy%base), ss)
+    any_lj = .false.
+    do l = 1, ss%n
+       s = strip(ss%line(l))
+       if (nwords(s) < 3) cycle
+       kind = upper(word(s, 3))
+       if (trim(kind) /= 'NCOUL' .and. trim(kind) /= 'COUL') cycle
+       if (trim(word(s, 1)) == trim(spname)) then
+          other = word(s, 2)
+       else if (trim(word(s, 2)) == trim(spname)) then
+          other = word(s, 1)
+       else
+          cycle
+       end if
+       self = trim(other) == trim(spname)
+       if (trim(upper(word(s, 4))) == 'OFF') cycle
+       if (self) then
+          if (trim(kind) == 'NCOUL') then
+             if (index(upper(s), 'LJ') == 0) call fail('species-species NCOUL model must be LJ: ' // trim(s))
+             any_lj = .true.
+             do b = 1, nst
+                do c = 1, nst
+                   k1 = pair_index(sy, sy%atoms(stypes(b) + 1)%name, sy%atoms(stypes(c) + 1)%name)
+                   if (k1 == 0) call fail('no atom-atom LJ entry for ' // trim(sy%atoms(stypes(b) + 1)%name) // &
+                        ' ' // trim(sy%atoms(stypes(c) + 1)%name))
+                   k2 = (b - 1) * nst + c
+                   if (.not. sy%pairs(k1)%ok) cycle
+                   P%lj_A(k2) = sy%pairs(k1)%A
+                   P%lj_B(k2) = sy%pairs(k1)%B
+                   P%lj_hicut2(k2) = sy%pairs(k1)%hicut**2
+                   P%lj_locut2(k2) = sy%pairs(k1)%locut**2
+                   P%lj_on(k2) = 1_c_int8_t
+                end do
+             end do
+          else
+             if (index(upper(s), 'EWALD') == 0) &
+                  call fail('species-species COUL model must be SUM EWALD (or OFF): ' // trim(s))
+             kappa_param = 0.0d0
+             rcut = 0.0d0
+             P%ewald_kmax = 0
+             do wv = 4, nwords(s)
+                tok = upper(word(s, wv))
+                if (index(tok, 'KAPPA@') == 1) read(tok(7:), *) kappa_param
+                if (index(tok, 'KMAX@') == 1) read(tok(6:), *) P%ewald_kmax
+                if (index(tok, 'RCUT@') == 1) read(tok(6:), *) rcut
+             end do
+             if (kappa_param <= 0.0d0 .or. P%ewald_kmax <= 0) call fail('Ewald needs KAPPA@ and KMAX@: ' // trim(s))
+             ! kappa in 1/A from the reference's kappa parameter (ewald.F90:
+             ! kappa = KAPPA / shortest cell edge); real-space cutoff by
+             ! default where erfc(kappa r) falls below 1e-9 (kappa r = 4.32)
+             P%ewald = 1
+             P%ewald_kappa = kappa_param / minval(sy%full)
+             if (rcut <= 0.0d0) rcut = min(4.32d0 / P%ewald_kappa, 0.5d0 * minval(sy%full))
+             P%ewald_rcut = rcut
+          end if
+       else
+          if (index(upper(s), 'MAP') == 0) &
+               call fail('species-framework interactions come from maps (PMAP@ / EMAP@): ' // trim(s))
+          do wv = 4, nwords(s)
+             tok = word(s, wv)
+             k1 = index(upper(tok), '@PMAP@')
+             k2 = index(upper(tok), '@EMAP@')
+             if (k1 > 0) call assign_map(tok(1:k1 - 1), tok(k1 + 6:), pmapf)
+             if (k2 > 0) call assign_map(tok(1:k2 - 1), tok(k2 + 6:), emapf)
+          end do
+       end if
+    end do
+  end subroutine read_species_file
+
+  ! map file of the sites of one atom type ('all': every site)
+  subroutine assign_map(aname, fname, dst)
+    character(len=*), intent(in) :: aname, fname
+    character(len=LL), intent(inout) :: dst(:)
+    integer :: b
+    logical :: hit
+    hit = .false.
+    do b = 1, nsite
+       if (trim(lower(aname)) == 'all' .or. trim(sy%atoms(sy%mols(isp)%atype(b) + 1)%name) == trim(aname)) then
+          dst(b) = fname
+          hit = .true.
+       end if
+    end do
+    if (.not. hit) call fail('map entry for an atom type the species does not have: ' // trim(aname))
+  end subroutine assign_map
+
+  ! a map file (current directory, $PMAPDIR, the ctrlfile's directory, or
+  ! potentials/pmaps), loaded once
+  function load_map(fname) result(m)
+    character(len=*), intent(in) :: fname
+    type(c_ptr) :: m
+    character(len=LL) :: path
+    integer :: q
+    do q = 1, nmaps
+       if (trim(mapnames(q)) == trim(fname)) then
+          m = maps(q)
+          return
+       end if
+    end do
+    if (exists(trim(fname))) then
+       path = fname
+    else
+       path = find_file(fname, 'PMAPDIR', 'pmaps', sy%base)
+    end if
+    call check(mh_map_load(ctx, trim(path) // c_null_char, sy%fund, m), 'mh_map_load ' // trim(path))
+    nmaps = nmaps + 1
+    maps(nmaps) = m
+    mapnames(nmaps) = fname
+  end function load_map
+
+  logical function exists(pth)
+    character(len=*), intent(in) :: pth
+    inquire(file=trim(pth), exist=exists)
+  end function exists
+
+  function basename(pth) result(o)
+    character(len=*), intent(in) :: pth
+    character(len=LL) :: o
+    integer :: q
+    q = index(trim(pth), '/', back=.true.)
+    o = pth(q + 1:)
+    if (len_trim(o) == 0) o = '-'
+  end function basename
+
+  subroutine check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    if (rc /= MH_OK) then
+       write(*, '(a, a, i3, a, a)') what, ' failed, code ', rc, ': ', mh_error_message()
+       stop 1
+    end if
+  end subroutine check
+
+end program music_gcmc

@@ -51,7 +51,7 @@ module music_hip_mod
   public :: mh_ewald_set, mh_map_build, mh_map_destroy, mh_map_info, mh_map_get, mh_map_write
   public :: mh_map_load, mh_map_interp
   public :: mh_gcmc_create, mh_gcmc_destroy, mh_gcmc_set_threads, mh_gcmc_autotune
-  public :: mh_gcmc_set_config, mh_gcmc_run, mh_gcmc_get_state, mh_gcmc_set_rng
+  public :: mh_gcmc_set_config, mh_gcmc_run, mh_gcmc_get_state, mh_gcmc_set_rng, mh_gcmc_get_moments
   public :: mh_gcmc_get_config, mh_gcmc_total_energy, mh_trial_energy
 
   interface
@@ -237,6 +237,13 @@ module music_hip_mod
        integer(c_int64_t), intent(out) :: nsteps(*), att(4, *), acc(4, *)
        integer(c_int) :: rc
      end function mh_gcmc_get_state
+
+     function mh_gcmc_get_moments(g, sumN, sumN2) bind(C, name="mh_gcmc_get_moments") result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       real(c_double), intent(out) :: sumN(*), sumN2(*)
+       integer(c_int) :: rc
+     end function mh_gcmc_get_moments
 
      function mh_gcmc_set_rng(g, rep, ix, iy) bind(C, name="mh_gcmc_set_rng") result(rc)
        import :: c_int, c_ptr, c_int32_t

@@ -31,7 +31,7 @@ module music_inputs
      character(len=NL), allocatable :: aname(:)
      integer(c_int), allocatable :: atype(:)       ! 0-based atom-type index
      real(c_double), allocatable :: xyz(:,:), q(:)  ! xyz(3, natoms)
-     logical :: transform = .true.
+     logical :: rigid = .false., translate = .true., rotate = .true.
      logical :: has_fundcell = .false.
      real(c_double) :: ell(3) = 0.0d0, angles(3) = 90.0d0
   end type molecule
@@ -39,6 +39,7 @@ module music_inputs
   type, public :: lj_pair
      character(len=NL) :: a1 = '', a2 = ''
      logical :: off = .false., ok = .false.
+     integer :: sigmix = 0, epsmix = 0   ! 1: LBMIX, 2: JORGMIX (lj_calc_mix)
      real(c_double) :: sig = 0.0d0, eps = 0.0d0, A = 0.0d0, B = 0.0d0
      real(c_double) :: hicut = HIGH_DIST_CUTOFF, locut = LOW_DIST_CUTOFF
   end type lj_pair
@@ -55,9 +56,27 @@ module music_inputs
      real(c_double) :: gridspace = 0.0d0, hipotcut = 0.0d0
   end type map_job
 
+  ! everything the drivers read from a ctrlfile and the files it names:
+  ! atom and molecule types, the simulation cell (a fundamental-cell species
+  ! replicated nxyz times, or a NULL cell of given edges) and the atom-atom
+  ! pairs of the forcefield
+  type, public :: sim_system
+     character(len=LL) :: ctrl = '', base = ''
+     type(text_file) :: t
+     integer :: nat = 0, nmol = 0, ifw = 0, ntot = 0, npairs = 0
+     type(atom_type), allocatable :: atoms(:)
+     type(molecule), allocatable :: mols(:)
+     integer :: nxyz(3) = 1
+     real(c_double) :: fund(3) = 0.0d0, full(3) = 0.0d0
+     real(c_double), allocatable :: xyz(:,:), q(:)   ! replicated fixed species
+     integer(c_int), allocatable :: typ(:)
+     character(len=NL) :: aafile = '', ssfile = ''
+     type(lj_pair), allocatable :: pairs(:)
+  end type sim_system
+
   public :: read_text, strip, lower, upper, find_section, next_nonblank, word, nwords
   public :: find_file, read_atom, read_molecule, read_atom_atom, read_ewald, lj_calc_AB
-  public :: dirname
+  public :: dirname, fail, need, read_system, mol_index, pair_index, principal_axes
 
 contains
 
@@ -285,7 +304,10 @@ contains
     do while (i <= t%n)
        s = strip(t%line(i))
        if (index(s, 'Coord_Info:') == 1) then
-          m%transform = index(lower(s), 'notransform') == 0
+          ! molecule.F90:1008-1028: coordsys, gcmodel type, flag
+          m%rigid = trim(upper(word(s, 4))) == 'RIGID'
+          m%translate = trim(upper(word(s, 5))) /= 'NOTRANSFORM'
+          m%rotate = m%translate .and. trim(upper(word(s, 5))) /= 'NOROTATE'
           i = i + 1
           read(strip(t%line(i)), *) n
           m%natoms = n
@@ -363,9 +385,19 @@ contains
                p%off = .true.
                p%hicut = 0.0d0
             case ('SIG')
-               if (index(upper(val), 'MIX') == 0) read(val, *) p%sig
+               if (trim(upper(val)) == 'LBMIX') then
+                  p%sigmix = 1
+               else if (trim(upper(val)) == 'JORGMIX') then
+                  p%sigmix = 2
+               else
+                  read(val, *) p%sig
+               end if
             case ('EPS')
-               if (index(upper(val), 'MIX') == 0) read(val, *) p%eps
+               if (index(upper(val), 'MIX') > 0) then
+                  p%epsmix = 1
+               else
+                  read(val, *) p%eps
+               end if
             case ('HICUT', 'TRUNC')
                read(val, *) p%hicut
             case ('LOCUT', 'LOWCUT')
@@ -376,6 +408,37 @@ contains
          if (p%ok) call lj_calc_AB(p%sig, p%eps, p%A, p%B)
        end associate
     end do
+    ! mixed pairs from the pure ones (lj_calc_mix, lj.F90:905-979): sigma
+    ! arithmetic (LBMIX) or geometric (JORGMIX), epsilon geometric
+    do i = 1, npairs
+       associate (p => pairs(i))
+         if (p%off .or. (p%sigmix == 0 .and. p%epsmix == 0)) cycle
+         if (p%sigmix == 0 .or. p%epsmix == 0) call fail('mixing rule needs both SIG and EPS: ' // &
+              trim(p%a1) // ' ' // trim(p%a2))
+         w = pure_pair(p%a1)
+         k = pure_pair(p%a2)
+         if (w == 0 .or. k == 0) call fail('mixing rule without the pure pairs: ' // trim(p%a1) // ' ' // &
+              trim(p%a2))
+         if (p%sigmix == 1) then
+            p%sig = (pairs(w)%sig + pairs(k)%sig) / 2.0d0
+         else
+            p%sig = sqrt(pairs(w)%sig * pairs(k)%sig)
+         end if
+         p%eps = sqrt(pairs(w)%eps * pairs(k)%eps)
+         p%ok = .true.
+         call lj_calc_AB(p%sig, p%eps, p%A, p%B)
+       end associate
+    end do
+  contains
+    integer function pure_pair(a)
+      character(len=*), intent(in) :: a
+      integer :: j
+      pure_pair = 0
+      do j = 1, npairs
+         if (trim(pairs(j)%a1) == trim(a) .and. trim(pairs(j)%a2) == trim(a) .and. pairs(j)%sigmix == 0 &
+              .and. pairs(j)%sig > 0.0d0) pure_pair = j
+      end do
+    end function pure_pair
   end subroutine read_atom_atom
 
   ! spc_spc_file: 's1 s2 COUL SUM EWALD KAPPA@k KMAX@n LOCUT@l [SFACTOR] ...'
@@ -412,5 +475,248 @@ contains
        return
     end do
   end function read_ewald
+
+  subroutine fail(msg)
+    character(len=*), intent(in) :: msg
+    write(*, '(a)') trim(msg)
+    stop 1
+  end subroutine fail
+
+  subroutine need(i, tag)
+    integer, intent(in) :: i
+    character(len=*), intent(in) :: tag
+    if (i < 0) call fail('ctrlfile section not found: ' // tag)
+  end subroutine need
+
+  integer function mol_index(s, name)
+    type(sim_system), intent(in) :: s
+    character(len=*), intent(in) :: name
+    integer :: j
+    do j = 1, s%nmol
+       if (trim(s%mols(j)%name) == trim(name)) then
+          mol_index = j
+          return
+       end if
+    end do
+    call fail('unknown species ' // trim(name))
+    mol_index = -1
+  end function mol_index
+
+  ! forcefield pair of two atom-type names (either order), 0 if absent
+  integer function pair_index(s, n1, n2)
+    type(sim_system), intent(in) :: s
+    character(len=*), intent(in) :: n1, n2
+    integer :: j
+    pair_index = 0
+    do j = 1, s%npairs
+       if ((trim(s%pairs(j)%a1) == trim(n1) .and. trim(s%pairs(j)%a2) == trim(n2)) .or. &
+           (trim(s%pairs(j)%a1) == trim(n2) .and. trim(s%pairs(j)%a2) == trim(n1))) pair_index = j
+    end do
+  end function pair_index
+
+  ! ctrlfile sections 'Atomic Types', 'Molecule Types', 'Simulation Cell
+  ! Information' (simcell_init: a fundamental-cell species and nxyz copies,
+  ! or NULL followed by the cell edges, angles, origin, effective edges,
+  ! nxyz and periodicity) and 'Forcefield Information'; the fixed species is
+  ! replicated to the simulation cell as simcell.F90:200-216 does (copies
+  ! along x, then y, then z)
+  subroutine read_system(ctrl, s)
+    character(len=*), intent(in) :: ctrl
+    type(sim_system), intent(inout) :: s
+    character(len=LL) :: w, fwname
+    integer :: i, k, n, ax, c
+    real(c_double) :: off(3), ang(3)
+    s%ctrl = ctrl
+    s%base = dirname(ctrl)
+    call read_text(ctrl, s%t)
+    i = find_section(s%t, 'Atomic Types')
+    call need(i, 'Atomic Types')
+    read(strip(s%t%line(i)), *) s%nat
+    allocate(s%atoms(s%nat))
+    i = i + 1
+    do k = 1, s%nat
+       i = next_nonblank(s%t, i)
+       s%atoms(k)%name = word(strip(s%t%line(i)), 1)
+       w = word(strip(s%t%line(i + 1)), 1)
+       call read_atom(find_file(w, 'ATOMSDIR', 'atoms', s%base), s%atoms(k))
+       i = i + 2
+    end do
+    i = find_section(s%t, 'Molecule Types')
+    call need(i, 'Molecule Types')
+    read(strip(s%t%line(i)), *) s%nmol
+    allocate(s%mols(s%nmol))
+    i = i + 1
+    do k = 1, s%nmol
+       i = next_nonblank(s%t, i)
+       s%mols(k)%name = word(strip(s%t%line(i)), 1)
+       w = word(strip(s%t%line(i + 1)), 1)
+       call read_molecule(find_file(w, 'MOLSDIR', 'molecules', s%base), s%atoms, s%mols(k))
+       i = i + 2
+    end do
+    i = find_section(s%t, 'Simulation Cell Information')
+    call need(i, 'Simulation Cell Information')
+    fwname = word(strip(s%t%line(i)), 1)
+    if (trim(upper(fwname)) == 'NULL') then
+       s%ifw = 0
+       read(strip(s%t%line(i + 1)), *) s%fund
+       read(strip(s%t%line(i + 2)), *) ang
+       read(strip(s%t%line(i + 5)), *) s%nxyz
+       if (any(abs(ang - 90.0d0) > 1.0d-9)) call fail('only orthorhombic cells are supported')
+       s%full = s%fund * s%nxyz
+       s%ntot = 0
+       allocate(s%xyz(3, 1), s%typ(1), s%q(1))
+    else
+       read(strip(s%t%line(i + 1)), *) s%nxyz
+       s%ifw = mol_index(s, fwname)
+       associate (fw => s%mols(s%ifw))
+         if (.not. fw%has_fundcell) call fail('fundamental cell molecule has no Fundcell_Info')
+         if (any(abs(fw%angles - 90.0d0) > 1.0d-9)) call fail('only orthorhombic cells are supported')
+         s%fund = fw%ell
+         s%full = s%fund * s%nxyz
+         n = fw%natoms
+         s%ntot = n * s%nxyz(1) * s%nxyz(2) * s%nxyz(3)
+         allocate(s%xyz(3, s%ntot), s%typ(s%ntot), s%q(s%ntot))
+         s%xyz(:, 1:n) = fw%xyz
+         s%typ(1:n) = fw%atype
+         s%q(1:n) = fw%q
+       end associate
+       do ax = 1, 3
+          do c = 1, s%nxyz(ax) - 1
+             off = 0.0d0
+             off(ax) = s%fund(ax) * c
+             do k = 1, n
+                s%xyz(:, c * n + k) = s%xyz(:, k) + off
+                s%typ(c * n + k) = s%typ(k)
+                s%q(c * n + k) = s%q(k)
+             end do
+          end do
+          n = n * s%nxyz(ax)
+       end do
+    end if
+    i = find_section(s%t, 'Forcefield Information')
+    call need(i, 'Forcefield Information')
+    s%aafile = word(strip(s%t%line(i + 2)), 1)
+    s%ssfile = word(strip(s%t%line(i + 3)), 1)
+    call read_atom_atom(find_file(s%aafile, 'NONE', '', s%base), s%pairs, s%npairs)
+  end subroutine read_system
+
+  ! molecule_genprincipleaxes (molecule.F90:830-926), applied to rigid
+  ! molecules of 3 or more atoms (molecule.F90:714-716, 843): shift to the
+  ! centre of mass computed with integer-truncated masses (molecule.F90:833)
+  ! unless NOTRANSFORM, and rotate onto the principal axes of inertia,
+  ! ascending moment, unless NOROTATE or NOTRANSFORM.  The
+  ! eigenvectors come from cyclic Jacobi rotations (the reference uses
+  ! EISPACK tred2/tql2); each is signed so that its component of largest
+  ! magnitude is positive (the first such on ties), the convention the
+  ! Python host applies as well.  Coordinates below 1e-12 A are set to 0.
+  subroutine principal_axes(m, atoms, ref)
+    type(molecule), intent(in) :: m
+    type(atom_type), intent(in) :: atoms(:)
+    real(c_double), intent(out) :: ref(3, m%natoms)
+    real(c_double) :: ms(m%natoms), com(3), r(3, m%natoms), I3(3, 3), V(3, 3), w(3), tv(3), tw
+    integer :: a, j, k, imax
+    do a = 1, m%natoms
+       ms(a) = aint(atoms(m%atype(a) + 1)%mass)
+    end do
+    r = m%xyz
+    if (.not. m%rigid .or. m%natoms < 3) then
+       ref = r
+       return
+    end if
+    if (m%translate) then
+       do j = 1, 3
+          com(j) = sum(m%xyz(j, :) * ms) / sum(ms)
+          r(j, :) = m%xyz(j, :) - com(j)
+       end do
+    end if
+    if (.not. m%rotate) then
+       ref = r
+       return
+    end if
+    I3 = 0.0d0
+    do a = 1, m%natoms
+       do j = 1, 3
+          do k = 1, 3
+             I3(j, k) = I3(j, k) - ms(a) * r(j, a) * r(k, a)
+          end do
+          I3(j, j) = I3(j, j) + ms(a) * sum(r(:, a)**2)
+       end do
+    end do
+    call jacobi3(I3, w, V)
+    ! ascending eigenvalues
+    do j = 1, 2
+       do k = j + 1, 3
+          if (w(k) < w(j)) then
+             tw = w(j); w(j) = w(k); w(k) = tw
+             tv = V(:, j); V(:, j) = V(:, k); V(:, k) = tv
+          end if
+       end do
+    end do
+    do j = 1, 3
+       imax = 1
+       do k = 2, 3
+          if (abs(V(k, j)) > abs(V(imax, j))) imax = k
+       end do
+       if (V(imax, j) < 0.0d0) V(:, j) = -V(:, j)
+    end do
+    do a = 1, m%natoms
+       do j = 1, 3
+          ref(j, a) = sum(r(:, a) * V(:, j))
+          if (abs(ref(j, a)) < 1.0d-12) ref(j, a) = 0.0d0
+       end do
+    end do
+  end subroutine principal_axes
+
+  ! eigenvalues w and eigenvectors V (columns) of a symmetric 3x3 matrix by
+  ! cyclic Jacobi rotations (Golub & Van Loan, Matrix Computations, 8.5)
+  subroutine jacobi3(A0, w, V)
+    real(c_double), intent(in) :: A0(3, 3)
+    real(c_double), intent(out) :: w(3), V(3, 3)
+    real(c_double) :: A(3, 3), th, t, c, sn, tau, apq, app, aqq, off, scale
+    integer :: sweep, p, q, k
+    A = A0
+    V = 0.0d0
+    do k = 1, 3
+       V(k, k) = 1.0d0
+    end do
+    scale = max(maxval(abs(A)), tiny(1.0d0))
+    do sweep = 1, 50
+       off = abs(A(1, 2)) + abs(A(1, 3)) + abs(A(2, 3))
+       if (off <= 1.0d-17 * scale) exit
+       do p = 1, 2
+          do q = p + 1, 3
+             apq = A(p, q)
+             if (apq == 0.0d0) cycle
+             app = A(p, p)
+             aqq = A(q, q)
+             th = (aqq - app) / (2.0d0 * apq)
+             t = sign(1.0d0, th) / (abs(th) + sqrt(th * th + 1.0d0))
+             c = 1.0d0 / sqrt(t * t + 1.0d0)
+             sn = t * c
+             tau = sn / (1.0d0 + c)
+             A(p, p) = app - t * apq
+             A(q, q) = aqq + t * apq
+             A(p, q) = 0.0d0
+             A(q, p) = 0.0d0
+             do k = 1, 3
+                if (k == p .or. k == q) cycle
+                th = A(k, p)
+                A(k, p) = th - sn * (A(k, q) + tau * th)
+                A(p, k) = A(k, p)
+                A(k, q) = A(k, q) + sn * (th - tau * A(k, q))
+                A(q, k) = A(k, q)
+             end do
+             do k = 1, 3
+                th = V(k, p)
+                V(k, p) = th - sn * (V(k, q) + tau * th)
+                V(k, q) = V(k, q) + sn * (th - tau * V(k, q))
+             end do
+          end do
+       end do
+    end do
+    do k = 1, 3
+       w(k) = A(k, k)
+    end do
+  end subroutine jacobi3
 
 end module music_inputs

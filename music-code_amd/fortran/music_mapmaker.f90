@@ -13,18 +13,12 @@ program music_mapmaker
   use music_hip_mod
   use music_inputs
   implicit none
-  character(len=LL) :: ctrl, base, s, fname, arg
-  type(text_file) :: t
-  type(atom_type), allocatable :: atoms(:)
-  type(molecule), allocatable :: mols(:)
-  type(lj_pair), allocatable :: pairs(:)
+  character(len=LL) :: ctrl, fname, arg
+  type(sim_system), target :: sy
   type(map_job), allocatable :: jobs(:)
   type(ewald_opts) :: ew
-  integer :: i, k, n, nat, nmol, npairs, njobs, ifw, iprobe, ptype, nxyz(3), ax, c, ntot, nt
-  character(len=NL) :: fwname, aafile, ssfile
-  real(c_double) :: fund(3), full(3), off(3)
-  real(c_double), allocatable :: xyz(:,:), q(:), ljA(:), ljB(:), h2(:), l2(:)
-  integer(c_int), allocatable :: typ(:)
+  integer :: i, k, nat, njobs, ifw, iprobe, ptype, nt
+  real(c_double), allocatable :: ljA(:), ljB(:), h2(:), l2(:)
   integer(c_int8_t), allocatable :: on(:)
   integer(c_int) :: prec, nk, npnts(3)
   integer(c_int64_t) :: nfsize, counts(4)
@@ -44,97 +38,38 @@ program music_mapmaker
   prec = MH_PREC_FP64
   call get_command_argument(2, arg)
   if (trim(lower(arg)) == 'fp32') prec = MH_PREC_FP32
-  base = dirname(ctrl)
-  call read_text(ctrl, t)
-
-  ! ---- atom types
-  i = find_section(t, 'Atomic Types')
-  call need(i, 'Atomic Types')
-  read(strip(t%line(i)), *) nat
-  allocate(atoms(nat))
-  i = i + 1
-  do k = 1, nat
-     i = next_nonblank(t, i)
-     atoms(k)%name = word(strip(t%line(i)), 1)
-     s = word(strip(t%line(i + 1)), 1)
-     call read_atom(find_file(s, 'ATOMSDIR', 'atoms', base), atoms(k))
-     i = i + 2
-  end do
-  ! ---- molecule types
-  i = find_section(t, 'Molecule Types')
-  call need(i, 'Molecule Types')
-  read(strip(t%line(i)), *) nmol
-  allocate(mols(nmol))
-  i = i + 1
-  do k = 1, nmol
-     i = next_nonblank(t, i)
-     mols(k)%name = word(strip(t%line(i)), 1)
-     s = word(strip(t%line(i + 1)), 1)
-     call read_molecule(find_file(s, 'MOLSDIR', 'molecules', base), atoms, mols(k))
-     i = i + 2
-  end do
-  ! ---- simulation cell (fundamental cell from a molecule's Fundcell_Info)
-  i = find_section(t, 'Simulation Cell Information')
-  call need(i, 'Simulation Cell Information')
-  fwname = word(strip(t%line(i)), 1)
-  read(strip(t%line(i + 1)), *) nxyz
-  ifw = mol_index(fwname)
-  if (.not. mols(ifw)%has_fundcell) call fail('fundamental cell molecule has no Fundcell_Info')
-  if (any(abs(mols(ifw)%angles - 90.0d0) > 1.0d-9)) call fail('only orthorhombic cells are supported')
-  fund = mols(ifw)%ell
-  full = fund * nxyz
-  ! replicate the fixed species: x, then y, then z
-  n = mols(ifw)%natoms
-  ntot = n * nxyz(1) * nxyz(2) * nxyz(3)
-  allocate(xyz(3, ntot), typ(ntot), q(ntot))
-  xyz(:, 1:n) = mols(ifw)%xyz
-  typ(1:n) = mols(ifw)%atype
-  q(1:n) = mols(ifw)%q
-  do ax = 1, 3
-     do c = 1, nxyz(ax) - 1
-        off = 0.0d0
-        off(ax) = fund(ax) * c
-        do k = 1, n
-           xyz(:, c * n + k) = xyz(:, k) + off
-           typ(c * n + k) = typ(k)
-           q(c * n + k) = q(k)
-        end do
-     end do
-     n = n * nxyz(ax)
-  end do
-  ! ---- forcefield files
-  i = find_section(t, 'Forcefield Information')
-  call need(i, 'Forcefield Information')
-  aafile = word(strip(t%line(i + 2)), 1)
-  ssfile = word(strip(t%line(i + 3)), 1)
-  call read_atom_atom(find_file(aafile, 'NONE', '', base), pairs, npairs)
+  call read_system(ctrl, sy)
+  if (sy%ifw == 0) call fail('the map maker needs a fundamental-cell species')
+  nat = sy%nat
+  ifw = sy%ifw
   ! ---- map jobs
-  i = find_section(t, 'Mapmaker Information')
+  i = find_section(sy%t, 'Mapmaker Information')
   call need(i, 'Mapmaker Information')
-  read(strip(t%line(i)), *) njobs
+  read(strip(sy%t%line(i)), *) njobs
   allocate(jobs(njobs))
   i = i + 1
   do k = 1, njobs
-     i = next_nonblank(t, i)
-     jobs(k)%mapsorb = word(strip(t%line(i)), 1)
-     jobs(k)%probe = word(strip(t%line(i + 1)), 1)
-     jobs(k)%inter = word(strip(t%line(i + 2)), 1)
-     jobs(k)%model = upper(word(strip(t%line(i + 2)), 2))
-     read(word(strip(t%line(i + 3)), 1), *) jobs(k)%gridspace
-     read(word(strip(t%line(i + 4)), 1), *) jobs(k)%hipotcut
-     jobs(k)%fname = word(strip(t%line(i + 5)), 1)
+     i = next_nonblank(sy%t, i)
+     jobs(k)%mapsorb = word(strip(sy%t%line(i)), 1)
+     jobs(k)%probe = word(strip(sy%t%line(i + 1)), 1)
+     jobs(k)%inter = word(strip(sy%t%line(i + 2)), 1)
+     jobs(k)%model = upper(word(strip(sy%t%line(i + 2)), 2))
+     read(word(strip(sy%t%line(i + 3)), 1), *) jobs(k)%gridspace
+     read(word(strip(sy%t%line(i + 4)), 1), *) jobs(k)%hipotcut
+     jobs(k)%fname = word(strip(sy%t%line(i + 5)), 1)
      i = i + 6
   end do
 
   call check(mh_ctx_create(0_c_int, ctx), 'mh_ctx_create')
-  call check(mh_framework_upload(ctx, int(ntot, c_int), xyz, typ, q, fund, full), 'mh_framework_upload')
+  call check(mh_framework_upload(ctx, int(sy%ntot, c_int), sy%xyz, sy%typ, sy%q, sy%fund, sy%full), &
+       'mh_framework_upload')
   nt = nat
   allocate(ljA(nt), ljB(nt), h2(nt), l2(nt), on(nt))
   do k = 1, njobs
-     if (mol_index(jobs(k)%mapsorb) /= ifw) call fail('the mapped species must be the fundamental cell species')
-     iprobe = mol_index(jobs(k)%probe)
-     if (mols(iprobe)%natoms /= 1) call fail('the probe must be a single atom (config.F90:754-759)')
-     ptype = mols(iprobe)%atype(1)
+     if (mol_index(sy, jobs(k)%mapsorb) /= ifw) call fail('the mapped species must be the fundamental cell species')
+     iprobe = mol_index(sy, jobs(k)%probe)
+     if (sy%mols(iprobe)%natoms /= 1) call fail('the probe must be a single atom (config.F90:754-759)')
+     ptype = sy%mols(iprobe)%atype(1)
      call system_clock(t0, rate)
      if (trim(jobs(k)%model) == 'LJ') then
         call probe_table(ptype)
@@ -142,10 +77,10 @@ program music_mapmaker
         call check(mh_map_build(ctx, MH_MAP_LJ, prec, 1.0_c_double, jobs(k)%gridspace, jobs(k)%hipotcut, &
              0_c_int, map), 'mh_map_build')
      else if (trim(jobs(k)%model) == 'EWALD') then
-        ew = read_ewald(find_file(ssfile, 'NONE', '', base), jobs(k)%probe, jobs(k)%mapsorb)
+        ew = read_ewald(find_file(sy%ssfile, 'NONE', '', sy%base), jobs(k)%probe, jobs(k)%mapsorb)
         if (.not. ew%found) call fail('no EWALD entry for the probe and map species in the spc_spc file')
         call check(mh_ewald_set(ctx, ew%kappa, int(ew%kmax, c_int), ew%locut, nk), 'mh_ewald_set')
-        call check(mh_map_build(ctx, MH_MAP_EWALD, prec, mols(iprobe)%q(1), jobs(k)%gridspace, &
+        call check(mh_map_build(ctx, MH_MAP_EWALD, prec, sy%mols(iprobe)%q(1), jobs(k)%gridspace, &
              jobs(k)%hipotcut, 0_c_int, map), 'mh_map_build')
      else
         call fail('unsupported map model ' // trim(jobs(k)%model))
@@ -169,19 +104,6 @@ program music_mapmaker
 
 contains
 
-  integer function mol_index(name)
-    character(len=*), intent(in) :: name
-    integer :: j
-    do j = 1, nmol
-       if (trim(mols(j)%name) == trim(name)) then
-          mol_index = j
-          return
-       end if
-    end do
-    call fail('unknown species ' // trim(name))
-    mol_index = -1
-  end function mol_index
-
   ! LJ parameters of the probe type against every framework atom type
   subroutine probe_table(pt)
     integer, intent(in) :: pt
@@ -192,26 +114,16 @@ contains
     l2 = 0.0d0
     on = 0_c_int8_t
     do a = 1, nt
-       j = pair_index(atoms(a)%name, atoms(pt + 1)%name)
+       j = pair_index(sy, sy%atoms(a)%name, sy%atoms(pt + 1)%name)
        if (j == 0) cycle
-       if (.not. pairs(j)%ok) cycle
-       ljA(a) = pairs(j)%A
-       ljB(a) = pairs(j)%B
-       h2(a) = pairs(j)%hicut**2
-       l2(a) = pairs(j)%locut**2
+       if (.not. sy%pairs(j)%ok) cycle
+       ljA(a) = sy%pairs(j)%A
+       ljB(a) = sy%pairs(j)%B
+       h2(a) = sy%pairs(j)%hicut**2
+       l2(a) = sy%pairs(j)%locut**2
        on(a) = 1_c_int8_t
     end do
   end subroutine probe_table
-
-  integer function pair_index(n1, n2)
-    character(len=*), intent(in) :: n1, n2
-    integer :: j
-    pair_index = 0
-    do j = 1, npairs
-       if ((trim(pairs(j)%a1) == trim(n1) .and. trim(pairs(j)%a2) == trim(n2)) .or. &
-           (trim(pairs(j)%a1) == trim(n2) .and. trim(pairs(j)%a2) == trim(n1))) pair_index = j
-    end do
-  end function pair_index
 
   ! map header atom records (mapmaker_hackheader, mapmaker.F90:372-388): one
   ! per atom type of the mapped species, ascending; the charge is that of the
@@ -226,8 +138,8 @@ contains
     nz = 0
     do a = 0, nat - 1
        seen = .false.
-       do j = 1, mols(ifw)%natoms
-          if (mols(ifw)%atype(j) == a) seen = .true.
+       do j = 1, sy%mols(ifw)%natoms
+          if (sy%mols(ifw)%atype(j) == a) seen = .true.
        end do
        if (seen) then
           nz = nz + 1
@@ -240,38 +152,26 @@ contains
     zpar = 0.0
     do r = 1, nz
        a = ztypes(r) + 1
-       do j = 1, min(25, len_trim(atoms(a)%name))
-          znames(25 * (r - 1) + j) = atoms(a)%name(j:j)
+       do j = 1, min(25, len_trim(sy%atoms(a)%name))
+          znames(25 * (r - 1) + j) = sy%atoms(a)%name(j:j)
        end do
-       do j = 1, min(2, len_trim(atoms(a)%symbol))
-          zsyms(2 * (r - 1) + j) = atoms(a)%symbol(j:j)
+       do j = 1, min(2, len_trim(sy%atoms(a)%symbol))
+          zsyms(2 * (r - 1) + j) = sy%atoms(a)%symbol(j:j)
        end do
-       if (r <= mols(ifw)%natoms) zpar(1, r) = real(mols(ifw)%q(r), c_float)
+       if (r <= sy%mols(ifw)%natoms) zpar(1, r) = real(sy%mols(ifw)%q(r), c_float)
        if (lj) then
-          j = pair_index(atoms(a)%name, atoms(pt + 1)%name)
+          j = pair_index(sy, sy%atoms(a)%name, sy%atoms(pt + 1)%name)
           if (j > 0) then
-             if (pairs(j)%ok) then
-                zpar(2, r) = real(pairs(j)%A, c_float)
-                zpar(3, r) = real(pairs(j)%B, c_float)
-                zpar(6, r) = real(pairs(j)%hicut, c_float)
-                zpar(7, r) = real(pairs(j)%locut, c_float)
+             if (sy%pairs(j)%ok) then
+                zpar(2, r) = real(sy%pairs(j)%A, c_float)
+                zpar(3, r) = real(sy%pairs(j)%B, c_float)
+                zpar(6, r) = real(sy%pairs(j)%hicut, c_float)
+                zpar(7, r) = real(sy%pairs(j)%locut, c_float)
              end if
           end if
        end if
     end do
   end subroutine header_atoms
-
-  subroutine need(i, tag)
-    integer, intent(in) :: i
-    character(len=*), intent(in) :: tag
-    if (i < 0) call fail('ctrlfile section not found: ' // tag)
-  end subroutine need
-
-  subroutine fail(msg)
-    character(len=*), intent(in) :: msg
-    write(*, '(a)') trim(msg)
-    stop 1
-  end subroutine fail
 
   subroutine check(rc, what)
     integer(c_int), intent(in) :: rc

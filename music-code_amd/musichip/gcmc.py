@@ -42,6 +42,7 @@ def _sig():
         "mh_gcmc_run": [_vp, _i],
         "mh_gcmc_get_state": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "mh_gcmc_set_rng": [_vp, _i, ctypes.c_int32, ctypes.c_int32],
+        "mh_gcmc_get_moments": [_vp, _vp, _vp],
         "mh_gcmc_get_config": [_vp, _i, P(_i), _vp, _vp, _vp],
         "mh_gcmc_total_energy": [_vp, _vp, _i],
         "mh_gcmc_set_trace": [_vp, _i],
@@ -155,7 +156,9 @@ class GcmcBatch:
         iy = np.zeros(R, dtype=np.int32)
         H.check(_sig().mh_gcmc_get_state(self.h, _p(N), _p(e), _p(sN), _p(ns), _p(att), _p(acc), _p(ix),
                                          _p(iy)), "mh_gcmc_get_state")
-        return dict(N=N, energy=e, sumN=sN, nsteps=ns, att=att, acc=acc, ix=ix, iy=iy)
+        sN2 = np.zeros(R)
+        H.check(_sig().mh_gcmc_get_moments(self.h, None, _p(sN2)), "mh_gcmc_get_moments")
+        return dict(N=N, energy=e, sumN=sN, sumN2=sN2, nsteps=ns, att=att, acc=acc, ix=ix, iy=iy)
 
     def set_rng(self, rep, ix, iy):
         H.check(_sig().mh_gcmc_set_rng(self.h, rep, int(ix), int(iy)), "mh_gcmc_set_rng")

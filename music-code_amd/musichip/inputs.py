@@ -93,7 +93,8 @@ def read_atom(path):
 # ------------------------------------------------------------- molecules
 def read_molecule(path):
     """Rigid 'Listed Cartesian' molecule file.  Returns dict with names,
-    xyz (as listed), charges, transform flag and optional fundcell."""
+    xyz (as listed), charges, rigid / translate / rotate flags and optional
+    fundcell."""
     L = _lines(path)
     mol = {"fundcell": None}
     i = 0
@@ -104,7 +105,12 @@ def read_molecule(path):
         elif s.startswith("Coord_Info:"):
             info = s.split(":", 1)[1].split()
             mol["coordinfo"] = info
-            mol["transform"] = not any(w.lower() == "notransform" for w in info)
+            # molecule.F90:1008-1028: info = 'Listed', coordsys, gcmodel, flag
+            flag = info[3].upper() if len(info) > 3 else ""
+            mol["rigid"] = len(info) > 2 and info[2].upper() == "RIGID"
+            mol["translate"] = flag != "NOTRANSFORM"
+            mol["rotate"] = flag not in ("NOROTATE", "NOTRANSFORM")
+            mol["transform"] = mol["translate"]
             i += 1
             n = _ints(L[i])[0]
             names, xyz, q = [], [], []
@@ -129,20 +135,25 @@ def read_molecule(path):
     return mol
 
 
-def principal_axes(xyz, masses):
+def principal_axes(xyz, masses, translate=True, rotate=True):
     """molecule_genprincipleaxes (molecule.F90:830-926): shift to the centre
     of mass computed with INTEGER-truncated masses (molecule.F90:833) and
     rotate onto the principal axes.  Eigenvectors come from numpy (the
-    reference uses EISPACK tred2/tql2); axis order ascending eigenvalue."""
+    reference uses EISPACK tred2/tql2); axis order ascending eigenvalue,
+    each eigenvector signed so that its component of largest magnitude is
+    positive (the first such on ties), as the Fortran driver does."""
     m = np.floor(np.asarray(masses, dtype=np.float64))
     com = (xyz * m[:, None]).sum(0) / m.sum()
-    r = xyz - com
-    if len(xyz) == 1:
+    r = xyz - com if translate else np.array(xyz, dtype=np.float64)
+    if not rotate:
         return r
     I = np.zeros((3, 3))
     for mi, v in zip(m, r):
         I += mi * (np.dot(v, v) * np.eye(3) - np.outer(v, v))
     w, V = np.linalg.eigh(I)
+    for j in range(3):
+        if V[np.argmax(np.abs(V[:, j])), j] < 0.0:
+            V[:, j] = -V[:, j]
     out = r @ V
     out[np.abs(out) < 1e-12] = 0.0
     return out
@@ -297,7 +308,7 @@ def read_atom_atom(path):
                     d.update(off=True, hicut=0.0, hicut2=0.0)
                 elif k == "SIG":
                     if "MIX" in v.upper():
-                        d["mix"] = True
+                        d["mix"] = v.upper()
                     else:
                         d["sig"] = float(v)
                 elif k == "EPS":
@@ -313,6 +324,20 @@ def read_atom_atom(path):
                 d["A"], d["B"] = lj_AB(d["sig"], d["eps"])
         out[(a1, a2)] = d
         out[(a2, a1)] = d
+    # mixed pairs from the pure ones (lj_calc_mix, lj.F90:905-979): sigma
+    # arithmetic (LBMIX) or geometric (JORGMIX), epsilon geometric
+    for (a1, a2), d in out.items():
+        if d["model"] != "LJ" or d["off"] or not d.get("mix") or "A" in d:
+            continue
+        p1, p2 = out.get((a1, a1)), out.get((a2, a2))
+        if not p1 or not p2 or p1.get("sig") is None or p2.get("sig") is None:
+            raise ValueError("mixing rule without the pure pairs: " + d["line"])
+        if d["mix"] == "JORGMIX":
+            d["sig"] = float(np.sqrt(p1["sig"] * p2["sig"]))
+        else:
+            d["sig"] = (p1["sig"] + p2["sig"]) / 2.0
+        d["eps"] = float(np.sqrt(p1["eps"] * p2["eps"]))
+        d["A"], d["B"] = lj_AB(d["sig"], d["eps"])
     return out
 
 
@@ -372,8 +397,10 @@ def load_system(ctrl_path):
         d["type"] = np.array([aidx[n] for n in d["atoms"]], dtype=np.int32)
         masses = [atypes[t]["mass"] for t in d["type"]]
         d["mass"] = float(np.sum(masses))
-        if d["transform"] and len(d["xyz"]) > 1:
-            d["ref"] = principal_axes(d["xyz"], masses)
+        # molecule_coordinit (molecule.F90:714-716): rigid molecules only,
+        # and molecule_genprincipleaxes returns for fewer than 3 atoms (:843)
+        if d.get("rigid") and len(d["xyz"]) >= 3:
+            d["ref"] = principal_axes(d["xyz"], masses, d["translate"], d["rotate"])
         else:
             d["ref"] = d["xyz"].copy()
         mols[m["name"]] = d

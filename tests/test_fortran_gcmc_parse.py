@@ -1,0 +1,104 @@
+"""The Fortran music_gcmc driver's ctrlfile reading, on the CPU ('check'
+stops before any device call): the parameters it would hand to
+mh_gcmc_create for test007 and for the two fixtures, against values built
+independently from the fixture numbers (and, for the principal axes and the
+LBMIX pair, from the Python host's readers), and the reference's pressure
+list forms (gcmcmoves_getpressures / gcmcmoves_genpressures,
+gcmcmoves.F90:430-646)."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from music_gcmc_run import FIX, GCMC, GOLD, bent_spec, parse_check, probe_spec, spec_test007
+
+pytestmark = pytest.mark.skipif(not os.path.exists(GCMC),
+                                reason="music_gcmc not built (make -C music-code_amd fortran)")
+
+
+def _check(ctrl, cwd=None):
+    out = subprocess.run([GCMC, ctrl, "check"], capture_output=True, text=True, timeout=60, cwd=cwd)
+    assert out.returncode == 0, out.stdout + out.stderr
+    return parse_check(out.stdout)
+
+
+def _same_spec(d, spec, npress):
+    assert len(d["sites"]) == len(spec["ref"])
+    for s, ref, t, q in zip(d["sites"], spec["ref"], spec["stype"], spec["sq"]):
+        assert np.allclose(s["ref"], ref, rtol=0, atol=1e-12)
+        assert s["type"] == t and s["q"] == pytest.approx(q, abs=1e-12)
+    A, B, h2, l2, on = spec["lj"]
+    for k in range(spec["nst"] ** 2):
+        if on[k]:
+            assert np.allclose(d["lj"][k], [A[k], B[k], h2[k], l2[k]], rtol=1e-14, atol=0)
+        else:
+            assert k not in d["lj"]
+    assert [m[0] for m in d["moves"]] == [m[0] for m in spec["moves"]]
+    assert d["delta"] == (pytest.approx(spec.get("dtrans", 0.0)), pytest.approx(spec.get("drot", 0.0)))
+    assert len(d["press"]) == npress
+    ew = spec.get("ewald")
+    if ew:
+        assert d["ewald"]["kappa"] == pytest.approx(ew["kappa"], rel=1e-15)
+        assert d["ewald"]["kmax"] == ew["kmax"]
+        assert d["ewald"]["rc"] == pytest.approx(ew["rc"], rel=1e-15)
+    else:
+        assert d["ewald"] is None
+
+
+def test_test007_parameters():
+    d = _check(os.path.join(GOLD, "test007-gcmc-CH4", "test007.ctr"))
+    assert (d["niter"], d["nmicro"], d["iprint"], d["block"], d["seed"]) == (20000, 1, 1000, 100, 22111971)
+    assert d["press"] == [10000.0]
+    _same_spec(d, spec_test007(), 1)
+    assert d["sites"][0]["pmap"] == "-" and d["sites"][0]["emap"] == "-"
+
+
+def test_probe_with_maps_parameters():
+    d = _check(os.path.join(FIX, "gcmc.ctr"))
+    assert d["press"] == [10.0, 100.0, 1000.0]
+    _same_spec(d, probe_spec(), 3)
+    assert d["sites"][0]["pmap"] == "Lattice.LJ.map" and d["sites"][0]["emap"] == "Lattice.EWALD.map"
+
+
+def test_bent_sorbate_parameters():
+    """Principal axes (Jacobi in the driver, numpy in the Python host, same
+    sign convention), the LBMIX pair and the Ewald parameters."""
+    d = _check(os.path.join(FIX, "bent.ctr"))
+    spec = bent_spec()
+    _same_spec(d, spec, 1)
+    ref = np.array(spec["ref"])
+    # principal frame: centre of (truncated) mass at the origin, inertia diagonal, ascending
+    m = np.array([15.0, 1.0, 1.0])
+    assert np.allclose((ref * m[:, None]).sum(0), 0.0, atol=1e-12)
+    inertia = sum(mi * (v @ v * np.eye(3) - np.outer(v, v)) for mi, v in zip(m, ref))
+    assert np.allclose(inertia - np.diag(np.diag(inertia)), 0.0, atol=1e-9)
+    assert np.all(np.diff(np.diag(inertia)) >= 0)
+    # the same distances as listed
+    from musichip import inputs
+    xyz = inputs.read_molecule(os.path.join(FIX, "Bent.mol"))["xyz"]
+    for i in range(3):
+        for j in range(3):
+            assert np.linalg.norm(ref[i] - ref[j]) == pytest.approx(np.linalg.norm(xyz[i] - xyz[j]), abs=1e-12)
+
+
+@pytest.mark.parametrize("line,npts,want", [
+    ("1, 1000", 4, [1.0, 10.0, 100.0, 1000.0]),          # > 2 decades: geometric
+    ("100, 200", 3, [100.0, 150.0, 200.0]),              # linear
+    ("5, 50, 500, +", 5, None),                          # continued on the next line
+])
+def test_pressure_lists(tmp_path, line, npts, want):
+    shutil.copytree(FIX, tmp_path / "fx")
+    ctrl = tmp_path / "fx" / "gcmc.ctr"
+    txt = ctrl.read_text()
+    txt = txt.replace("10, 100, 1000      # Fugacity (kPa)(Range)", line + "   # Fugacity (kPa)(Range)" +
+                      ("\n7000, 9000" if line.endswith("+") else ""))
+    txt = txt.replace("3                  # No. of simulation points", "%d                  # No. of simulation points" % npts)
+    ctrl.write_text(txt)
+    # the molecule and atom files live in tests/golden/potentials
+    shutil.copytree(os.path.join(GOLD, "potentials"), tmp_path / "potentials")
+    d = _check(str(ctrl))
+    if want is None:
+        want = [5.0, 50.0, 500.0, 7000.0, 9000.0]
+    assert np.allclose(d["press"], want, rtol=1e-14)

@@ -1,8 +1,12 @@
 """The Fortran iso_c_binding layer (music-code_amd/fortran/music_hip_mod.f90)
-driven by a Fortran program: test012's LJ map built through bind(C)
+and the Fortran drivers over it: test012's LJ map built through bind(C)
 interfaces gives the reference's counts and the same map-file bytes as the
-C-ABI called from Python."""
+C-ABI called from Python; music_mapmaker reads the reference's ctrlfiles and
+writes the same map files as the Python host; music_gcmc reproduces the
+reference's printed test007 statistics and matches oracle chains on map and
+Ewald systems, batched and chained."""
 import hashlib
+import json
 import os
 import subprocess
 import tempfile
@@ -10,8 +14,10 @@ import tempfile
 import numpy as np
 import pytest
 
+import music_gcmc_run as MG
 from musichip import hip as H
 from musichip import inputs
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -79,3 +85,136 @@ def test_fortran_music_mapmaker_matches_python_host(hip, case):
             assert ("%12d" * 4) % tuple(m.counts) in out.stdout.replace("\n", " ") or \
                 all(str(c) in out.stdout for c in m.counts)
             m.close()
+
+
+# ----------------------------------------------------------- music_gcmc
+REF7 = json.load(open(os.path.join(GOLD, "reference_outputs.json")))["test007"]
+
+
+def test_music_gcmc_reproduces_test007_output(hip):
+    """test007.ctr through the Fortran driver: at every printed step the
+    reference's acceptance counts and N exactly, its block / cumulative
+    averages and standard deviation of N to the printed 2 decimals, its final
+    energy and loading (tests/test007-gcmc-CH4/output_reference:168-361)."""
+    print("case: music_gcmc test007.ctr FP64", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        text = MG.run([os.path.join(GOLD, "test007-gcmc-CH4", "test007.ctr")], td)
+    blocks, ends = MG.parse(text)
+    assert len(blocks) == len(REF7["checkpoints"]) == 20
+    for b, cp in zip(blocks, REF7["checkpoints"]):
+        assert b["step"] == cp["step"]
+        assert b["moves"]["INSERT"] == tuple(cp["insert"])
+        assert b["moves"]["DELETE"] == tuple(cp["delete"])
+        assert b["moves"]["TRANSLATE"] == tuple(cp["translate"])
+        assert b["N"] == cp["N"]
+        for k, rk in (("blk", "N_blk"), ("cum", "N_cum"), ("std", "N_std")):
+            assert abs(b[k] - cp[rk]) <= 0.0101, (cp["step"], k, b[k], cp[rk])
+    e = ends[1]
+    assert abs(e["actual"] - REF7["final_energy_kJ"]) < 1.5e-3
+    assert abs(e["stored"] - REF7["final_energy_kJ"]) < 1.5e-3
+    assert e["dev"] < 1e-9
+    assert abs(e["loading"] - REF7["N_cum_step20000"]) <= 0.0101
+
+
+def _fixture_maps(hip, td):
+    """Lattice.LJ.map and Lattice.EWALD.map by music_mapmaker from test012.ctr
+    and test013.ctr, loaded back for the oracle."""
+    for case in ("test012-pmapgen/test012.ctr", "test013-emapgen/test013.ctr"):
+        out = subprocess.run([MG.MAPMAKER, os.path.join(GOLD, case)], cwd=td, capture_output=True, text=True,
+                             timeout=300)
+        print(out.stdout, flush=True)
+        assert out.returncode == 0, out.stdout + out.stderr
+    fund = [20.0, 20.0, 12.0]
+    L = np.array([20.0, 20.0, 24.0])
+    hip.upload_framework(np.zeros((1, 3)), np.zeros(1, dtype=np.int32), None, fund, L)
+    pm = hip.load_map(os.path.join(td, "Lattice.LJ.map"), fund)
+    em = hip.load_map(os.path.join(td, "Lattice.EWALD.map"), fund)
+    mlist = []
+    for m in (pm, em):
+        ptr, vals = m.get()
+        mlist.append((m.npnts, m.step, ptr, vals))
+        m.close()
+    return mlist
+
+
+def _oracle_blocks(g, steps, every=1000, block=100):
+    """(att/acc, N, block mean, cumulative mean) of an oracle chain at every
+    print step, its counters as the reference's statistics (block of the
+    last `block` steps)."""
+    out = []
+    base = g.st.nsteps
+    sumN0 = g.st.sumN
+    done = 0
+    while done < steps:
+        g.run(every - block)
+        s0 = g.st.sumN
+        g.run(block)
+        done += every
+        out.append(dict(att=[g.st.att[i] for i in range(4)], acc=[g.st.acc[i] for i in range(4)], N=g.st.N,
+                        blk=(g.st.sumN - s0) / block, cum=(g.st.sumN - sumN0) / (g.st.nsteps - base)))
+    return out
+
+
+def _compare(blocks, oracle, tags=("INSERT", "DELETE", "TRANSLATE", "ROTATE")):
+    assert len(blocks) == len(oracle)
+    for b, o in zip(blocks, oracle):
+        for i, t in enumerate(tags):
+            if t in b["moves"]:
+                assert b["moves"][t] == (o["acc"][i], o["att"][i]), (b["step"], t)
+        assert b["N"] == o["N"]
+        assert abs(b["blk"] - o["blk"]) <= 0.0051 and abs(b["cum"] - o["cum"]) <= 0.0051
+
+
+def test_music_gcmc_maps_batched_and_chained(hip):
+    """gcmc.ctr (probe in the charged lattice, LJ + Ewald maps from
+    music_mapmaker, 3 state points): the batched run (independent chains,
+    seeds iseed + p - 1) and the chained run (the reference's sequence:
+    configuration and random stream carried, statistics reset) against
+    oracle chains on the same maps, move for move at every printed step."""
+    print("case: music_gcmc gcmc-maps/gcmc.ctr, batched and chain, FP64", flush=True)
+    spec = MG.probe_spec()
+    press = [10.0, 100.0, 1000.0]
+    with tempfile.TemporaryDirectory() as td:
+        mlist = _fixture_maps(hip, td)
+        spec = dict(spec, maps=mlist, site_pmap=[0], site_emap=[1])
+        ctrl = os.path.join(MG.FIX, "gcmc.ctr")
+        blocks, ends = MG.parse(MG.run([ctrl], td))
+        for p in range(3):
+            g = O.Gcmc(spec, press[p], 1892134 + p)
+            _compare([b for b in blocks if b["sim"] == p + 1], _oracle_blocks(g, 3000))
+            assert abs(ends[p + 1]["stored"] - g.st.energy * 4.184) < 1.5e-3
+            assert ends[p + 1]["dev"] < 1e-8
+        # chained: one oracle chain through the three points
+        cblocks, cends = MG.parse(MG.run([ctrl, "chain"], td))
+        g = O.Gcmc(spec, press[0], 1892134)
+        vol = float(np.prod(spec["full_ell"]))
+        Ns = []
+        for p in range(3):
+            if p:
+                g.st.B = O.adams_B(press[p], vol, spec["T"])
+                for i in range(4):
+                    g.st.att[i] = 0
+                    g.st.acc[i] = 0
+                g.st.sumN, g.st.nsteps = 0.0, 0
+            _compare([b for b in cblocks if b["sim"] == p + 1], _oracle_blocks(g, 3000))
+            Ns.append(g.st.N)
+        assert [b["N"] for b in blocks if b["sim"] == 1] == [b["N"] for b in cblocks if b["sim"] == 1]
+        assert Ns[-1] > 0
+
+
+def test_music_gcmc_bent_sorbate_ewald(hip):
+    """bent.ctr (3-site rigid sorbate on its principal axes, LBMIX pair,
+    Ewald between sorbates, rotations) through the driver against the oracle
+    chain built from the fixture values, move for move; the FP32 build runs
+    the same input to within statistical agreement of the loading."""
+    print("case: music_gcmc gcmc-maps/bent.ctr FP64 and FP32", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        ctrl = os.path.join(MG.FIX, "bent.ctr")
+        blocks, ends = MG.parse(MG.run([ctrl], td))
+        g = O.Gcmc(MG.bent_spec(), 5000.0, 424242)
+        _compare(blocks, _oracle_blocks(g, 2000, every=500, block=250))
+        assert abs(ends[1]["stored"] - g.st.energy * 4.184) < 1.5e-3
+        assert ends[1]["dev"] < 1e-8
+        b32, e32 = MG.parse(MG.run([ctrl, "fp32"], td))
+        assert len(b32) == 4 and e32[1]["dev"] < 1e-3
+        assert b32[-1]["moves"]["ROTATE"][1] > 0
