@@ -143,7 +143,40 @@ int mh_map_interp(mh_ctx *ctx, const mh_map *map, int n, const double *xyz, doub
  * exclusion; no reference counterpart: the reference stops,
  * ssdriver.F90:489-498).  Failures (LJ locut, untabulated map corner,
  * U > max_nrg) reject the move without an RNG draw. */
-enum { MH_MV_INSERT = 0, MH_MV_DELETE = 1, MH_MV_TRANSLATE = 2, MH_MV_ROTATE = 3 };
+/* ------------------------------------------------------------- bias map
+ * Bias map of biased insertion / deletion (bmap_init + bmap_cubeinit,
+ * bmap.F90:69-403), built on the device from a pmap at bias_temperature
+ * (K): the cubelets whose lower-corner node is tabulated with V <= 30
+ * kcal/mol, in the reference's order (x outer, z inner), Boltzmann weights
+ * exp(-V / kT_bias) (exponent clamped to +-30; energies above
+ * -log(1e-6 sum) / rti lowered to it when min / sum <= 1e-6) and their
+ * cumulative probabilities, summed in that order. */
+typedef struct mh_bmap mh_bmap;
+int mh_bmap_create(mh_ctx *ctx, const mh_map *pmap, double bias_temperature, mh_bmap **out);
+int mh_bmap_destroy(mh_bmap *b);
+/* numtab: biased cubelets; ncubelets: all cubelets; cum_wt: the weight total;
+ * clamp_energy: the energy above which weights were clamped (+inf: none).
+ * Any pointer may be NULL. */
+int mh_bmap_info(const mh_bmap *b, int64_t *numtab, int64_t *ncubelets, double *cum_wt, double *clamp_energy);
+/* Per biased cubelet in list order: id (i * (ny - 1) + j) * (nz - 1) + k
+ * (0-based), weight, cumulative probability.  Any pointer may be NULL. */
+int mh_bmap_get(const mh_bmap *b, int32_t *cube, double *wt, double *cum_prob);
+/* The list position (0-based) the GCMC kernel's search returns for each key
+ * in [0, 1): that of bmap_getbiasindex (bmap.F90:575-636). */
+int mh_bmap_search(const mh_bmap *b, int n, const double *keys, int64_t *index);
+
+/* BINSERT / BDELETE: biased insertion and deletion (rigidmoves_binsert,
+ * rigidmoves.F90:1386-1442 with rigidmoves_comFromBmap :1938-1969;
+ * rigidmoves_bdelete :1578-1654) with the bias map params->bias.  A
+ * biased insertion draws a cubelet by weight, a point in it and a unit cell
+ * of the simulation cell, and enters the acceptance with 1 / (w ncubelets);
+ * a biased deletion enters with w ncubelets of the molecule's cubelet (w =
+ * weight / total).  Attempts and acceptances count as insert (index 0) and
+ * delete (index 1).  Deleting a molecule whose cubelet is not in the bias
+ * map stops the reference; here the replica stops with error 4 and
+ * mh_gcmc_run returns MH_ESTATE. */
+enum { MH_MV_INSERT = 0, MH_MV_DELETE = 1, MH_MV_TRANSLATE = 2, MH_MV_ROTATE = 3, MH_MV_BINSERT = 4,
+       MH_MV_BDELETE = 5 };
 #define MH_MAX_SITES 8
 
 typedef struct {
@@ -167,6 +200,8 @@ typedef struct {
     double delta_trans, delta_rot;       /* fixed displacements (A, rad) */
     double max_nrg;                      /* kcal/mol (1e10 in the reference) */
     int precision;                       /* MH_PREC_FP64 or MH_PREC_FP32 */
+    const mh_bmap *bias;                 /* bias map of BINSERT / BDELETE, or NULL;
+                                            kept alive by the caller while the batch lives */
 } mh_gcmc_params;
 
 typedef struct mh_gcmc mh_gcmc;

@@ -31,6 +31,7 @@
 #include "mh_device.h"
 #include "mh_internal.h"
 #include "mh_mc_common.h"
+#include "mh_bmap_dev.h"
 
 namespace mh {
 
@@ -61,6 +62,8 @@ struct GDev {
     double cumwt[4];
     double insdel, dtrans, drot, maxnrg, rti, molconst;
     double inv_ns, inv_nss;  // 1/ns, 1/ns^2 for the pair-item decomposition
+    BiasRef bias;            // BINSERT / BDELETE
+    int ncell[3];            // unit cells per axis of the simulation cell
     int prec, maxN;
     // e^{i (kx x + ky y)} table per (set, site) in dynamic LDS at byte
     // offset exy_off: [2][ns][kmax + 1][2 kmax + 1] (double2, float2 in FP32)
@@ -672,19 +675,41 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
         }
         if (mv >= P.nmoves) mv = P.nmoves - 1;
         const int mtype = P.mtype[mv];
+        const bool biased = mtype >= MH_MV_BINSERT;
+        const int mbase = biased ? mtype - MH_MV_BINSERT : mtype;  // INSERT / DELETE for the biased pair
+        double biasf = 1.0;  // biasfactor of rigidmoves_binsert / rigidmoves_bdelete
         int accepted = 0;
         double de = 0.0;
         bool doit = true;
         int mol = -1;
         // ---- trial generation (identical in every lane)
         double tcom[3], tang[3] = {0.0, 0.0, 0.0};
-        if (mtype == MH_MV_INSERT) {
+        if (mbase == MH_MV_INSERT) {
             att[0]++;
             if (N >= P.maxN) {
                 err = 2;  // capacity
                 break;
             }
-            for (int i = 0; i < 3; i++) tcom[i] = __dmul_rn(P.full[i], rranf(rng));
+            if (!biased) {
+                for (int i = 0; i < 3; i++) tcom[i] = __dmul_rn(P.full[i], rranf(rng));
+            } else {
+                // rigidmoves_comFromBmap: a cubelet by weight, a point in it
+                // (bmap_getbiaspt), a unit cell of the simulation cell
+                const BiasRef &Bm = P.bias;
+                const long long t = bias_index(Bm, rranf(rng));
+                const long long c = Bm.cube[t];
+                const long long ci[3] = {c / ((long long)Bm.ncub[1] * Bm.ncub[2]), (c / Bm.ncub[2]) % Bm.ncub[1],
+                                         c % Bm.ncub[2]};
+                biasf = __ddiv_rn(__ddiv_rn(1.0, Bm.wn[t]), Bm.ncubelets);
+                for (int i = 0; i < 3; i++) tcom[i] = __dmul_rn((double)ci[i], Bm.step[i]);
+                for (int i = 0; i < 3; i++) tcom[i] = __dadd_rn(tcom[i], __dmul_rn(rranf(rng), Bm.step[i]));
+                for (int i = 0; i < 3; i++)
+                    if (tcom[i] >= Bm.eff[i]) tcom[i] = __dmul_rn((double)0.9999990f, Bm.eff[i]);
+                for (int i = 0; i < 3; i++) {
+                    const int icell = (int)(rranf(rng) * (double)P.ncell[i]);
+                    tcom[i] = __dadd_rn(tcom[i], __dmul_rn(P.fund[i], (double)icell));
+                }
+            }
             if (ns != 1) {
                 double costheta = __dsub_rn(1.0, __dmul_rn(rranf(rng), 2.0));
                 tang[1] = acos(costheta);
@@ -696,11 +721,26 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                 doit = false;
             } else {
                 mol = (int)(rranf(rng) * N);
-                if (mtype == MH_MV_DELETE) att[1]++;
-                else att[mtype]++;
+                att[mbase]++;
                 for (int i = 0; i < 3; i++) {
                     tcom[i] = com[3 * mol + i];
                     tang[i] = ang[3 * mol + i];
+                }
+                if (mbase == MH_MV_DELETE && biased) {
+                    // rigidmoves_bdelete: bmap_getcellwt of the centre of mass
+                    // mapped into the unit cell (simcell_maptouc)
+                    const BiasRef &Bm = P.bias;
+                    long long ci[3];
+                    for (int i = 0; i < 3; i++) {
+                        ci[i] = (long long)(maptouc(tcom[i], P.fund[i]) / Bm.step[i]);
+                        if (ci[i] > Bm.ncub[i] - 1) ci[i] = Bm.ncub[i] - 1;
+                    }
+                    const int t = Bm.cindex[(ci[0] * Bm.ncub[1] + ci[1]) * Bm.ncub[2] + ci[2]];
+                    if (t == 0) {
+                        err = 4;  // the reference stops: no bias map index (bmap.F90:469-477)
+                        break;
+                    }
+                    biasf = __dmul_rn(Bm.ncubelets, Bm.wn[t - 1]);
                 }
                 if (mtype == MH_MV_TRANSLATE) {
                     for (int i = 0; i < 3; i++) {
@@ -724,13 +764,13 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
             }
         }
         if (doit) {
-            const bool hasA = (mtype != MH_MV_DELETE);
-            const bool hasB = (mtype != MH_MV_INSERT);
+            const bool hasA = (mbase != MH_MV_DELETE);
+            const bool hasB = (mbase != MH_MV_INSERT);
             // ---- site coordinates: set B = the molecule as stored; set A =
             // the trial (or, for a deletion, the stored molecule)
             const int n3 = 3 * ns;
             if (hasB && tid >= 64 - n3 && tid < 64) sh.posB[tid - (64 - n3)] = pos[(long long)mol * n3 + tid - (64 - n3)];
-            if (mtype == MH_MV_DELETE) {
+            if (mbase == MH_MV_DELETE) {
                 if (tid < n3) sh.posA[tid] = pos[(long long)mol * n3 + tid];
             } else if (ns == 1) {
                 if (tid < 3) sh.posA[tid] = wrap_full(tcom[tid], P.full[tid]);
@@ -751,7 +791,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
             }
             __syncthreads();
             double o[6];
-            if (mtype == MH_MV_DELETE) {
+            if (mbase == MH_MV_DELETE) {
                 // the molecule's energy in the current system: its sites as set A
                 move_energy<NT, PREC, KPL>(P, sh, pos, N, mol, true, false, -1.0, K, S, dS, exy, o);
             } else {
@@ -759,12 +799,12 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
             }
             const double mA = o[0], pA = o[1], mB = o[2], pB = o[3], eK = o[4];
             bool ok = o[5] != 0.0;
-            if (mtype == MH_MV_INSERT) {
+            if (mbase == MH_MV_INSERT) {
                 double U = __dadd_rn(mA, pA);
                 if (P.ewald) U = U + eK + P.molconst;
                 if (ok && U > P.maxnrg) ok = false;
                 if (ok) {
-                    double bias = 1.0 / P.insdel;
+                    double bias = biasf / P.insdel;
                     double u = -P.rti * U;
                     u = u + log(bias);
                     u = u + B;
@@ -784,14 +824,14 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                         de = U;
                     }
                 }
-            } else if (mtype == MH_MV_DELETE) {
+            } else if (mbase == MH_MV_DELETE) {
                 if (!ok) {
                     err = 3;  // an existing molecule failed to evaluate (mcmoves.F90:599-603)
                     break;
                 }
                 double U = __dadd_rn(mA, pA);
                 if (P.ewald) U = U - eK + P.molconst;
-                double bias = 1.0 * P.insdel;
+                double bias = biasf * P.insdel;
                 double u = -P.rti * (-U);
                 u = u + log(bias);
                 u = u + (-B);
@@ -1299,13 +1339,31 @@ static int build_system(mh_ctx *ctx, const mh_gcmc_params *p, int maxN, const do
     for (int i = 0; i < p->nmoves; i++) wsum += p->move_weight[i];
     double c = 0.0;
     for (int i = 0; i < p->nmoves; i++) {
-        MH_CHECK(p->move_type[i] >= 0 && p->move_type[i] <= 3, MH_EINVAL, w + ": bad move type");
-        H.mtype[i] = p->move_type[i];
+        const int mt = p->move_type[i];
+        MH_CHECK(mt >= 0 && mt <= MH_MV_BDELETE, MH_EINVAL, w + ": bad move type");
+        MH_CHECK(mt < MH_MV_BINSERT || p->bias, MH_EINVAL, w + ": BINSERT / BDELETE need a bias map");
+        H.mtype[i] = mt;
         double wn = p->move_weight[i] / wsum;
         c = c + wn;
         H.cumwt[i] = c;
-        if (p->move_type[i] == MH_MV_INSERT) wi = wn;
-        if (p->move_type[i] == MH_MV_DELETE) wd = wn;
+        // mcmoveset_setInsDelRatio: the basic INSERT / DELETE tags
+        if (mt == MH_MV_INSERT || mt == MH_MV_BINSERT) {
+            MH_CHECK(wi == 0.0, MH_EINVAL, w + ": one insertion move per move set");
+            wi = wn;
+        }
+        if (mt == MH_MV_DELETE || mt == MH_MV_BDELETE) {
+            MH_CHECK(wd == 0.0, MH_EINVAL, w + ": one deletion move per move set");
+            wd = wn;
+        }
+    }
+    for (int i = 0; i < 3; i++) H.ncell[i] = (int)std::lround(full[i] / fund[i]);
+    if (p->bias) {
+        const mh_bmap *bm = p->bias;
+        MH_CHECK(bm->ctx == ctx, MH_EINVAL, w + ": the bias map belongs to another context");
+        for (int i = 0; i < 3; i++)
+            MH_CHECK(std::fabs(bm->eff[i] - fund[i]) <= 1e-9 * fund[i], MH_EINVAL,
+                     w + ": the bias map's cell differs from the fundamental cell");
+        H.bias = bias_ref(bm);
     }
     H.insdel = (wi > 0.0 && wd > 0.0) ? wi / wd : 1.0;  // mcmoveset.F90:424-430
     H.dtrans = p->delta_trans;
@@ -1627,6 +1685,8 @@ int mh_gcmc_run(mh_gcmc *g, int nsteps) {
     MH_HIP(hipMemcpy(err.data(), g->err.p, 4 * g->nrep, hipMemcpyDeviceToHost));
     for (int i = 0; i < g->nrep; i++) {
         MH_CHECK(err[i] != 2, MH_ECAPACITY, "mh_gcmc_run: a replica reached maxN molecules");
+        MH_CHECK(err[i] != 4, MH_ESTATE,
+                 "mh_gcmc_run: BDELETE picked a molecule whose cubelet is not in the bias map");
         MH_CHECK(err[i] == 0, MH_ESTATE, "mh_gcmc_run: an existing molecule failed to evaluate");
     }
     g->tr_base = g->tr_base;  // trace window stays anchored at the step set_trace saw

@@ -132,6 +132,18 @@ struct KTimer {
 };
 }  // namespace mh
 
+// bias map of biased insertion / deletion (mh_bmap.hip)
+struct mh_bmap {
+    mh_ctx *ctx = nullptr;
+    int ncub[3] = {0, 0, 0};  // cubelets per axis
+    long long ncubelets = 0, numtab = 0;
+    double step[3] = {0, 0, 0}, eff[3] = {0, 0, 0};
+    double cum_wt = 0.0, clamp_energy = 0.0;
+    bool clamped = false;
+    int gbits = 0;
+    mh::DBuf cube, cindex, guide, cum, wt, wn;
+};
+
 struct mh_map {
     mh_ctx *ctx = nullptr;
     bool retained = false;

@@ -48,6 +48,15 @@ __device__ __forceinline__ double wrap_full(double x, double L) {
 // simcell_maptouc + mapheader_getIndices along one axis (simcell.F90:634-668,
 // mapheader.F90:277-299): 0-based lower / upper node (upper wraps to 0 at the
 // last node, interpolate.F90:640-660) and the fractional offset t.
+// simcell_maptouc on one axis (simcell.F90:634-668): the coordinate folded
+// into [0, ell) with the reference's fmod form
+__device__ __forceinline__ double maptouc(double c, double ell) {
+    double f = c * (1.0 / ell);
+    const double fr = f - trunc(f);  // == fmod(f, 1.0), exact
+    f = (f >= 0.0) ? fr : fr + 1.0;
+    return f * ell;
+}
+
 __device__ __forceinline__ bool cell_axis(double c, double ell, double st, int nb, int &i0, int &i1, double &t) {
     double f = c * (1.0 / ell);
     double fr = f - trunc(f);  // == fmod(f, 1.0), exact

@@ -1,7 +1,9 @@
 ! music_gcmc -- GCMC driver over libmusic_hip (usage: music_gcmc ctrlfile
 ! [fp32] [chain] [maxn=N] [check]), the drop-in for the reference's
 ! drivers/music_gcmc.F90:79-257 for one rigid species with the rigid
-! insert / delete / translate / rotate moves.  It reads the ctrlfile's
+! insert / delete / translate / rotate moves and the biased insert / delete
+! (BINSERT with a bias pmap and temperature, BDELETE; rigidmoves_initInsBias,
+! rigidmoves.F90:593-675).  It reads the ctrlfile's
 ! 'General Information' (iterations, print interval, seed; general.F90),
 ! atom and molecule types, simulation cell, forcefield and 'GCMC
 ! Information' sections (gcmc_init, gcmc.F90:100-200; gcmcmoves_init and
@@ -42,7 +44,9 @@ program music_gcmc
   real(c_double) :: T, wts(4), kappa_param, rcut
   real(c_double), allocatable :: press(:)
   integer :: stypes(MH_MAX_SITES)
-  character(len=LL) :: pmapf(MH_MAX_SITES), emapf(MH_MAX_SITES)
+  character(len=LL) :: pmapf(MH_MAX_SITES), emapf(MH_MAX_SITES), biasf
+  real(c_double) :: bias_T
+  type(c_ptr) :: bmap
   type(c_ptr) :: ctx, g
   type(c_ptr), allocatable :: maps(:)
   character(len=LL), allocatable :: mapnames(:)
@@ -113,6 +117,8 @@ program music_gcmc
   i = i + 3
   P%delta_trans = 0.0d0
   P%delta_rot = 0.0d0
+  biasf = ''
+  bias_T = 0.0d0
   do k = 1, nmv
      mvname(k) = upper(word(strip(sy%t%line(i)), 1))
      i = i + 1
@@ -129,6 +135,18 @@ program music_gcmc
         P%move_type(k) = MH_MV_ROTATE
         call read_delta(i, P%delta_rot)
         i = i + 1
+     case ('BINSERT')
+        ! bias pmap file (MAP_BIAS), then the bias temperature; cavity bias
+        ! ('CAVITY', 'file, CAVITY') is not offered
+        P%move_type(k) = MH_MV_BINSERT
+        line = strip(sy%t%line(i))
+        if (index(line, ',') > 0 .or. trim(upper(word(line, 1))) == 'CAVITY') &
+             call fail('cavity-biased insertion is not offered (MAP_BIAS only)')
+        biasf = word(line, 1)
+        read(word(strip(sy%t%line(i + 1)), 1), *) bias_T
+        i = i + 2
+     case ('BDELETE')
+        P%move_type(k) = MH_MV_BDELETE
      case default
         call fail('move type ' // trim(mvname(k)) // ' is not offered (rigid moves only)')
      end select
@@ -199,6 +217,10 @@ program music_gcmc
   end do
   if (P%ewald /= 0) write(*, '(a, es24.16, a, i0, a, es24.16)') '  ewald: kappa ', P%ewald_kappa, ' kmax ', &
        P%ewald_kmax, ' rcut ', P%ewald_rcut
+  if (len_trim(biasf) > 0) write(*, '(a, a, a, f10.3)') '  bias: ', trim(biasf), ' T ', bias_T
+  if (any(P%move_type(1:nmv) == MH_MV_BINSERT) .neqv. any(P%move_type(1:nmv) == MH_MV_BDELETE)) &
+       call fail('BINSERT and BDELETE come as a pair (moves.F90:827-840)')
+  flush(6)
   if (check_only) go to 900
 
   ! ---- device: the simulation cell (and framework, for reference), maps
@@ -220,6 +242,11 @@ program music_gcmc
      if (len_trim(pmapf(a)) > 0) P%site_pmap(a) = load_map(pmapf(a))
      if (len_trim(emapf(a)) > 0) P%site_emap(a) = load_map(emapf(a))
   end do
+  bmap = c_null_ptr
+  if (len_trim(biasf) > 0) then
+     call check(mh_bmap_create(ctx, load_map(biasf), bias_T, bmap), 'mh_bmap_create ' // trim(biasf))
+     P%bias = bmap
+  end if
   np = 1
   call system_clock(t0, rate)
   if (chain) then
@@ -233,6 +260,7 @@ program music_gcmc
   secs = real(t1 - t0, c_double) / real(rate, c_double)
   write(*, '(a, f10.3, a, es10.3, a)') 'Main Program   Time: ', secs, ' sec   (', &
        real(npts, c_double) * niter * nmicro / max(secs, 1.0d-9), ' steps/s over all state points)'
+  if (c_associated(bmap)) call check(mh_bmap_destroy(bmap), 'mh_bmap_destroy')
   do k = 1, nmaps
      call check(mh_map_destroy(maps(k)), 'mh_map_destroy')
   end do
@@ -381,7 +409,7 @@ contains
        write(*, '(4x, a, f10.1, a)') 'Species Name: ' // trim(spname) // '    Pressure:', &
             press(merge(np, r, chain)), ' kPa'
        do k = 1, nmv
-          mt = P%move_type(k) + 1
+          mt = mod(P%move_type(k), 4) + 1  ! BINSERT / BDELETE count as insert / delete
           ratio = 0.0d0
           if (att(mt, r) > 0) ratio = real(acc(mt, r), c_double) / real(att(mt, r), c_double)
           lab = trim(basic_tag(P%move_type(k))) // ' (' // trim(mvname(k)) // ')'
@@ -409,9 +437,9 @@ contains
     integer(c_int), intent(in) :: mt
     character(len=9) :: o
     select case (mt)
-    case (MH_MV_INSERT)
+    case (MH_MV_INSERT, MH_MV_BINSERT)
        o = 'INSERT'
-    case (MH_MV_DELETE)
+    case (MH_MV_DELETE, MH_MV_BDELETE)
        o = 'DELETE'
     case (MH_MV_TRANSLATE)
        o = 'TRANSLATE'

@@ -19,7 +19,7 @@ module music_hip_mod
   integer(c_int), parameter, public :: MH_MAP_LJ = 1, MH_MAP_EWALD = 2, MH_MAP_BUCK = 3
   integer(c_int), parameter, public :: MH_PREC_FP64 = 0, MH_PREC_FP32 = 1
   integer(c_int), parameter, public :: MH_MV_INSERT = 0, MH_MV_DELETE = 1, MH_MV_TRANSLATE = 2, &
-       MH_MV_ROTATE = 3
+       MH_MV_ROTATE = 3, MH_MV_BINSERT = 4, MH_MV_BDELETE = 5
   integer, parameter, public :: MH_MAX_SITES = 8
 
   ! mh_gcmc_params (field order and types as in the C struct)
@@ -44,6 +44,7 @@ module music_hip_mod
      real(c_double) :: delta_trans, delta_rot
      real(c_double) :: max_nrg
      integer(c_int) :: precision
+     type(c_ptr) :: bias = c_null_ptr
   end type mh_gcmc_params
 
   public :: mh_error_message
@@ -53,6 +54,7 @@ module music_hip_mod
   public :: mh_gcmc_create, mh_gcmc_destroy, mh_gcmc_set_threads, mh_gcmc_autotune
   public :: mh_gcmc_set_config, mh_gcmc_run, mh_gcmc_get_state, mh_gcmc_set_rng, mh_gcmc_get_moments
   public :: mh_gcmc_get_config, mh_gcmc_total_energy, mh_trial_energy
+  public :: mh_bmap_create, mh_bmap_destroy, mh_bmap_info
 
   interface
      function c_mh_last_error() bind(C, name="mh_last_error") result(p)
@@ -269,6 +271,28 @@ module music_hip_mod
        integer(c_int), value :: refresh
        integer(c_int) :: rc
      end function mh_gcmc_total_energy
+
+     function mh_bmap_create(ctx, pmap, bias_temperature, b) bind(C, name="mh_bmap_create") result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, pmap
+       real(c_double), value :: bias_temperature
+       type(c_ptr), intent(out) :: b
+       integer(c_int) :: rc
+     end function mh_bmap_create
+
+     function mh_bmap_destroy(b) bind(C, name="mh_bmap_destroy") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: b
+       integer(c_int) :: rc
+     end function mh_bmap_destroy
+
+     function mh_bmap_info(b, numtab, ncubelets, cum_wt, clamp_energy) bind(C, name="mh_bmap_info") result(rc)
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: b
+       integer(c_int64_t), intent(out) :: numtab, ncubelets
+       real(c_double), intent(out) :: cum_wt, clamp_energy
+       integer(c_int) :: rc
+     end function mh_bmap_info
 
      function mh_trial_energy(g, rep, n, com, ang, e, ok) bind(C, name="mh_trial_energy") result(rc)
        import :: c_int, c_ptr, c_double, c_int8_t

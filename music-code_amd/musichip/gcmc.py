@@ -12,9 +12,9 @@ import numpy as np
 from . import constants as C
 from . import hip as H
 
-MH_MV_INSERT, MH_MV_DELETE, MH_MV_TRANSLATE, MH_MV_ROTATE = 0, 1, 2, 3
+MH_MV_INSERT, MH_MV_DELETE, MH_MV_TRANSLATE, MH_MV_ROTATE, MH_MV_BINSERT, MH_MV_BDELETE = 0, 1, 2, 3, 4, 5
 MOVES = {"RINSERT": 0, "RDELETE": 1, "RTRANSLATE": 2, "RROTATE": 3,
-         "INSERT": 0, "DELETE": 1, "TRANSLATE": 2, "ROTATE": 3}
+         "INSERT": 0, "DELETE": 1, "TRANSLATE": 2, "ROTATE": 3, "BINSERT": 4, "BDELETE": 5}
 
 _d, _i, _vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
 
@@ -26,7 +26,7 @@ class GcmcParams(ctypes.Structure):
                 ("lj_on", ctypes.c_uint8 * 64), ("ewald", _i), ("ewald_kappa", _d), ("ewald_kmax", _i),
                 ("ewald_rcut", _d), ("temperature", _d), ("nmoves", _i), ("move_type", _i * 4),
                 ("move_weight", _d * 4), ("delta_trans", _d), ("delta_rot", _d), ("max_nrg", _d),
-                ("precision", _i)]
+                ("precision", _i), ("bias", _vp)]
 
 
 def _sig():
@@ -101,6 +101,8 @@ def make_params(spec, maps=None, precision=H.MH_PREC_FP64):
     p.delta_rot = spec.get("drot", 0.0)
     p.max_nrg = spec.get("max_nrg", C.VERY_LRG_NRG)
     p.precision = precision
+    bias = spec.get("bias_map")  # a hip.BiasMap for BINSERT / BDELETE
+    p.bias = bias.h.value if bias is not None else None
     return p
 
 
@@ -123,6 +125,7 @@ class GcmcBatch:
         self.h = h
         H._LIVE["gcmc"].add(self)
         self.maps = maps  # keep the map objects alive
+        self.bias_map = spec.get("bias_map")
         if threads:
             self.set_threads(threads)
 

@@ -24,12 +24,13 @@ _i64 = ctypes.c_int64
 _LIB = None
 # live handles, released children first at interpreter exit (before the HIP
 # runtime itself is torn down)
-_LIVE = {"gcmc": weakref.WeakSet(), "map": weakref.WeakSet(), "ctx": weakref.WeakSet()}
+_LIVE = {"gcmc": weakref.WeakSet(), "bmap": weakref.WeakSet(), "map": weakref.WeakSet(),
+         "ctx": weakref.WeakSet()}
 
 
 @atexit.register
 def _release_all():
-    for kind in ("gcmc", "map", "ctx"):
+    for kind in ("gcmc", "bmap", "map", "ctx"):
         for obj in list(_LIVE[kind]):
             try:
                 obj.close()
@@ -80,6 +81,11 @@ def lib():
         "mh_map_load": [_vp, ctypes.c_char_p, _vp, ctypes.POINTER(_vp)],
         "mh_map_from_host": [_vp, _vp, _i64, _vp, _vp, _vp, ctypes.POINTER(_vp)],
         "mh_map_interp": [_vp, _vp, _i, _vp, _d, _vp, _vp],
+        "mh_bmap_create": [_vp, _vp, _d, ctypes.POINTER(_vp)],
+        "mh_bmap_destroy": [_vp],
+        "mh_bmap_info": [_vp, _vp, _vp, _vp, _vp],
+        "mh_bmap_get": [_vp, _vp, _vp, _vp],
+        "mh_bmap_search": [_vp, _i, _vp, _vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -142,6 +148,48 @@ class Map:
     def close(self):
         if self.h:
             lib().mh_map_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BiasMap:
+    """Bias map of biased insertion / deletion on the device (mh_bmap), from
+    a pmap at a bias temperature (bmap_init + bmap_cubeinit, bmap.F90:69-403)."""
+
+    def __init__(self, pmap, temperature):
+        self.ctx = pmap.ctx
+        h = _vp()
+        check(lib().mh_bmap_create(self.ctx.h, pmap.h, float(temperature), ctypes.byref(h)), "mh_bmap_create")
+        self.h = h
+        _LIVE["bmap"].add(self)
+        self.pmap = pmap
+        nt, nc, cw, ce = _i64(), _i64(), _d(), _d()
+        check(lib().mh_bmap_info(self.h, ctypes.byref(nt), ctypes.byref(nc), ctypes.byref(cw), ctypes.byref(ce)),
+              "mh_bmap_info")
+        self.numtab, self.ncubelets, self.cum_wt, self.clamp_energy = nt.value, nc.value, cw.value, ce.value
+
+    def get(self):
+        """-> (cubelet ids, weights, cumulative probabilities) in list order"""
+        cube = np.zeros(self.numtab, dtype=np.int32)
+        wt = np.zeros(self.numtab)
+        cum = np.zeros(self.numtab)
+        check(lib().mh_bmap_get(self.h, _p(cube), _p(wt), _p(cum)), "mh_bmap_get")
+        return cube, wt, cum
+
+    def search(self, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.float64)
+        out = np.zeros(len(keys), dtype=np.int64)
+        check(lib().mh_bmap_search(self.h, len(keys), _p(keys), _p(out)), "mh_bmap_search")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().mh_bmap_destroy(self.h)
             self.h = None
 
     def __del__(self):

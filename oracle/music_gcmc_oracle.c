@@ -19,6 +19,11 @@
  *   rigidcoords_toxyz     rigidcoords.F90:254-276, matrix_b2feuler matrix.F90:383-403
  *   simcell_pbc           simcell.F90:521-547
  *   config_delmol         config.F90:1401-1413 (last molecule moves into the slot)
+ *   rigidmoves_binsert    rigidmoves.F90:1386-1442, rigidmoves_comFromBmap :1938-1969
+ *                         (index, 3 in-cubelet, 3 unit-cell draws, +3 if multi-site)
+ *   rigidmoves_bdelete    rigidmoves.F90:1578-1654 (bias of the molecule's cubelet)
+ *   bmap_cubeinit / bmap_getbiasindex / bmap_getbiaspt / bmap_getcellwt
+ *                         bmap.F90:181-636
  * Molecule energy = map terms (interpolate_int per site, factor q for the
  * emap) + LJ pairs with every other molecule (lj_snglint, nderivs = 0) +
  * Ewald terms.  Failure (LJ locut, untabulated map corner, U > max_nrg)
@@ -242,6 +247,113 @@ static void record(mo_gcmc_state *st, int step, int mtype, int acc, double de)
     t->dE = de;
 }
 
+/* bmap_cubeinit (bmap.F90:181-403): weights exp(-u / kT_bias), exponent
+ * clamped to +-30, over the cubelets whose lower-corner node (ptr(k, j, i))
+ * is tabulated with u <= 30 kcal/mol (BMAP_DEFAULT_CUTOFF), loop x outer, y,
+ * z inner; when min(wt) / sum(wt) <= 1e-6 the energies above
+ * -log(1e-6 sum) / rti are lowered to it first.  (When the ratio is larger
+ * the reference compares against an unset local, maxnrgval; here nothing is
+ * clamped.)  Cumulative probabilities are running sums over cum_wt, both
+ * summed in that order.  ptr is [nx][ny][nz] (z fastest, 1-based, 0 =
+ * untabulated); V is the map's V record [nfsize]. */
+int mo_bmap_build(mo_bmap *b, const int npnts[3], const double step[3], const double eff[3], const int32_t *ptr,
+                  const float *V, double T_bias)
+{
+    memset(b, 0, sizeof *b);
+    const int nx1 = npnts[0] - 1, ny1 = npnts[1] - 1, nz1 = npnts[2] - 1;
+    b->ncub[0] = nx1;
+    b->ncub[1] = ny1;
+    b->ncub[2] = nz1;
+    b->ncubelets = (long long)nx1 * ny1 * nz1;
+    for (int i = 0; i < 3; i++) {
+        b->step[i] = step[i];
+        b->eff[i] = eff[i];
+    }
+    const double rti = 1.0 / (T_bias * MO_KCALMOLE_KB);
+    const double ucut = 30.0;
+    double sum = 0.0, mn = 1.0e20;
+    long long numtab = 0;
+    for (int i = 0; i < nx1; i++)
+        for (int j = 0; j < ny1; j++)
+            for (int k = 0; k < nz1; k++) {
+                int p = ptr[((long long)i * npnts[1] + j) * npnts[2] + k];
+                if (p == 0) continue;
+                double u = (double)V[p - 1];
+                if (!(u <= ucut)) continue;
+                double e = -rti * u;
+                if (e > MO_MAX_EXP) e = MO_MAX_EXP;
+                else if (e < -MO_MAX_EXP) e = -MO_MAX_EXP;
+                double wt = exp(e);
+                if (wt < mn) mn = wt;
+                sum = sum + wt;
+                numtab++;
+            }
+    if (numtab == 0) return -1;
+    b->clamped = !(mn / sum > 1.0e-6);
+    b->maxnrgval = b->clamped ? -log(sum * 1.0e-6) / rti : INFINITY;
+    b->cube = malloc(sizeof(int32_t) * numtab);
+    b->wt = malloc(sizeof(double) * numtab);
+    b->cum_prob = malloc(sizeof(double) * numtab);
+    b->cube_index = calloc((size_t)b->ncubelets, sizeof(int32_t));
+    long long t = 0;
+    b->cum_wt = 0.0;
+    for (int i = 0; i < nx1; i++)
+        for (int j = 0; j < ny1; j++)
+            for (int k = 0; k < nz1; k++) {
+                int p = ptr[((long long)i * npnts[1] + j) * npnts[2] + k];
+                if (p == 0) continue;
+                double u = (double)V[p - 1];
+                if (!(u <= ucut)) continue;
+                if (u > b->maxnrgval) u = b->maxnrgval;
+                double e = -rti * u;
+                if (e > MO_MAX_EXP) e = MO_MAX_EXP;
+                else if (e < -MO_MAX_EXP) e = -MO_MAX_EXP;
+                long long c = ((long long)i * ny1 + j) * nz1 + k;
+                b->cube[t] = (int32_t)c;
+                b->wt[t] = exp(e);
+                b->cube_index[c] = (int32_t)(t + 1);
+                b->cum_wt = b->cum_wt + b->wt[t];
+                t++;
+            }
+    b->numtab = numtab;
+    double run = 0.0;
+    for (t = 0; t < numtab; t++) {
+        run = run + b->wt[t];
+        b->cum_prob[t] = run / b->cum_wt;
+    }
+    return 0;
+}
+
+void mo_bmap_free(mo_bmap *b)
+{
+    free(b->cube);
+    free(b->wt);
+    free(b->cum_prob);
+    free(b->cube_index);
+    memset(b, 0, sizeof *b);
+}
+
+/* bmap_getbiasindex (bmap.F90:575-636), the reference's bisection on the
+ * cumulative probabilities; 0-based result */
+long long mo_bmap_index(const mo_bmap *b, double key)
+{
+    long long low = 1, high = b->numtab, mid = 1;
+    for (;;) {
+        if (low > high) break;
+        mid = (low + high) / 2;
+        double cp = b->cum_prob[mid - 1];
+        if (key < cp) {
+            high = mid - 1;
+        } else if (key > cp) {
+            low = mid + 1;
+            mid = low;
+        } else {
+            break;
+        }
+    }
+    return mid - 1;
+}
+
 /* mcmoves_attempt (mcmoves.F90:290-335) */
 static int attempt(const mo_gcmc_sys *s, mo_gcmc_state *st, double deltaE, double bias, int has_x,
                    double addxtra, int has_p, double prefactor)
@@ -283,14 +395,36 @@ int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
         if (mv >= s->nmoves) mv = s->nmoves - 1;
         int mtype = s->move_type[mv];
         int N = st->N;
-        if (mtype == MO_MV_INSERT) {
+        if (mtype == MO_MV_INSERT || mtype == MO_MV_BINSERT) {
             st->att[0]++;
             int accepted = 0;
             double de = 0.0;
             if (N >= st->maxN) return -3; /* capacity of the molecule arrays */
             {
                 double com[3], ang[3] = {0.0, 0.0, 0.0};
-                for (int i = 0; i < 3; i++) com[i] = s->full_ell[i] * mo_rranf(&st->rng);
+                double biasfactor = 1.0;
+                if (mtype == MO_MV_INSERT) {
+                    for (int i = 0; i < 3; i++) com[i] = s->full_ell[i] * mo_rranf(&st->rng);
+                } else {
+                    /* rigidmoves_comFromBmap: a cubelet by its weight, a point
+                     * in it (bmap_getbiaspt), a unit cell of the simulation cell */
+                    const mo_bmap *b = s->bias;
+                    long long t = mo_bmap_index(b, mo_rranf(&st->rng));
+                    long long c = b->cube[t];
+                    long long ci[3] = {c / ((long long)b->ncub[1] * b->ncub[2]), (c / b->ncub[2]) % b->ncub[1],
+                                       c % b->ncub[2]};
+                    double wt = b->wt[t] / b->cum_wt;
+                    biasfactor = 1.0 / wt / (double)b->ncubelets;
+                    for (int i = 0; i < 3; i++) com[i] = (double)ci[i] * b->step[i];
+                    for (int i = 0; i < 3; i++) com[i] = com[i] + mo_rranf(&st->rng) * b->step[i];
+                    for (int i = 0; i < 3; i++)
+                        if (com[i] >= b->eff[i]) com[i] = (double)0.9999990f * b->eff[i];
+                    for (int i = 0; i < 3; i++) {
+                        int nc = (int)lround(s->full_ell[i] / s->fund_ell[i]);
+                        int icell = (int)(mo_rranf(&st->rng) * nc);
+                        com[i] = com[i] + s->fund_ell[i] * icell;
+                    }
+                }
                 if (ns != 1) {
                     double costheta = 1.0 - mo_rranf(&st->rng) * 2.0;
                     ang[1] = acos(costheta);
@@ -311,7 +445,7 @@ int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
                     if (U > s->max_nrg) ok = 0;
                 }
                 if (ok) {
-                    double bias = 1.0 / s->insdelratio;
+                    double bias = biasfactor / s->insdelratio;
                     accepted = attempt(s, st, U, bias, 1, st->B, 1, 1.0 / (double)(float)(N + 1));
                     if (accepted) {
                         memcpy(st->com + 3 * N, com, sizeof com);
@@ -331,12 +465,30 @@ int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
                 }
             }
             record(st, step, mtype, accepted, de);
-        } else if (mtype == MO_MV_DELETE) {
+        } else if (mtype == MO_MV_DELETE || mtype == MO_MV_BDELETE) {
             int accepted = 0;
             double de = 0.0;
             if (N != 0) {
                 int mol = (int)(mo_rranf(&st->rng) * N);
                 st->att[1]++;
+                double biasfactor = 1.0;
+                if (mtype == MO_MV_BDELETE) {
+                    /* rigidmoves_bdelete: bmap_getcellwt of the centre of mass
+                     * mapped into the unit cell (simcell_maptouc); a cubelet
+                     * outside the bias map stops the reference */
+                    const mo_bmap *b = s->bias;
+                    double uc[3];
+                    mo_maptouc(s->fund_ell, st->com + 3 * mol, uc);
+                    long long ci[3];
+                    for (int i = 0; i < 3; i++) {
+                        ci[i] = (long long)(uc[i] / b->step[i]);
+                        if (ci[i] > b->ncub[i] - 1) ci[i] = b->ncub[i] - 1;
+                    }
+                    long long t = b->cube_index[(ci[0] * b->ncub[1] + ci[1]) * b->ncub[2] + ci[2]];
+                    if (t == 0) return -4;
+                    double wt = b->wt[t - 1] / b->cum_wt;
+                    biasfactor = (double)b->ncubelets * wt;
+                }
                 const double *r = st->r + 3 * ns * mol;
                 double um, up;
                 int ok = mol_energy(s, st, r, mol, &um, &up);
@@ -346,7 +498,7 @@ int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
                     U = U - drecip(st, ak, bk, -1.0, NULL, NULL, 0.0) + mconst;
                 }
                 if (!ok) return -1; /* mcmoves.F90:599-603: an existing molecule must evaluate */
-                double bias = 1.0 * s->insdelratio;
+                double bias = biasfactor * s->insdelratio;
                 accepted = attempt(s, st, -U, bias, 1, -st->B, 1, 1.0 * N);
                 if (accepted) {
                     int last = N - 1;

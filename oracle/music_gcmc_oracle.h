@@ -8,7 +8,31 @@
 extern "C" {
 #endif
 
-enum { MO_MV_INSERT = 0, MO_MV_DELETE = 1, MO_MV_TRANSLATE = 2, MO_MV_ROTATE = 3 };
+enum { MO_MV_INSERT = 0, MO_MV_DELETE = 1, MO_MV_TRANSLATE = 2, MO_MV_ROTATE = 3, MO_MV_BINSERT = 4,
+       MO_MV_BDELETE = 5 };
+
+/* Bias map of biased insertion / deletion (bmap_init + bmap_cubeinit,
+ * bmap.F90:69-403): the cubelets of a pmap whose lower-corner node is
+ * tabulated with V <= 30 kcal/mol, in the reference's loop order (x outer, z
+ * inner), Boltzmann weights at the bias temperature and their cumulative
+ * probabilities. */
+typedef struct {
+    int ncub[3];           /* cubelets per axis (npnts - 1) */
+    long long ncubelets;   /* all cubelets, header%ncubelets */
+    long long numtab;      /* biased cubelets */
+    double step[3], eff[3];
+    double cum_wt;
+    int clamped;           /* energies above maxnrgval were clamped */
+    double maxnrgval;
+    int32_t *cube;         /* [numtab] cubelet id (i * ny1 + j) * nz1 + k, 0-based */
+    double *wt;            /* [numtab] weight */
+    double *cum_prob;      /* [numtab] */
+    int32_t *cube_index;   /* [ncubelets] 1-based position in the list, 0 = not biased */
+} mo_bmap;
+int mo_bmap_build(mo_bmap *b, const int npnts[3], const double step[3], const double eff[3], const int32_t *ptr,
+                  const float *V, double T_bias);
+void mo_bmap_free(mo_bmap *b);
+long long mo_bmap_index(const mo_bmap *b, double key);
 #define MO_MAXS 8
 #define MO_MAXMAPS 8
 
@@ -35,6 +59,7 @@ typedef struct {
     int move_type[4];
     double cumwt[4];
     double insdelratio, dtrans, drot, max_nrg;
+    const mo_bmap *bias;                      /* BINSERT / BDELETE, or NULL */
 } mo_gcmc_sys;
 
 typedef struct {

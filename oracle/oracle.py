@@ -228,9 +228,50 @@ def pothermite(npnts, step, ptr, vals8, coords, fund_ell=None, force=False):
 
 
 # ------------------------------------------------------------------ GCMC
-MV_INSERT, MV_DELETE, MV_TRANSLATE, MV_ROTATE = 0, 1, 2, 3
+MV_INSERT, MV_DELETE, MV_TRANSLATE, MV_ROTATE, MV_BINSERT, MV_BDELETE = 0, 1, 2, 3, 4, 5
 MOVE_NAMES = {"RINSERT": 0, "RDELETE": 1, "RTRANSLATE": 2, "RROTATE": 3,
-              "INSERT": 0, "DELETE": 1, "TRANSLATE": 2, "ROTATE": 3}
+              "INSERT": 0, "DELETE": 1, "TRANSLATE": 2, "ROTATE": 3, "BINSERT": 4, "BDELETE": 5}
+
+
+class MoBmap(ctypes.Structure):
+    _fields_ = [("ncub", _i * 3), ("ncubelets", _ll), ("numtab", _ll), ("step", _d * 3), ("eff", _d * 3),
+                ("cum_wt", _d), ("clamped", _i), ("maxnrgval", _d), ("cube", _vp), ("wt", _vp),
+                ("cum_prob", _vp), ("cube_index", _vp)]
+
+
+class Bmap:
+    """The oracle's bias map (mo_bmap_build, a restatement of bmap_cubeinit,
+    bmap.F90:181-403) of a compacted map (ptr [nx][ny][nz] 1-based, vals8
+    [8][nfsize]) at a bias temperature."""
+
+    def __init__(self, npnts, step, eff, ptr, vals8, T_bias):
+        L = _gcmc_lib()
+        self.b = MoBmap()
+        npn = np.ascontiguousarray(npnts, dtype=np.int32)
+        st = np.ascontiguousarray(step, dtype=np.float64)
+        ef = np.ascontiguousarray(eff, dtype=np.float64)
+        self._ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+        self._V = np.ascontiguousarray(np.asarray(vals8, dtype=np.float32)[0])
+        rc = L.mo_bmap_build(ctypes.byref(self.b), _p(npn), _p(st), _p(ef), _p(self._ptr), _p(self._V),
+                             float(T_bias))
+        if rc:
+            raise ValueError("mo_bmap_build failed (no cubelet below 30 kcal/mol)")
+        n = self.b.numtab
+        self.numtab, self.ncubelets, self.cum_wt = n, self.b.ncubelets, self.b.cum_wt
+        self.clamp_energy = self.b.maxnrgval
+        self.cube = np.ctypeslib.as_array(ctypes.cast(self.b.cube, ctypes.POINTER(ctypes.c_int32)), (n,)).copy()
+        self.wt = np.ctypeslib.as_array(ctypes.cast(self.b.wt, ctypes.POINTER(_d)), (n,)).copy()
+        self.cum = np.ctypeslib.as_array(ctypes.cast(self.b.cum_prob, ctypes.POINTER(_d)), (n,)).copy()
+
+    def index(self, key):
+        """bmap_getbiasindex (the reference's bisection), 0-based"""
+        return _gcmc_lib().mo_bmap_index(ctypes.byref(self.b), float(key))
+
+    def __del__(self):
+        try:
+            _gcmc_lib().mo_bmap_free(ctypes.byref(self.b))
+        except Exception:
+            pass
 
 
 class GcmcSys(ctypes.Structure):
@@ -243,7 +284,7 @@ class GcmcSys(ctypes.Structure):
                 ("lj_l2", _d * 64), ("lj_on", _i * 64),
                 ("ewald", _i), ("kmax", _i), ("kappa", _d), ("rc2_real", _d), ("T", _d),
                 ("nmoves", _i), ("move_type", _i * 4), ("cumwt", _d * 4),
-                ("insdelratio", _d), ("dtrans", _d), ("drot", _d), ("max_nrg", _d)]
+                ("insdelratio", _d), ("dtrans", _d), ("drot", _d), ("max_nrg", _d), ("bias", _vp)]
 
 
 class GcmcTrace(ctypes.Structure):
@@ -277,6 +318,11 @@ def _gcmc_lib():
         L.mo_adams_B.argtypes = [_d, _d, _d, _d]
         L.mo_adams_B.restype = _d
         L.mo_rng_init.argtypes = [_vp, ctypes.c_int32]
+        L.mo_bmap_build.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _d]
+        L.mo_bmap_build.restype = _i
+        L.mo_bmap_free.argtypes = [_vp]
+        L.mo_bmap_index.argtypes = [_vp, _d]
+        L.mo_bmap_index.restype = _ll
         L._gcmc_sig = True
     return L
 
@@ -289,7 +335,8 @@ class Gcmc:
     """One GCMC chain on the CPU oracle.  spec: dict with full_ell, fund_ell,
     ref [ns,3], stype [ns], sq [ns], nst, lj (A, B, h2, l2, on: [nst*nst]),
     maps: list of (npnts, step, ptr, vals8) and site_pmap/site_emap,
-    ewald (kappa, kmax, rc), T, moves [(type, weight)], dtrans, drot."""
+    ewald (kappa, kmax, rc), T, moves [(type, weight)], dtrans, drot, and
+    for BINSERT / BDELETE bias (a Bmap)."""
 
     def __init__(self, spec, fugacity_kPa, seed, maxN=2048, trace=0):
         L = _gcmc_lib()
@@ -336,14 +383,17 @@ class Gcmc:
             s.move_type[i] = MOVE_NAMES[t] if isinstance(t, str) else int(t)
             c = c + w / wsum
             s.cumwt[i] = c
-            if s.move_type[i] == 0:
+            if s.move_type[i] in (0, 4):
                 wi = w / wsum
-            if s.move_type[i] == 1:
+            if s.move_type[i] in (1, 5):
                 wd = w / wsum
         s.insdelratio = (wi / wd) if (wi and wd) else 1.0
         s.dtrans = spec.get("dtrans", 0.0)
         s.drot = spec.get("drot", 0.0)
         s.max_nrg = spec.get("max_nrg", 1.0e10)
+        # BINSERT / BDELETE: spec["bias"] is a Bmap
+        self.bias = spec.get("bias")
+        s.bias = ctypes.cast(ctypes.byref(self.bias.b), _vp) if self.bias is not None else None
         self.sys = s
         st = GcmcState()
         st.maxN = maxN

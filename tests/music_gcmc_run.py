@@ -72,7 +72,7 @@ def parse(text):
 
 def parse_check(text):
     """Parameters printed by 'music_gcmc ctrlfile check'."""
-    d = dict(press=[], sites=[], lj={}, moves=[], ewald=None)
+    d = dict(press=[], sites=[], lj={}, moves=[], ewald=None, bias=None)
     for line in text.splitlines():
         s = line.split()
         if line.startswith("  state point"):
@@ -86,6 +86,8 @@ def parse_check(text):
             d["moves"].append((s[2], float(s[-1])))
         elif line.startswith("  delta translate"):
             d["delta"] = (float(s[-2]), float(s[-1]))
+        elif line.startswith("  bias:"):
+            d["bias"] = (s[1], float(s[3]))
         elif line.startswith("  ewald:"):
             d["ewald"] = dict(kappa=float(s[2]), kmax=int(s[4]), rc=float(s[6]))
         elif line.startswith("  iterations"):

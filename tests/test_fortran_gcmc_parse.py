@@ -62,6 +62,15 @@ def test_probe_with_maps_parameters():
     assert d["sites"][0]["pmap"] == "Lattice.LJ.map" and d["sites"][0]["emap"] == "Lattice.EWALD.map"
 
 
+def test_biased_moves_parameters():
+    """BINSERT with its bias file and temperature lines, BDELETE."""
+    d = _check(os.path.join(FIX, "gcmc_bias.ctr"))
+    assert [m[0] for m in d["moves"]] == ["BINSERT", "BDELETE"]
+    assert d["bias"] == ("Lattice.LJ.map", 300.0)
+    spec = dict(probe_spec(), moves=[("BINSERT", 1.0), ("BDELETE", 1.0)], dtrans=0.0)
+    _same_spec(d, spec, 3)
+
+
 def test_bent_sorbate_parameters():
     """Principal axes (Jacobi in the driver, numpy in the Python host, same
     sign convention), the LBMIX pair and the Ewald parameters."""

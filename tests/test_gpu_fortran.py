@@ -218,3 +218,23 @@ def test_music_gcmc_bent_sorbate_ewald(hip):
         b32, e32 = MG.parse(MG.run([ctrl, "fp32"], td))
         assert len(b32) == 4 and e32[1]["dev"] < 1e-3
         assert b32[-1]["moves"]["ROTATE"][1] > 0
+
+
+def test_music_gcmc_biased_moves(hip):
+    """gcmc_bias.ctr: BINSERT (bias map Lattice.LJ.map at 300 K) / BDELETE
+    for the probe in the charged lattice through the driver, three state
+    points, against oracle chains with the oracle's bias map, move for move."""
+    print("case: music_gcmc gcmc-maps/gcmc_bias.ctr FP64", flush=True)
+    spec = dict(MG.probe_spec(), moves=[("BINSERT", 1.0), ("BDELETE", 1.0)])
+    press = [10.0, 100.0, 1000.0]
+    with tempfile.TemporaryDirectory() as td:
+        mlist = _fixture_maps(hip, td)
+        npn, step, ptr, vals = mlist[0]
+        bias = O.Bmap(npn, step, [20.0, 20.0, 12.0], ptr, vals, 300.0)
+        spec = dict(spec, maps=mlist, site_pmap=[0], site_emap=[1], bias=bias)
+        blocks, ends = MG.parse(MG.run([os.path.join(MG.FIX, "gcmc_bias.ctr")], td))
+        for p in range(3):
+            g = O.Gcmc(spec, press[p], 1892134 + p)
+            _compare([b for b in blocks if b["sim"] == p + 1], _oracle_blocks(g, 3000))
+            assert abs(ends[p + 1]["stored"] - g.st.energy * 4.184) < 1.5e-3
+        assert sum(b["moves"]["INSERT"][0] for b in blocks if b["it"] == 3000) > 20
