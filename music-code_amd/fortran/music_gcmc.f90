@@ -29,6 +29,17 @@
 ! end of each point the recomputed and stored energies (kJ/mol), their
 ! deviation (interact_checknrgs) and the loading per unit cell.  With
 ! 'check' it prints the parsed parameters and stops before any device call.
+!
+! Restart files: every icrash iterations the configuration of each state
+! point goes to <restart file>.<simno + simstart - 1> in the reference's
+! layout (config_writerestartfile, config.F90:2357-2430), and its
+! random-number state and full-precision configuration to the same name +
+! '.state' (the reference keeps neither).  A 'RESTARTFILE name' source in
+! 'Configuration Initialization' (config_readrestartfile, config.F90:
+! 2114-2247) starts every state point from that configuration; with
+! 'resume' the configuration and random-number state come from name.state
+! instead, so a run split at a restart gives the same trajectory as an
+! unbroken one.
 program music_gcmc
   use, intrinsic :: iso_c_binding
   use music_hip_mod
@@ -39,8 +50,9 @@ program music_gcmc
   character(len=LL) :: ctrl, arg, line
   character(len=NL) :: spname, mvname(4)
   integer :: i, j, k, a, niter, iprint, iseed, nmicro, npts, bsize, nspc, isp, nmv, nsite, nst, np
-  integer :: maxn, nunit
-  logical :: chain, check_only
+  integer :: maxn, nunit, icrash, simstart
+  logical :: chain, check_only, resume
+  character(len=LL) :: desc, resname, srcfile
   real(c_double) :: T, wts(4), kappa_param, rcut
   real(c_double), allocatable :: press(:)
   integer :: stypes(MH_MAX_SITES)
@@ -57,12 +69,13 @@ program music_gcmc
 
   call get_command_argument(1, ctrl)
   if (len_trim(ctrl) == 0) then
-     write(*, '(a)') 'usage: music_gcmc ctrlfile [fp32] [chain] [maxn=N] [check]'
+     write(*, '(a)') 'usage: music_gcmc ctrlfile [fp32] [chain] [resume] [maxn=N] [check]'
      stop 1
   end if
   P%precision = MH_PREC_FP64
   chain = .false.
   check_only = .false.
+  resume = .false.
   maxn = 512
   do i = 2, command_argument_count()
      call get_command_argument(i, arg)
@@ -72,6 +85,8 @@ program music_gcmc
         chain = .true.
      else if (trim(lower(arg)) == 'check') then
         check_only = .true.
+     else if (trim(lower(arg)) == 'resume') then
+        resume = .true.
      else if (index(lower(arg), 'maxn=') == 1) then
         read(arg(6:), *) maxn
      else
@@ -84,9 +99,14 @@ program music_gcmc
   ! crash and config intervals, first simulation number, seed
   i = find_section(sy%t, 'General Information')
   call need(i, 'General Information')
+  desc = sy%t%line(i)
+  desc = desc(1:min(len_trim(desc), 96))  ! general.F90:87 reads it into 2 strLen characters
   read(strip(sy%t%line(i + 1)), *) niter
   read(strip(sy%t%line(i + 2)), *) iprint
+  read(strip(sy%t%line(i + 3)), *) icrash
+  read(strip(sy%t%line(i + 5)), *) simstart
   read(strip(sy%t%line(i + 6)), *) iseed
+  resname = word(strip(sy%t%line(i + 8)), 1)
 
   ! ---- GCMC Information
   i = find_section(sy%t, 'GCMC Information')
@@ -193,6 +213,7 @@ program music_gcmc
   emapf = ''
   call read_species_file()
   nunit = sy%nxyz(1) * sy%nxyz(2) * sy%nxyz(3)
+  call read_config_source()
 
   write(*, '(a)') 'Welcome to MuSiC (GCMC implementation, libmusic_hip)'
   write(*, '(2a)') 'Reading from control file ', trim(ctrl)
@@ -218,6 +239,8 @@ program music_gcmc
   if (P%ewald /= 0) write(*, '(a, es24.16, a, i0, a, es24.16)') '  ewald: kappa ', P%ewald_kappa, ' kmax ', &
        P%ewald_kmax, ' rcut ', P%ewald_rcut
   if (len_trim(biasf) > 0) write(*, '(a, a, a, f10.3)') '  bias: ', trim(biasf), ' T ', bias_T
+  if (len_trim(srcfile) > 0) write(*, '(2a)') '  start configuration: ', trim(srcfile)
+  write(*, '(a, i0, 2a)') '  restart files every ', icrash, ' iterations: ', trim(resname)
   if (any(P%move_type(1:nmv) == MH_MV_BINSERT) .neqv. any(P%move_type(1:nmv) == MH_MV_BDELETE)) &
        call fail('BINSERT and BDELETE come as a pair (moves.F90:827-840)')
   flush(6)
@@ -292,6 +315,7 @@ contains
     if (chain) seeds(1) = int(iseed, c_int32_t)
     call check(mh_gcmc_create(ctx, P, int(nr, c_int), press(q0:q1), seeds, int(maxn, c_int), gnew), &
          'mh_gcmc_create')
+    if (len_trim(srcfile) > 0 .and. .not. carry) call start_from_restart(gnew, nr)
     if (carry) then
        allocate(com(3, maxn), ang(3, maxn), rr(3, maxn * nsite))
        call check(mh_gcmc_get_config(g, 0_c_int, ncarry, com, ang, rr), 'mh_gcmc_get_config')
@@ -314,7 +338,7 @@ contains
     ! events: every print step and the start of its statistics block
     ! (a block holds the last ((s - 1) mod bsize) + 1 steps)
     nev = 0
-    allocate(ev(2 * (niter / max(iprint, 1) + 2)), blk(niter / max(iprint, 1) + 2))
+    allocate(ev(2 * (niter / max(iprint, 1) + 2) + niter / max(icrash, 1) + 1), blk(niter / max(iprint, 1) + 2))
     n = 0
     do it = iprint, niter, iprint
        s = int(it, 8) * nmicro
@@ -325,6 +349,13 @@ contains
        nev = nev + 1
        ev(nev) = s
     end do
+    ! restart files at every icrash iterations (music_gcmc.F90:231-233)
+    if (icrash > 0) then
+       do it = icrash, niter, icrash
+          nev = nev + 1
+          ev(nev) = int(it, 8) * nmicro
+       end do
+    end if
     total = int(niter, 8) * nmicro
     allocate(snap(nr, 0:0))
     sprev = 0
@@ -344,6 +375,13 @@ contains
              call show_stats(k * iprint, s, s - blk(k), snap(:, 0))
           end if
        end do
+       if (icrash > 0 .and. s > 0) then
+          if (mod(s, int(icrash, 8) * nmicro) == 0) then
+             do r = 1, nr
+                call write_restart(restart_name(q0 + r - 1), s / nmicro, r - 1)
+             end do
+          end if
+       end if
        ! a block start (after the print: the next block may start at this
        ! print step): keep sum N for the block average
        do k = 1, n
@@ -432,6 +470,202 @@ contains
        write(*, '(a)') ' '
     end do
   end subroutine show_stats
+
+  function restart_name(simno) result(o)
+    integer, intent(in) :: simno
+    character(len=LL) :: o
+    character(len=16) :: num
+    write(num, '(i0)') simno + simstart - 1
+    o = trim(resname) // '.' // trim(num)
+  end function restart_name
+
+  ! the reference's vector_display(v, "f13.5"): three f13.5 fields, the
+  ! first left-adjusted
+  function vec13(v) result(o)
+    real(c_double), intent(in) :: v(3)
+    character(len=64) :: o
+    write(o, '(3f13.5)') v
+    o = adjustl(o)
+  end function vec13
+
+  ! principal (unfolded) site coordinates of a molecule: rigidcoords_toxyz
+  ! (rigidcoords.F90:254-276), centre of mass + Euler matrix x body frame
+  subroutine toxyz(com, ang, rp)
+    real(c_double), intent(in) :: com(3), ang(3)
+    real(c_double), intent(out) :: rp(3, nsite)
+    real(c_double) :: sp, cp, st, ct, ss, cs, M(3, 3)
+    integer :: b
+    if (nsite == 1) then
+       rp(:, 1) = com
+       return
+    end if
+    sp = sin(ang(1)); cp = cos(ang(1)); st = sin(ang(2)); ct = cos(ang(2)); ss = sin(ang(3)); cs = cos(ang(3))
+    M(1, :) = [cs * cp - ct * sp * ss, -ss * cp - ct * sp * cs, st * sp]
+    M(2, :) = [cs * sp + ct * cp * ss, -ss * sp + ct * cp * cs, -st * cp]
+    M(3, :) = [ss * st, cs * st, ct]
+    do b = 1, nsite
+       rp(:, b) = com + matmul(M, P%ref(:, b))
+    end do
+  end subroutine toxyz
+
+  ! config_writerestartfile (config.F90:2357-2430) for replica rep, and the
+  ! replica's random-number state beside it
+  subroutine write_restart(path, iter, rep)
+    character(len=*), intent(in) :: path
+    integer(8), intent(in) :: iter
+    integer, intent(in) :: rep
+    integer :: u, n, m, b, dof
+    integer(c_int) :: nmol
+    real(c_double), allocatable :: com(:,:), ang(:,:), rr(:,:), rp(:,:)
+    integer(c_int32_t), allocatable :: Nc(:), ix(:), iy(:)
+    real(c_double), allocatable :: en(:), sN(:)
+    integer(c_int64_t), allocatable :: nst(:), att(:,:), acc(:,:)
+    character(len=48) :: nm
+    allocate(com(3, maxn), ang(3, maxn), rr(3, maxn * nsite), rp(3, nsite))
+    call check(mh_gcmc_get_config(g, int(rep, c_int), nmol, com, ang, rr), 'mh_gcmc_get_config')
+    n = nmol
+    dof = sy%mols(isp)%dof
+    if (dof == 0) dof = merge(3, 6, nsite == 1)
+    nm = spname
+    open(newunit=u, file=trim(path), status='replace', action='write')
+    write(u, '(1x, a)') trim(desc)
+    write(u, '(i12)') iter
+    write(u, '(2x, es22.15e3)') 0.0d0
+    write(u, '(2a)') '_MOLECULE_NAME_: ', nm
+    write(u, '(i20, a)') n, ' # Nmoles'
+    write(u, '(i20, a)') nsite, ' # Natoms'
+    write(u, '(a)') ' _Principal Coordinates Only_'
+    do m = 1, n
+       call toxyz(com(:, m), ang(:, m), rp)
+       do b = 1, nsite
+          write(u, '(a)') trim(vec13(rp(:, b)))
+       end do
+    end do
+    write(u, '(a)') ' _Velocities_'
+    do m = 1, n * nsite
+       write(u, '(a)') trim(vec13([0.0d0, 0.0d0, 0.0d0]))
+    end do
+    write(u, '(a)') ' _Generalized Coordinates_'
+    do m = 1, n
+       write(u, '(a)') 'FULLY_RIGID'
+       write(u, '(a, 3f13.5, i5)') trim(vec13(com(:, m))), ang(2, m), ang(1, m), ang(3, m), dof
+    end do
+    close(u)
+    allocate(Nc(nrep_now()), ix(nrep_now()), iy(nrep_now()), en(nrep_now()), sN(nrep_now()), nst(nrep_now()))
+    allocate(att(4, nrep_now()), acc(4, nrep_now()))
+    call check(mh_gcmc_get_state(g, Nc, en, sN, nst, att, acc, ix, iy), 'mh_gcmc_get_state')
+    open(newunit=u, file=trim(path) // '.state', status='replace', action='write')
+    write(u, '(2i12)') ix(rep + 1), iy(rep + 1)
+    write(u, '(i12)') n
+    do m = 1, n
+       write(u, '(6es25.17e3)') com(:, m), ang(:, m)
+    end do
+    close(u)
+  end subroutine write_restart
+
+  integer function nrep_now()
+    nrep_now = merge(1, npts, chain)
+  end function nrep_now
+
+  ! 'Configuration Initialization' (config_init, config.F90:220-246): the
+  ! source of the GCMC species; GCMC / NULL start empty, RESTARTFILE reads
+  ! config_readrestartfile's layout
+  subroutine read_config_source()
+    integer :: ii
+    character(len=LL) :: w1, src
+    srcfile = ''
+    ii = find_section(sy%t, 'Configuration Initialization')
+    if (ii < 0) return
+    do while (ii + 1 <= sy%t%n)
+       w1 = strip(sy%t%line(ii))
+       if (len_trim(w1) == 0 .or. verify(trim(w1), '-') == 0) exit
+       if (trim(word(w1, 1)) == trim(spname)) then
+          src = strip(sy%t%line(ii + 1))
+          select case (trim(upper(word(src, 1))))
+          case ('RESTARTFILE')
+             srcfile = word(src, 2)
+             if (.not. exists(trim(srcfile))) srcfile = trim(sy%base) // '/' // trim(srcfile)
+             if (.not. exists(trim(srcfile))) call fail('restart file not found: ' // trim(word(src, 2)))
+          case ('GCMC', 'NULL', 'DFLT_SIZE', 'DEFAULT')
+          case default
+             call fail('configuration source ' // trim(word(src, 1)) // ' is not offered (GCMC or RESTARTFILE)')
+          end select
+          return
+       end if
+       ii = ii + 2
+    end do
+  end subroutine read_config_source
+
+  ! config_readrestartfile (config.F90:2114-2247): the species' molecules
+  ! from their generalized coordinates (centre of mass, theta, phi, psi) into
+  ! every replica of batch gb; with 'resume' the saved random-number state too
+  subroutine start_from_restart(gb, nr)
+    type(c_ptr), intent(in) :: gb
+    integer, intent(in) :: nr
+    type(text_file) :: rf
+    integer :: ii, nm, na, m, r, u, ios
+    real(c_double), allocatable :: com(:,:), ang(:,:)
+    real(c_double) :: v(7)
+    integer(c_int32_t) :: rx, ry
+    character(len=LL) :: rng
+    call read_text(srcfile, rf)
+    ii = 1
+    do while (ii <= rf%n)
+       if (index(rf%line(ii), '_MOLECULE_NAME_:') == 1) then
+          if (trim(word(rf%line(ii), 2)) == trim(spname)) exit
+       end if
+       ii = ii + 1
+    end do
+    if (ii > rf%n) call fail('species ' // trim(spname) // ' not in the restart file')
+    read(strip(rf%line(ii + 1)), *) nm
+    read(strip(rf%line(ii + 2)), *) na
+    if (na /= nsite) call fail('the restart file has another number of atoms per molecule')
+    if (nm > maxn) call fail('the restart file holds more molecules than maxn')
+    ! principal coordinates and velocities (3 values per atom, any per line)
+    ii = ii + 4 + lines_for(ii + 4, nm * na)
+    ii = ii + 1 + lines_for(ii + 1, nm * na)
+    ii = ii + 1
+    allocate(com(3, max(nm, 1)), ang(3, max(nm, 1)))
+    do m = 1, nm
+       if (index(upper(rf%line(ii)), 'RIGID') > 0 .or. index(upper(rf%line(ii)), 'FLEXIBLE') > 0) ii = ii + 1
+       v = 0.0d0
+       read(strip(rf%line(ii)), *, iostat=ios) v(1:6)
+       if (ios /= 0) call fail('cannot read generalized coordinates: ' // trim(rf%line(ii)))
+       com(:, m) = v(1:3)
+       ang(:, m) = [v(5), v(4), v(6)]  ! file order theta, phi, psi
+       ii = ii + 1
+    end do
+    if (resume) then
+       rng = trim(srcfile) // '.state'
+       if (.not. exists(trim(rng))) call fail('resume: no saved state ' // trim(rng))
+       open(newunit=u, file=trim(rng), status='old', action='read')
+       read(u, *) rx, ry
+       read(u, *) m
+       if (m /= nm) call fail('resume: the saved state does not belong to the restart file')
+       do m = 1, nm
+          read(u, *) com(:, m), ang(:, m)
+       end do
+       close(u)
+    end if
+    do r = 1, nr
+       call check(mh_gcmc_set_config(gb, int(r - 1, c_int), int(nm, c_int), com, ang), 'mh_gcmc_set_config')
+       if (resume) call check(mh_gcmc_set_rng(gb, int(r - 1, c_int), rx, ry), 'mh_gcmc_set_rng')
+    end do
+    write(*, '(a, i0, 2a)') '  started from ', nm, ' molecules of ', trim(srcfile)
+  contains
+    ! lines holding n 3-vectors from line i0 on
+    integer function lines_for(i0, n)
+      integer, intent(in) :: i0, n
+      integer :: got, l
+      got = 0
+      l = i0
+      do while (got < n)
+         got = got + nwords(strip(rf%line(l))) / 3
+         l = l + 1
+      end do
+      lines_for = l - i0
+    end function lines_for
+  end subroutine start_from_restart
 
   function basic_tag(mt) result(o)
     integer(c_int), intent(in) :: mt

@@ -32,6 +32,7 @@ module music_inputs
      integer(c_int), allocatable :: atype(:)       ! 0-based atom-type index
      real(c_double), allocatable :: xyz(:,:), q(:)  ! xyz(3, natoms)
      logical :: rigid = .false., translate = .true., rotate = .true.
+     integer :: dof = 0                              ! Molecule_DOF (0: not given)
      logical :: has_fundcell = .false.
      real(c_double) :: ell(3) = 0.0d0, angles(3) = 90.0d0
   end type molecule
@@ -331,6 +332,8 @@ contains
              end if
           end do
           i = i + n
+       else if (index(s, 'Molecule_DOF:') == 1) then
+          read(s(14:), *) m%dof
        else if (index(s, 'Fundcell_Info:') == 1) then
           m%has_fundcell = .true.
           read(strip(t%line(i + 1)), *) m%ell

@@ -138,3 +138,22 @@ def bent_spec():
                 lj=(A, B, h2, l2, [1] * 4), T=300.0, ewald=dict(kappa=kappa, kmax=5, rc=min(4.32 / kappa, 10.0)),
                 moves=[("RINSERT", 1.0), ("RDELETE", 1.0), ("RTRANSLATE", 1.0), ("RROTATE", 1.0)],
                 dtrans=0.3, drot=0.3)
+
+
+def read_restart_rp(path):
+    """Principal coordinates [nmoles][natoms][3] of each species of a restart
+    file (config_writerestartfile, config.F90:2357-2430)."""
+    lines = open(path).read().split("\n")
+    out = {}
+    i = 0
+    while i < len(lines):
+        if lines[i].startswith("_MOLECULE_NAME_:"):
+            name = lines[i].split(":", 1)[1].split()[0]
+            nm = int(lines[i + 1].split()[0])
+            na = int(lines[i + 2].split()[0])
+            j = i + 4
+            rp = np.array([[float(v) for v in lines[j + k].split()[:3]] for k in range(nm * na)])
+            out[name] = rp.reshape(nm, na, 3)
+            i = j + nm * na
+        i += 1
+    return out

@@ -98,7 +98,7 @@ def test_bent_sorbate_parameters():
     ("5, 50, 500, +", 5, None),                          # continued on the next line
 ])
 def test_pressure_lists(tmp_path, line, npts, want):
-    shutil.copytree(FIX, tmp_path / "fx")
+    shutil.copytree(FIX, tmp_path / "fx", copy_function=shutil.copyfile)
     ctrl = tmp_path / "fx" / "gcmc.ctr"
     txt = ctrl.read_text()
     txt = txt.replace("10, 100, 1000      # Fugacity (kPa)(Range)", line + "   # Fugacity (kPa)(Range)" +
@@ -106,7 +106,7 @@ def test_pressure_lists(tmp_path, line, npts, want):
     txt = txt.replace("3                  # No. of simulation points", "%d                  # No. of simulation points" % npts)
     ctrl.write_text(txt)
     # the molecule and atom files live in tests/golden/potentials
-    shutil.copytree(os.path.join(GOLD, "potentials"), tmp_path / "potentials")
+    shutil.copytree(os.path.join(GOLD, "potentials"), tmp_path / "potentials", copy_function=shutil.copyfile)
     d = _check(str(ctrl))
     if want is None:
         want = [5.0, 50.0, 500.0, 7000.0, 9000.0]

@@ -130,3 +130,16 @@ def test_ewald_total_is_kappa_independent():
     mols = [g.r[m] for m in range(6)]
     e_np = _ewald_numpy(None, mols, np.array(sp["sq"]), L, 0.45, 14, 8.99)
     assert abs(e_or - e_np) < 2e-4 * max(1.0, abs(e_np)), (e_or, e_np)
+
+
+def test_test007_final_configuration_equals_reference_restart():
+    """The reference's restart file written at the end of test007
+    (tests/test007-gcmc-CH4/100.res.1: 21 molecules, principal coordinates
+    f13.5) against the oracle chain's final molecules, slot by slot."""
+    import music_gcmc_run as MG
+    spec, fug, seed, niter = _test007_spec()
+    g = O.Gcmc(spec, fug, seed)
+    g.run(niter)
+    rp = MG.read_restart_rp(os.path.join(GOLD, "test007-gcmc-CH4", "100.res.1"))["CH4"]
+    assert g.st.N == len(rp) == 21
+    assert np.abs(g.rp[:g.st.N] - rp).max() < 5.01e-6

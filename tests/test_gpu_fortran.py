@@ -238,3 +238,51 @@ def test_music_gcmc_biased_moves(hip):
             _compare([b for b in blocks if b["sim"] == p + 1], _oracle_blocks(g, 3000))
             assert abs(ends[p + 1]["stored"] - g.st.energy * 4.184) < 1.5e-3
         assert sum(b["moves"]["INSERT"][0] for b in blocks if b["it"] == 3000) > 20
+
+
+def _t007_tree(td, **edits):
+    """test007's inputs in td (ctrlfile edited line by line: key -> (old, new))."""
+    import shutil
+    # contents only: the checked-out tree may be read-only
+    shutil.copytree(os.path.join(GOLD, "test007-gcmc-CH4"), os.path.join(td, "t7"), dirs_exist_ok=True,
+                    copy_function=shutil.copyfile)
+    if not os.path.exists(os.path.join(td, "potentials")):
+        shutil.copytree(os.path.join(GOLD, "potentials"), os.path.join(td, "potentials"),
+                        copy_function=shutil.copyfile)
+    ctrl = os.path.join(td, "t7", "test007.ctr")
+    txt = open(os.path.join(GOLD, "test007-gcmc-CH4", "test007.ctr")).read()
+    for old, new in edits.values():
+        assert old in txt, old
+        txt = txt.replace(old, new, 1)
+    open(ctrl, "w").write(txt)
+    return ctrl
+
+
+ICRASH = ("100000        # Steps between writes to restart file", "%d        # Steps between writes to restart file")
+NITER = ("20000         # No. of MD/MCiterations", "%d         # No. of MD/MCiterations")
+
+
+def test_music_gcmc_restart_file_and_resume(hip):
+    """Restart files: test007 with a restart written at step 20000 is the
+    reference's own restart file (tests/test007-gcmc-CH4/100.res.1) byte for
+    byte; a run split at 10000 and resumed from its restart (full-precision
+    state and random-number state) ends in the same file."""
+    print("case: music_gcmc restart files and resume (test007)", flush=True)
+    gold = open(os.path.join(GOLD, "test007-gcmc-CH4", "100.res.1"), "rb").read()
+    with tempfile.TemporaryDirectory() as td:
+        ctrl = _t007_tree(td, icrash=(ICRASH[0], ICRASH[1] % 20000))
+        MG.run([ctrl], td)
+        assert open(os.path.join(td, "temp.res.1"), "rb").read() == gold
+    with tempfile.TemporaryDirectory() as td:
+        ctrl = _t007_tree(td, icrash=(ICRASH[0], ICRASH[1] % 10000), niter=(NITER[0], NITER[1] % 10000))
+        MG.run([ctrl], td)
+        assert os.path.exists(os.path.join(td, "temp.res.1.state"))
+        ctrl2 = _t007_tree(td, icrash=(ICRASH[0], ICRASH[1] % 10000), niter=(NITER[0], NITER[1] % 10000),
+                           src=("GCMC NULL # Source of configs", "RESTARTFILE ../temp.res.1 # Source of configs"),
+                           res=("temp.res   # Name of Restart File", "temp2.res   # Name of Restart File"))
+        out = MG.run([ctrl2, "resume"], td)
+        assert "started from" in out
+        got = open(os.path.join(td, "temp2.res.1"), "rb").read().split(b"\n")
+        want = gold.split(b"\n")
+        assert got[1].strip() == b"10000" and want[1].strip() == b"20000"
+        assert got[:1] + got[2:] == want[:1] + want[2:]

@@ -45,6 +45,12 @@ def test_gpu_reproduces_test007_counts(hip):
         assert s["N"][0] == int(cp["N"]), cp["step"]
     s = b.state()
     assert abs(s["energy"][0] * 4.184 - REF7["final_energy_kJ"]) < 5e-4
+    # the final molecules, slot by slot, against the reference's restart file
+    import music_gcmc_run as MG
+    rp = MG.read_restart_rp(os.path.join(GOLD, "test007-gcmc-CH4", "100.res.1"))["CH4"]
+    com, _, _ = b.config(0)
+    assert len(com) == len(rp) == 21
+    assert np.abs(com - rp[:, 0, :]).max() < 5.01e-6
     tot = b.total_energy()
     assert np.allclose(tot, s["energy"], rtol=1e-10, atol=1e-10)
     # the other seeds against the oracle
