@@ -23,6 +23,9 @@ namespace mh {
 
 constexpr int BX = 8, BY = 8, BZ = 4, NT = BX * BY * BZ;
 constexpr int SEG = 512;   // atoms examined (and culled into LDS) per pass
+// FP32 kernels: 480 keeps a workgroup's LDS (atom records + the FP64
+// derivative sums) under 40 KB, i.e. four workgroups per CU
+constexpr int SEG32 = 480;
 
 constexpr double kPiR = 3.1415927;  // defaults.F90:33
 constexpr double kTokcal = 332.064;                       // ewald.F90:60
@@ -417,21 +420,23 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
                                                  const double *__restrict__ recip,
                                                  float *__restrict__ out32,
                                                  double *__restrict__ out64) {
-    __shared__ float s_x[SEG], s_y[SEG], s_z[SEG], s_xl[SEG], s_yl[SEG], s_zl[SEG];
-    __shared__ float s_p0[SEG], s_p1[SEG];
-    __shared__ int s_a[SEG], s_t[SEG];
+    // culled atoms, one LDS record per atom read with one ds_read_b128:
+    // (x, y, z hi parts relative to the brick centre, p0 = A or charge q);
+    // LJ adds (x, y, z lo parts, B) and the FP32 cutoff band (h2 -+ 1e-6 h2)
+    __shared__ float4 s_a4[SEG32];
+    __shared__ float4 s_b4[KIND == K_LJ ? SEG32 : 1];
+    __shared__ float2 s_hb[KIND == K_LJ ? SEG32 : 1];
+    __shared__ int s_a[SEG32];
     __shared__ int s_wtot[NT / 64];
     // FP64 sums of the 7 derivatives, one lane-private column per node
     // (added once per 32-atom tile; keeps them out of the VGPR budget)
     __shared__ double s_acc[7][NT];
-    __shared__ float s_h2f[kMaxTypes];
     __shared__ PairTab tab;
     const int tid = threadIdx.x;
     if (KIND == K_LJ) {
         for (int i = tid; i < (int)(sizeof(PairTab) / sizeof(double)); i += NT)
             reinterpret_cast<double *>(&tab)[i] = reinterpret_cast<const double *>(tab_g)[i];
         __syncthreads();
-        if (tid < kMaxTypes) s_h2f[tid] = (float)tab.h2[tid];
     }
     const int nbx = (g.nx + BX - 1) / BX, nby = (g.ny + BY - 1) / BY;
     const int b = blockIdx.x;
@@ -460,8 +465,8 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
     double S0 = 0.0;
     for (int i = 0; i < 7; i++) s_acc[i][tid] = 0.0;
     bool failed = false;
-    for (int seg0 = 0; seg0 < g.natoms; seg0 += SEG) {
-        const int segn = min(SEG, g.natoms - seg0);
+    for (int seg0 = 0; seg0 < g.natoms; seg0 += SEG32) {
+        const int segn = min(SEG32, g.natoms - seg0);
         int cnt = 0;
         for (int c0 = 0; c0 < segn; c0 += NT) {
             int a = seg0 + c0 + tid;
@@ -491,22 +496,17 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
                 double fy = fold1(cen[1] - ay, g.full[1]);
                 double fz = fold1(cen[2] - az, g.full[2]);
                 float hx = (float)fx, hy = (float)fy, hz = (float)fz;
-                s_x[pos] = hx;
-                s_y[pos] = hy;
-                s_z[pos] = hz;
-                s_xl[pos] = (float)(fx - (double)hx);
-                s_yl[pos] = (float)(fy - (double)hy);
-                s_zl[pos] = (float)(fz - (double)hz);
                 s_a[pos] = a;
                 if (KIND == K_LJ) {
-                    int t = typ[a];
-                    s_t[pos] = t;
-                    s_p0[pos] = (float)tab.A[t];
-                    s_p1[pos] = (float)tab.B[t];
+                    const int t = typ[a];
+                    s_a4[pos] = make_float4(hx, hy, hz, (float)tab.A[t]);
+                    s_b4[pos] = make_float4((float)(fx - (double)hx), (float)(fy - (double)hy),
+                                            (float)(fz - (double)hz), (float)tab.B[t]);
+                    const float h2f = (float)tab.h2[t];
+                    const float band = h2f * 1.0e-6f;
+                    s_hb[pos] = make_float2(h2f - band, h2f + band);
                 } else {
-                    s_t[pos] = 0;
-                    s_p0[pos] = (float)(g.charge1 * qv[a]);
-                    s_p1[pos] = 0.0f;
+                    s_a4[pos] = make_float4(hx, hy, hz, (float)(g.charge1 * qv[a]));
                 }
             }
             cnt += add;
@@ -520,14 +520,17 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
                     // LJ: hi/lo atom offsets (steep repulsion); Ewald pairs
                     // beyond the FP64 radius are smooth: hi part only
                     float sx, sy, sz;
+                    const float4 a4 = s_a4[j];
+                    float4 b4;
                     if (KIND == K_LJ) {
-                        sx = (dx0 + s_xl[j]) + s_x[j];
-                        sy = (dy0 + s_yl[j]) + s_y[j];
-                        sz = (dz0 + s_zl[j]) + s_z[j];
+                        b4 = s_b4[j];
+                        sx = (dx0 + b4.x) + a4.x;
+                        sy = (dy0 + b4.y) + a4.y;
+                        sz = (dz0 + b4.z) + a4.z;
                     } else {
-                        sx = dx0 + s_x[j];
-                        sy = dy0 + s_y[j];
-                        sz = dz0 + s_z[j];
+                        sx = dx0 + a4.x;
+                        sy = dy0 + a4.y;
+                        sz = dz0 + a4.z;
                     }
                     if (refold) {
                         sx = foldf(sx, Lx);
@@ -536,14 +539,13 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
                     }
                     float r2 = sx * sx + sy * sy + sz * sz;
                     if (KIND == K_LJ) {
-                        const int t = s_t[j];
-                        const float h2f = s_h2f[t];
-                        const float band = h2f * 1.0e-6f;
-                        if (r2 > h2f + band) continue;
+                        const float2 hb = s_hb[j];
+                        if (r2 > hb.y) continue;
                         // near-atom pairs and pairs within rounding of hicut:
                         // FP64 separation and reference cutoff decisions
-                        if (r2 < r2safe || r2 > h2f - band) {
+                        if (r2 < r2safe || r2 > hb.x) {
                             int a = s_a[j];
+                            const int t = typ[a];
                             // node position recomputed here (not kept live in the loop)
                             double ex = fold1((double)x * g.step[0] - xyz[3 * a], g.full[0]);
                             double ey = fold1((double)y * g.step[1] - xyz[3 * a + 1], g.full[1]);
@@ -554,7 +556,7 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
                             for (int i = 0; i < 7; i++) s_acc[i][tid] += o.v[i + 1];
                             continue;
                         }
-                        lj_pair32(s_p0[j], s_p1[j], sx, sy, sz, r2, pf, hf);
+                        lj_pair32(a4.w, b4.w, sx, sy, sz, r2, pf, hf);
                     } else {
                         if (r2 > g.rc2f) continue;
                         if (r2 < r2safe) {
@@ -569,7 +571,7 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
                             for (int i = 0; i < 7; i++) s_acc[i][tid] += o.v[i + 1];
                             continue;
                         }
-                        ewald_real_acc32(kappaf, s_p0[j], sx, sy, sz, r2, S0, hf);
+                        ewald_real_acc32(kappaf, a4.w, sx, sy, sz, r2, S0, hf);
                     }
                 }
                 if (KIND == K_LJ) S0 += (double)pf;
