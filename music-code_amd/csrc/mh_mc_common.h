@@ -74,13 +74,37 @@ __device__ __forceinline__ bool cell_axis(double c, double ell, double st, int n
 // bit i = upper node along the axis for corner i / derivative order for slot i
 constexpr int kCX = 0xB2, kCY = 0xD4, kCZ = 0xE8;
 
-// block reduction of NV doubles per lane, fixed order (lane tree then waves)
+// v from the lane CTRL names (DPP on both 32-bit halves); lanes with no
+// source (and rows outside ROWS) read 0
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// wavefront sum in registers, fixed order: row_shr 1, 2, 4, 8 leave each
+// 16-lane row's sum in its lane 15; row_bcast15 and row_bcast31 carry the row
+// sums up to lane 63, whose value is returned to every lane
+__device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_f64<0x111>(v);
+    v += dpp_f64<0x112>(v);
+    v += dpp_f64<0x114>(v);
+    v += dpp_f64<0x118>(v);
+    v += dpp_f64<0x142, 0xa>(v);
+    v += dpp_f64<0x143, 0xc>(v);
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
+
+// block reduction of NV doubles per lane, fixed order (wave sums, then the
+// waves in order)
 template <int NT, int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double *red /*[NT/64][NV]*/) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i < NV; i++)
-        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off);
+    for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
     if (lane == 0)
         for (int i = 0; i < NV; i++) red[wave * NV + i] = v[i];
     __syncthreads();
