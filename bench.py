@@ -104,12 +104,14 @@ def cpu_baseline(cfg, tabs, seconds=15.0):
                        "one thread, -O3 -march=native, %.1f s)" % (sample, total_t))
 
 
-def gcmc_leg(H, fp32_steps, fp64_steps, equil_steps, chains=4, n_single=8):
+def gcmc_leg(H, fp32_steps, fp64_steps, equil_steps, chains=(4, 32), n_single=8):
     """Batched isotherm: the 64 CO2-like state points advanced together in
     FP32 pair arithmetic (FP64 accumulation), one chain per state point and
-    `chains` independent chains per state point (seeds 3000 + i + 1000 c),
-    against the plain FP64 kernel run one state point at a time on the same
-    equilibrated configurations."""
+    each count in `chains` of independent chains per state point (seeds
+    3000 + i + 1000 c; workgroup size by the autotuner, capacity 256
+    molecules per chain, so several chains share a CU), against the plain
+    FP64 kernel run one state point at a time on the same equilibrated
+    configurations."""
     from musichip import gcmc as G
     from musichip import workloads as W
     ctx = H.Context(0)
@@ -119,7 +121,7 @@ def gcmc_leg(H, fp32_steps, fp64_steps, equil_steps, chains=4, n_single=8):
     def batch(nchains, prec):
         fug = np.tile(cs["fugacities"], nchains)
         seeds = [cs["seeds"][i] + 1000 * c for c in range(nchains) for i in range(npt)]
-        return G.GcmcBatch(ctx, cs["spec"], fug, seeds, maxN=1024, maps=cs["maps"], precision=prec, threads=256)
+        return G.GcmcBatch(ctx, cs["spec"], fug, seeds, maxN=256, maps=cs["maps"], precision=prec, threads=256)
 
     def timed(b, steps):
         s0 = b.state()
@@ -133,7 +135,7 @@ def gcmc_leg(H, fp32_steps, fp64_steps, equil_steps, chains=4, n_single=8):
                         "(30 A, 768 atoms), pmaps C/O at 0.2 A, sorbate LJ 12.8 A + Ewald kappa 0.25/A "
                         "kmax 7 (nk %d half-space), moves 1:1:1:1" % (((2 * 7 + 1) ** 3 - 1) // 2))
     cfgs = None
-    for nch in (1, chains):
+    for nch in (1,) + tuple(chains):
         b = batch(nch, H.MH_PREC_FP32)
         t = time.perf_counter()
         b.run(equil_steps)
@@ -172,10 +174,12 @@ def gcmc_leg(H, fp32_steps, fp64_steps, equil_steps, chains=4, n_single=8):
     out["fp64_single_state_point_steps_per_s"] = rate64
     out["fp64_single_state_point_attempted_moves_per_s"] = tot_att / tot_t
     out["fp64_single_state_point_threads"] = tuned64
-    for nch in (1, chains):
+    best = 0.0
+    for nch in (1,) + tuple(chains):
         k = "fp32_batched_%dchains_steps_per_s" % (npt * nch)
         out["speedup_%dchains_vs_fp64_single" % (npt * nch)] = out[k] / rate64
-    out["fp32_batched_steps_per_s"] = out["fp32_batched_%dchains_steps_per_s" % (npt * chains)]
+        best = max(best, out[k])
+    out["fp32_batched_steps_per_s"] = best
     # FP32 vs FP64 trial energies on the equilibrated configurations
     rng = np.random.default_rng(11)
     L = np.asarray(cs["spec"]["full_ell"])
@@ -194,7 +198,7 @@ def gcmc_leg(H, fp32_steps, fp64_steps, equil_steps, chains=4, n_single=8):
             worst = max(worst, float(np.max(np.abs(e32[both] - e64[both]) / np.maximum(np.abs(e64[both]), 1.0))))
     out["fp32_vs_fp64_trial_energy_max_rel_err"] = worst
     out["steps"] = dict(equil=equil_steps, fp32=fp32_steps, fp64_per_state_point=fp64_steps,
-                        fp64_state_points=[int(x) for x in picks], chains_per_state_point=chains)
+                        fp64_state_points=[int(x) for x in picks], chains_per_state_point=list(chains))
     b64.close()
     b32.close()
     for m in cs["map_objs"]:
@@ -294,7 +298,8 @@ def main():
     ap.add_argument("--gcmc-fp64-steps", type=int, default=1000)
     ap.add_argument("--gcmc-equil", type=int, default=30000)
     ap.add_argument("--no-gcmc", action="store_true")
-    ap.add_argument("--gcmc-chains", type=int, default=4)
+    ap.add_argument("--gcmc-chains", default="4,32",
+                    help="comma-separated chains per state point of the batched FP32 runs")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-hmc", action="store_true")
     ap.add_argument("--hmc-moves", type=int, default=20)
@@ -402,7 +407,8 @@ def main():
 
     gcmc = None
     if not args.no_gcmc:
-        gcmc = gcmc_leg(H, args.gcmc_steps, args.gcmc_fp64_steps, args.gcmc_equil, args.gcmc_chains)
+        gcmc = gcmc_leg(H, args.gcmc_steps, args.gcmc_fp64_steps, args.gcmc_equil,
+                        tuple(int(c) for c in str(args.gcmc_chains).split(",") if c))
 
     hmc = None
     if not args.no_hmc:
