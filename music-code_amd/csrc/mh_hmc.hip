@@ -35,6 +35,13 @@ namespace mh {
 constexpr int HS = MH_MAX_SITES;
 constexpr int HT = 16;
 constexpr int JLEV = 20;  // jump-ahead levels: up to 2^20 draws per move
+// neighbour lists: radius hicut + kSkin, rebuilt at the start of every move
+// and whenever a site has moved more than half the skin since the last build
+// (pairs left out then stay beyond hicut); entries j | b << 13 (partner site
+// j < 8192, its atom index b < 8)
+constexpr float kSkin = 1.0f;
+constexpr float kHalfSkin2 = 0.49f * 0.49f;
+static_assert(HS <= 8, "list entries hold the partner's atom index in 3 bits");
 
 struct HDev {
     int ns, nst, nb, na, nt, nmd, prec, maxN;
@@ -42,9 +49,11 @@ struct HDev {
     double mass[HS], ainv[HS], sigma[HS];
     MapRef map[HS];
     double full[3], fund[3];
-    double A[64], B[64], h2[64], l2[64];
-    float Af[64], Bf[64], h2f[64], l2f[64];
-    int on[64];
+    // LJ of the site pair (a, b) (index a * HS + b): A, B, hicut^2 (-1 where
+    // the type pair is switched off), locut^2; list radius^2 (-1 where off)
+    float4 pp32[HS * HS];
+    double pp64[HS * HS][4];
+    float rl2[HS * HS];
     int bi[HT][2];
     double bk[HT], br0[HT];
     int ai[HT][3];
@@ -63,6 +72,8 @@ struct HRep {
     long long *att, *acc, *fail;
     double *rp, *r;             // [rep][maxN][ns][3]
     double *rp0, *r0;           // saved start configuration of the move
+    uint16_t *nbl;              // [rep][cap][cap] neighbour lists, entry-major
+    int cap;                    // maxN * nsites
 };
 
 constexpr double kScaleKE = 0.01;   // (A/ps)^2 amu -> kJ/mol, defaults.F90:76
@@ -134,6 +145,9 @@ __device__ bool hermite_force(const MapRef &m, const double *r, double p[4]) {
         hd[d][1][1] = st * u * (3.0 * u - 2.0);
     }
     double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+    // two corners per pass: all eight in flight would hold 64 corner values
+    // and push the HMC kernel into scratch
+#pragma unroll 2
     for (int c = 0; c < 8; c++) {
         const int cx = (kCX >> c) & 1, cy = (kCY >> c) & 1, cz = (kCZ >> c) & 1;
         const long long node = ((long long)(cx ? i1[0] : i0[0]) * m.n[1] + (cy ? i1[1] : i0[1])) * m.n[2] +
@@ -263,10 +277,53 @@ __device__ void bonded_site(const HDev &P, const double *R, int a, double f[3], 
     }
 }
 
-// energy (block total) and forces on the lane's sites
+// minimum image of a float separation (single fold for |d| < 1.5 L)
+__device__ __forceinline__ float fold32(float d, float L, float iL) { return d - L * rintf(d * iL); }
+
+// neighbour lists of the lane's sites from the positions r32: partner sites j
+// of the other molecules, ascending, within the list radius of the site pair;
+// the lane's build positions into rb
+template <int NT, int SPL>
+__device__ void hmc_lists(const HDev &P, const float4 *r32, int N, uint16_t *nbl, int cap, int (&cnt)[SPL],
+                          float (&rb)[SPL][3]) {
+    const int tid = threadIdx.x, ns = P.ns, nsite = N * ns;
+    const float Lx = (float)P.full[0], Ly = (float)P.full[1], Lz = (float)P.full[2];
+    const float iLx = 1.0f / Lx, iLy = 1.0f / Ly, iLz = 1.0f / Lz;
+#pragma unroll
+    for (int k = 0; k < SPL; k++) {
+        cnt[k] = 0;
+        const int s = tid + k * NT;
+        if (s >= nsite) continue;
+        const int m = s / ns, a = s - m * ns;
+        const float4 pi = r32[s];
+        rb[k][0] = pi.x;
+        rb[k][1] = pi.y;
+        rb[k][2] = pi.z;
+        uint16_t *lst = nbl + s;
+        const float *rl2 = P.rl2 + a * HS;
+        int c = 0, mj = 0, b = 0;
+        for (int j = 0; j < nsite; j++) {
+            if (mj != m) {
+                const float4 pj = r32[j];
+                const float x = fold32(pi.x - pj.x, Lx, iLx), y = fold32(pi.y - pj.y, Ly, iLy),
+                            z = fold32(pi.z - pj.z, Lz, iLz);
+                if (x * x + y * y + z * z <= rl2[b]) {
+                    lst[(long long)c * cap] = (uint16_t)(j | (b << 13));
+                    c++;
+                }
+            }
+            if (++b == ns) b = 0, mj++;
+        }
+        cnt[k] = c;
+    }
+}
+
+// energy (block total) and forces on the lane's sites; sorbate-sorbate LJ
+// over the neighbour lists
 template <int NT, int PREC, int SPL>
-__device__ void hmc_forces(const HDev &P, const double *rp, const double *r, int N, double (&F)[SPL][3],
-                           double &U, int &fail, double *red) {
+__device__ void hmc_forces(const HDev &P, const double *rp, const double *r, const float4 *r32, int N,
+                           const uint16_t *nbl, int cap, const int (&cnt)[SPL], double (&F)[SPL][3], double &U,
+                           int &fail, double *red) {
     const int tid = threadIdx.x, ns = P.ns, nsite = N * ns;
     double e = 0.0, bad = 0.0;
 #pragma unroll
@@ -286,53 +343,57 @@ __device__ void hmc_forces(const HDev &P, const double *rp, const double *r, int
                 bad = 1.0;
             }
         }
-        // sorbate-sorbate LJ against every site of the other molecules
-        const int ta = P.st[a];
+        // sorbate-sorbate LJ against the listed partners in site order: the
+        // list holds every partner within hicut, so the sums are those of the
+        // loop over all sites of the other molecules that skips pairs beyond
+        // hicut and switched-off type pairs
+        const uint16_t *lst = nbl + s;
+        const int n = cnt[k];
         double el = 0.0;
-        float elf = 0.0f, fxf = 0.0f, fyf = 0.0f, fzf = 0.0f;
-        for (int j = 0; j < nsite; j++) {
-            const int mj = j / ns;
-            if (mj == m) {
-                j = mj * ns + ns - 1;
-                continue;
-            }
-            const int b = j - mj * ns;
-            const int t = ta * P.nst + P.st[b];
-            if (!P.on[t]) continue;
-            const double sx = fold64(ri[0] - r[3 * j], P.full[0]);
-            const double sy = fold64(ri[1] - r[3 * j + 1], P.full[1]);
-            const double sz = fold64(ri[2] - r[3 * j + 2], P.full[2]);
-            if (PREC == MH_PREC_FP64) {
+        if (PREC == MH_PREC_FP64) {
+            for (int q = 0; q < n; q++) {
+                const int ent = lst[(long long)q * cap];
+                const int j = ent & 0x1fff;
+                const double *pr = P.pp64[a * HS + (ent >> 13)];
+                const double sx = fold64(ri[0] - r[3 * j], P.full[0]);
+                const double sy = fold64(ri[1] - r[3 * j + 1], P.full[1]);
+                const double sz = fold64(ri[2] - r[3 * j + 2], P.full[2]);
                 const double r2 = sx * sx + sy * sy + sz * sz;
-                if (r2 > P.h2[t]) continue;
-                if (r2 < P.l2[t]) {
-                    bad = 1.0;
-                    continue;
-                }
+                if (r2 > pr[2]) continue;
+                if (r2 < pr[3]) bad = 1.0;
                 const double r2i = 1.0 / r2, r6i = r2i * r2i * r2i, r12i = r6i * r6i;
-                el += P.A[t] * r12i - P.B[t] * r6i;
-                const double fm = (12.0 * P.A[t] * r12i - 6.0 * P.B[t] * r6i) * r2i;
+                el += pr[0] * r12i - pr[1] * r6i;
+                const double fm = (12.0 * pr[0] * r12i - 6.0 * pr[1] * r6i) * r2i;
                 f[0] += sx * fm;
                 f[1] += sy * fm;
                 f[2] += sz * fm;
-            } else {
+            }
+        } else {
 #pragma clang fp contract(fast)
-                const float x = (float)sx, y = (float)sy, z = (float)sz;
+            const float Lx = (float)P.full[0], Ly = (float)P.full[1], Lz = (float)P.full[2];
+            const float iLx = 1.0f / Lx, iLy = 1.0f / Ly, iLz = 1.0f / Lz;
+            const float4 pi = r32[s];
+            const float4 *pp = P.pp32 + a * HS;
+            float elf = 0.0f, fxf = 0.0f, fyf = 0.0f, fzf = 0.0f;
+            bool badf = false;
+#pragma unroll 4
+            for (int q = 0; q < n; q++) {
+                const int ent = lst[(long long)q * cap];
+                const float4 pj = r32[ent & 0x1fff];
+                const float4 pr = pp[ent >> 13];
+                const float x = fold32(pi.x - pj.x, Lx, iLx), y = fold32(pi.y - pj.y, Ly, iLy),
+                            z = fold32(pi.z - pj.z, Lz, iLz);
                 const float r2 = x * x + y * y + z * z;
-                if (r2 > P.h2f[t]) continue;
-                if (r2 < P.l2f[t]) {
-                    bad = 1.0;
-                    continue;
-                }
-                const float r2i = rcp_fast(r2), r6i = r2i * r2i * r2i, r12i = r6i * r6i;
-                elf += P.Af[t] * r12i - P.Bf[t] * r6i;
-                const float fm = (12.0f * P.Af[t] * r12i - 6.0f * P.Bf[t] * r6i) * r2i;
+                const bool in = !(r2 > pr.z);
+                badf |= in && r2 < pr.w;
+                const float r2i = in ? rcp_fast(r2) : 0.0f, r6i = r2i * r2i * r2i, r12i = r6i * r6i;
+                elf += pr.x * r12i - pr.y * r6i;
+                const float fm = (12.0f * pr.x * r12i - 6.0f * pr.y * r6i) * r2i;
                 fxf += x * fm;
                 fyf += y * fm;
                 fzf += z * fm;
             }
-        }
-        if (PREC != MH_PREC_FP64) {
+            if (badf) bad = 1.0;
             el = elf;
             f[0] += fxf;
             f[1] += fyf;
@@ -350,6 +411,11 @@ __device__ void hmc_forces(const HDev &P, const double *rp, const double *r, int
     fail = v[1] != 0.0;
 }
 
+// float copy of site s's cell position (pair and list arithmetic)
+__device__ __forceinline__ void put32(float4 *r32, const double *r, int s) {
+    r32[s] = make_float4((float)r[3 * s], (float)r[3 * s + 1], (float)r[3 * s + 2], 0.0f);
+}
+
 template <int NT, int PREC, int SPL>
 __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R, int nmoves) {
     extern __shared__ double dyn[];
@@ -364,10 +430,14 @@ __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R,
     const int ns = P.ns, N = R.N[rep], nsite = N * ns;
     const long long off = (long long)rep * P.maxN * ns * 3;
     double *rp = dyn, *r = dyn + (long long)P.maxN * ns * 3;
+    float4 *r32 = reinterpret_cast<float4 *>(dyn + (long long)P.maxN * ns * 6);
+    uint16_t *nbl = R.nbl + (long long)rep * R.cap * R.cap;
     for (int i = tid; i < nsite * 3; i += NT) {
         rp[i] = R.rp[off + i];
         r[i] = R.r[off + i];
     }
+    for (int i = tid; i < nsite; i += NT)
+        r32[i] = make_float4((float)R.r[off + 3 * i], (float)R.r[off + 3 * i + 1], (float)R.r[off + 3 * i + 2], 0.0f);
     Rng rng{R.ix[rep], R.iy[rep], R.first[rep], R.iseed[rep]};
     rng_first(rng);
     int ghave = R.ghave[rep];
@@ -375,6 +445,8 @@ __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R,
     long long att = R.att[rep], acc = R.acc[rep], nfail = R.fail[rep];
     int err = 0;
     double v[SPL][3], a[SPL][3], F[SPL][3];
+    float rb[SPL][3];
+    int cnt[SPL];
     __syncthreads();
     const double dt = P.dt, dt2 = dt * dt * 0.5;
     for (int mv = 0; mv < nmoves; mv++) {
@@ -415,9 +487,10 @@ __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R,
                 R.r0[off + 3 * s + i] = r[3 * s + i];
             }
         }
+        hmc_lists<NT, SPL>(P, r32, N, nbl, R.cap, cnt, rb);
         double U0;
         int f0;
-        hmc_forces<NT, PREC, SPL>(P, rp, r, N, F, U0, f0, red);  // includes barriers (s_z2 visible after)
+        hmc_forces<NT, PREC, SPL>(P, rp, r, r32, N, nbl, R.cap, cnt, F, U0, f0, red);  // s_z2 visible after
         if (ghave && (used & 1u)) gz2 = s_z2;
         if (f0 || U0 > P.max_nrg0) {
             err = 1;  // the reference stops here (hybridmoves.F90:224-229)
@@ -439,6 +512,7 @@ __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R,
         bool good = true;
         double U1 = U0;
         for (int step = 0; step < P.nmd; step++) {
+            int moved = 0;
 #pragma unroll
             for (int k = 0; k < SPL; k++) {
                 const int s = tid + k * NT;
@@ -448,10 +522,14 @@ __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R,
                     r[3 * s + i] = r[3 * s + i] + v[k][i] * dt + a[k][i] * dt2;
                     v[k][i] = v[k][i] + a[k][i] * 0.5 * dt;
                 }
+                put32(r32, r, s);
+                const float dx = (float)r[3 * s] - rb[k][0], dy = (float)r[3 * s + 1] - rb[k][1],
+                            dz = (float)r[3 * s + 2] - rb[k][2];
+                if (dx * dx + dy * dy + dz * dz > kHalfSkin2) moved = 1;
             }
-            __syncthreads();
+            if (__syncthreads_or(moved)) hmc_lists<NT, SPL>(P, r32, N, nbl, R.cap, cnt, rb);
             int fl;
-            hmc_forces<NT, PREC, SPL>(P, rp, r, N, F, U1, fl, red);
+            hmc_forces<NT, PREC, SPL>(P, rp, r, r32, N, nbl, R.cap, cnt, F, U1, fl, red);
             if (fl || U1 > P.max_nrg_md) {
                 good = false;
                 break;
@@ -499,6 +577,7 @@ __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R,
                 const int s = tid + k * NT;
                 if (s >= nsite) continue;
                 for (int i = 0; i < 3; i++) r[3 * s + i] = wrap_full(r[3 * s + i], P.full[i]);
+                put32(r32, r, s);
             }
         } else {
             energy = U0;
@@ -510,6 +589,7 @@ __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R,
                     rp[3 * s + i] = R.rp0[off + 3 * s + i];
                     r[3 * s + i] = R.r0[off + 3 * s + i];
                 }
+                put32(r32, r, s);
             }
         }
         __syncthreads();
@@ -548,14 +628,20 @@ __global__ void __launch_bounds__(NT) k_hmc_forces(const HDev *__restrict__ Pg, 
     const int ns = P.ns, N = R.N[rep], nsite = N * ns;
     const long long off = (long long)rep * P.maxN * ns * 3;
     double *rp = dyn, *r = dyn + (long long)P.maxN * ns * 3;
+    float4 *r32 = reinterpret_cast<float4 *>(dyn + (long long)P.maxN * ns * 6);
+    uint16_t *nbl = R.nbl + (long long)rep * R.cap * R.cap;
     for (int i = tid; i < nsite * 3; i += NT) {
         rp[i] = R.rp[off + i];
         r[i] = R.r[off + i];
     }
+    for (int i = tid; i < nsite; i += NT)
+        r32[i] = make_float4((float)R.r[off + 3 * i], (float)R.r[off + 3 * i + 1], (float)R.r[off + 3 * i + 2], 0.0f);
     __syncthreads();
     double F[SPL][3], U;
-    int fl;
-    hmc_forces<NT, PREC, SPL>(P, rp, r, N, F, U, fl, red);
+    float rb[SPL][3];
+    int fl, cnt[SPL];
+    hmc_lists<NT, SPL>(P, r32, N, nbl, R.cap, cnt, rb);
+    hmc_forces<NT, PREC, SPL>(P, rp, r, r32, N, nbl, R.cap, cnt, F, U, fl, red);
 #pragma unroll
     for (int k = 0; k < SPL; k++) {
         const int s = tid + k * NT;
@@ -574,11 +660,11 @@ using namespace mh;
 
 struct mh_hmc {
     mh_ctx *ctx = nullptr;
-    int nrep = 0, maxN = 0, ns = 0, prec = 0, threads = 256, spl = 1;
+    int nrep = 0, maxN = 0, ns = 0, prec = 0, threads = 512, spl = 1;
     size_t lds = 0;
     HDev host;
     DBuf dP;
-    DBuf N, err, ix, iy, first, iseed, ghave, gz2, energy, dH, att, acc, fail, rp, r, rp0, r0;
+    DBuf N, err, ix, iy, first, iseed, ghave, gz2, energy, dH, att, acc, fail, rp, r, rp0, r0, nbl;
     HRep rep() {
         HRep h;
         h.N = N.as<int>();
@@ -598,6 +684,8 @@ struct mh_hmc {
         h.r = r.as<double>();
         h.rp0 = rp0.as<double>();
         h.r0 = r0.as<double>();
+        h.nbl = nbl.as<uint16_t>();
+        h.cap = maxN * ns;
         return h;
     }
 };
@@ -658,7 +746,7 @@ static int plan_hmc(mh_hmc *h) {
     MH_CHECK(h->spl > 0, MH_ECAPACITY, "mh_hmc: maxN * nsites exceeds 16 sites per lane at this workgroup size");
     int dev_max = 65536;
     (void)hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->ctx->device);
-    h->lds = (size_t)h->maxN * h->ns * 6 * sizeof(double);
+    h->lds = (size_t)h->maxN * h->ns * (6 * sizeof(double) + sizeof(float4));
     MH_CHECK((long long)h->lds + (long long)sizeof(HDev) + 4096 <= dev_max, MH_ECAPACITY,
              "mh_hmc: maxN * nsites too large for LDS (positions of a replica are held in LDS)");
     return MH_OK;
@@ -714,17 +802,19 @@ int mh_hmc_create(mh_ctx *ctx, const mh_hmc_params *p, int nrep, const int32_t *
             }
         }
     }
-    for (int t = 0; t < p->ntypes * p->ntypes; t++) {
-        H.A[t] = p->lj_A[t];
-        H.B[t] = p->lj_B[t];
-        H.h2[t] = p->lj_hicut2[t];
-        H.l2[t] = p->lj_locut2[t];
-        H.Af[t] = (float)p->lj_A[t];
-        H.Bf[t] = (float)p->lj_B[t];
-        H.h2f[t] = (float)p->lj_hicut2[t];
-        H.l2f[t] = (float)p->lj_locut2[t];
-        H.on[t] = p->lj_on[t] ? 1 : 0;
-    }
+    for (int a = 0; a < H.ns; a++)
+        for (int b = 0; b < H.ns; b++) {
+            const int t = H.st[a] * p->ntypes + H.st[b], q = a * HS + b;
+            const bool on = p->lj_on[t] != 0;
+            const double h2 = on ? p->lj_hicut2[t] : -1.0;
+            H.pp64[q][0] = p->lj_A[t];
+            H.pp64[q][1] = p->lj_B[t];
+            H.pp64[q][2] = h2;
+            H.pp64[q][3] = p->lj_locut2[t];
+            H.pp32[q] = make_float4((float)p->lj_A[t], (float)p->lj_B[t], (float)h2, (float)p->lj_locut2[t]);
+            const double rl = on ? std::sqrt(std::max(p->lj_hicut2[t], 0.0)) + kSkin : 0.0;
+            H.rl2[q] = on ? (float)(rl * rl) : -1.0f;
+        }
     auto inr = [&](int i) { return i >= 0 && i < p->nsites; };
     for (int q = 0; q < H.nb; q++) {
         MH_CHECK(inr(p->bond[q][0]) && inr(p->bond[q][1]), MH_EINVAL, "mh_hmc_create: bond atom out of range");
@@ -773,7 +863,8 @@ int mh_hmc_create(mh_ctx *ctx, const mh_hmc_params *p, int nrep, const int32_t *
         (rc = h->ghave.alloc(4 * R)) || (rc = h->gz2.alloc(8 * R)) || (rc = h->energy.alloc(8 * R)) ||
         (rc = h->dH.alloc(8 * R)) || (rc = h->att.alloc(8 * R)) || (rc = h->acc.alloc(8 * R)) ||
         (rc = h->fail.alloc(8 * R)) || (rc = h->rp.alloc(8 * nc)) || (rc = h->r.alloc(8 * nc)) ||
-        (rc = h->rp0.alloc(8 * nc)) || (rc = h->r0.alloc(8 * nc)))
+        (rc = h->rp0.alloc(8 * nc)) || (rc = h->r0.alloc(8 * nc)) ||
+        (rc = h->nbl.alloc(2 * R * (size_t)maxN * h->ns * maxN * h->ns)))
         return rc;
     std::vector<int32_t> zi(R, 0), ones(R, 1), sd(seeds, seeds + R);
     MH_HIP(hipMemcpy(h->first.p, ones.data(), 4 * R, hipMemcpyHostToDevice));

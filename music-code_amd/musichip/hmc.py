@@ -127,6 +127,10 @@ class HmcBatch:
         rp = np.ascontiguousarray(rp, dtype=np.float64).reshape(-1, self.ns, 3)
         H.check(_sig().mh_hmc_set_config(self.h, rep, len(rp), _p(rp)), "mh_hmc_set_config")
 
+    def set_threads(self, n):
+        """Workgroup size per replica (128, 256 or 512)."""
+        H.check(_sig().mh_hmc_set_threads(self.h, int(n)), "mh_hmc_set_threads")
+
     def run(self, nmoves):
         H.check(_sig().mh_hmc_run(self.h, int(nmoves)), "mh_hmc_run")
 

@@ -33,16 +33,21 @@ def sysdata():
     return fw
 
 
-@pytest.mark.parametrize("mname,nmol", [("butane", 3), ("propane", 3)])
-def test_gpu_hmc_fp64_matches_oracle(hip, sysdata, mname, nmol):
-    print("case: HMC %s x %d, FP64 energy/forces and 30-move trajectory" % (mname, nmol), flush=True)
+@pytest.mark.parametrize("mname,nmol,dt,nmd", [("butane", 3, 0.004, 10), ("propane", 3, 0.004, 10),
+                                               ("butane", 7, 0.003, 40)])
+def test_gpu_hmc_fp64_matches_oracle(hip, sysdata, mname, nmol, dt, nmd):
+    """4 fs steps: large enough that some moves are rejected (restore path).
+    40 steps of 3 fs move sites by more than the neighbour-list half skin
+    (0.49 A) within a move, so the lists are rebuilt during the burst; the
+    oracle loops over all site pairs."""
+    print("case: HMC %s x %d, dt %g x %d, FP64 energy/forces and 30-move trajectory" % (mname, nmol, dt, nmd),
+          flush=True)
     model = synth.butane_ua() if mname == "butane" else HS.propane_ua()
     dev, host = _device_maps(hip, sysdata, model)
-    # 4 fs steps: large enough that some moves are rejected (restore path)
-    sp = HS.spec_with_maps(model, host, dt=0.004)
+    sp = HS.spec_with_maps(model, host, dt=dt, nmd=nmd)
     seeds = [101, 202]
     rps = [HS.place(model, host, nmol, seed=s) for s in (11, 12)]
-    b = HM.HmcBatch(hip, sp, dev, seeds, maxN=8, precision=H.MH_PREC_FP64)
+    b = HM.HmcBatch(hip, sp, dev, seeds, maxN=max(8, nmol), precision=H.MH_PREC_FP64)
     ors = []
     for r in range(2):
         b.set_config(r, rps[r])
@@ -54,12 +59,15 @@ def test_gpu_hmc_fp64_matches_oracle(hip, sysdata, mname, nmol):
         assert okg and oko
         assert abs(eg - eo) < 1e-10 * max(1.0, abs(eo))
         assert np.abs(Fg - Fo).max() < 1e-10 * max(1.0, np.abs(Fo).max())
+    max_disp = 0.0
     for mv in range(30):
         b.run(1)
         s = b.state()
         for r in range(2):
+            before = ors[r].rp[:ors[r].st.N].copy()
             ors[r].move()
             st = ors[r].st
+            max_disp = max(max_disp, np.abs(ors[r].rp[:st.N] - before).max())
             assert (s["att"][r], s["acc"][r], s["fail"][r]) == (st.att, st.acc, st.fail), (mv, r)
             if np.isnan(st.last_dH):
                 assert np.isnan(s["last_dH"][r])
@@ -71,8 +79,11 @@ def test_gpu_hmc_fp64_matches_oracle(hip, sysdata, mname, nmol):
         assert np.abs(rp - ors[r].rp[:n]).max() < 1e-8
         assert np.abs(rr - ors[r].r[:n]).max() < 1e-8
     s = b.state()
-    print("accepted %s of %s" % (list(s["acc"]), list(s["att"])), flush=True)
+    print("accepted %s of %s, largest site displacement in a move %.2f A" %
+          (list(s["acc"]), list(s["att"]), max_disp), flush=True)
     assert (s["acc"] > 5).all()
+    if nmd == 40:
+        assert max_disp > 1.0
     b.close()
 
 
