@@ -297,6 +297,7 @@ typedef struct mh_hmc mh_hmc;
 
 int mh_hmc_create(mh_ctx *ctx, const mh_hmc_params *p, int nrep, const int32_t *seeds, int maxN, mh_hmc **out);
 int mh_hmc_destroy(mh_hmc *h);
+/* Threads per replica: 128, 256, 512 (default) or 1024. */
 int mh_hmc_set_threads(mh_hmc *h, int threads);
 /* n molecules of a replica, principal (unwrapped) site coordinates
  * rp[n][nsites][3]; the simulation-cell coordinates are their pbc images. */

@@ -15,12 +15,13 @@
 //      the move as rejected without an RNG draw.
 //   4. Metropolis on dU + dKE / 4.184 (mcmoves_perturb, mcmoves.F90:371-426).
 // Layout: site positions rp (principal) and r (cell) FP64 in LDS for the
-// launch; each lane owns sites s = tid + k NT (k < SPL) whose velocities,
-// accelerations and forces stay in registers.  Forces per owned site: map
-// (forcehermite, interpolate.F90:189-584), LJ against every site of the
-// other molecules (single-fold minimum image), and the bonded terms that
-// contain the site (each term is evaluated by the lanes of its atoms, its
-// energy counted once by the lane of its first atom).
+// launch, with 32-bit fixed-point copies of r (pair arithmetic) and the
+// neighbour-list bookkeeping; each lane owns sites s = tid + k NT, whose
+// velocities and accelerations it keeps in global memory.  Forces per owned
+// site: map (forcehermite, interpolate.F90:189-584), LJ against the sites of
+// the other molecules in its neighbour list (minimum image), and the bonded
+// terms that contain the site (each term is evaluated by the lanes of its
+// atoms, its energy counted once by the lane of its first atom).
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -35,13 +36,11 @@ namespace mh {
 constexpr int HS = MH_MAX_SITES;
 constexpr int HT = 16;
 constexpr int JLEV = 20;  // jump-ahead levels: up to 2^20 draws per move
-// neighbour lists: radius hicut + kSkin, rebuilt at the start of every move
-// and whenever a site has moved more than half the skin since the last build
-// (pairs left out then stay beyond hicut); entries j | b << 13 (partner site
-// j < 8192, its atom index b < 8)
+// neighbour lists: partner sites within hicut + kSkin, rebuilt whenever a
+// site has moved more than half the skin since the last build (pairs left out
+// then stay beyond hicut)
 constexpr float kSkin = 1.0f;
 constexpr float kHalfSkin2 = 0.49f * 0.49f;
-static_assert(HS <= 8, "list entries hold the partner's atom index in 3 bits");
 
 struct HDev {
     int ns, nst, nb, na, nt, nmd, prec, maxN;
@@ -54,6 +53,9 @@ struct HDev {
     float4 pp32[HS * HS];
     double pp64[HS * HS][4];
     float rl2[HS * HS];
+    // fixed-point cell positions: 2^32 / L per axis, and its inverse (A per unit)
+    double fxs[3];
+    float fxu[3];
     int bi[HT][2];
     double bk[HT], br0[HT];
     int ai[HT][3];
@@ -72,6 +74,7 @@ struct HRep {
     long long *att, *acc, *fail;
     double *rp, *r;             // [rep][maxN][ns][3]
     double *rp0, *r0;           // saved start configuration of the move
+    double *v, *a;              // [rep][maxN][ns][3] velocities, accelerations
     uint16_t *nbl;              // [rep][cap][cap] neighbour lists, entry-major
     int cap;                    // maxN * nsites
 };
@@ -125,38 +128,47 @@ __device__ double gauss_pair(const HDev &P, const Rng &S0, unsigned p, int secon
 }
 
 // forcehermite (interpolate.F90:189-584) at a cell position: p = V, -dV/dx,
-// -dV/dy, -dV/dz; false where a corner is untabulated
+// -dV/dy, -dV/dz; false where a corner is untabulated.  The eight corner
+// records are loaded before any arithmetic so their latencies overlap.
 template <int PREC>
 __device__ bool hermite_force(const MapRef &m, const double *r, double p[4]) {
     int i0[3], i1[3];
     double t[3];
     for (int d = 0; d < 3; d++)
         if (!cell_axis(r[d], m.ell[d], m.step[d], m.n[d], i0[d], i1[d], t[d])) return false;
-    double h[3][2][2], hd[3][2][2];
-    for (int d = 0; d < 3; d++) {
-        const double st = m.step[d], u = t[d], w = 1 - t[d];
-        h[d][0][0] = w * w * (3.0 - 2.0 * w);
-        h[d][1][0] = u * u * (3.0 - 2.0 * u);
-        h[d][0][1] = (-st) * w * w * (w - 1.0);
-        h[d][1][1] = st * u * u * (u - 1.0);
-        hd[d][0][0] = -(6.0 * w * (1.0 - w));
-        hd[d][1][0] = 6.0 * u * (1.0 - u);
-        hd[d][0][1] = -((-st) * w * (3.0 * w - 2.0));
-        hd[d][1][1] = st * u * (3.0 * u - 2.0);
-    }
-    double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
-    // two corners per pass: all eight in flight would hold 64 corner values
-    // and push the HMC kernel into scratch
-#pragma unroll 2
+    float4 rec[16];
+#pragma unroll
     for (int c = 0; c < 8; c++) {
         const int cx = (kCX >> c) & 1, cy = (kCY >> c) & 1, cz = (kCZ >> c) & 1;
         const long long node = ((long long)(cx ? i1[0] : i0[0]) * m.n[1] + (cy ? i1[1] : i0[1])) * m.n[2] +
                                (cz ? i1[2] : i0[2]);
-        const float4 *rec = reinterpret_cast<const float4 *>(m.dense + 8 * node);
-        const float4 a = rec[0], b = rec[1];
-        if (a.x != a.x) return false;
-        const float f[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        if (PREC == MH_PREC_FP64) {
+        const float4 *q = reinterpret_cast<const float4 *>(m.dense + 8 * node);
+        rec[2 * c] = q[0];
+        rec[2 * c + 1] = q[1];
+    }
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 8; c++) ok = ok && rec[2 * c].x == rec[2 * c].x;
+    if (!ok) return false;
+    if (PREC == MH_PREC_FP64) {
+        double h[3][2][2], hd[3][2][2];
+        for (int d = 0; d < 3; d++) {
+            const double st = m.step[d], u = t[d], w = 1 - t[d];
+            h[d][0][0] = w * w * (3.0 - 2.0 * w);
+            h[d][1][0] = u * u * (3.0 - 2.0 * u);
+            h[d][0][1] = (-st) * w * w * (w - 1.0);
+            h[d][1][1] = st * u * u * (u - 1.0);
+            hd[d][0][0] = -(6.0 * w * (1.0 - w));
+            hd[d][1][0] = 6.0 * u * (1.0 - u);
+            hd[d][0][1] = -((-st) * w * (3.0 * w - 2.0));
+            hd[d][1][1] = st * u * (3.0 * u - 2.0);
+        }
+        double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int cx = (kCX >> c) & 1, cy = (kCY >> c) & 1, cz = (kCZ >> c) & 1;
+            const float4 a = rec[2 * c], b = rec[2 * c + 1];
+            const float f[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 const int dx = (kCX >> k) & 1, dy = (kCY >> k) & 1, dz = (kCZ >> k) & 1;
@@ -167,14 +179,37 @@ __device__ bool hermite_force(const MapRef &m, const double *r, double p[4]) {
                 s3 = s3 + fv * hi * dj * hk;
                 s4 = s4 + fv * hi * hj * dk;
             }
-        } else {
+        }
+        p[0] = s1;
+        p[1] = -s2 / m.step[0];
+        p[2] = -s3 / m.step[1];
+        p[3] = -s4 / m.step[2];
+    } else {
 #pragma clang fp contract(fast)
+        float h[3][2][2], hd[3][2][2];
+        for (int d = 0; d < 3; d++) {
+            const float st = (float)m.step[d], u = (float)t[d], w = 1.0f - u;
+            h[d][0][0] = w * w * (3.0f - 2.0f * w);
+            h[d][1][0] = u * u * (3.0f - 2.0f * u);
+            h[d][0][1] = (-st) * w * w * (w - 1.0f);
+            h[d][1][1] = st * u * u * (u - 1.0f);
+            hd[d][0][0] = -(6.0f * w * (1.0f - w));
+            hd[d][1][0] = 6.0f * u * (1.0f - u);
+            hd[d][0][1] = -((-st) * w * (3.0f * w - 2.0f));
+            hd[d][1][1] = st * u * (3.0f * u - 2.0f);
+        }
+        double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int cx = (kCX >> c) & 1, cy = (kCY >> c) & 1, cz = (kCZ >> c) & 1;
+            const float4 a = rec[2 * c], b = rec[2 * c + 1];
+            const float f[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
             float q1 = 0.0f, q2 = 0.0f, q3 = 0.0f, q4 = 0.0f;
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 const int dx = (kCX >> k) & 1, dy = (kCY >> k) & 1, dz = (kCZ >> k) & 1;
-                const float hi = (float)h[0][cx][dx], hj = (float)h[1][cy][dy], hk = (float)h[2][cz][dz];
-                const float di = (float)hd[0][cx][dx], dj = (float)hd[1][cy][dy], dk = (float)hd[2][cz][dz];
+                const float hi = h[0][cx][dx], hj = h[1][cy][dy], hk = h[2][cz][dz];
+                const float di = hd[0][cx][dx], dj = hd[1][cy][dy], dk = hd[2][cz][dz];
                 q1 += f[k] * hi * hj * hk;
                 q2 += f[k] * di * hj * hk;
                 q3 += f[k] * hi * dj * hk;
@@ -185,11 +220,11 @@ __device__ bool hermite_force(const MapRef &m, const double *r, double p[4]) {
             s3 += q3;
             s4 += q4;
         }
+        p[0] = s1;
+        p[1] = -s2 / m.step[0];
+        p[2] = -s3 / m.step[1];
+        p[3] = -s4 / m.step[2];
     }
-    p[0] = s1;
-    p[1] = -s2 / m.step[0];
-    p[2] = -s3 / m.step[1];
-    p[3] = -s4 / m.step[2];
     return true;
 }
 
@@ -277,146 +312,204 @@ __device__ void bonded_site(const HDev &P, const double *R, int a, double f[3], 
     }
 }
 
-// minimum image of a float separation (single fold for |d| < 1.5 L)
-__device__ __forceinline__ float fold32(float d, float L, float iL) { return d - L * rintf(d * iL); }
+// fixed-point cell position of site s (x 2^32 / L, wrapped mod 2^32: the
+// difference of two such coordinates read as int32 is the minimum image) and
+// its atom index a
+__device__ __forceinline__ void put_fx(const HDev &P, uint4 *ru, const double *r, int s, int a) {
+    uint32_t u[3];
+    for (int d = 0; d < 3; d++) u[d] = (uint32_t)(unsigned long long)(long long)rint(r[3 * s + d] * P.fxs[d]);
+    ru[s] = make_uint4(u[0], u[1], u[2], (uint32_t)a);
+}
 
-// neighbour lists of the lane's sites from the positions r32: partner sites j
-// of the other molecules, ascending, within the list radius of the site pair;
-// the lane's build positions into rb
-template <int NT, int SPL>
-__device__ void hmc_lists(const HDev &P, const float4 *r32, int N, uint16_t *nbl, int cap, int (&cnt)[SPL],
-                          float (&rb)[SPL][3]) {
-    const int tid = threadIdx.x, ns = P.ns, nsite = N * ns;
-    const float Lx = (float)P.full[0], Ly = (float)P.full[1], Lz = (float)P.full[2];
-    const float iLx = 1.0f / Lx, iLy = 1.0f / Ly, iLz = 1.0f / Lz;
-#pragma unroll
-    for (int k = 0; k < SPL; k++) {
-        cnt[k] = 0;
-        const int s = tid + k * NT;
-        if (s >= nsite) continue;
-        const int m = s / ns, a = s - m * ns;
-        const float4 pi = r32[s];
-        rb[k][0] = pi.x;
-        rb[k][1] = pi.y;
-        rb[k][2] = pi.z;
-        uint16_t *lst = nbl + s;
+// true where one of the lane's sites has moved more than half the skin (in
+// principal coordinates, which are never wrapped) since its list was built
+template <int NT>
+__device__ __forceinline__ int moved_far(const double *rp, const float4 *nb, int nsite) {
+    int moved = 0;
+#pragma unroll 1
+    for (int s = threadIdx.x; s < nsite; s += NT) {
+        const float4 b = nb[s];
+        const float dx = (float)rp[3 * s] - b.x, dy = (float)rp[3 * s + 1] - b.y, dz = (float)rp[3 * s + 2] - b.z;
+        if (dx * dx + dy * dy + dz * dz > kHalfSkin2) moved = 1;
+    }
+    return moved;
+}
+
+// neighbour lists of the lane's sites: partner sites j of the other molecules,
+// ascending, within the list radius of the site pair, stored entry-major
+// (entry q of site s at q * cap + s); nb[s] = build position (principal) and
+// the entry count
+template <int NT>
+__device__ void hmc_lists(const HDev &P, const uint4 *ru, const double *rp, int N, uint16_t *nbl, int cap,
+                          float4 *nb) {
+    const int ns = P.ns, nsite = N * ns;
+    const float ux = P.fxu[0], uy = P.fxu[1], uz = P.fxu[2];
+#pragma unroll 1
+    for (int s = threadIdx.x; s < nsite; s += NT) {
+        const int m = s / ns, a = s - m * ns, j0 = m * ns;
+        const uint4 pi = ru[s];
         const float *rl2 = P.rl2 + a * HS;
-        int c = 0, mj = 0, b = 0;
-        for (int j = 0; j < nsite; j++) {
-            if (mj != m) {
-                const float4 pj = r32[j];
-                const float x = fold32(pi.x - pj.x, Lx, iLx), y = fold32(pi.y - pj.y, Ly, iLy),
-                            z = fold32(pi.z - pj.z, Lz, iLz);
-                if (x * x + y * y + z * z <= rl2[b]) {
-                    lst[(long long)c * cap] = (uint16_t)(j | (b << 13));
-                    c++;
-                }
-            }
-            if (++b == ns) b = 0, mj++;
-        }
-        cnt[k] = c;
-    }
-}
-
-// energy (block total) and forces on the lane's sites; sorbate-sorbate LJ
-// over the neighbour lists
-template <int NT, int PREC, int SPL>
-__device__ void hmc_forces(const HDev &P, const double *rp, const double *r, const float4 *r32, int N,
-                           const uint16_t *nbl, int cap, const int (&cnt)[SPL], double (&F)[SPL][3], double &U,
-                           int &fail, double *red) {
-    const int tid = threadIdx.x, ns = P.ns, nsite = N * ns;
-    double e = 0.0, bad = 0.0;
-#pragma unroll
-    for (int k = 0; k < SPL; k++) {
-        F[k][0] = F[k][1] = F[k][2] = 0.0;
-        const int s = tid + k * NT;
-        if (s >= nsite) continue;
-        const int m = s / ns, a = s - m * ns;
-        const double *ri = r + 3 * s;
-        double f[3] = {0.0, 0.0, 0.0};
-        if (P.has_map[a]) {
-            double p[4];
-            if (hermite_force<PREC>(P.map[a], ri, p)) {
-                e += p[0];
-                for (int d = 0; d < 3; d++) f[d] += p[1 + d];
-            } else {
-                bad = 1.0;
-            }
-        }
-        // sorbate-sorbate LJ against the listed partners in site order: the
-        // list holds every partner within hicut, so the sums are those of the
-        // loop over all sites of the other molecules that skips pairs beyond
-        // hicut and switched-off type pairs
-        const uint16_t *lst = nbl + s;
-        const int n = cnt[k];
-        double el = 0.0;
-        if (PREC == MH_PREC_FP64) {
-            for (int q = 0; q < n; q++) {
-                const int ent = lst[(long long)q * cap];
-                const int j = ent & 0x1fff;
-                const double *pr = P.pp64[a * HS + (ent >> 13)];
-                const double sx = fold64(ri[0] - r[3 * j], P.full[0]);
-                const double sy = fold64(ri[1] - r[3 * j + 1], P.full[1]);
-                const double sz = fold64(ri[2] - r[3 * j + 2], P.full[2]);
-                const double r2 = sx * sx + sy * sy + sz * sz;
-                if (r2 > pr[2]) continue;
-                if (r2 < pr[3]) bad = 1.0;
-                const double r2i = 1.0 / r2, r6i = r2i * r2i * r2i, r12i = r6i * r6i;
-                el += pr[0] * r12i - pr[1] * r6i;
-                const double fm = (12.0 * pr[0] * r12i - 6.0 * pr[1] * r6i) * r2i;
-                f[0] += sx * fm;
-                f[1] += sy * fm;
-                f[2] += sz * fm;
-            }
-        } else {
-#pragma clang fp contract(fast)
-            const float Lx = (float)P.full[0], Ly = (float)P.full[1], Lz = (float)P.full[2];
-            const float iLx = 1.0f / Lx, iLy = 1.0f / Ly, iLz = 1.0f / Lz;
-            const float4 pi = r32[s];
-            const float4 *pp = P.pp32 + a * HS;
-            float elf = 0.0f, fxf = 0.0f, fyf = 0.0f, fzf = 0.0f;
-            bool badf = false;
+        unsigned o = (unsigned)s;  // offset of entry c
+        int c = 0;
 #pragma unroll 4
-            for (int q = 0; q < n; q++) {
-                const int ent = lst[(long long)q * cap];
-                const float4 pj = r32[ent & 0x1fff];
-                const float4 pr = pp[ent >> 13];
-                const float x = fold32(pi.x - pj.x, Lx, iLx), y = fold32(pi.y - pj.y, Ly, iLy),
-                            z = fold32(pi.z - pj.z, Lz, iLz);
-                const float r2 = x * x + y * y + z * z;
-                const bool in = !(r2 > pr.z);
-                badf |= in && r2 < pr.w;
-                const float r2i = in ? rcp_fast(r2) : 0.0f, r6i = r2i * r2i * r2i, r12i = r6i * r6i;
-                elf += pr.x * r12i - pr.y * r6i;
-                const float fm = (12.0f * pr.x * r12i - 6.0f * pr.y * r6i) * r2i;
-                fxf += x * fm;
-                fyf += y * fm;
-                fzf += z * fm;
+        for (int j = 0; j < nsite; j++) {
+            const uint4 pj = ru[j];
+            const float x = (float)(int)(pi.x - pj.x) * ux, y = (float)(int)(pi.y - pj.y) * uy,
+                        z = (float)(int)(pi.z - pj.z) * uz;
+            if ((unsigned)(j - j0) >= (unsigned)ns && x * x + y * y + z * z <= rl2[pj.w]) {
+                nbl[o] = (uint16_t)j;
+                o += (unsigned)cap;
+                c++;
             }
-            if (badf) bad = 1.0;
-            el = elf;
-            f[0] += fxf;
-            f[1] += fyf;
-            f[2] += fzf;
         }
-        e += 0.5 * el;  // each pair is seen from both of its sites
-        bonded_site(P, rp + 3 * (m * ns), a, f, e);
-        F[k][0] = f[0];
-        F[k][1] = f[1];
-        F[k][2] = f[2];
+        nb[s] = make_float4((float)rp[3 * s], (float)rp[3 * s + 1], (float)rp[3 * s + 2], __int_as_float(c));
     }
-    double v[2] = {e, bad};
-    block_sum<NT, 2>(v, red);
-    U = v[0];
-    fail = v[1] != 0.0;
 }
 
-// float copy of site s's cell position (pair and list arithmetic)
-__device__ __forceinline__ void put32(float4 *r32, const double *r, int s) {
-    r32[s] = make_float4((float)r[3 * s], (float)r[3 * s + 1], (float)r[3 * s + 2], 0.0f);
+// f(j) for the entries of a neighbour list in order; the next group of eight
+// entries is loaded while a group is processed
+template <class Fn>
+__device__ __forceinline__ void list_loop(const uint16_t *nbl, unsigned s, int n, unsigned cap, Fn &&f) {
+    int q = 0;
+    unsigned o = s;  // offset of entry q
+    if (n >= 8) {
+        int cur[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) cur[u] = nbl[o + u * cap];
+        for (; q + 8 <= n; q += 8) {
+            int nxt[8];
+            const bool more = q + 16 <= n;
+            o += 8 * cap;
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) nxt[u] = nbl[o + u * cap];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) f(cur[u]);
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) cur[u] = nxt[u];
+            }
+        }
+    }
+    for (; q < n; q++, o += cap) f(nbl[o]);
 }
 
-template <int NT, int PREC, int SPL>
+// force on site s (map, sorbate-sorbate LJ over its neighbour list, bonded
+// terms) into f; its energy share into e; bad on a failed evaluation
+template <int PREC>
+__device__ __forceinline__ void site_force(const HDev &P, const double *rp, const double *r, const uint4 *ru,
+                                           const uint16_t *nbl, int cap, int s, int n, double f[3], double &e,
+                                           double &bad) {
+    const int ns = P.ns, m = s / ns, a = s - m * ns;
+    const double *ri = r + 3 * s;
+    f[0] = f[1] = f[2] = 0.0;
+    if (P.has_map[a]) {
+        double p[4];
+        if (hermite_force<PREC>(P.map[a], ri, p)) {
+            e += p[0];
+            for (int d = 0; d < 3; d++) f[d] += p[1 + d];
+        } else {
+            bad = 1.0;
+        }
+    }
+    // sorbate-sorbate LJ against the listed partners in site order: the list
+    // holds every partner within hicut, so the sums are those of the loop over
+    // all sites of the other molecules that skips pairs beyond hicut and
+    // switched-off type pairs
+    double el = 0.0;
+    if (PREC == MH_PREC_FP64) {
+        const double *pp = P.pp64[a * HS];
+        list_loop(nbl, (unsigned)s, n, (unsigned)cap, [&](int j) {
+            const double *pr = pp + 4 * ru[j].w;
+            const double sx = fold64(ri[0] - r[3 * j], P.full[0]);
+            const double sy = fold64(ri[1] - r[3 * j + 1], P.full[1]);
+            const double sz = fold64(ri[2] - r[3 * j + 2], P.full[2]);
+            const double r2 = sx * sx + sy * sy + sz * sz;
+            if (r2 > pr[2]) return;
+            if (r2 < pr[3]) bad = 1.0;
+            const double r2i = 1.0 / r2, r6i = r2i * r2i * r2i, r12i = r6i * r6i;
+            el += pr[0] * r12i - pr[1] * r6i;
+            const double fm = (12.0 * pr[0] * r12i - 6.0 * pr[1] * r6i) * r2i;
+            f[0] += sx * fm;
+            f[1] += sy * fm;
+            f[2] += sz * fm;
+        });
+    } else {
+#pragma clang fp contract(fast)
+        const float ux = P.fxu[0], uy = P.fxu[1], uz = P.fxu[2];
+        const uint4 pi = ru[s];
+        const float4 *pp = P.pp32 + a * HS;
+        float elf = 0.0f, fxf = 0.0f, fyf = 0.0f, fzf = 0.0f;
+        bool badf = false;
+        list_loop(nbl, (unsigned)s, n, (unsigned)cap, [&](int j) {
+            const uint4 pj = ru[j];
+            const float x = (float)(int)(pi.x - pj.x) * ux, y = (float)(int)(pi.y - pj.y) * uy,
+                        z = (float)(int)(pi.z - pj.z) * uz;
+            const float r2 = x * x + y * y + z * z;
+            const float4 pr = pp[pj.w];
+            const bool in = !(r2 > pr.z);
+            badf |= in && r2 < pr.w;
+            // A/r^12 - B/r^6 and its force factor (12 A/r^12 - 6 B/r^6)/r^2
+            const float r2i = in ? rcp_fast(r2) : 0.0f, r6i = r2i * r2i * r2i;
+            const float ta = pr.x * r6i * r6i, tb = pr.y * r6i, de = ta - tb;
+            elf += de;
+            const float fm = (de + ta) * (6.0f * r2i);
+            fxf += x * fm;
+            fyf += y * fm;
+            fzf += z * fm;
+        });
+        if (badf) bad = 1.0;
+        el = elf;
+        f[0] += fxf;
+        f[1] += fyf;
+        f[2] += fzf;
+    }
+    e += 0.5 * el;  // each pair is seen from both of its sites
+    bonded_site(P, rp + 3 * (m * ns), a, f, e);
+}
+
+// energy (block total) and forces of the configuration.  With Fo the forces
+// are stored there; otherwise each site's acceleration goes to ga and, after
+// a velocity-Verlet step (step > 0), its velocity gv takes the second half
+// kick.  ke = block total of m v^2 with the velocities after this call.
+// Accelerations and velocities written for an evaluation that fails are not
+// used (the move ends there).
+template <int NT, int PREC>
+__device__ void hmc_forces(const HDev &P, const double *rp, const double *r, const uint4 *ru, const float4 *nb,
+                           int N, const uint16_t *nbl, int cap, double *gv, double *ga, double *Fo, int step,
+                           double &U, int &fail, double &ke, double *red) {
+    const int ns = P.ns, nsite = N * ns;
+    double e = 0.0, bad = 0.0, k2 = 0.0;
+#pragma unroll 1
+    for (int s = threadIdx.x; s < nsite; s += NT) {
+        double f[3];
+        site_force<PREC>(P, rp, r, ru, nbl, cap, s, __float_as_int(nb[s].w), f, e, bad);
+        if (Fo) {
+            for (int i = 0; i < 3; i++) Fo[3 * s + i] = f[i];
+            continue;
+        }
+        const int at = s - (s / ns) * ns;
+        double v[3];
+        for (int i = 0; i < 3; i++) {
+            const double acc = f[i] * (1.0 / P.mass[at]) * kScaleF;
+            ga[3 * s + i] = acc;
+            v[i] = gv[3 * s + i];
+            if (step > 0) {
+                v[i] = v[i] + acc * 0.5 * P.dt;
+                gv[3 * s + i] = v[i];
+            }
+        }
+        k2 = k2 + (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * P.mass[at];
+    }
+    double w[3] = {e, bad, k2};
+    block_sum<NT, 3>(w, red);
+    U = w[0];
+    fail = w[1] != 0.0;
+    ke = w[2];
+}
+
+template <int NT, int PREC>
 __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R, int nmoves) {
     extern __shared__ double dyn[];
     __shared__ HDev P;
@@ -428,37 +521,34 @@ __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R,
     __syncthreads();
     if (R.err[rep]) return;
     const int ns = P.ns, N = R.N[rep], nsite = N * ns;
-    const long long off = (long long)rep * P.maxN * ns * 3;
-    double *rp = dyn, *r = dyn + (long long)P.maxN * ns * 3;
-    float4 *r32 = reinterpret_cast<float4 *>(dyn + (long long)P.maxN * ns * 6);
+    const long long off = (long long)rep * R.cap * 3;
+    double *rp = dyn, *r = dyn + (long long)R.cap * 3;
+    uint4 *ru = reinterpret_cast<uint4 *>(dyn + (long long)R.cap * 6);
+    float4 *nb = reinterpret_cast<float4 *>(ru + R.cap);
     uint16_t *nbl = R.nbl + (long long)rep * R.cap * R.cap;
+    double *gv = R.v + off, *ga = R.a + off;
     for (int i = tid; i < nsite * 3; i += NT) {
         rp[i] = R.rp[off + i];
         r[i] = R.r[off + i];
     }
-    for (int i = tid; i < nsite; i += NT)
-        r32[i] = make_float4((float)R.r[off + 3 * i], (float)R.r[off + 3 * i + 1], (float)R.r[off + 3 * i + 2], 0.0f);
+    for (int i = tid; i < nsite; i += NT) put_fx(P, ru, R.r + off, i, i % ns);
     Rng rng{R.ix[rep], R.iy[rep], R.first[rep], R.iseed[rep]};
     rng_first(rng);
     int ghave = R.ghave[rep];
     double gz2 = R.gz2[rep], energy = R.energy[rep], lastdH = R.dH[rep];
     long long att = R.att[rep], acc = R.acc[rep], nfail = R.fail[rep];
     int err = 0;
-    double v[SPL][3], a[SPL][3], F[SPL][3];
-    float rb[SPL][3];
-    int cnt[SPL];
     __syncthreads();
+    hmc_lists<NT>(P, ru, rp, N, nbl, R.cap, nb);
     const double dt = P.dt, dt2 = dt * dt * 0.5;
     for (int mv = 0; mv < nmoves; mv++) {
         att++;
-        // ---- Gaussian velocities (hybridmoves_boltzVel)
+        // ---- Gaussian velocities (hybridmoves_boltzVel); start configuration saved
         const unsigned G = 3u * nsite, c = ghave ? 1u : 0u;
         const unsigned used = G >= c ? G - c : 0u;  // draws-consuming Gaussians
         const unsigned npairs = (used + 1u) / 2u;
-#pragma unroll
-        for (int k = 0; k < SPL; k++) {
-            const int s = tid + k * NT;
-            if (s >= nsite) continue;
+#pragma unroll 1
+        for (int s = tid; s < nsite; s += NT) {
             const int at = s - (s / ns) * ns;
             for (int i = 0; i < 3; i++) {
                 const unsigned g = 3u * s + i;
@@ -469,7 +559,9 @@ __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R,
                     z = gauss_pair(P, rng, gg >> 1, gg & 1);
                     if ((used & 1u) && gg == used - 1u) s_z2 = gauss_pair(P, rng, gg >> 1, 1);
                 }
-                v[k][i] = 0.0 + P.sigma[at] * z;
+                gv[3 * s + i] = 0.0 + P.sigma[at] * z;
+                R.rp0[off + 3 * s + i] = rp[3 * s + i];
+                R.r0[off + 3 * s + i] = r[3 * s + i];
             }
         }
         if (G < c) {  // no atoms: the saved value stays pending
@@ -477,87 +569,44 @@ __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R,
             ghave = (used & 1u) ? 1 : 0;
         }
         rng_jump(P, rng, 2u * npairs);
-        // ---- start configuration: save, energy, forces, kinetic energy
-#pragma unroll
-        for (int k = 0; k < SPL; k++) {
-            const int s = tid + k * NT;
-            if (s >= nsite) continue;
-            for (int i = 0; i < 3; i++) {
-                R.rp0[off + 3 * s + i] = rp[3 * s + i];
-                R.r0[off + 3 * s + i] = r[3 * s + i];
-            }
-        }
-        hmc_lists<NT, SPL>(P, r32, N, nbl, R.cap, cnt, rb);
-        double U0;
-        int f0;
-        hmc_forces<NT, PREC, SPL>(P, rp, r, r32, N, nbl, R.cap, cnt, F, U0, f0, red);  // s_z2 visible after
-        if (ghave && (used & 1u)) gz2 = s_z2;
-        if (f0 || U0 > P.max_nrg0) {
-            err = 1;  // the reference stops here (hybridmoves.F90:224-229)
-            break;
-        }
-        double ke = 0.0;
-#pragma unroll
-        for (int k = 0; k < SPL; k++) {
-            const int s = tid + k * NT;
-            if (s >= nsite) continue;
-            const int at = s - (s / ns) * ns;
-            ke = ke + (v[k][0] * v[k][0] + v[k][1] * v[k][1] + v[k][2] * v[k][2]) * P.mass[at];
-            for (int i = 0; i < 3; i++) a[k][i] = F[k][i] * (1.0 / P.mass[at]) * kScaleF;
-        }
-        double kv[1] = {ke};
-        block_sum<NT, 1>(kv, red);
-        const double ke0 = 0.5 * kv[0] * kScaleKE;
-        // ---- NVE velocity Verlet
+        // ---- energy and forces at the start (step 0) and after each NVE
+        // velocity-Verlet step (velverlet_nveIntegrate)
         bool good = true;
-        double U1 = U0;
-        for (int step = 0; step < P.nmd; step++) {
-            int moved = 0;
-#pragma unroll
-            for (int k = 0; k < SPL; k++) {
-                const int s = tid + k * NT;
-                if (s >= nsite) continue;
-                for (int i = 0; i < 3; i++) {
-                    rp[3 * s + i] = rp[3 * s + i] + v[k][i] * dt + a[k][i] * dt2;
-                    r[3 * s + i] = r[3 * s + i] + v[k][i] * dt + a[k][i] * dt2;
-                    v[k][i] = v[k][i] + a[k][i] * 0.5 * dt;
+        double U0 = 0.0, U1 = 0.0, ke0 = 0.0, ke1 = 0.0;
+        for (int step = 0; step <= P.nmd; step++) {
+            if (step > 0) {
+#pragma unroll 1
+                for (int s = tid; s < nsite; s += NT) {
+                    for (int i = 0; i < 3; i++) {
+                        const double vi = gv[3 * s + i], ai = ga[3 * s + i];
+                        rp[3 * s + i] = rp[3 * s + i] + vi * dt + ai * dt2;
+                        r[3 * s + i] = r[3 * s + i] + vi * dt + ai * dt2;
+                        gv[3 * s + i] = vi + ai * 0.5 * dt;
+                    }
+                    put_fx(P, ru, r, s, s - (s / ns) * ns);
                 }
-                put32(r32, r, s);
-                const float dx = (float)r[3 * s] - rb[k][0], dy = (float)r[3 * s + 1] - rb[k][1],
-                            dz = (float)r[3 * s + 2] - rb[k][2];
-                if (dx * dx + dy * dy + dz * dz > kHalfSkin2) moved = 1;
             }
-            if (__syncthreads_or(moved)) hmc_lists<NT, SPL>(P, r32, N, nbl, R.cap, cnt, rb);
+            if (__syncthreads_or(moved_far<NT>(rp, nb, nsite))) hmc_lists<NT>(P, ru, rp, N, nbl, R.cap, nb);
             int fl;
-            hmc_forces<NT, PREC, SPL>(P, rp, r, r32, N, nbl, R.cap, cnt, F, U1, fl, red);
-            if (fl || U1 > P.max_nrg_md) {
+            double k2;
+            hmc_forces<NT, PREC>(P, rp, r, ru, nb, N, nbl, R.cap, gv, ga, nullptr, step, U1, fl, k2, red);
+            if (step == 0) {
+                if (ghave && (used & 1u)) gz2 = s_z2;  // written before the block sums
+                if (fl || U1 > P.max_nrg0) {
+                    err = 1;  // the reference stops here (hybridmoves.F90:224-229)
+                    break;
+                }
+                U0 = U1;
+                ke0 = 0.5 * k2 * kScaleKE;
+            } else if (fl || U1 > P.max_nrg_md) {
                 good = false;
                 break;
             }
-#pragma unroll
-            for (int k = 0; k < SPL; k++) {
-                const int s = tid + k * NT;
-                if (s >= nsite) continue;
-                const int at = s - (s / ns) * ns;
-                for (int i = 0; i < 3; i++) {
-                    a[k][i] = F[k][i] * (1.0 / P.mass[at]) * kScaleF;
-                    v[k][i] = v[k][i] + a[k][i] * 0.5 * dt;
-                }
-            }
+            ke1 = 0.5 * k2 * kScaleKE;
         }
+        if (err) break;
         bool accepted = false;
         if (good) {
-            ke = 0.0;
-#pragma unroll
-            for (int k = 0; k < SPL; k++) {
-                const int s = tid + k * NT;
-                if (s >= nsite) continue;
-                const int at = s - (s / ns) * ns;
-                ke = ke + (v[k][0] * v[k][0] + v[k][1] * v[k][1] + v[k][2] * v[k][2]) * P.mass[at];
-            }
-            kv[0] = ke;
-            block_sum<NT, 1>(kv, red);
-            const double ke1 = 0.5 * kv[0] * kScaleKE;
             const double dH = (U1 - U0) + (ke1 - ke0) / 4.184;
             lastdH = dH;
             double u = -P.rti * dH;
@@ -572,24 +621,20 @@ __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R,
         if (accepted) {
             acc++;
             energy = U1;
-#pragma unroll
-            for (int k = 0; k < SPL; k++) {
-                const int s = tid + k * NT;
-                if (s >= nsite) continue;
+#pragma unroll 1
+            for (int s = tid; s < nsite; s += NT) {
                 for (int i = 0; i < 3; i++) r[3 * s + i] = wrap_full(r[3 * s + i], P.full[i]);
-                put32(r32, r, s);
+                put_fx(P, ru, r, s, s - (s / ns) * ns);
             }
         } else {
             energy = U0;
-#pragma unroll
-            for (int k = 0; k < SPL; k++) {
-                const int s = tid + k * NT;
-                if (s >= nsite) continue;
+#pragma unroll 1
+            for (int s = tid; s < nsite; s += NT) {
                 for (int i = 0; i < 3; i++) {
                     rp[3 * s + i] = R.rp0[off + 3 * s + i];
                     r[3 * s + i] = R.r0[off + 3 * s + i];
                 }
-                put32(r32, r, s);
+                put_fx(P, ru, r, s, s - (s / ns) * ns);
             }
         }
         __syncthreads();
@@ -615,7 +660,7 @@ __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R,
 }
 
 // energy and forces of one replica's configuration (tests, checks)
-template <int NT, int PREC, int SPL>
+template <int NT, int PREC>
 __global__ void __launch_bounds__(NT) k_hmc_forces(const HDev *__restrict__ Pg, HRep R, int rep, double *F_out,
                                                    double *e_out) {
     extern __shared__ double dyn[];
@@ -626,28 +671,21 @@ __global__ void __launch_bounds__(NT) k_hmc_forces(const HDev *__restrict__ Pg, 
         reinterpret_cast<int *>(&P)[i] = reinterpret_cast<const int *>(Pg)[i];
     __syncthreads();
     const int ns = P.ns, N = R.N[rep], nsite = N * ns;
-    const long long off = (long long)rep * P.maxN * ns * 3;
-    double *rp = dyn, *r = dyn + (long long)P.maxN * ns * 3;
-    float4 *r32 = reinterpret_cast<float4 *>(dyn + (long long)P.maxN * ns * 6);
+    const long long off = (long long)rep * R.cap * 3;
+    double *rp = dyn, *r = dyn + (long long)R.cap * 3;
+    uint4 *ru = reinterpret_cast<uint4 *>(dyn + (long long)R.cap * 6);
+    float4 *nb = reinterpret_cast<float4 *>(ru + R.cap);
     uint16_t *nbl = R.nbl + (long long)rep * R.cap * R.cap;
     for (int i = tid; i < nsite * 3; i += NT) {
         rp[i] = R.rp[off + i];
         r[i] = R.r[off + i];
     }
-    for (int i = tid; i < nsite; i += NT)
-        r32[i] = make_float4((float)R.r[off + 3 * i], (float)R.r[off + 3 * i + 1], (float)R.r[off + 3 * i + 2], 0.0f);
+    for (int i = tid; i < nsite; i += NT) put_fx(P, ru, R.r + off, i, i % ns);
     __syncthreads();
-    double F[SPL][3], U;
-    float rb[SPL][3];
-    int fl, cnt[SPL];
-    hmc_lists<NT, SPL>(P, r32, N, nbl, R.cap, cnt, rb);
-    hmc_forces<NT, PREC, SPL>(P, rp, r, r32, N, nbl, R.cap, cnt, F, U, fl, red);
-#pragma unroll
-    for (int k = 0; k < SPL; k++) {
-        const int s = tid + k * NT;
-        if (s < nsite)
-            for (int i = 0; i < 3; i++) F_out[3 * s + i] = F[k][i];
-    }
+    hmc_lists<NT>(P, ru, rp, N, nbl, R.cap, nb);
+    double U, k2;
+    int fl;
+    hmc_forces<NT, PREC>(P, rp, r, ru, nb, N, nbl, R.cap, nullptr, nullptr, F_out, 0, U, fl, k2, red);
     if (tid == 0) {
         e_out[0] = U;
         e_out[1] = fl ? 0.0 : 1.0;
@@ -660,11 +698,11 @@ using namespace mh;
 
 struct mh_hmc {
     mh_ctx *ctx = nullptr;
-    int nrep = 0, maxN = 0, ns = 0, prec = 0, threads = 512, spl = 1;
+    int nrep = 0, maxN = 0, ns = 0, prec = 0, threads = 512;
     size_t lds = 0;
     HDev host;
     DBuf dP;
-    DBuf N, err, ix, iy, first, iseed, ghave, gz2, energy, dH, att, acc, fail, rp, r, rp0, r0, nbl;
+    DBuf N, err, ix, iy, first, iseed, ghave, gz2, energy, dH, att, acc, fail, rp, r, rp0, r0, v, a, nbl;
     HRep rep() {
         HRep h;
         h.N = N.as<int>();
@@ -684,6 +722,8 @@ struct mh_hmc {
         h.r = r.as<double>();
         h.rp0 = rp0.as<double>();
         h.r0 = r0.as<double>();
+        h.v = v.as<double>();
+        h.a = a.as<double>();
         h.nbl = nbl.as<uint16_t>();
         h.cap = maxN * ns;
         return h;
@@ -704,31 +744,12 @@ static uint32_t gf2_apply_host(const uint32_t *col, uint32_t x) {
     return y;
 }
 
-// sites per lane for the persistent kernel
-static int pick_spl(int sites, int threads) {
-    const int need = (sites + threads - 1) / threads;
-    for (int k : {1, 2, 4, 8, 16})
-        if (need <= k) return k;
-    return 0;
-}
-
 template <int NT, int PREC>
 static int launch_hmc_nt(mh_hmc *h, int nmoves) {
-    const HDev *P = h->dP.as<HDev>();
-    HRep R = h->rep();
-    auto go = [&](auto kern) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)h->lds);
-        hipLaunchKernelGGL(kern, dim3(h->nrep), dim3(NT), h->lds, h->ctx->stream, P, R, nmoves);
-    };
-    switch (h->spl) {
-    case 1: go(k_hmc<NT, PREC, 1>); break;
-    case 2: go(k_hmc<NT, PREC, 2>); break;
-    case 4: go(k_hmc<NT, PREC, 4>); break;
-    case 8: go(k_hmc<NT, PREC, 8>); break;
-    case 16: go(k_hmc<NT, PREC, 16>); break;
-    default: return MH_ECAPACITY;
-    }
+    auto kern = k_hmc<NT, PREC>;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)h->lds);
+    hipLaunchKernelGGL(kern, dim3(h->nrep), dim3(NT), h->lds, h->ctx->stream, h->dP.as<HDev>(), h->rep(), nmoves);
     return MH_OK;
 }
 
@@ -736,18 +757,20 @@ template <int PREC>
 static int launch_hmc(mh_hmc *h, int nmoves) {
     switch (h->threads) {
     case 128: return launch_hmc_nt<128, PREC>(h, nmoves);
-    case 512: return launch_hmc_nt<512, PREC>(h, nmoves);
-    default: return launch_hmc_nt<256, PREC>(h, nmoves);
+    case 256: return launch_hmc_nt<256, PREC>(h, nmoves);
+    case 1024: return launch_hmc_nt<1024, PREC>(h, nmoves);
+    default: return launch_hmc_nt<512, PREC>(h, nmoves);
     }
 }
 
+// LDS per site: rp and r (FP64), fixed-point position, list bookkeeping
+constexpr size_t kLdsPerSite = 6 * sizeof(double) + sizeof(uint4) + sizeof(float4);
+
 static int plan_hmc(mh_hmc *h) {
-    h->spl = pick_spl(h->maxN * h->ns, h->threads);
-    MH_CHECK(h->spl > 0, MH_ECAPACITY, "mh_hmc: maxN * nsites exceeds 16 sites per lane at this workgroup size");
     int dev_max = 65536;
     (void)hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->ctx->device);
-    h->lds = (size_t)h->maxN * h->ns * (6 * sizeof(double) + sizeof(float4));
-    MH_CHECK((long long)h->lds + (long long)sizeof(HDev) + 4096 <= dev_max, MH_ECAPACITY,
+    h->lds = (size_t)h->maxN * h->ns * kLdsPerSite;
+    MH_CHECK((long long)h->lds + (long long)sizeof(HDev) + 8192 <= dev_max, MH_ECAPACITY,
              "mh_hmc: maxN * nsites too large for LDS (positions of a replica are held in LDS)");
     return MH_OK;
 }
@@ -841,6 +864,8 @@ int mh_hmc_create(mh_ctx *ctx, const mh_hmc_params *p, int nrep, const int32_t *
     for (int i = 0; i < 3; i++) {
         H.full[i] = ctx->full_ell[i];
         H.fund[i] = ctx->fund_ell[i];
+        H.fxs[i] = 4294967296.0 / ctx->full_ell[i];
+        H.fxu[i] = (float)(ctx->full_ell[i] / 4294967296.0);
     }
     H.T = p->temperature;
     H.dt = p->dt_ps;
@@ -863,7 +888,8 @@ int mh_hmc_create(mh_ctx *ctx, const mh_hmc_params *p, int nrep, const int32_t *
         (rc = h->ghave.alloc(4 * R)) || (rc = h->gz2.alloc(8 * R)) || (rc = h->energy.alloc(8 * R)) ||
         (rc = h->dH.alloc(8 * R)) || (rc = h->att.alloc(8 * R)) || (rc = h->acc.alloc(8 * R)) ||
         (rc = h->fail.alloc(8 * R)) || (rc = h->rp.alloc(8 * nc)) || (rc = h->r.alloc(8 * nc)) ||
-        (rc = h->rp0.alloc(8 * nc)) || (rc = h->r0.alloc(8 * nc)) ||
+        (rc = h->rp0.alloc(8 * nc)) || (rc = h->r0.alloc(8 * nc)) || (rc = h->v.alloc(8 * nc)) ||
+        (rc = h->a.alloc(8 * nc)) ||
         (rc = h->nbl.alloc(2 * R * (size_t)maxN * h->ns * maxN * h->ns)))
         return rc;
     std::vector<int32_t> zi(R, 0), ones(R, 1), sd(seeds, seeds + R);
@@ -888,8 +914,8 @@ int mh_hmc_destroy(mh_hmc *h) {
 }
 
 int mh_hmc_set_threads(mh_hmc *h, int threads) {
-    MH_CHECK(h && (threads == 128 || threads == 256 || threads == 512), MH_EINVAL,
-             "mh_hmc_set_threads: threads must be 128, 256 or 512");
+    MH_CHECK(h && (threads == 128 || threads == 256 || threads == 512 || threads == 1024), MH_EINVAL,
+             "mh_hmc_set_threads: threads must be 128, 256, 512 or 1024");
     const int old = h->threads;
     h->threads = threads;
     int rc = plan_hmc(h);
@@ -974,28 +1000,14 @@ int mh_hmc_energy_forces(mh_hmc *h, int rep, double *e, double *F, int *ok) {
     if ((rc = dF.alloc(8 * (nc + 1))) || (rc = de.alloc(16))) return rc;
     HRep R = h->rep();
     const HDev *P = h->dP.as<HDev>();
-    const int NT = 256;
-    const int spl = pick_spl(h->maxN * h->ns, NT);
     auto go = [&](auto kern) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)h->lds);
-        hipLaunchKernelGGL(kern, dim3(1), dim3(NT), h->lds, h->ctx->stream, P, R, rep, dF.as<double>(),
+        hipLaunchKernelGGL(kern, dim3(1), dim3(256), h->lds, h->ctx->stream, P, R, rep, dF.as<double>(),
                            de.as<double>());
     };
-#define MH_HMC_F(S)                                                                                   \
-    case S:                                                                                          \
-        if (h->prec == MH_PREC_FP32) go(k_hmc_forces<NT, MH_PREC_FP32, S>);                          \
-        else go(k_hmc_forces<NT, MH_PREC_FP64, S>);                                                  \
-        break;
-    switch (spl) {
-        MH_HMC_F(1)
-        MH_HMC_F(2)
-        MH_HMC_F(4)
-        MH_HMC_F(8)
-        MH_HMC_F(16)
-    default: set_error("mh_hmc_energy_forces: too many sites"); return MH_ECAPACITY;
-    }
-#undef MH_HMC_F
+    if (h->prec == MH_PREC_FP32) go(k_hmc_forces<256, MH_PREC_FP32>);
+    else go(k_hmc_forces<256, MH_PREC_FP64>);
     MH_HIP(hipGetLastError());
     MH_HIP(hipStreamSynchronize(h->ctx->stream));
     double hv[2];

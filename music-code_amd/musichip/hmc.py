@@ -128,7 +128,7 @@ class HmcBatch:
         H.check(_sig().mh_hmc_set_config(self.h, rep, len(rp), _p(rp)), "mh_hmc_set_config")
 
     def set_threads(self, n):
-        """Workgroup size per replica (128, 256 or 512)."""
+        """Workgroup size per replica (128, 256, 512 or 1024)."""
         H.check(_sig().mh_hmc_set_threads(self.h, int(n)), "mh_hmc_set_threads")
 
     def run(self, nmoves):
