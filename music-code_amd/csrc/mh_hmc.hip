@@ -127,33 +127,48 @@ __device__ double gauss_pair(const HDev &P, const Rng &S0, unsigned p, int secon
     return second ? rad * sin(twopi * r2) : rad * cos(twopi * r2);
 }
 
-// forcehermite (interpolate.F90:189-584) at a cell position: p = V, -dV/dx,
-// -dV/dy, -dV/dz; false where a corner is untabulated.  The eight corner
-// records are loaded before any arithmetic so their latencies overlap.
-template <int PREC>
-__device__ bool hermite_force(const MapRef &m, const double *r, double p[4]) {
-    int i0[3], i1[3];
-    double t[3];
-    for (int d = 0; d < 3; d++)
-        if (!cell_axis(r[d], m.ell[d], m.step[d], m.n[d], i0[d], i1[d], t[d])) return false;
+// forcehermite (interpolate.F90:189-584) in two parts, so that other work
+// can run while the corner records are in flight.  hermite_fetch: the cell of
+// a position (false outside the tabulated range) and the loads of its eight
+// corner records (global address space, so waits on LDS reads do not wait on
+// them).
+struct Corners {
     float4 rec[16];
+    double t[3];
+};
+
+typedef float F4v __attribute__((ext_vector_type(4)));
+typedef const F4v __attribute__((address_space(1))) *GlobalF4;
+
+__device__ __forceinline__ bool hermite_fetch(const MapRef &m, const double *r, Corners &g) {
+    int i0[3], i1[3];
+    for (int d = 0; d < 3; d++)
+        if (!cell_axis(r[d], m.ell[d], m.step[d], m.n[d], i0[d], i1[d], g.t[d])) return false;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const int cx = (kCX >> c) & 1, cy = (kCY >> c) & 1, cz = (kCZ >> c) & 1;
         const long long node = ((long long)(cx ? i1[0] : i0[0]) * m.n[1] + (cy ? i1[1] : i0[1])) * m.n[2] +
                                (cz ? i1[2] : i0[2]);
-        const float4 *q = reinterpret_cast<const float4 *>(m.dense + 8 * node);
-        rec[2 * c] = q[0];
-        rec[2 * c + 1] = q[1];
+        const GlobalF4 q = (GlobalF4)(m.dense + 8 * node);
+        const F4v lo = q[0], hi = q[1];
+        g.rec[2 * c] = make_float4(lo.x, lo.y, lo.z, lo.w);
+        g.rec[2 * c + 1] = make_float4(hi.x, hi.y, hi.z, hi.w);
     }
+    return true;
+}
+
+// hermite_eval: p = V, -dV/dx, -dV/dy, -dV/dz from the fetched corners;
+// false where a corner is untabulated
+template <int PREC>
+__device__ __forceinline__ bool hermite_eval(const MapRef &m, const Corners &g, double p[4]) {
     bool ok = true;
 #pragma unroll
-    for (int c = 0; c < 8; c++) ok = ok && rec[2 * c].x == rec[2 * c].x;
+    for (int c = 0; c < 8; c++) ok = ok && g.rec[2 * c].x == g.rec[2 * c].x;
     if (!ok) return false;
     if (PREC == MH_PREC_FP64) {
         double h[3][2][2], hd[3][2][2];
         for (int d = 0; d < 3; d++) {
-            const double st = m.step[d], u = t[d], w = 1 - t[d];
+            const double st = m.step[d], u = g.t[d], w = 1 - g.t[d];
             h[d][0][0] = w * w * (3.0 - 2.0 * w);
             h[d][1][0] = u * u * (3.0 - 2.0 * u);
             h[d][0][1] = (-st) * w * w * (w - 1.0);
@@ -167,7 +182,7 @@ __device__ bool hermite_force(const MapRef &m, const double *r, double p[4]) {
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const int cx = (kCX >> c) & 1, cy = (kCY >> c) & 1, cz = (kCZ >> c) & 1;
-            const float4 a = rec[2 * c], b = rec[2 * c + 1];
+            const float4 a = g.rec[2 * c], b = g.rec[2 * c + 1];
             const float f[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
             for (int k = 0; k < 8; k++) {
@@ -188,7 +203,7 @@ __device__ bool hermite_force(const MapRef &m, const double *r, double p[4]) {
 #pragma clang fp contract(fast)
         float h[3][2][2], hd[3][2][2];
         for (int d = 0; d < 3; d++) {
-            const float st = (float)m.step[d], u = (float)t[d], w = 1.0f - u;
+            const float st = (float)m.step[d], u = (float)g.t[d], w = 1.0f - u;
             h[d][0][0] = w * w * (3.0f - 2.0f * w);
             h[d][1][0] = u * u * (3.0f - 2.0f * u);
             h[d][0][1] = (-st) * w * w * (w - 1.0f);
@@ -202,7 +217,7 @@ __device__ bool hermite_force(const MapRef &m, const double *r, double p[4]) {
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const int cx = (kCX >> c) & 1, cy = (kCY >> c) & 1, cz = (kCZ >> c) & 1;
-            const float4 a = rec[2 * c], b = rec[2 * c + 1];
+            const float4 a = g.rec[2 * c], b = g.rec[2 * c + 1];
             const float f[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
             float q1 = 0.0f, q2 = 0.0f, q3 = 0.0f, q4 = 0.0f;
 #pragma unroll
@@ -404,9 +419,17 @@ __device__ __forceinline__ void site_force(const HDev &P, const double *rp, cons
     const int ns = P.ns, m = s / ns, a = s - m * ns;
     const double *ri = r + 3 * s;
     f[0] = f[1] = f[2] = 0.0;
-    if (P.has_map[a]) {
+    // map corners requested first; in FP32 the bonded terms are evaluated
+    // while they arrive (in FP64 that costs more registers than it saves).
+    // The bonded terms are added last.
+    Corners g;
+    const bool mapped = P.has_map[a] != 0;
+    const bool inside = mapped && hermite_fetch(P.map[a], ri, g);
+    double fb[3] = {0.0, 0.0, 0.0}, eb = 0.0;
+    if (PREC != MH_PREC_FP64) bonded_site(P, rp + 3 * (m * ns), a, fb, eb);
+    if (mapped) {
         double p[4];
-        if (hermite_force<PREC>(P.map[a], ri, p)) {
+        if (inside && hermite_eval<PREC>(P.map[a], g, p)) {
             e += p[0];
             for (int d = 0; d < 3; d++) f[d] += p[1 + d];
         } else {
@@ -466,7 +489,9 @@ __device__ __forceinline__ void site_force(const HDev &P, const double *rp, cons
         f[2] += fzf;
     }
     e += 0.5 * el;  // each pair is seen from both of its sites
-    bonded_site(P, rp + 3 * (m * ns), a, f, e);
+    if (PREC == MH_PREC_FP64) bonded_site(P, rp + 3 * (m * ns), a, fb, eb);
+    e += eb;
+    for (int d = 0; d < 3; d++) f[d] += fb[d];
 }
 
 // energy (block total) and forces of the configuration.  With Fo the forces
