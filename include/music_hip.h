@@ -297,7 +297,8 @@ typedef struct mh_hmc mh_hmc;
 
 int mh_hmc_create(mh_ctx *ctx, const mh_hmc_params *p, int nrep, const int32_t *seeds, int maxN, mh_hmc **out);
 int mh_hmc_destroy(mh_hmc *h);
-/* Threads per replica: 128, 256, 512 (default) or 1024. */
+/* Threads per replica: 128, 256, 512 or 1024 (default 512; 1024 in FP64 when
+   maxN * nsites > 512). */
 int mh_hmc_set_threads(mh_hmc *h, int threads);
 /* n molecules of a replica, principal (unwrapped) site coordinates
  * rp[n][nsites][3]; the simulation-cell coordinates are their pbc images. */

@@ -75,15 +75,17 @@ def co2_isotherm_system(ctx, map_precision=H.MH_PREC_FP64, npoints=64, kmax=7):
                 seeds=[3000 + i for i in range(npoints)])
 
 
-def butane_hmc_system(ctx, nrep=256, nmin=50, nmax=200, seed=5000, load_steps=100000, map_precision=H.MH_PREC_FP32,
-                      log=None):
+def butane_hmc_system(ctx, nrep=256, nmin=50, nmax=200, seed=5000, load_steps=1000000,
+                      map_precision=H.MH_PREC_FP32, log=None):
     """HMC workload: flexible united-atom n-butane (synth.butane_ua) in the
     neutral 40 A framework of synth.config5 (768 atoms, seed 5000), maps for
     CH3 and CH2 at 0.2 A, nrep replicas with N ~ U[nmin, nmax] chains (seed
     + 1).  Start configurations are loaded on the device by a GCMC of rigid
-    all-trans chains at 10^7 kPa (insert/delete/translate/rotate 1:1:1:1,
-    seeds 5100 + i) and truncated to each replica's N (fewer where the
-    framework holds fewer).  HMC: 10 velocity-Verlet steps of 1 fs, 300 K."""
+    all-trans chains at 10^9 kPa (insert/delete/translate/rotate 2:1:1:1,
+    seeds 5100 + i; 10^6 steps load 156-176 chains, the insertions of rigid
+    chains into the dense fluid being rare) and truncated to each replica's N
+    (fewer where the loading holds fewer).  HMC: 10 velocity-Verlet steps of
+    1 fs, 300 K."""
     from . import gcmc as G
     from . import hmc as HM
     cfg = synth.config5(seed)
@@ -100,10 +102,10 @@ def butane_hmc_system(ctx, nrep=256, nmin=50, nmax=200, seed=5000, load_steps=10
     A, B, h2, l2, on = hs["lj"]
     gspec = dict(full_ell=L, fund_ell=cfg["fund_ell"], ref=model["chain"], stype=model["stype"],
                  sq=[0.0] * len(model["stype"]), nst=hs["nst"], lj=(A, B, h2, l2, on), T=300.0,
-                 moves=[("RINSERT", 1.0), ("RDELETE", 1.0), ("RTRANSLATE", 1.0), ("RROTATE", 1.0)],
+                 moves=[("RINSERT", 2.0), ("RDELETE", 1.0), ("RTRANSLATE", 1.0), ("RROTATE", 1.0)],
                  dtrans=0.4, drot=0.4)
     gmaps = {a: (maps[t], None) for a, t in enumerate(model["stype"])}
-    b = G.GcmcBatch(ctx, gspec, [1.0e7] * nrep, [5100 + i for i in range(nrep)], maxN=512, maps=gmaps,
+    b = G.GcmcBatch(ctx, gspec, [1.0e9] * nrep, [5100 + i for i in range(nrep)], maxN=512, maps=gmaps,
                     precision=H.MH_PREC_FP32)
     b.run(load_steps)
     if log:

@@ -104,3 +104,32 @@ def test_gpu_hmc_fp32_forces_vs_fp64(hip, sysdata):
     assert rel_e < 1e-5 and rel_f < 1e-4
     b64.close()
     b32.close()
+
+
+@pytest.mark.parametrize("prec", [H.MH_PREC_FP64, H.MH_PREC_FP32])
+def test_gpu_hmc_workgroup_sizes_agree(hip, sysdata, prec):
+    """The same replicas advanced with 128, 256, 512 and 1024 lanes per
+    replica: identical acceptance counts, energies and coordinates (the lane
+    count changes only the order of the block sums)."""
+    print("case: HMC workgroup sizes, precision %d" % prec, flush=True)
+    model = synth.butane_ua()
+    dev, host = _device_maps(hip, sysdata, model)
+    sp = HS.spec_with_maps(model, host, dt=0.003, nmd=10)
+    rps = [HS.place(model, host, 7, seed=s) for s in (11, 12)]
+    res = {}
+    for thr in (128, 256, 512, 1024):
+        b = HM.HmcBatch(hip, sp, dev, [31, 32], maxN=8, precision=prec, threads=thr)
+        for r in range(2):
+            b.set_config(r, rps[r])
+        b.run(12)
+        s = b.state()
+        res[thr] = (s, [b.config(r) for r in range(2)])
+        b.close()
+    s0, c0 = res[512]
+    assert (s0["acc"] > 3).all()
+    for thr, (s, c) in res.items():
+        assert list(s["acc"]) == list(s0["acc"]) and list(s["att"]) == list(s0["att"]), thr
+        assert np.allclose(s["energy"], s0["energy"], rtol=1e-9, atol=1e-9), thr
+        for r in range(2):
+            assert np.abs(c[r][0] - c0[r][0]).max() < 1e-8, thr
+
