@@ -315,6 +315,26 @@ int mh_hmc_get_config(mh_hmc *h, int rep, int *n, double *rp, double *r);
  * configuration in the batch precision; ok = 0 on a failed evaluation. */
 int mh_hmc_energy_forces(mh_hmc *h, int rep, double *e, double *F, int *ok);
 
+/* ---- plain molecular dynamics on the same replicas (md_doSim with the NVE
+ * velocity-Verlet integrator, md.F90:592-623, velverlet.F90:148-288; the
+ * reference's MD driver music.F90 with the "MD Information" ctrlfile
+ * section).  Velocities v[n][nsites][3] in A/ps (the restart file's
+ * _Velocities_ section, config.F90:2114-2430). */
+int mh_hmc_set_velocities(mh_hmc *h, int rep, int n, const double *v);
+int mh_hmc_get_velocities(mh_hmc *h, int rep, int *n, double *v);
+/* Quantities per replica and step in the trace of mh_hmc_md: energy parts
+ * (kcal/mol) framework map, sorbate-sorbate LJ, stretch, bending, torsion;
+ * sum m v^2 over sites and sum over molecules |sum m v|^2 / M^2
+ * (config_kineticTemp, config.F90:2528-2615; amu A^2/ps^2 and A^2/ps^2);
+ * total momentum sum m v (amu A/ps). */
+#define MH_MD_NQ 10
+/* nsteps NVE velocity-Verlet steps (dt of the params) on every replica from
+ * the current positions and velocities; trace[nrep][nsteps][MH_MD_NQ]
+ * (host memory) receives the values after each step.  A failed energy
+ * evaluation (overlap below locut, a site off its map) stops the replica
+ * and returns MH_ESTATE (the reference stops the run). */
+int mh_hmc_md(mh_hmc *h, int nsteps, double *trace);
+
 /* ------------------------------------------------------------ slot pools
  * Continuous batching of GCMC requests (no counterpart in the reference,
  * which runs one simulation per process).  A pool is an mh_gcmc handle

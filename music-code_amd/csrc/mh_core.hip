@@ -146,7 +146,7 @@ static int check_cell(const double e[3]) {
 int mh_framework_upload(mh_ctx *ctx, int natoms, const double *xyz, const int *type,
                         const double *q, const double fund_ell[3], const double full_ell[3]) {
     MH_CHECK(ctx, MH_EINVAL, "mh_framework_upload: ctx is NULL");
-    MH_CHECK(natoms > 0 && xyz && type && fund_ell && full_ell, MH_EINVAL,
+    MH_CHECK(natoms >= 0 && (natoms == 0 || (xyz && type)) && fund_ell && full_ell, MH_EINVAL,
              "mh_framework_upload: bad arguments");
     MH_CHECK(check_cell(fund_ell) && check_cell(full_ell), MH_EINVAL,
              "mh_framework_upload: cell edges must be positive");
@@ -166,6 +166,8 @@ int mh_framework_upload(mh_ctx *ctx, int natoms, const double *xyz, const int *t
         ctx->fund_ell[i] = fund_ell[i];
         ctx->full_ell[i] = full_ell[i];
     }
+    ctx->ew.set = false;
+    if (natoms == 0) return MH_OK;  // a cell without framework atoms (sorbate-only systems)
     int rc;
     if ((rc = ctx->xyz.alloc(sizeof(double) * 3 * natoms))) return rc;
     if ((rc = ctx->type.alloc(sizeof(int) * natoms))) return rc;
@@ -173,7 +175,6 @@ int mh_framework_upload(mh_ctx *ctx, int natoms, const double *xyz, const int *t
     MH_HIP(hipMemcpy(ctx->xyz.p, ctx->hxyz.data(), sizeof(double) * 3 * natoms, hipMemcpyHostToDevice));
     MH_HIP(hipMemcpy(ctx->type.p, type, sizeof(int) * natoms, hipMemcpyHostToDevice));
     MH_HIP(hipMemcpy(ctx->q.p, ctx->hq.data(), sizeof(double) * natoms, hipMemcpyHostToDevice));
-    ctx->ew.set = false;
     return MH_OK;
 }
 

@@ -63,6 +63,7 @@ struct HDev {
     int ti[HT][4];
     double tc[HT][6];
     double T, dt, rti, max_nrg_md, max_nrg0;
+    double mmassi2;  // 1 / M / M, M = molecule mass
     uint32_t xsj[JLEV][32];     // columns of M^(2^k), M = xorshift(13, 17, 5) on GF(2)^32
     unsigned long long pmj[JLEV];  // 16807^(2^k) mod (2^31 - 1)
 };
@@ -248,8 +249,9 @@ __device__ __forceinline__ double dot3(const double *a, const double *b) {
 }
 
 // bonded terms containing atom `a` of molecule R (principal coordinates):
-// force on a into f, energy of terms whose first atom is a into e
-__device__ void bonded_site(const HDev &P, const double *R, int a, double f[3], double &e) {
+// force on a into f, energy of terms whose first atom is a into eb (stretch,
+// bending, torsion)
+__device__ void bonded_site(const HDev &P, const double *R, int a, double f[3], double (&eb)[3]) {
     for (int q = 0; q < P.nb; q++) {  // harmonic_getinteraction, harmonic.F90:130-160
         const int i = P.bi[q][0], j = P.bi[q][1];
         if (a != i && a != j) continue;
@@ -261,7 +263,7 @@ __device__ void bonded_site(const HDev &P, const double *R, int a, double f[3], 
         const double k_ = P.bk[q], req = P.br0[q];
         const double fm = -k_ * (r - req);
         for (int k = 0; k < 3; k++) f[k] += (a == j) ? vru[k] * fm : vru[k] * fm * (-1.0);
-        if (a == i) e += 0.5 * k_ * (r - req) * (r - req);
+        if (a == i) eb[0] += 0.5 * k_ * (r - req) * (r - req);
     }
     for (int q = 0; q < P.na; q++) {  // harmonica, harmonica.F90:100-160
         const int i = P.ai[q][0], j = P.ai[q][1], l = P.ai[q][2];
@@ -283,7 +285,7 @@ __device__ void bonded_site(const HDev &P, const double *R, int a, double f[3], 
             const double a3 = (v1[k] - v2[k] * (adotb / r22)) * at;
             f[k] += (a == i) ? a1 : (a == l) ? a3 : (a1 + a3) * (-1.0);
         }
-        if (a == i) e += 0.5 * P.ak[q] * (th - P.at0[q]) * (th - P.at0[q]);
+        if (a == i) eb[1] += 0.5 * P.ak[q] * (th - P.at0[q]) * (th - P.at0[q]);
     }
     for (int q = 0; q < P.nt; q++) {  // cosexpansion_getinteraction, cosexpansion.F90:88-196
         const int i1 = P.ti[q][0], i2 = P.ti[q][1], i3 = P.ti[q][2], i4 = P.ti[q][3];
@@ -323,7 +325,7 @@ __device__ void bonded_site(const HDev &P, const double *R, int a, double f[3], 
             else g = dc;
             f[k] += g * ffac;
         }
-        if (a == i1) e += sum;
+        if (a == i1) eb[2] += sum;
     }
 }
 
@@ -410,12 +412,16 @@ __device__ __forceinline__ void list_loop(const uint16_t *nbl, unsigned s, int n
     for (; q < n; q++, o += cap) f(nbl[o]);
 }
 
+// energy parts of a configuration: framework map, sorbate-sorbate LJ,
+// stretch, bending, torsion
+constexpr int kParts = 5;
+
 // force on site s (map, sorbate-sorbate LJ over its neighbour list, bonded
-// terms) into f; its energy share into e; bad on a failed evaluation
+// terms) into f; its energy shares added to ep; bad on a failed evaluation
 template <int PREC>
 __device__ __forceinline__ void site_force(const HDev &P, const double *rp, const double *r, const uint4 *ru,
-                                           const uint16_t *nbl, int cap, int s, int n, double f[3], double &e,
-                                           double &bad) {
+                                           const uint16_t *nbl, int cap, int s, int n, double f[3],
+                                           double (&ep)[kParts], double &bad) {
     const int ns = P.ns, m = s / ns, a = s - m * ns;
     const double *ri = r + 3 * s;
     f[0] = f[1] = f[2] = 0.0;
@@ -425,12 +431,12 @@ __device__ __forceinline__ void site_force(const HDev &P, const double *rp, cons
     Corners g;
     const bool mapped = P.has_map[a] != 0;
     const bool inside = mapped && hermite_fetch(P.map[a], ri, g);
-    double fb[3] = {0.0, 0.0, 0.0}, eb = 0.0;
+    double fb[3] = {0.0, 0.0, 0.0}, eb[3] = {0.0, 0.0, 0.0};
     if (PREC != MH_PREC_FP64) bonded_site(P, rp + 3 * (m * ns), a, fb, eb);
     if (mapped) {
         double p[4];
         if (inside && hermite_eval<PREC>(P.map[a], g, p)) {
-            e += p[0];
+            ep[0] += p[0];
             for (int d = 0; d < 3; d++) f[d] += p[1 + d];
         } else {
             bad = 1.0;
@@ -488,28 +494,31 @@ __device__ __forceinline__ void site_force(const HDev &P, const double *rp, cons
         f[1] += fyf;
         f[2] += fzf;
     }
-    e += 0.5 * el;  // each pair is seen from both of its sites
+    ep[1] += 0.5 * el;  // each pair is seen from both of its sites
     if (PREC == MH_PREC_FP64) bonded_site(P, rp + 3 * (m * ns), a, fb, eb);
-    e += eb;
+    for (int k = 0; k < 3; k++) ep[2 + k] += eb[k];
     for (int d = 0; d < 3; d++) f[d] += fb[d];
 }
 
-// energy (block total) and forces of the configuration.  With Fo the forces
-// are stored there; otherwise each site's acceleration goes to ga and, after
-// a velocity-Verlet step (step > 0), its velocity gv takes the second half
+// energy and forces of the configuration.  With Fo the forces are stored
+// there; otherwise each site's acceleration goes to ga and, after a
+// velocity-Verlet step (step > 0), its velocity gv takes the second half
 // kick.  ke = block total of m v^2 with the velocities after this call.
 // Accelerations and velocities written for an evaluation that fails are not
-// used (the move ends there).
-template <int NT, int PREC>
-__device__ void hmc_forces(const HDev &P, const double *rp, const double *r, const uint4 *ru, const float4 *nb,
-                           int N, const uint16_t *nbl, int cap, double *gv, double *ga, double *Fo, int step,
-                           double &U, int &fail, double &ke, double *red) {
+// used (the move or run ends there).  With MD, parts = block totals of the
+// energy parts and mom = total momentum sum m v; without, parts[0] = the
+// total energy.
+template <int NT, int PREC, bool MD>
+__device__ void hmc_forces_parts(const HDev &P, const double *rp, const double *r, const uint4 *ru,
+                                 const float4 *nb, int N, const uint16_t *nbl, int cap, double *gv, double *ga,
+                                 double *Fo, int step, double (&parts)[kParts], int &fail, double &ke,
+                                 double (&mom)[3], double *red) {
     const int ns = P.ns, nsite = N * ns;
-    double e = 0.0, bad = 0.0, k2 = 0.0;
+    double ep[kParts] = {0.0, 0.0, 0.0, 0.0, 0.0}, bad = 0.0, k2 = 0.0, px = 0.0, py = 0.0, pz = 0.0;
 #pragma unroll 1
     for (int s = threadIdx.x; s < nsite; s += NT) {
         double f[3];
-        site_force<PREC>(P, rp, r, ru, nbl, cap, s, __float_as_int(nb[s].w), f, e, bad);
+        site_force<PREC>(P, rp, r, ru, nbl, cap, s, __float_as_int(nb[s].w), f, ep, bad);
         if (Fo) {
             for (int i = 0; i < 3; i++) Fo[3 * s + i] = f[i];
             continue;
@@ -526,12 +535,36 @@ __device__ void hmc_forces(const HDev &P, const double *rp, const double *r, con
             }
         }
         k2 = k2 + (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * P.mass[at];
+        if (MD) {
+            px += P.mass[at] * v[0];
+            py += P.mass[at] * v[1];
+            pz += P.mass[at] * v[2];
+        }
     }
-    double w[3] = {e, bad, k2};
-    block_sum<NT, 3>(w, red);
-    U = w[0];
-    fail = w[1] != 0.0;
-    ke = w[2];
+    if (MD) {
+        double w[kParts + 5] = {ep[0], ep[1], ep[2], ep[3], ep[4], bad, k2, px, py, pz};
+        block_sum<NT, kParts + 5>(w, red);
+        for (int k = 0; k < kParts; k++) parts[k] = w[k];
+        fail = w[kParts] != 0.0;
+        ke = w[kParts + 1];
+        for (int d = 0; d < 3; d++) mom[d] = w[kParts + 2 + d];
+    } else {
+        double w[3] = {ep[0] + ep[1] + ep[2] + ep[3] + ep[4], bad, k2};
+        block_sum<NT, 3>(w, red);
+        parts[0] = w[0];
+        fail = w[1] != 0.0;
+        ke = w[2];
+    }
+}
+
+// hmc_forces_parts with the total energy U
+template <int NT, int PREC>
+__device__ void hmc_forces(const HDev &P, const double *rp, const double *r, const uint4 *ru, const float4 *nb,
+                           int N, const uint16_t *nbl, int cap, double *gv, double *ga, double *Fo, int step,
+                           double &U, int &fail, double &ke, double *red) {
+    double parts[kParts], mom[3];
+    hmc_forces_parts<NT, PREC, false>(P, rp, r, ru, nb, N, nbl, cap, gv, ga, Fo, step, parts, fail, ke, mom, red);
+    U = parts[0];
 }
 
 template <int NT, int PREC>
@@ -684,6 +717,97 @@ __global__ void __launch_bounds__(NT) k_hmc(const HDev *__restrict__ Pg, HRep R,
     }
 }
 
+// plain NVE molecular dynamics of every replica (md_doSim with
+// velverlet_nveIntegrate, md.F90:592-623, velverlet.F90:148-288): nsteps
+// velocity-Verlet steps from the current positions and velocities; per step
+// the block totals go to trace[rep][step][MH_MD_NQ]: energy parts (map, LJ,
+// stretch, bending, torsion; kcal/mol), sum m v^2 over sites, sum over
+// molecules |sum m v|^2 / M (amu A^2/ps^2) and the total momentum (amu A/ps).
+// A failed evaluation stops the replica with err 2 (the reference stops).
+template <int NT, int PREC>
+__global__ void __launch_bounds__(NT) k_md(const HDev *__restrict__ Pg, HRep R, int nsteps, double *trace) {
+    extern __shared__ double dyn[];
+    __shared__ HDev P;
+    __shared__ double red[NT / 64 * 16];
+    const int tid = threadIdx.x, rep = blockIdx.x;
+    for (int i = tid; i < (int)(sizeof(HDev) / 4); i += NT)
+        reinterpret_cast<int *>(&P)[i] = reinterpret_cast<const int *>(Pg)[i];
+    __syncthreads();
+    if (R.err[rep]) return;
+    const int ns = P.ns, N = R.N[rep], nsite = N * ns;
+    const long long off = (long long)rep * R.cap * 3;
+    double *rp = dyn, *r = dyn + (long long)R.cap * 3;
+    uint4 *ru = reinterpret_cast<uint4 *>(dyn + (long long)R.cap * 6);
+    float4 *nb = reinterpret_cast<float4 *>(ru + R.cap);
+    double *pm = reinterpret_cast<double *>(nb + R.cap);  // [cap][3] site momenta
+    uint16_t *nbl = R.nbl + (long long)rep * R.cap * R.cap;
+    double *gv = R.v + off, *ga = R.a + off;
+    for (int i = tid; i < nsite * 3; i += NT) {
+        rp[i] = R.rp[off + i];
+        r[i] = R.r[off + i];
+    }
+    for (int i = tid; i < nsite; i += NT) put_fx(P, ru, R.r + off, i, i % ns);
+    __syncthreads();
+    hmc_lists<NT>(P, ru, rp, N, nbl, R.cap, nb);
+    const double dt = P.dt, dt2 = dt * dt * 0.5;
+    double parts[kParts], mom[3], k2;
+    int fl, err = 0;
+    // accelerations of the start configuration
+    hmc_forces_parts<NT, PREC, true>(P, rp, r, ru, nb, N, nbl, R.cap, gv, ga, nullptr, 0, parts, fl, k2, mom, red);
+    if (fl) err = 2;
+    for (int step = 1; step <= nsteps && !err; step++) {
+#pragma unroll 1
+        for (int s = tid; s < nsite; s += NT) {
+            for (int i = 0; i < 3; i++) {
+                const double vi = gv[3 * s + i], ai = ga[3 * s + i];
+                rp[3 * s + i] = rp[3 * s + i] + vi * dt + ai * dt2;
+                r[3 * s + i] = r[3 * s + i] + vi * dt + ai * dt2;
+                gv[3 * s + i] = vi + ai * 0.5 * dt;
+            }
+            put_fx(P, ru, r, s, s - (s / ns) * ns);
+        }
+        if (__syncthreads_or(moved_far<NT>(rp, nb, nsite))) hmc_lists<NT>(P, ru, rp, N, nbl, R.cap, nb);
+        hmc_forces_parts<NT, PREC, true>(P, rp, r, ru, nb, N, nbl, R.cap, gv, ga, nullptr, step, parts, fl, k2,
+                                         mom, red);
+        // velverlet_nveIntegrate evaluates with max_nrg = 1000 (velverlet.F90:197-198)
+        if (fl || parts[0] + parts[1] + parts[2] + parts[3] + parts[4] > P.max_nrg_md) {
+            err = 2;
+            break;
+        }
+        // molecular kinetic term sum_m |sum m v|^2 / M^2 (config_kineticTemp,
+        // config.F90:2579-2607): site momenta in LDS, summed per molecule by
+        // one lane
+#pragma unroll 1
+        for (int s = tid; s < nsite; s += NT) {
+            const double ms = P.mass[s - (s / ns) * ns];
+            for (int i = 0; i < 3; i++) pm[3 * s + i] = gv[3 * s + i] * ms;
+        }
+        __syncthreads();
+        double kmol = 0.0;
+#pragma unroll 1
+        for (int m = tid; m < N; m += NT) {
+            double q[3] = {0.0, 0.0, 0.0};
+            for (int a = 0; a < ns; a++)
+                for (int i = 0; i < 3; i++) q[i] = pm[3 * (m * ns + a) + i] + q[i];
+            kmol = (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) * P.mmassi2 + kmol;
+        }
+        double w[1] = {kmol};
+        block_sum<NT, 1>(w, red);
+        if (tid == 0) {
+            double *o = trace + ((long long)rep * nsteps + (step - 1)) * MH_MD_NQ;
+            for (int k = 0; k < kParts; k++) o[k] = parts[k];
+            o[5] = k2;
+            o[6] = w[0];
+            for (int d = 0; d < 3; d++) o[7 + d] = mom[d];
+        }
+    }
+    for (int i = tid; i < nsite * 3; i += NT) {
+        R.rp[off + i] = rp[i];
+        R.r[off + i] = r[i];
+    }
+    if (tid == 0 && err) R.err[rep] = err;
+}
+
 // energy and forces of one replica's configuration (tests, checks)
 template <int NT, int PREC>
 __global__ void __launch_bounds__(NT) k_hmc_forces(const HDev *__restrict__ Pg, HRep R, int rep, double *F_out,
@@ -800,6 +924,25 @@ static int plan_hmc(mh_hmc *h) {
     return MH_OK;
 }
 
+template <int NT, int PREC>
+static void launch_md_nt(mh_hmc *h, int nsteps, double *dtrace, size_t lds) {
+    auto kern = k_md<NT, PREC>;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL(kern, dim3(h->nrep), dim3(NT), lds, h->ctx->stream, h->dP.as<HDev>(), h->rep(), nsteps,
+                       dtrace);
+}
+
+template <int PREC>
+static void launch_md(mh_hmc *h, int nsteps, double *dtrace, size_t lds) {
+    switch (h->threads) {
+    case 128: launch_md_nt<128, PREC>(h, nsteps, dtrace, lds); break;
+    case 256: launch_md_nt<256, PREC>(h, nsteps, dtrace, lds); break;
+    case 1024: launch_md_nt<1024, PREC>(h, nsteps, dtrace, lds); break;
+    default: launch_md_nt<512, PREC>(h, nsteps, dtrace, lds); break;
+    }
+}
+
 extern "C" {
 
 int mh_hmc_create(mh_ctx *ctx, const mh_hmc_params *p, int nrep, const int32_t *seeds, int maxN, mh_hmc **out) {
@@ -895,6 +1038,9 @@ int mh_hmc_create(mh_ctx *ctx, const mh_hmc_params *p, int nrep, const int32_t *
         H.fxs[i] = 4294967296.0 / ctx->full_ell[i];
         H.fxu[i] = (float)(ctx->full_ell[i] / 4294967296.0);
     }
+    double mmass = 0.0;
+    for (int a = 0; a < H.ns; a++) mmass += H.mass[a];
+    H.mmassi2 = 1.0 / mmass / mmass;
     H.T = p->temperature;
     H.dt = p->dt_ps;
     H.rti = 1.0 / ((8.314e-3 / 4.184) * p->temperature);
@@ -1016,6 +1162,57 @@ int mh_hmc_get_config(mh_hmc *h, int rep, int *n, double *rp, double *r) {
     const size_t nc = (size_t)hn * h->ns * 3, off = (size_t)rep * h->maxN * h->ns * 3;
     if (nc && rp) MH_HIP(hipMemcpy(rp, h->rp.as<double>() + off, 8 * nc, hipMemcpyDeviceToHost));
     if (nc && r) MH_HIP(hipMemcpy(r, h->r.as<double>() + off, 8 * nc, hipMemcpyDeviceToHost));
+    return MH_OK;
+}
+
+int mh_hmc_set_velocities(mh_hmc *h, int rep, int n, const double *v) {
+    MH_CHECK(h && rep >= 0 && rep < h->nrep && n >= 0 && (n == 0 || v), MH_EINVAL,
+             "mh_hmc_set_velocities: bad arguments");
+    MH_HIP(hipSetDevice(h->ctx->device));
+    int hn = 0;
+    MH_HIP(hipMemcpy(&hn, h->N.as<int>() + rep, 4, hipMemcpyDeviceToHost));
+    MH_CHECK(n == hn, MH_EINVAL, "mh_hmc_set_velocities: n differs from the replica's molecule count");
+    const size_t nc = (size_t)n * h->ns * 3, off = (size_t)rep * h->maxN * h->ns * 3;
+    if (nc) MH_HIP(hipMemcpy(h->v.as<double>() + off, v, 8 * nc, hipMemcpyHostToDevice));
+    return MH_OK;
+}
+
+int mh_hmc_get_velocities(mh_hmc *h, int rep, int *n, double *v) {
+    MH_CHECK(h && rep >= 0 && rep < h->nrep && n, MH_EINVAL, "mh_hmc_get_velocities: bad arguments");
+    MH_HIP(hipSetDevice(h->ctx->device));
+    int hn = 0;
+    MH_HIP(hipMemcpy(&hn, h->N.as<int>() + rep, 4, hipMemcpyDeviceToHost));
+    *n = hn;
+    const size_t nc = (size_t)hn * h->ns * 3, off = (size_t)rep * h->maxN * h->ns * 3;
+    if (nc && v) MH_HIP(hipMemcpy(v, h->v.as<double>() + off, 8 * nc, hipMemcpyDeviceToHost));
+    return MH_OK;
+}
+
+int mh_hmc_md(mh_hmc *h, int nsteps, double *trace) {
+    MH_CHECK(h && nsteps >= 0 && (nsteps == 0 || trace), MH_EINVAL, "mh_hmc_md: bad arguments");
+    if (!nsteps) return MH_OK;
+    MH_HIP(hipSetDevice(h->ctx->device));
+    const size_t lds = h->lds + (size_t)h->maxN * h->ns * 3 * sizeof(double);
+    int dev_max = 65536;
+    (void)hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->ctx->device);
+    MH_CHECK((long long)lds + (long long)sizeof(HDev) + 8192 <= dev_max, MH_ECAPACITY,
+             "mh_hmc_md: maxN * nsites too large for LDS");
+    const size_t nt = (size_t)h->nrep * nsteps * MH_MD_NQ;
+    DBuf dtrace;
+    int rc;
+    if ((rc = dtrace.alloc(8 * nt))) return rc;
+    MH_HIP(hipMemsetAsync(dtrace.p, 0, 8 * nt, h->ctx->stream));
+    KTimer t(h->ctx, h->prec == MH_PREC_FP32 ? "md32" : "md64");
+    if (h->prec == MH_PREC_FP32) launch_md<MH_PREC_FP32>(h, nsteps, dtrace.as<double>(), lds);
+    else launch_md<MH_PREC_FP64>(h, nsteps, dtrace.as<double>(), lds);
+    MH_HIP(hipGetLastError());
+    MH_HIP(hipStreamSynchronize(h->ctx->stream));
+    t.stop();
+    MH_HIP(hipMemcpy(trace, dtrace.p, 8 * nt, hipMemcpyDeviceToHost));
+    std::vector<int> err(h->nrep);
+    MH_HIP(hipMemcpy(err.data(), h->err.p, 4 * h->nrep, hipMemcpyDeviceToHost));
+    for (int i = 0; i < h->nrep; i++)
+        MH_CHECK(err[i] == 0, MH_ESTATE, "mh_hmc_md: a replica's energy evaluation failed (overlap or off-map site)");
     return MH_OK;
 }
 

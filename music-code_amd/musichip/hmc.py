@@ -12,6 +12,7 @@ from . import hip as H
 from .inputs import lj_AB
 
 _d, _i, _vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+MD_NQ = 10  # MH_MD_NQ
 
 
 class HmcParams(ctypes.Structure):
@@ -58,6 +59,9 @@ def _sig():
         "mh_hmc_get_state": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "mh_hmc_get_config": [_vp, _i, P(_i), _vp, _vp],
         "mh_hmc_energy_forces": [_vp, _i, P(_d), _vp, P(_i)],
+        "mh_hmc_set_velocities": [_vp, _i, _i, _vp],
+        "mh_hmc_get_velocities": [_vp, _i, P(_i), _vp],
+        "mh_hmc_md": [_vp, _i, _vp],
     }
     for n, a in sig.items():
         f = getattr(L, n)
@@ -130,6 +134,22 @@ class HmcBatch:
     def set_threads(self, n):
         """Workgroup size per replica (128, 256, 512 or 1024)."""
         H.check(_sig().mh_hmc_set_threads(self.h, int(n)), "mh_hmc_set_threads")
+
+    def set_velocities(self, rep, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, self.ns, 3)
+        H.check(_sig().mh_hmc_set_velocities(self.h, rep, len(v), _p(v)), "mh_hmc_set_velocities")
+
+    def velocities(self, rep):
+        n = ctypes.c_int()
+        v = np.zeros((self.maxN, self.ns, 3))
+        H.check(_sig().mh_hmc_get_velocities(self.h, rep, ctypes.byref(n), _p(v)), "mh_hmc_get_velocities")
+        return v[:n.value].copy()
+
+    def md(self, nsteps):
+        """nsteps NVE velocity-Verlet steps; trace [nrep][nsteps][MD_NQ]."""
+        tr = np.zeros((self.nrep, int(nsteps), MD_NQ))
+        H.check(_sig().mh_hmc_md(self.h, int(nsteps), _p(tr)), "mh_hmc_md")
+        return tr
 
     def run(self, nmoves):
         H.check(_sig().mh_hmc_run(self.h, int(nmoves)), "mh_hmc_run")

@@ -278,6 +278,77 @@ def _read_gcmc_section(L):
     return g
 
 
+def read_md_section(path):
+    """The ctrlfile's "MD Information" section (md_initParams, md.F90:83-202;
+    moves_init / integrate_init for one INTEGRATE move type): velocity source
+    (a restart filename or Generate), steps per move, dt (ps), temperature,
+    I/O intervals, ensemble and integrator."""
+    L = _lines(path)
+    i = find_section(L, "MD Information")
+    nmv = _ints(L[i])[0]
+    i += 1
+    src = _strip(L[i]).split()[0]
+    i += 2  # the line after the velocity source is a separator
+    mv = []
+    for _ in range(nmv):
+        kind = _strip(L[i]).split()[0].upper()
+        vals = [_strip(L[i + k]).split()[0] for k in range(1, 8)]
+        mv.append(dict(kind=kind, nsteps=int(vals[0]), dt=float(vals[1]), T=float(vals[2]),
+                       iprint=int(vals[3]), idata=int(vals[4]), ensemble=vals[5].upper(),
+                       integrator=vals[6].upper()))
+        i += 8
+    return dict(velocity_source=src, moves=mv)
+
+
+def read_config_init(path):
+    """The ctrlfile's "Configuration Initialization" section: per molecule
+    type its source (RESTARTFILE <file>, FIXED, ...)."""
+    L = _lines(path)
+    i = find_section(L, "Configuration Initialization")
+    out = {}
+    while i + 1 < len(L):
+        name = _strip(L[i]).split()
+        src = _strip(L[i + 1]).split()
+        if not name or _strip(L[i]).startswith("-"):
+            break
+        out[name[0]] = src
+        i += 2
+    return out
+
+
+def read_restartfile(path):
+    """A restart file in the layout of config_writerestartfile
+    (config.F90:2114-2430): description, iteration, time, then per molecule
+    type _MOLECULE_NAME_, nmoles, natoms, the principal coordinates
+    [nmoles][natoms][3], optional _Velocities_ (A/ps) and the generalized
+    coordinates.  Returns dict(desc, iteration, time, species={name: dict(
+    rp, v)})."""
+    with open(path) as f:
+        L = [l.rstrip("\n") for l in f]
+    out = dict(desc=L[0].strip(), iteration=int(L[1].split()[0]), time=float(L[2].split()[0].replace("D", "E")),
+               species={})
+    i = 3
+    while i < len(L):
+        if not L[i].strip().startswith("_MOLECULE_NAME_:"):
+            i += 1
+            continue
+        name = L[i].split(":", 1)[1].split()[0]
+        nmol = int(L[i + 1].split()[0])
+        nat = int(L[i + 2].split()[0])
+        i += 3
+        if L[i].strip() != "_Principal Coordinates Only_":
+            raise ValueError("%s: expected _Principal Coordinates Only_ at line %d" % (path, i + 1))
+        n = nmol * nat
+        rp = np.array([[float(x) for x in L[i + 1 + k].split()[:3]] for k in range(n)]).reshape(nmol, nat, 3)
+        i += 1 + n
+        v = None
+        if i < len(L) and L[i].strip() == "_Velocities_":
+            v = np.array([[float(x) for x in L[i + 1 + k].split()[:3]] for k in range(n)]).reshape(nmol, nat, 3)
+            i += 1 + n
+        out["species"][name] = dict(rp=rp, v=v, nmoles=nmol, natoms=nat)
+    return out
+
+
 # ------------------------------------------------------------ forcefield
 def _kv(tok):
     if "@" in tok:

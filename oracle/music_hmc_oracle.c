@@ -110,8 +110,32 @@ static double torsion_term(const double *cn, const double *r1, const double *r2,
     return sum;
 }
 
+static double energy_forces(const mo_hmc_sys *s, int N, const double *rp, const double *r, double *F, int *ok,
+                            double *parts);
+
 double mo_hmc_energy_forces(const mo_hmc_sys *s, int N, const double *rp, const double *r, double *F, int *ok)
 {
+    return energy_forces(s, N, rp, r, F, ok, NULL);
+}
+
+double mo_hmc_energy_parts(const mo_hmc_sys *s, int N, const double *rp, const double *r, double *F, int *ok,
+                           double *parts)
+{
+    for (int k = 0; k < MO_HMC_NPARTS; k++) parts[k] = 0.0;
+    return energy_forces(s, N, rp, r, F, ok, parts);
+}
+
+/* the total e is summed term by term in the order below; parts (may be NULL)
+ * collects the same terms by kind: map, LJ, stretch, bending, torsion */
+static double energy_forces(const mo_hmc_sys *s, int N, const double *rp, const double *r, double *F, int *ok,
+                            double *parts)
+{
+#define PART(k, x) \
+    do {           \
+        double t_ = (x); \
+        e += t_;   \
+        if (parts) parts[k] += t_; \
+    } while (0)
     const int ns = s->ns;
     *ok = 1;
     if (F) memset(F, 0, sizeof(double) * 3 * (size_t)N * ns);
@@ -128,7 +152,7 @@ double mo_hmc_energy_forces(const mo_hmc_sys *s, int N, const double *rp, const 
                 *ok = 0;
                 return 0.0;
             }
-            e += p[0];
+            PART(0, p[0]);
             if (F)
                 for (int i = 0; i < 3; i++) F[((size_t)m * ns + a) * 3 + i] += p[1 + i];
         }
@@ -149,7 +173,7 @@ double mo_hmc_energy_forces(const mo_hmc_sys *s, int N, const double *rp, const 
                         return 0.0;
                     }
                     double r2i = 1.0 / r2, r6i = r2i * r2i * r2i, r12i = r6i * r6i;
-                    e += s->lj_A[t] * r12i - s->lj_B[t] * r6i;
+                    PART(1, s->lj_A[t] * r12i - s->lj_B[t] * r6i);
                     if (F) {
                         double f = (12.0 * s->lj_A[t] * r12i - 6.0 * s->lj_B[t] * r6i) * r2i;
                         for (int i = 0; i < 3; i++) {
@@ -164,17 +188,18 @@ double mo_hmc_energy_forces(const mo_hmc_sys *s, int N, const double *rp, const 
         double *G = F ? F + (size_t)m * ns * 3 : NULL;
 #define S3(i) (G ? G + 3 * (i) : NULL)
         for (int q = 0; q < s->nb; q++)
-            e += bond_term(s->bk[q], s->br0[q], R + 3 * s->bi[q][0], R + 3 * s->bi[q][1], S3(s->bi[q][0]),
-                           S3(s->bi[q][1]));
+            PART(2, bond_term(s->bk[q], s->br0[q], R + 3 * s->bi[q][0], R + 3 * s->bi[q][1], S3(s->bi[q][0]),
+                              S3(s->bi[q][1])));
         for (int q = 0; q < s->na; q++)
-            e += angle_term(s->ak[q], s->at0[q], R + 3 * s->ai[q][0], R + 3 * s->ai[q][1], R + 3 * s->ai[q][2],
-                            S3(s->ai[q][0]), S3(s->ai[q][1]), S3(s->ai[q][2]));
+            PART(3, angle_term(s->ak[q], s->at0[q], R + 3 * s->ai[q][0], R + 3 * s->ai[q][1], R + 3 * s->ai[q][2],
+                               S3(s->ai[q][0]), S3(s->ai[q][1]), S3(s->ai[q][2])));
         for (int q = 0; q < s->nt; q++)
-            e += torsion_term(s->tc[q], R + 3 * s->ti[q][0], R + 3 * s->ti[q][1], R + 3 * s->ti[q][2],
-                              R + 3 * s->ti[q][3], S3(s->ti[q][0]), S3(s->ti[q][1]), S3(s->ti[q][2]),
-                              S3(s->ti[q][3]));
+            PART(4, torsion_term(s->tc[q], R + 3 * s->ti[q][0], R + 3 * s->ti[q][1], R + 3 * s->ti[q][2],
+                                 R + 3 * s->ti[q][3], S3(s->ti[q][0]), S3(s->ti[q][1]), S3(s->ti[q][2]),
+                                 S3(s->ti[q][3])));
 #undef S3
     }
+#undef PART
     return e;
 }
 
@@ -252,6 +277,60 @@ int mo_hmc_nve(const mo_hmc_sys *s, int N, double *rp, double *r, double *v, int
     free(F);
     free(acc);
     return good;
+}
+
+/* md_doSim with velverlet_nveIntegrate (md.F90:592-623, velverlet.F90:148-288):
+ * nsteps steps from rp, r, v; after each step trace[step][MO_MD_NQ] = energy
+ * parts, sum m v^2, sum_m |sum m v|^2 / M^2 (config_kineticTemp,
+ * config.F90:2528-2615), total momentum.  Returns the number of steps done
+ * (fewer on a failed evaluation). */
+int mo_hmc_md(const mo_hmc_sys *s, int N, double *rp, double *r, double *v, int nsteps, double *trace)
+{
+    const int ns = s->ns;
+    const size_t n3 = (size_t)N * ns * 3;
+    double *F = malloc(sizeof(double) * n3), *acc = malloc(sizeof(double) * n3);
+    const double dt = s->dt, dt2 = dt * dt * 0.5;
+    double mmass = 0.0, parts[MO_HMC_NPARTS];
+    for (int a = 0; a < ns; a++) mmass += s->mass[a];
+    const double mmassi2 = 1.0 / mmass / mmass;
+    int ok, done = 0;
+    energy_forces(s, N, rp, r, F, &ok, NULL);
+    if (ok) {
+        accel(s, N, F, acc);
+        for (int k = 0; k < nsteps; k++) {
+            for (size_t o = 0; o < n3; o++) {
+                rp[o] = rp[o] + v[o] * dt + acc[o] * dt2;
+                r[o] = r[o] + v[o] * dt + acc[o] * dt2;
+                v[o] = v[o] + acc[o] * 0.5 * dt;
+            }
+            double U = mo_hmc_energy_parts(s, N, rp, r, F, &ok, parts);
+            if (!ok || U > s->max_nrg_md) break;
+            accel(s, N, F, acc);
+            for (size_t o = 0; o < n3; o++) v[o] = v[o] + acc[o] * 0.5 * dt;
+            double *t = trace + (size_t)k * MO_MD_NQ;
+            for (int q = 0; q < MO_HMC_NPARTS; q++) t[q] = parts[q];
+            double k2 = 0.0, km = 0.0, P[3] = {0.0, 0.0, 0.0};
+            for (int m = 0; m < N; m++) {
+                double q[3] = {0.0, 0.0, 0.0};
+                for (int a = 0; a < ns; a++) {
+                    const double *vv = v + ((size_t)m * ns + a) * 3;
+                    k2 = k2 + dot3(vv, vv) * s->mass[a];
+                    for (int i = 0; i < 3; i++) {
+                        q[i] = vv[i] * s->mass[a] + q[i];
+                        P[i] += s->mass[a] * vv[i];
+                    }
+                }
+                km = dot3(q, q) * mmassi2 + km;
+            }
+            t[5] = k2;
+            t[6] = km;
+            for (int i = 0; i < 3; i++) t[7 + i] = P[i];
+            done++;
+        }
+    }
+    free(F);
+    free(acc);
+    return done;
 }
 
 int mo_hmc_move(const mo_hmc_sys *s, mo_hmc_state *st)

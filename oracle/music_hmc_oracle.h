@@ -78,6 +78,14 @@ int mo_hmc_move(const mo_hmc_sys *s, mo_hmc_state *st);
  * nsteps steps, writes total energy (U + KE/4.184, kcal/mol) after every
  * step into etot[nsteps + 1]; returns 0 on failure. */
 int mo_hmc_nve(const mo_hmc_sys *s, int N, double *rp, double *r, double *v, int nsteps, double *etot);
+/* energy parts map, LJ, stretch, bending, torsion (kcal/mol) */
+#define MO_HMC_NPARTS 5
+double mo_hmc_energy_parts(const mo_hmc_sys *s, int N, const double *rp, const double *r, double *F, int *ok,
+                           double *parts);
+/* plain NVE MD: per step the energy parts, sum m v^2, sum_m |sum m v|^2 / M^2
+ * and the total momentum (MO_MD_NQ values); returns the steps done */
+#define MO_MD_NQ 10
+int mo_hmc_md(const mo_hmc_sys *s, int N, double *rp, double *r, double *v, int nsteps, double *trace);
 
 #ifdef __cplusplus
 }

@@ -484,6 +484,8 @@ def _hmc_lib():
     L.mo_hmc_move.restype = _i
     L.mo_hmc_nve.argtypes = [_vp, _i, _vp, _vp, _vp, _i, _vp]
     L.mo_hmc_nve.restype = _i
+    L.mo_hmc_md.argtypes = [_vp, _i, _vp, _vp, _vp, _i, _vp]
+    L.mo_hmc_md.restype = _i
     L._hmc_sig = True
     return L
 
@@ -582,6 +584,19 @@ class Hmc:
 
     def gaussian(self, sigma=1.0):
         return _hmc_lib().mo_hmc_gaussian(ctypes.byref(self.st), 0.0, sigma)
+
+    def md(self, v, nsteps):
+        """Plain NVE MD from the current configuration and velocities v
+        [N][ns][3] (mo_hmc_md); returns (steps done, trace [nsteps][10],
+        final rp, r, v)."""
+        n = self.st.N
+        rp = np.ascontiguousarray(self.rp[:n].copy())
+        r = np.ascontiguousarray(self.r[:n].copy())
+        v = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(n, -1, 3).copy())
+        tr = np.zeros((nsteps, 10))
+        done = _hmc_lib().mo_hmc_md(ctypes.byref(self.sys), n, rp.ctypes.data, r.ctypes.data, v.ctypes.data, nsteps,
+                                    tr.ctypes.data)
+        return done, tr, rp, r, v
 
     def nve(self, v, nsteps):
         """Plain NVE trajectory from velocities v [N][ns][3]; total energy
