@@ -226,6 +226,12 @@ int mh_gcmc_autotune(mh_gcmc *g, int probe_steps, int *threads_out, int *exy_out
 int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const double *ang);
 /* nsteps MC steps on every replica in one kernel launch. */
 int mh_gcmc_run(mh_gcmc *g, int nsteps);
+/* ncycles MC cycles of nsteps steps (gcmc_dosim per cycle, the driver loop
+ * music_gcmc.F90:203-237): the persistent launch of one cycle is captured
+ * once in a hipGraph and replayed per cycle, with one host synchronization at
+ * the end; re-captured when the batch's launch configuration or buffers
+ * change.  Same results as ncycles calls of mh_gcmc_run(g, nsteps). */
+int mh_gcmc_run_cycles(mh_gcmc *g, int nsteps, int ncycles);
 /* Per replica: N, running energy (kcal/mol), sum of N over steps, steps,
  * att/acc [nrep][4] by move type, RNG state (for checkpoints).  Any
  * pointer may be NULL. */

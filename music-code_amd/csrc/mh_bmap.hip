@@ -241,7 +241,7 @@ int mh_bmap_create(mh_ctx *ctx, const mh_map *pmap, double bias_temperature, mh_
 int mh_bmap_destroy(mh_bmap *b) {
     if (!b) return MH_OK;
     mh_ctx *ctx = b->ctx;
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     delete b;
     ctx_release(ctx);
     return MH_OK;

@@ -1085,6 +1085,18 @@ struct mh_gcmc {
     DBuf tr_move, tr_acc, tr_N, tr_dE;
     int tr_cap = 0;
     long long tr_base = 0;
+    // hipGraph of one cycle's launch (mh_gcmc_run_cycles) and what it baked in
+    hipGraphExec_t gexec = nullptr;
+    struct GraphKey {
+        Rep r;
+        const void *P;
+        int nsteps, threads, prec, kpl;
+        bool ldsmol, exy_off;
+        size_t lds_bytes;
+    } gkey;
+    ~mh_gcmc() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+    }
     Rep rep() {
         Rep r;
         r.N = N.as<int>();
@@ -1690,6 +1702,52 @@ int mh_gcmc_run(mh_gcmc *g, int nsteps) {
         MH_CHECK(err[i] == 0, MH_ESTATE, "mh_gcmc_run: an existing molecule failed to evaluate");
     }
     g->tr_base = g->tr_base;  // trace window stays anchored at the step set_trace saw
+    return MH_OK;
+}
+
+int mh_gcmc_run_cycles(mh_gcmc *g, int nsteps, int ncycles) {
+    MH_CHECK(g && nsteps >= 0 && ncycles >= 0, MH_EINVAL, "mh_gcmc_run_cycles: bad arguments");
+    if (nsteps == 0 || ncycles == 0) return MH_OK;
+    MH_HIP(hipSetDevice(g->ctx->device));
+    mh_gcmc::GraphKey k;
+    std::memset(&k, 0, sizeof k);
+    k.r = g->rep();
+    k.P = g->dP.p;
+    k.nsteps = nsteps;
+    k.threads = g->threads;
+    k.prec = g->prec;
+    k.kpl = g->kpl;
+    k.ldsmol = g->ldsmol;
+    k.exy_off = g->exy_off;
+    k.lds_bytes = g->lds_bytes;
+    if (!g->gexec || std::memcmp(&k, &g->gkey, sizeof k) != 0) {
+        if (g->gexec) {
+            MH_HIP(hipGraphExecDestroy(g->gexec));
+            g->gexec = nullptr;
+        }
+        hipStream_t st = g->ctx->stream;
+        MH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        launch_any(g, nsteps);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(st, &graph);
+        MH_HIP(ec);
+        const hipError_t ei = hipGraphInstantiate(&g->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        MH_HIP(ei);
+        g->gkey = k;
+    }
+    KTimer t(g->ctx, g->prec == MH_PREC_FP32 ? "gcmc32" : "gcmc64");
+    for (int c = 0; c < ncycles; c++) MH_HIP(hipGraphLaunch(g->gexec, g->ctx->stream));
+    MH_HIP(hipStreamSynchronize(g->ctx->stream));
+    t.stop();
+    std::vector<int> err(g->nrep);
+    MH_HIP(hipMemcpy(err.data(), g->err.p, 4 * g->nrep, hipMemcpyDeviceToHost));
+    for (int i = 0; i < g->nrep; i++) {
+        MH_CHECK(err[i] != 2, MH_ECAPACITY, "mh_gcmc_run_cycles: a replica reached maxN molecules");
+        MH_CHECK(err[i] != 4, MH_ESTATE,
+                 "mh_gcmc_run_cycles: BDELETE picked a molecule whose cubelet is not in the bias map");
+        MH_CHECK(err[i] == 0, MH_ESTATE, "mh_gcmc_run_cycles: an existing molecule failed to evaluate");
+    }
     return MH_OK;
 }
 

@@ -52,7 +52,8 @@ module music_hip_mod
   public :: mh_ewald_set, mh_map_build, mh_map_destroy, mh_map_info, mh_map_get, mh_map_write
   public :: mh_map_load, mh_map_interp
   public :: mh_gcmc_create, mh_gcmc_destroy, mh_gcmc_set_threads, mh_gcmc_autotune
-  public :: mh_gcmc_set_config, mh_gcmc_run, mh_gcmc_get_state, mh_gcmc_set_rng, mh_gcmc_get_moments
+  public :: mh_gcmc_set_config, mh_gcmc_run, mh_gcmc_run_cycles, mh_gcmc_get_state, mh_gcmc_set_rng, &
+            mh_gcmc_get_moments
   public :: mh_gcmc_get_config, mh_gcmc_total_energy, mh_trial_energy
   public :: mh_bmap_create, mh_bmap_destroy, mh_bmap_info
 
@@ -229,6 +230,13 @@ module music_hip_mod
        integer(c_int), value :: nsteps
        integer(c_int) :: rc
      end function mh_gcmc_run
+
+     function mh_gcmc_run_cycles(g, nsteps, ncycles) bind(C, name="mh_gcmc_run_cycles") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: g
+       integer(c_int), value :: nsteps, ncycles
+       integer(c_int) :: rc
+     end function mh_gcmc_run_cycles
 
      function mh_gcmc_get_state(g, N, energy, sumN, nsteps, att, acc, ix, iy) &
           bind(C, name="mh_gcmc_get_state") result(rc)

@@ -40,6 +40,7 @@ def _sig():
         "mh_gcmc_set_threads": [_vp, _i],
         "mh_gcmc_set_config": [_vp, _i, _i, _vp, _vp],
         "mh_gcmc_run": [_vp, _i],
+        "mh_gcmc_run_cycles": [_vp, _i, _i],
         "mh_gcmc_get_state": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "mh_gcmc_set_rng": [_vp, _i, ctypes.c_int32, ctypes.c_int32],
         "mh_gcmc_get_moments": [_vp, _vp, _vp],
@@ -146,6 +147,10 @@ class GcmcBatch:
 
     def run(self, nsteps):
         H.check(_sig().mh_gcmc_run(self.h, int(nsteps)), "mh_gcmc_run")
+
+    def run_cycles(self, nsteps, ncycles):
+        """ncycles cycles of nsteps steps replayed from one captured hipGraph."""
+        H.check(_sig().mh_gcmc_run_cycles(self.h, int(nsteps), int(ncycles)), "mh_gcmc_run_cycles")
 
     def state(self):
         R = self.nrep
