@@ -202,6 +202,14 @@ typedef struct {
     int precision;                       /* MH_PREC_FP64 or MH_PREC_FP32 */
     const mh_bmap *bias;                 /* bias map of BINSERT / BDELETE, or NULL;
                                             kept alive by the caller while the batch lives */
+    /* plain Coulomb between sorbate sites (coulomb_snglint, coulomb.F90:258-327;
+     * atom_atom_file COUL lines under an spc_spc "COUL BASIC"): for the type
+     * pairs with coul_on, 331.5 shield q_a q_b (1/r - 1/hicut) kcal/mol within
+     * hicut (1/r for all minimum-image pairs when hicut = 0); r < locut fails
+     * the evaluation.  Exclusive with ewald. */
+    int coul;
+    uint8_t coul_on[64];                 /* [ntypes][ntypes] */
+    double coul_hicut, coul_locut, coul_shield;
 } mh_gcmc_params;
 
 typedef struct mh_gcmc mh_gcmc;

@@ -54,6 +54,10 @@ struct GDev {
     int on[64];
     int ewald, kmax, nk;
     double kappa, rc2;
+    // plain Coulomb: 331.5 shield, hicut^2 (0: none), 1/hicut (0: none), locut^2
+    int coul;
+    int con[64];
+    double cfac, ch2, cshift, cl2;
     const double *wk;      // [nk]
     const int8_t *kidx;    // [nk][4] (mx, my, mz, pad)
     double full[3], fund[3];
@@ -207,8 +211,21 @@ __device__ bool site_map_energy(const GDev &P, int a, const double *r, double &e
     return true;
 }
 
+// plain Coulomb of a site pair, coulomb_snglint (coulomb.F90:271-306):
+// (331.5 shield q1 q2)(1/r - 1/hicut) within hicut; false below locut.  FP64
+// in both kernel precisions: without a cutoff thousands of +-1/r terms
+// cancel, and FP32 pair terms would leave ~1e-5 of the net.
+__device__ __forceinline__ bool coul_energy64(const GDev &P, int t, double qa, double qb, double r2, double &e) {
+    if (P.con[t] && !(P.ch2 > 0.0 && r2 > P.ch2)) {
+        if (r2 < P.cl2) return false;
+        const double r = sqrt(r2);
+        e = __dadd_rn(e, __dmul_rn(__dmul_rn(__dmul_rn(P.cfac, qa), qb), __dsub_rn(1.0 / r, P.cshift)));
+    }
+    return true;
+}
+
 // pair energy of a site pair (lj_snglint without derivatives + Ewald real
-// space).  Returns false on an LJ locut violation.
+// space + plain Coulomb).  Returns false on an LJ or Coulomb locut violation.
 __device__ __forceinline__ bool pair_energy64(const GDev &P, int t, double qa, double qb, double sx, double sy,
                                               double sz, double &e) {
     double r2 = __dadd_rn(__dadd_rn(__dmul_rn(sx, sx), __dmul_rn(sy, sy)), __dmul_rn(sz, sz));
@@ -223,7 +240,7 @@ __device__ __forceinline__ bool pair_energy64(const GDev &P, int t, double qa, d
         double rr = sqrt(r2);
         e = __dadd_rn(e, __dmul_rn(__ddiv_rn(__dmul_rn(__dmul_rn(qa, qb), erfc_as64(P.kappa * rr)), rr), kTokcalG));
     }
-    return true;
+    return P.coul ? coul_energy64(P, t, qa, qb, r2, e) : true;
 }
 
 __device__ __forceinline__ bool pair_energy32(const GDev &P, int t, float qq, float kappa, float rc2, float sx,
@@ -437,22 +454,24 @@ __device__ __forceinline__ void move_energy(const GDev &P, Shared &sh, const dou
         } else {
             const float qq = P.ewald ? (float)(P.q[a] * P.q[b] * kTokcalG) : 0.0f;
             if (hasA) {
-                float sx = (float)fold64(sh.posA[3 * a] - xb, P.full[0]);
-                float sy = (float)fold64(sh.posA[3 * a + 1] - yb, P.full[1]);
-                float sz = (float)fold64(sh.posA[3 * a + 2] - zb, P.full[2]);
-                if (!pair_energy32(P, t, qq, kap32, rc32, sx, sy, sz, eAf)) failA = 1.0;
+                const double dx = fold64(sh.posA[3 * a] - xb, P.full[0]);
+                const double dy = fold64(sh.posA[3 * a + 1] - yb, P.full[1]);
+                const double dz = fold64(sh.posA[3 * a + 2] - zb, P.full[2]);
+                if (!pair_energy32(P, t, qq, kap32, rc32, (float)dx, (float)dy, (float)dz, eAf)) failA = 1.0;
+                if (P.coul && !coul_energy64(P, t, P.q[a], P.q[b], dx * dx + dy * dy + dz * dz, eA)) failA = 1.0;
             }
             if (hasB) {
-                float sx = (float)fold64(sh.posB[3 * a] - xb, P.full[0]);
-                float sy = (float)fold64(sh.posB[3 * a + 1] - yb, P.full[1]);
-                float sz = (float)fold64(sh.posB[3 * a + 2] - zb, P.full[2]);
-                pair_energy32(P, t, qq, kap32, rc32, sx, sy, sz, eBf);
+                const double dx = fold64(sh.posB[3 * a] - xb, P.full[0]);
+                const double dy = fold64(sh.posB[3 * a + 1] - yb, P.full[1]);
+                const double dz = fold64(sh.posB[3 * a + 2] - zb, P.full[2]);
+                pair_energy32(P, t, qq, kap32, rc32, (float)dx, (float)dy, (float)dz, eBf);
+                if (P.coul) coul_energy64(P, t, P.q[a], P.q[b], dx * dx + dy * dy + dz * dz, eB);
             }
         }
     }
-    if (PREC != MH_PREC_FP64) {
-        eA = (double)eAf;
-        eB = (double)eBf;
+    if (PREC != MH_PREC_FP64) {  // FP64 plain-Coulomb sums already in eA, eB
+        eA += (double)eAf;
+        eB += (double)eBf;
     }
     // reciprocal-space delta over the lane's k vectors
     double eK = 0.0;
@@ -1272,6 +1291,10 @@ static int build_system(mh_ctx *ctx, const mh_gcmc_params *p, int maxN, const do
         MH_CHECK(p->ewald_kmax >= 1 && p->ewald_kmax <= GKMAX, MH_EINVAL, w + ": ewald_kmax out of range [1,15]");
         MH_CHECK(p->ewald_kappa > 0.0, MH_EINVAL, w + ": ewald_kappa must be > 0");
     }
+    if (p->coul) {
+        MH_CHECK(!p->ewald, MH_EINVAL, w + ": plain Coulomb and Ewald are exclusive");
+        MH_CHECK(p->coul_hicut >= 0.0 && p->coul_locut >= 0.0, MH_EINVAL, w + ": Coulomb cutoffs must be >= 0");
+    }
     GDev &H = S.H;
     std::memset(&H, 0, sizeof H);
     H.ns = p->nsites;
@@ -1309,6 +1332,14 @@ static int build_system(mh_ctx *ctx, const mh_gcmc_params *p, int maxN, const do
     for (int i = 0; i < 3; i++) {
         H.full[i] = full[i];
         H.fund[i] = fund[i];
+    }
+    H.coul = p->coul ? 1 : 0;
+    if (H.coul) {
+        for (int t = 0; t < p->ntypes * p->ntypes; t++) H.con[t] = p->coul_on[t] ? 1 : 0;
+        H.cfac = 331.5 * p->coul_shield;
+        H.ch2 = p->coul_hicut * p->coul_hicut;
+        H.cshift = p->coul_hicut > 0.0 ? 1.0 / p->coul_hicut : 0.0;
+        H.cl2 = p->coul_locut * p->coul_locut;
     }
     H.ewald = p->ewald ? 1 : 0;
     H.kmax = p->ewald ? p->ewald_kmax : 0;

@@ -45,6 +45,9 @@ module music_hip_mod
      real(c_double) :: max_nrg
      integer(c_int) :: precision
      type(c_ptr) :: bias = c_null_ptr
+     integer(c_int) :: coul = 0
+     integer(c_int8_t) :: coul_on(64) = 0_c_int8_t
+     real(c_double) :: coul_hicut = 0.0d0, coul_locut = 1.0d-3, coul_shield = 1.0d0
   end type mh_gcmc_params
 
   public :: mh_error_message

@@ -26,7 +26,8 @@ class GcmcParams(ctypes.Structure):
                 ("lj_on", ctypes.c_uint8 * 64), ("ewald", _i), ("ewald_kappa", _d), ("ewald_kmax", _i),
                 ("ewald_rcut", _d), ("temperature", _d), ("nmoves", _i), ("move_type", _i * 4),
                 ("move_weight", _d * 4), ("delta_trans", _d), ("delta_rot", _d), ("max_nrg", _d),
-                ("precision", _i), ("bias", _vp)]
+                ("precision", _i), ("bias", _vp), ("coul", _i), ("coul_on", ctypes.c_uint8 * 64),
+                ("coul_hicut", _d), ("coul_locut", _d), ("coul_shield", _d)]
 
 
 def _sig():
@@ -70,9 +71,9 @@ def _p(a):
 
 
 def make_params(spec, maps=None, precision=H.MH_PREC_FP64):
-    """spec as for the oracle's Gcmc (ref, stype, sq, nst, lj, ewald, T,
-    moves, dtrans, drot); maps: dict site index -> (pmap Map or None,
-    emap Map or None)."""
+    """spec as for the oracle's Gcmc (ref, stype, sq, nst, lj, ewald or
+    coul, T, moves, dtrans, drot); maps: dict site index -> (pmap Map or
+    None, emap Map or None)."""
     p = GcmcParams()
     ref = np.asarray(spec["ref"], dtype=np.float64).reshape(-1, 3)
     p.nsites = len(ref)
@@ -93,6 +94,12 @@ def make_params(spec, maps=None, precision=H.MH_PREC_FP64):
     ew = spec.get("ewald")
     if ew:
         p.ewald, p.ewald_kappa, p.ewald_kmax, p.ewald_rcut = 1, ew["kappa"], ew["kmax"], ew["rc"]
+    cl = spec.get("coul")  # plain Coulomb: dict(on=[nst*nst], hicut, locut, shield)
+    if cl:
+        p.coul = 1
+        for t in range(p.ntypes * p.ntypes):
+            p.coul_on[t] = 1 if cl["on"][t] else 0
+        p.coul_hicut, p.coul_locut, p.coul_shield = cl["hicut"], cl["locut"], cl.get("shield", 1.0)
     p.temperature = spec["T"]
     p.nmoves = len(spec["moves"])
     for i, (t, w) in enumerate(spec["moves"]):

@@ -230,6 +230,13 @@ static int mol_energy(const mo_gcmc_sys *s, const mo_gcmc_state *st, const doubl
                     double rr = sqrt(r2);
                     up = up + s->sq[a] * s->sq[b] * erfc_as(s->kappa * rr) / rr * MO_TOKCAL;
                 }
+                /* coulomb_snglint (coulomb.F90:271-306) */
+                if (s->coul && s->coul_on[t] && !(s->coul_hicut > 0.0 && r2 > s->coul_hicut * s->coul_hicut)) {
+                    if (r2 < s->coul_locut * s->coul_locut) return 0;
+                    double rr = sqrt(r2);
+                    double shift = s->coul_hicut > 0.0 ? 1.0 / s->coul_hicut : 0.0;
+                    up = up + (331.5 * s->coul_shield * s->sq[a] * s->sq[b]) * (1.0 / rr - shift);
+                }
             }
     }
     *umap = um;
@@ -604,7 +611,7 @@ double mo_gcmc_total_energy(const mo_gcmc_sys *s, const mo_gcmc_state *st)
     for (int m = 0; m < st->N; m++) {
         tmp.N = m; /* molecules 0..m-1 only */
         double um, up;
-        mol_energy(s, &tmp, st->r + 3 * s->ns * m, -1, &um, &up);
+        if (!mol_energy(s, &tmp, st->r + 3 * s->ns * m, -1, &um, &up)) return NAN; /* a failed evaluation */
         e += um + up;
     }
     if (s->ewald) {

@@ -60,6 +60,12 @@ typedef struct {
     double cumwt[4];
     double insdelratio, dtrans, drot, max_nrg;
     const mo_bmap *bias;                      /* BINSERT / BDELETE, or NULL */
+    /* plain Coulomb (coulomb_snglint, coulomb.F90:258-327) for the type pairs
+     * with coul_on: 331.5 shield q q (1/r - 1/hicut) within hicut (hicut 0:
+     * no cutoff); r < locut fails */
+    int coul;
+    int coul_on[64];
+    double coul_hicut, coul_locut, coul_shield;
 } mo_gcmc_sys;
 
 typedef struct {

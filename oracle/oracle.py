@@ -284,7 +284,9 @@ class GcmcSys(ctypes.Structure):
                 ("lj_l2", _d * 64), ("lj_on", _i * 64),
                 ("ewald", _i), ("kmax", _i), ("kappa", _d), ("rc2_real", _d), ("T", _d),
                 ("nmoves", _i), ("move_type", _i * 4), ("cumwt", _d * 4),
-                ("insdelratio", _d), ("dtrans", _d), ("drot", _d), ("max_nrg", _d), ("bias", _vp)]
+                ("insdelratio", _d), ("dtrans", _d), ("drot", _d), ("max_nrg", _d), ("bias", _vp),
+                ("coul", _i), ("coul_on", _i * 64), ("coul_hicut", _d), ("coul_locut", _d),
+                ("coul_shield", _d)]
 
 
 class GcmcTrace(ctypes.Structure):
@@ -335,8 +337,9 @@ class Gcmc:
     """One GCMC chain on the CPU oracle.  spec: dict with full_ell, fund_ell,
     ref [ns,3], stype [ns], sq [ns], nst, lj (A, B, h2, l2, on: [nst*nst]),
     maps: list of (npnts, step, ptr, vals8) and site_pmap/site_emap,
-    ewald (kappa, kmax, rc), T, moves [(type, weight)], dtrans, drot, and
-    for BINSERT / BDELETE bias (a Bmap)."""
+    ewald (kappa, kmax, rc) or coul (on [nst*nst], hicut, locut, shield), T,
+    moves [(type, weight)], dtrans, drot, and for BINSERT / BDELETE bias (a
+    Bmap)."""
 
     def __init__(self, spec, fugacity_kPa, seed, maxN=2048, trace=0):
         L = _gcmc_lib()
@@ -373,6 +376,12 @@ class Gcmc:
         ew = spec.get("ewald")
         if ew:
             s.ewald, s.kappa, s.kmax, s.rc2_real = 1, ew["kappa"], ew["kmax"], ew["rc"] ** 2
+        cl = spec.get("coul")  # plain Coulomb: dict(on=[nst*nst], hicut, locut, shield)
+        if cl:
+            s.coul = 1
+            for t in range(s.nst * s.nst):
+                s.coul_on[t] = int(cl["on"][t])
+            s.coul_hicut, s.coul_locut, s.coul_shield = cl["hicut"], cl["locut"], cl.get("shield", 1.0)
         s.T = spec["T"]
         mv = spec["moves"]
         s.nmoves = len(mv)
