@@ -238,6 +238,8 @@ program music_gcmc
   end do
   if (P%ewald /= 0) write(*, '(a, es24.16, a, i0, a, es24.16)') '  ewald: kappa ', P%ewald_kappa, ' kmax ', &
        P%ewald_kmax, ' rcut ', P%ewald_rcut
+  if (P%coul /= 0) write(*, '(a, es24.16, a, es24.16, a, es24.16)') '  plain coulomb: hicut ', P%coul_hicut, &
+       ' locut ', P%coul_locut, ' shield ', P%coul_shield
   if (len_trim(biasf) > 0) write(*, '(a, a, a, f10.3)') '  bias: ', trim(biasf), ' T ', bias_T
   if (len_trim(srcfile) > 0) write(*, '(2a)') '  start configuration: ', trim(srcfile)
   write(*, '(a, i0, 2a)') '  restart files every ', icrash, ' iterations: ', trim(resname)
@@ -799,9 +801,12 @@ contains
                    P%lj_on(k2) = 1_c_int8_t
                 end do
              end do
+          else if (trim(upper(word(s, 4))) == 'BASIC' .and. index(upper(s), 'WFCOUL') == 0) then
+             ! plain Coulomb from the atom-atom COUL lines (coulomb_snglint)
+             call plain_coulomb()
           else
              if (index(upper(s), 'EWALD') == 0) &
-                  call fail('species-species COUL model must be SUM EWALD (or OFF): ' // trim(s))
+                  call fail('species-species COUL model must be SUM EWALD, BASIC (plain COUL) or OFF: ' // trim(s))
              kappa_param = 0.0d0
              rcut = 0.0d0
              P%ewald_kmax = 0
@@ -833,6 +838,36 @@ contains
        end if
     end do
   end subroutine read_species_file
+
+  ! plain Coulomb between the sorbate's site types: per type pair the
+  ! atom_atom COUL line (on unless OFF); one hicut and shield constant for all
+  ! pairs (the kernel's form), locut LOW_DIST_CUTOFF
+  subroutine plain_coulomb()
+    integer :: b, c
+    logical :: found, off, first
+    real(c_double) :: hc, sh
+    character(len=LL) :: aa
+    aa = find_file(sy%aafile, 'NONE', '', sy%base)
+    P%coul = 1
+    P%coul_locut = LOW_DIST_CUTOFF
+    first = .true.
+    do b = 1, nst
+       do c = 1, nst
+          call read_coul_pair(aa, sy%atoms(stypes(b) + 1)%name, sy%atoms(stypes(c) + 1)%name, found, off, hc, sh)
+          if (.not. found) call fail('no atom-atom COUL entry for ' // trim(sy%atoms(stypes(b) + 1)%name) // &
+               ' ' // trim(sy%atoms(stypes(c) + 1)%name))
+          if (off) cycle
+          P%coul_on((b - 1) * nst + c) = 1_c_int8_t
+          if (first) then
+             P%coul_hicut = hc
+             P%coul_shield = sh
+             first = .false.
+          else if (hc /= P%coul_hicut .or. sh /= P%coul_shield) then
+             call fail('plain Coulomb: one HICUT and SHIELDCONST for all site pairs is offered')
+          end if
+       end do
+    end do
+  end subroutine plain_coulomb
 
   ! map file of the sites of one atom type ('all': every site)
   subroutine assign_map(aname, fname, dst)

@@ -75,7 +75,7 @@ module music_inputs
      type(lj_pair), allocatable :: pairs(:)
   end type sim_system
 
-  public :: read_text, strip, lower, upper, find_section, next_nonblank, word, nwords
+  public :: read_text, strip, lower, upper, find_section, next_nonblank, word, nwords, read_coul_pair
   public :: find_file, read_atom, read_molecule, read_atom_atom, read_ewald, lj_calc_AB
   public :: dirname, fail, need, read_system, mol_index, pair_index, principal_axes
 
@@ -443,6 +443,55 @@ contains
       end do
     end function pure_pair
   end subroutine read_atom_atom
+
+  ! atom_atom_file 'a1 a2 COUL [OFF] [HICUT@h] [SHIELDCONST@s]' for one atom
+  ! pair (coulomb_init, coulomb.F90:60-231: locut LOW_DIST_CUTOFF, hicut 0 =
+  ! no cutoff, shield 1).  found = .false. when the file has no such line;
+  ! Q1@/Q2@ (charges from this file), LOCUT@ and TRUNC@ stop the reference
+  ! and are refused here.
+  subroutine read_coul_pair(path, a1, a2, found, off, hicut, shield)
+    character(len=*), intent(in) :: path, a1, a2
+    logical, intent(out) :: found, off
+    real(c_double), intent(out) :: hicut, shield
+    type(text_file) :: t
+    character(len=LL) :: s, tok, key, val
+    integer :: i, w, k
+    found = .false.
+    off = .false.
+    hicut = 0.0d0
+    shield = 1.0d0
+    call read_text(path, t)
+    do i = 1, t%n
+       s = strip(t%line(i))
+       if (nwords(s) < 3) cycle
+       if (upper(trim(word(s, 3))) /= 'COUL') cycle
+       if (.not. ((trim(word(s, 1)) == trim(a1) .and. trim(word(s, 2)) == trim(a2)) .or. &
+                  (trim(word(s, 1)) == trim(a2) .and. trim(word(s, 2)) == trim(a1)))) cycle
+       found = .true.
+       do w = 4, nwords(s)
+          tok = word(s, w)
+          k = index(tok, '@')
+          if (k > 0) then
+             key = upper(tok(1:k - 1))
+             val = tok(k + 1:)
+          else
+             key = upper(tok)
+             val = ''
+          end if
+          select case (trim(key))
+          case ('OFF')
+             off = .true.
+          case ('HICUT')
+             read(val, *) hicut
+          case ('SHIELDCONST')
+             read(val, *) shield
+          case default
+             call fail('atom-atom COUL option not offered: ' // trim(s))
+          end select
+       end do
+       return
+    end do
+  end subroutine read_coul_pair
 
   ! spc_spc_file: 's1 s2 COUL SUM EWALD KAPPA@k KMAX@n LOCUT@l [SFACTOR] ...'
   function read_ewald(path, s1, s2) result(e)

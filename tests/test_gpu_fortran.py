@@ -220,6 +220,30 @@ def test_music_gcmc_bent_sorbate_ewald(hip):
         assert b32[-1]["moves"]["ROTATE"][1] > 0
 
 
+def test_music_gcmc_bent_sorbate_plain_coulomb(hip):
+    """bent.ctr with the sorbate electrostatics switched to plain Coulomb
+    (spc_spc 'COUL BASIC', atom_atom 'COUL HICUT@9.0' lines): the driver
+    against the oracle chain with the same Coulomb term, move for move."""
+    import shutil
+    print("case: music_gcmc bent sorbate with plain Coulomb, FP64", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        shutil.copytree(MG.FIX, os.path.join(td, "f"), copy_function=shutil.copyfile)
+        d = os.path.join(td, "f")
+        with open(os.path.join(d, "spc_spc_bent"), "w") as f:
+            f.write("Bent     Bent     NCOUL BASIC LJ FAST\nBent     Bent     COUL  BASIC FAST\n")
+        with open(os.path.join(d, "atom_atom_file"), "a") as f:
+            f.write("Ow       Ow     COUL HICUT@9.0\nHw       Hw     COUL HICUT@9.0\nOw       Hw     COUL HICUT@9.0\n")
+        blocks, ends = MG.parse(MG.run([os.path.join(d, "bent.ctr")], td))
+        spec = MG.bent_spec()
+        spec.pop("ewald")
+        spec["coul"] = dict(on=[1] * 4, hicut=9.0, locut=1.0e-3, shield=1.0)
+        g = O.Gcmc(spec, 5000.0, 424242)
+        _compare(blocks, _oracle_blocks(g, 2000, every=500, block=250))
+        assert abs(ends[1]["stored"] - g.st.energy * 4.184) < 1.5e-3
+        assert ends[1]["dev"] < 1e-8
+        assert blocks[-1]["N"] > 3
+
+
 def test_music_gcmc_biased_moves(hip):
     """gcmc_bias.ctr: BINSERT (bias map Lattice.LJ.map at 300 K) / BDELETE
     for the probe in the charged lattice through the driver, three state
