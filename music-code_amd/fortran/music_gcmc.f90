@@ -44,13 +44,18 @@ program music_gcmc
   use, intrinsic :: iso_c_binding
   use music_hip_mod
   use music_inputs
+  use music_datafile
   implicit none
   type(sim_system) :: sy
   type(mh_gcmc_params), target :: P
   character(len=LL) :: ctrl, arg, line
   character(len=NL) :: spname, mvname(4)
   integer :: i, j, k, a, niter, iprint, iseed, nmicro, npts, bsize, nspc, isp, nmv, nsite, nst, np
-  integer :: maxn, nunit, icrash, simstart
+  integer :: maxn, nunit, icrash, simstart, iconfig
+  character(len=LL) :: confname
+  type(df_contents) :: dfc
+  logical :: dfon
+  integer, allocatable :: dfu(:)
   logical :: chain, check_only, resume
   character(len=LL) :: desc, resname, srcfile
   real(c_double) :: T, wts(4), kappa_param, rcut
@@ -104,9 +109,21 @@ program music_gcmc
   read(strip(sy%t%line(i + 1)), *) niter
   read(strip(sy%t%line(i + 2)), *) iprint
   read(strip(sy%t%line(i + 3)), *) icrash
+  read(strip(sy%t%line(i + 4)), *) iconfig
   read(strip(sy%t%line(i + 5)), *) simstart
   read(strip(sy%t%line(i + 6)), *) iseed
   resname = word(strip(sy%t%line(i + 8)), 1)
+  confname = word(strip(sy%t%line(i + 9)), 1)
+  ! ---- Main Datafile Information (datafile_initContents): the contents of
+  ! the configuration file written every iconfig iterations
+  dfon = .false.
+  i = find_section(sy%t, 'Main Datafile Information')
+  if (i > 0) then
+     call df_parse_contents(sy%t%line(i), dfc, dfon)
+     if (.not. dfon) call fail('unknown field in the Main Datafile Information line: ' // trim(sy%t%line(i)))
+     if (dfc%time) call fail('datafile: the GCMC driver stores no time (the reference stops on "time")')
+     if (dfc%intra_nrg) call fail('datafile: intra_energy is not stored (the reference stops on it)')
+  end if
 
   ! ---- GCMC Information
   i = find_section(sy%t, 'GCMC Information')
@@ -242,6 +259,11 @@ program music_gcmc
        ' locut ', P%coul_locut, ' shield ', P%coul_shield
   if (len_trim(biasf) > 0) write(*, '(a, a, a, f10.3)') '  bias: ', trim(biasf), ' T ', bias_T
   if (len_trim(srcfile) > 0) write(*, '(2a)') '  start configuration: ', trim(srcfile)
+  if (dfon) then
+     write(*, '(a, i0, 2a)') '  configuration files every ', iconfig, ' iterations: ', trim(confname)
+     if (dfc%pair_nrg .and. (sy%nmol > 1 .or. P%ewald /= 0 .or. P%coul /= 0 .or. any(P%site_q(1:nsite) /= 0.0d0))) &
+          call fail('datafile: pair_energy is stored for a single neutral species without framework')
+  end if
   write(*, '(a, i0, 2a)') '  restart files every ', icrash, ' iterations: ', trim(resname)
   if (any(P%move_type(1:nmv) == MH_MV_BINSERT) .neqv. any(P%move_type(1:nmv) == MH_MV_BDELETE)) &
        call fail('BINSERT and BDELETE come as a pair (moves.F90:827-840)')
@@ -340,7 +362,8 @@ contains
     ! events: every print step and the start of its statistics block
     ! (a block holds the last ((s - 1) mod bsize) + 1 steps)
     nev = 0
-    allocate(ev(2 * (niter / max(iprint, 1) + 2) + niter / max(icrash, 1) + 1), blk(niter / max(iprint, 1) + 2))
+    allocate(ev(2 * (niter / max(iprint, 1) + 2) + niter / max(icrash, 1) + niter / max(iconfig, 1) + 2), &
+         blk(niter / max(iprint, 1) + 2))
     n = 0
     do it = iprint, niter, iprint
        s = int(it, 8) * nmicro
@@ -356,6 +379,17 @@ contains
        do it = icrash, niter, icrash
           nev = nev + 1
           ev(nev) = int(it, 8) * nmicro
+       end do
+    end if
+    ! configuration files at every iconfig iterations (music_gcmc.F90:226-228)
+    if (dfon .and. iconfig > 0) then
+       do it = iconfig, niter, iconfig
+          nev = nev + 1
+          ev(nev) = int(it, 8) * nmicro
+       end do
+       allocate(dfu(nr))
+       do r = 1, nr
+          dfu(r) = open_datafile(q0 + r - 1)
        end do
     end if
     total = int(niter, 8) * nmicro
@@ -377,6 +411,13 @@ contains
              call show_stats(k * iprint, s, s - blk(k), snap(:, 0))
           end if
        end do
+       if (dfon .and. iconfig > 0 .and. s > 0) then
+          if (mod(s, int(iconfig, 8) * nmicro) == 0) then
+             do r = 1, nr
+                call write_datafile_config(dfu(r), r - 1)
+             end do
+          end if
+       end if
        if (icrash > 0 .and. s > 0) then
           if (mod(s, int(icrash, 8) * nmicro) == 0) then
              do r = 1, nr
@@ -409,8 +450,60 @@ contains
             'loading (molec/uc):', sN(r) / real(max(nsteps(r), 1_c_int64_t), c_double) / nunit
        write(*, '(a)') '------------------------------------------------------------'
     end do
+    if (allocated(dfu)) then
+       do r = 1, nr
+          close(dfu(r))
+       end do
+       deallocate(dfu)
+    end if
     if (.not. chain .or. q1 == npts) call check(mh_gcmc_destroy(g), 'mh_gcmc_destroy')
   end subroutine run_points
+
+  ! the configuration file of one simulation (datafile_initout as called by
+  ! music_gcmc.F90:171-182): name.<simno + simstart - 1>
+  integer function open_datafile(simno) result(u)
+    integer, intent(in) :: simno
+    character(len=16) :: num
+    character(len=NL), allocatable :: names(:)
+    logical, allocatable :: fixed(:)
+    integer :: k
+    write(num, '(i0)') simno + simstart - 1
+    allocate(names(sy%nmol), fixed(sy%nmol))
+    do k = 1, sy%nmol
+       names(k) = sy%mols(k)%name
+       fixed(k) = k == sy%ifw
+    end do
+    u = df_open(trim(confname) // '.' // trim(num), ctrl, simno, df_siminfo(ctrl), dfc, press(simno), T, 'GCMC', &
+         names, fixed, niter, iconfig)
+  end function open_datafile
+
+  ! one configuration of replica rep: principal site coordinates, the total
+  ! energy (kJ/mol) and, for a single neutral species, its LJ pair energy
+  subroutine write_datafile_config(u, rep)
+    integer, intent(in) :: u, rep
+    integer(c_int) :: nmol
+    real(c_double), allocatable :: com(:,:), ang(:,:), rr(:,:), rp(:,:), pair(:,:), en(:), sN(:)
+    integer(c_int32_t), allocatable :: Nc(:), ix(:), iy(:)
+    integer(c_int64_t), allocatable :: nst(:), att(:,:), acc(:,:)
+    logical, allocatable :: fixed(:)
+    integer :: m, k, npair
+    allocate(com(3, maxn), ang(3, maxn), rr(3, maxn * nsite), rp(3, maxn * nsite))
+    call check(mh_gcmc_get_config(g, int(rep, c_int), nmol, com, ang, rr), 'mh_gcmc_get_config')
+    do m = 1, nmol
+       call toxyz(com(:, m), ang(:, m), rp(:, (m - 1) * nsite + 1:m * nsite))
+    end do
+    allocate(Nc(nrep_now()), ix(nrep_now()), iy(nrep_now()), en(nrep_now()), sN(nrep_now()), nst(nrep_now()))
+    allocate(att(4, nrep_now()), acc(4, nrep_now()))
+    call check(mh_gcmc_get_state(g, Nc, en, sN, nst, att, acc, ix, iy), 'mh_gcmc_get_state')
+    npair = sy%nmol * (sy%nmol + 1) / 2
+    allocate(pair(2, npair), fixed(sy%nmol))
+    pair = 0.0d0
+    pair(1, 1) = en(rep + 1) * 4.184d0
+    do k = 1, sy%nmol
+       fixed(k) = k == sy%ifw
+    end do
+    call df_write_config(u, dfc, sy%nmol, fixed, isp, int(nmol), nsite, rp, pair, en(rep + 1) * 4.184d0)
+  end subroutine write_datafile_config
 
   ! nsteps GCMC steps on every replica, in launches of at most 2^30 steps
   subroutine run_steps(ns)

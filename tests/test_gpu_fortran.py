@@ -310,3 +310,38 @@ def test_music_gcmc_restart_file_and_resume(hip):
         want = gold.split(b"\n")
         assert got[1].strip() == b"10000" and want[1].strip() == b"20000"
         assert got[:1] + got[2:] == want[:1] + want[2:]
+
+
+ICONFIG = ("500000         # Steps between writes to config. file", "%d         # Steps between writes to config. file")
+
+
+def test_music_gcmc_configuration_datafile(hip):
+    """Configuration files (datafile.F90 layout): test007 with a
+    configuration every 1000 steps writes temp.con.1 whose header holds the
+    ctrlfile, whose extra-info lines are datafile_writeExtraInfo's, and whose
+    20 configurations hold the N printed at the same steps, the stored
+    energy, and at step 20000 the molecules of the reference's own restart
+    file (tests/test007-gcmc-CH4/100.res.1) to its print precision."""
+    from musichip import datafile as D
+    print("case: music_gcmc configuration datafile (test007)", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        ctrl = _t007_tree(td, iconfig=(ICONFIG[0], ICONFIG[1] % 1000))
+        blocks, ends = MG.parse(MG.run([ctrl], td))
+        d = D.read(os.path.join(td, "temp.con.1"))
+        txt = open(ctrl).read().split("\n")
+    if txt and txt[-1] == "":
+        txt = txt[:-1]
+    assert [x.rstrip() for x in d["ctrl_lines"]] == [x[:96].rstrip() for x in txt]
+    assert d["simno"] == 1 and d["flags"] == dict(tot_nrg=True, pair_nrg=True, intra_nrg=False, posns=True,
+                                                   vels=False, time=False)
+    assert d["extra"] == D.extra_info_lines(1, 10000.0, 300.0, "GCMC", ["CH4"], [False], 20000, 1000)
+    cf = d["configs"]
+    assert len(cf) == 20 and len(blocks) == 20
+    for b, c in zip(blocks, cf):
+        assert c["species"][0]["nmoles"] == b["N"], b["step"]
+        assert c["pair"][0] == (c["total"], 0.0) and c["kin_intra"][0] == (0.0, 0.0)
+    assert abs(cf[-1]["total"] - ends[1]["stored"]) < 1e-3
+    want = MG.read_restart_rp(os.path.join(GOLD, "test007-gcmc-CH4", "100.res.1"))["CH4"]
+    got = cf[-1]["species"][0]["rp"]
+    assert got.shape == want.shape
+    assert np.abs(got - want).max() < 6e-6
