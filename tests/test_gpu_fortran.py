@@ -345,3 +345,40 @@ def test_music_gcmc_configuration_datafile(hip):
     got = cf[-1]["species"][0]["rp"]
     assert got.shape == want.shape
     assert np.abs(got - want).max() < 6e-6
+
+
+def test_music_post_isotherm_from_driver_datafiles(hip, monkeypatch):
+    """music_post over configuration files music_gcmc wrote: test007 at three
+    pressures (100 .. 10000 kPa, linear spacing as gcmc_genpressures gives
+    for two decades) with a configuration every 1000 steps; the isotherm
+    holds each file's pressure and the mean N over the configurations the
+    skips and blocks keep."""
+    from musichip import datafile as D
+    from musichip import post as P
+    print("case: music_post isotherm over music_gcmc datafiles (test007, 3 pressures)", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        ctrl = _t007_tree(td, iconfig=(ICONFIG[0], ICONFIG[1] % 1000),
+                          npress=("1                    # No. of pressure values",
+                                  "3                    # No. of pressure values"),
+                          fug=("10000             # Fugacity Range (kPa)", "100, 10000        # Fugacity Range (kPa)"))
+        MG.run([ctrl], td)
+        monkeypatch.chdir(td)
+        open("post.ctr", "w").write(
+            "-- Post Processor Information --\nGCMC\ntemp.con\n1, 3\nregen.ctr\npost.out\n20, 0\n"
+            "-- Post : Energy Average Info --\n4\n-- Post : Loading Average Info --\n4\n")
+        run = P.PostRun("post.ctr")
+        run.run()
+        # 20 configurations, the first 4 skipped: blocks of Int(15/4)+1 = 4
+        assert (run.firstconf, run.lastconf) == (5, 20)
+        iso = open("isotherm.CH4").read().split("\n")
+        zs = []
+        for q in range(3):
+            d = D.read("temp.con.%d" % (q + 1))
+            p = float(d["extra"][1].split()[0])
+            assert abs(p - (100.0, 5050.0, 10000.0)[q]) < 1e-9
+            N = np.array([c["species"][0]["nmoles"] for c in d["configs"]], dtype=float)
+            assert len(N) == 20
+            zs.append(N[4:20].mean())
+            assert iso[4 + q] == "%-15s%10s%12.7f" % (P.real2str(p, 15)[:15], "", zs[-1])
+        assert zs[0] < zs[2]
+        assert iso[7].startswith("# Results of a Langmuir Fit")
