@@ -177,57 +177,90 @@ __device__ __forceinline__ void ewald_real_pair64(double kappa, double charge1, 
 //   d3 term  -q (15 (u + g) + (10 k^2 r^2 + 4 k^4 r^4) g) / r^6;
 // the potential goes straight into the FP64 sum (terms cancel strongly),
 // the 7 derivatives accumulate into the FP32 tile sums with FMAs.
-__device__ __forceinline__ void ewald_real_acc32(float kappa, float qq, float sx, float sy, float sz, float r2,
+// Constants of the FP32 real-space pair, formed once per launch from kappa
+// (workgroup-uniform, so they stay in SGPRs): kap2 = kappa^2, cexp =
+// -kappa^2 log2(e) (exp(-k^2 r^2) = exp2(cexp r^2)), pk = p kappa of the
+// A&S 7.1.26 argument, twoksp = 2 kappa / sqrt(pi) with the reference's
+// rounded 1/sqrt(pi) = 0.5641896.
+struct EwC {
+    float kap2, cexp, pk, twoksp, a5;
+};
+
+__device__ __forceinline__ EwC ewc_make(double kappa) {
+    EwC c;
+    c.kap2 = (float)(kappa * kappa);
+    c.cexp = (float)(-kappa * kappa * 1.4426950408889634);
+    c.pk = (float)(0.3275911 * kappa);
+    c.twoksp = (float)(2.0 * 0.5641896 * kappa);
+    c.a5 = 1.061405429f;  // leading A&S coefficient, held in a register (one literal per VOP2)
+    return c;
+}
+
+// q erfc(kr)/r (A&S 7.1.26, as erfc_ref64) and its derivatives at the
+// separation (sx, sy, sz), r2 = |s|^2: the potential added to the FP64 node
+// sum, the 7 derivatives to FP32 tile partials.  With s1 = u + g, g =
+// (2 kappa / sqrt(pi)) q e^{-k^2 r^2}, w = 1/r^2:
+//   d/dx = -s1 w x;  d2/dxdy = (3 s1 + 2 k^2 r^2 g) w^2 x y;
+//   d3/dxdydz = -(15 s1 + (10 + 4 k^2 r^2) k^2 r^2 g) w^3 x y z.
+__device__ __forceinline__ void ewald_real_acc32(const EwC &c, float qq, float sx, float sy, float sz, float r2,
                                                  double &S0, float *hf) {
-#pragma clang fp contract(fast)
-    const float twoksp = 2.0f * 0.5641896f * kappa;
-    const float a1 = 0.254829592f, a2 = -0.284496736f, a3 = 1.421413741f, a4 = -1.453152027f,
-                a5 = 1.061405429f, p = 0.3275911f;
+    const float a1 = 0.254829592f, a2 = -0.284496736f, a3 = 1.421413741f, a4 = -1.453152027f;
     const float ri1 = __frsqrt_rn(r2);
-    const float kr = kappa * (r2 * ri1);
-    const float tt = rcp_fast(1.0f + p * kr);
-    const float ek = __expf(-kr * kr);
+    const float r = r2 * ri1;
+    const float tt = rcp_fast(fmaf(c.pk, r, 1.0f));
+    const float k2r2 = c.kap2 * r2;
+    const float ek = __builtin_amdgcn_exp2f(c.cexp * r2);
     const float qek = qq * ek;
-    const float u = tt * (a1 + tt * (a2 + tt * (a3 + tt * (a4 + tt * a5)))) * qek * ri1;  // q erfc(kr)/r
-    const float g = twoksp * qek;
-    const float w = ri1 * ri1;
-    const float k2r2 = kr * kr;
-    const float s1 = u + g;
+    const float poly = tt * fmaf(tt, fmaf(tt, fmaf(tt, fmaf(tt, c.a5, a4), a3), a2), a1);
+    const float pu = poly * qek;
+    const float u = pu * ri1;  // q erfc(kr)/r
+    const float g = c.twoksp * qek;
+    const float s1 = fmaf(pu, ri1, g);
     S0 += (double)u;
+    const float w = ri1 * ri1;
     const float t1 = -s1 * w;
-    hf[0] += sx * t1;
-    hf[1] += sy * t1;
-    hf[2] += sz * t1;
+    hf[0] = fmaf(sx, t1, hf[0]);
+    hf[1] = fmaf(sy, t1, hf[1]);
+    hf[2] = fmaf(sz, t1, hf[2]);
     const float w2 = w * w;
-    const float t2 = (3.0f * s1 + 2.0f * k2r2 * g) * w2;
+    const float gk = g * k2r2;
+    const float t2 = fmaf(2.0f, gk, 3.0f * s1) * w2;
+    const float sxy = sx * sy;
+    hf[3] = fmaf(t2, sxy, hf[3]);
     const float sxt2 = sx * t2;
-    hf[3] += sxt2 * sy;
-    hf[4] += sxt2 * sz;
-    hf[5] += t2 * sy * sz;
-    const float t3 = -(15.0f * s1 + (10.0f + 4.0f * k2r2) * k2r2 * g) * (w2 * w);
-    hf[6] += t3 * sx * sy * sz;
+    hf[4] = fmaf(sxt2, sz, hf[4]);
+    const float syt2 = sy * t2;
+    hf[5] = fmaf(syt2, sz, hf[5]);
+    const float t3 = fmaf(fmaf(4.0f, k2r2, 10.0f), gk, 15.0f * s1) * (w2 * w);
+    const float t3xy = t3 * sxy;
+    hf[6] = fmaf(-t3xy, sz, hf[6]);
 }
 
 // FP32 LJ pair with derivatives
 __device__ __forceinline__ void lj_pair32(float A, float B, float sx, float sy, float sz, float r2,
                                           float &pot, float *h) {
-#pragma clang fp contract(fast)
-    float r2i = rcp_fast(r2);
-    float r6i = r2i * r2i * r2i;
-    float r12i = r6i * r6i;
-    pot += A * r12i - B * r6i;
-    float fac1 = (12.0f * A * r12i - 6.0f * B * r6i) * r2i;
-    h[0] -= sx * fac1;
-    h[1] -= sy * fac1;
-    h[2] -= sz * fac1;
-    float r10i = r12i * r2;
-    float r16i = r10i * r6i;
-    float f2 = 168.0f * A * r16i - 48.0f * B * r10i;
-    h[3] += f2 * sx * sy;
-    h[4] += f2 * sx * sz;
-    h[5] += f2 * sy * sz;
-    float f3 = (-2688.0f * A * r16i + 480.0f * B * r10i) * r2i;
-    h[6] += f3 * sx * sy * sz;
+    // with X = A/r^12, Y = B/r^6 and w = 1/r^2:
+    //   V = X - Y;  dV/dx = -(12 X - 6 Y) w x;
+    //   d2V/dxdy = (168 X - 48 Y) w^2 x y;  d3V/dxdydz = (480 Y - 2688 X) w^3 x y z
+    const float w = rcp_fast(r2);
+    const float w2 = w * w;
+    const float r6i = w2 * w;
+    const float X = A * (r6i * r6i);
+    const float Y = B * r6i;
+    pot = (pot + X) - Y;
+    const float fac1 = fmaf(12.0f, X, -6.0f * Y) * w;
+    h[0] = fmaf(-sx, fac1, h[0]);
+    h[1] = fmaf(-sy, fac1, h[1]);
+    h[2] = fmaf(-sz, fac1, h[2]);
+    const float f2 = fmaf(168.0f, X, -48.0f * Y) * w2;
+    const float f2x = f2 * sx;
+    h[3] = fmaf(f2x, sy, h[3]);
+    h[4] = fmaf(f2x, sz, h[4]);
+    const float f2y = f2 * sy;
+    h[5] = fmaf(f2y, sz, h[5]);
+    const float f3 = fmaf(-2688.0f, X, 480.0f * Y) * (w2 * w);
+    const float f3x = f3 * sx;
+    h[6] = fmaf(f3x * sy, sz, h[6]);
 }
 
 __device__ __forceinline__ void kahan_add(double &s, double &c, double v) {
@@ -456,7 +489,7 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
     }
     const float dx0 = (float)(px - cen[0]), dy0 = (float)(py - cen[1]), dz0 = (float)(pz - cen[2]);
     const float r2safe = (float)g.r2safe;
-    const float kappaf = (float)g.kappa;
+    const EwC ewc = ewc_make(g.kappa);
     const float Lx = (float)g.full[0], Ly = (float)g.full[1], Lz = (float)g.full[2];
     const bool refold = g.refold != 0;
     __syncthreads();
@@ -537,7 +570,8 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
                         sy = foldf(sy, Ly);
                         sz = foldf(sz, Lz);
                     }
-                    float r2 = sx * sx + sy * sy + sz * sz;
+                    // Ewald: fused (the cutoff is smooth); LJ: the rounding its hicut band assumes
+                    const float r2 = (KIND == K_EWALD) ? fmaf(sx, sx, fmaf(sy, sy, sz * sz)) : sx * sx + sy * sy + sz * sz;
                     if (KIND == K_LJ) {
                         const float2 hb = s_hb[j];
                         if (r2 > hb.y) continue;
@@ -551,9 +585,13 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
                             double ey = fold1((double)y * g.step[1] - xyz[3 * a + 1], g.full[1]);
                             double ez = fold1((double)z * g.step[2] - xyz[3 * a + 2], g.full[2]);
                             const Near o = near_lj64(tab.A[t], tab.B[t], tab.h2[t], tab.l2[t], ex, ey, ez);
-                            if (!o.ok) { failed = true; break; }
-                            S0 += o.v[0];
-                            for (int i = 0; i < 7; i++) s_acc[i][tid] += o.v[i + 1];
+                            // no early exit from the tile: a failed node is VERY_LRG_NRG and its
+                            // derivative columns (unlike the FP64 kernel's) sum every other pair
+                            failed = failed || !o.ok;
+                            if (o.ok) {
+                                S0 += o.v[0];
+                                for (int i = 0; i < 7; i++) s_acc[i][tid] += o.v[i + 1];
+                            }
                             continue;
                         }
                         lj_pair32(a4.w, b4.w, sx, sy, sz, r2, pf, hf);
@@ -566,12 +604,16 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
                             double ey = fold1((double)y * g.step[1] - xyz[3 * a + 1], g.full[1]);
                             double ez = fold1((double)z * g.step[2] - xyz[3 * a + 2], g.full[2]);
                             const Near o = near_ewald64(g.kappa, g.charge1, qv[a], ex, ey, ez);
-                            if (!o.ok) { failed = true; break; }
-                            S0 += o.v[0];
-                            for (int i = 0; i < 7; i++) s_acc[i][tid] += o.v[i + 1];
+                            // no early exit from the tile: a failed node is VERY_LRG_NRG and its
+                            // derivative columns (unlike the FP64 kernel's) sum every other pair
+                            failed = failed || !o.ok;
+                            if (o.ok) {
+                                S0 += o.v[0];
+                                for (int i = 0; i < 7; i++) s_acc[i][tid] += o.v[i + 1];
+                            }
                             continue;
                         }
-                        ewald_real_acc32(kappaf, a4.w, sx, sy, sz, r2, S0, hf);
+                        ewald_real_acc32(ewc, a4.w, sx, sy, sz, r2, S0, hf);
                     }
                 }
                 if (KIND == K_LJ) S0 += (double)pf;
