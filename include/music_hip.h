@@ -210,6 +210,13 @@ typedef struct {
     int coul;
     uint8_t coul_on[64];                 /* [ntypes][ntypes] */
     double coul_hicut, coul_locut, coul_shield;
+    /* adjustable displacements ("adjust delta" = 1 on the RTRANSLATE /
+     * RROTATE lines): after every attempt of the move its displacement is
+     * scaled by 0.95 while the move's cumulative acceptance ratio is below
+     * 0.49 and by 1.05 while above 0.51, within [0.01, 2.0]
+     * (rigidmoves_adjustdeltatrans / _adjustdeltarot, rigidmoves.F90:2586-2617,
+     * called from mcmoves_perturb, mcmoves.F90:460-461).  Per replica. */
+    int adjust_trans, adjust_rot;
 } mh_gcmc_params;
 
 typedef struct mh_gcmc mh_gcmc;
@@ -254,6 +261,11 @@ int mh_gcmc_get_moments(mh_gcmc *g, double *sumN, double *sumN2);
 /* Restore a checkpoint: N and RNG state per replica (the configuration via
  * mh_gcmc_set_config).  The reference does not save the RNG state. */
 int mh_gcmc_set_rng(mh_gcmc *g, int rep, int32_t ix, int32_t iy);
+/* Displacements in use per replica, delta[nrep][2] = (translate A, rotate
+ * rad): the params' values, changed by adjustable moves. */
+int mh_gcmc_get_deltas(mh_gcmc *g, double *delta);
+/* Restore a replica's displacements (a checkpoint of adjustable moves). */
+int mh_gcmc_set_deltas(mh_gcmc *g, int rep, double delta_trans, double delta_rot);
 int mh_gcmc_get_config(mh_gcmc *g, int rep, int *n, double *com, double *ang, double *r);
 /* Full FP64 recompute of every replica's energy (interact_checknrgs); with
  * refresh != 0 the structure factors and running energies are reset to it. */

@@ -65,6 +65,7 @@ struct GDev {
     int mtype[4];
     double cumwt[4];
     double insdel, dtrans, drot, maxnrg, rti, molconst;
+    int adj_trans, adj_rot;  // adjustable displacements (rigidmoves_adjustdelta*)
     double inv_ns, inv_nss;  // 1/ns, 1/ns^2 for the pair-item decomposition
     BiasRef bias;            // BINSERT / BDELETE
     int ncell[3];            // unit cells per axis of the simulation cell
@@ -74,12 +75,22 @@ struct GDev {
     int use_exy, exy_off;
 };
 
+// rigidmoves_adjustdeltatrans / _adjustdeltarot (rigidmoves.F90:2586-2617):
+// steer the cumulative acceptance ratio of the move towards 1/2, the
+// displacement kept within [0.01, 2.0] (MIN/MAX_TRANS_DELTA, :216-217)
+__device__ __forceinline__ double adjust_delta(double d, double ratio) {
+    if (ratio < 0.49) return fmax(__dmul_rn(d, 0.95), 0.01);
+    if (ratio > 0.51) return fmin(__dmul_rn(d, 1.05), 2.0);
+    return d;
+}
+
 struct Rep {
     int *N, *err;
     int32_t *ix, *iy, *first, *iseed;
     double *B, *energy, *sumN, *sumN2;
     long long *nsteps, *att, *acc;
     double *com, *ang, *pos;  // [rep][maxN][3], [rep][maxN][3], [rep][maxN][ns][3]
+    double *delta;            // [rep][2] translate / rotate displacement in use
     double2 *S, *dS;          // [rep][nk]
     int *tr_move, *tr_acc, *tr_N;
     double *tr_dE;
@@ -642,6 +653,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
         att[i] = R.att[4 * rep + i];
         acc[i] = R.acc[4 * rep + i];
     }
+    double dtr = R.delta[2 * rep], drt = R.delta[2 * rep + 1];
     const double B = R.B[rep];
     double *gcom = R.com + (long long)rep * P.maxN * 3;
     double *gang = R.ang + (long long)rep * P.maxN * 3;
@@ -763,13 +775,13 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                 }
                 if (mtype == MH_MV_TRANSLATE) {
                     for (int i = 0; i < 3; i++) {
-                        double dv = __dmul_rn(__dsub_rn(__dmul_rn(rranf(rng), 2.0), 1.0), P.dtrans);
+                        double dv = __dmul_rn(__dsub_rn(__dmul_rn(rranf(rng), 2.0), 1.0), dtr);
                         tcom[i] = __dadd_rn(tcom[i], dv);
                     }
                 } else if (mtype == MH_MV_ROTATE) {
-                    double dphi = __dmul_rn(__dsub_rn(__dmul_rn(rranf(rng), 2.0), 1.0), P.drot);
-                    double dpsi = __dmul_rn(__dsub_rn(__dmul_rn(rranf(rng), 2.0), 1.0), P.drot);
-                    double dcos = __dmul_rn(__dsub_rn(__dmul_rn(rranf(rng), 2.0), 1.0), P.drot);
+                    double dphi = __dmul_rn(__dsub_rn(__dmul_rn(rranf(rng), 2.0), 1.0), drt);
+                    double dpsi = __dmul_rn(__dsub_rn(__dmul_rn(rranf(rng), 2.0), 1.0), drt);
+                    double dcos = __dmul_rn(__dsub_rn(__dmul_rn(rranf(rng), 2.0), 1.0), drt);
                     if (ns != 1) {  // rigidmoves_rotperturb (rigidmoves.F90:1690-1707)
                         double phi = __dadd_rn(tang[0], dphi);
                         double psi = __dadd_rn(tang[2], dpsi);
@@ -894,6 +906,11 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                         de = dU;
                     }
                 }
+                // moves_adjustParams after every attempt (mcmoves.F90:460-461)
+                if (mtype == MH_MV_TRANSLATE && P.adj_trans)
+                    dtr = adjust_delta(dtr, (double)acc[2] / (double)att[2]);
+                else if (mtype == MH_MV_ROTATE && P.adj_rot)
+                    drt = adjust_delta(drt, (double)acc[3] / (double)att[3]);
             }
         }
         if (tid == 0 && R.tr_cap > 0) {
@@ -943,6 +960,8 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
             R.att[4 * rep + i] = att[i];
             R.acc[4 * rep + i] = acc[i];
         }
+        R.delta[2 * rep] = dtr;
+        R.delta[2 * rep + 1] = drt;
     }
 }
 
@@ -1100,7 +1119,7 @@ struct mh_gcmc {
     DBuf dP;                    // GDev[sys.size()]
     DBuf sysidx, budget;        // pool only
     DBuf N, err, ix, iy, first, iseed, B, energy, sumN, sumN2, nsteps, att, acc;
-    DBuf com, ang, pos, S, dS;
+    DBuf com, ang, pos, S, dS, delta;
     DBuf tr_move, tr_acc, tr_N, tr_dE;
     int tr_cap = 0;
     long long tr_base = 0;
@@ -1134,6 +1153,7 @@ struct mh_gcmc {
         r.com = com.as<double>();
         r.ang = ang.as<double>();
         r.pos = pos.as<double>();
+        r.delta = delta.as<double>();
         r.S = S.as<double2>();
         r.dS = dS.as<double2>();
         r.tr_move = tr_move.as<int>();
@@ -1411,6 +1431,8 @@ static int build_system(mh_ctx *ctx, const mh_gcmc_params *p, int maxN, const do
     H.insdel = (wi > 0.0 && wd > 0.0) ? wi / wd : 1.0;  // mcmoveset.F90:424-430
     H.dtrans = p->delta_trans;
     H.drot = p->delta_rot;
+    H.adj_trans = p->adjust_trans != 0;
+    H.adj_rot = p->adjust_rot != 0;
     H.maxnrg = p->max_nrg;
     H.rti = 1.0 / ((8.314e-3 / 4.184) * p->temperature);
     H.prec = p->precision;
@@ -1437,7 +1459,7 @@ static int alloc_replicas(mh_gcmc *g) {
         (rc = g->iy.alloc(4 * R)) || (rc = g->first.alloc(4 * R)) || (rc = g->iseed.alloc(4 * R)) ||
         (rc = g->B.alloc(8 * R)) || (rc = g->energy.alloc(8 * R)) || (rc = g->sumN.alloc(8 * R)) ||
         (rc = g->sumN2.alloc(8 * R)) || (rc = g->nsteps.alloc(8 * R)) || (rc = g->att.alloc(32 * R)) ||
-        (rc = g->acc.alloc(32 * R)) ||
+        (rc = g->acc.alloc(32 * R)) || (rc = g->delta.alloc(16 * R)) ||
         (rc = g->com.alloc(sizeof(double) * 3 * R * maxN)) || (rc = g->ang.alloc(sizeof(double) * 3 * R * maxN)) ||
         (rc = g->pos.alloc(sizeof(double) * 3 * R * maxN * g->ns)) ||
         (rc = g->S.alloc(sizeof(double2) * (R * g->kstride + 1))) ||
@@ -1453,7 +1475,7 @@ static int alloc_replicas(mh_gcmc *g) {
 // fresh chain state of replicas [r0, r0 + n): N = 0, RNG from seeds, B from
 // the fugacity and the system volume (gcmc_genfuglist, gcmc.F90:303-319)
 static int init_replicas(mh_gcmc *g, const std::vector<int> &reps, const double *fug, const int32_t *seeds,
-                         double volume, double T) {
+                         double volume, double T, double dtrans, double drot) {
     const double sivolume = volume * 1.0e-30 * 6.0221367e23;
     for (size_t i = 0; i < reps.size(); i++) {
         MH_CHECK(fug[i] > 0.0, MH_EINVAL, "gcmc: fugacity must be > 0");
@@ -1474,6 +1496,8 @@ static int init_replicas(mh_gcmc *g, const std::vector<int> &reps, const double 
         MH_HIP(hipMemcpy(g->nsteps.as<long long>() + r, &zl, 8, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->att.as<long long>() + 4 * r, zl4, 32, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->acc.as<long long>() + 4 * r, zl4, 32, hipMemcpyHostToDevice));
+        const double d2[2] = {dtrans, drot};
+        MH_HIP(hipMemcpy(g->delta.as<double>() + 2 * r, d2, 16, hipMemcpyHostToDevice));
         if (g->kstride)
             MH_HIP(hipMemset(g->S.as<double2>() + (long long)r * g->kstride, 0, sizeof(double2) * g->kstride));
     }
@@ -1505,7 +1529,8 @@ int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double 
     if ((rc = alloc_replicas(g.get()))) return rc;
     std::vector<int> reps(nrep);
     for (int i = 0; i < nrep; i++) reps[i] = i;
-    if ((rc = init_replicas(g.get(), reps, fugacity_kPa, seeds, g->sys[0].volume, p->temperature))) return rc;
+    if ((rc = init_replicas(g.get(), reps, fugacity_kPa, seeds, g->sys[0].volume, p->temperature, p->delta_trans,
+                                p->delta_rot))) return rc;
     if ((rc = plan(g.get()))) return rc;
     ctx_retain(ctx);
     *out = g.release();
@@ -1601,7 +1626,7 @@ int mh_pool_admit(mh_gcmc *g, const mh_gcmc_params *p, const double fund_ell[3],
         S = SysHost();
         return rc;
     }
-    if ((rc = init_replicas(g, reps, fugacity_kPa, seeds, S.volume, p->temperature))) {
+    if ((rc = init_replicas(g, reps, fugacity_kPa, seeds, S.volume, p->temperature, p->delta_trans, p->delta_rot))) {
         S = SysHost();
         return rc;
     }
@@ -1817,6 +1842,24 @@ int mh_gcmc_set_rng(mh_gcmc *g, int rep, int32_t ix, int32_t iy) {
     return MH_OK;
 }
 
+int mh_gcmc_get_deltas(mh_gcmc *g, double *delta) {
+    MH_CHECK(g && delta, MH_EINVAL, "mh_gcmc_get_deltas: bad arguments");
+    MH_HIP(hipSetDevice(g->ctx->device));
+    MH_HIP(hipStreamSynchronize(g->ctx->stream));
+    MH_HIP(hipMemcpy(delta, g->delta.p, 16 * (size_t)g->nrep, hipMemcpyDeviceToHost));
+    return MH_OK;
+}
+
+int mh_gcmc_set_deltas(mh_gcmc *g, int rep, double delta_trans, double delta_rot) {
+    MH_CHECK(g && rep >= 0 && rep < g->nrep, MH_EINVAL, "mh_gcmc_set_deltas: bad replica");
+    MH_CHECK(delta_trans >= 0.0 && delta_rot >= 0.0, MH_EINVAL, "mh_gcmc_set_deltas: negative displacement");
+    MH_HIP(hipSetDevice(g->ctx->device));
+    MH_HIP(hipStreamSynchronize(g->ctx->stream));
+    const double d2[2] = {delta_trans, delta_rot};
+    MH_HIP(hipMemcpy(g->delta.as<double>() + 2 * rep, d2, 16, hipMemcpyHostToDevice));
+    return MH_OK;
+}
+
 int mh_gcmc_get_config(mh_gcmc *g, int rep, int *n, double *com, double *ang, double *r) {
     MH_CHECK(g && rep >= 0 && rep < g->nrep && n, MH_EINVAL, "mh_gcmc_get_config: bad arguments");
     MH_HIP(hipSetDevice(g->ctx->device));
@@ -1929,7 +1972,8 @@ int mh_gcmc_autotune(mh_gcmc *g, int probe_steps, int *threads_out, int *exy_out
     hipStream_t st = g->ctx->stream;
     // snapshot of every buffer a run modifies
     std::vector<DBuf *> bufs = {&g->N, &g->err, &g->ix, &g->iy, &g->first, &g->iseed, &g->energy, &g->sumN,
-                                &g->sumN2, &g->nsteps, &g->att, &g->acc, &g->com, &g->ang, &g->pos, &g->S};
+                                &g->sumN2, &g->nsteps, &g->att, &g->acc, &g->com, &g->ang, &g->pos, &g->S,
+                                &g->delta};
     if (g->pool) bufs.push_back(&g->budget);
     const int nb = (int)bufs.size();
     std::vector<DBuf> snap(nb);

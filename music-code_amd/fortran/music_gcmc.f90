@@ -166,11 +166,11 @@ program music_gcmc
         P%move_type(k) = MH_MV_DELETE
      case ('RTRANSLATE', 'TRANSLATE')
         P%move_type(k) = MH_MV_TRANSLATE
-        call read_delta(i, P%delta_trans)
+        call read_delta(i, P%delta_trans, P%adjust_trans)
         i = i + 1
      case ('RROTATE', 'ROTATE')
         P%move_type(k) = MH_MV_ROTATE
-        call read_delta(i, P%delta_rot)
+        call read_delta(i, P%delta_rot, P%adjust_rot)
         i = i + 1
      case ('BINSERT')
         ! bias pmap file (MAP_BIAS), then the bias temperature; cavity bias
@@ -526,12 +526,13 @@ contains
     real(c_double), intent(in) :: sblk(:)
     integer :: r, nr, k, mt
     integer(c_int32_t), allocatable :: Nc(:), ix(:), iy(:)
-    real(c_double), allocatable :: en(:), sN(:), sN2(:)
+    real(c_double), allocatable :: en(:), sN(:), sN2(:), dl(:,:)
     integer(c_int64_t), allocatable :: nst(:), att(:,:), acc(:,:)
     real(c_double) :: cum, dev, blkavg, ratio
     character(len=32) :: lab
     nr = size(sblk)
-    allocate(Nc(nr), ix(nr), iy(nr), en(nr), sN(nr), sN2(nr), nst(nr), att(4, nr), acc(4, nr))
+    allocate(Nc(nr), ix(nr), iy(nr), en(nr), sN(nr), sN2(nr), nst(nr), att(4, nr), acc(4, nr), dl(2, nr))
+    call check(mh_gcmc_get_deltas(g, dl), 'mh_gcmc_get_deltas')
     call check(mh_gcmc_get_state(g, Nc, en, sN, nst, att, acc, ix, iy), 'mh_gcmc_get_state')
     call check(mh_gcmc_get_moments(g, sN, sN2), 'mh_gcmc_get_moments')
     do r = 1, nr
@@ -548,10 +549,10 @@ contains
           lab = trim(basic_tag(P%move_type(k))) // ' (' // trim(mvname(k)) // ')'
           if (P%move_type(k) == MH_MV_TRANSLATE) then
              write(*, '(6x, a26, a, f5.3, 3x, a, i0, a, i0, a, f6.3)') lab, 'Acc. Ratio: ', ratio, '(', acc(mt, r), &
-                  ' of ', att(mt, r), ')  Max disp: ', P%delta_trans
+                  ' of ', att(mt, r), ')  Max disp: ', dl(1, r)
           else if (P%move_type(k) == MH_MV_ROTATE) then
              write(*, '(6x, a26, a, f5.3, 3x, a, i0, a, i0, a, f6.3)') lab, 'Acc. Ratio: ', ratio, '(', acc(mt, r), &
-                  ' of ', att(mt, r), ')  Max disp: ', P%delta_rot
+                  ' of ', att(mt, r), ')  Max disp: ', dl(2, r)
           else
              write(*, '(6x, a26, a, f5.3, 3x, a, i0, a, i0, a)') lab, 'Acc. Ratio: ', ratio, '(', acc(mt, r), &
                   ' of ', att(mt, r), ')'
@@ -611,7 +612,7 @@ contains
     integer, intent(in) :: rep
     integer :: u, n, m, b, dof
     integer(c_int) :: nmol
-    real(c_double), allocatable :: com(:,:), ang(:,:), rr(:,:), rp(:,:)
+    real(c_double), allocatable :: com(:,:), ang(:,:), rr(:,:), rp(:,:), dl(:,:)
     integer(c_int32_t), allocatable :: Nc(:), ix(:), iy(:)
     real(c_double), allocatable :: en(:), sN(:)
     integer(c_int64_t), allocatable :: nst(:), att(:,:), acc(:,:)
@@ -655,6 +656,9 @@ contains
     do m = 1, n
        write(u, '(6es25.17e3)') com(:, m), ang(:, m)
     end do
+    allocate(dl(2, nrep_now()))
+    call check(mh_gcmc_get_deltas(g, dl), 'mh_gcmc_get_deltas')
+    write(u, '(2es25.17e3)') dl(:, rep + 1)
     close(u)
   end subroutine write_restart
 
@@ -700,7 +704,7 @@ contains
     type(text_file) :: rf
     integer :: ii, nm, na, m, r, u, ios
     real(c_double), allocatable :: com(:,:), ang(:,:)
-    real(c_double) :: v(7)
+    real(c_double) :: v(7), dres(2)
     integer(c_int32_t) :: rx, ry
     character(len=LL) :: rng
     call read_text(srcfile, rf)
@@ -740,11 +744,14 @@ contains
        do m = 1, nm
           read(u, *) com(:, m), ang(:, m)
        end do
+       read(u, *, iostat=ios) dres
+       if (ios /= 0) dres = [P%delta_trans, P%delta_rot]
        close(u)
     end if
     do r = 1, nr
        call check(mh_gcmc_set_config(gb, int(r - 1, c_int), int(nm, c_int), com, ang), 'mh_gcmc_set_config')
        if (resume) call check(mh_gcmc_set_rng(gb, int(r - 1, c_int), rx, ry), 'mh_gcmc_set_rng')
+       if (resume) call check(mh_gcmc_set_deltas(gb, int(r - 1, c_int), dres(1), dres(2)), 'mh_gcmc_set_deltas')
     end do
     write(*, '(a, i0, 2a)') '  started from ', nm, ' molecules of ', trim(srcfile)
   contains
@@ -843,15 +850,17 @@ contains
 
   ! 'delta, adjust' of a translate / rotate move; adjustable steps are not
   ! offered
-  subroutine read_delta(ii, d)
+  ! displacement and its "adjust delta" flag (0 = fixed, 1 = adjusted
+  ! towards a 50 % acceptance after every attempt, rigidmoves.F90:2586-2617)
+  subroutine read_delta(ii, d, adj)
     integer, intent(in) :: ii
     real(c_double), intent(out) :: d
-    integer :: adj
+    integer(c_int), intent(out) :: adj
     line = strip(sy%t%line(ii))
     read(word(line, 1), *) d
     adj = 0
     if (nwords(line) > 1) read(word(line, 2), *) adj
-    if (adj /= 0) call fail('adjustable move sizes are not offered (adjust flag must be 0)')
+    if (adj /= 0 .and. adj /= 1) call fail('adjust delta flag must be 0 or 1')
   end subroutine read_delta
 
   ! species-species file entries of the GCMC species

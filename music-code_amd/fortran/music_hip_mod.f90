@@ -48,6 +48,7 @@ module music_hip_mod
      integer(c_int) :: coul = 0
      integer(c_int8_t) :: coul_on(64) = 0_c_int8_t
      real(c_double) :: coul_hicut = 0.0d0, coul_locut = 1.0d-3, coul_shield = 1.0d0
+     integer(c_int) :: adjust_trans = 0, adjust_rot = 0
   end type mh_gcmc_params
 
   public :: mh_error_message
@@ -56,7 +57,7 @@ module music_hip_mod
   public :: mh_map_load, mh_map_interp
   public :: mh_gcmc_create, mh_gcmc_destroy, mh_gcmc_set_threads, mh_gcmc_autotune
   public :: mh_gcmc_set_config, mh_gcmc_run, mh_gcmc_run_cycles, mh_gcmc_get_state, mh_gcmc_set_rng, &
-            mh_gcmc_get_moments
+            mh_gcmc_get_moments, mh_gcmc_get_deltas, mh_gcmc_set_deltas
   public :: mh_gcmc_get_config, mh_gcmc_total_energy, mh_trial_energy
   public :: mh_bmap_create, mh_bmap_destroy, mh_bmap_info
 
@@ -265,6 +266,21 @@ module music_hip_mod
        integer(c_int32_t), value :: ix, iy
        integer(c_int) :: rc
      end function mh_gcmc_set_rng
+
+     function mh_gcmc_get_deltas(g, delta) bind(C, name="mh_gcmc_get_deltas") result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       real(c_double) :: delta(*)
+       integer(c_int) :: rc
+     end function mh_gcmc_get_deltas
+
+     function mh_gcmc_set_deltas(g, rep, dtrans, drot) bind(C, name="mh_gcmc_set_deltas") result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: g
+       integer(c_int), value :: rep
+       real(c_double), value :: dtrans, drot
+       integer(c_int) :: rc
+     end function mh_gcmc_set_deltas
 
      function mh_gcmc_get_config(g, rep, n, com, ang, r) bind(C, name="mh_gcmc_get_config") result(rc)
        import :: c_int, c_ptr, c_double

@@ -27,7 +27,8 @@ class GcmcParams(ctypes.Structure):
                 ("ewald_rcut", _d), ("temperature", _d), ("nmoves", _i), ("move_type", _i * 4),
                 ("move_weight", _d * 4), ("delta_trans", _d), ("delta_rot", _d), ("max_nrg", _d),
                 ("precision", _i), ("bias", _vp), ("coul", _i), ("coul_on", ctypes.c_uint8 * 64),
-                ("coul_hicut", _d), ("coul_locut", _d), ("coul_shield", _d)]
+                ("coul_hicut", _d), ("coul_locut", _d), ("coul_shield", _d),
+                ("adjust_trans", _i), ("adjust_rot", _i)]
 
 
 def _sig():
@@ -44,6 +45,8 @@ def _sig():
         "mh_gcmc_run_cycles": [_vp, _i, _i],
         "mh_gcmc_get_state": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "mh_gcmc_set_rng": [_vp, _i, ctypes.c_int32, ctypes.c_int32],
+        "mh_gcmc_get_deltas": [_vp, _vp],
+        "mh_gcmc_set_deltas": [_vp, _i, _d, _d],
         "mh_gcmc_get_moments": [_vp, _vp, _vp],
         "mh_gcmc_get_config": [_vp, _i, P(_i), _vp, _vp, _vp],
         "mh_gcmc_total_energy": [_vp, _vp, _i],
@@ -107,6 +110,8 @@ def make_params(spec, maps=None, precision=H.MH_PREC_FP64):
         p.move_weight[i] = w
     p.delta_trans = spec.get("dtrans", 0.0)
     p.delta_rot = spec.get("drot", 0.0)
+    p.adjust_trans = int(spec.get("adjust_trans", 0))
+    p.adjust_rot = int(spec.get("adjust_rot", 0))
     p.max_nrg = spec.get("max_nrg", C.VERY_LRG_NRG)
     p.precision = precision
     bias = spec.get("bias_map")  # a hip.BiasMap for BINSERT / BDELETE
@@ -174,6 +179,15 @@ class GcmcBatch:
         sN2 = np.zeros(R)
         H.check(_sig().mh_gcmc_get_moments(self.h, None, _p(sN2)), "mh_gcmc_get_moments")
         return dict(N=N, energy=e, sumN=sN, sumN2=sN2, nsteps=ns, att=att, acc=acc, ix=ix, iy=iy)
+
+    def deltas(self):
+        """Displacements in use, [nrep][2] = (translate A, rotate rad)."""
+        d = np.zeros((self.nrep, 2))
+        H.check(_sig().mh_gcmc_get_deltas(self.h, _p(d)), "mh_gcmc_get_deltas")
+        return d
+
+    def set_deltas(self, rep, dtrans, drot):
+        H.check(_sig().mh_gcmc_set_deltas(self.h, rep, float(dtrans), float(drot)), "mh_gcmc_set_deltas")
 
     def set_rng(self, rep, ix, iy):
         H.check(_sig().mh_gcmc_set_rng(self.h, rep, int(ix), int(iy)), "mh_gcmc_set_rng")

@@ -376,6 +376,16 @@ static int attempt(const mo_gcmc_sys *s, mo_gcmc_state *st, double deltaE, doubl
     return boltz > mo_rranf(&st->rng);
 }
 
+/* rigidmoves_adjustdeltatrans / _adjustdeltarot (rigidmoves.F90:2586-2617):
+ * 0.95 below a cumulative acceptance ratio of 0.49, 1.05 above 0.51, the
+ * displacement bounded by MIN/MAX_TRANS_DELTA = 0.01, 2.0 (:216-217) */
+static double adjust_delta(double d, double ratio)
+{
+    if (ratio < 0.49) return fmax(d * 0.95, 0.01);
+    if (ratio > 0.51) return fmin(d * 1.05, 2.0);
+    return d;
+}
+
 int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
 {
     const int ns = s->ns;
@@ -541,12 +551,14 @@ int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
                 memcpy(ang, st->ang + 3 * mol, sizeof ang);
                 if (mtype == MO_MV_TRANSLATE) {
                     double dv[3];
-                    for (int i = 0; i < 3; i++) dv[i] = ((mo_rranf(&st->rng) * 2.0) - 1.0) * s->dtrans;
+                    const double dtr = s->adjust_trans ? st->delta[0] : s->dtrans;
+                    for (int i = 0; i < 3; i++) dv[i] = ((mo_rranf(&st->rng) * 2.0) - 1.0) * dtr;
                     for (int i = 0; i < 3; i++) com[i] = com[i] + dv[i];
                 } else {
-                    double dphi = ((mo_rranf(&st->rng) * 2.0) - 1.0) * s->drot;
-                    double dpsi = ((mo_rranf(&st->rng) * 2.0) - 1.0) * s->drot;
-                    double dcos = ((mo_rranf(&st->rng) * 2.0) - 1.0) * s->drot;
+                    const double drt = s->adjust_rot ? st->delta[1] : s->drot;
+                    double dphi = ((mo_rranf(&st->rng) * 2.0) - 1.0) * drt;
+                    double dpsi = ((mo_rranf(&st->rng) * 2.0) - 1.0) * drt;
+                    double dcos = ((mo_rranf(&st->rng) * 2.0) - 1.0) * drt;
                     if (ns != 1) {
                         double phi = ang[0] + dphi;
                         double psi = ang[2] + dpsi;
@@ -589,6 +601,11 @@ int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
                         de = dU;
                     }
                 }
+                /* moves_adjustParams after every attempt (mcmoves.F90:460-461) */
+                if (mtype == MO_MV_TRANSLATE && s->adjust_trans)
+                    st->delta[0] = adjust_delta(st->delta[0], (double)st->acc[2] / (double)st->att[2]);
+                else if (mtype != MO_MV_TRANSLATE && s->adjust_rot)
+                    st->delta[1] = adjust_delta(st->delta[1], (double)st->acc[3] / (double)st->att[3]);
             }
             record(st, step, mtype, accepted, de);
         }

@@ -66,6 +66,9 @@ typedef struct {
     int coul;
     int coul_on[64];
     double coul_hicut, coul_locut, coul_shield;
+    /* adjustable displacements (rigidmoves.F90:2586-2617): the state's delta
+     * is used and updated after every attempt of the move */
+    int adjust_trans, adjust_rot;
 } mo_gcmc_sys;
 
 typedef struct {
@@ -85,6 +88,7 @@ typedef struct {
     long long nsteps;
     mo_gcmc_trace *trace;
     int trace_cap;
+    double delta[2]; /* translate / rotate displacement of adjustable moves */
 } mo_gcmc_state;
 
 void mo_gcmc_kvecs(const mo_gcmc_sys *s, int *nk, double *kv, double *w);

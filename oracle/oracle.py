@@ -286,7 +286,7 @@ class GcmcSys(ctypes.Structure):
                 ("nmoves", _i), ("move_type", _i * 4), ("cumwt", _d * 4),
                 ("insdelratio", _d), ("dtrans", _d), ("drot", _d), ("max_nrg", _d), ("bias", _vp),
                 ("coul", _i), ("coul_on", _i * 64), ("coul_hicut", _d), ("coul_locut", _d),
-                ("coul_shield", _d)]
+                ("coul_shield", _d), ("adjust_trans", _i), ("adjust_rot", _i)]
 
 
 class GcmcTrace(ctypes.Structure):
@@ -302,7 +302,8 @@ class GcmcState(ctypes.Structure):
     _fields_ = [("N", _i), ("maxN", _i), ("com", _vp), ("ang", _vp), ("rp", _vp), ("r", _vp),
                 ("nk", _i), ("kv", _vp), ("wk", _vp), ("Sre", _vp), ("Sim", _vp),
                 ("energy", _d), ("B", _d), ("rng", _Rng), ("att", _ll * 4), ("acc", _ll * 4),
-                ("sumN", _d), ("nsteps", _ll), ("trace", _vp), ("trace_cap", _i)]
+                ("sumN", _d), ("nsteps", _ll), ("trace", _vp), ("trace_cap", _i),
+                ("delta", _d * 2)]
 
 
 def _gcmc_lib():
@@ -399,6 +400,8 @@ class Gcmc:
         s.insdelratio = (wi / wd) if (wi and wd) else 1.0
         s.dtrans = spec.get("dtrans", 0.0)
         s.drot = spec.get("drot", 0.0)
+        s.adjust_trans = int(spec.get("adjust_trans", 0))
+        s.adjust_rot = int(spec.get("adjust_rot", 0))
         s.max_nrg = spec.get("max_nrg", 1.0e10)
         # BINSERT / BDELETE: spec["bias"] is a Bmap
         self.bias = spec.get("bias")
@@ -406,6 +409,7 @@ class Gcmc:
         self.sys = s
         st = GcmcState()
         st.maxN = maxN
+        st.delta[0], st.delta[1] = s.dtrans, s.drot
         self.com = np.zeros((maxN, 3))
         self.ang = np.zeros((maxN, 3))
         self.rp = np.zeros((maxN, ns, 3))

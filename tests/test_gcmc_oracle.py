@@ -143,3 +143,43 @@ def test_test007_final_configuration_equals_reference_restart():
     rp = MG.read_restart_rp(os.path.join(GOLD, "test007-gcmc-CH4", "100.res.1"))["CH4"]
     assert g.st.N == len(rp) == 21
     assert np.abs(g.rp[:g.st.N] - rp).max() < 5.01e-6
+
+
+def _replay_deltas(tr, d0):
+    """The displacements rigidmoves_adjustdeltatrans / _adjustdeltarot
+    (rigidmoves.F90:2586-2617) leave after the attempts in a trace: every
+    TRANSLATE (2) / ROTATE (3) step with molecules present is an attempt."""
+    d = list(d0)
+    att, acc = [0, 0], [0, 0]
+    for mv, a, n, _ in tr:
+        mv = int(mv)
+        if mv not in (2, 3) or int(n) == 0:
+            continue
+        k = mv - 2
+        att[k] += 1
+        acc[k] += int(a)
+        ratio = acc[k] / att[k]
+        if ratio < 0.49:
+            d[k] = max(d[k] * 0.95, 0.01)
+        elif ratio > 0.51:
+            d[k] = min(d[k] * 1.05, 2.0)
+    return d, att
+
+
+def test_adjustable_displacements_follow_acceptance():
+    # both move away from their start, within [0.01, 2.0]
+    sp = co2_spec()
+    sp.update(adjust_trans=1, adjust_rot=1, dtrans=1.9, drot=1.9)
+    g = O.Gcmc(sp, 3000.0, 3011, trace=4000)
+    g.run(4000)
+    tr = g.trace_array(4000)
+    want, att = _replay_deltas(tr, (1.9, 1.9))
+    assert min(att) > 300
+    assert list(g.st.delta) == want
+    assert want[0] != 1.9 and want[1] != 1.9 and all(0.01 <= x <= 2.0 for x in want)
+    assert list(g.st.att[2:4]) == att
+    # adjust off: the displacements stay as given
+    sp0 = co2_spec()
+    g0 = O.Gcmc(sp0, 3000.0, 3011)
+    g0.run(500)
+    assert list(g0.st.delta) == [0.4, 0.4]

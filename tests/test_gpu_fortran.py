@@ -382,3 +382,35 @@ def test_music_post_isotherm_from_driver_datafiles(hip, monkeypatch):
             assert iso[4 + q] == "%-15s%10s%12.7f" % (P.real2str(p, 15)[:15], "", zs[-1])
         assert zs[0] < zs[2]
         assert iso[7].startswith("# Results of a Langmuir Fit")
+
+
+ADJ_T = ("0.4, 0              # Del-R Ang., adjust delta?(0=NO, 1=YES)", "1.9, 1              # Del-R Ang., adjust delta?(0=NO, 1=YES)")
+ADJ_R = ("0.4, 0              # Del-Theta Rad., adjust delta?(0=NO, 1=YES)", "0.4, 1              # Del-Theta Rad., adjust delta?(0=NO, 1=YES)")
+
+
+def test_music_gcmc_adjustable_displacements_resume(hip):
+    """test007 with an adjustable translate displacement: the printed
+    "Max disp" moves away from its start within [0.01, 2.0], and a run split
+    at 10000 steps and resumed from its restart (.state holds the
+    displacements) ends in the restart file of the unbroken run."""
+    import re
+    print("case: music_gcmc adjustable displacements, unbroken vs resumed (test007)", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        ctrl = _t007_tree(td, icrash=(ICRASH[0], ICRASH[1] % 20000), adjt=ADJ_T, adjr=ADJ_R)
+        out = MG.run([ctrl], td)
+        disp = [float(x) for x in re.findall(r"Max disp:\s+([0-9.]+)", out)]
+        # test007 moves: insert, delete, translate (one "Max disp" per block)
+        assert len(disp) >= 20 and all(0.01 <= d <= 2.0 for d in disp)
+        assert any(abs(d - 1.9) > 1e-3 for d in disp)
+        full = open(os.path.join(td, "temp.res.1"), "rb").read().split(b"\n")
+    with tempfile.TemporaryDirectory() as td:
+        ctrl = _t007_tree(td, icrash=(ICRASH[0], ICRASH[1] % 10000), niter=(NITER[0], NITER[1] % 10000),
+                          adjt=ADJ_T, adjr=ADJ_R)
+        MG.run([ctrl], td)
+        ctrl2 = _t007_tree(td, icrash=(ICRASH[0], ICRASH[1] % 10000), niter=(NITER[0], NITER[1] % 10000),
+                           src=("GCMC NULL # Source of configs", "RESTARTFILE ../temp.res.1 # Source of configs"),
+                           res=("temp.res   # Name of Restart File", "temp2.res   # Name of Restart File"),
+                           adjt=ADJ_T, adjr=ADJ_R)
+        MG.run([ctrl2, "resume"], td)
+        got = open(os.path.join(td, "temp2.res.1"), "rb").read().split(b"\n")
+        assert got[:1] + got[2:] == full[:1] + full[2:]

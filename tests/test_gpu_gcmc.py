@@ -283,3 +283,30 @@ def test_gpu_plain_coulomb_trajectory_matches_oracle(hip, hicut):
     assert rel.max() < 1e-5
     b.close()
     b32.close()
+
+
+def test_gpu_adjustable_displacements_match_oracle(hip):
+    """Adjustable translate / rotate displacements (rigidmoves_adjustdelta*,
+    after every attempt): FP64 trajectories move for move against the
+    oracle and the same final displacements, bit for bit; a checkpoint of
+    the displacements restores them."""
+    print("case: adjustable displacements vs oracle", flush=True)
+    spec, maps, L = small_charged_system(hip)
+    sp = dict(spec)
+    sp.update(adjust_trans=1, adjust_rot=1, dtrans=1.5, drot=1.5)
+    fugs = [3000.0, 30000.0]
+    seeds = [6101, 6102]
+    b = G.GcmcBatch(hip, sp, fugs, seeds, maxN=512, maps=maps)
+    b.set_trace(3000)
+    b.run(3000)
+    d = b.deltas()
+    for r in range(2):
+        g = _oracle_for(sp, maps, fugs[r], seeds[r])
+        g.run(3000)
+        to, tg = g.trace_array(3000), b.trace(r)
+        assert np.array_equal(tg[:, :3].astype(int), to[:, :3].astype(int)), r
+        assert list(d[r]) == list(g.st.delta), (r, d[r], list(g.st.delta))
+        assert d[r][0] != 1.5 and d[r][1] != 1.5
+    b.set_deltas(0, 0.25, 0.125)
+    assert list(b.deltas()[0]) == [0.25, 0.125]
+    b.close()
