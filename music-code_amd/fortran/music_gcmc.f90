@@ -806,7 +806,10 @@ contains
     if (trim(adjustl(f(nf))) == '+') nf = nf - 1
     if (npts == 1 .and. nf > 1) call fail('one state point but more than one pressure given')
     if (nf == 1 .and. npts == 1) then
-       read(word(f(1), 1), *, err=90) press(1)
+       read(word(f(1), 1), *, iostat=q) press(1)
+       if (q /= 0) call read_pressure_file(word(f(1), 1))
+    else if (nf == 1) then
+       call read_pressure_file(word(f(1), 1))
     else if (nf == 2) then
        read(word(f(1), 1), *, err=90) press(1)
        read(word(f(2), 1), *, err=90) pincr
@@ -826,11 +829,41 @@ contains
           read(word(f(q), 1), *, err=90) press(q)
        end do
     else
-       call fail('could not parse the pressure line (pressure files are not read)')
+       call fail('could not parse the pressure line')
     end if
     return
-90  call fail('could not parse the pressure line (pressure files are not read)')
+90  call fail('could not parse the pressure line')
   end subroutine read_pressures
+
+  ! gcmcmoves_readpressures (gcmcmoves.F90:575-611): in the named file
+  ! (relative to the working directory) the first line holding the species
+  ! name, any case, then the number of points, then the pressures (kPa)
+  subroutine read_pressure_file(name)
+    character(len=*), intent(in) :: name
+    type(text_file) :: pf
+    integer :: k, j, n, ios
+    character(len=:), allocatable :: buf
+    if (.not. exists(trim(name))) call fail('pressure file not found: ' // trim(name))
+    call read_text(trim(name), pf)
+    k = 0
+    do j = 1, pf%n
+       if (index(lower(pf%line(j)), lower(trim(spname))) > 0) then
+          k = j
+          exit
+       end if
+    end do
+    if (k == 0 .or. k >= pf%n) call fail('species ' // trim(spname) // ' not found in pressure file ' // trim(name))
+    read(pf%line(k + 1), *, iostat=ios) n
+    if (ios /= 0 .or. n /= npts) &
+         call fail('the number of points in pressure file ' // trim(name) // ' differs from the ctrlfile')
+    buf = ''
+    do j = k + 2, pf%n
+       buf = buf // ' ' // trim(pf%line(j))
+    end do
+    read(buf, *, iostat=ios) press(1:npts)
+    if (ios /= 0) call fail('could not read ' // trim(name) // ': too few pressures')
+    write(*, '(3a)') '  pressures read from ', trim(name), ' (gcmcmoves_readpressures)'
+  end subroutine read_pressure_file
 
   ! fields of a line split at commas, appended to f(nf+1:)
   subroutine split_commas(s, f, nf)

@@ -111,3 +111,25 @@ def test_pressure_lists(tmp_path, line, npts, want):
     if want is None:
         want = [5.0, 50.0, 500.0, 7000.0, 9000.0]
     assert np.allclose(d["press"], want, rtol=1e-14)
+
+
+def test_pressure_file(tmp_path):
+    """A file name on the fugacity line (gcmcmoves_readpressures,
+    gcmcmoves.F90:575-611): the line holding the species name, then the
+    number of points, then the pressures, possibly over several lines."""
+    shutil.copytree(os.path.join(GOLD, "test007-gcmc-CH4"), tmp_path / "t7", dirs_exist_ok=True,
+                    copy_function=shutil.copyfile)
+    shutil.copytree(os.path.join(GOLD, "potentials"), tmp_path / "potentials", copy_function=shutil.copyfile)
+    txt = open(os.path.join(GOLD, "test007-gcmc-CH4", "test007.ctr")).read()
+    txt = txt.replace("1                    # No. of pressure values", "3                    # No. of pressure values", 1)
+    txt = txt.replace("10000             # Fugacity Range (kPa)", "press.dat         # Fugacity Range (kPa)", 1)
+    ctrl = tmp_path / "t7" / "test007.ctr"
+    ctrl.write_text(txt)
+    (tmp_path / "press.dat").write_text("# points\nCH4 pressures (kPa)\n3\n 10.0  250.5\n 7000.0\n")
+    out = subprocess.run([GCMC, str(ctrl), "check"], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert out.returncode == 0, out.stdout[-1500:]
+    fug = [float(l.split()[-1]) for l in out.stdout.splitlines() if "fugacity (kPa)" in l]
+    assert fug == [10.0, 250.5, 7000.0]
+    (tmp_path / "press.dat").write_text("CH4\n2\n 10.0 20.0\n")
+    out = subprocess.run([GCMC, str(ctrl), "check"], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert out.returncode != 0 and "differs from the ctrlfile" in out.stdout + out.stderr
