@@ -266,6 +266,11 @@ int mh_gcmc_set_rng(mh_gcmc *g, int rep, int32_t ix, int32_t iy);
 int mh_gcmc_get_deltas(mh_gcmc *g, double *delta);
 /* Restore a replica's displacements (a checkpoint of adjustable moves). */
 int mh_gcmc_set_deltas(mh_gcmc *g, int rep, double delta_trans, double delta_rot);
+/* Restore a replica's attempt / acceptance counts att[4], acc[4] by move
+ * type (a checkpoint): adjustable displacements follow the cumulative ratio
+ * acc / att of their move (moves_adjustParams, moves.F90:553-560), so an
+ * exact resume needs the counts as well as the displacements. */
+int mh_gcmc_set_counts(mh_gcmc *g, int rep, const int64_t *att, const int64_t *acc);
 int mh_gcmc_get_config(mh_gcmc *g, int rep, int *n, double *com, double *ang, double *r);
 /* Full FP64 recompute of every replica's energy (interact_checknrgs); with
  * refresh != 0 the structure factors and running energies are reset to it. */

@@ -77,10 +77,12 @@ struct GDev {
 
 // rigidmoves_adjustdeltatrans / _adjustdeltarot (rigidmoves.F90:2586-2617):
 // steer the cumulative acceptance ratio of the move towards 1/2, the
-// displacement kept within [0.01, 2.0] (MIN/MAX_TRANS_DELTA, :216-217)
+// displacement kept within [0.01, 2.0] (MIN/MAX_TRANS_DELTA, :216-217).
+// The thresholds are default-real (float32) literals compared with the
+// FP64 ratio succ/att (moves.F90:553-560), i.e. 0.49f and 0.51f widened.
 __device__ __forceinline__ double adjust_delta(double d, double ratio) {
-    if (ratio < 0.49) return fmax(__dmul_rn(d, 0.95), 0.01);
-    if (ratio > 0.51) return fmin(__dmul_rn(d, 1.05), 2.0);
+    if (ratio < (double)0.49f) return fmax(__dmul_rn(d, 0.95), 0.01);
+    if (ratio > (double)0.51f) return fmin(__dmul_rn(d, 1.05), 2.0);
     return d;
 }
 
@@ -1857,6 +1859,20 @@ int mh_gcmc_set_deltas(mh_gcmc *g, int rep, double delta_trans, double delta_rot
     MH_HIP(hipStreamSynchronize(g->ctx->stream));
     const double d2[2] = {delta_trans, delta_rot};
     MH_HIP(hipMemcpy(g->delta.as<double>() + 2 * rep, d2, 16, hipMemcpyHostToDevice));
+    return MH_OK;
+}
+
+int mh_gcmc_set_counts(mh_gcmc *g, int rep, const int64_t *att, const int64_t *acc) {
+    MH_CHECK(g && rep >= 0 && rep < g->nrep && att && acc, MH_EINVAL, "mh_gcmc_set_counts: bad arguments");
+    for (int i = 0; i < 4; ++i)
+        MH_CHECK(att[i] >= 0 && acc[i] >= 0 && acc[i] <= att[i], MH_EINVAL,
+                 "mh_gcmc_set_counts: counts must satisfy 0 <= acc <= att");
+    MH_HIP(hipSetDevice(g->ctx->device));
+    MH_HIP(hipStreamSynchronize(g->ctx->stream));
+    long long a[4], c[4];
+    for (int i = 0; i < 4; ++i) a[i] = att[i], c[i] = acc[i];
+    MH_HIP(hipMemcpy(g->att.as<long long>() + 4 * rep, a, 32, hipMemcpyHostToDevice));
+    MH_HIP(hipMemcpy(g->acc.as<long long>() + 4 * rep, c, 32, hipMemcpyHostToDevice));
     return MH_OK;
 }
 

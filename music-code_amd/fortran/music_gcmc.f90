@@ -659,6 +659,8 @@ contains
     allocate(dl(2, nrep_now()))
     call check(mh_gcmc_get_deltas(g, dl), 'mh_gcmc_get_deltas')
     write(u, '(2es25.17e3)') dl(:, rep + 1)
+    ! attempt / acceptance counts: adjustable displacements follow their ratio
+    write(u, '(8i21)') att(:, rep + 1), acc(:, rep + 1)
     close(u)
   end subroutine write_restart
 
@@ -706,6 +708,8 @@ contains
     real(c_double), allocatable :: com(:,:), ang(:,:)
     real(c_double) :: v(7), dres(2)
     integer(c_int32_t) :: rx, ry
+    integer(c_int64_t) :: cnt(8)
+    logical :: have_cnt
     character(len=LL) :: rng
     call read_text(srcfile, rf)
     ii = 1
@@ -746,12 +750,19 @@ contains
        end do
        read(u, *, iostat=ios) dres
        if (ios /= 0) dres = [P%delta_trans, P%delta_rot]
+       have_cnt = .false.
+       if (ios == 0) then
+          read(u, *, iostat=ios) cnt
+          have_cnt = ios == 0
+       end if
        close(u)
     end if
     do r = 1, nr
        call check(mh_gcmc_set_config(gb, int(r - 1, c_int), int(nm, c_int), com, ang), 'mh_gcmc_set_config')
        if (resume) call check(mh_gcmc_set_rng(gb, int(r - 1, c_int), rx, ry), 'mh_gcmc_set_rng')
        if (resume) call check(mh_gcmc_set_deltas(gb, int(r - 1, c_int), dres(1), dres(2)), 'mh_gcmc_set_deltas')
+       if (resume .and. have_cnt) &
+            call check(mh_gcmc_set_counts(gb, int(r - 1, c_int), cnt(1:4), cnt(5:8)), 'mh_gcmc_set_counts')
     end do
     write(*, '(a, i0, 2a)') '  started from ', nm, ' molecules of ', trim(srcfile)
   contains

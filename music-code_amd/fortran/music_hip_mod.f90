@@ -57,7 +57,7 @@ module music_hip_mod
   public :: mh_map_load, mh_map_interp
   public :: mh_gcmc_create, mh_gcmc_destroy, mh_gcmc_set_threads, mh_gcmc_autotune
   public :: mh_gcmc_set_config, mh_gcmc_run, mh_gcmc_run_cycles, mh_gcmc_get_state, mh_gcmc_set_rng, &
-            mh_gcmc_get_moments, mh_gcmc_get_deltas, mh_gcmc_set_deltas
+            mh_gcmc_get_moments, mh_gcmc_get_deltas, mh_gcmc_set_deltas, mh_gcmc_set_counts
   public :: mh_gcmc_get_config, mh_gcmc_total_energy, mh_trial_energy
   public :: mh_bmap_create, mh_bmap_destroy, mh_bmap_info
 
@@ -281,6 +281,14 @@ module music_hip_mod
        real(c_double), value :: dtrans, drot
        integer(c_int) :: rc
      end function mh_gcmc_set_deltas
+
+     function mh_gcmc_set_counts(g, rep, att, acc) bind(C, name="mh_gcmc_set_counts") result(rc)
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: g
+       integer(c_int), value :: rep
+       integer(c_int64_t), intent(in) :: att(4), acc(4)
+       integer(c_int) :: rc
+     end function mh_gcmc_set_counts
 
      function mh_gcmc_get_config(g, rep, n, com, ang, r) bind(C, name="mh_gcmc_get_config") result(rc)
        import :: c_int, c_ptr, c_double

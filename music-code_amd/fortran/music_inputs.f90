@@ -653,122 +653,286 @@ contains
   end subroutine read_system
 
   ! molecule_genprincipleaxes (molecule.F90:830-926), applied to rigid
-  ! molecules of 3 or more atoms (molecule.F90:714-716, 843): shift to the
-  ! centre of mass computed with integer-truncated masses (molecule.F90:833)
-  ! unless NOTRANSFORM, and rotate onto the principal axes of inertia,
-  ! ascending moment, unless NOROTATE or NOTRANSFORM.  The
-  ! eigenvectors come from cyclic Jacobi rotations (the reference uses
-  ! EISPACK tred2/tql2); each is signed so that its component of largest
-  ! magnitude is positive (the first such on ties), the convention the
-  ! Python host applies as well.  Coordinates below 1e-12 A are set to 0.
+  ! molecules of 3 or more atoms (molecule.F90:714-716, 843), in the
+  ! reference's arithmetic order: unless NOTRANSFORM, shift by the centre of
+  ! mass of the real masses (molecule_getrefcom, molecule.F90:777-798: sum of
+  ! r m times the reciprocal of the mass sum); form the inertia tensor with
+  ! integer-truncated masses (molecule.F90:833, 872-892); diagonalise it by
+  ! Householder tridiagonalisation and implicit QL (tred2 / tql2, restated in
+  ! tred2_tql2 below) and, unless NOROTATE or NOTRANSFORM, rotate every atom
+  ! by the transposed eigenvector matrix (matrix_m_mult_v, matrix.F90:137-150).
   subroutine principal_axes(m, atoms, ref)
     type(molecule), intent(in) :: m
     type(atom_type), intent(in) :: atoms(:)
     real(c_double), intent(out) :: ref(3, m%natoms)
-    real(c_double) :: ms(m%natoms), com(3), r(3, m%natoms), I3(3, 3), V(3, 3), w(3), tv(3), tw
-    integer :: a, j, k, imax
-    do a = 1, m%natoms
-       ms(a) = aint(atoms(m%atype(a) + 1)%mass)
-    end do
+    real(c_double) :: ms(m%natoms), com(3), r(3, m%natoms), I3(3, 3), z(3, 3), d(3), msum, inv, moi, rsq, v
+    integer :: a, i, j, k, mi
     r = m%xyz
     if (.not. m%rigid .or. m%natoms < 3) then
        ref = r
        return
     end if
+    do a = 1, m%natoms
+       ms(a) = atoms(m%atype(a) + 1)%mass
+    end do
     if (m%translate) then
-       do j = 1, 3
-          com(j) = sum(m%xyz(j, :) * ms) / sum(ms)
-          r(j, :) = m%xyz(j, :) - com(j)
+       com = 0.0d0
+       msum = 0.0d0
+       do a = 1, m%natoms
+          msum = msum + ms(a)
+          com = com + m%xyz(:, a) * ms(a)
+       end do
+       inv = 1.0d0 / msum
+       com = com * inv
+       do a = 1, m%natoms
+          r(:, a) = r(:, a) + (-com)
        end do
     end if
+    do i = 1, 3
+       do j = 1, 3
+          moi = 0.0d0
+          do a = 1, m%natoms
+             mi = int(ms(a))
+             if (i == j) then
+                rsq = r(1, a) * r(1, a) + r(2, a) * r(2, a) + r(3, a) * r(3, a)
+                moi = moi + real(mi, c_double) * (rsq - r(i, a) * r(i, a))
+             else
+                moi = moi - real(mi, c_double) * r(i, a) * r(j, a)
+             end if
+          end do
+          I3(i, j) = moi
+       end do
+    end do
+    call tred2_tql2(I3, d, z)
     if (.not. m%rotate) then
        ref = r
        return
     end if
-    I3 = 0.0d0
     do a = 1, m%natoms
-       do j = 1, 3
-          do k = 1, 3
-             I3(j, k) = I3(j, k) - ms(a) * r(j, a) * r(k, a)
+       do k = 1, 3
+          v = 0.0d0
+          do j = 1, 3
+             v = v + z(j, k) * r(j, a)
           end do
-          I3(j, j) = I3(j, j) + ms(a) * sum(r(:, a)**2)
-       end do
-    end do
-    call jacobi3(I3, w, V)
-    ! ascending eigenvalues
-    do j = 1, 2
-       do k = j + 1, 3
-          if (w(k) < w(j)) then
-             tw = w(j); w(j) = w(k); w(k) = tw
-             tv = V(:, j); V(:, j) = V(:, k); V(:, k) = tv
-          end if
-       end do
-    end do
-    do j = 1, 3
-       imax = 1
-       do k = 2, 3
-          if (abs(V(k, j)) > abs(V(imax, j))) imax = k
-       end do
-       if (V(imax, j) < 0.0d0) V(:, j) = -V(:, j)
-    end do
-    do a = 1, m%natoms
-       do j = 1, 3
-          ref(j, a) = sum(r(:, a) * V(:, j))
-          if (abs(ref(j, a)) < 1.0d-12) ref(j, a) = 0.0d0
+          ref(k, a) = v
        end do
     end do
   end subroutine principal_axes
 
-  ! eigenvalues w and eigenvectors V (columns) of a symmetric 3x3 matrix by
-  ! cyclic Jacobi rotations (Golub & Van Loan, Matrix Computations, 8.5)
-  subroutine jacobi3(A0, w, V)
-    real(c_double), intent(in) :: A0(3, 3)
-    real(c_double), intent(out) :: w(3), V(3, 3)
-    real(c_double) :: A(3, 3), th, t, c, sn, tau, apq, app, aqq, off, scale
-    integer :: sweep, p, q, k
-    A = A0
-    V = 0.0d0
-    do k = 1, 3
-       V(k, k) = 1.0d0
+  ! sqrt(a^2 + b^2) by the iteration of EISPACK's pythag (Moler and
+  ! Morrison, IBM J. Res. Dev. 27 (1983) 577)
+  real(c_double) function pythag(a, b) result(p)
+    real(c_double), intent(in) :: a, b
+    real(c_double) :: r, s, t, u
+    p = max(abs(a), abs(b))
+    if (p == 0.0d0) return
+    r = (min(abs(a), abs(b)) / p)**2
+    do
+       t = 4.0d0 + r
+       if (t == 4.0d0) exit
+       s = r / t
+       u = 1.0d0 + 2.0d0 * s
+       p = u * p
+       r = (s / u)**2 * r
     end do
-    scale = max(maxval(abs(A)), tiny(1.0d0))
-    do sweep = 1, 50
-       off = abs(A(1, 2)) + abs(A(1, 3)) + abs(A(2, 3))
-       if (off <= 1.0d-17 * scale) exit
-       do p = 1, 2
-          do q = p + 1, 3
-             apq = A(p, q)
-             if (apq == 0.0d0) cycle
-             app = A(p, p)
-             aqq = A(q, q)
-             th = (aqq - app) / (2.0d0 * apq)
-             t = sign(1.0d0, th) / (abs(th) + sqrt(th * th + 1.0d0))
-             c = 1.0d0 / sqrt(t * t + 1.0d0)
-             sn = t * c
-             tau = sn / (1.0d0 + c)
-             A(p, p) = app - t * apq
-             A(q, q) = aqq + t * apq
-             A(p, q) = 0.0d0
-             A(q, p) = 0.0d0
-             do k = 1, 3
-                if (k == p .or. k == q) cycle
-                th = A(k, p)
-                A(k, p) = th - sn * (A(k, q) + tau * th)
-                A(p, k) = A(k, p)
-                A(k, q) = A(k, q) + sn * (th - tau * A(k, q))
-                A(q, k) = A(k, q)
+  end function pythag
+
+  ! Eigenvalues d (ascending) and eigenvectors z (columns) of the symmetric
+  ! 3x3 matrix a (lower triangle read): Householder reduction to
+  ! tridiagonal form with the transformations accumulated (EISPACK tred2;
+  ! Martin, Reinsch and Wilkinson, Numer. Math. 11 (1968) 181), then the
+  ! implicit QL method (EISPACK tql2; Bowdler, Martin, Reinsch and
+  ! Wilkinson, Numer. Math. 11 (1968) 293).  The steps are those of the
+  ! published algorithms, which the reference calls (molecule.F90:896-897),
+  ! so the eigenvectors carry the reference's signs, also for equal moments.
+  subroutine tred2_tql2(a, d, z)
+    real(c_double), intent(in) :: a(3, 3)
+    real(c_double), intent(out) :: d(3), z(3, 3)
+    integer, parameter :: n = 3
+    real(c_double) :: e(3), f, g, h, hh, scale, c, c2, c3, dl1, el1, p, r, s, s2, tst1, tst2
+    integer :: i, j, k, l, m, ii, l1, l2, it
+    do i = 1, n
+       do j = i, n
+          z(j, i) = a(j, i)
+       end do
+       d(i) = a(n, i)
+    end do
+    do i = n, 2, -1
+       l = i - 1
+       h = 0.0d0
+       scale = 0.0d0
+       if (l >= 2) then
+          do k = 1, l
+             scale = scale + abs(d(k))
+          end do
+       end if
+       if (l < 2 .or. scale == 0.0d0) then
+          e(i) = d(l)
+          do j = 1, l
+             d(j) = z(l, j)
+             z(i, j) = 0.0d0
+             z(j, i) = 0.0d0
+          end do
+       else
+          do k = 1, l
+             d(k) = d(k) / scale
+             h = h + d(k) * d(k)
+          end do
+          f = d(l)
+          g = -sign(sqrt(h), f)
+          e(i) = scale * g
+          h = h - f * g
+          d(l) = f - g
+          do j = 1, l
+             e(j) = 0.0d0
+          end do
+          do j = 1, l
+             f = d(j)
+             z(j, i) = f
+             g = e(j) + z(j, j) * f
+             do k = j + 1, l
+                g = g + z(k, j) * d(k)
+                e(k) = e(k) + z(k, j) * f
              end do
-             do k = 1, 3
-                th = V(k, p)
-                V(k, p) = th - sn * (V(k, q) + tau * th)
-                V(k, q) = V(k, q) + sn * (th - tau * V(k, q))
+             e(j) = g
+          end do
+          f = 0.0d0
+          do j = 1, l
+             e(j) = e(j) / h
+             f = f + e(j) * d(j)
+          end do
+          hh = f / (h + h)
+          do j = 1, l
+             e(j) = e(j) - hh * d(j)
+          end do
+          do j = 1, l
+             f = d(j)
+             g = e(j)
+             do k = j, l
+                z(k, j) = z(k, j) - f * e(k) - g * d(k)
+             end do
+             d(j) = z(l, j)
+             z(i, j) = 0.0d0
+          end do
+       end if
+       d(i) = h
+    end do
+    do i = 2, n
+       l = i - 1
+       z(n, l) = z(l, l)
+       z(l, l) = 1.0d0
+       h = d(i)
+       if (h /= 0.0d0) then
+          do k = 1, l
+             d(k) = z(k, i) / h
+          end do
+          do j = 1, l
+             g = 0.0d0
+             do k = 1, l
+                g = g + z(k, i) * z(k, j)
+             end do
+             do k = 1, l
+                z(k, j) = z(k, j) - g * d(k)
              end do
           end do
+       end if
+       do k = 1, l
+          z(k, i) = 0.0d0
        end do
     end do
-    do k = 1, 3
-       w(k) = A(k, k)
+    do i = 1, n
+       d(i) = z(n, i)
+       z(n, i) = 0.0d0
     end do
-  end subroutine jacobi3
+    z(n, n) = 1.0d0
+    e(1) = 0.0d0
+    ! implicit QL
+    do i = 2, n
+       e(i - 1) = e(i)
+    end do
+    f = 0.0d0
+    tst1 = 0.0d0
+    e(n) = 0.0d0
+    do l = 1, n
+       it = 0
+       h = abs(d(l)) + abs(e(l))
+       if (tst1 < h) tst1 = h
+       do m = l, n
+          tst2 = tst1 + abs(e(m))
+          if (tst2 == tst1) exit
+       end do
+       if (m /= l) then
+          do
+             if (it == 30) call fail('principal axes: tql2 did not converge')
+             it = it + 1
+             l1 = l + 1
+             l2 = l1 + 1
+             g = d(l)
+             p = (d(l1) - g) / (2.0d0 * e(l))
+             r = pythag(p, 1.0d0)
+             d(l) = e(l) / (p + sign(r, p))
+             d(l1) = e(l) * (p + sign(r, p))
+             dl1 = d(l1)
+             h = g - d(l)
+             do i = l2, n
+                d(i) = d(i) - h
+             end do
+             f = f + h
+             p = d(m)
+             c = 1.0d0
+             c2 = c
+             el1 = e(l1)
+             s = 0.0d0
+             s2 = 0.0d0
+             c3 = 0.0d0
+             do ii = 1, m - l
+                c3 = c2
+                c2 = c
+                s2 = s
+                i = m - ii
+                g = c * e(i)
+                h = c * p
+                r = pythag(p, e(i))
+                e(i + 1) = s * r
+                s = e(i) / r
+                c = p / r
+                p = c * d(i) - s * g
+                d(i + 1) = h + s * (c * g + s * d(i))
+                do k = 1, n
+                   h = z(k, i + 1)
+                   z(k, i + 1) = s * z(k, i) + c * h
+                   z(k, i) = c * z(k, i) - s * h
+                end do
+             end do
+             p = -s * s2 * c3 * el1 * e(l) / dl1
+             e(l) = s * p
+             d(l) = c * p
+             tst2 = tst1 + abs(e(l))
+             if (.not. tst2 > tst1) exit
+          end do
+       end if
+       d(l) = d(l) + f
+    end do
+    do ii = 2, n
+       i = ii - 1
+       k = i
+       p = d(i)
+       do j = ii, n
+          if (d(j) < p) then
+             k = j
+             p = d(j)
+          end if
+       end do
+       if (k /= i) then
+          d(k) = d(i)
+          d(i) = p
+          do j = 1, n
+             p = z(j, i)
+             z(j, i) = z(j, k)
+             z(j, k) = p
+          end do
+       end if
+    end do
+  end subroutine tred2_tql2
 
 end module music_inputs

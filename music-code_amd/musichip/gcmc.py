@@ -47,6 +47,7 @@ def _sig():
         "mh_gcmc_set_rng": [_vp, _i, ctypes.c_int32, ctypes.c_int32],
         "mh_gcmc_get_deltas": [_vp, _vp],
         "mh_gcmc_set_deltas": [_vp, _i, _d, _d],
+        "mh_gcmc_set_counts": [_vp, _i, _vp, _vp],
         "mh_gcmc_get_moments": [_vp, _vp, _vp],
         "mh_gcmc_get_config": [_vp, _i, P(_i), _vp, _vp, _vp],
         "mh_gcmc_total_energy": [_vp, _vp, _i],
@@ -188,6 +189,12 @@ class GcmcBatch:
 
     def set_deltas(self, rep, dtrans, drot):
         H.check(_sig().mh_gcmc_set_deltas(self.h, rep, float(dtrans), float(drot)), "mh_gcmc_set_deltas")
+
+    def set_counts(self, rep, att, acc):
+        """Restore a replica's attempt / acceptance counts by move type."""
+        a = np.ascontiguousarray(att, dtype=np.int64)
+        c = np.ascontiguousarray(acc, dtype=np.int64)
+        H.check(_sig().mh_gcmc_set_counts(self.h, rep, _p(a), _p(c)), "mh_gcmc_set_counts")
 
     def set_rng(self, rep, ix, iy):
         H.check(_sig().mh_gcmc_set_rng(self.h, rep, int(ix), int(iy)), "mh_gcmc_set_rng")

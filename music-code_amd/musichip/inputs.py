@@ -12,6 +12,7 @@ Formats follow the reference readers:
 Only the subset needed by the GCMC energy hot path is read: LJ / EWALD
 pair models, rigid molecules, orthorhombic cells.
 """
+import math
 import os
 import re
 
@@ -135,28 +136,252 @@ def read_molecule(path):
     return mol
 
 
-def principal_axes(xyz, masses, translate=True, rotate=True):
-    """molecule_genprincipleaxes (molecule.F90:830-926): shift to the centre
-    of mass computed with INTEGER-truncated masses (molecule.F90:833) and
-    rotate onto the principal axes.  Eigenvectors come from numpy (the
-    reference uses EISPACK tred2/tql2); axis order ascending eigenvalue,
-    each eigenvector signed so that its component of largest magnitude is
-    positive (the first such on ties), as the Fortran driver does."""
-    m = np.floor(np.asarray(masses, dtype=np.float64))
-    com = (xyz * m[:, None]).sum(0) / m.sum()
-    r = xyz - com if translate else np.array(xyz, dtype=np.float64)
-    if not rotate:
-        return r
+def _pythag(a, b):
+    """sqrt(a^2 + b^2) without destructive underflow or overflow: the
+    iteration of EISPACK's pythag (Moler and Morrison, IBM J. Res. Dev. 27
+    (1983) 577), which tql2 uses."""
+    p = max(abs(a), abs(b))
+    if p == 0.0:
+        return p
+    r = (min(abs(a), abs(b)) / p) ** 2
+    while True:
+        t = 4.0 + r
+        if t == 4.0:
+            return p
+        s = r / t
+        u = 1.0 + 2.0 * s
+        p = u * p
+        r = (s / u) ** 2 * r
+
+
+def _tred2(a):
+    """Householder reduction of a symmetric matrix to tridiagonal form with
+    the transformations accumulated (EISPACK tred2; Martin, Reinsch and
+    Wilkinson, Numer. Math. 11 (1968) 181).  Reads the lower triangle
+    a[j][i], j >= i.  Returns the diagonal d, the subdiagonal e (e[0] = 0)
+    and the orthogonal z.  The operations follow the published algorithm
+    step for step, so that the eigenvectors tql2 derives from z carry the
+    same signs as those of the reference (molecule.F90:896-897)."""
+    n = a.shape[0]
+    z = np.zeros((n, n))
+    d = np.zeros(n)
+    e = np.zeros(n)
+    for i in range(n):
+        for j in range(i, n):
+            z[j, i] = a[j, i]
+        d[i] = a[n - 1, i]
+    if n > 1:
+        for i in range(n - 1, 0, -1):
+            l = i - 1
+            h = 0.0
+            scale = 0.0
+            if l >= 1:
+                for k in range(l + 1):
+                    scale = scale + abs(d[k])
+            if l < 1 or scale == 0.0:
+                e[i] = d[l]
+                for j in range(l + 1):
+                    d[j] = z[l, j]
+                    z[i, j] = 0.0
+                    z[j, i] = 0.0
+            else:
+                for k in range(l + 1):
+                    d[k] = d[k] / scale
+                    h = h + d[k] * d[k]
+                f = d[l]
+                g = -math.copysign(math.sqrt(h), f)
+                e[i] = scale * g
+                h = h - f * g
+                d[l] = f - g
+                for j in range(l + 1):
+                    e[j] = 0.0
+                for j in range(l + 1):
+                    f = d[j]
+                    z[j, i] = f
+                    g = e[j] + z[j, j] * f
+                    for k in range(j + 1, l + 1):
+                        g = g + z[k, j] * d[k]
+                        e[k] = e[k] + z[k, j] * f
+                    e[j] = g
+                f = 0.0
+                for j in range(l + 1):
+                    e[j] = e[j] / h
+                    f = f + e[j] * d[j]
+                hh = f / (h + h)
+                for j in range(l + 1):
+                    e[j] = e[j] - hh * d[j]
+                for j in range(l + 1):
+                    f = d[j]
+                    g = e[j]
+                    for k in range(j, l + 1):
+                        z[k, j] = z[k, j] - f * e[k] - g * d[k]
+                    d[j] = z[l, j]
+                    z[i, j] = 0.0
+            d[i] = h
+        for i in range(1, n):
+            l = i - 1
+            z[n - 1, l] = z[l, l]
+            z[l, l] = 1.0
+            h = d[i]
+            if h != 0.0:
+                for k in range(l + 1):
+                    d[k] = z[k, i] / h
+                for j in range(l + 1):
+                    g = 0.0
+                    for k in range(l + 1):
+                        g = g + z[k, i] * z[k, j]
+                    for k in range(l + 1):
+                        z[k, j] = z[k, j] - g * d[k]
+            for k in range(l + 1):
+                z[k, i] = 0.0
+    for i in range(n):
+        d[i] = z[n - 1, i]
+        z[n - 1, i] = 0.0
+    z[n - 1, n - 1] = 1.0
+    e[0] = 0.0
+    return d, e, z
+
+
+def _tql2(d, e, z):
+    """Eigenvalues and eigenvectors of the tridiagonal (d, e) by the implicit
+    QL method, z accumulating the rotations, then sorted ascending (EISPACK
+    tql2; Bowdler, Martin, Reinsch and Wilkinson, Numer. Math. 11 (1968)
+    293).  In place; raises after 30 iterations on one eigenvalue, where the
+    reference stops (molecule.F90:898-902)."""
+    n = len(d)
+    if n == 1:
+        return
+    for i in range(1, n):
+        e[i - 1] = e[i]
+    f = 0.0
+    tst1 = 0.0
+    e[n - 1] = 0.0
+    for l in range(n):
+        j = 0
+        h = abs(d[l]) + abs(e[l])
+        if tst1 < h:
+            tst1 = h
+        m = l
+        while m < n:
+            tst2 = tst1 + abs(e[m])
+            if tst2 == tst1:
+                break
+            m += 1
+        if m != l:
+            while True:
+                if j == 30:
+                    raise ValueError("tql2: no convergence for eigenvalue %d" % (l + 1))
+                j += 1
+                l1 = l + 1
+                l2 = l1 + 1
+                g = d[l]
+                p = (d[l1] - g) / (2.0 * e[l])
+                r = _pythag(p, 1.0)
+                d[l] = e[l] / (p + math.copysign(r, p))
+                d[l1] = e[l] * (p + math.copysign(r, p))
+                dl1 = d[l1]
+                h = g - d[l]
+                for i in range(l2, n):
+                    d[i] = d[i] - h
+                f = f + h
+                p = d[m]
+                c = 1.0
+                c2 = c
+                el1 = e[l1]
+                s = 0.0
+                s2 = 0.0
+                c3 = 0.0
+                for ii in range(1, m - l + 1):
+                    c3 = c2
+                    c2 = c
+                    s2 = s
+                    i = m - ii
+                    g = c * e[i]
+                    h = c * p
+                    r = _pythag(p, e[i])
+                    e[i + 1] = s * r
+                    s = e[i] / r
+                    c = p / r
+                    p = c * d[i] - s * g
+                    d[i + 1] = h + s * (c * g + s * d[i])
+                    for k in range(n):
+                        h = z[k, i + 1]
+                        z[k, i + 1] = s * z[k, i] + c * h
+                        z[k, i] = c * z[k, i] - s * h
+                p = -s * s2 * c3 * el1 * e[l] / dl1
+                e[l] = s * p
+                d[l] = c * p
+                tst2 = tst1 + abs(e[l])
+                if not tst2 > tst1:
+                    break
+        d[l] = d[l] + f
+    for ii in range(1, n):
+        i = ii - 1
+        k = i
+        p = d[i]
+        for j in range(ii, n):
+            if d[j] < p:
+                k = j
+                p = d[j]
+        if k != i:
+            d[k] = d[i]
+            d[i] = p
+            for j in range(n):
+                p = z[j, i]
+                z[j, i] = z[j, k]
+                z[j, k] = p
+
+
+def principal_axes(xyz, masses, translate=True, rotate=True, moments=False):
+    """molecule_genprincipleaxes (molecule.F90:830-926), in the reference's
+    arithmetic order: shift by the centre of mass of the real masses
+    (molecule_getrefcom, molecule.F90:777-798: sum of r m, times the
+    reciprocal of the mass sum), form the inertia tensor with the masses
+    truncated to integers (molecule.F90:833, 872-892), diagonalise it by
+    tred2/tql2 and rotate the atoms by the transposed eigenvector matrix
+    (rot_xform * r, matrix_m_mult_v, matrix.F90:137-150).  The principal
+    moments come back too when moments is set."""
+    xyz = np.asarray(xyz, dtype=np.float64)
+    mreal = [float(x) for x in masses]
+    n = len(xyz)
+    r = np.array(xyz, dtype=np.float64)
+    if translate:
+        com = np.zeros(3)
+        msum = 0.0
+        for a in range(n):
+            msum = msum + mreal[a]
+            for c in range(3):
+                com[c] = com[c] + xyz[a][c] * mreal[a]
+        inv = 1.0 / msum
+        for c in range(3):
+            com[c] = com[c] * inv
+        for a in range(n):
+            for c in range(3):
+                r[a, c] = r[a, c] + (-com[c])
+    mint = [float(int(x)) for x in mreal]
     I = np.zeros((3, 3))
-    for mi, v in zip(m, r):
-        I += mi * (np.dot(v, v) * np.eye(3) - np.outer(v, v))
-    w, V = np.linalg.eigh(I)
-    for j in range(3):
-        if V[np.argmax(np.abs(V[:, j])), j] < 0.0:
-            V[:, j] = -V[:, j]
-    out = r @ V
-    out[np.abs(out) < 1e-12] = 0.0
-    return out
+    for i in range(3):
+        for j in range(3):
+            moi = 0.0
+            for a in range(n):
+                if i == j:
+                    rsq = r[a, 0] * r[a, 0] + r[a, 1] * r[a, 1] + r[a, 2] * r[a, 2]
+                    moi = moi + mint[a] * (rsq - r[a, i] * r[a, i])
+                else:
+                    moi = moi - mint[a] * r[a, i] * r[a, j]
+            I[i, j] = moi
+    d, e, z = _tred2(I)
+    _tql2(d, e, z)
+    out = r
+    if rotate:
+        out = np.zeros_like(r)
+        for a in range(n):
+            for k in range(3):
+                v = 0.0
+                for c in range(3):
+                    v = v + z[c, k] * r[a, c]
+                out[a, k] = v
+    return (out, d.copy()) if moments else out
 
 
 # ------------------------------------------------------------- ctrlfile

@@ -48,10 +48,12 @@
 
 #define TWOPI_R (2.0 * 3.1415927)
 
-/* matrix_b2feuler (matrix.F90:383-403), rigidcoords_toxyz */
-static void toxyz(const mo_gcmc_sys *s, const double com[3], const double ang[3], double *rp)
+/* rigidcoords_toxyz (rigidcoords.F90:254-276) with matrix_b2feuler
+ * (matrix.F90:383-403): rp = com + R(phi, theta, psi) ref, the product in
+ * matrix_m_mult_v's order (matrix.F90:137-150); a single site sits at com */
+void mo_euler_xyz(int ns, const double *ref, const double com[3], const double ang[3], double *rp)
 {
-    if (s->ns == 1) {
+    if (ns == 1) {
         rp[0] = com[0];
         rp[1] = com[1];
         rp[2] = com[2];
@@ -69,10 +71,9 @@ static void toxyz(const mo_gcmc_sys *s, const double com[3], const double ang[3]
     m[2][0] = ss * st;
     m[2][1] = cs * st;
     m[2][2] = ct;
-    for (int a = 0; a < s->ns; a++) {
-        const double *v = s->ref[a];
+    for (int a = 0; a < ns; a++) {
+        const double *v = ref + 3 * a;
         for (int i = 0; i < 3; i++) {
-            /* matrix * vector: row dot (loop order j = 1..3) */
             double d = 0.0;
             d = d + m[i][0] * v[0];
             d = d + m[i][1] * v[1];
@@ -80,6 +81,28 @@ static void toxyz(const mo_gcmc_sys *s, const double com[3], const double ang[3]
             rp[3 * a + i] = com[i] + d;
         }
     }
+}
+
+static void toxyz(const mo_gcmc_sys *s, const double com[3], const double ang[3], double *rp)
+{
+    mo_euler_xyz(s->ns, &s->ref[0][0], com, ang, rp);
+}
+
+/* coulomb_snglint (coulomb.F90:258-327) for one site pair with charges
+ * from the molecules (charges_fromAA_FILE false): skipped beyond hicut when
+ * hicut > 0, failure (return 0) below locut, else
+ * 331.5 shield q1 q2 (1/r - 1/hicut), or 1/r when hicut = 0. */
+int mo_coul_pair(double hicut, double locut, double shield, double r2, double q1, double q2, double *e)
+{
+    *e = 0.0;
+    if (hicut > 0.0 && r2 > hicut * hicut) return 1;
+    if (r2 < locut * locut) return 0;
+    double rr = sqrt(r2);
+    if (hicut > 0.0)
+        *e = (331.5 * shield * q1 * q2) * (1.0 / rr - 1.0 / hicut);
+    else
+        *e = (331.5 * shield * q1 * q2) * (1.0 / rr);
+    return 1;
 }
 
 /* simcell_pbcarray (simcell.F90:521-547) */
@@ -230,12 +253,11 @@ static int mol_energy(const mo_gcmc_sys *s, const mo_gcmc_state *st, const doubl
                     double rr = sqrt(r2);
                     up = up + s->sq[a] * s->sq[b] * erfc_as(s->kappa * rr) / rr * MO_TOKCAL;
                 }
-                /* coulomb_snglint (coulomb.F90:271-306) */
-                if (s->coul && s->coul_on[t] && !(s->coul_hicut > 0.0 && r2 > s->coul_hicut * s->coul_hicut)) {
-                    if (r2 < s->coul_locut * s->coul_locut) return 0;
-                    double rr = sqrt(r2);
-                    double shift = s->coul_hicut > 0.0 ? 1.0 / s->coul_hicut : 0.0;
-                    up = up + (331.5 * s->coul_shield * s->sq[a] * s->sq[b]) * (1.0 / rr - shift);
+                if (s->coul && s->coul_on[t]) {
+                    double e;
+                    if (!mo_coul_pair(s->coul_hicut, s->coul_locut, s->coul_shield, r2, s->sq[a], s->sq[b], &e))
+                        return 0;
+                    up = up + e;
                 }
             }
     }
@@ -378,11 +400,13 @@ static int attempt(const mo_gcmc_sys *s, mo_gcmc_state *st, double deltaE, doubl
 
 /* rigidmoves_adjustdeltatrans / _adjustdeltarot (rigidmoves.F90:2586-2617):
  * 0.95 below a cumulative acceptance ratio of 0.49, 1.05 above 0.51, the
- * displacement bounded by MIN/MAX_TRANS_DELTA = 0.01, 2.0 (:216-217) */
+ * displacement bounded by MIN/MAX_TRANS_DELTA = 0.01, 2.0 (:216-217).
+ * 0.49 and 0.51 are default-real literals and the ratio succ/(att*one) is
+ * REAL(8) (moves.F90:553-560): the comparison widens the float32 values. */
 static double adjust_delta(double d, double ratio)
 {
-    if (ratio < 0.49) return fmax(d * 0.95, 0.01);
-    if (ratio > 0.51) return fmin(d * 1.05, 2.0);
+    if (ratio < (double)0.49f) return fmax(d * 0.95, 0.01);
+    if (ratio > (double)0.51f) return fmin(d * 1.05, 2.0);
     return d;
 }
 

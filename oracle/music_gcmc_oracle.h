@@ -98,6 +98,8 @@ double mo_gcmc_total_energy(const mo_gcmc_sys *s, const mo_gcmc_state *st);
 void mo_gcmc_recompute_sk(const mo_gcmc_sys *s, mo_gcmc_state *st);
 void mo_gcmc_set_mol(const mo_gcmc_sys *s, mo_gcmc_state *st, int m, const double com[3], const double ang[3]);
 double mo_adams_B(double fug_kPa, double volume_A3, double T, double Zig);
+void mo_euler_xyz(int ns, const double *ref, const double com[3], const double ang[3], double *rp);
+int mo_coul_pair(double hicut, double locut, double shield, double r2, double q1, double q2, double *e);
 
 #ifdef __cplusplus
 }

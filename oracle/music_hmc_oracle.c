@@ -110,6 +110,18 @@ static double torsion_term(const double *cn, const double *r1, const double *r2,
     return sum;
 }
 
+/* One bonded term on its own (for checks against the reference's
+ * harmonic / harmonica / cosexpansion): kind 1 bond (par k, req; 2 atoms),
+ * 2 angle (par k, theta0; 3 atoms), 3 torsion (par c0..c5; 4 atoms).
+ * r[natoms][3] -> energy, forces f[natoms][3] (overwritten). */
+double mo_bonded_term(int kind, const double *par, const double *r, double *f)
+{
+    for (int i = 0; i < 12; i++) f[i] = 0.0;
+    if (kind == 1) return bond_term(par[0], par[1], r, r + 3, f, f + 3);
+    if (kind == 2) return angle_term(par[0], par[1], r, r + 3, r + 6, f, f + 3, f + 6);
+    return torsion_term(par, r, r + 3, r + 6, r + 9, f, f + 3, f + 6, f + 9);
+}
+
 static double energy_forces(const mo_hmc_sys *s, int N, const double *rp, const double *r, double *F, int *ok,
                             double *parts);
 

@@ -87,6 +87,8 @@ double mo_hmc_energy_parts(const mo_hmc_sys *s, int N, const double *rp, const d
 #define MO_MD_NQ 10
 int mo_hmc_md(const mo_hmc_sys *s, int N, double *rp, double *r, double *v, int nsteps, double *trace);
 
+double mo_bonded_term(int kind, const double *par, const double *r, double *f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -326,8 +326,45 @@ def _gcmc_lib():
         L.mo_bmap_free.argtypes = [_vp]
         L.mo_bmap_index.argtypes = [_vp, _d]
         L.mo_bmap_index.restype = _ll
+        L.mo_euler_xyz.argtypes = [_i, _vp, _vp, _vp, _vp]
+        L.mo_coul_pair.argtypes = [_d, _d, _d, _d, _d, _d, ctypes.POINTER(_d)]
+        L.mo_coul_pair.restype = _i
+        L.mo_bonded_term.argtypes = [_i, _vp, _vp, _vp]
+        L.mo_bonded_term.restype = _d
         L._gcmc_sig = True
     return L
+
+
+def euler_xyz(ref, com, ang):
+    """Site coordinates com + R(phi, theta, psi) ref of one rigid molecule."""
+    ref = np.ascontiguousarray(ref, dtype=np.float64)
+    com = np.ascontiguousarray(com, dtype=np.float64)
+    ang = np.ascontiguousarray(ang, dtype=np.float64)
+    out = np.zeros_like(ref)
+    _gcmc_lib().mo_euler_xyz(len(ref), _p(ref), _p(com), _p(ang), _p(out))
+    return out
+
+
+def coul_pair(hicut, locut, shield, sep, q1, q2):
+    """(ok, energy) of one Coulomb site pair at separation sep."""
+    sep = [float(x) for x in sep]
+    r2 = 0.0
+    for x in sep:
+        r2 = r2 + x * x
+    e = _d()
+    ok = _gcmc_lib().mo_coul_pair(hicut, locut, shield, r2, q1, q2, ctypes.byref(e))
+    return bool(ok), e.value
+
+
+def bonded_term(kind, par, coords):
+    """(energy, forces [natoms][3]) of one bond (1), angle (2) or torsion (3)."""
+    par = np.ascontiguousarray(par, dtype=np.float64)
+    r = np.zeros(12)
+    c = np.asarray(coords, dtype=np.float64).ravel()
+    r[:len(c)] = c
+    f = np.zeros(12)
+    u = _gcmc_lib().mo_bonded_term(kind, _p(par), _p(r), _p(f))
+    return u, f[:len(c)].reshape(-1, 3)
 
 
 def adams_B(fug_kPa, volume, T, Zig=1.0):

@@ -15,6 +15,16 @@
 !   EWALD   (see read order below)       -> ok, V, hot(2:8) per node
 !   COMPACT (see read order below)       -> counts, writes a map file
 !   HERMITE (see read order below)       -> ok, V per point
+!   COULOMB (see read order below)       -> ok, V per pair (coulomb_snglint)
+!   EULER   (see read order below)       -> site coordinates (matrix_b2feuler)
+!   GAUSS seed n                         -> n values of random_gaussian(0, 1)
+!   BONDED  (see read order below)       -> energies and forces of harmonic,
+!                                           harmonica and cosexpansion terms
+!   PAXES ctrlfile name molfile          -> principal moments and body-frame
+!                                           coordinates (molecule_init)
+!   R4CMP n ratio(1..n)                  -> ratio < 0.49, ratio > 0.51 with
+!                                           default-real literals, as in
+!                                           rigidmoves_adjustdeltatrans
 program ref_harness
   Use defaults, Only: RDbl, RSgl
   Use random, Only: random_init, rranf
@@ -26,6 +36,17 @@ program ref_harness
   Use ewald, Only: ewald_init, ewald_initCalc, ewald_getInteractionHOT
   Use pmap, Only: Pmap_Params, pmap_newmap, pmap_addpoint, pmap_compact, &
       pmap_write, pmap_pmapinit, pmap_int
+  Use coulomb, Only: Coulomb_Params, coulomb_snglint
+  Use matrix, Only: MatrixType, matrix_b2feuler, Operator(*)
+  Use random, Only: random_gaussian
+  Use harmonic, Only: HarmonicParams, harmonic_getinteraction
+  Use harmonica, Only: HarmonicAModel, harmonica_getinteraction
+  Use cosexpansion, Only: CosExpansionModel, cosexpansion_getinteraction
+  Use defaults, Only: d_ctrl_file
+  Use file, Only: file_settag
+  Use atom, Only: atom_init
+  Use molecule, Only: MolecularParams, molecule_init, molecule_getatomcoords, &
+      molecule_getmoi
   Implicit None
 
   Character(len=32) :: job
@@ -43,6 +64,18 @@ program ref_harness
     Call job_compact()
   Case ('HERMITE')
     Call job_hermite()
+  Case ('COULOMB')
+    Call job_coulomb()
+  Case ('EULER')
+    Call job_euler()
+  Case ('GAUSS')
+    Call job_gauss()
+  Case ('BONDED')
+    Call job_bonded()
+  Case ('PAXES')
+    Call job_paxes()
+  Case ('R4CMP')
+    Call job_r4cmp()
   Case Default
     Write(0,*) 'unknown job ', Trim(job)
     Stop 2
@@ -253,5 +286,156 @@ Contains
       Write(*,'(L2,ES26.17E3)') ok, out%nrg
     End Do
   End Subroutine job_hermite
+
+  ! COULOMB input: hicut locut shield; npairs; npairs x (sx sy sz q1 q2)
+  ! (hicut <= 0: no cutoff, as coulomb_init leaves it without HICUT)
+  Subroutine job_coulomb()
+    Type(Coulomb_Params) :: par
+    Type(EnergyPlus) :: res, res2
+    Type(VecType) :: sep
+    Real(RDbl) :: hic, loc, sh, v(5)
+    Integer :: n, i
+    Logical :: ok
+    Read(*,*) hic, loc, sh
+    par%off = .False.
+    par%initialized = .True.
+    par%hicut = hic
+    par%hicut2 = hic*hic
+    par%locut = loc
+    par%locut2 = loc*loc
+    par%shield_const = sh
+    par%charges_fromAA_FILE = .False.
+    par%Q1 = 0.0_RDbl
+    par%Q2 = 0.0_RDbl
+    Call storebase_init(res, 0)
+    Call storebase_init(res2, 0)
+    Read(*,*) n
+    Do i = 1, n
+      Read(*,*) v
+      sep%comp = v(1:3)
+      Call storebase_zero(res)
+      ok = coulomb_snglint(par, sep, v(4), v(5), res, res2)
+      Write(*,'(L2,ES26.17E3)') ok, res%nrg
+    End Do
+  End Subroutine job_coulomb
+
+  ! EULER input: natoms; natoms x (x y z) body frame; n;
+  ! n x (comx comy comz phi theta psi) -> natoms rows of xyz per molecule,
+  ! com + matrix_b2feuler(phi, theta, psi) * ref as rigidcoords_toxyz
+  ! (rigidcoords.F90:254-276)
+  Subroutine job_euler()
+    Type(MatrixType) :: m
+    Type(VecType), Allocatable :: ref(:)
+    Type(VecType) :: x
+    Real(RDbl) :: v(6)
+    Integer :: na, n, i, a
+    Read(*,*) na
+    Allocate(ref(na))
+    Do a = 1, na
+      Read(*,*) ref(a)%comp
+    End Do
+    Read(*,*) n
+    Do i = 1, n
+      Read(*,*) v
+      m = matrix_b2feuler(v(4), v(5), v(6))
+      Do a = 1, na
+        x = m*ref(a)
+        Write(*,'(3ES26.17E3)') v(1:3) + x%comp
+      End Do
+    End Do
+  End Subroutine job_euler
+
+  Subroutine job_gauss()
+    Integer :: seed, n, i
+    Read(*,*) seed, n
+    Call random_init(seed)
+    Do i = 1, n
+      Write(*,'(ES26.17E3)') random_gaussian(0.0_RDbl, 1.0_RDbl)
+    End Do
+  End Subroutine job_gauss
+
+  ! BONDED input: kind (1 harmonic bond k req, 2 harmonica angle k theta0,
+  ! 3 cosexpansion c0..c5), its parameters; n; n x (coordinates of the 2, 3
+  ! or 4 atoms) -> energy and the forces on each atom
+  Subroutine job_bonded()
+    Type(HarmonicParams) :: hb
+    Type(HarmonicAModel), Pointer :: ha
+    Type(CosExpansionModel) :: cx
+    Type(VecType) :: r(4), f(4)
+    Real(RDbl) :: p(6), c(12), u
+    Integer :: kind, n, i, na, a
+    Read(*,*) kind
+    Select Case (kind)
+    Case (1)
+      Read(*,*) p(1:2)
+      hb%k = p(1)
+      hb%req = p(2)
+      na = 2
+    Case (2)
+      Read(*,*) p(1:2)
+      Allocate(ha)
+      ha%ktheta = p(1)
+      ha%thetaeq = p(2)
+      na = 3
+    Case Default
+      Read(*,*) p(1:6)
+      cx%cn = p(1:6)
+      na = 4
+    End Select
+    Read(*,*) n
+    Do i = 1, n
+      Read(*,*) c(1:3*na)
+      Do a = 1, na
+        r(a)%comp = c(3*a-2:3*a)
+      End Do
+      Select Case (kind)
+      Case (1)
+        Call harmonic_getinteraction(hb, r(1:2), .False., u, f(1:2))
+      Case (2)
+        Call harmonica_getinteraction(ha, r(1:3), .False., u, f(1:3))
+      Case Default
+        Call cosexpansion_getinteraction(cx, r(1:4), .False., u, f(1:4))
+      End Select
+      Write(*,'(13ES26.17E3)') u, (f(a)%comp, a = 1, na)
+    End Do
+  End Subroutine job_bonded
+
+  ! PAXES input: control file (its atom types section), molecule name, file
+  ! name in $MOLSDIR -> natoms, principal moments, body-frame coordinates
+  ! after molecule_init (molecule_coordinit -> molecule_genprincipleaxes,
+  ! molecule.F90:982-1112, 830-926)
+  Subroutine job_paxes()
+    Character(len=255) :: ctrl, nm, fn
+    Type(MolecularParams) :: mol
+    Type(VecType) :: x
+    Real(RDbl) :: moi(3)
+    Integer :: a
+    Read(*,'(a)') ctrl
+    Read(*,'(a)') nm
+    Read(*,'(a)') fn
+    Call file_settag(Trim(ctrl), d_ctrl_file)
+    Call atom_init()
+    Call molecule_init(mol, Trim(nm), Trim(fn))
+    moi = molecule_getmoi(mol)
+    Write(*,'(A,I6)') 'NATOMS', mol%natoms
+    Write(*,'(A,3ES26.17E3)') 'MOI', moi
+    Do a = 1, mol%natoms
+      x = molecule_getatomcoords(mol, a)
+      Write(*,'(A,3ES26.17E3)') 'XYZ', x%comp
+    End Do
+  End Subroutine job_paxes
+
+  ! R4CMP: the comparison of rigidmoves_adjustdeltatrans
+  ! (rigidmoves.F90:2592-2597): a REAL(8) ratio against the literals 0.49
+  ! and 0.51 written without a kind, exactly as there
+  Subroutine job_r4cmp()
+    Integer :: n, i
+    Real(RDbl) :: ratio
+    Read(*,*) n
+    Do i = 1, n
+      Read(*,*) ratio
+      Write(*,'(2L2)') ratio < 0.49, ratio > 0.51
+    End Do
+  End Subroutine job_r4cmp
 
 End Program ref_harness

@@ -1,7 +1,8 @@
 """Generate tests/golden/ref_vectors.json from oracle/_ref.
 
 oracle/_ref/ref_harness is the reference's own Fortran modules (random,
-utils, lj, simcell, ewald, pmap, interpolate) compiled with flang from the
+utils, lj, simcell, ewald, pmap, interpolate, coulomb, matrix, molecule with
+eispack, harmonic, harmonica, cosexpansion) compiled with flang from the
 reference source tree (oracle/Makefile.ref).  This script feeds it seeded
 inputs and stores inputs + outputs as small JSON vectors so that the CPU
 oracle and the HIP library can be checked without the reference present.
@@ -184,6 +185,149 @@ def gen_compact_and_hermite():
                              ok=[r[0] for r in hrows], V=[r[1] for r in hrows]))
 
 
+# atoms of the principal-axes molecules: name, symbol, mass
+PAX_ATOMS = [("Ow", "O", 15.9994), ("Hw", "H", 1.008), ("Cx", "C", 12.011), ("Ox", "O", 15.9994),
+             ("Nx", "N", 14.0067), ("Sx", "S", 32.066)]
+
+
+def paxes_molecules():
+    """Molecules whose principal frames the reference computes: the bent
+    3-site sorbate of tests/golden/gcmc-maps, a linear CO2 along a random
+    axis (one zero and two equal moments), an asymmetric 4-site molecule, a
+    tetrahedral 5-site molecule (three equal moments), a planar 4-site
+    molecule with a 3-fold axis (two equal moments) and one with NOROTATE."""
+    rng = np.random.default_rng(6060)
+    mols = []
+    bent = inputs.read_molecule(os.path.join(GOLD, "gcmc-maps", "Bent.mol"))
+    mols.append(("Bent", list(zip(bent["atoms"], bent["xyz"].tolist())), ""))
+    ax = rng.normal(size=3)
+    ax /= np.linalg.norm(ax)
+    c0 = np.array([0.3, -1.2, 2.0])
+    mols.append(("Lin", [("Ox", (c0 - 1.16 * ax).tolist()), ("Cx", c0.tolist()), ("Ox", (c0 + 1.16 * ax).tolist())], ""))
+    mols.append(("Asym", [(n, (rng.normal(size=3) * 1.3).tolist()) for n in ("Cx", "Nx", "Ox", "Hw")], ""))
+    t = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], dtype=float) * 0.629
+    mols.append(("Tet", [("Cx", [0.0, 0.0, 0.0])] + [("Hw", v.tolist()) for v in t], ""))
+    tri = [[1.3 * np.cos(a), 1.3 * np.sin(a), 0.0] for a in (0.0, 2 * np.pi / 3, 4 * np.pi / 3)]
+    mols.append(("Tri", [("Sx", [0.0, 0.0, 0.0])] + [("Ox", v) for v in tri], ""))
+    mols.append(("Norot", [(n, (rng.normal(size=3) * 1.1 + 0.5).tolist()) for n in ("Cx", "Ox", "Hw")], "NOROTATE"))
+    return mols
+
+
+def write_paxes_inputs(td, mols):
+    """Atom files, a control file holding the atom types and one molecule
+    file per molecule (the reference's formats: atom.F90:68-264,
+    molecule.F90:481-618, 982-1112)."""
+    for nm, sym, mass in PAX_ATOMS:
+        with open(os.path.join(td, nm + ".atm"), "w") as f:
+            f.write("Atom_Name: %s\nAtom_Symbol: %s\nAtom_SS_Charge: 0.0\nAtom_SZ_Charge: 0.0\n"
+                    "Atom_Mass: %s\nAtom_Valency: 1\n" % (nm, sym, repr(mass)))
+    with open(os.path.join(td, "pax.ctr"), "w") as f:
+        f.write("------ General Information ------\nprincipal axes\n")
+        f.write("------ Atomic Types ------\n%d # number of atomic types\n\n" % len(PAX_ATOMS))
+        f.write("\n\n".join("%s\n%s.atm" % (nm, nm) for nm, _, _ in PAX_ATOMS) + "\n")
+    for name, atoms, flag in mols:
+        with open(os.path.join(td, name + ".mol"), "w") as f:
+            f.write("Molecule_Name: %s\n\nCoord_Info: Listed Cartesian Rigid %s\n" % (name, flag))
+            f.write("%d # number of atoms in molecule\n" % len(atoms))
+            for i, (an, x) in enumerate(atoms):
+                f.write("%d %s %s 0.0 0 0\n" % (i + 1, f17(*x), an))
+            f.write("\nConnect_Info: Generate\n  0\n")
+
+
+def gen_paxes():
+    mols = paxes_molecules()
+    res = []
+    with tempfile.TemporaryDirectory() as td:
+        write_paxes_inputs(td, mols)
+        env = dict(os.environ, ATOMSDIR=td + "/", MOLSDIR=td + "/")
+        for name, atoms, flag in mols:
+            out, _ = run(["PAXES", "pax.ctr", name, name + ".mol"], env=env, cwd=td)
+            moi, xyz = None, []
+            for l in out.splitlines():
+                f = l.split()
+                if not f:
+                    continue
+                if f[0] == "MOI":
+                    moi = [float(v) for v in f[1:4]]
+                elif f[0] == "XYZ":
+                    xyz.append([float(v) for v in f[1:4]])
+            res.append(dict(name=name, atoms=[a for a, _ in atoms], xyz_in=[x for _, x in atoms], flag=flag,
+                            moi=moi, xyz=xyz))
+    return dict(atoms=[list(a) for a in PAX_ATOMS], molecules=res)
+
+
+def gen_coulomb():
+    """coulomb_snglint on separations around the cutoffs and random ones."""
+    rng = np.random.default_rng(4041)
+    cases = []
+    for hic, loc, sh in ((9.5, 0.5, 1.0), (0.0, 0.8, 0.7), (12.0, 1e-3, 1.0)):
+        seps = [rng.normal(size=3) * 4.0 for _ in range(60)]
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        for d in (0.3, 0.5, 0.8 + 1e-9, 9.5 - 1e-9, 9.5, 9.5 + 1e-9, 11.999, 12.0001, 20.0):
+            seps.append(u * d)
+        qs = rng.uniform(-1, 1, (len(seps), 2))
+        lines = ["COULOMB", f17(hic, loc, sh), str(len(seps))]
+        lines += [f17(*sv, *q) for sv, q in zip(seps, qs)]
+        rows = parse_rows(run(lines)[0])
+        cases.append(dict(hicut=hic, locut=loc, shield=sh, sep=[list(map(float, v)) for v in seps],
+                          q=qs.tolist(), ok=[r[0] for r in rows], V=[r[1] for r in rows]))
+    return cases
+
+
+def gen_euler():
+    """rigidcoords_toxyz with matrix_b2feuler on a bent 3-site body frame."""
+    rng = np.random.default_rng(5051)
+    ref = [[0.0, 0.0, -1.16], [0.3, 0.1, 0.0], [-0.2, 0.4, 1.16]]
+    rows = []
+    for _ in range(40):
+        com = rng.uniform(2, 8, 3)
+        ang = [rng.uniform(-np.pi, np.pi), np.arccos(rng.uniform(-1, 1)), rng.uniform(-np.pi, np.pi)]
+        rows.append(list(com) + ang)
+    rows.append([5.0, 5.0, 5.0, 0.0, 0.0, 0.0])
+    rows.append([5.0, 5.0, 5.0, 3.1415927, 3.1415927, -3.1415927])
+    lines = ["EULER", "3"] + [f17(*v) for v in ref] + [str(len(rows))] + [f17(*v) for v in rows]
+    out = [[float(x) for x in l.split()] for l in run(lines)[0].splitlines() if l.strip()]
+    return dict(ref=ref, com_ang=rows, xyz=out)
+
+
+def gen_gauss():
+    res = {}
+    for seed in (5200, 22111971):
+        v = [float(x) for x in run(["GAUSS", "%d %d" % (seed, 401)])[0].split()]
+        res[str(seed)] = v
+    return res
+
+
+def gen_bonded():
+    rng = np.random.default_rng(7071)
+    out = []
+    # harmonic bond (k kcal/mol/A^2, req A), harmonica (k kcal/mol/rad^2,
+    # theta0 rad), cosexpansion c0..c5 (kcal/mol); 2, 3, 4 atoms
+    for kind, par, na in ((1, [900.0, 1.54], 2), (2, [124.2, 1.9896753], 3),
+                          (3, [2.007, 4.012, 0.271, -6.290, 0.0, 0.0], 4)):
+        pts = []
+        for _ in range(30):
+            p = [rng.normal(size=3) * 0.2]
+            for a in range(1, na):
+                d = rng.normal(size=3)
+                p.append(p[-1] + 1.54 * d / np.linalg.norm(d))
+            pts.append(np.concatenate(p))
+        lines = ["BONDED", str(kind), f17(*par), str(len(pts))] + [f17(*v) for v in pts]
+        vals = [[float(x) for x in l.split()] for l in run(lines)[0].splitlines() if l.strip()]
+        out.append(dict(kind=kind, par=par, coords=[v.tolist() for v in pts], u=[v[0] for v in vals],
+                        f=[v[1:] for v in vals]))
+    return out
+
+
+def gen_r4cmp():
+    rs = [0.49, 0.51, 49 / 100, 51 / 100, 98 / 200, 0.4900000095367432, 0.5099999904632568,
+          0.49000000953674317, 0.5, 0.48, 0.52]
+    out = run(["R4CMP", str(len(rs))] + ["%.17g" % r for r in rs])[0]
+    rows = [[t == "T" for t in l.split()] for l in out.splitlines() if l.strip()]
+    return dict(ratio=rs, below=[r[0] for r in rows], above=[r[1] for r in rows])
+
+
 def main():
     if not os.path.exists(HARNESS):
         raise SystemExit("build oracle/_ref first: make -f oracle/Makefile.ref")
@@ -193,6 +337,12 @@ def main():
     g["ljnode"] = gen_ljnode()
     g["ewald"] = gen_ewald()
     g.update(gen_compact_and_hermite())
+    g["paxes"] = gen_paxes()
+    g["coulomb"] = gen_coulomb()
+    g["euler"] = gen_euler()
+    g["gauss"] = gen_gauss()
+    g["bonded"] = gen_bonded()
+    g["r4cmp"] = gen_r4cmp()
     with open(os.path.join(GOLD, "ref_vectors.json"), "w") as f:
         json.dump(g, f)
     print("wrote", os.path.join(GOLD, "ref_vectors.json"))

@@ -72,15 +72,17 @@ def test_biased_moves_parameters():
 
 
 def test_bent_sorbate_parameters():
-    """Principal axes (Jacobi in the driver, numpy in the Python host, same
-    sign convention), the LBMIX pair and the Ewald parameters."""
+    """Principal axes (tred2 / tql2 order in the driver and in the Python
+    host, pinned to the reference in test_oracle_ref_moves.py), the LBMIX
+    pair and the Ewald parameters."""
     d = _check(os.path.join(FIX, "bent.ctr"))
     spec = bent_spec()
     _same_spec(d, spec, 1)
     ref = np.array(spec["ref"])
-    # principal frame: centre of (truncated) mass at the origin, inertia diagonal, ascending
+    # principal frame: centre of (real) mass at the origin, inertia of the
+    # truncated masses diagonal, ascending
+    assert np.allclose((ref * np.array([15.9994, 1.008, 1.008])[:, None]).sum(0), 0.0, atol=1e-12)
     m = np.array([15.0, 1.0, 1.0])
-    assert np.allclose((ref * m[:, None]).sum(0), 0.0, atol=1e-12)
     inertia = sum(mi * (v @ v * np.eye(3) - np.outer(v, v)) for mi, v in zip(m, ref))
     assert np.allclose(inertia - np.diag(np.diag(inertia)), 0.0, atol=1e-9)
     assert np.all(np.diff(np.diag(inertia)) >= 0)

@@ -388,29 +388,38 @@ ADJ_T = ("0.4, 0              # Del-R Ang., adjust delta?(0=NO, 1=YES)", "1.9, 1
 ADJ_R = ("0.4, 0              # Del-Theta Rad., adjust delta?(0=NO, 1=YES)", "0.4, 1              # Del-Theta Rad., adjust delta?(0=NO, 1=YES)")
 
 
+PRESS_1E5 = ("10000             # Fugacity Range (kPa)", "100000            # Fugacity Range (kPa)")
+
+
 def test_music_gcmc_adjustable_displacements_resume(hip):
-    """test007 with an adjustable translate displacement: the printed
-    "Max disp" moves away from its start within [0.01, 2.0], and a run split
-    at 10000 steps and resumed from its restart (.state holds the
-    displacements) ends in the restart file of the unbroken run."""
+    """test007 at 10^5 kPa (about 100 molecules, translate acceptance near
+    1/2) with an adjustable translate displacement: the printed "Max disp"
+    leaves its start and stays inside (0.01, 2.0), and a run split at 10000
+    steps and resumed from its restart (.state holds the displacements and
+    the attempt / acceptance counts that drive them) prints the same
+    displacements from step 10000 on and ends in the restart file of the
+    unbroken run."""
     import re
-    print("case: music_gcmc adjustable displacements, unbroken vs resumed (test007)", flush=True)
+    print("case: music_gcmc adjustable displacements, unbroken vs resumed (test007 at 1e5 kPa)", flush=True)
     with tempfile.TemporaryDirectory() as td:
-        ctrl = _t007_tree(td, icrash=(ICRASH[0], ICRASH[1] % 20000), adjt=ADJ_T, adjr=ADJ_R)
+        ctrl = _t007_tree(td, icrash=(ICRASH[0], ICRASH[1] % 20000), adjt=ADJ_T, adjr=ADJ_R, p=PRESS_1E5)
         out = MG.run([ctrl], td)
         disp = [float(x) for x in re.findall(r"Max disp:\s+([0-9.]+)", out)]
         # test007 moves: insert, delete, translate (one "Max disp" per block)
         assert len(disp) >= 20 and all(0.01 <= d <= 2.0 for d in disp)
         assert any(abs(d - 1.9) > 1e-3 for d in disp)
+        assert 0.02 < disp[-1] < 1.99, disp[-5:]
         full = open(os.path.join(td, "temp.res.1"), "rb").read().split(b"\n")
     with tempfile.TemporaryDirectory() as td:
         ctrl = _t007_tree(td, icrash=(ICRASH[0], ICRASH[1] % 10000), niter=(NITER[0], NITER[1] % 10000),
-                          adjt=ADJ_T, adjr=ADJ_R)
+                          adjt=ADJ_T, adjr=ADJ_R, p=PRESS_1E5)
         MG.run([ctrl], td)
         ctrl2 = _t007_tree(td, icrash=(ICRASH[0], ICRASH[1] % 10000), niter=(NITER[0], NITER[1] % 10000),
                            src=("GCMC NULL # Source of configs", "RESTARTFILE ../temp.res.1 # Source of configs"),
                            res=("temp.res   # Name of Restart File", "temp2.res   # Name of Restart File"),
-                           adjt=ADJ_T, adjr=ADJ_R)
-        MG.run([ctrl2, "resume"], td)
+                           adjt=ADJ_T, adjr=ADJ_R, p=PRESS_1E5)
+        out2 = MG.run([ctrl2, "resume"], td)
+        disp2 = [float(x) for x in re.findall(r"Max disp:\s+([0-9.]+)", out2)]
+        assert disp2[-len(disp) // 2:] == disp[-len(disp) // 2:]
         got = open(os.path.join(td, "temp2.res.1"), "rb").read().split(b"\n")
         assert got[:1] + got[2:] == full[:1] + full[2:]
