@@ -932,6 +932,15 @@ contains
        if (self) then
           if (trim(kind) == 'NCOUL') then
              if (index(upper(s), 'LJ') == 0) call fail('species-species NCOUL model must be LJ: ' // trim(s))
+             ! ssbasic_init options (ssbasic.F90:125-177): FAST is what the
+             ! kernel does; a centre-of-mass cutoff (COMCUT@, COMTRUNC@),
+             ! SMOOTH@ and NEIGHBOR change the energy and are not offered
+             do wv = 4, nwords(s)
+                tok = upper(word(s, wv))
+                if (trim(tok) == 'BASIC' .or. trim(tok) == 'LJ' .or. trim(tok) == 'FAST') cycle
+                call fail('species-species option ' // trim(word(s, wv)) // ' is not offered (BASIC LJ FAST): ' &
+                     // trim(s))
+             end do
              any_lj = .true.
              do b = 1, nst
                 do c = 1, nst
@@ -948,7 +957,14 @@ contains
                 end do
              end do
           else if (trim(upper(word(s, 4))) == 'BASIC' .and. index(upper(s), 'WFCOUL') == 0) then
-             ! plain Coulomb from the atom-atom COUL lines (coulomb_snglint)
+             ! plain Coulomb from the atom-atom COUL lines (coulomb_snglint);
+             ! the ssbasic options other than FAST are refused as for NCOUL
+             do wv = 5, nwords(s)
+                tok = upper(word(s, wv))
+                if (trim(tok) == 'FAST') cycle
+                call fail('species-species option ' // trim(word(s, wv)) // ' is not offered (BASIC FAST): ' &
+                     // trim(s))
+             end do
              call plain_coulomb()
           else
              if (index(upper(s), 'EWALD') == 0) &
@@ -964,11 +980,13 @@ contains
              end do
              if (kappa_param <= 0.0d0 .or. P%ewald_kmax <= 0) call fail('Ewald needs KAPPA@ and KMAX@: ' // trim(s))
              ! kappa in 1/A from the reference's kappa parameter (ewald.F90:
-             ! kappa = KAPPA / shortest cell edge); real-space cutoff by
-             ! default where erfc(kappa r) falls below 1e-9 (kappa r = 4.32)
+             ! kappa = KAPPA / shortest cell edge); without RCUT@ every
+             ! minimum-image pair enters the real-space sum, as the ewald
+             ! module's default rcut = -1 (ewald.F90:111-114, 812): a cutoff
+             ! beyond the longest minimum-image distance
              P%ewald = 1
              P%ewald_kappa = kappa_param / minval(sy%full)
-             if (rcut <= 0.0d0) rcut = min(4.32d0 / P%ewald_kappa, 0.5d0 * minval(sy%full))
+             if (rcut <= 0.0d0) rcut = 2.0d0 * maxval(sy%full)
              P%ewald_rcut = rcut
           end if
        else

@@ -119,7 +119,8 @@ def bent_spec():
     """bent.ctr: sites Ow, Hw, Hw on their principal axes (masses 15.9994,
     1.008, truncated), LJ Ow-Ow 3.166 / 78.2, Hw-Hw 1.2 / 10, Ow-Hw by
     Lorentz-Berthelot (sigma mean, eps geometric mean), hicut 9; Ewald
-    kappa 6.0 / 20 A, kmax 5, real-space cutoff min(4.32 / kappa, L / 2)."""
+    kappa 6.0 / 20 A, kmax 5, no real-space cutoff (every minimum-image
+    pair: the driver's default, 2 L)."""
     m = inputs.read_molecule(os.path.join(FIX, "Bent.mol"))
     ref = inputs.principal_axes(m["xyz"], [15.9994, 1.008, 1.008])
     pars = {(0, 0): (3.166, 78.2, 0.5), (1, 1): (1.2, 10.0, 0.3),
@@ -135,7 +136,7 @@ def bent_spec():
             l2.append(lo * lo)
     kappa = 6.0 / 20.0
     return dict(full_ell=[20.0] * 3, fund_ell=[20.0] * 3, ref=ref, stype=[0, 1, 1], sq=[-0.8, 0.4, 0.4], nst=2,
-                lj=(A, B, h2, l2, [1] * 4), T=300.0, ewald=dict(kappa=kappa, kmax=5, rc=min(4.32 / kappa, 10.0)),
+                lj=(A, B, h2, l2, [1] * 4), T=300.0, ewald=dict(kappa=kappa, kmax=5, rc=40.0),
                 moves=[("RINSERT", 1.0), ("RDELETE", 1.0), ("RTRANSLATE", 1.0), ("RROTATE", 1.0)],
                 dtrans=0.3, drot=0.3)
 

@@ -135,3 +135,28 @@ def test_pressure_file(tmp_path):
     (tmp_path / "press.dat").write_text("CH4\n2\n 10.0 20.0\n")
     out = subprocess.run([GCMC, str(ctrl), "check"], capture_output=True, text=True, timeout=60, cwd=tmp_path)
     assert out.returncode != 0 and "differs from the ctrlfile" in out.stdout + out.stderr
+
+
+@pytest.mark.parametrize("line", ["Bent     Bent     NCOUL BASIC LJ FAST COMCUT@9.0",
+                                  "Bent     Bent     NCOUL BASIC LJ FAST SMOOTH@8.0",
+                                  "Bent     Bent     NCOUL BASIC LJ FAST NEIGHBOR"])
+def test_unsupported_species_options_fail(tmp_path, line):
+    """ssbasic options that change the energy (the centre-of-mass cutoff of
+    ssbasic_mmint, ssbasic.F90:139-160, 745-756; smoothing; neighbour lists)
+    stop the driver with a message instead of being ignored."""
+    shutil.copytree(FIX, tmp_path / "fx", copy_function=shutil.copyfile)
+    ss = tmp_path / "fx" / "spc_spc_bent"
+    txt = ss.read_text().replace("Bent     Bent     NCOUL BASIC LJ FAST", line)
+    ss.write_text(txt)
+    out = subprocess.run([GCMC, str(tmp_path / "fx" / "bent.ctr"), "check"], capture_output=True, text=True,
+                         timeout=60)
+    assert out.returncode != 0
+    assert "is not offered" in out.stdout + out.stderr
+
+
+def test_ewald_default_has_no_real_space_cutoff():
+    """Without RCUT@ every minimum-image pair enters the real-space sum (the
+    ewald module's default rcut = -1, ewald.F90:111-114): the cutoff handed
+    to the kernel exceeds every minimum-image distance of the 20 A cell."""
+    d = _check(os.path.join(FIX, "bent.ctr"))
+    assert d["ewald"]["rc"] > 20.0 * np.sqrt(3.0) / 2.0
