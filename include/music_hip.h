@@ -78,8 +78,16 @@ int mh_ff_set_buck(mh_ctx *ctx, int ntypes, const double *A, const double *rho,
 
 /* Ewald parameters and framework structure factor (ewald_init,
  * ewald_initCalc -> ewald_sfactor, ewald.F90:99-216, 558-681).  kappa =
- * kappa_param / min(full_ell).  Returns the number of accepted k vectors. */
-int mh_ewald_set(mh_ctx *ctx, double kappa_param, int kmax, double locut, int *nk_out);
+ * kappa_param / min(full_ell).  Returns the number of accepted k vectors.
+ * sign_mode selects the reciprocal-space part of Ewald maps:
+ * MH_EWALD_PARITY reproduces ewald_skwaldHOT (ewald.F90:966-1005), whose
+ * potential and second derivatives use s cos - si sin (the potential of the
+ * framework inverted through the origin) while the first and third
+ * derivatives use s sin - si cos; MH_EWALD_PHYSICS gives the potential of
+ * the framework itself, s cos + si sin, with all seven derivatives its own.
+ * The two agree when the framework is centrosymmetric about the origin. */
+enum { MH_EWALD_PARITY = 0, MH_EWALD_PHYSICS = 1 };
+int mh_ewald_set(mh_ctx *ctx, double kappa_param, int kmax, double locut, int sign_mode, int *nk_out);
 /* copy out k[nk][3], s[nk], si[nk] (any pointer may be NULL) */
 int mh_ewald_get(mh_ctx *ctx, double *k, double *s, double *si, double *kappa);
 

@@ -217,8 +217,10 @@ int mh_ff_set_buck(mh_ctx *ctx, int ntypes, const double *A, const double *rho,
     return MH_OK;
 }
 
-int mh_ewald_set(mh_ctx *ctx, double kappa_param, int kmax, double locut, int *nk_out) {
+int mh_ewald_set(mh_ctx *ctx, double kappa_param, int kmax, double locut, int sign_mode, int *nk_out) {
     MH_CHECK(ctx, MH_EINVAL, "mh_ewald_set: ctx is NULL");
+    MH_CHECK(sign_mode == MH_EWALD_PARITY || sign_mode == MH_EWALD_PHYSICS, MH_EINVAL,
+             "mh_ewald_set: sign_mode must be MH_EWALD_PARITY or MH_EWALD_PHYSICS");
     MH_CHECK(ctx->natoms > 0, MH_ESTATE, "mh_ewald_set: upload the framework first");
     MH_CHECK(kmax >= 1 && kmax <= 40, MH_EINVAL, "mh_ewald_set: kmax out of range [1,40]");
     MH_CHECK(kappa_param > 0.0, MH_EINVAL, "mh_ewald_set: kappa must be > 0");
@@ -231,6 +233,7 @@ int mh_ewald_set(mh_ctx *ctx, double kappa_param, int kmax, double locut, int *n
     ew.kappa = kappa_param / mn;
     ew.kmax = kmax;
     ew.locut = locut;
+    ew.sign_mode = sign_mode;
     double ialpha = 1.0 / ell[0], ibeta = 1.0 / ell[1], igamma = 1.0 / ell[2];
     double ikap = 1.0 / ew.kappa;
     double pk = kPi * ikap;

@@ -59,6 +59,7 @@ struct Ewald {
     bool set = false;
     double kappa_param = 0, kappa = 0, locut = 0;
     int kmax = 0, nk = 0;
+    int sign_mode = 0;    // MH_EWALD_PARITY or MH_EWALD_PHYSICS
     DBuf k, s, si;        // accepted vectors, reference order
     DBuf ccoef;           // complex coefficient grid c(mx,my,mz) = s + i si, 0 if not kept
     std::vector<double> hk, hs, hsi;

@@ -649,9 +649,13 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
 }
 
 // ------------------------------------------------ Ewald reciprocal part
-// V(r) = Re sum_k c_k e^{ik.r}, with c_k = s_k + i si_k (ewald_skwaldHOT,
-// ewald.F90:966-1005: realPart = s cos - si sin); the derivative terms use
-// imagPart = s sin - si cos = Im(conj(c_k) e^{ik.r}).  k lies on the lattice
+// Parity mode: V(r) = Re sum_k c_k e^{ik.r}, with c_k = s_k + i si_k
+// (ewald_skwaldHOT, ewald.F90:966-1005: realPart = s cos - si sin, which
+// also feeds the second derivatives); the first and third derivative terms
+// use imagPart = s sin - si cos = Im(conj(c_k) e^{ik.r}).  Physics mode:
+// the framework's own potential Re sum conj(c_k) e^{ik.r} = s cos + si sin
+// with dV/dx = -kx Im(.), d2V/dxdy = -kx ky Re(.), d3V = +kx ky kz Im(.),
+// all from the conj(c) stage.  k lies on the lattice
 // 2 pi (mx/Lx, my/Ly, mz/Lz) and the nodes on the grid lattice, so every
 // output is a separable sum evaluated axis by axis:
 //   stage Z: Zc_w(mx,my,z) = sum_mz c kz^w e^{i kz z}, Zb_w likewise with conj(c)
@@ -693,7 +697,7 @@ __global__ void __launch_bounds__(256) k_recip_xy(int side, int nx, int ny, int 
                                                   const double2 *__restrict__ Ey,
                                                   const double *__restrict__ kxv,
                                                   const double *__restrict__ kyv, double charge1,
-                                                  double *__restrict__ recip) {
+                                                  int physics, double *__restrict__ recip) {
     // block per (y, z) line
     __shared__ double2 Y[MAXSIDE * 8];
     const int line = blockIdx.x;
@@ -701,7 +705,7 @@ __global__ void __launch_bounds__(256) k_recip_xy(int side, int nx, int ny, int 
     for (int t = threadIdx.x; t < side * 8; t += blockDim.x) {
         int mx = t / 8, o = t % 8;
         // o: 0 Yc00, 1 Yc10, 2 Yc01, 3 Yc11, 4 Yb00, 5 Yb10, 6 Yb01, 7 Yb11
-        int zsel = (o < 4) ? ((o & 2) ? 1 : 0) : ((o & 2) ? 3 : 2);
+        int zsel = (o < 4 && !physics) ? ((o & 2) ? 1 : 0) : ((o & 2) ? 3 : 2);
         bool kyw = (o & 1);
         double2 acc = make_double2(0, 0);
         for (int my = 0; my < side; my++) {
@@ -744,7 +748,7 @@ __global__ void __launch_bounds__(256) k_recip_xy(int side, int nx, int ny, int 
         o[4] = -h5 * charge1;
         o[5] = -h6 * charge1;
         o[6] = -h7 * charge1;
-        o[7] = -h8 * charge1;
+        o[7] = (physics ? h8 : -h8) * charge1;
     }
 }
 
@@ -998,7 +1002,8 @@ static int build_recip(mh_ctx *ctx, const int np[3], const double step[3], doubl
     MH_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_recip_xy, dim3((unsigned)((long long)np[1] * np[2])), dim3(256), 0,
                        ctx->stream, side, np[0], np[1], np[2], Z, (const double2 *)dE[0],
-                       (const double2 *)dE[1], dk[0], dk[1], charge1, recip);
+                       (const double2 *)dE[1], dk[0], dk[1], charge1, ew.sign_mode == MH_EWALD_PHYSICS ? 1 : 0,
+                       recip);
     MH_HIP(hipGetLastError());
     return MH_OK;
 }

@@ -18,6 +18,7 @@ module music_hip_mod
        MH_EIO = 4, MH_ESTATE = 5, MH_ECAPACITY = 6
   integer(c_int), parameter, public :: MH_MAP_LJ = 1, MH_MAP_EWALD = 2, MH_MAP_BUCK = 3
   integer(c_int), parameter, public :: MH_PREC_FP64 = 0, MH_PREC_FP32 = 1
+  integer(c_int), parameter, public :: MH_EWALD_PARITY = 0, MH_EWALD_PHYSICS = 1
   integer(c_int), parameter, public :: MH_MV_INSERT = 0, MH_MV_DELETE = 1, MH_MV_TRANSLATE = 2, &
        MH_MV_ROTATE = 3, MH_MV_BINSERT = 4, MH_MV_BDELETE = 5
   integer, parameter, public :: MH_MAX_SITES = 8
@@ -114,11 +115,11 @@ module music_hip_mod
        integer(c_int) :: rc
      end function mh_ff_set_buck
 
-     function mh_ewald_set(ctx, kappa_param, kmax, locut, nk) bind(C, name="mh_ewald_set") result(rc)
+     function mh_ewald_set(ctx, kappa_param, kmax, locut, sign_mode, nk) bind(C, name="mh_ewald_set") result(rc)
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx
        real(c_double), value :: kappa_param, locut
-       integer(c_int), value :: kmax
+       integer(c_int), value :: kmax, sign_mode
        integer(c_int), intent(out) :: nk
        integer(c_int) :: rc
      end function mh_ewald_set

@@ -20,7 +20,7 @@ program music_mapmaker
   integer :: i, k, nat, njobs, ifw, iprobe, ptype, nt
   real(c_double), allocatable :: ljA(:), ljB(:), h2(:), l2(:)
   integer(c_int8_t), allocatable :: on(:)
-  integer(c_int) :: prec, nk, npnts(3)
+  integer(c_int) :: prec, sign_mode, nk, npnts(3)
   integer(c_int64_t) :: nfsize, counts(4)
   real(c_double) :: step(3)
   type(c_ptr) :: ctx, map
@@ -32,12 +32,18 @@ program music_mapmaker
 
   call get_command_argument(1, ctrl)
   if (len_trim(ctrl) == 0) then
-     write(*, '(a)') 'usage: music_mapmaker ctrlfile [fp32]'
+     write(*, '(a)') 'usage: music_mapmaker ctrlfile [fp32] [physics]'
      stop 1
   end if
+  ! fp32: FP32 pair arithmetic with FP64 sums; physics: Ewald maps of the
+  ! framework itself (the reference's reciprocal-space signs otherwise)
   prec = MH_PREC_FP64
-  call get_command_argument(2, arg)
-  if (trim(lower(arg)) == 'fp32') prec = MH_PREC_FP32
+  sign_mode = MH_EWALD_PARITY
+  do i = 2, command_argument_count()
+     call get_command_argument(i, arg)
+     if (trim(lower(arg)) == 'fp32') prec = MH_PREC_FP32
+     if (trim(lower(arg)) == 'physics') sign_mode = MH_EWALD_PHYSICS
+  end do
   call read_system(ctrl, sy)
   if (sy%ifw == 0) call fail('the map maker needs a fundamental-cell species')
   nat = sy%nat
@@ -79,7 +85,7 @@ program music_mapmaker
      else if (trim(jobs(k)%model) == 'EWALD') then
         ew = read_ewald(find_file(sy%ssfile, 'NONE', '', sy%base), jobs(k)%probe, jobs(k)%mapsorb)
         if (.not. ew%found) call fail('no EWALD entry for the probe and map species in the spc_spc file')
-        call check(mh_ewald_set(ctx, ew%kappa, int(ew%kmax, c_int), ew%locut, nk), 'mh_ewald_set')
+        call check(mh_ewald_set(ctx, ew%kappa, int(ew%kmax, c_int), ew%locut, sign_mode, nk), 'mh_ewald_set')
         call check(mh_map_build(ctx, MH_MAP_EWALD, prec, sy%mols(iprobe)%q(1), jobs(k)%gridspace, &
              jobs(k)%hipotcut, 0_c_int, map), 'mh_map_build')
      else

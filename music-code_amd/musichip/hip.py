@@ -16,6 +16,7 @@ LIBPATH = os.path.join(os.path.dirname(HERE), "lib", "libmusic_hip.so")
 
 MH_MAP_LJ, MH_MAP_EWALD, MH_MAP_BUCK = 1, 2, 3
 MH_PREC_FP64, MH_PREC_FP32 = 0, 1
+MH_EWALD_PARITY, MH_EWALD_PHYSICS = 0, 1
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -69,7 +70,7 @@ def lib():
         "mh_framework_upload": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
         "mh_ff_set_lj": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
         "mh_ff_set_buck": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
-        "mh_ewald_set": [_vp, _d, _i, _d, ctypes.POINTER(_i)],
+        "mh_ewald_set": [_vp, _d, _i, _d, _i, ctypes.POINTER(_i)],
         "mh_ewald_get": [_vp, _vp, _vp, _vp, _vp],
         "mh_map_build": [_vp, _i, _i, _d, _d, _d, _i, ctypes.POINTER(_vp)],
         "mh_map_destroy": [_vp],
@@ -235,10 +236,12 @@ class Context:
         a = [np.ascontiguousarray(v, dtype=np.float64) for v in (A, rho, C, D, hicut2, locut2)]
         check(lib().mh_ff_set_buck(self.h, len(a[0]), *[_p(v) for v in a]), "mh_ff_set_buck")
 
-    def set_ewald(self, kappa_param, kmax, locut):
+    def set_ewald(self, kappa_param, kmax, locut, sign_mode=MH_EWALD_PARITY):
+        """Ewald parameters and structure factor; sign_mode MH_EWALD_PARITY
+        (the reference's skwaldHOT signs) or MH_EWALD_PHYSICS."""
         nk = _i()
-        check(lib().mh_ewald_set(self.h, float(kappa_param), int(kmax), float(locut), ctypes.byref(nk)),
-              "mh_ewald_set")
+        check(lib().mh_ewald_set(self.h, float(kappa_param), int(kmax), float(locut), int(sign_mode),
+                                 ctypes.byref(nk)), "mh_ewald_set")
         return nk.value
 
     def ewald_get(self, nk):

@@ -49,9 +49,11 @@ def header_zatoms(system, mapsorb, probe_type, lj=True):
     return out
 
 
-def make_maps(ctrl_path, ctx=None, precision=H.MH_PREC_FP64, outdir=None, write=True):
-    """Build every map of the ctrlfile's Mapmaker section.  Returns a list of
-    (job, Map, seconds)."""
+def make_maps(ctrl_path, ctx=None, precision=H.MH_PREC_FP64, outdir=None, write=True,
+              sign_mode=H.MH_EWALD_PARITY):
+    """Build every map of the ctrlfile's Mapmaker section (Ewald maps with the
+    reference's reciprocal-space signs unless sign_mode is MH_EWALD_PHYSICS).
+    Returns a list of (job, Map, seconds)."""
     system = inputs.load_system(ctrl_path)
     own = ctx is None
     ctx = ctx or H.Context(0)
@@ -70,7 +72,7 @@ def make_maps(ctrl_path, ctx=None, precision=H.MH_PREC_FP64, outdir=None, write=
             zat = header_zatoms(system, job["mapsorb"], ptype)
         elif job["model"].upper() == "EWALD":
             e = system["ss"][(job["probe"], job["mapsorb"], "COUL")]["ewald"]
-            ctx.set_ewald(e["kappa"], e["kmax"], e["locut"])
+            ctx.set_ewald(e["kappa"], e["kmax"], e["locut"], sign_mode)
             m = ctx.build_map(H.MH_MAP_EWALD, precision, job["gridspace"], job["hipotcut"],
                               probe_q=float(probe["q"][0]))
             zat = header_zatoms(system, job["mapsorb"], ptype, lj=False)

@@ -233,3 +233,80 @@ def test_small_charged_config_fp64_and_fp32(hip):
     _, e32 = m4.get_full()
     live = ref[:, 0] < 100.0 / 4.184
     assert fp32_err(e32, ref, live, "charged box Ewald") < TOL32_EWALD
+
+
+def _fd_columns(V, npn, h):
+    """Central differences of node values V (x fastest) -> the 7 derivative
+    columns at interior nodes, order x, y, z, xy, xz, yz, xyz."""
+    v = V.reshape(npn[2], npn[1], npn[0]).transpose(2, 1, 0)  # [x][y][z]
+    c = (slice(1, -1),) * 3
+
+    def sh(dx, dy, dz):
+        return v[1 + dx:npn[0] - 1 + dx, 1 + dy:npn[1] - 1 + dy, 1 + dz:npn[2] - 1 + dz]
+    out = [(sh(1, 0, 0) - sh(-1, 0, 0)) / (2 * h[0]), (sh(0, 1, 0) - sh(0, -1, 0)) / (2 * h[1]),
+           (sh(0, 0, 1) - sh(0, 0, -1)) / (2 * h[2])]
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        def e(sa, sb):
+            d = [0, 0, 0]
+            d[a], d[b] = sa, sb
+            return sh(*d)
+        out.append((e(1, 1) - e(1, -1) - e(-1, 1) + e(-1, -1)) / (4 * h[a] * h[b]))
+    t = 0
+    for sx in (1, -1):
+        for sy in (1, -1):
+            for sz in (1, -1):
+                t = t + sx * sy * sz * sh(sx, sy, sz)
+    out.append(t / (8 * h[0] * h[1] * h[2]))
+    return out, c
+
+
+def test_ewald_physics_mode_derivatives_are_those_of_the_potential(hip):
+    """A framework that is not centrosymmetric about the origin: in physics
+    mode (MH_EWALD_PHYSICS) every derivative column equals the finite
+    differences of the map's own potential; in parity mode (the reference's
+    signs, ewald.F90:987-1002) the potential belongs to the inverted
+    framework, so some columns disagree with the differences of V; and both
+    modes give the same real-space part, the parity mode staying
+    bit-identical to the oracle."""
+    rng = np.random.default_rng(808)
+    L = np.array([12.0, 12.0, 12.0])
+    n = 24
+    xyz = rng.uniform(0, 1, (n, 3)) * L
+    q = rng.uniform(-1, 1, n)
+    q -= q.mean()
+    hip.upload_framework(xyz, np.zeros(n, dtype=np.int32), q, L, L)
+    h = 0.1
+    npn = (L / h).astype(int) + 1
+    step = L / (npn - 1)
+    errs = {}
+    maps = {}
+    for mode in (H.MH_EWALD_PHYSICS, H.MH_EWALD_PARITY):
+        print("case: Ewald map sign mode %d, finite differences" % mode, flush=True)
+        hip.set_ewald(4.0, 7, 1e-12, sign_mode=mode)
+        m = hip.build_map(H.MH_MAP_EWALD, H.MH_PREC_FP64, h, 1.0e6, probe_q=1.0, keep_full=True)
+        _, f64 = m.get_full()
+        maps[mode] = f64
+        fd, _ = _fd_columns(f64[:, 0], npn, step)
+        # nodes at least 2.5 A from every atom (smooth region)
+        g = np.stack(np.meshgrid(*[np.arange(k) * s for k, s in zip(npn, step)], indexing="ij"), -1)[1:-1, 1:-1, 1:-1]
+        dmin = np.full(g.shape[:3], np.inf)
+        for a in xyz:
+            d = g - a
+            d -= L * np.round(d / L)
+            dmin = np.minimum(dmin, np.sqrt((d * d).sum(-1)))
+        far = dmin > 2.5
+        cols = f64[:, 1:].reshape(npn[2], npn[1], npn[0], 7).transpose(2, 1, 0, 3)[1:-1, 1:-1, 1:-1]
+        e = []
+        for c in range(7):
+            a, b = cols[..., c][far], fd[c][far]
+            e.append(float(np.abs(a - b).max() / np.abs(b).max()))
+        errs[mode] = e
+        print("  max |analytic - FD| / max |FD| per column:", ["%.2e" % x for x in e], flush=True)
+        m.close()
+    assert max(errs[H.MH_EWALD_PHYSICS]) < 1e-2
+    assert max(errs[H.MH_EWALD_PARITY]) > 0.1
+    # parity mode = the oracle (reference arithmetic, pinned to the flang build)
+    sf = O.ewald_sfactor(xyz, q, L, 4.0, 7, 1e-12)
+    ref = O.build_nodes(2, dict(xyz=xyz, q=q, charge1=1.0, kappa=sf["kappa"], k=sf["k"], s=sf["s"],
+                                si=sf["si"], full_ell=L), npn, step, 0, 20000)
+    rel_close(maps[H.MH_EWALD_PARITY][:20000], ref, TOL64, "parity-mode Ewald nodes")
