@@ -38,15 +38,17 @@ HIPOT_KJ = 100.0
 FUGACITIES_KPA = (100.0, 1000.0, 5000.0)
 
 
-def screening_request(rid, steps=10000):
-    """Seeded synthetic request (seed 4000 + rid): cell edges U[10, 45] A,
-    100-6000 atoms per cell (capped at 0.1 atoms / A^3), charged fraction
-    U[0, 0.3] with q ~ U[-1, 1] neutralised, LJ types as the methane
-    framework; the cell is replicated so that 2 rc < L_full on every axis."""
-    rng = np.random.default_rng(4000 + rid)
+def screening_request(rid, steps=10000, framework=None):
+    """Seeded synthetic request: framework `framework` (default rid; seed
+    4000 + framework) with cell edges U[10, 45] A, U[100, 6000] atoms per
+    cell, charged fraction U[0, 0.3] with q ~ U[-1, 1] neutralised, LJ
+    types as the methane framework; the cell is replicated so that
+    2 rc < L_full on every axis.  The GCMC seeds follow the request id
+    (4000 + rid + 10000 j for the j-th fugacity)."""
+    fid = rid if framework is None else framework
+    rng = np.random.default_rng(4000 + fid)
     L = rng.uniform(10.0, 45.0, 3)
     n = int(rng.integers(100, 6001))
-    n = max(1, min(n, int(0.1 * np.prod(L))))
     xyz = rng.uniform(0.0, 1.0, (n, 3)) * L
     eps, sig, typ = synth.fw_types(rng, n)
     q = np.zeros(n)
@@ -58,8 +60,20 @@ def screening_request(rid, steps=10000):
     rep = (np.floor(2.0 * HICUT / L) + 1).astype(int)
     shifts = np.array([[i, j, k] for i in range(rep[0]) for j in range(rep[1]) for k in range(rep[2])], dtype=float)
     full_xyz = (xyz[None, :, :] + shifts[:, None, :] * L).reshape(-1, 3)
-    return dict(rid=rid, L=L, full=L * rep, xyz=full_xyz, type=np.tile(typ, len(shifts)), q=np.tile(q, len(shifts)),
-                eps=eps, sig=sig, steps=steps, seeds=[4000 + rid + 10000 * j for j in range(len(FUGACITIES_KPA))])
+    return dict(rid=rid, framework=fid, L=L, full=L * rep, xyz=full_xyz, type=np.tile(typ, len(shifts)),
+                q=np.tile(q, len(shifts)), eps=eps, sig=sig, steps=steps,
+                seeds=[4000 + rid + 10000 * j for j in range(len(FUGACITIES_KPA))])
+
+
+def request_frameworks(n, popular=16, share=0.25, seed=4998):
+    """Framework of each of n requests: a fraction `share` of them revisits
+    one of `popular` frameworks (a screening campaign returning to its
+    leading candidates), the others each have their own."""
+    rng = np.random.default_rng(seed)
+    fw = np.arange(n)
+    pick = rng.random(n) < share
+    fw[pick] = rng.integers(0, popular, int(pick.sum()))
+    return [int(f) for f in fw]
 
 
 def framework_key(req):
@@ -95,7 +109,7 @@ class MapCache:
     def __init__(self, budget_bytes):
         self.budget = budget_bytes
         self.entries = collections.OrderedDict()  # key -> [map, bytes, pins]
-        self.hits = self.misses = 0
+        self.hits = self.misses = self.evictions = 0
 
     def get(self, key):
         e = self.entries.get(key)
@@ -126,6 +140,7 @@ class MapCache:
             if pins == 0:
                 m.close()
                 del self.entries[k]
+                self.evictions += 1
 
 
 class Server:
@@ -193,8 +208,10 @@ class Server:
                     mem_waits += 1
                     break
                 pending.popleft()
+                miss0 = self.cache.misses
                 key, m = self.build_map(requests[i])
-                map_pts += map_nodes(requests[i])
+                if self.cache.misses != miss0:
+                    map_pts += map_nodes(requests[i])
                 used += fp
                 peak = max(peak, used)
                 prepared.append((i, key, m, fp))
@@ -241,6 +258,7 @@ class Server:
                      map_pts_per_s=map_pts / makespan, launches=launches, chunk=self.chunk,
                      peak_admitted_bytes=peak, hbm_budget_bytes=self.hbm_budget, admission_mem_waits=mem_waits,
                      map_cache_hits=self.cache.hits, map_cache_misses=self.cache.misses,
+                     map_cache_evictions=self.cache.evictions,
                      max_active_requests=self.max_active)
         return results, stats
 

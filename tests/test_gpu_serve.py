@@ -63,3 +63,27 @@ def test_serving_alone_equals_mix():
     # with 12 slots at most 4 requests are admitted together
     assert st2["max_active_requests"] == 4
     print("alone %s / mix %s" % (st1, st2), flush=True)
+
+
+def test_serving_map_cache_reuse_and_eviction():
+    """Requests that revisit frameworks hit the device map cache; with a
+    cache budget of about two maps, unpinned maps are evicted least recently
+    used and rebuilt on a later miss.  Every request's results equal those
+    of the same request served alone on a fresh server."""
+    print("case: screening requests with repeated frameworks, small map cache", flush=True)
+    fws = [0, 1, 0, 2, 1, 3, 0, 2]
+    reqs = [serve.screening_request(r, steps=1000, framework=f) for r, f in enumerate(fws)]
+    per_map = max(serve.map_nodes(r) for r in reqs) * 68
+    srv = serve.Server(nslots=6, chunk=250, maxN=256, hbm_budget_gb=4.0, cache_gb=2.2 * per_map / 1e9)
+    arr = np.array([0.0, 0.0, 0.05, 0.05, 0.1, 0.1, 0.15, 0.2])
+    mix, st = srv.run(reqs, arr)
+    srv.close()
+    print(st, flush=True)
+    assert st["map_cache_hits"] > 0 and st["map_cache_evictions"] > 0
+    assert st["map_cache_hits"] + st["map_cache_misses"] == len(reqs)
+    for i in (2, 5, 7):
+        srv = serve.Server(nslots=6, chunk=250, maxN=256, hbm_budget_gb=4.0)
+        alone, _ = srv.run([reqs[i]], np.array([0.0]))
+        srv.close()
+        for k in ("N", "energy", "mean_N", "att", "acc", "err"):
+            assert np.array_equal(alone[0][k], mix[i][k]), (i, k)
