@@ -104,6 +104,200 @@ void mo_minimage(const double ell[3], const double a[3], const double b[3], doub
     }
 }
 
+/* ------------------------------------------------------------ cells */
+/* matrix_inverse (matrix.F90:195-229): LU decomposition with implicit
+ * scaling and partial pivoting, then one back substitution per column
+ * (matrixops_ludcmpReal / matrixops_lubksbReal, matrixops.F90:31-161) */
+static void lu_inverse3(const double m[3][3], double inv[3][3])
+{
+    double a[3][3], vv[3];
+    int indx[3];
+    memcpy(a, m, sizeof a);
+    for (int i = 0; i < 3; i++) {
+        double aamax = 0.0;
+        for (int j = 0; j < 3; j++)
+            if (fabs(a[i][j]) > aamax) aamax = fabs(a[i][j]);
+        vv[i] = 1.0 / aamax;
+    }
+    for (int j = 0; j < 3; j++) {
+        for (int i = 1; i < j; i++) {
+            double sum = a[i][j];
+            for (int k = 0; k < i; k++) sum = sum - a[i][k] * a[k][j];
+            a[i][j] = sum;
+        }
+        double aamax = 0.0;
+        int imax = j;
+        for (int i = j; i < 3; i++) {
+            double sum = a[i][j];
+            if (j > 0) {
+                for (int k = 0; k < j; k++) sum = sum - a[i][k] * a[k][j];
+                a[i][j] = sum;
+            }
+            double dum = vv[i] * fabs(sum);
+            if (dum >= aamax) {
+                imax = i;
+                aamax = dum;
+            }
+        }
+        if (j != imax) {
+            for (int k = 0; k < 3; k++) {
+                double dum = a[imax][k];
+                a[imax][k] = a[j][k];
+                a[j][k] = dum;
+            }
+            vv[imax] = vv[j];
+        }
+        indx[j] = imax;
+        if (j != 2) {
+            if (a[j][j] == 0.0) a[j][j] = (double)1.0e-20f;
+            double dum = 1.0 / a[j][j];
+            for (int i = j + 1; i < 3; i++) a[i][j] = a[i][j] * dum;
+        }
+    }
+    if (a[2][2] == 0.0) a[2][2] = (double)1.0e-20f;
+    for (int col = 0; col < 3; col++) {
+        double b[3] = {0.0, 0.0, 0.0};
+        b[col] = 1.0;
+        int ii = -1;
+        for (int i = 0; i < 3; i++) {
+            int ll = indx[i];
+            double sum = b[ll];
+            b[ll] = b[i];
+            if (ii >= 0) {
+                for (int j = ii; j < i; j++) sum = sum - a[i][j] * b[j];
+            } else if (sum != 0.0) {
+                ii = i;
+            }
+            b[i] = sum;
+        }
+        for (int i = 2; i >= 0; i--) {
+            double sum = b[i];
+            for (int j = i + 1; j < 3; j++) sum = sum - a[i][j] * b[j];
+            b[i] = sum / a[i][i];
+        }
+        for (int i = 0; i < 3; i++) inv[i][col] = b[i];
+    }
+}
+
+/* fundcell_init (fundcell.F90:60-112) + fundcell_initxform (:378-412) +
+ * fundcell_construct (:345-371) + fundcell_genlatticevects (:174-208).
+ * The unslant matrix is formed from the angles rounded to default real
+ * (float32) with single-precision cosines, as the reference declares them. */
+void mo_cell_init(mo_cell *c, const double fell[3], const double ang_deg[3], const double origin[3],
+                  const double eff[3], const int ncell[3])
+{
+    const double degtorad = MO_PI / 180.0;
+    double err = fabs(ang_deg[0] - 90.0) + fabs(ang_deg[1] - 90.0) + fabs(ang_deg[2] - 90.0);
+    c->ortho = err > 0.001 ? 0 : 1;
+    double abc = ang_deg[0] * degtorad, aac = ang_deg[1] * degtorad, aab = ang_deg[2] * degtorad;
+    for (int i = 0; i < 3; i++) {
+        c->fell[i] = fell[i];
+        c->finv[i] = 1.0 / fell[i];
+        c->origin[i] = origin[i];
+        c->eff[i] = eff[i];
+        double rs = (double)ncell[i];
+        c->ell[i] = fell[i] * rs;
+        c->half[i] = (0.5 * fell[i]) * rs;
+    }
+    float alpha = (float)abc, beta = (float)aac, gamma = (float)aab;
+    float cg = cosf(gamma), cb = cosf(beta), ca = cosf(alpha);
+    memset(c->U, 0, sizeof c->U);
+    c->U[0][0] = 1.0;
+    c->U[0][1] = (double)cg;
+    c->U[0][2] = (double)cb;
+    float cg2 = cg * cg;
+    c->U[1][1] = sqrt(1.0 - (double)cg2);
+    float num = ca - cg * cb;
+    c->U[1][2] = (double)num / c->U[1][1];
+    c->U[2][2] = sqrt(1.0 - c->U[0][2] * c->U[0][2] - c->U[1][2] * c->U[1][2]);
+    lu_inverse3(c->U, c->S);
+    c->lvec[0][0] = c->ell[0];
+    c->lvec[0][1] = 0.0;
+    c->lvec[0][2] = 0.0;
+    c->lvec[1][0] = c->ell[1] * cos(aab);
+    c->lvec[1][1] = c->ell[1] * sin(aab);
+    c->lvec[1][2] = 0.0;
+    c->lvec[2][0] = c->ell[2] * cos(aac);
+    c->lvec[2][1] = c->ell[2] * (cos(abc) - cos(aac) * cos(aab)) / sin(aab);
+    double m2 = 0.0;
+    m2 = m2 + c->lvec[2][0] * c->lvec[2][0];
+    m2 = m2 + c->lvec[2][1] * c->lvec[2][1];
+    m2 = m2 + 0.0 * 0.0;
+    c->lvec[2][2] = sqrt(c->ell[2] * c->ell[2] - m2);
+}
+
+/* matrix_m_mult_v (matrix.F90:137-150) */
+static void mat_vec(const double m[3][3], const double v[3], double out[3])
+{
+    for (int r = 0; r < 3; r++) {
+        double s = 0.0;
+        s = s + m[r][0] * v[0];
+        s = s + m[r][1] * v[1];
+        s = s + m[r][2] * v[2];
+        out[r] = s;
+    }
+}
+
+void mo_cell_minimage(const mo_cell *c, const double a[3], const double b[3], double sep[3])
+{
+    if (c->ortho) {
+        mo_minimage(c->ell, a, b, sep);
+        return;
+    }
+    double va[3], vb[3], d[3];
+    mat_vec(c->S, a, va);
+    mat_vec(c->S, b, vb);
+    for (int i = 0; i < 3; i++) {
+        double x = va[i] - vb[i];
+        if (x > c->half[i]) x = x - c->ell[i];
+        else if (x < -c->half[i]) x = x + c->ell[i];
+        d[i] = x;
+    }
+    mat_vec(c->U, d, sep);
+}
+
+void mo_cell_maptouc(const mo_cell *c, const double r[3], double out[3])
+{
+    double v[3], w[3];
+    for (int i = 0; i < 3; i++) v[i] = r[i] - c->origin[i];
+    if (!c->ortho) {
+        mat_vec(c->S, v, w);
+        memcpy(v, w, sizeof v);
+    }
+    for (int i = 0; i < 3; i++) {
+        double d = v[i] * c->finv[i];
+        if (d >= 0.0) d = fmod(d, 1.0);
+        else d = fmod(d, 1.0) + 1.0;
+        v[i] = d * c->fell[i];
+    }
+    if (!c->ortho) {
+        mat_vec(c->U, v, w);
+        memcpy(v, w, sizeof v);
+    }
+    for (int i = 0; i < 3; i++) out[i] = v[i] + c->origin[i];
+}
+
+void mo_cell_pbc(const mo_cell *c, const double r[3], double out[3])
+{
+    double v[3];
+    if (c->ortho) memcpy(v, r, sizeof v);
+    else mat_vec(c->S, r, v);
+    for (int i = 0; i < 3; i++) {
+        double idx = floor(v[i] / c->ell[i]);
+        v[i] = v[i] - idx * c->ell[i];
+    }
+    if (c->ortho) memcpy(out, v, sizeof v);
+    else mat_vec(c->U, v, out);
+}
+
+/* minimum image in the given cell, or the orthorhombic full_ell */
+static inline void mi(const double full_ell[3], const mo_cell *cell, const double a[3], const double b[3],
+                      double sep[3])
+{
+    if (cell) mo_cell_minimage(cell, a, b, sep);
+    else mo_minimage(full_ell, a, b, sep);
+}
+
 static inline double normsq(const double v[3])
 {
     double s = 0.0;
@@ -159,16 +353,16 @@ int mo_lj_pair_hot(double A, double B, double hicut2, double locut2,
 
 /* Node loop: pairmodel.F90:892-909 (hot += hotatm per atom, stop at the
  * first locut violation), node assembly mapmaker.F90:500-547. */
-int mo_lj_node(const double p[3], int natoms, const double *xyz, const int *type,
-               const double *A, const double *B, const double *hicut2,
-               const double *locut2, const double full_ell[3], double out[8])
+static int lj_node_c(const double p[3], int natoms, const double *xyz, const int *type,
+                     const double *A, const double *B, const double *hicut2,
+                     const double *locut2, const double full_ell[3], const mo_cell *cell, double out[8])
 {
     double hot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double pot = 0.0;
     int ok = 1;
     for (int a = 0; a < natoms; a++) {
         double sep[3];
-        mo_minimage(full_ell, p, xyz + 3 * a, sep);
+        mi(full_ell, cell, p, xyz + 3 * a, sep);
         double hotatm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         double nrg = 0.0;
         int t = type[a];
@@ -186,21 +380,28 @@ int mo_lj_node(const double p[3], int natoms, const double *xyz, const int *type
     return ok;
 }
 
+int mo_lj_node(const double p[3], int natoms, const double *xyz, const int *type,
+               const double *A, const double *B, const double *hicut2,
+               const double *locut2, const double full_ell[3], double out[8])
+{
+    return lj_node_c(p, natoms, xyz, type, A, B, hicut2, locut2, full_ell, NULL, out);
+}
+
 /* Buckingham, closed form: V = A exp(-r/rho) - C r^-6 - D r^-8 per pair
  * (buck.F90:329).  Derivatives of a radial V(r):
  *   dV/dx = x f1, f1 = V'/r;  d2V/dxdy = x y g2, g2 = f1'/r;
  *   d3V/dxdydz = x y z g3, g3 = g2'/r. */
-int mo_buck_node(const double p[3], int natoms, const double *xyz, const int *type,
-                 const double *A, const double *rho, const double *C, const double *D,
-                 const double *hicut2, const double *locut2, const double full_ell[3],
-                 double out[8])
+static int buck_node_c(const double p[3], int natoms, const double *xyz, const int *type,
+                       const double *A, const double *rho, const double *C, const double *D,
+                       const double *hicut2, const double *locut2, const double full_ell[3],
+                       const mo_cell *cell, double out[8])
 {
     double hot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double pot = 0.0;
     int ok = 1;
     for (int a = 0; a < natoms; a++) {
         double sep[3];
-        mo_minimage(full_ell, p, xyz + 3 * a, sep);
+        mi(full_ell, cell, p, xyz + 3 * a, sep);
         int t = type[a];
         double r2 = normsq(sep);
         if (r2 > hicut2[t]) continue;
@@ -231,6 +432,14 @@ int mo_buck_node(const double p[3], int natoms, const double *xyz, const int *ty
     out[0] = ok ? pot : MO_VERY_LRG_NRG;
     for (int i = 1; i < 8; i++) out[i] = hot[i];
     return ok;
+}
+
+int mo_buck_node(const double p[3], int natoms, const double *xyz, const int *type,
+                 const double *A, const double *rho, const double *C, const double *D,
+                 const double *hicut2, const double *locut2, const double full_ell[3],
+                 double out[8])
+{
+    return buck_node_c(p, natoms, xyz, type, A, rho, C, D, hicut2, locut2, full_ell, NULL, out);
 }
 
 /* ------------------------------------------------------------- Ewald */
@@ -295,10 +504,10 @@ int mo_ewald_sfactor(int natoms, const double *xyz, const double *q, const doubl
 
 /* ewald.F90:334-398 (getInteractionHOT), :856-918 (rwaldHOT),
  * :966-1005 (skwaldHOT); node assembly mapmaker.F90:536-543. */
-int mo_ewald_node(const double p[3], int natoms, const double *xyz, const double *q,
-                  double charge1, double kappa, int nk, const double *k,
-                  const double *s, const double *si, const double full_ell[3],
-                  double out[8])
+static int ewald_node_c(const double p[3], int natoms, const double *xyz, const double *q,
+                        double charge1, double kappa, int nk, const double *k,
+                        const double *s, const double *si, const double full_ell[3],
+                        const mo_cell *cell, double out[8])
 {
     const double risqrtpi = 0.5641896;
     double hot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -307,7 +516,7 @@ int mo_ewald_node(const double p[3], int natoms, const double *xyz, const double
     /* pass 1: low-cut check over all atoms (ewald.F90:373-375) */
     for (int i = 0; i < natoms; i++) {
         double sep[3];
-        mo_minimage(full_ell, p, xyz + 3 * i, sep);
+        mi(full_ell, cell, p, xyz + 3 * i, sep);
         if (normsq(sep) < lowcut2) {
             out[0] = MO_VERY_LRG_NRG;
             for (int j = 1; j < 8; j++) out[j] = 0.0;
@@ -318,7 +527,7 @@ int mo_ewald_node(const double p[3], int natoms, const double *xyz, const double
     double kappa3 = kappa * kappa * kappa;
     for (int i = 0; i < natoms; i++) {
         double sep[3];
-        mo_minimage(full_ell, p, xyz + 3 * i, sep);
+        mi(full_ell, cell, p, xyz + 3 * i, sep);
         double d2 = normsq(sep);
         double dist = sqrt(d2);
         double kr = kappa * dist;
@@ -373,6 +582,14 @@ int mo_ewald_node(const double p[3], int natoms, const double *xyz, const double
     return 1;
 }
 
+int mo_ewald_node(const double p[3], int natoms, const double *xyz, const double *q,
+                  double charge1, double kappa, int nk, const double *k,
+                  const double *s, const double *si, const double full_ell[3],
+                  double out[8])
+{
+    return ewald_node_c(p, natoms, xyz, q, charge1, kappa, nk, k, s, si, full_ell, NULL, out);
+}
+
 /* -------------------------------------------------------------- grid */
 /* config.F90:762-786 */
 void mo_grid_dims(const double eff[3], double gridspace, int npnts[3], double step[3])
@@ -392,15 +609,45 @@ void mo_build_nodes(int kind, const mo_framework *fw, const int npnts[3], const 
         double p[3] = {(double)x * step[0], (double)y * step[1], (double)z * step[2]};
         double *o = vals + 8 * (m - m0);
         if (kind == 1)
-            mo_lj_node(p, fw->natoms, fw->xyz, fw->type, fw->A, fw->B, fw->hicut2,
-                       fw->locut2, fw->full_ell, o);
+            lj_node_c(p, fw->natoms, fw->xyz, fw->type, fw->A, fw->B, fw->hicut2,
+                      fw->locut2, fw->full_ell, fw->cell, o);
         else if (kind == 2)
-            mo_ewald_node(p, fw->natoms, fw->xyz, fw->q, fw->charge1, fw->kappa, fw->nk,
-                          fw->k, fw->s, fw->si, fw->full_ell, o);
+            ewald_node_c(p, fw->natoms, fw->xyz, fw->q, fw->charge1, fw->kappa, fw->nk,
+                         fw->k, fw->s, fw->si, fw->full_ell, fw->cell, o);
         else
-            mo_buck_node(p, fw->natoms, fw->xyz, fw->type, fw->bA, fw->brho, fw->bC, fw->bD,
-                         fw->hicut2, fw->locut2, fw->full_ell, o);
+            buck_node_c(p, fw->natoms, fw->xyz, fw->type, fw->bA, fw->brho, fw->bC, fw->bD,
+                        fw->hicut2, fw->locut2, fw->full_ell, fw->cell, o);
     }
+}
+
+/* one node at an arbitrary point p (same kinds as mo_build_nodes) */
+int mo_node_at(int kind, const mo_framework *fw, const double p[3], double out[8])
+{
+    if (kind == 1)
+        return lj_node_c(p, fw->natoms, fw->xyz, fw->type, fw->A, fw->B, fw->hicut2, fw->locut2, fw->full_ell,
+                         fw->cell, out);
+    if (kind == 2)
+        return ewald_node_c(p, fw->natoms, fw->xyz, fw->q, fw->charge1, fw->kappa, fw->nk, fw->k, fw->s, fw->si,
+                            fw->full_ell, fw->cell, out);
+    return buck_node_c(p, fw->natoms, fw->xyz, fw->type, fw->bA, fw->brho, fw->bC, fw->bD, fw->hicut2, fw->locut2,
+                       fw->full_ell, fw->cell, out);
+}
+
+/* simcell_uniformRandPt (simcell.F90:268-295): a uniform point of the
+ * simulation cell, ell_i r_i when orthorhombic, else (lvec1 r1 + lvec2 r2)
+ * + lvec3 r3 */
+void mo_cell_uniform(const mo_cell *c, mo_rng *rng, double out[3])
+{
+    if (c->ortho) {
+        for (int i = 0; i < 3; i++) out[i] = c->ell[i] * mo_rranf(rng);
+        return;
+    }
+    double v[3][3];
+    for (int k = 0; k < 3; k++) {
+        double r = mo_rranf(rng);
+        for (int i = 0; i < 3; i++) v[k][i] = c->lvec[k][i] * r;
+    }
+    for (int i = 0; i < 3; i++) out[i] = (v[0][i] + v[1][i]) + v[2][i];
 }
 
 /* -------------------------------------------------------- compaction */
@@ -620,14 +867,20 @@ static inline double hd1f(double x, double ds) { return ds * x * (3.0 * x - 2.0)
 /* shared setup of pothermite/forcehermite (interpolate.F90:641-723) */
 static int herm_setup(const int npnts[3], const double step[3], const int32_t *ptr,
                       const float *vals8, long long nfsize, const double coord[3],
-                      double ft[64], double tuv[3])
+                      double ft[64], double tuv[3], int wrap)
 {
     int idx[3], idxp[3];
     for (int d = 0; d < 3; d++) {
         idx[d] = (int)(coord[d] / step[d] + 1.0);
         tuv[d] = (coord[d] - (double)(idx[d] - 1) * step[d]) / step[d];
         idxp[d] = idx[d] + 1;
-        if (idx[d] == npnts[d]) idxp[d] = 1;
+        /* the wrap of the upper corner holds for orthorhombic cells only
+         * (interpolate.F90:675-679); without it a point in the last
+         * cubelet has no upper corner and fails */
+        if (idx[d] == npnts[d]) {
+            if (!wrap) return 0;
+            idxp[d] = 1;
+        }
         if (idx[d] < 1 || idx[d] > npnts[d]) return 0; /* outside the grid */
     }
     long long ny = npnts[1], nz = npnts[2];
@@ -671,11 +924,11 @@ static void dbasis(double t, double step, double h[2][2])
 }
 
 /* interpolate.F90:603-854 */
-int mo_pothermite(const int npnts[3], const double step[3], const int32_t *ptr,
-                  const float *vals8, long long nfsize, const double coord[3], double *p)
+static int pothermite_w(const int npnts[3], const double step[3], const int32_t *ptr,
+                        const float *vals8, long long nfsize, const double coord[3], double *p, int wrap)
 {
     double ft[64], tuv[3];
-    if (!herm_setup(npnts, step, ptr, vals8, nfsize, coord, ft, tuv)) {
+    if (!herm_setup(npnts, step, ptr, vals8, nfsize, coord, ft, tuv, wrap)) {
         *p = 0.0;
         return 0;
     }
@@ -693,12 +946,24 @@ int mo_pothermite(const int npnts[3], const double step[3], const int32_t *ptr,
     return 1;
 }
 
+int mo_pothermite(const int npnts[3], const double step[3], const int32_t *ptr,
+                  const float *vals8, long long nfsize, const double coord[3], double *p)
+{
+    return pothermite_w(npnts, step, ptr, vals8, nfsize, coord, p, 1);
+}
+
+int mo_pothermite_nowrap(const int npnts[3], const double step[3], const int32_t *ptr, const float *vals8,
+                         long long nfsize, const double coord[3], double *p)
+{
+    return pothermite_w(npnts, step, ptr, vals8, nfsize, coord, p, 0);
+}
+
 /* interpolate.F90:189-584 */
 int mo_forcehermite(const int npnts[3], const double step[3], const int32_t *ptr,
                     const float *vals8, long long nfsize, const double coord[3], double p[4])
 {
     double ft[64], tuv[3];
-    if (!herm_setup(npnts, step, ptr, vals8, nfsize, coord, ft, tuv)) {
+    if (!herm_setup(npnts, step, ptr, vals8, nfsize, coord, ft, tuv, 1)) {
         p[0] = p[1] = p[2] = p[3] = 0.0;
         return 0;
     }

@@ -12,8 +12,9 @@
  * reference's own Fortran sources, closed forms) is listed in
  * music_oracle.c next to the function.
  *
- * Conventions: Angstrom, kcal/mol, elementary charge.  Orthorhombic cells
- * with origin 0 only.  Arrays are C-contiguous; xyz arrays are [n][3].
+ * Conventions: Angstrom, kcal/mol, elementary charge.  Arrays are
+ * C-contiguous; xyz arrays are [n][3].  Cells are orthorhombic with origin
+ * 0 unless a mo_cell (slanted cells, fundcell.F90) is given.
  */
 #ifndef MUSIC_ORACLE_H
 #define MUSIC_ORACLE_H
@@ -47,6 +48,31 @@ void mo_rng_init(mo_rng *r, int32_t iseed);
 double mo_rranf(mo_rng *r);
 /* fill n draws (test helper) */
 void mo_rranf_n(int32_t iseed, int n, double *out, int32_t *ix_out, int32_t *iy_out);
+
+/* ---- cells: fundcell.F90 / simcell.F90 ----
+ * A fundamental cell (edges fell, angles alpha = bc, beta = ac, gamma = ab
+ * in degrees, origin, effective box eff) and the simulation cell made of
+ * ncell copies per axis.  U maps slanted to Cartesian coordinates, S = U^-1
+ * (fundcell_initxform, fundcell.F90:378-412; matrix_inverse, matrix.F90:
+ * 195-229); lvec[i] are the simulation cell's lattice vectors
+ * (fundcell_genlatticevects, fundcell.F90:174-208). */
+typedef struct {
+    int ortho;                /* orthoflag: every angle within 0.001 degree of 90 */
+    double fell[3], finv[3];  /* fundamental cell edges and their inverses */
+    double ell[3], half[3];   /* simulation cell edges and halves */
+    double origin[3];         /* fundamental cell origin */
+    double eff[3];            /* fundamental cell effective (map) box */
+    double U[3][3], S[3][3];  /* unslant, slant [row][column] */
+    double lvec[3][3];        /* simulation cell lattice vectors [vector][xyz] */
+} mo_cell;
+void mo_cell_init(mo_cell *c, const double fell[3], const double ang_deg[3], const double origin[3],
+                  const double eff[3], const int ncell[3]);
+/* simcell_minimage (fundcell_snglMinImage, fundcell.F90:582-623): sep = a - b */
+void mo_cell_minimage(const mo_cell *c, const double a[3], const double b[3], double sep[3]);
+/* simcell_maptouc (simcell.F90:634-668) */
+void mo_cell_maptouc(const mo_cell *c, const double r[3], double out[3]);
+/* simcell_pbcpt (simcell.F90:460-484) */
+void mo_cell_pbc(const mo_cell *c, const double r[3], double out[3]);
 
 /* ---- math helpers ---- */
 double mo_erfc(double x);                                   /* utils.F90:2090-2101 */
@@ -108,9 +134,14 @@ typedef struct {
     int nk;
     const double *k, *s, *si;
     double full_ell[3];
+    const mo_cell *cell;  /* slanted cell, or NULL: orthorhombic full_ell */
 } mo_framework;
 void mo_build_nodes(int kind, const mo_framework *fw, const int npnts[3], const double step[3],
                     long long m0, long long m1, double *vals /* [(m1-m0)][8] */);
+/* one node at an arbitrary point */
+int mo_node_at(int kind, const mo_framework *fw, const double p[3], double out[8]);
+/* simcell_uniformRandPt (simcell.F90:268-295) */
+void mo_cell_uniform(const mo_cell *c, mo_rng *rng, double out[3]);
 
 /* ---- compaction (pmap.F90:509-656) ----
  * r: FP32 potential per node in grid order (x fastest), cutoff in kcal/mol.
@@ -135,7 +166,10 @@ int mo_map_read(const char *path, int32_t *ptr, float *vals8);
 
 /* ---- interpolation (interpolate.F90:104-158, :603-874; simcell.F90:634-668) ---- */
 void mo_maptouc(const double fund_ell[3], const double r[3], double out[3]);
-/* returns 1 if all 8 corners tabulated, p = potential */
+/* returns 1 if all 8 corners tabulated, p = potential (periodic corner
+ * wrap as for orthorhombic cells; mo_pothermite_nowrap without it) */
+int mo_pothermite_nowrap(const int npnts[3], const double step[3], const int32_t *ptr, const float *vals8,
+                         long long nfsize, const double coord[3], double *p);
 int mo_pothermite(const int npnts[3], const double step[3], const int32_t *ptr,
                   const float *vals8, long long nfsize, const double coord[3], double *p);
 /* forcehermite: p[0]=V, p[1..3] = gradient terms (interpolate.F90:189-584) */

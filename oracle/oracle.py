@@ -56,6 +56,15 @@ def lib(native=False):
     L.mo_forcehermite.argtypes = [_vp, _vp, _vp, _vp, _ll, _vp, _vp]
     L.mo_forcehermite.restype = _i
     L.mo_maptouc.argtypes = [_vp, _vp, _vp]
+    L.mo_pothermite_nowrap.argtypes = [_vp, _vp, _vp, _vp, _ll, _vp, _vp]
+    L.mo_pothermite_nowrap.restype = _i
+    L.mo_cell_init.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+    L.mo_cell_minimage.argtypes = [_vp, _vp, _vp, _vp]
+    L.mo_cell_maptouc.argtypes = [_vp, _vp, _vp]
+    L.mo_cell_pbc.argtypes = [_vp, _vp, _vp]
+    L.mo_node_at.argtypes = [_i, _vp, _vp, _vp]
+    L.mo_node_at.restype = _i
+    L.mo_cell_uniform.argtypes = [_vp, _vp, _vp]
     if not native:
         _LIB = L
     return L
@@ -73,12 +82,58 @@ def erfc(x):
     return np.array([lib().mo_erfc(float(v)) for v in np.atleast_1d(x)])
 
 
+class MoCell(ctypes.Structure):
+    _fields_ = [("ortho", _i), ("fell", _d * 3), ("finv", _d * 3), ("ell", _d * 3), ("half", _d * 3),
+                ("origin", _d * 3), ("eff", _d * 3), ("U", (_d * 3) * 3), ("S", (_d * 3) * 3),
+                ("lvec", (_d * 3) * 3)]
+
+
+def cell(fund_ell, angles_deg, eff=None, ncell=(1, 1, 1), origin=(0.0, 0.0, 0.0)):
+    """A (possibly slanted) simulation cell: fundcell_init + fundcell_construct."""
+    c = MoCell()
+    fe = np.ascontiguousarray(fund_ell, dtype=np.float64)
+    an = np.ascontiguousarray(angles_deg, dtype=np.float64)
+    og = np.ascontiguousarray(origin, dtype=np.float64)
+    ef = np.ascontiguousarray(fund_ell if eff is None else eff, dtype=np.float64)
+    nc = np.ascontiguousarray(ncell, dtype=np.int32)
+    lib().mo_cell_init(ctypes.byref(c), _p(fe), _p(an), _p(og), _p(ef), _p(nc))
+    return c
+
+
+def cell_matrices(c):
+    return (np.array([[c.U[i][j] for j in range(3)] for i in range(3)]),
+            np.array([[c.S[i][j] for j in range(3)] for i in range(3)]),
+            np.array([[c.lvec[i][j] for j in range(3)] for i in range(3)]))
+
+
+def cell_minimage(c, a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.zeros(3)
+    lib().mo_cell_minimage(ctypes.byref(c), _p(a), _p(b), _p(out))
+    return out
+
+
+def cell_maptouc(c, r):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.zeros(3)
+    lib().mo_cell_maptouc(ctypes.byref(c), _p(r), _p(out))
+    return out
+
+
+def cell_pbc(c, r):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.zeros(3)
+    lib().mo_cell_pbc(ctypes.byref(c), _p(r), _p(out))
+    return out
+
+
 class _Framework(ctypes.Structure):
     _fields_ = [("natoms", _i), ("xyz", _vp), ("type", _vp), ("q", _vp),
                 ("A", _vp), ("B", _vp), ("hicut2", _vp), ("locut2", _vp),
                 ("bA", _vp), ("brho", _vp), ("bC", _vp), ("bD", _vp),
                 ("charge1", _d), ("kappa", _d), ("nk", _i),
-                ("k", _vp), ("s", _vp), ("si", _vp), ("full_ell", _d * 3)]
+                ("k", _vp), ("s", _vp), ("si", _vp), ("full_ell", _d * 3), ("cell", _vp)]
 
 
 def ewald_sfactor(xyz, q, ell, kappa_param, kmax, locut, nkmax=50000):
@@ -99,15 +154,37 @@ def ewald_sfactor(xyz, q, ell, kappa_param, kmax, locut, nkmax=50000):
                 kidx=kidx[:nk].copy(), kappa=kap.value)
 
 
-def build_nodes(kind, fw, npnts, step, m0=0, m1=None):
-    """Evaluate grid nodes m0..m1 (config_makegrid order) -> [n, 8] FP64.
+def node_at(kind, fw, points):
+    """Nodes at arbitrary points [n,3] -> (ok[n], vals[n,8]); fw as in
+    build_nodes (with an optional slanted 'cell')."""
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    st, keep = _framework_struct(fw)
+    ok = np.zeros(len(pts), dtype=bool)
+    out = np.zeros((len(pts), 8))
+    o = np.zeros(8)
+    for i in range(len(pts)):
+        ok[i] = bool(lib().mo_node_at(kind, ctypes.byref(st), _p(pts[i]), _p(o)))
+        out[i] = o
+    return ok, out
 
-    fw: dict with xyz, type, q, A, B, hicut2, locut2 (per type), charge1,
-    kappa, k, s, si, full_ell (and bA, brho, bC, bD for Buckingham)."""
-    npnts = np.ascontiguousarray(npnts, dtype=np.int32)
-    step = np.ascontiguousarray(step, dtype=np.float64)
-    if m1 is None:
-        m1 = int(np.prod(npnts.astype(np.int64)))
+
+def cell_uniform(c, seed, n):
+    """n points of simcell_uniformRandPt after random_init(seed)."""
+    _gcmc_lib()
+    r = _Rng()
+    lib().mo_rng_init(ctypes.byref(r), seed)
+    out = np.zeros((n, 3))
+    v = np.zeros(3)
+    for i in range(n):
+        lib().mo_cell_uniform(ctypes.byref(c), ctypes.byref(r), _p(v))
+        out[i] = v
+    return out
+
+
+def _framework_struct(fw):
+    """mo_framework from a dict (xyz, type, q, A, B, hicut2, locut2, bA,
+    brho, bC, bD, charge1, kappa, k, s, si, full_ell, optional cell);
+    returns the struct and the arrays it points into."""
     keep = {}
 
     def arr(name, dt=np.float64):
@@ -139,6 +216,23 @@ def build_nodes(kind, fw, npnts, step, m0=0, m1=None):
     st.si = arr("si")
     for i in range(3):
         st.full_ell[i] = float(fw["full_ell"][i])
+    c = fw.get("cell")
+    st.cell = ctypes.cast(ctypes.byref(c), _vp) if c is not None else None
+    keep["cell"] = c
+    return st, keep
+
+
+def build_nodes(kind, fw, npnts, step, m0=0, m1=None):
+    """Evaluate grid nodes m0..m1 (config_makegrid order) -> [n, 8] FP64.
+
+    fw: dict with xyz, type, q, A, B, hicut2, locut2 (per type), charge1,
+    kappa, k, s, si, full_ell (and bA, brho, bC, bD for Buckingham; a
+    slanted 'cell' from cell())."""
+    npnts = np.ascontiguousarray(npnts, dtype=np.int32)
+    step = np.ascontiguousarray(step, dtype=np.float64)
+    if m1 is None:
+        m1 = int(np.prod(npnts.astype(np.int64)))
+    st, keep = _framework_struct(fw)
     out = np.zeros((m1 - m0, 8))
     lib().mo_build_nodes(kind, ctypes.byref(st), _p(npnts), _p(step), m0, m1, _p(out))
     return out
@@ -195,9 +289,11 @@ def map_read(path):
     return dict(npnts=np3, nfsize=nf.value, pcut=pc.value, ptr=ptr, vals=vals)
 
 
-def pothermite(npnts, step, ptr, vals8, coords, fund_ell=None, force=False):
-    """Interpolate at coords [n,3]; if fund_ell given, coords are first folded
-    with simcell_maptouc.  Returns (ok[n], V[n]) or (ok, p[n,4]) if force."""
+def pothermite(npnts, step, ptr, vals8, coords, fund_ell=None, force=False, cell=None):
+    """Interpolate at coords [n,3]; if fund_ell (or a slanted cell) is given,
+    coords are first folded with simcell_maptouc.  Returns (ok[n], V[n]) or
+    (ok, p[n,4]) if force.  A slanted cell interpolates without the
+    periodic corner wrap (interpolate.F90:675-679)."""
     npnts = np.ascontiguousarray(npnts, dtype=np.int32)
     step = np.ascontiguousarray(step, dtype=np.float64)
     ptr = np.ascontiguousarray(ptr, dtype=np.int32)
@@ -210,7 +306,9 @@ def pothermite(npnts, step, ptr, vals8, coords, fund_ell=None, force=False):
     fe = np.ascontiguousarray(fund_ell, dtype=np.float64) if fund_ell is not None else None
     c = np.zeros(3)
     for i in range(n):
-        if fe is not None:
+        if cell is not None:
+            L.mo_cell_maptouc(ctypes.byref(cell), _p(coords[i]), _p(c))
+        elif fe is not None:
             L.mo_maptouc(_p(fe), _p(coords[i]), _p(c))
         else:
             c[:] = coords[i]
@@ -221,8 +319,8 @@ def pothermite(npnts, step, ptr, vals8, coords, fund_ell=None, force=False):
             out[i] = p
         else:
             v = ctypes.c_double()
-            ok[i] = bool(L.mo_pothermite(_p(npnts), _p(step), _p(ptr), _p(vals8),
-                                         vals8.shape[1], _p(c), ctypes.byref(v)))
+            f = L.mo_pothermite_nowrap if (cell is not None and not cell.ortho) else L.mo_pothermite
+            ok[i] = bool(f(_p(npnts), _p(step), _p(ptr), _p(vals8), vals8.shape[1], _p(c), ctypes.byref(v)))
             out[i] = v.value
     return ok, out
 

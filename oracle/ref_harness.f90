@@ -25,6 +25,12 @@
 !   R4CMP n ratio(1..n)                  -> ratio < 0.49, ratio > 0.51 with
 !                                           default-real literals, as in
 !                                           rigidmoves_adjustdeltatrans
+!   FCELL   (see read order below)       -> slant / unslant matrices, lattice
+!                                           vectors, minimum images, maptouc,
+!                                           pbc and uniform random points of a
+!                                           (possibly slanted) cell
+!   LJNODEX / HERMITEX                   -> LJNODE / HERMITE in a cell given
+!                                           by a cell block (fundcell_init)
 program ref_harness
   Use defaults, Only: RDbl, RSgl
   Use random, Only: random_init, rranf
@@ -47,6 +53,9 @@ program ref_harness
   Use atom, Only: atom_init
   Use molecule, Only: MolecularParams, molecule_init, molecule_getatomcoords, &
       molecule_getmoi
+  Use fundcell, Only: Fundamental_Cell, fundcell_init, fundcell_construct
+  Use vector, Only: IntVecType
+  Use simcell, Only: simcell_maptouc, simcell_pbc, simcell_uniformRandPt
   Implicit None
 
   Character(len=32) :: job
@@ -57,13 +66,13 @@ program ref_harness
   Case ('ERFC')
     Call job_erfc()
   Case ('LJNODE')
-    Call job_ljnode()
+    Call job_ljnode(.False.)
   Case ('EWALD')
     Call job_ewald()
   Case ('COMPACT')
     Call job_compact()
   Case ('HERMITE')
-    Call job_hermite()
+    Call job_hermite(.False.)
   Case ('COULOMB')
     Call job_coulomb()
   Case ('EULER')
@@ -76,6 +85,12 @@ program ref_harness
     Call job_paxes()
   Case ('R4CMP')
     Call job_r4cmp()
+  Case ('FCELL')
+    Call job_fcell()
+  Case ('LJNODEX')
+    Call job_ljnode(.True.)
+  Case ('HERMITEX')
+    Call job_hermite(.True.)
   Case Default
     Write(0,*) 'unknown job ', Trim(job)
     Stop 2
@@ -100,6 +115,32 @@ Contains
     cell%fullC%origin%comp = 0.0_RDbl
   End Subroutine set_cell
 
+  ! cell block: fund edges a b c; angles alpha beta gamma (degrees); origin;
+  ! eff; cells per axis.  fundcell_init reads the first four lines from a
+  ! unit (fundcell.F90:60-112); the simulation cell is fundcell_construct of
+  ! it (:345-371).
+  Subroutine read_cell(cell)
+    Type(SimCell_Params), Intent(Out) :: cell
+    Real(RDbl) :: v(12)
+    Integer :: nc(3), u
+    Type(IntVecType) :: sz
+    Read(*,*) v(1:3)
+    Read(*,*) v(4:6)
+    Read(*,*) v(7:9)
+    Read(*,*) v(10:12)
+    Read(*,*) nc
+    Open(newunit=u, status='SCRATCH', form='FORMATTED')
+    Write(u,'(3ES26.17E3)') v(1:3)
+    Write(u,'(3ES26.17E3)') v(4:6)
+    Write(u,'(3ES26.17E3)') v(7:9)
+    Write(u,'(3ES26.17E3)') v(10:12)
+    Rewind(u)
+    Call fundcell_init(cell%fundC, u)
+    Close(u)
+    sz%comp = nc
+    Call fundcell_construct(cell%fundC, sz, cell%fullC)
+  End Subroutine read_cell
+
   Subroutine job_rng()
     Integer :: seed, n, i
     Read(*,*) seed, n
@@ -123,7 +164,8 @@ Contains
   ! ntypes x (A B hicut2 locut2); nnodes; nnodes x (x y z)
   ! Loop structure of pairmodel_mmint5 (pairmodel.F90:892-909) and node
   ! assembly of mapmaker_makemap (mapmaker.F90:500-547).
-  Subroutine job_ljnode()
+  Subroutine job_ljnode(xcell)
+    Logical, Intent(In) :: xcell
     Type(SimCell_Params) :: cell
     Real(RDbl) :: full(3), p(3), pot
     Integer :: natoms, ntypes, nnodes, a, t, nd
@@ -135,8 +177,12 @@ Contains
     Type(EnergyPlus) :: ffout, opp
     Real(RDbl) :: hot(8), hotatm(8)
     Logical :: ok
-    Read(*,*) full
-    Call set_cell(cell, full, full)
+    If (xcell) Then
+      Call read_cell(cell)
+    Else
+      Read(*,*) full
+      Call set_cell(cell, full, full)
+    End If
     Read(*,*) natoms
     Allocate(atv(natoms), typ(natoms), sep(1,natoms))
     Do a = 1, natoms
@@ -262,7 +308,8 @@ Contains
 
   ! HERMITE input: map file name (in directory $PMAPDIR); fund(3); npts;
   ! npts x (x y z)
-  Subroutine job_hermite()
+  Subroutine job_hermite(xcell)
+    Logical, Intent(In) :: xcell
     Character(len=255) :: fname
     Type(Pmap_Params), Pointer :: map
     Type(SimCell_Params) :: cell
@@ -273,7 +320,11 @@ Contains
     Logical :: ok
     Read(*,'(a)') fname
     Read(*,*) fund
-    Call set_cell(cell, fund, fund)
+    If (xcell) Then
+      Call read_cell(cell)
+    Else
+      Call set_cell(cell, fund, fund)
+    End If
     Nullify(map)
     Call pmap_pmapinit(map, fund, Trim(fname))
     Call storebase_init(out, 0)
@@ -437,5 +488,45 @@ Contains
       Write(*,'(2L2)') ratio < 0.49, ratio > 0.51
     End Do
   End Subroutine job_r4cmp
+
+  ! FCELL input: cell block; npairs; npairs x (x1 y1 z1 x2 y2 z2); npts;
+  ! npts x (x y z); seed nrand.  Output: ORTHO, U (unslant, row-major), S
+  ! (slant), L (full-cell lattice vectors), per pair SEP (simcell_minimage,
+  ! fundcell.F90:582-623) and dist2, per point UC (simcell_maptouc) and PBC
+  ! (simcell_pbc), then RAND (simcell_uniformRandPt) after random_init(seed)
+  Subroutine job_fcell()
+    Type(SimCell_Params) :: cell
+    Type(VecType) :: a, b, sv, pt, o
+    Real(RDbl) :: v(6), d2
+    Integer :: n, i, j, seed
+    Call read_cell(cell)
+    Write(*,'(A,L2)') 'ORTHO', cell%fullC%orthoflag
+    Write(*,'(A,9ES26.17E3)') 'U', ((cell%fullC%unslantmatrix%comp(i,j), j = 1, 3), i = 1, 3)
+    Write(*,'(A,9ES26.17E3)') 'S', ((cell%fullC%slantmatrix%comp(i,j), j = 1, 3), i = 1, 3)
+    Write(*,'(A,9ES26.17E3)') 'L', (cell%fullC%lvec(i)%comp, i = 1, 3)
+    Read(*,*) n
+    Do i = 1, n
+      Read(*,*) v
+      a%comp = v(1:3)
+      b%comp = v(4:6)
+      Call simcell_minimage(cell, a, b, sv, d2)
+      Write(*,'(A,4ES26.17E3)') 'SEP', sv%comp, d2
+    End Do
+    Read(*,*) n
+    Do i = 1, n
+      Read(*,*) v(1:3)
+      pt%comp = v(1:3)
+      o = simcell_maptouc(cell, pt)
+      Write(*,'(A,3ES26.17E3)') 'UC', o%comp
+      Call simcell_pbc(cell, pt, o)
+      Write(*,'(A,3ES26.17E3)') 'PBC', o%comp
+    End Do
+    Read(*,*) seed, n
+    Call random_init(seed)
+    Do i = 1, n
+      o = simcell_uniformRandPt(cell)
+      Write(*,'(A,3ES26.17E3)') 'RAND', o%comp
+    End Do
+  End Subroutine job_fcell
 
 End Program ref_harness

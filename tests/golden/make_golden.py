@@ -177,12 +177,18 @@ def gen_compact_and_hermite():
         hl = ["HERMITE", "t.map", f17(*eff), str(len(pts))] + [f17(*p) for p in pts]
         hout, _ = run(hl, env=env, cwd=td)
         hrows = parse_rows(hout)
+        # the same map in a monoclinic cell whose effective box is the map's
+        xc = ([eff[0] * 0.9, eff[1], eff[2] * 0.95], [90.0, 104.0, 90.0], eff, [1, 1, 1])
+        pts_x = rng.uniform(-0.5, 1.5, (80, 3)) * np.array(eff)
+        hl = ["HERMITEX", "t.map", f17(*eff)] + _cell_lines(xc) + [str(len(pts_x))] + [f17(*p) for p in pts_x]
+        xrows = parse_rows(run(hl, env=env, cwd=td)[0])
     return dict(compact=dict(npnts=npnts, cutoff=cutoff, pcut=41.84, zatoms=zat,
                              vals=vals.tolist(), counts=counts,
                              file_sha256=hashlib.sha256(data).hexdigest(),
                              file_size=len(data)),
                 hermite=dict(npnts=npnts, eff=eff, points=pts.tolist(),
-                             ok=[r[0] for r in hrows], V=[r[1] for r in hrows]))
+                             ok=[r[0] for r in hrows], V=[r[1] for r in hrows]),
+                hermitex=dict(cell=xc, points=pts_x.tolist(), ok=[r[0] for r in xrows], V=[r[1] for r in xrows]))
 
 
 # atoms of the principal-axes molecules: name, symbol, mass
@@ -328,6 +334,72 @@ def gen_r4cmp():
     return dict(ratio=rs, below=[r[0] for r in rows], above=[r[1] for r in rows])
 
 
+# cells: (fund edges, angles alpha beta gamma, eff, cells per axis)
+CELLS = [([12.0, 13.0, 14.0], [90.0, 90.0, 90.0], [12.0, 13.0, 14.0], [2, 1, 1]),
+         ([12.0, 13.0, 14.0], [90.0, 101.5, 90.0], [14.7, 13.0, 13.8], [2, 1, 1]),
+         ([11.0, 12.5, 13.5], [85.0, 97.0, 110.0], [15.0, 13.5, 13.5], [1, 2, 1])]
+
+
+def _cell_lines(c):
+    ell, ang, eff, nc = c
+    return [f17(*ell), f17(*ang), "0 0 0", f17(*eff), "%d %d %d" % tuple(nc)]
+
+
+def gen_fcell():
+    """Slant / unslant matrices, lattice vectors, minimum images, maptouc,
+    pbc and uniform random points of orthorhombic, monoclinic and
+    triclinic cells (fundcell.F90, simcell.F90)."""
+    rng = np.random.default_rng(9090)
+    out = []
+    for c in CELLS:
+        full = np.array(c[0]) * np.array(c[3])
+        pairs = np.concatenate([rng.uniform(-0.5, 1.5, (60, 3)) * full, rng.uniform(-0.5, 1.5, (60, 3)) * full], 1)
+        pts = rng.uniform(-1.0, 2.0, (60, 3)) * full
+        lines = ["FCELL"] + _cell_lines(c) + [str(len(pairs))] + [f17(*v) for v in pairs]
+        lines += [str(len(pts))] + [f17(*v) for v in pts] + ["5151 40"]
+        res = dict(ell=c[0], angles=c[1], eff=c[2], ncell=c[3], pairs=pairs.tolist(), points=pts.tolist(),
+                   sep=[], uc=[], pbc=[], rand=[])
+        for l in run(lines)[0].splitlines():
+            f = l.split()
+            if not f:
+                continue
+            if f[0] == "ORTHO":
+                res["ortho"] = f[1] == "T"
+            elif f[0] in ("U", "S", "L"):
+                res[f[0]] = [float(x) for x in f[1:]]
+            elif f[0] == "SEP":
+                res["sep"].append([float(x) for x in f[1:]])
+            elif f[0] in ("UC", "PBC", "RAND"):
+                res[f[0].lower()].append([float(x) for x in f[1:]])
+        out.append(res)
+    return out
+
+
+def gen_ljnodex():
+    """LJ nodes of the test012 lattice atoms in the slanted cells."""
+    s = inputs.load_system(os.path.join(GOLD, "test012-pmapgen", "test012.ctr"))
+    probe = s["mols"]["Probe"]
+    A, B, hc2, lc2, on = inputs.lj_table_for_probe(s, int(probe["type"][0]))
+    rng = np.random.default_rng(12013)
+    out = []
+    for c in CELLS[1:]:
+        full = np.array(c[0]) * np.array(c[3])
+        xyz = rng.uniform(0, 1, (80, 3)) * full
+        typ = rng.integers(0, len(A), 80)
+        nodes = rng.uniform(0, 1, (60, 3)) * np.array(c[2])
+        lines = ["LJNODEX"] + _cell_lines(c) + [str(len(xyz))]
+        lines += [f17(*x) + " %d" % t for x, t in zip(xyz, typ)]
+        lines.append(str(len(A)))
+        # hicut below half the shortest width so that the single fold is exact
+        h2 = [min(h, 4.5 ** 2) for h in hc2]
+        lines += [f17(A[t], B[t], h2[t], lc2[t]) for t in range(len(A))]
+        lines += [str(len(nodes))] + [f17(*p) for p in nodes]
+        rows = parse_rows(run(lines)[0])
+        out.append(dict(cell=c, xyz=xyz.tolist(), type=typ.tolist(), A=list(A), B=list(B), hicut2=h2,
+                        locut2=list(lc2), nodes=nodes.tolist(), ok=[r[0] for r in rows], vals=[r[1:] for r in rows]))
+    return out
+
+
 def main():
     if not os.path.exists(HARNESS):
         raise SystemExit("build oracle/_ref first: make -f oracle/Makefile.ref")
@@ -343,6 +415,8 @@ def main():
     g["gauss"] = gen_gauss()
     g["bonded"] = gen_bonded()
     g["r4cmp"] = gen_r4cmp()
+    g["fcell"] = gen_fcell()
+    g["ljnodex"] = gen_ljnodex()
     with open(os.path.join(GOLD, "ref_vectors.json"), "w") as f:
         json.dump(g, f)
     print("wrote", os.path.join(GOLD, "ref_vectors.json"))
