@@ -76,6 +76,19 @@ int mh_ff_set_buck(mh_ctx *ctx, int ntypes, const double *A, const double *rho,
                    const double *C, const double *D, const double *hicut2,
                    const double *locut2);
 
+/* Slanted fundamental cell of the framework just uploaded (fundcell_init,
+ * fundcell_initxform, fundcell.F90:60-112, 378-412): angles alpha (b^c),
+ * beta (a^c), gamma (a^b) in degrees and the effective (orthorhombic) box of
+ * the maps, eff.  Within 0.001 degree of 90 the cell is orthorhombic and
+ * only eff is taken.  A slanted cell folds minimum images in its slanted
+ * basis (fundcell_multMinImage, :640-709) in the map build -- the FP64
+ * node kernel serves both precisions there -- and map interpolation folds
+ * points into the slanted unit cell without the periodic wrap of the upper
+ * corner (simcell_maptouc, simcell.F90:634-668; interpolate.F90:675-679).
+ * GCMC and HMC batches refuse slanted cells.  mh_framework_upload resets the
+ * cell to orthorhombic with eff = fund_ell. */
+int mh_framework_set_cell(mh_ctx *ctx, const double angles_deg[3], const double eff[3]);
+
 /* Ewald parameters and framework structure factor (ewald_init,
  * ewald_initCalc -> ewald_sfactor, ewald.F90:99-216, 558-681).  kappa =
  * kappa_param / min(full_ell).  Returns the number of accepted k vectors.
@@ -124,7 +137,10 @@ int mh_map_get_full(const mh_map *map, float *f32, double *f64);
 int mh_map_write(const mh_map *map, const char *path, int nzatoms, const char *names25,
                  const char *symbols2, const float *zparams7, float pcut_kJ);
 /* pmap_readmap (pmap.F90:321-358): read a map file onto the device.
- * fund_ell gives the grid spacing eff/(nbr-1) (mapheader.F90:140-146). */
+ * fund_ell gives the grid spacing eff/(nbr-1) (mapheader.F90:140-146): the
+ * map's effective box, which is the fundamental cell unless the context
+ * holds a slanted cell (mh_framework_set_cell), whose points then fold into
+ * that cell as in the map build. */
 int mh_map_load(mh_ctx *ctx, const char *path, const double fund_ell[3], mh_map **out);
 /* Upload a map held by the caller (same layout as mh_map_get). */
 int mh_map_from_host(mh_ctx *ctx, const int npnts[3], int64_t nfsize, const int32_t *ptr,

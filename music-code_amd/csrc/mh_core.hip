@@ -165,7 +165,9 @@ int mh_framework_upload(mh_ctx *ctx, int natoms, const double *xyz, const int *t
     for (int i = 0; i < 3; i++) {
         ctx->fund_ell[i] = fund_ell[i];
         ctx->full_ell[i] = full_ell[i];
+        ctx->eff[i] = fund_ell[i];
     }
+    ctx->slanted = false;
     ctx->ew.set = false;
     if (natoms == 0) return MH_OK;  // a cell without framework atoms (sorbate-only systems)
     int rc;
@@ -175,6 +177,127 @@ int mh_framework_upload(mh_ctx *ctx, int natoms, const double *xyz, const int *t
     MH_HIP(hipMemcpy(ctx->xyz.p, ctx->hxyz.data(), sizeof(double) * 3 * natoms, hipMemcpyHostToDevice));
     MH_HIP(hipMemcpy(ctx->type.p, type, sizeof(int) * natoms, hipMemcpyHostToDevice));
     MH_HIP(hipMemcpy(ctx->q.p, ctx->hq.data(), sizeof(double) * natoms, hipMemcpyHostToDevice));
+    return MH_OK;
+}
+
+// matrix_inverse (matrix.F90:195-229): LU with implicit scaling and partial
+// pivoting, one back substitution per column (matrixops_ludcmpReal /
+// lubksbReal, matrixops.F90:31-161); row-major 3x3
+static void lu_inverse3(const double m[9], double inv[9]) {
+    double a[3][3], vv[3];
+    int indx[3];
+    for (int i = 0; i < 9; i++) a[i / 3][i % 3] = m[i];
+    for (int i = 0; i < 3; i++) {
+        double aamax = 0.0;
+        for (int j = 0; j < 3; j++)
+            if (std::fabs(a[i][j]) > aamax) aamax = std::fabs(a[i][j]);
+        vv[i] = 1.0 / aamax;
+    }
+    for (int j = 0; j < 3; j++) {
+        for (int i = 1; i < j; i++) {
+            double sum = a[i][j];
+            for (int k = 0; k < i; k++) sum = sum - a[i][k] * a[k][j];
+            a[i][j] = sum;
+        }
+        double aamax = 0.0;
+        int imax = j;
+        for (int i = j; i < 3; i++) {
+            double sum = a[i][j];
+            if (j > 0) {
+                for (int k = 0; k < j; k++) sum = sum - a[i][k] * a[k][j];
+                a[i][j] = sum;
+            }
+            double dum = vv[i] * std::fabs(sum);
+            if (dum >= aamax) {
+                imax = i;
+                aamax = dum;
+            }
+        }
+        if (j != imax) {
+            for (int k = 0; k < 3; k++) std::swap(a[imax][k], a[j][k]);
+            vv[imax] = vv[j];
+        }
+        indx[j] = imax;
+        if (j != 2) {
+            if (a[j][j] == 0.0) a[j][j] = (double)1.0e-20f;
+            double dum = 1.0 / a[j][j];
+            for (int i = j + 1; i < 3; i++) a[i][j] = a[i][j] * dum;
+        }
+    }
+    if (a[2][2] == 0.0) a[2][2] = (double)1.0e-20f;
+    for (int col = 0; col < 3; col++) {
+        double b[3] = {0.0, 0.0, 0.0};
+        b[col] = 1.0;
+        int ii = -1;
+        for (int i = 0; i < 3; i++) {
+            const int ll = indx[i];
+            double sum = b[ll];
+            b[ll] = b[i];
+            if (ii >= 0) {
+                for (int j = ii; j < i; j++) sum = sum - a[i][j] * b[j];
+            } else if (sum != 0.0) {
+                ii = i;
+            }
+            b[i] = sum;
+        }
+        for (int i = 2; i >= 0; i--) {
+            double sum = b[i];
+            for (int j = i + 1; j < 3; j++) sum = sum - a[i][j] * b[j];
+            b[i] = sum / a[i][i];
+        }
+        for (int i = 0; i < 3; i++) inv[3 * i + col] = b[i];
+    }
+}
+
+int mh_framework_set_cell(mh_ctx *ctx, const double angles_deg[3], const double eff[3]) {
+    MH_CHECK(ctx && angles_deg && eff, MH_EINVAL, "mh_framework_set_cell: bad arguments");
+    MH_CHECK(check_cell(eff), MH_EINVAL, "mh_framework_set_cell: effective box edges must be positive");
+    for (int i = 0; i < 3; i++)
+        MH_CHECK(angles_deg[i] > 0.0 && angles_deg[i] < 180.0, MH_EINVAL,
+                 "mh_framework_set_cell: angles must lie in (0, 180) degrees");
+    // fundcell_init (fundcell.F90:60-112): orthorhombic within 0.001 degree
+    const double err = std::fabs(angles_deg[0] - 90.0) + std::fabs(angles_deg[1] - 90.0) +
+                       std::fabs(angles_deg[2] - 90.0);
+    for (int i = 0; i < 3; i++) ctx->eff[i] = eff[i];
+    ctx->slanted = err > 0.001;
+    if (!ctx->slanted) return MH_OK;
+    // fundcell_initxform (fundcell.F90:378-412): the angles in default real,
+    // single-precision cosines
+    const double degtorad = 3.1415927 / 180.0;
+    const float alpha = (float)(angles_deg[0] * degtorad), beta = (float)(angles_deg[1] * degtorad),
+                gamma = (float)(angles_deg[2] * degtorad);
+    const float cg = cosf(gamma), cb = cosf(beta), ca = cosf(alpha);
+    double U[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    U[0] = 1.0;
+    U[1] = (double)cg;
+    U[2] = (double)cb;
+    const float cg2 = cg * cg;
+    U[4] = std::sqrt(1.0 - (double)cg2);
+    const float num = ca - cg * cb;
+    U[5] = (double)num / U[4];
+    U[8] = std::sqrt(1.0 - U[2] * U[2] - U[5] * U[5]);
+    MH_CHECK(std::isfinite(U[5]) && std::isfinite(U[8]) && U[8] > 0.0, MH_EINVAL,
+             "mh_framework_set_cell: the angles do not form a cell");
+    double S[9];
+    lu_inverse3(U, S);
+    for (int i = 0; i < 9; i++) {
+        ctx->U[i] = U[i];
+        ctx->S[i] = S[i];
+    }
+    // slanted atom coordinates S a (fundcell_slant, matrix_m_mult_v order)
+    if (ctx->natoms > 0) {
+        std::vector<double> xs(3 * (size_t)ctx->natoms);
+        for (int a = 0; a < ctx->natoms; a++)
+            for (int r = 0; r < 3; r++) {
+                double s = 0.0;
+                for (int c = 0; c < 3; c++) s = s + S[3 * r + c] * ctx->hxyz[3 * a + c];
+                xs[3 * a + r] = s;
+            }
+        int rc;
+        if ((rc = ctx->xyz_s.alloc(sizeof(double) * xs.size()))) return rc;
+        MH_HIP(hipSetDevice(ctx->device));
+        MH_HIP(hipMemcpy(ctx->xyz_s.p, xs.data(), sizeof(double) * xs.size(), hipMemcpyHostToDevice));
+    }
     return MH_OK;
 }
 

@@ -11,23 +11,55 @@ namespace mh {
 static __device__ __forceinline__ double h0f(double x) { return x * x * (3.0 - 2.0 * x); }
 static __device__ __forceinline__ double h1f(double x, double ds) { return ds * x * x * (x - 1.0); }
 
+// A slanted fundamental cell (fundcell.F90:378-412): U maps slanted to
+// Cartesian coordinates, S = U^-1, row-major; origin 0.
+struct SlantCell {
+    double U[9], S[9];
+};
+
+// matrix_m_mult_v (matrix.F90:137-150): row r = ((0 + m r1 v1) + m r2 v2) + m r3 v3
+static __device__ __forceinline__ void mat_vec3(const double *m, const double *v, double *o) {
+    for (int r = 0; r < 3; r++) {
+        double s = 0.0;
+        s = __dadd_rn(s, __dmul_rn(m[3 * r], v[0]));
+        s = __dadd_rn(s, __dmul_rn(m[3 * r + 1], v[1]));
+        s = __dadd_rn(s, __dmul_rn(m[3 * r + 2], v[2]));
+        o[r] = s;
+    }
+}
+
 static __device__ __forceinline__ bool hermite_point(const int32_t *__restrict__ ptr, const float *__restrict__ vals,
                               int nx, int ny, int nz, double sx, double sy, double sz,
                               double ex, double ey, double ez, double rx, double ry, double rz,
-                              double &p) {
-    // simcell_maptouc (simcell.F90:634-668), orthorhombic, origin 0
+                              double &p, const SlantCell *sc = nullptr) {
+    // simcell_maptouc (simcell.F90:634-668), origin 0: slanted cells fold in
+    // their slanted basis, and without the periodic wrap of the upper corner
+    // (interpolate.F90:675-679) a point in the last cubelet fails
     double c[3] = {rx, ry, rz};
     const double ell[3] = {ex, ey, ez};
     const double st[3] = {sx, sy, sz};
     const int nb[3] = {nx, ny, nz};
     int idx[3], idp[3];
     double tuv[3];
+    if (sc) {
+        double v[3];
+        mat_vec3(sc->S, c, v);
+        for (int d = 0; d < 3; d++) {
+            double f = __dmul_rn(v[d], 1.0 / ell[d]);
+            f = (f >= 0.0) ? fmod(f, 1.0) : __dadd_rn(fmod(f, 1.0), 1.0);
+            v[d] = __dmul_rn(f, ell[d]);
+        }
+        mat_vec3(sc->U, v, c);
+    } else {
+        for (int d = 0; d < 3; d++) {
+            double f = c[d] * (1.0 / ell[d]);
+            f = (f >= 0.0) ? fmod(f, 1.0) : fmod(f, 1.0) + 1.0;
+            c[d] = f * ell[d];
+        }
+    }
     for (int d = 0; d < 3; d++) {
-        double f = c[d] * (1.0 / ell[d]);
-        f = (f >= 0.0) ? fmod(f, 1.0) : fmod(f, 1.0) + 1.0;
-        c[d] = f * ell[d];
         idx[d] = (int)(c[d] / st[d] + 1.0);  // mapheader_getIndices, Int()
-        if (idx[d] < 1 || idx[d] > nb[d]) {
+        if (idx[d] < 1 || idx[d] > nb[d] || (sc && idx[d] == nb[d])) {
             p = 0.0;
             return false;
         }

@@ -1515,6 +1515,10 @@ int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double 
     MH_CHECK(maxN >= 1 && maxN <= (1 << 20), MH_EINVAL, "mh_gcmc_create: maxN out of range");
     MH_CHECK(p->precision == MH_PREC_FP64 || p->precision == MH_PREC_FP32, MH_EINVAL,
              "mh_gcmc_create: bad precision");
+    MH_CHECK(!ctx->slanted, MH_EINVAL, "mh_gcmc_create: slanted cells are not offered for GCMC");
+    for (int a = 0; a < p->nsites && a < MH_MAX_SITES; a++)
+        MH_CHECK((!p->site_pmap[a] || !p->site_pmap[a]->slanted) && (!p->site_emap[a] || !p->site_emap[a]->slanted),
+                 MH_EINVAL, "mh_gcmc_create: maps of slanted cells are not offered for GCMC");
     MH_HIP(hipSetDevice(ctx->device));
     std::unique_ptr<mh_gcmc> g(new mh_gcmc());
     g->ctx = ctx;
@@ -1601,6 +1605,9 @@ int mh_pool_admit(mh_gcmc *g, const mh_gcmc_params *p, const double fund_ell[3],
              "mh_pool_admit: bad arguments");
     MH_CHECK(nrep >= 1 && nsteps >= 0, MH_EINVAL, "mh_pool_admit: bad nrep / nsteps");
     MH_CHECK(p->nsites <= g->ns, MH_EINVAL, "mh_pool_admit: more sites than the pool's max_sites");
+    for (int a = 0; a < p->nsites && a < MH_MAX_SITES; a++)
+        MH_CHECK((!p->site_pmap[a] || !p->site_pmap[a]->slanted) && (!p->site_emap[a] || !p->site_emap[a]->slanted),
+                 MH_EINVAL, "mh_pool_admit: maps of slanted cells are not offered for GCMC");
     MH_HIP(hipSetDevice(g->ctx->device));
     MH_HIP(hipStreamSynchronize(g->ctx->stream));  // no launch in flight while slots change
     int s = -1;

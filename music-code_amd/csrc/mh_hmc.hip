@@ -947,6 +947,7 @@ extern "C" {
 
 int mh_hmc_create(mh_ctx *ctx, const mh_hmc_params *p, int nrep, const int32_t *seeds, int maxN, mh_hmc **out) {
     MH_CHECK(ctx && p && seeds && out, MH_EINVAL, "mh_hmc_create: bad arguments");
+    MH_CHECK(!ctx->slanted, MH_EINVAL, "mh_hmc_create: slanted cells are not offered for HMC");
     MH_CHECK(nrep >= 1 && nrep <= 65535 && maxN >= 1, MH_EINVAL, "mh_hmc_create: bad nrep / maxN");
     MH_CHECK(p->nsites >= 1 && p->nsites <= HS && p->ntypes >= 1 && p->ntypes <= 8, MH_EINVAL,
              "mh_hmc_create: nsites in [1,8], ntypes in [1,8]");

@@ -77,6 +77,12 @@ struct mh_ctx {
     // framework
     int natoms = 0;
     double fund_ell[3] = {0, 0, 0}, full_ell[3] = {0, 0, 0};
+    // slanted cell (mh_framework_set_cell): unslant / slant matrices,
+    // effective box of the maps (the fundamental cell edges when not set)
+    bool slanted = false;
+    double U[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, S[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double eff[3] = {0, 0, 0};
+    mh::DBuf xyz_s; // double [natoms][3] slanted atom coordinates S a
     mh::DBuf xyz;   // double [natoms][3]
     mh::DBuf type;  // int [natoms]
     mh::DBuf q;     // double [natoms]
@@ -104,20 +110,21 @@ void ctx_release(mh_ctx *ctx);
 // Scratch buffer 'name' of at least 'bytes' bytes owned by the context.
 void *ws_get(mh_ctx *ctx, const std::string &name, size_t bytes, int *rc);
 
-// Records events around a region on ctx->stream when timing is enabled.
+// Records events around a region on ctx->stream when timing is enabled
+// (ctx may be NULL: no timing).
 struct KTimer {
     mh_ctx *ctx;
     const char *name;
     hipEvent_t a = nullptr, b = nullptr;
     KTimer(mh_ctx *c, const char *n) : ctx(c), name(n) {
-        if (ctx->timing) {
+        if (ctx && ctx->timing) {
             (void)hipEventCreate(&a);
             (void)hipEventCreate(&b);
             (void)hipEventRecord(a, ctx->stream);
         }
     }
     void stop() {
-        if (!ctx->timing || !a) return;
+        if (!ctx || !ctx->timing || !a) return;
         (void)hipEventRecord(b, ctx->stream);
         (void)hipEventSynchronize(b);
         float ms = 0.0f;
@@ -151,6 +158,8 @@ struct mh_map {
     int npnts[3] = {0, 0, 0};
     double step[3] = {0, 0, 0};
     double fund_ell[3] = {0, 0, 0};
+    bool slanted = false;  // the framework's cell: maptouc in the slanted basis, no corner wrap
+    double U[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, S[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     int64_t nfsize = 0;
     int64_t counts[4] = {0, 0, 0, 0};
     mh::DBuf ptr;    // int32 [nx][ny][nz] z fastest, 1-based

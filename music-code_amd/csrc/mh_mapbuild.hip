@@ -15,6 +15,9 @@
 // their original order and streams them to every lane.
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "mh_device.h"
 #include "mh_internal.h"
@@ -48,6 +51,11 @@ struct GridP {
     double kappa, charge1;
     float rc2f;      // FP32 Ewald real-space cutoff^2
     int refold;      // FP32 path: fold each separation again (cutoff near L/2)
+    // slanted cell (FP64 kernel): atoms arrive in slanted coordinates S a,
+    // a node p is slanted to S p, the difference folded on the full cell's
+    // edges and unslanted by U (fundcell_multMinImage, fundcell.F90:640-709)
+    int slanted;
+    double U[9], S[9];
 };
 
 __device__ __forceinline__ double fold1(double d, double L) {
@@ -298,7 +306,15 @@ __global__ void __launch_bounds__(NT) k_nodes64(GridP g, const PairTab *__restri
     const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
     const int x = bx * BX + (tid % BX), y = by * BY + ((tid / BX) % BY), z = bz * BZ + tid / (BX * BY);
     const bool valid = (x < g.nx) && (y < g.ny) && (z < g.nz);
-    const double px = (double)x * g.step[0], py = (double)y * g.step[1], pz = (double)z * g.step[2];
+    double px = (double)x * g.step[0], py = (double)y * g.step[1], pz = (double)z * g.step[2];
+    if (g.slanted) {
+        const double pc[3] = {px, py, pz};
+        double ps[3];
+        mat_vec3(g.S, pc, ps);
+        px = ps[0];
+        py = ps[1];
+        pz = ps[2];
+    }
     // brick bounding box
     const double lo[3] = {(double)(bx * BX) * g.step[0], (double)(by * BY) * g.step[1], (double)(bz * BZ) * g.step[2]};
     const double hi[3] = {(double)min(bx * BX + BX - 1, g.nx - 1) * g.step[0],
@@ -348,6 +364,14 @@ __global__ void __launch_bounds__(NT) k_nodes64(GridP g, const PairTab *__restri
                 double sx = fold1(px - s_x[j], g.full[0]);
                 double sy = fold1(py - s_y[j], g.full[1]);
                 double sz = fold1(pz - s_z[j], g.full[2]);
+                if (g.slanted) {
+                    const double d[3] = {sx, sy, sz};
+                    double c[3];
+                    mat_vec3(g.U, d, c);
+                    sx = c[0];
+                    sy = c[1];
+                    sz = c[2];
+                }
                 if (KIND == K_LJ) {
                     int t = s_t[j];
                     double hh[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -913,27 +937,34 @@ __global__ void k_assign(long long n, int nx, int ny, int nz, const uint8_t *__r
     }
 }
 
+__global__ void k_fill_i32(long long n, int32_t v, int32_t *__restrict__ p) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u64(int n, unsigned long long *__restrict__ p) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 }  // namespace mh
 
 using namespace mh;
 
 // ------------------------------------------------------------ host side
-static int scan_exclusive(mh_ctx *ctx, int level, long long n, const int32_t *in, int32_t *out,
-                          int64_t *total) {
+// exclusive prefix sum of in[0..n) into out, level buffers bsum[l], boff[l]
+// of at least n / 1024^(l+1) + 1 entries; total (level 0) on the host
+static int scan_exclusive_b(mh_ctx *ctx, int level, long long n, const int32_t *in, int32_t *out,
+                            int32_t *const *bsum, int32_t *const *boff, int64_t *total) {
     hipStream_t st = ctx->stream;
     long long nb = (n + SCAN_B - 1) / SCAN_B;
-    int rc;
-    std::string lv = std::to_string(level);
-    int32_t *bsum = (int32_t *)ws_get(ctx, "scan_bsum" + lv, sizeof(int32_t) * (nb + 1), &rc);
-    if (rc) return rc;
-    int32_t *boff = (int32_t *)ws_get(ctx, "scan_boff" + lv, sizeof(int32_t) * (nb + 1), &rc);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(SCAN_B), 0, st, n, in, out, bsum);
+    MH_CHECK(level < 4, MH_ECAPACITY, "scan: too many levels");
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(SCAN_B), 0, st, n, in, out, bsum[level]);
     MH_HIP(hipGetLastError());
     if (nb > 1) {
         int64_t sub = 0;
-        if ((rc = scan_exclusive(ctx, level + 1, nb, bsum, boff, &sub))) return rc;
-        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(SCAN_B), 0, st, n, out, boff);
+        int rc;
+        if ((rc = scan_exclusive_b(ctx, level + 1, nb, bsum[level], boff[level], bsum, boff, &sub))) return rc;
+        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(SCAN_B), 0, st, n, out, boff[level]);
         MH_HIP(hipGetLastError());
     }
     if (level == 0) {
@@ -944,6 +975,76 @@ static int scan_exclusive(mh_ctx *ctx, int level, long long n, const int32_t *in
         MH_HIP(hipStreamSynchronize(st));
         *total = (int64_t)lastx + lasti;
     }
+    return MH_OK;
+}
+
+static int scan_exclusive(mh_ctx *ctx, long long n, const int32_t *in, int32_t *out, int64_t *total) {
+    int32_t *bsum[4] = {nullptr, nullptr, nullptr, nullptr}, *boff[4] = {nullptr, nullptr, nullptr, nullptr};
+    long long m = n;
+    int rc = MH_OK;
+    for (int l = 0; l < 4 && m > 0; l++) {
+        long long nb = (m + SCAN_B - 1) / SCAN_B;
+        std::string lv = std::to_string(l);
+        bsum[l] = (int32_t *)ws_get(ctx, "scan_bsum" + lv, sizeof(int32_t) * (nb + 1), &rc);
+        if (rc) return rc;
+        boff[l] = (int32_t *)ws_get(ctx, "scan_boff" + lv, sizeof(int32_t) * (nb + 1), &rc);
+        if (rc) return rc;
+        if (nb == 1) break;
+        m = nb;
+    }
+    return scan_exclusive_b(ctx, 0, n, in, out, bsum, boff, total);
+}
+
+// compaction buffers (pmap_compact restated for the device)
+struct CompactBufs {
+    uint8_t *below;
+    int32_t *owner, *cnt, *base;
+    unsigned long long *tallies, *partial;
+};
+
+// below / owner / count / tallies / scan: returns the number of tabulated
+// records (host) and leaves ptr zeroed
+static int compact_count(mh_ctx *ctx, const int np[3], long long nodes, const float *full32, float cut,
+                         const CompactBufs &b, int32_t *const *bsum, int32_t *const *boff, int32_t *ptr,
+                         int64_t *total) {
+    hipStream_t st = ctx->stream;
+    unsigned gb = (unsigned)((nodes + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, 4, b.tallies);
+    hipLaunchKernelGGL(k_fill_i32, dim3(gb), dim3(256), 0, st, nodes, 0, ptr);
+    hipLaunchKernelGGL(k_below, dim3(gb), dim3(256), 0, st, nodes, np[0], np[1], np[2], full32, cut, b.below);
+    hipLaunchKernelGGL(k_owner, dim3(gb), dim3(256), 0, st, nodes, np[0], np[1], np[2], b.below, b.owner);
+    hipLaunchKernelGGL(k_count, dim3(gb), dim3(256), 0, st, nodes, np[0], np[1], np[2], b.below, b.owner, b.cnt,
+                       b.partial);
+    hipLaunchKernelGGL(k_tally, dim3(1), dim3(1024), 0, st, (long long)gb, b.partial, b.tallies);
+    if (hipGetLastError() != hipSuccess) {
+        set_error("map compaction: kernel launch failed");
+        return MH_EHIP;
+    }
+    if (bsum) return scan_exclusive_b(ctx, 0, nodes, b.cnt, b.base, bsum, boff, total);
+    return scan_exclusive(ctx, nodes, b.cnt, b.base, total);
+}
+
+// numbering and record copies; counts = acceptCut, acceptNeighbor, naccept, reject
+static int compact_assign(mh_ctx *ctx, const int np[3], long long nodes, const float *full32,
+                          const CompactBufs &b, int32_t *ptr, float *vals, int64_t total, int64_t counts[4]) {
+    hipStream_t st = ctx->stream;
+    unsigned gb = (unsigned)((nodes + 255) / 256);
+    hipLaunchKernelGGL(k_assign, dim3(gb), dim3(256), 0, st, nodes, np[0], np[1], np[2], b.below, b.owner, b.base,
+                       full32, ptr, vals);
+    if (hipGetLastError() != hipSuccess) {
+        set_error("map compaction: assign kernel launch failed");
+        return MH_EHIP;
+    }
+    unsigned long long t[4];
+    if (hipMemcpyAsync(t, b.tallies, sizeof t, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_error("map compaction: synchronisation failed");
+        return MH_EHIP;
+    }
+    counts[0] = (int64_t)t[0];
+    counts[1] = (int64_t)t[1];
+    counts[2] = total;
+    counts[3] = (int64_t)t[2];
     return MH_OK;
 }
 
@@ -1015,58 +1116,13 @@ static double real_space_cutoff(double kappa) {
     return 4.32 / kappa;
 }
 
-extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q,
-                            double gridspace, double hipotcut_kJ, int keep_full, mh_map **out) {
-    MH_CHECK(ctx && out, MH_EINVAL, "mh_map_build: bad arguments");
-    MH_CHECK(ctx->natoms > 0, MH_ESTATE, "mh_map_build: upload the framework first");
-    MH_CHECK(kind == MH_MAP_LJ || kind == MH_MAP_EWALD || kind == MH_MAP_BUCK, MH_EINVAL,
-             "mh_map_build: unknown map kind");
-    MH_CHECK(precision == MH_PREC_FP64 || precision == MH_PREC_FP32, MH_EINVAL,
-             "mh_map_build: unknown precision");
-    MH_CHECK(gridspace > 0.0, MH_EINVAL, "mh_map_build: gridspace must be > 0");
-    MH_CHECK(kind != MH_MAP_LJ || ctx->lj_set, MH_ESTATE, "mh_map_build: call mh_ff_set_lj first");
-    MH_CHECK(kind != MH_MAP_BUCK || ctx->buck_set, MH_ESTATE, "mh_map_build: call mh_ff_set_buck first");
-    MH_CHECK(kind != MH_MAP_EWALD || ctx->ew.set, MH_ESTATE, "mh_map_build: call mh_ewald_set first");
+// pair tables, grid parameters and the node kernel of one map: FP32 node
+// values into full32 (and FP64 into o64 when not NULL); dt is device memory
+// for the pair table
+static int run_nodes(mh_ctx *ctx, int kind, int precision, double probe_q, const int np[3], const double step[3],
+                     PairTab *dt, float *full32, double *o64, bool timed) {
     const LJTable &lj = ctx->lj;
-    MH_CHECK(kind != MH_MAP_LJ || ctx->max_type < lj.ntypes, MH_EINVAL,
-             "mh_map_build: framework atom type without LJ parameters");
-    MH_CHECK(kind != MH_MAP_BUCK || ctx->max_type < ctx->buck.ntypes, MH_EINVAL,
-             "mh_map_build: framework atom type without Buckingham parameters");
-    MH_HIP(hipSetDevice(ctx->device));
-    int np[3];
-    double step[3];
-    grid_dims(ctx->fund_ell, gridspace, np, step);
-    MH_CHECK(np[0] >= 3 && np[1] >= 3 && np[2] >= 3, MH_EINVAL,
-             "mh_map_build: at least 3 grid nodes per axis are required");
-    const long long nodes = (long long)np[0] * np[1] * np[2];
-    MH_CHECK(nodes < (1LL << 31), MH_ECAPACITY, "mh_map_build: more than 2^31 grid nodes");
-    mh_map *map = new mh_map();
-    map->ctx = ctx;
-    for (int i = 0; i < 3; i++) {
-        map->npnts[i] = np[i];
-        map->step[i] = step[i];
-        map->fund_ell[i] = ctx->fund_ell[i];
-    }
-    int rc = MH_OK;
-    auto fail = [&](int code) {
-        delete map;
-        return code;
-    };
-    // node values: owned by the map when kept, else context scratch
-    DBuf full32k, full64k;
-    float *full32 = nullptr;
-    double *full64 = nullptr;
     double *recip = nullptr;
-    const bool want64 = keep_full != 0;
-    if (keep_full) {
-        if ((rc = full32k.alloc(sizeof(float) * 8 * nodes))) return fail(rc);
-        if ((rc = full64k.alloc(sizeof(double) * 8 * nodes))) return fail(rc);
-        full32 = full32k.as<float>();
-        full64 = full64k.as<double>();
-    } else {
-        full32 = (float *)ws_get(ctx, "full32", sizeof(float) * 8 * nodes, &rc);
-        if (rc) return fail(rc);
-    }
     // pair tables
     PairTab tab;
     std::memset(&tab, 0, sizeof tab);
@@ -1096,10 +1152,8 @@ extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q
             maxh2 = fmax(maxh2, bk.hicut2[t]);
         }
     }
-    PairTab *dt = (PairTab *)ws_get(ctx, "pairtab", sizeof(PairTab), &rc);
-    if (rc) return fail(rc);
     if (hipMemcpyAsync(dt, &tab, sizeof tab, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
-        return fail(MH_EHIP);
+        return MH_EHIP;
     GridP g;
     g.nx = np[0];
     g.ny = np[1];
@@ -1114,18 +1168,30 @@ extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q
     g.r2safe = 0.0;
     g.rc2f = 0.0f;
     g.refold = 0;
+    g.slanted = ctx->slanted ? 1 : 0;
+    for (int i = 0; i < 9; i++) {
+        g.U[i] = ctx->U[i];
+        g.S[i] = ctx->S[i];
+    }
+    // slanted cells run the FP64 kernel for either precision, every atom
+    // staged for every brick (the brick bound is orthorhombic)
+    const bool slanted = ctx->slanted;
     if (kind == MH_MAP_EWALD) {
-        if ((rc = build_recip(ctx, np, step, probe_q, &recip))) return fail(rc);
-        if (precision == MH_PREC_FP32) {
+        int rc;
+        if ((rc = build_recip(ctx, np, step, probe_q, &recip))) return rc;
+        if (precision == MH_PREC_FP32 && !slanted) {
             double rc_real = real_space_cutoff(ctx->ew.kappa);
             g.cull2 = rc_real * rc_real * (1.0 + 1e-6);
             g.rc2f = (float)(rc_real * rc_real);
-            g.r2safe = 9.0;  // the large near terms (r < 3 A) are evaluated in FP64
+            // pair terms within 5 A are evaluated in FP64: with the FP32
+            // erfc / exp / reciprocal each near 1 ulp, the FP32 sum of the
+            // remaining terms keeps V within 1e-5 of max(|V|, 1 kcal/mol)
+            g.r2safe = 25.0;
         } else {
             g.cull2 = 1e300;  // the reference sums every atom (ewald.F90:874)
         }
     } else {
-        g.cull2 = maxh2 * (1.0 + 1e-9) + 1e-9;
+        g.cull2 = slanted ? 1e300 : maxh2 * (1.0 + 1e-9) + 1e-9;
         g.r2safe = fmax(2.25, 4.0 * maxl2);
     }
     {
@@ -1139,95 +1205,133 @@ extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q
     }
     const int nbx = (np[0] + BX - 1) / BX, nby = (np[1] + BY - 1) / BY, nbz = (np[2] + BZ - 1) / BZ;
     const long long nblk = (long long)nbx * nby * nbz;
-    const bool fp32 = (precision == MH_PREC_FP32) && kind != MH_MAP_BUCK;
-    double *o64 = want64 ? full64 : nullptr;
+    const bool fp32 = (precision == MH_PREC_FP32) && kind != MH_MAP_BUCK && !slanted;
+    const double *axyz = slanted ? ctx->xyz_s.as<double>() : ctx->xyz.as<double>();
     static const char *knames[2][4] = {
         {"", "nodes64_lj", "nodes64_ewald_real", "nodes64_buck"},
         {"", "nodes32_lj", "nodes32_ewald_real", "nodes64_buck"}};
-    KTimer tnode(ctx, knames[fp32 ? 1 : 0][kind]);
+    KTimer tnode(timed ? ctx : nullptr, knames[fp32 ? 1 : 0][kind]);
     if (kind == MH_MAP_LJ) {
         if (fp32)
             hipLaunchKernelGGL(k_nodes32<K_LJ>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
-                               ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
+                               axyz, ctx->type.as<int>(), ctx->q.as<double>(),
                                (const double *)nullptr, full32, o64);
         else
             hipLaunchKernelGGL(k_nodes64<K_LJ>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
-                               ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
+                               axyz, ctx->type.as<int>(), ctx->q.as<double>(),
                                (const double *)nullptr, full32, o64);
     } else if (kind == MH_MAP_BUCK) {
         hipLaunchKernelGGL(k_nodes64<K_BUCK>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
-                           ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
+                           axyz, ctx->type.as<int>(), ctx->q.as<double>(),
                            (const double *)nullptr, full32, o64);
     } else {
         if (fp32)
             hipLaunchKernelGGL(k_nodes32<K_EWALD>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
-                               ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
+                               axyz, ctx->type.as<int>(), ctx->q.as<double>(),
                                recip, full32, o64);
         else
             hipLaunchKernelGGL(k_nodes64<K_EWALD>, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, g, dt,
-                               ctx->xyz.as<double>(), ctx->type.as<int>(), ctx->q.as<double>(),
+                               axyz, ctx->type.as<int>(), ctx->q.as<double>(),
                                recip, full32, o64);
     }
     if (hipGetLastError() != hipSuccess) {
         set_error("mh_map_build: node kernel launch failed");
-        return fail(MH_EHIP);
+        return MH_EHIP;
     }
     tnode.stop();
+    return MH_OK;
+}
+
+extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q,
+                            double gridspace, double hipotcut_kJ, int keep_full, mh_map **out) {
+    MH_CHECK(ctx && out, MH_EINVAL, "mh_map_build: bad arguments");
+    MH_CHECK(ctx->natoms > 0, MH_ESTATE, "mh_map_build: upload the framework first");
+    MH_CHECK(kind == MH_MAP_LJ || kind == MH_MAP_EWALD || kind == MH_MAP_BUCK, MH_EINVAL,
+             "mh_map_build: unknown map kind");
+    MH_CHECK(precision == MH_PREC_FP64 || precision == MH_PREC_FP32, MH_EINVAL,
+             "mh_map_build: unknown precision");
+    MH_CHECK(gridspace > 0.0, MH_EINVAL, "mh_map_build: gridspace must be > 0");
+    MH_CHECK(kind != MH_MAP_LJ || ctx->lj_set, MH_ESTATE, "mh_map_build: call mh_ff_set_lj first");
+    MH_CHECK(kind != MH_MAP_BUCK || ctx->buck_set, MH_ESTATE, "mh_map_build: call mh_ff_set_buck first");
+    MH_CHECK(kind != MH_MAP_EWALD || ctx->ew.set, MH_ESTATE, "mh_map_build: call mh_ewald_set first");
+    const LJTable &lj = ctx->lj;
+    MH_CHECK(kind != MH_MAP_LJ || ctx->max_type < lj.ntypes, MH_EINVAL,
+             "mh_map_build: framework atom type without LJ parameters");
+    MH_CHECK(kind != MH_MAP_BUCK || ctx->max_type < ctx->buck.ntypes, MH_EINVAL,
+             "mh_map_build: framework atom type without Buckingham parameters");
+    MH_HIP(hipSetDevice(ctx->device));
+    int np[3];
+    double step[3];
+    grid_dims(ctx->eff, gridspace, np, step);
+    MH_CHECK(np[0] >= 3 && np[1] >= 3 && np[2] >= 3, MH_EINVAL,
+             "mh_map_build: at least 3 grid nodes per axis are required");
+    const long long nodes = (long long)np[0] * np[1] * np[2];
+    MH_CHECK(nodes < (1LL << 31), MH_ECAPACITY, "mh_map_build: more than 2^31 grid nodes");
+    mh_map *map = new mh_map();
+    map->ctx = ctx;
+    for (int i = 0; i < 3; i++) {
+        map->npnts[i] = np[i];
+        map->step[i] = step[i];
+        map->fund_ell[i] = ctx->fund_ell[i];
+    }
+    map->slanted = ctx->slanted;
+    for (int i = 0; i < 9; i++) {
+        map->U[i] = ctx->U[i];
+        map->S[i] = ctx->S[i];
+    }
+    int rc = MH_OK;
+    auto fail = [&](int code) {
+        delete map;
+        return code;
+    };
+    // node values: owned by the map when kept, else context scratch
+    DBuf full32k, full64k;
+    float *full32 = nullptr;
+    double *full64 = nullptr;
+    const bool want64 = keep_full != 0;
+    if (keep_full) {
+        if ((rc = full32k.alloc(sizeof(float) * 8 * nodes))) return fail(rc);
+        if ((rc = full64k.alloc(sizeof(double) * 8 * nodes))) return fail(rc);
+        full32 = full32k.as<float>();
+        full64 = full64k.as<double>();
+    } else {
+        full32 = (float *)ws_get(ctx, "full32", sizeof(float) * 8 * nodes, &rc);
+        if (rc) return fail(rc);
+    }
+    double *o64 = want64 ? full64 : nullptr;
+    PairTab *dt = (PairTab *)ws_get(ctx, "pairtab", sizeof(PairTab), &rc);
+    if (rc) return fail(rc);
+    if ((rc = run_nodes(ctx, kind, precision, probe_q, np, step, dt, full32, o64, true))) {
+        if (rc == MH_EHIP) set_error("mh_map_build: node kernel launch failed");
+        return fail(rc);
+    }
     KTimer tcomp(ctx, "compaction");
-    // compaction
-    uint8_t *below = (uint8_t *)ws_get(ctx, "below", nodes, &rc);
+    CompactBufs cb;
+    cb.below = (uint8_t *)ws_get(ctx, "below", nodes, &rc);
     if (rc) return fail(rc);
-    int32_t *owner = (int32_t *)ws_get(ctx, "owner", sizeof(int32_t) * nodes, &rc);
+    cb.owner = (int32_t *)ws_get(ctx, "owner", sizeof(int32_t) * nodes, &rc);
     if (rc) return fail(rc);
-    int32_t *cnt = (int32_t *)ws_get(ctx, "cnt", sizeof(int32_t) * nodes, &rc);
+    cb.cnt = (int32_t *)ws_get(ctx, "cnt", sizeof(int32_t) * nodes, &rc);
     if (rc) return fail(rc);
-    int32_t *base = (int32_t *)ws_get(ctx, "base", sizeof(int32_t) * nodes, &rc);
+    cb.base = (int32_t *)ws_get(ctx, "base", sizeof(int32_t) * nodes, &rc);
     if (rc) return fail(rc);
-    unsigned long long *tallies = (unsigned long long *)ws_get(ctx, "tallies", 64, &rc);
+    cb.tallies = (unsigned long long *)ws_get(ctx, "tallies", 64, &rc);
+    if (rc) return fail(rc);
+    cb.partial = (unsigned long long *)ws_get(ctx, "cpartial",
+                                              sizeof(unsigned long long) * 3 * ((nodes + 255) / 256), &rc);
     if (rc) return fail(rc);
     if ((rc = map->ptr.alloc(sizeof(int32_t) * nodes))) return fail(rc);
-    if (hipMemsetAsync(tallies, 0, sizeof(unsigned long long) * 4, ctx->stream) != hipSuccess ||
-        hipMemsetAsync(map->ptr.p, 0, sizeof(int32_t) * nodes, ctx->stream) != hipSuccess)
-        return fail(MH_EHIP);
-    float cut = (float)(hipotcut_kJ / 4.184);  // mapmaker.F90:562, Real(cutoff,RSgl)
-    unsigned gb = (unsigned)((nodes + 255) / 256);
-    hipLaunchKernelGGL(k_below, dim3(gb), dim3(256), 0, ctx->stream, nodes, np[0], np[1], np[2],
-                       full32, cut, below);
-    hipLaunchKernelGGL(k_owner, dim3(gb), dim3(256), 0, ctx->stream, nodes, np[0], np[1], np[2],
-                       below, owner);
-    unsigned long long *partial = (unsigned long long *)ws_get(ctx, "cpartial", sizeof(unsigned long long) * 3 * gb, &rc);
-    if (rc) return fail(rc);
-    hipLaunchKernelGGL(k_count, dim3(gb), dim3(256), 0, ctx->stream, nodes, np[0], np[1], np[2],
-                       below, owner, cnt, partial);
-    hipLaunchKernelGGL(k_tally, dim3(1), dim3(1024), 0, ctx->stream, (long long)gb, partial, tallies);
-    if (hipGetLastError() != hipSuccess) {
-        set_error("mh_map_build: compaction kernel launch failed");
-        return fail(MH_EHIP);
-    }
+    const float cut = (float)(hipotcut_kJ / 4.184);  // mapmaker.F90:562, Real(cutoff,RSgl)
     int64_t total = 0;
-    if ((rc = scan_exclusive(ctx, 0, nodes, cnt, base, &total)))
+    if ((rc = compact_count(ctx, np, nodes, full32, cut, cb, nullptr, nullptr, map->ptr.as<int32_t>(), &total)))
         return fail(rc);
     MH_CHECK(total < (1LL << 31), MH_ECAPACITY, "mh_map_build: more than 2^31 tabulated points");
     map->nfsize = total;
     if ((rc = map->vals.alloc(sizeof(float) * 8 * (total + 1)))) return fail(rc);
-    hipLaunchKernelGGL(k_assign, dim3(gb), dim3(256), 0, ctx->stream, nodes, np[0], np[1], np[2],
-                       below, owner, base,
-                       full32, map->ptr.as<int32_t>(), map->vals.as<float>());
-    if (hipGetLastError() != hipSuccess) {
-        set_error("mh_map_build: assign kernel launch failed");
-        return fail(MH_EHIP);
-    }
-    unsigned long long t[4];
-    if (hipMemcpyAsync(t, tallies, sizeof t, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        set_error("mh_map_build: synchronisation failed");
-        return fail(MH_EHIP);
-    }
+    if ((rc = compact_assign(ctx, np, nodes, full32, cb, map->ptr.as<int32_t>(), map->vals.as<float>(), total,
+                             map->counts)))
+        return fail(rc);
     tcomp.stop();
-    map->counts[0] = (int64_t)t[0];
-    map->counts[1] = (int64_t)t[1];
-    map->counts[2] = total;
-    map->counts[3] = (int64_t)t[2];
     if (keep_full) {
         map->full32.p = full32k.p;
         map->full32.n = full32k.n;

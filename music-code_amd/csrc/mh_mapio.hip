@@ -11,12 +11,12 @@ namespace mh {
 __global__ void k_interp(int n, const double *__restrict__ xyz, const int32_t *__restrict__ ptr,
                          const float *__restrict__ vals, int nx, int ny, int nz, double sx,
                          double sy, double sz, double ex, double ey, double ez, double factor,
-                         double *__restrict__ e, uint8_t *__restrict__ ok) {
+                         int slanted, SlantCell sc, double *__restrict__ e, uint8_t *__restrict__ ok) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double p;
     bool good = hermite_point(ptr, vals, nx, ny, nz, sx, sy, sz, ex, ey, ez, xyz[3 * i],
-                              xyz[3 * i + 1], xyz[3 * i + 2], p);
+                              xyz[3 * i + 1], xyz[3 * i + 2], p, slanted ? &sc : nullptr);
     e[i] = good ? p * factor : 0.0;
     ok[i] = good ? 1 : 0;
 }
@@ -174,6 +174,16 @@ static int upload_map(mh_ctx *ctx, const int np[3], int64_t nf, const int32_t *p
         m->fund_ell[i] = fund_ell[i];
         m->step[i] = fund_ell[i] / ((double)np[i] - 1.0);  // mapheader.F90:140-146
     }
+    // a context with a slanted cell: fund_ell is the map's effective box,
+    // points fold into the context's slanted fundamental cell
+    if (ctx->slanted) {
+        m->slanted = true;
+        for (int i = 0; i < 3; i++) m->fund_ell[i] = ctx->fund_ell[i];
+        for (int i = 0; i < 9; i++) {
+            m->U[i] = ctx->U[i];
+            m->S[i] = ctx->S[i];
+        }
+    }
     m->nfsize = nf;
     m->counts[2] = nf;
     int rc;
@@ -255,10 +265,16 @@ int mh_map_interp(mh_ctx *ctx, const mh_map *map, int n, const double *xyz, doub
         (rc = dok.alloc(n)))
         return rc;
     MH_HIP(hipMemcpyAsync(dx.p, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+    SlantCell sc;
+    for (int i = 0; i < 9; i++) {
+        sc.U[i] = map->U[i];
+        sc.S[i] = map->S[i];
+    }
     hipLaunchKernelGGL(k_interp, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, dx.as<double>(),
                        map->ptr.as<int32_t>(), map->vals.as<float>(), map->npnts[0], map->npnts[1],
                        map->npnts[2], map->step[0], map->step[1], map->step[2], map->fund_ell[0],
-                       map->fund_ell[1], map->fund_ell[2], factor, de.as<double>(), dok.as<uint8_t>());
+                       map->fund_ell[1], map->fund_ell[2], factor, map->slanted ? 1 : 0, sc, de.as<double>(),
+                       dok.as<uint8_t>());
     MH_HIP(hipGetLastError());
     MH_HIP(hipMemcpyAsync(e, de.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
     MH_HIP(hipMemcpyAsync(ok, dok.p, n, hipMemcpyDeviceToHost, ctx->stream));

@@ -99,6 +99,7 @@ program music_gcmc
      end if
   end do
   call read_system(ctrl, sy)
+  if (sy%slanted) call fail('slanted cells are not offered by music_gcmc (music_mapmaker builds their maps)')
 
   ! ---- General Information: description, iterations, print interval,
   ! crash and config intervals, first simulation number, seed
@@ -654,11 +655,11 @@ contains
     write(u, '(2i12)') ix(rep + 1), iy(rep + 1)
     write(u, '(i12)') n
     do m = 1, n
-       write(u, '(6es25.17e3)') com(:, m), ang(:, m)
+       write(u, '(6es26.17e3)') com(:, m), ang(:, m)
     end do
     allocate(dl(2, nrep_now()))
     call check(mh_gcmc_get_deltas(g, dl), 'mh_gcmc_get_deltas')
-    write(u, '(2es25.17e3)') dl(:, rep + 1)
+    write(u, '(2es26.17e3)') dl(:, rep + 1)
     ! attempt / acceptance counts: adjustable displacements follow their ratio
     write(u, '(8i21)') att(:, rep + 1), acc(:, rep + 1)
     close(u)

@@ -53,7 +53,7 @@ module music_hip_mod
   end type mh_gcmc_params
 
   public :: mh_error_message
-  public :: mh_ctx_create, mh_ctx_destroy, mh_framework_upload, mh_ff_set_lj, mh_ff_set_buck
+  public :: mh_ctx_create, mh_ctx_destroy, mh_framework_upload, mh_framework_set_cell, mh_ff_set_lj, mh_ff_set_buck
   public :: mh_ewald_set, mh_map_build, mh_map_destroy, mh_map_info, mh_map_get, mh_map_write
   public :: mh_map_load, mh_map_interp
   public :: mh_gcmc_create, mh_gcmc_destroy, mh_gcmc_set_threads, mh_gcmc_autotune
@@ -96,6 +96,13 @@ module music_hip_mod
        integer(c_int), intent(in) :: types(*)
        integer(c_int) :: rc
      end function mh_framework_upload
+
+     function mh_framework_set_cell(ctx, angles_deg, eff) bind(C, name="mh_framework_set_cell") result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: angles_deg(3), eff(3)
+       integer(c_int) :: rc
+     end function mh_framework_set_cell
 
      function mh_ff_set_lj(ctx, ntypes, A, B, hicut2, locut2, on) bind(C, name="mh_ff_set_lj") result(rc)
        import :: c_int, c_ptr, c_double, c_int8_t

@@ -5,7 +5,7 @@
 ! fundcell.F90), the atom-atom forcefield file (lj_init, lj.F90:88-262) and
 ! the species-species file (ssdriver.F90:81-253, EWALD options of
 ! ewald_init, ewald.F90:99-180).  Only the subset the GCMC energy path
-! needs: LJ and Ewald pair models, orthorhombic cells.
+! needs: LJ and Ewald pair models; slanted cells for the map driver.
 module music_inputs
   use, intrinsic :: iso_c_binding
   implicit none
@@ -34,7 +34,7 @@ module music_inputs
      logical :: rigid = .false., translate = .true., rotate = .true.
      integer :: dof = 0                              ! Molecule_DOF (0: not given)
      logical :: has_fundcell = .false.
-     real(c_double) :: ell(3) = 0.0d0, angles(3) = 90.0d0
+     real(c_double) :: ell(3) = 0.0d0, angles(3) = 90.0d0, eff(3) = 0.0d0
   end type molecule
 
   type, public :: lj_pair
@@ -69,6 +69,10 @@ module music_inputs
      type(molecule), allocatable :: mols(:)
      integer :: nxyz(3) = 1
      real(c_double) :: fund(3) = 0.0d0, full(3) = 0.0d0
+     ! cell angles (degrees), effective (map) box, lattice vectors lvec(:, i)
+     ! of the fundamental cell; slanted: not orthorhombic within 0.001 degree
+     real(c_double) :: angles(3) = 90.0d0, eff(3) = 0.0d0, lvec(3, 3) = 0.0d0
+     logical :: slanted = .false.
      real(c_double), allocatable :: xyz(:,:), q(:)   ! replicated fixed species
      integer(c_int), allocatable :: typ(:)
      character(len=NL) :: aafile = '', ssfile = ''
@@ -338,6 +342,7 @@ contains
           m%has_fundcell = .true.
           read(strip(t%line(i + 1)), *) m%ell
           read(strip(t%line(i + 2)), *) m%angles
+          read(strip(t%line(i + 4)), *) m%eff
           i = i + 4
        end if
        i = i + 1
@@ -612,18 +617,21 @@ contains
        s%ifw = 0
        read(strip(s%t%line(i + 1)), *) s%fund
        read(strip(s%t%line(i + 2)), *) ang
+       read(strip(s%t%line(i + 4)), *) s%eff
        read(strip(s%t%line(i + 5)), *) s%nxyz
-       if (any(abs(ang - 90.0d0) > 1.0d-9)) call fail('only orthorhombic cells are supported')
+       s%angles = ang
        s%full = s%fund * s%nxyz
        s%ntot = 0
        allocate(s%xyz(3, 1), s%typ(1), s%q(1))
+       call lattice_vectors(s%fund, s%angles, s%lvec)
     else
        read(strip(s%t%line(i + 1)), *) s%nxyz
        s%ifw = mol_index(s, fwname)
        associate (fw => s%mols(s%ifw))
          if (.not. fw%has_fundcell) call fail('fundamental cell molecule has no Fundcell_Info')
-         if (any(abs(fw%angles - 90.0d0) > 1.0d-9)) call fail('only orthorhombic cells are supported')
          s%fund = fw%ell
+         s%angles = fw%angles
+         s%eff = fw%eff
          s%full = s%fund * s%nxyz
          n = fw%natoms
          s%ntot = n * s%nxyz(1) * s%nxyz(2) * s%nxyz(3)
@@ -632,10 +640,12 @@ contains
          s%typ(1:n) = fw%atype
          s%q(1:n) = fw%q
        end associate
+       ! copies along the lattice vectors a, then b, then c (simcell.F90:
+       ! 200-216, molecule_expand, molecule.F90:637-700: shift lvec * c)
+       call lattice_vectors(s%fund, s%angles, s%lvec)
        do ax = 1, 3
           do c = 1, s%nxyz(ax) - 1
-             off = 0.0d0
-             off(ax) = s%fund(ax) * c
+             off = s%lvec(:, ax) * real(c, c_double)
              do k = 1, n
                 s%xyz(:, c * n + k) = s%xyz(:, k) + off
                 s%typ(c * n + k) = s%typ(k)
@@ -645,12 +655,36 @@ contains
           n = n * s%nxyz(ax)
        end do
     end if
+    ! fundcell_init (fundcell.F90:96-102): orthorhombic within 0.001 degree
+    s%slanted = sum(abs(s%angles - 90.0d0)) > 0.001d0
+    if (all(s%eff <= 0.0d0)) s%eff = s%fund
     i = find_section(s%t, 'Forcefield Information')
     call need(i, 'Forcefield Information')
     s%aafile = word(strip(s%t%line(i + 2)), 1)
     s%ssfile = word(strip(s%t%line(i + 3)), 1)
     call read_atom_atom(find_file(s%aafile, 'NONE', '', s%base), s%pairs, s%npairs)
   end subroutine read_system
+
+  ! fundcell_genlatticevects (fundcell.F90:174-208): a along x, b in the xy
+  ! plane, c completing the edge lengths; angles in radians with the
+  ! reference's pi = 3.1415927 (defaults.F90:33-35)
+  subroutine lattice_vectors(ell, ang, lv)
+    real(c_double), intent(in) :: ell(3), ang(3)
+    real(c_double), intent(out) :: lv(3, 3)
+    real(c_double), parameter :: degtorad = 3.1415927d0 / 180.0d0
+    real(c_double) :: abc, aac, aab, m2
+    abc = ang(1) * degtorad
+    aac = ang(2) * degtorad
+    aab = ang(3) * degtorad
+    lv(:, 1) = [ell(1), 0.0d0, 0.0d0]
+    lv(:, 2) = [ell(2) * cos(aab), ell(2) * sin(aab), 0.0d0]
+    lv(1, 3) = ell(3) * cos(aac)
+    lv(2, 3) = ell(3) * (cos(abc) - cos(aac) * cos(aab)) / sin(aab)
+    m2 = 0.0d0
+    m2 = m2 + lv(1, 3) * lv(1, 3)
+    m2 = m2 + lv(2, 3) * lv(2, 3)
+    lv(3, 3) = sqrt(ell(3) * ell(3) - m2)
+  end subroutine lattice_vectors
 
   ! molecule_genprincipleaxes (molecule.F90:830-926), applied to rigid
   ! molecules of 3 or more atoms (molecule.F90:714-716, 843), in the

@@ -69,6 +69,8 @@ program music_mapmaker
   call check(mh_ctx_create(0_c_int, ctx), 'mh_ctx_create')
   call check(mh_framework_upload(ctx, int(sy%ntot, c_int), sy%xyz, sy%typ, sy%q, sy%fund, sy%full), &
        'mh_framework_upload')
+  ! the cell angles and the effective box the grid spans (config_makegrid)
+  call check(mh_framework_set_cell(ctx, sy%angles, sy%eff), 'mh_framework_set_cell')
   nt = nat
   allocate(ljA(nt), ljB(nt), h2(nt), l2(nt), on(nt))
   do k = 1, njobs

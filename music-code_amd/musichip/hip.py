@@ -71,6 +71,7 @@ def lib():
         "mh_ff_set_lj": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
         "mh_ff_set_buck": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp],
         "mh_ewald_set": [_vp, _d, _i, _d, _i, ctypes.POINTER(_i)],
+        "mh_framework_set_cell": [_vp, _vp, _vp],
         "mh_ewald_get": [_vp, _vp, _vp, _vp, _vp],
         "mh_map_build": [_vp, _i, _i, _d, _d, _d, _i, ctypes.POINTER(_vp)],
         "mh_map_destroy": [_vp],
@@ -226,6 +227,13 @@ class Context:
         fu = np.ascontiguousarray(full_ell, dtype=np.float64)
         check(lib().mh_framework_upload(self.h, len(types), _p(xyz), _p(types), _p(q), _p(fe), _p(fu)),
               "mh_framework_upload")
+
+    def set_cell(self, angles_deg, eff):
+        """Cell angles (degrees) and the effective map box of the framework
+        just uploaded (mh_framework_set_cell)."""
+        an = np.ascontiguousarray(angles_deg, dtype=np.float64)
+        ef = np.ascontiguousarray(eff, dtype=np.float64)
+        check(lib().mh_framework_set_cell(self.h, _p(an), _p(ef)), "mh_framework_set_cell")
 
     def set_lj(self, A, B, hicut2, locut2, on=None):
         a = [np.ascontiguousarray(v, dtype=np.float64) for v in (A, B, hicut2, locut2)]

@@ -10,7 +10,7 @@ Formats follow the reference readers:
   spc_spc_file lines                           ssdriver.F90:81-253
   mapmaker section                             mapmaker.F90:69-312
 Only the subset needed by the GCMC energy hot path is read: LJ / EWALD
-pair models, rigid molecules, orthorhombic cells.
+pair models, rigid molecules; slanted cells for the map path.
 """
 import math
 import os
@@ -705,29 +705,50 @@ def load_system(ctrl_path):
         fc = mols[sc["fundcell_source"]]["fundcell"]
     else:
         fc = sc["fund"]
-    if np.any(np.abs(np.asarray(fc["angles"]) - 90.0) > 1e-9):
-        raise NotImplementedError("only orthorhombic cells are supported")
     fund = np.asarray(fc["ell"], dtype=np.float64)
+    angles = np.asarray(fc["angles"], dtype=np.float64)
+    eff = np.asarray(fc["eff"], dtype=np.float64) if fc.get("eff") is not None else fund.copy()
     nxyz = np.asarray(sc["nxyz"], dtype=np.int64)
     full = fund * nxyz
+    lvec = lattice_vectors(fund, angles)
     # replicate the framework (simcell.F90:200-216 -> molecule_expand,
-    # molecule.F90:637-700): copies along x, then y, then z
+    # molecule.F90:637-700): copies along the fundamental cell's lattice
+    # vectors a, then b, then c, copy i shifted by lvec * i
     if sc["fundcell_source"]:
         fw = mols[sc["fundcell_source"]]
         xyz, typ, q = fw["xyz"], fw["type"], fw["q"]
         for ax in range(3):
             n = int(nxyz[ax])
-            off = np.zeros(3)
-            off[ax] = fund[ax]
-            xyz = np.concatenate([xyz + off * i for i in range(n)])
+            off = lvec[ax]
+            xyz = np.concatenate([xyz + off * float(i) for i in range(n)])
             typ = np.concatenate([typ] * n)
             q = np.concatenate([q] * n)
         fw["full_xyz"], fw["full_type"], fw["full_q"] = xyz, typ, q
     ff = c["forcefield"]
     aa = read_atom_atom(find_file(ff["atom_atom_file"], "NONE", base))
     ss = read_spc_spc(find_file(ff["spc_spc_file"], "NONE", base))
+    slanted = float(np.sum(np.abs(angles - 90.0))) > 0.001  # fundcell_init orthoflag
     return dict(ctrl=c, atypes=atypes, aidx=aidx, mols=mols, fund_ell=fund, full_ell=full,
-                nxyz=nxyz, aa=aa, ss=ss, seed=c["general"]["iseed"])
+                nxyz=nxyz, aa=aa, ss=ss, seed=c["general"]["iseed"], angles=angles, eff=eff,
+                slanted=slanted, lvec=lvec)
+
+
+def lattice_vectors(ell, angles_deg):
+    """fundcell_genlatticevects (fundcell.F90:174-208): a along x, b in the
+    xy plane, c completing the edge lengths; the angles in radians with the
+    reference's pi = 3.1415927 (defaults.F90:33-35).  Rows a, b, c."""
+    degtorad = 3.1415927 / 180.0
+    abc, aac, aab = [float(a) * degtorad for a in angles_deg]
+    e = [float(x) for x in ell]
+    l1 = [e[0], 0.0, 0.0]
+    l2 = [e[1] * math.cos(aab), e[1] * math.sin(aab), 0.0]
+    c1 = e[2] * math.cos(aac)
+    c2 = e[2] * (math.cos(abc) - math.cos(aac) * math.cos(aab)) / math.sin(aab)
+    m2 = 0.0
+    m2 = m2 + c1 * c1
+    m2 = m2 + c2 * c2
+    m2 = m2 + 0.0 * 0.0
+    return np.array([l1, l2, [c1, c2, math.sqrt(e[2] * e[2] - m2)]])
 
 
 def lj_table_for_probe(system, probe_type, ntypes=None):

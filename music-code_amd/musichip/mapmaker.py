@@ -24,6 +24,9 @@ def setup_framework(ctx, system, mapsorb):
     fw = system["mols"][mapsorb]
     ctx.upload_framework(fw["full_xyz"], fw["full_type"], fw["full_q"], system["fund_ell"],
                          system["full_ell"])
+    # cell angles and the effective box the grid spans (config_makegrid,
+    # config.F90:762-766, on the fundamental cell's eff)
+    ctx.set_cell(system["angles"], system["eff"])
     return fw
 
 

@@ -60,9 +60,11 @@ def test_fp32_maps_and_kernel_trial_energies_vs_fp64(hip, co2_maps):
         com, ang, _ = b64.config(r)
         cfgs.append((com, ang))
         b32.set_config(r, com, ang)
-        tc = rng.uniform(0, 1, (512, 3)) * L
-        ta = np.stack([rng.uniform(-np.pi, np.pi, 512), np.arccos(rng.uniform(-1, 1, 512)),
-                       rng.uniform(-np.pi, np.pi, 512)], 1)
+        # about one random placement in ten lands on tabulated corners
+        nt = 4096
+        tc = rng.uniform(0, 1, (nt, 3)) * L
+        ta = np.stack([rng.uniform(-np.pi, np.pi, nt), np.arccos(rng.uniform(-1, 1, nt)),
+                       rng.uniform(-np.pi, np.pi, nt)], 1)
         ok64, e64 = b64.trial_energy(r, tc, ta)
         ok32, e32 = b32.trial_energy(r, tc, ta)
         # the same failures (untabulated corners, overlaps) on both sides
@@ -81,12 +83,13 @@ def test_fp32_maps_and_kernel_trial_energies_vs_fp64(hip, co2_maps):
     g = _oracle_for(s64["spec"], s64["maps"], fug[1], 5)
     g.set_config(com, ang)
     e0 = g.total_energy()
-    tc = rng.uniform(0, 1, (16, 3)) * L
-    ta = np.stack([rng.uniform(-np.pi, np.pi, 16), np.arccos(rng.uniform(-1, 1, 16)),
-                   rng.uniform(-np.pi, np.pi, 16)], 1)
+    # about one random placement in ten is tabulated: 160 placements
+    tc = rng.uniform(0, 1, (160, 3)) * L
+    ta = np.stack([rng.uniform(-np.pi, np.pi, 160), np.arccos(rng.uniform(-1, 1, 160)),
+                   rng.uniform(-np.pi, np.pi, 160)], 1)
     ok64, e64 = b64.trial_energy(1, tc, ta)
     checked = 0
-    for i in range(16):
+    for i in range(160):
         g.set_config(np.vstack([com, tc[i:i + 1]]), np.vstack([ang, ta[i:i + 1]]))
         e1 = g.total_energy()
         if not ok64[i]:
@@ -124,7 +127,7 @@ def test_fp32_map_derivatives_and_interpolation_vs_fp64(hip):
         _, a32 = m32.get_full()
         live = a64[:, 0] < 100.0 / 4.184
         scale = np.maximum(np.abs(a64[live, 0]), FLOOR)
-        for c in range(1, 8):
+        for c in range(8):
             e = np.abs(a32[live, c] - a64[live, c]) * h ** order[c] / scale
             print("  column %d (order %d): max |d32 - d64| h^n / max(|V|, 1) = %.3e" % (c, order[c], e.max()),
                   flush=True)

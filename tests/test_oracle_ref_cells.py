@@ -77,3 +77,25 @@ def test_hermite_in_slanted_cell_without_wrap(golden):
     assert list(ok) == gh["ok"]
     assert np.array_equal(V[ok], np.array(gh["V"])[ok])
     assert 0 < sum(gh["ok"]) < len(gh["ok"])
+
+
+def test_host_lattice_vectors_and_replication(golden, tmp_path):
+    """The Python host's lattice vectors (fundcell_genlatticevects) equal the
+    reference's bit for bit, and a framework is replicated along them
+    (simcell.F90:200-216 -> molecule_expand): copy i along axis k is shifted
+    by i lvec_k, so even a 90-degree cell carries the reference's
+    truncated-pi components."""
+    import os
+    import shutil
+    from musichip import inputs
+    for g in golden["fcell"]:
+        L = inputs.lattice_vectors(np.array(g["ell"]) * np.array(g["ncell"]), g["angles"])
+        assert np.array_equal(L.ravel(), np.array(g["L"]))
+    here = os.path.dirname(os.path.abspath(__file__))
+    s = inputs.load_system(os.path.join(here, "golden", "test012-pmapgen", "test012.ctr"))
+    fw = s["mols"]["Lattice"]
+    n = len(fw["xyz"])
+    assert list(s["nxyz"]) == [1, 1, 2]
+    lv = inputs.lattice_vectors(s["fund_ell"], s["angles"])
+    assert lv[2][0] != 0.0  # cos of the truncated-pi right angle
+    assert np.array_equal(fw["full_xyz"][n:], fw["xyz"] + lv[2] * 1.0)
