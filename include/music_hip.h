@@ -115,6 +115,24 @@ int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q, double gr
                  double hipotcut_kJ, int keep_full, mh_map **out);
 int mh_map_destroy(mh_map *map);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mh_map_build for LJ and Buckingham maps that calls no
+ * hipMalloc / hipFree / allocating library routine.  The node values and
+ * compaction arrays live in a file-scope device array (about 100 MB per
+ * device, held until the process ends) of at most 2^21 grid nodes; larger
+ * grids return MH_ECAPACITY.  The compacted map goes to the caller's device
+ * buffers: ptr_dev int32[nx*ny*nz] (the mh_map_get layout) and vals_dev
+ * float[vals_capacity][8] (record-major: r, rx, ry, rz, rxy, rxz, ryz, rxyz
+ * per tabulated point).  ptr_dev == NULL only fills npnts_out (size query).
+ * nfsize_out receives the tabulated count even when it exceeds
+ * vals_capacity (then MH_ECAPACITY, nothing written to vals_dev).  Calls
+ * on one device share the scratch array: calls that overlap on one device
+ * race. */
+int mh_map_build_static(mh_ctx *ctx, int kind, int precision, double gridspace, double hipotcut_kJ,
+                        int32_t *ptr_dev, float *vals_dev, int64_t vals_capacity, int npnts_out[3],
+                        int64_t *nfsize_out, int64_t counts_out[4]);
+
 /* npnts[3] (nodes per axis), nfsize (tabulated points), counts[4] =
  * accepted below cutoff, accepted neighbours, total, rejected (pmap.F90:
  * 651-654), step[3] (grid spacing). */

@@ -112,6 +112,9 @@ int mh_ctx_create(int device, mh_ctx **out) {
     MH_HIP(hipGetDeviceCount(&ndev));
     MH_CHECK(device >= 0 && device < ndev, MH_EINVAL, "mh_ctx_create: no such device");
     MH_HIP(hipSetDevice(device));
+    // device address of the map builder's static scratch on this device; a
+    // failure is recorded there and only mh_map_build_static refuses to run
+    (void)static_scratch_init(device);
     mh_ctx *c = new mh_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -109,6 +109,8 @@ int map_dense(mh_ctx *ctx, mh_map *m, const float **out);
 void ctx_release(mh_ctx *ctx);
 // Scratch buffer 'name' of at least 'bytes' bytes owned by the context.
 void *ws_get(mh_ctx *ctx, const std::string &name, size_t bytes, int *rc);
+// once per device (mh_ctx_create): address of mh_map_build_static's scratch
+int static_scratch_init(int device);
 
 // Records events around a region on ctx->stream when timing is enabled
 // (ctx may be NULL: no timing).

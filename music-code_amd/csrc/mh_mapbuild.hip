@@ -946,9 +946,54 @@ __global__ void k_fill_u64(int n, unsigned long long *__restrict__ p) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
+// Scratch of mh_map_build_static: a file-scope device array (one copy per
+// device, held until the process ends), carved as below; its device address
+// per device comes from hipGetSymbolAddress (the array's name is a host
+// stand-in, not a device address).
+constexpr long long kStaticMaxNodes = 1LL << 21;
+constexpr size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr long long kStaticBlocks256 = kStaticMaxNodes / 256;
+constexpr long long kStaticScan0 = kStaticMaxNodes / 1024 + 1;
+constexpr size_t kOffFull32 = 0;
+constexpr size_t kOffBelow = kOffFull32 + al256(sizeof(float) * 8 * kStaticMaxNodes);
+constexpr size_t kOffOwner = kOffBelow + al256(kStaticMaxNodes);
+constexpr size_t kOffCnt = kOffOwner + al256(sizeof(int32_t) * kStaticMaxNodes);
+constexpr size_t kOffBase = kOffCnt + al256(sizeof(int32_t) * kStaticMaxNodes);
+constexpr size_t kOffPartial = kOffBase + al256(sizeof(int32_t) * kStaticMaxNodes);
+constexpr size_t kOffTallies = kOffPartial + al256(sizeof(unsigned long long) * 3 * kStaticBlocks256);
+constexpr size_t kOffScan = kOffTallies + 256;
+constexpr size_t kScanLevelBytes = al256(sizeof(int32_t) * kStaticScan0);
+constexpr size_t kOffPairTab = kOffScan + 8 * kScanLevelBytes;
+constexpr size_t kStaticScratchBytes = kOffPairTab + al256(sizeof(PairTab));
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
 }  // namespace mh
 
 using namespace mh;
+
+// device address of `scratch` per device, set once by static_scratch_init
+static std::mutex g_scratch_mu;
+static std::vector<unsigned char *> g_scratch_ptr;
+
+// called by mh_ctx_create with `device` current
+int mh::static_scratch_init(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (g_scratch_ptr[device]) return MH_OK;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        return MH_EHIP;
+    }
+    g_scratch_ptr[device] = (unsigned char *)p;
+    return MH_OK;
+}
+
+static unsigned char *static_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
 
 // ------------------------------------------------------------ host side
 // exclusive prefix sum of in[0..n) into out, level buffers bsum[l], boff[l]
@@ -1344,4 +1389,68 @@ extern "C" int mh_map_build(mh_ctx *ctx, int kind, int precision, double probe_q
     ctx_retain(ctx);
     *out = map;
     return MH_OK;
+}
+
+// The allocation-free variant of mh_map_build for LJ and Buckingham maps:
+// the caller's device buffers take the compacted map, the node values and
+// compaction arrays live in the file-scope scratch array (kStaticMaxNodes
+// grid nodes at most).  No hipMalloc / hipFree / allocating library call;
+// counters and ptr are cleared by fill kernels.  Calls on one device share
+// the scratch, so calls that overlap on one device race.
+extern "C" int mh_map_build_static(mh_ctx *ctx, int kind, int precision, double gridspace, double hipotcut_kJ,
+                                   int32_t *ptr_dev, float *vals_dev, int64_t vals_capacity, int npnts_out[3],
+                                   int64_t *nfsize_out, int64_t counts_out[4]) {
+    MH_CHECK(ctx && npnts_out, MH_EINVAL, "mh_map_build_static: bad arguments");
+    MH_CHECK(ctx->natoms > 0, MH_ESTATE, "mh_map_build_static: upload the framework first");
+    MH_CHECK(kind == MH_MAP_LJ || kind == MH_MAP_BUCK, MH_EINVAL,
+             "mh_map_build_static: LJ and Buckingham maps only (Ewald maps need per-node reciprocal scratch)");
+    MH_CHECK(precision == MH_PREC_FP64 || precision == MH_PREC_FP32, MH_EINVAL,
+             "mh_map_build_static: unknown precision");
+    MH_CHECK(gridspace > 0.0, MH_EINVAL, "mh_map_build_static: gridspace must be > 0");
+    MH_CHECK(kind != MH_MAP_LJ || ctx->lj_set, MH_ESTATE, "mh_map_build_static: call mh_ff_set_lj first");
+    MH_CHECK(kind != MH_MAP_BUCK || ctx->buck_set, MH_ESTATE, "mh_map_build_static: call mh_ff_set_buck first");
+    MH_CHECK(kind != MH_MAP_LJ || ctx->max_type < ctx->lj.ntypes, MH_EINVAL,
+             "mh_map_build_static: framework atom type without LJ parameters");
+    MH_CHECK(kind != MH_MAP_BUCK || ctx->max_type < ctx->buck.ntypes, MH_EINVAL,
+             "mh_map_build_static: framework atom type without Buckingham parameters");
+    int np[3];
+    double step[3];
+    grid_dims(ctx->eff, gridspace, np, step);
+    for (int i = 0; i < 3; i++) npnts_out[i] = np[i];
+    if (!ptr_dev) return MH_OK;  // size query
+    MH_CHECK(vals_dev && nfsize_out && counts_out, MH_EINVAL, "mh_map_build_static: bad output arguments");
+    MH_CHECK(np[0] >= 3 && np[1] >= 3 && np[2] >= 3, MH_EINVAL,
+             "mh_map_build_static: at least 3 grid nodes per axis are required");
+    const long long nodes = (long long)np[0] * np[1] * np[2];
+    MH_CHECK(nodes <= kStaticMaxNodes, MH_ECAPACITY,
+             "mh_map_build_static: more grid nodes than the static scratch holds (2^21); use mh_map_build");
+    MH_HIP(hipSetDevice(ctx->device));
+    unsigned char *sc = static_scratch(ctx->device);
+    MH_CHECK(sc, MH_ESTATE, "mh_map_build_static: no scratch address for this device (mh_ctx_create sets it)");
+    float *full32 = (float *)(sc + kOffFull32);
+    CompactBufs cb;
+    cb.below = (uint8_t *)(sc + kOffBelow);
+    cb.owner = (int32_t *)(sc + kOffOwner);
+    cb.cnt = (int32_t *)(sc + kOffCnt);
+    cb.base = (int32_t *)(sc + kOffBase);
+    cb.partial = (unsigned long long *)(sc + kOffPartial);
+    cb.tallies = (unsigned long long *)(sc + kOffTallies);
+    int32_t *bsum[4], *boff[4];
+    for (int l = 0; l < 4; l++) {
+        bsum[l] = (int32_t *)(sc + kOffScan + (2 * l) * kScanLevelBytes);
+        boff[l] = (int32_t *)(sc + kOffScan + (2 * l + 1) * kScanLevelBytes);
+    }
+    PairTab *dt = (PairTab *)(sc + kOffPairTab);
+    int rc;
+    if ((rc = run_nodes(ctx, kind, precision, 0.0, np, step, dt, full32, nullptr, false))) {
+        if (rc == MH_EHIP) set_error("mh_map_build_static: node kernel launch failed");
+        return rc;
+    }
+    const float cut = (float)(hipotcut_kJ / 4.184);  // mapmaker.F90:562, Real(cutoff,RSgl)
+    int64_t total = 0;
+    if ((rc = compact_count(ctx, np, nodes, full32, cut, cb, bsum, boff, ptr_dev, &total))) return rc;
+    *nfsize_out = total;
+    MH_CHECK(total <= vals_capacity, MH_ECAPACITY,
+             "mh_map_build_static: more tabulated points than vals_capacity records (nfsize_out holds the count)");
+    return compact_assign(ctx, np, nodes, full32, cb, ptr_dev, vals_dev, total, counts_out);
 }
