@@ -17,6 +17,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mh_device.h"
@@ -48,6 +49,7 @@ struct GridP {
     int natoms;
     double cull2;    // atoms farther than this from the brick are skipped
     double r2safe;   // FP32 path: pairs closer than this are evaluated in FP64
+    double r2pot;    // FP32 Ewald: the potential term of pairs closer than this in FP64
     double kappa, charge1;
     float rc2f;      // FP32 Ewald real-space cutoff^2
     int refold;      // FP32 path: fold each separation again (cutoff near L/2)
@@ -210,8 +212,8 @@ __device__ __forceinline__ EwC ewc_make(double kappa) {
 // (2 kappa / sqrt(pi)) q e^{-k^2 r^2}, w = 1/r^2:
 //   d/dx = -s1 w x;  d2/dxdy = (3 s1 + 2 k^2 r^2 g) w^2 x y;
 //   d3/dxdydz = -(15 s1 + (10 + 4 k^2 r^2) k^2 r^2 g) w^3 x y z.
-__device__ __forceinline__ void ewald_real_acc32(const EwC &c, float qq, float sx, float sy, float sz, float r2,
-                                                 double &S0, float *hf) {
+__device__ __forceinline__ float ewald_real_acc32(const EwC &c, float qq, float sx, float sy, float sz, float r2,
+                                                  float *hf) {
     const float a1 = 0.254829592f, a2 = -0.284496736f, a3 = 1.421413741f, a4 = -1.453152027f;
     const float ri1 = __frsqrt_rn(r2);
     const float r = r2 * ri1;
@@ -224,7 +226,6 @@ __device__ __forceinline__ void ewald_real_acc32(const EwC &c, float qq, float s
     const float u = pu * ri1;  // q erfc(kr)/r
     const float g = c.twoksp * qek;
     const float s1 = fmaf(pu, ri1, g);
-    S0 += (double)u;
     const float w = ri1 * ri1;
     const float t1 = -s1 * w;
     hf[0] = fmaf(sx, t1, hf[0]);
@@ -242,6 +243,47 @@ __device__ __forceinline__ void ewald_real_acc32(const EwC &c, float qq, float s
     const float t3 = fmaf(fmaf(4.0f, k2r2, 10.0f), gk, 15.0f * s1) * (w2 * w);
     const float t3xy = t3 * sxy;
     hf[6] = fmaf(-t3xy, sz, hf[6]);
+    return u;
+}
+
+// A near pair of the FP32 Ewald kernel in FP64: returns q erfc(kr)/r
+// (A&S 7.1.26 as erfc_ref64) and adds the 7 derivative terms to col(i, v)
+// one at a time, in the factored form of ewald_real_acc32 (few live values).
+template <class Col>
+__device__ __forceinline__ double ewald_near64(double kappa, double qq, double sx, double sy, double sz, double d2,
+                                               Col col) {
+    const double a1 = (double)0.254829592f, a2 = (double)-0.284496736f, a3 = (double)1.421413741f,
+                 a4 = (double)-1.453152027f, a5 = (double)1.061405429f, p = (double)0.3275911f;
+    const double r = sqrt(d2);
+    const double kr = kappa * r;
+    const double t = 1.0 / (1.0 + p * kr);
+    const double ek = exp(-(kr * kr));
+    const double ri = 1.0 / r;
+    const double u = qq * (t * (a1 + t * (a2 + t * (a3 + t * (a4 + t * a5))))) * ek * ri;
+    const double g = 2.0 * 0.5641896 * kappa * qq * ek;
+    const double s1 = u + g;
+    const double w = ri * ri;
+    const double t1 = -s1 * w;
+    col(0, sx * t1);
+    col(1, sy * t1);
+    col(2, sz * t1);
+    const double k2r2 = kappa * kappa * d2;
+    const double t2 = (3.0 * s1 + 2.0 * k2r2 * g) * (w * w);
+    col(3, t2 * sx * sy);
+    col(4, t2 * sx * sz);
+    col(5, t2 * sy * sz);
+    const double t3 = -(15.0 * s1 + (10.0 + 4.0 * k2r2) * k2r2 * g) * (w * w * w);
+    col(6, t3 * sx * sy * sz);
+    return u;
+}
+
+// q erfc(kr)/r of one pair in FP64 (the potential term of
+// ewald_rwaldHOT, ewald.F90:874-916, as ewald_real_pair64)
+__device__ __forceinline__ double ewald_real_pot64(double kappa, double charge1, double q2, double d2) {
+    const double dist = sqrt(d2);
+    const double erfkr = erfc_ref64(kappa * dist);
+    const double ri1 = 1.0 / dist;
+    return __dmul_rn(__dmul_rn(__dmul_rn(charge1, q2), erfkr), ri1);
 }
 
 // FP32 LJ pair with derivatives
@@ -418,46 +460,20 @@ __global__ void __launch_bounds__(NT) k_nodes64(GridP g, const PairTab *__restri
 }
 
 // FP32 node kernel (LJ or Ewald real space).  Separations are formed
-// relative to the brick centre c: sep = (p - c) + fold(c - a) with fold(c - a)
-// held as a hi+lo float pair, so the FP32 separation keeps ~1e-8 A absolute
-// accuracy even next to an atom.  When a cutoff plus the brick half-diagonal
-// reaches half the cell, each component is folded again (same single fold as
-// the reference).  LJ: FP32 pair terms summed in FP32 over tiles of 32 atoms,
-// FP64 Kahan sums across tiles.  Ewald: every FP32 term is added to FP64
-// sums.  Pairs closer than sqrt(r2safe) and LJ pairs within rounding of
-// hicut are evaluated with the FP64 routine (reference cutoff decisions).
+// relative to the brick centre c: sep = (p - c) + fold(c - a), both parts
+// rounded to FP32 (|p - c| is under a brick half-diagonal, so the FP32
+// separation of a near atom keeps ~1e-7 A absolute accuracy).  When a cutoff
+// plus the brick half-diagonal reaches half the cell, each component is
+// folded again (same single fold as the reference).  FP32 pair terms are
+// summed in FP32 over tiles of 32 atoms and the tiles in FP64 (the Ewald
+// potential term goes to the FP64 sum per pair: its terms cancel strongly).
+// Pairs closer than sqrt(r2safe) and LJ pairs within rounding of hicut are
+// evaluated with the FP64 routine (reference cutoff decisions).
 __device__ __forceinline__ float foldf(float d, float L) {
     float h = 0.5f * L;
     d = (d > h) ? d - L : d;
     d = (d < -h) ? d + L : d;
     return d;
-}
-
-// Near pairs of the FP32 kernels in FP64 (reference decisions), out of line
-// so the rare path does not hold registers in the pair loop.
-struct Near {
-    double v[8];
-    int ok;
-};
-
-__device__ __forceinline__ Near near_lj64(double A, double B, double h2, double l2, double ex, double ey, double ez) {
-    Near o;
-    double pot = 0.0, h[7] = {0, 0, 0, 0, 0, 0, 0};
-    o.ok = lj_pair64(A, B, h2, l2, ex, ey, ez, pot, h) != 0;
-    o.v[0] = pot;
-    for (int i = 0; i < 7; i++) o.v[i + 1] = h[i];
-    return o;
-}
-
-__device__ __forceinline__ Near near_ewald64(double kappa, double charge1, double q, double ex, double ey, double ez) {
-    Near o;
-    double pot = 0.0, h[7] = {0, 0, 0, 0, 0, 0, 0};
-    const double d2 = ex * ex + ey * ey + ez * ez;
-    o.ok = !(d2 < kLowCut2);
-    if (o.ok) ewald_real_pair64(kappa, charge1, q, ex, ey, ez, d2, pot, h);
-    o.v[0] = pot;
-    for (int i = 0; i < 7; i++) o.v[i + 1] = h[i];
-    return o;
 }
 
 // a workgroup-uniform double kept in SGPRs (frees VGPRs in the pair loop)
@@ -468,6 +484,19 @@ __device__ __forceinline__ double uniform_double(double v) {
     return __longlong_as_double(((unsigned long long)hi << 32) | lo);
 }
 
+// LJ pair coefficients of one listed atom (FP32, formed once per brick):
+// with u = 1/r^6 and w = 1/r^2,
+//   V      = (A u - B) u
+//   dV/dx  = -(12 A u - 6 B) u w x
+//   d2V/dxdy = (168 A u - 48 B) u w^2 x y
+//   d3V/dxdydz = (480 B - 2688 A u) u w^3 x y z
+// c1 = (A, -B, 12 A, -6 B), c2 = (168 A, -48 B, -2688 A, 480 B).
+struct LJRec {
+    float4 p;       // x, y, z relative to the brick centre, hicut^2 (1 - 1e-6)
+    float4 c1, c2;  // pair coefficients
+    float4 e;       // hicut^2 (1 + 1e-6), unused
+};
+
 // at most 128 VGPRs: four waves per SIMD hide the transcendental and LDS latency
 template <int KIND>
 __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__restrict__ tab_g,
@@ -477,17 +506,17 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
                                                  const double *__restrict__ recip,
                                                  float *__restrict__ out32,
                                                  double *__restrict__ out64) {
-    // culled atoms, one LDS record per atom read with one ds_read_b128:
-    // (x, y, z hi parts relative to the brick centre, p0 = A or charge q);
-    // LJ adds (x, y, z lo parts, B) and the FP32 cutoff band (h2 -+ 1e-6 h2)
-    __shared__ float4 s_a4[SEG32];
-    __shared__ float4 s_b4[KIND == K_LJ ? SEG32 : 1];
-    __shared__ float2 s_hb[KIND == K_LJ ? SEG32 : 1];
+    // culled atoms, one LDS record per atom: (x, y, z relative to the brick
+    // centre, p0) with p0 = hicut^2 (1 - 1e-6) for LJ and the charge product
+    // for Ewald; LJ adds its pair coefficients and hicut^2 (1 + 1e-6) with the
+    // atom index (the FP64 path's band around the cutoff)
+    __shared__ float4 s_a4[KIND == K_LJ ? 1 : SEG32];
+    __shared__ LJRec s_lj[KIND == K_LJ ? SEG32 : 1];
     __shared__ int s_a[SEG32];
     __shared__ int s_wtot[NT / 64];
-    // FP64 sums of the 7 derivatives, one lane-private column per node
-    // (added once per 32-atom tile; keeps them out of the VGPR budget)
-    __shared__ double s_acc[7][NT];
+    // Ewald: FP64 sums of the 7 derivatives, one lane-private column per node
+    // (keeps them out of the VGPR budget of the longer Ewald pair)
+    __shared__ double s_acc[KIND == K_EWALD ? 7 : 1][NT];
     __shared__ PairTab tab;
     const int tid = threadIdx.x;
     if (KIND == K_LJ) {
@@ -512,15 +541,24 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
         cen[d] = uniform_double(cen[d]);
     }
     const float dx0 = (float)(px - cen[0]), dy0 = (float)(py - cen[1]), dz0 = (float)(pz - cen[2]);
-    const float r2safe = (float)g.r2safe;
+    const float r2safe = (float)g.r2safe, r2pot = (float)g.r2pot;
     const EwC ewc = ewc_make(g.kappa);
     const float Lx = (float)g.full[0], Ly = (float)g.full[1], Lz = (float)g.full[2];
     const bool refold = g.refold != 0;
+    // FP64 node sums: FP32 tile partials (32 atoms) and the FP64 near pairs;
+    // the potential in a register, the derivatives in registers (LJ) or in
+    // s_acc (Ewald)
+    double S[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (KIND == K_EWALD)
+        for (int i = 0; i < 7; i++) s_acc[i][tid] = 0.0;
+    // (volatile: the Ewald columns stay in LDS, out of the registers of the
+    // longer pair loop)
+    volatile double *acc = &s_acc[0][tid];
+    auto dsum = [&](int i, double v) {  // derivative column i (0..6)
+        if (KIND == K_EWALD) acc[i * NT] = acc[i * NT] + v;
+        else S[i + 1] += v;
+    };
     __syncthreads();
-    // FP64 node sums: FP32 tile partials (32 atoms) and FP64 near pairs;
-    // the potential in a register, the derivatives in s_acc
-    double S0 = 0.0;
-    for (int i = 0; i < 7; i++) s_acc[i][tid] = 0.0;
     bool failed = false;
     for (int seg0 = 0; seg0 < g.natoms; seg0 += SEG32) {
         const int segn = min(SEG32, g.natoms - seg0);
@@ -549,19 +587,21 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
             }
             if (keep) {
                 int pos = off + __popcll(m & ((1ull << lane) - 1ull));
-                double fx = fold1(cen[0] - ax, g.full[0]);
-                double fy = fold1(cen[1] - ay, g.full[1]);
-                double fz = fold1(cen[2] - az, g.full[2]);
-                float hx = (float)fx, hy = (float)fy, hz = (float)fz;
+                const float hx = (float)fold1(cen[0] - ax, g.full[0]);
+                const float hy = (float)fold1(cen[1] - ay, g.full[1]);
+                const float hz = (float)fold1(cen[2] - az, g.full[2]);
                 s_a[pos] = a;
                 if (KIND == K_LJ) {
                     const int t = typ[a];
-                    s_a4[pos] = make_float4(hx, hy, hz, (float)tab.A[t]);
-                    s_b4[pos] = make_float4((float)(fx - (double)hx), (float)(fy - (double)hy),
-                                            (float)(fz - (double)hz), (float)tab.B[t]);
+                    const float A = (float)tab.A[t], B = (float)tab.B[t];
                     const float h2f = (float)tab.h2[t];
                     const float band = h2f * 1.0e-6f;
-                    s_hb[pos] = make_float2(h2f - band, h2f + band);
+                    LJRec rec;
+                    rec.p = make_float4(hx, hy, hz, h2f - band);
+                    rec.c1 = make_float4(A, -B, 12.0f * A, -6.0f * B);
+                    rec.c2 = make_float4(168.0f * A, -48.0f * B, -2688.0f * A, 480.0f * B);
+                    rec.e = make_float4(h2f + band, 0.0f, 0.0f, 0.0f);
+                    s_lj[pos] = rec;
                 } else {
                     s_a4[pos] = make_float4(hx, hy, hz, (float)(g.charge1 * qv[a]));
                 }
@@ -571,86 +611,121 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
         }
         if (valid && !failed) {
             for (int j0 = 0; j0 < cnt && !failed; j0 += 32) {
-                float pf = 0.0f, hf[7] = {0, 0, 0, 0, 0, 0, 0};
+                float hf[8] = {0, 0, 0, 0, 0, 0, 0, 0};
                 const int j1 = min(cnt, j0 + 32);
-                for (int j = j0; j < j1; j++) {
-                    // LJ: hi/lo atom offsets (steep repulsion); Ewald pairs
-                    // beyond the FP64 radius are smooth: hi part only
-                    float sx, sy, sz;
-                    const float4 a4 = s_a4[j];
-                    float4 b4;
-                    if (KIND == K_LJ) {
-                        b4 = s_b4[j];
-                        sx = (dx0 + b4.x) + a4.x;
-                        sy = (dy0 + b4.y) + a4.y;
-                        sz = (dz0 + b4.z) + a4.z;
-                    } else {
-                        sx = dx0 + a4.x;
-                        sy = dy0 + a4.y;
-                        sz = dz0 + a4.z;
-                    }
-                    if (refold) {
+                // one listed atom against the lane's node (unrolled by two
+                // below: the LDS reads of a pair share one address)
+                auto pair = [&](const int j, auto refold_c) {
+                    const float4 a4 = (KIND == K_LJ) ? s_lj[j].p : s_a4[j];
+                    float sx = dx0 + a4.x, sy = dy0 + a4.y, sz = dz0 + a4.z;
+                    if (decltype(refold_c)::value) {
                         sx = foldf(sx, Lx);
                         sy = foldf(sy, Ly);
                         sz = foldf(sz, Lz);
                     }
-                    // Ewald: fused (the cutoff is smooth); LJ: the rounding its hicut band assumes
-                    const float r2 = (KIND == K_EWALD) ? fmaf(sx, sx, fmaf(sy, sy, sz * sz)) : sx * sx + sy * sy + sz * sz;
+                    const float r2 = fmaf(sx, sx, fmaf(sy, sy, sz * sz));
                     if (KIND == K_LJ) {
-                        const float2 hb = s_hb[j];
-                        if (r2 > hb.y) continue;
-                        // near-atom pairs and pairs within rounding of hicut:
-                        // FP64 separation and reference cutoff decisions
-                        if (r2 < r2safe || r2 > hb.x) {
-                            int a = s_a[j];
-                            const int t = typ[a];
-                            // node position recomputed here (not kept live in the loop)
-                            double ex = fold1((double)x * g.step[0] - xyz[3 * a], g.full[0]);
-                            double ey = fold1((double)y * g.step[1] - xyz[3 * a + 1], g.full[1]);
-                            double ez = fold1((double)z * g.step[2] - xyz[3 * a + 2], g.full[2]);
-                            const Near o = near_lj64(tab.A[t], tab.B[t], tab.h2[t], tab.l2[t], ex, ey, ez);
-                            // no early exit from the tile: a failed node is VERY_LRG_NRG and its
-                            // derivative columns (unlike the FP64 kernel's) sum every other pair
-                            failed = failed || !o.ok;
-                            if (o.ok) {
-                                S0 += o.v[0];
-                                for (int i = 0; i < 7; i++) s_acc[i][tid] += o.v[i + 1];
+                        // FP32 pairs: r2safe <= r2 <= hicut^2 (1 - 1e-6); the
+                        // near pairs and the band around hicut go to FP64
+                        const float hhi = s_lj[j].e.x;
+                        const bool in32 = (r2 <= a4.w) & !(r2 < r2safe);
+                        const bool near = !in32 & (r2 <= hhi);
+                        if (__builtin_expect(__any(near), 0)) {
+                            if (near) {
+                                const int a = s_a[j];
+                                const int t = typ[a];
+                                // node position recomputed here (not kept live in the loop)
+                                const double ex = fold1((double)x * g.step[0] - xyz[3 * a], g.full[0]);
+                                const double ey = fold1((double)y * g.step[1] - xyz[3 * a + 1], g.full[1]);
+                                const double ez = fold1((double)z * g.step[2] - xyz[3 * a + 2], g.full[2]);
+                                double pot = 0.0, h[7] = {0, 0, 0, 0, 0, 0, 0};
+                                // no early exit from the tile: a failed node is VERY_LRG_NRG and its
+                                // derivative columns (unlike the FP64 kernel's) sum every other pair
+                                if (lj_pair64(tab.A[t], tab.B[t], tab.h2[t], tab.l2[t], ex, ey, ez, pot, h)) {
+                                    S[0] += pot;
+#pragma unroll
+                                    for (int i = 0; i < 7; i++) dsum(i, h[i]);
+                                } else {
+                                    failed = true;
+                                }
                             }
-                            continue;
                         }
-                        lj_pair32(a4.w, b4.w, sx, sy, sz, r2, pf, hf);
+                        // w = 1/r^2, u = w^3: V = g0 u, first derivatives
+                        // g1 w^4, second g2 w^5, third g3 u^2
+                        const LJRec rc = s_lj[j];
+                        const float w = in32 ? rcp_fast(r2) : 0.0f;
+                        const float w2 = w * w;
+                        const float u = w2 * w;
+                        const float w4 = w2 * w2;
+                        const float g0 = fmaf(rc.c1.x, u, rc.c1.y);
+                        const float g1 = fmaf(rc.c1.z, u, rc.c1.w);
+                        const float g2 = fmaf(rc.c2.x, u, rc.c2.y);
+                        const float g3 = fmaf(rc.c2.z, u, rc.c2.w);
+                        hf[0] = fmaf(g0, u, hf[0]);
+                        const float f1 = g1 * w4;
+                        hf[1] = fmaf(-sx, f1, hf[1]);
+                        hf[2] = fmaf(-sy, f1, hf[2]);
+                        hf[3] = fmaf(-sz, f1, hf[3]);
+                        const float f2 = g2 * (w4 * w);
+                        const float sxy = sx * sy;
+                        hf[4] = fmaf(f2, sxy, hf[4]);
+                        const float f2z = f2 * sz;
+                        hf[5] = fmaf(f2z, sx, hf[5]);
+                        hf[6] = fmaf(f2z, sy, hf[6]);
+                        const float f3 = g3 * (u * u);
+                        hf[7] = fmaf(f3 * sxy, sz, hf[7]);
                     } else {
-                        if (r2 > g.rc2f) continue;
-                        if (r2 < r2safe) {
-                            int a = s_a[j];
+                        if (r2 > g.rc2f) return;
+                        // pairs within sqrt(r2safe): every term in FP64; within
+                        // sqrt(r2pot): the potential term in FP64 (the largest
+                        // terms, whose FP32 rounding dominates V), the
+                        // derivatives in FP32; beyond: FP32
+                        const bool full64 = r2 < r2safe;
+                        const float u = ewald_real_acc32(ewc, full64 ? 0.0f : a4.w, sx, sy, sz, r2, hf + 1);
+                        if (r2 < r2pot) {
+                            const int a = s_a[j];
                             // node position recomputed here (not kept live in the loop)
-                            double ex = fold1((double)x * g.step[0] - xyz[3 * a], g.full[0]);
-                            double ey = fold1((double)y * g.step[1] - xyz[3 * a + 1], g.full[1]);
-                            double ez = fold1((double)z * g.step[2] - xyz[3 * a + 2], g.full[2]);
-                            const Near o = near_ewald64(g.kappa, g.charge1, qv[a], ex, ey, ez);
-                            // no early exit from the tile: a failed node is VERY_LRG_NRG and its
-                            // derivative columns (unlike the FP64 kernel's) sum every other pair
-                            failed = failed || !o.ok;
-                            if (o.ok) {
-                                S0 += o.v[0];
-                                for (int i = 0; i < 7; i++) s_acc[i][tid] += o.v[i + 1];
+                            const double ex = fold1((double)x * g.step[0] - xyz[3 * a], g.full[0]);
+                            const double ey = fold1((double)y * g.step[1] - xyz[3 * a + 1], g.full[1]);
+                            const double ez = fold1((double)z * g.step[2] - xyz[3 * a + 2], g.full[2]);
+                            const double d2 = ex * ex + ey * ey + ez * ez;
+                            // no early exit from the tile: a failed node is VERY_LRG_NRG
+                            if (d2 < kLowCut2) {
+                                failed = true;
+                            } else if (full64) {
+                                S[0] += ewald_near64(g.kappa, g.charge1 * qv[a], ex, ey, ez, d2, dsum);
+                            } else {
+                                S[0] += ewald_real_pot64(g.kappa, g.charge1, qv[a], d2);
                             }
-                            continue;
+                        } else {
+                            S[0] += (double)u;
                         }
-                        ewald_real_acc32(ewc, a4.w, sx, sy, sz, r2, S0, hf);
                     }
-                }
-                if (KIND == K_LJ) S0 += (double)pf;
-                for (int i = 0; i < 7; i++) s_acc[i][tid] += (double)hf[i];
+                                };
+                // the refold test is workgroup-uniform: one loop per case
+                auto tile = [&](auto refold_c) {
+                    int j = j0;
+#pragma unroll 1
+                    for (; j + 2 <= j1; j += 2) {
+                        pair(j, refold_c);
+                        pair(j + 1, refold_c);
+                    }
+                    if (j < j1) pair(j, refold_c);
+                };
+                if (refold) tile(std::true_type());
+                else tile(std::false_type());
+                if (KIND == K_LJ) S[0] += (double)hf[0];
+#pragma unroll
+                for (int i = 1; i < 8; i++) dsum(i - 1, (double)hf[i]);
             }
         }
         __syncthreads();
     }
     if (!valid) return;
     const long long m = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
-    double o[8], S[8];
-    S[0] = S0;
-    for (int i = 0; i < 7; i++) S[i + 1] = s_acc[i][tid];
+    double o[8];
+    if (KIND == K_EWALD)
+        for (int i = 0; i < 7; i++) S[i + 1] = s_acc[i][tid];
     if (KIND == K_EWALD) {
         if (failed) {
             o[0] = kVeryLrg;
@@ -1211,6 +1286,7 @@ static int run_nodes(mh_ctx *ctx, int kind, int precision, double probe_q, const
     g.kappa = ctx->ew.kappa;
     g.charge1 = probe_q;
     g.r2safe = 0.0;
+    g.r2pot = 0.0;
     g.rc2f = 0.0f;
     g.refold = 0;
     g.slanted = ctx->slanted ? 1 : 0;
@@ -1228,10 +1304,12 @@ static int run_nodes(mh_ctx *ctx, int kind, int precision, double probe_q, const
             double rc_real = real_space_cutoff(ctx->ew.kappa);
             g.cull2 = rc_real * rc_real * (1.0 + 1e-6);
             g.rc2f = (float)(rc_real * rc_real);
-            // pair terms within 5 A are evaluated in FP64: with the FP32
-            // erfc / exp / reciprocal each near 1 ulp, the FP32 sum of the
-            // remaining terms keeps V within 1e-5 of max(|V|, 1 kcal/mol)
-            g.r2safe = 25.0;
+            // pairs within 3 A are evaluated in FP64 and the potential
+            // term of pairs within 5 A: with the FP32 erfc / exp /
+            // reciprocal each near 1 ulp, the FP32 sum of the remaining
+            // terms keeps V within 1e-5 of max(|V|, 1 kcal/mol)
+            g.r2safe = 9.0;
+            g.r2pot = 25.0;
         } else {
             g.cull2 = 1e300;  // the reference sums every atom (ewald.F90:874)
         }

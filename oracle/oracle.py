@@ -30,7 +30,8 @@ def lib(native=False):
         return _LIB
     name = "liboracle_native.so" if native else "liboracle.so"
     path = os.path.join(HERE, name)
-    if not os.path.exists(path):
+    srcs = [os.path.join(HERE, f) for f in os.listdir(HERE) if f.endswith((".c", ".h"))]
+    if not os.path.exists(path) or os.path.getmtime(path) < max(os.path.getmtime(f) for f in srcs):
         subprocess.check_call(["make", "-s", "-C", HERE, "native" if native else "all"])
     L = ctypes.CDLL(path)
     L.mo_rranf_n.argtypes = [ctypes.c_int32, _i, _vp, _vp, _vp]
@@ -384,7 +385,7 @@ class GcmcSys(ctypes.Structure):
                 ("nmoves", _i), ("move_type", _i * 4), ("cumwt", _d * 4),
                 ("insdelratio", _d), ("dtrans", _d), ("drot", _d), ("max_nrg", _d), ("bias", _vp),
                 ("coul", _i), ("coul_on", _i * 64), ("coul_hicut", _d), ("coul_locut", _d),
-                ("coul_shield", _d), ("adjust_trans", _i), ("adjust_rot", _i)]
+                ("coul_shield", _d), ("adjust_trans", _i), ("adjust_rot", _i), ("cell", _vp)]
 
 
 class GcmcTrace(ctypes.Structure):
@@ -475,7 +476,7 @@ class Gcmc:
     maps: list of (npnts, step, ptr, vals8) and site_pmap/site_emap,
     ewald (kappa, kmax, rc) or coul (on [nst*nst], hicut, locut, shield), T,
     moves [(type, weight)], dtrans, drot, and for BINSERT / BDELETE bias (a
-    Bmap)."""
+    Bmap); a slanted simulation cell as 'cell' (from cell())."""
 
     def __init__(self, spec, fugacity_kPa, seed, maxN=2048, trace=0):
         L = _gcmc_lib()
@@ -541,6 +542,8 @@ class Gcmc:
         # BINSERT / BDELETE: spec["bias"] is a Bmap
         self.bias = spec.get("bias")
         s.bias = ctypes.cast(ctypes.byref(self.bias.b), _vp) if self.bias is not None else None
+        self.cell = spec.get("cell")
+        s.cell = ctypes.cast(ctypes.byref(self.cell), _vp) if self.cell is not None else None
         self.sys = s
         st = GcmcState()
         st.maxN = maxN
