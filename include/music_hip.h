@@ -10,8 +10,9 @@
  * thread at a time.
  *
  * Units: Angstrom, kcal/mol (maps and energies), elementary charge, K.
- * Cells are orthorhombic with the origin at 0 (the reference's fast path,
- * fundcell.F90:640-709); other cells are rejected with MH_EINVAL.
+ * Cells have their origin at 0 and are orthorhombic (the reference's fast
+ * path, fundcell.F90:640-709) unless mh_framework_set_cell gives a slanted
+ * one; the calls that do not handle slanted cells reject them with MH_EINVAL.
  * Indices passed in are 0-based; Fortran callers subtract 1.
  */
 #ifndef MUSIC_HIP_H
@@ -85,8 +86,12 @@ int mh_ff_set_buck(mh_ctx *ctx, int ntypes, const double *A, const double *rho,
  * node kernel serves both precisions there -- and map interpolation folds
  * points into the slanted unit cell without the periodic wrap of the upper
  * corner (simcell_maptouc, simcell.F90:634-668; interpolate.F90:675-679).
- * GCMC and HMC batches refuse slanted cells.  mh_framework_upload resets the
- * cell to orthorhombic with eff = fund_ell. */
+ * GCMC batches (mh_gcmc_create) in a slanted cell insert along its lattice
+ * vectors (simcell_uniformRandPt, simcell.F90:274-295), fold sites and pair
+ * separations in the slanted basis and take the cell volume as (l1 x l2).l3
+ * for the Adams B; they refuse sorbate Ewald sums and biased insertion
+ * there.  Slot pools and HMC batches refuse slanted cells.
+ * mh_framework_upload resets the cell to orthorhombic with eff = fund_ell. */
 int mh_framework_set_cell(mh_ctx *ctx, const double angles_deg[3], const double eff[3]);
 
 /* Ewald parameters and framework structure factor (ewald_init,

@@ -169,6 +169,7 @@ int mh_framework_upload(mh_ctx *ctx, int natoms, const double *xyz, const int *t
         ctx->fund_ell[i] = fund_ell[i];
         ctx->full_ell[i] = full_ell[i];
         ctx->eff[i] = fund_ell[i];
+        ctx->ang_deg[i] = 90.0;
     }
     ctx->slanted = false;
     ctx->ew.set = false;
@@ -261,7 +262,10 @@ int mh_framework_set_cell(mh_ctx *ctx, const double angles_deg[3], const double 
     // fundcell_init (fundcell.F90:60-112): orthorhombic within 0.001 degree
     const double err = std::fabs(angles_deg[0] - 90.0) + std::fabs(angles_deg[1] - 90.0) +
                        std::fabs(angles_deg[2] - 90.0);
-    for (int i = 0; i < 3; i++) ctx->eff[i] = eff[i];
+    for (int i = 0; i < 3; i++) {
+        ctx->eff[i] = eff[i];
+        ctx->ang_deg[i] = angles_deg[i];
+    }
     ctx->slanted = err > 0.001;
     if (!ctx->slanted) return MH_OK;
     // fundcell_initxform (fundcell.F90:378-412): the angles in default real,

@@ -73,6 +73,11 @@ struct GDev {
     // e^{i (kx x + ky y)} table per (set, site) in dynamic LDS at byte
     // offset exy_off: [2][ns][kmax + 1][2 kmax + 1] (double2, float2 in FP32)
     int use_exy, exy_off;
+    // slanted simulation cell: unslant U / slant S matrices (fundcell.F90:
+    // 378-412) and the simulation cell's lattice vectors (insertion points)
+    int slanted;
+    SlantCell sc;
+    double lvec[3][3];
 };
 
 // rigidmoves_adjustdeltatrans / _adjustdeltarot (rigidmoves.F90:2586-2617):
@@ -135,8 +140,55 @@ __device__ __forceinline__ double site_coord(const GDev &P, int a, int i, const 
     return wrap_full(__dadd_rn(com[i], d), P.full[i]);
 }
 
+// simcell_pbc of one position in a slanted cell (fundcell.F90:640-709 basis):
+// slanted coordinates folded into [0, ell), back to Cartesian
+__device__ __forceinline__ void slant_pbc(const GDev &P, double r[3]) {
+    double v[3];
+    mat_vec3(P.sc.S, r, v);
+    for (int i = 0; i < 3; i++) v[i] = wrap_full(v[i], P.full[i]);
+    mat_vec3(P.sc.U, v, r);
+}
+
+// simcell_minimage of a site pair (fundcell_snglMinImage, fundcell.F90:
+// 582-623): sep = a - b, folded once in the slanted basis when slanted
+__device__ __forceinline__ void min_image(const GDev &P, const double *a, const double *b, double sep[3],
+                                          bool slanted) {
+    if (!slanted) {
+        for (int i = 0; i < 3; i++) sep[i] = fold64(a[i] - b[i], P.full[i]);
+        return;
+    }
+    double va[3], vb[3], d[3];
+    mat_vec3(P.sc.S, a, va);
+    mat_vec3(P.sc.S, b, vb);
+    for (int i = 0; i < 3; i++) d[i] = fold64(va[i] - vb[i], P.full[i]);
+    mat_vec3(P.sc.U, d, sep);
+}
+
 // all site coordinates of one molecule by one lane (host-side paths)
 __device__ void site_coords(const GDev &P, const double com[3], const double ang[3], double *out) {
+    if (P.slanted) {
+        if (P.ns == 1) {
+            for (int i = 0; i < 3; i++) out[i] = com[i];
+        } else {
+            double sp, cp, st, ct, ss, cs;
+            sincos(ang[0], &sp, &cp);
+            sincos(ang[1], &st, &ct);
+            sincos(ang[2], &ss, &cs);
+            for (int i = 0; i < 3; i++) {
+                double m[3];
+                euler_row(i, sp, cp, st, ct, ss, cs, m);
+                for (int a = 0; a < P.ns; a++) {
+                    double d = 0.0;
+                    d = __dadd_rn(d, __dmul_rn(m[0], P.ref[a][0]));
+                    d = __dadd_rn(d, __dmul_rn(m[1], P.ref[a][1]));
+                    d = __dadd_rn(d, __dmul_rn(m[2], P.ref[a][2]));
+                    out[3 * a + i] = __dadd_rn(com[i], d);
+                }
+            }
+        }
+        for (int a = 0; a < P.ns; a++) slant_pbc(P, out + 3 * a);
+        return;
+    }
     if (P.ns == 1) {
         for (int i = 0; i < 3; i++) out[i] = wrap_full(com[i], P.full[i]);
         return;
@@ -157,11 +209,32 @@ __device__ void site_coords(const GDev &P, const double com[3], const double ang
 // Hermite contribution of one cell corner: sum over the 8 stored values of
 // f[s] hx hy hz (FP64 products in the reference order, or FP32).
 template <int PREC>
-__device__ __forceinline__ bool corner_term(const MapRef &m, const double *r, int corner, double &part) {
+__device__ __forceinline__ bool corner_term(const MapRef &m, const double *r, int corner, double &part,
+                                            const SlantCell *sc = nullptr) {
     int i0[3], i1[3];
     double t[3];
-    for (int d = 0; d < 3; d++)
-        if (!cell_axis(r[d], m.ell[d], m.step[d], m.n[d], i0[d], i1[d], t[d])) return false;
+    if (sc) {
+        // simcell_maptouc in the slanted basis, then the node cell without
+        // the periodic wrap of the upper corner (interpolate.F90:675-679)
+        double v[3], c[3];
+        mat_vec3(sc->S, r, v);
+        for (int d = 0; d < 3; d++) {
+            double f = __dmul_rn(v[d], 1.0 / m.ell[d]);
+            f = (f >= 0.0) ? fmod(f, 1.0) : __dadd_rn(fmod(f, 1.0), 1.0);
+            v[d] = __dmul_rn(f, m.ell[d]);
+        }
+        mat_vec3(sc->U, v, c);
+        for (int d = 0; d < 3; d++) {
+            const int idx = (int)(c[d] / m.step[d] + 1.0);
+            if (idx < 1 || idx >= m.n[d]) return false;
+            t[d] = (c[d] - (double)(idx - 1) * m.step[d]) / m.step[d];
+            i0[d] = idx - 1;
+            i1[d] = idx;
+        }
+    } else {
+        for (int d = 0; d < 3; d++)
+            if (!cell_axis(r[d], m.ell[d], m.step[d], m.n[d], i0[d], i1[d], t[d])) return false;
+    }
     const int cx = (kCX >> corner) & 1, cy = (kCY >> corner) & 1, cz = (kCZ >> corner) & 1;
     const long long node = ((long long)(cx ? i1[0] : i0[0]) * m.n[1] + (cy ? i1[1] : i0[1])) * m.n[2] +
                            (cz ? i1[2] : i0[2]);
@@ -214,7 +287,7 @@ __device__ bool site_map_energy(const GDev &P, int a, const double *r, double &e
         double s = 0.0;
         for (int c = 0; c < 8; c++) {
             double part;
-            if (!corner_term<MH_PREC_FP64>(m, r, c, part)) return false;
+            if (!corner_term<MH_PREC_FP64>(m, r, c, part, P.slanted ? &P.sc : nullptr)) return false;
             s = __dadd_rn(s, part);
         }
         (w == 0 ? pm : em) = s;
@@ -364,7 +437,7 @@ __device__ __forceinline__ double2 k_delta(const GDev &P, const Shared &sh, int 
 // every other molecule (skip = its slot) and the reciprocal-space delta,
 // whose dS(k) is left in K.d (registers) or dSg (KPL == 0).
 // out: map A, pairs A, map B, pairs B, reciprocal delta, ok(A).
-template <int NT, int PREC, int KPL>
+template <int NT, int PREC, int KPL, bool SL = false>
 __device__ __forceinline__ void move_energy(const GDev &P, Shared &sh, const double *__restrict__ pos, int N,
                                             int skip, bool hasA, bool hasB, double asign, KReg<KPL> &K,
                                             const double2 *__restrict__ Sg, double2 *__restrict__ dSg,
@@ -380,8 +453,9 @@ __device__ __forceinline__ void move_energy(const GDev &P, Shared &sh, const dou
         double pp = 0.0, pe = 0.0;
         int ok = 1;
         if (present) {
-            if (P.has_p[a] && !corner_term<PREC>(P.pm[a], r, corner, pp)) ok = 0;
-            if (P.has_e[a] && !corner_term<PREC>(P.em[a], r, corner, pe)) ok = 0;
+            const SlantCell *sc = SL ? &P.sc : nullptr;
+            if (P.has_p[a] && !corner_term<PREC>(P.pm[a], r, corner, pp, sc)) ok = 0;
+            if (P.has_e[a] && !corner_term<PREC>(P.em[a], r, corner, pe, sc)) ok = 0;
         }
         for (int off = 1; off < 8; off <<= 1) {
             pp += __shfl_xor(pp, off);
@@ -451,34 +525,32 @@ __device__ __forceinline__ void move_energy(const GDev &P, Shared &sh, const dou
         const double *pj = pos + ((long long)j * ns + b) * 3;
         const double xb = pj[0], yb = pj[1], zb = pj[2];
         const int t = P.st[a] * P.nst + P.st[b];
+        const double pb[3] = {xb, yb, zb};
         if (PREC == MH_PREC_FP64) {
             if (hasA) {
-                double sx = fold64(sh.posA[3 * a] - xb, P.full[0]);
-                double sy = fold64(sh.posA[3 * a + 1] - yb, P.full[1]);
-                double sz = fold64(sh.posA[3 * a + 2] - zb, P.full[2]);
-                if (!pair_energy64(P, t, P.q[a], P.q[b], sx, sy, sz, eA)) failA = 1.0;
+                double sp[3];
+                min_image(P, &sh.posA[3 * a], pb, sp, SL);
+                if (!pair_energy64(P, t, P.q[a], P.q[b], sp[0], sp[1], sp[2], eA)) failA = 1.0;
             }
             if (hasB) {
-                double sx = fold64(sh.posB[3 * a] - xb, P.full[0]);
-                double sy = fold64(sh.posB[3 * a + 1] - yb, P.full[1]);
-                double sz = fold64(sh.posB[3 * a + 2] - zb, P.full[2]);
-                pair_energy64(P, t, P.q[a], P.q[b], sx, sy, sz, eB);
+                double sp[3];
+                min_image(P, &sh.posB[3 * a], pb, sp, SL);
+                pair_energy64(P, t, P.q[a], P.q[b], sp[0], sp[1], sp[2], eB);
             }
         } else {
             const float qq = P.ewald ? (float)(P.q[a] * P.q[b] * kTokcalG) : 0.0f;
             if (hasA) {
-                const double dx = fold64(sh.posA[3 * a] - xb, P.full[0]);
-                const double dy = fold64(sh.posA[3 * a + 1] - yb, P.full[1]);
-                const double dz = fold64(sh.posA[3 * a + 2] - zb, P.full[2]);
-                if (!pair_energy32(P, t, qq, kap32, rc32, (float)dx, (float)dy, (float)dz, eAf)) failA = 1.0;
-                if (P.coul && !coul_energy64(P, t, P.q[a], P.q[b], dx * dx + dy * dy + dz * dz, eA)) failA = 1.0;
+                double d[3];
+                min_image(P, &sh.posA[3 * a], pb, d, SL);
+                if (!pair_energy32(P, t, qq, kap32, rc32, (float)d[0], (float)d[1], (float)d[2], eAf)) failA = 1.0;
+                if (P.coul && !coul_energy64(P, t, P.q[a], P.q[b], d[0] * d[0] + d[1] * d[1] + d[2] * d[2], eA))
+                    failA = 1.0;
             }
             if (hasB) {
-                const double dx = fold64(sh.posB[3 * a] - xb, P.full[0]);
-                const double dy = fold64(sh.posB[3 * a + 1] - yb, P.full[1]);
-                const double dz = fold64(sh.posB[3 * a + 2] - zb, P.full[2]);
-                pair_energy32(P, t, qq, kap32, rc32, (float)dx, (float)dy, (float)dz, eBf);
-                if (P.coul) coul_energy64(P, t, P.q[a], P.q[b], dx * dx + dy * dy + dz * dz, eB);
+                double d[3];
+                min_image(P, &sh.posB[3 * a], pb, d, SL);
+                pair_energy32(P, t, qq, kap32, rc32, (float)d[0], (float)d[1], (float)d[2], eBf);
+                if (P.coul) coul_energy64(P, t, P.q[a], P.q[b], d[0] * d[0] + d[1] * d[1] + d[2] * d[2], eB);
             }
         }
     }
@@ -627,8 +699,12 @@ __device__ __forceinline__ void sk_accept(const GDev &P, KReg<KPL> &K, double2 *
 
 // The persistent GCMC loop.  LDSMOL: molecules held in dynamic LDS
 // ([maxN][ns][3] sites, [maxN][3] com, [maxN][3] angles, FP64).
-template <int NT, int PREC, int KPL, bool LDSMOL>
+// KPLX >= 0: k vectors per lane in registers (orthorhombic cell); KPLX < 0:
+// slanted cell (no sorbate Ewald sum).
+template <int NT, int PREC, int KPLX, bool LDSMOL>
 __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep R, int nsteps) {
+    constexpr int KPL = KPLX > 0 ? KPLX : 0;
+    constexpr bool SL = KPLX < 0;
     extern __shared__ double dyn[];
     __shared__ Shared sh;
     __shared__ GDev P;
@@ -724,7 +800,18 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                 break;
             }
             if (!biased) {
-                for (int i = 0; i < 3; i++) tcom[i] = __dmul_rn(P.full[i], rranf(rng));
+                if constexpr (SL) {
+                    // simcell_uniformRandPt (simcell.F90:274-295): one draw
+                    // per lattice vector, summed in order
+                    double v[3][3];
+                    for (int k = 0; k < 3; k++) {
+                        const double rk = rranf(rng);
+                        for (int i = 0; i < 3; i++) v[k][i] = __dmul_rn(P.lvec[k][i], rk);
+                    }
+                    for (int i = 0; i < 3; i++) tcom[i] = __dadd_rn(__dadd_rn(v[0][i], v[1][i]), v[2][i]);
+                } else {
+                    for (int i = 0; i < 3; i++) tcom[i] = __dmul_rn(P.full[i], rranf(rng));
+                }
             } else {
                 // rigidmoves_comFromBmap: a cubelet by weight, a point in it
                 // (bmap_getbiaspt), a unit cell of the simulation cell
@@ -805,6 +892,8 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
             if (hasB && tid >= 64 - n3 && tid < 64) sh.posB[tid - (64 - n3)] = pos[(long long)mol * n3 + tid - (64 - n3)];
             if (mbase == MH_MV_DELETE) {
                 if (tid < n3) sh.posA[tid] = pos[(long long)mol * n3 + tid];
+            } else if (SL) {
+                if (tid == 0) site_coords(P, tcom, tang, sh.posA);
             } else if (ns == 1) {
                 if (tid < 3) sh.posA[tid] = wrap_full(tcom[tid], P.full[tid]);
             } else {
@@ -826,9 +915,9 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
             double o[6];
             if (mbase == MH_MV_DELETE) {
                 // the molecule's energy in the current system: its sites as set A
-                move_energy<NT, PREC, KPL>(P, sh, pos, N, mol, true, false, -1.0, K, S, dS, exy, o);
+                move_energy<NT, PREC, KPL, SL>(P, sh, pos, N, mol, true, false, -1.0, K, S, dS, exy, o);
             } else {
-                move_energy<NT, PREC, KPL>(P, sh, pos, N, mol, hasA, hasB, 1.0, K, S, dS, exy, o);
+                move_energy<NT, PREC, KPL, SL>(P, sh, pos, N, mol, hasA, hasB, 1.0, K, S, dS, exy, o);
             }
             const double mA = o[0], pA = o[1], mB = o[2], pB = o[3], eK = o[4];
             bool ok = o[5] != 0.0;
@@ -989,7 +1078,11 @@ __global__ void __launch_bounds__(NT) k_trial(const GDev *__restrict__ Ptab, Rep
     __syncthreads();
     double o[6];
     KReg<0> K;
-    move_energy<NT, PREC, 0>(P, sh, pos, N, -1, true, false, 1.0, K, S, dSbuf + (long long)t * P.nk, nullptr, o);
+    if (P.slanted)
+        move_energy<NT, PREC, 0, true>(P, sh, pos, N, -1, true, false, 1.0, K, S, dSbuf + (long long)t * P.nk,
+                                       nullptr, o);
+    else
+        move_energy<NT, PREC, 0>(P, sh, pos, N, -1, true, false, 1.0, K, S, dSbuf + (long long)t * P.nk, nullptr, o);
     if (tid == 0) {
         double U = __dadd_rn(o[0], o[1]);
         if (P.ewald) U = U + o[4] + P.molconst;
@@ -1055,10 +1148,10 @@ __global__ void __launch_bounds__(NT) k_total(const GDev *__restrict__ Ptab, Rep
             const double *pj = pos + (long long)j * ns * 3;
             for (int a = 0; a < ns; a++)
                 for (int b = 0; b < ns; b++) {
-                    double sx = fold64(pm[3 * a] - pj[3 * b], P.full[0]);
-                    double sy = fold64(pm[3 * a + 1] - pj[3 * b + 1], P.full[1]);
-                    double sz = fold64(pm[3 * a + 2] - pj[3 * b + 2], P.full[2]);
-                    if (!pair_energy64(P, P.st[a] * P.nst + P.st[b], P.q[a], P.q[b], sx, sy, sz, e)) bad = 1.0;
+                    double sp[3];
+                    min_image(P, pm + 3 * a, pj + 3 * b, sp, P.slanted != 0);
+                    if (!pair_energy64(P, P.st[a] * P.nst + P.st[b], P.q[a], P.q[b], sp[0], sp[1], sp[2], e))
+                        bad = 1.0;
                 }
         }
     }
@@ -1113,6 +1206,7 @@ struct mh_gcmc {
     long long kstride = 0;  // S(k) entries per replica
     int kpl = 0;            // k vectors per lane held in registers (0: S(k) in HBM)
     bool ldsmol = false;    // molecules held in LDS during a launch
+    bool slanted = false;   // single system in a slanted cell
     size_t lds_bytes = 0;   // dynamic LDS of the launch
     bool exy_off = false;   // autotuner: e^{i(kx x + ky y)} tables disabled
     bool pool = false;      // slot pool (per-replica system and step budget)
@@ -1210,6 +1304,11 @@ static void launch_lds(mh_gcmc *g, const GDev *P, const Rep &r, int nsteps) {
 
 template <int PREC, int NT>
 static void launch_nt(mh_gcmc *g, const GDev *P, const Rep &r, int nsteps) {
+    if (g->slanted) {  // slanted cell: its own instantiation (no k vectors)
+        if (g->ldsmol) launch_lds<PREC, NT, -1>(g, P, r, nsteps);
+        else hipLaunchKernelGGL((k_gcmc<NT, PREC, -1, false>), dim3(g->nrep), dim3(NT), 0, g->ctx->stream, P, r, nsteps);
+        return;
+    }
     if (!g->ldsmol) {
         hipLaunchKernelGGL((k_gcmc<NT, PREC, 0, false>), dim3(g->nrep), dim3(NT), 0, g->ctx->stream, P, r, nsteps);
         return;
@@ -1301,7 +1400,7 @@ static int plan(mh_gcmc *g) {
 
 // Sorbate model, maps, cell and k vectors of one system.
 static int build_system(mh_ctx *ctx, const mh_gcmc_params *p, int maxN, const double fund[3], const double full[3],
-                        SysHost &S, const char *who) {
+                        SysHost &S, const char *who, bool ctx_cell) {
     std::string w(who);
     MH_CHECK(p->nsites >= 1 && p->nsites <= GS, MH_EINVAL, w + ": nsites out of range [1,8]");
     MH_CHECK(p->ntypes >= 1 && p->ntypes <= 8, MH_EINVAL, w + ": ntypes out of range [1,8]");
@@ -1370,7 +1469,42 @@ static int build_system(mh_ctx *ctx, const mh_gcmc_params *p, int maxN, const do
     // k vectors of the half space (mx > 0, or mx = 0 & my > 0, or mx = my = 0 & mz > 0)
     std::vector<double> wv;
     std::vector<int8_t> ki;
-    const double V = H.full[0] * H.full[1] * H.full[2];
+    double V = H.full[0] * H.full[1] * H.full[2];
+    if (ctx_cell && ctx->slanted) {
+        // the simulation cell's lattice vectors (fundcell_genlatticevects,
+        // fundcell.F90:174-208, double trigonometry of the angles) and its
+        // volume (fundcell_genvolume: (l1 x l2) . l3, vector.F90:405-429)
+        MH_CHECK(!p->ewald, MH_EINVAL, w + ": sorbate Ewald sums are not offered in slanted cells");
+        MH_CHECK(!p->bias, MH_EINVAL, w + ": biased insertion is not offered in slanted cells");
+        H.slanted = 1;
+        for (int i = 0; i < 9; i++) {
+            H.sc.U[i] = ctx->U[i];
+            H.sc.S[i] = ctx->S[i];
+        }
+        const double degtorad = 3.1415927 / 180.0;
+        const double abc = ctx->ang_deg[0] * degtorad, aac = ctx->ang_deg[1] * degtorad,
+                     aab = ctx->ang_deg[2] * degtorad;
+        double ell[3];
+        for (int i = 0; i < 3; i++) ell[i] = fund[i] * (double)std::lround(full[i] / fund[i]);
+        double (*L)[3] = H.lvec;
+        L[0][0] = ell[0];
+        L[0][1] = 0.0;
+        L[0][2] = 0.0;
+        L[1][0] = ell[1] * std::cos(aab);
+        L[1][1] = ell[1] * std::sin(aab);
+        L[1][2] = 0.0;
+        L[2][0] = ell[2] * std::cos(aac);
+        L[2][1] = ell[2] * (std::cos(abc) - std::cos(aac) * std::cos(aab)) / std::sin(aab);
+        double m2 = 0.0;
+        m2 = m2 + L[2][0] * L[2][0];
+        m2 = m2 + L[2][1] * L[2][1];
+        m2 = m2 + 0.0 * 0.0;
+        L[2][2] = std::sqrt(ell[2] * ell[2] - m2);
+        const double cx = L[0][1] * L[1][2] - L[0][2] * L[1][1];
+        const double cy = -(L[0][0] * L[1][2] - L[0][2] * L[1][0]);
+        const double cz = L[0][0] * L[1][1] - L[0][1] * L[1][0];
+        V = cx * L[2][0] + cy * L[2][1] + cz * L[2][2];
+    }
     S.volume = V;
     if (H.ewald) {
         int K = H.kmax;
@@ -1515,10 +1649,10 @@ int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double 
     MH_CHECK(maxN >= 1 && maxN <= (1 << 20), MH_EINVAL, "mh_gcmc_create: maxN out of range");
     MH_CHECK(p->precision == MH_PREC_FP64 || p->precision == MH_PREC_FP32, MH_EINVAL,
              "mh_gcmc_create: bad precision");
-    MH_CHECK(!ctx->slanted, MH_EINVAL, "mh_gcmc_create: slanted cells are not offered for GCMC");
     for (int a = 0; a < p->nsites && a < MH_MAX_SITES; a++)
-        MH_CHECK((!p->site_pmap[a] || !p->site_pmap[a]->slanted) && (!p->site_emap[a] || !p->site_emap[a]->slanted),
-                 MH_EINVAL, "mh_gcmc_create: maps of slanted cells are not offered for GCMC");
+        MH_CHECK((!p->site_pmap[a] || p->site_pmap[a]->slanted == ctx->slanted) &&
+                     (!p->site_emap[a] || p->site_emap[a]->slanted == ctx->slanted),
+                 MH_EINVAL, "mh_gcmc_create: the maps were built for another cell shape than the context's");
     MH_HIP(hipSetDevice(ctx->device));
     std::unique_ptr<mh_gcmc> g(new mh_gcmc());
     g->ctx = ctx;
@@ -1527,8 +1661,9 @@ int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double 
     g->prec = p->precision;
     g->sys.resize(1);
     int rc;
-    if ((rc = build_system(ctx, p, maxN, ctx->fund_ell, ctx->full_ell, g->sys[0], "mh_gcmc_create"))) return rc;
+    if ((rc = build_system(ctx, p, maxN, ctx->fund_ell, ctx->full_ell, g->sys[0], "mh_gcmc_create", true))) return rc;
     g->ns = p->nsites;
+    g->slanted = g->sys[0].H.slanted != 0;
     g->nk = g->sys[0].H.nk;
     g->kstride = g->nk;
     if ((rc = g->dP.alloc(sizeof(GDev)))) return rc;
@@ -1626,7 +1761,7 @@ int mh_pool_admit(mh_gcmc *g, const mh_gcmc_params *p, const double fund_ell[3],
     SysHost &S = g->sys[s];
     mh_gcmc_params q = *p;
     q.precision = g->prec;
-    int rc = build_system(g->ctx, &q, g->maxN, fund_ell, full_ell, S, "mh_pool_admit");
+    int rc = build_system(g->ctx, &q, g->maxN, fund_ell, full_ell, S, "mh_pool_admit", false);
     if (!rc && S.H.nk > g->kstride) {
         set_error("mh_pool_admit: more k vectors than the pool's kmax_cap");
         rc = MH_ECAPACITY;
@@ -1717,6 +1852,24 @@ int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const doub
             if (ang) ha[3 * m + i] = ang[3 * m + i];
         }
         // host restatement of site_coords (rigidcoords_toxyz + simcell_pbc)
+        auto slant_pbc_h = [&](double *r) {
+            double v[3];
+            for (int i = 0; i < 3; i++) {
+                double t = 0.0;
+                for (int c = 0; c < 3; c++) t = t + H.sc.S[3 * i + c] * r[c];
+                v[i] = t - std::floor(t / H.full[i]) * H.full[i];
+            }
+            for (int i = 0; i < 3; i++) {
+                double t = 0.0;
+                for (int c = 0; c < 3; c++) t = t + H.sc.U[3 * i + c] * v[c];
+                r[i] = t;
+            }
+        };
+        if (ns == 1 && H.slanted) {
+            for (int i = 0; i < 3; i++) hp[3 * m + i] = hc[3 * m + i];
+            slant_pbc_h(&hp[3 * m]);
+            continue;
+        }
         if (ns == 1) {
             for (int i = 0; i < 3; i++) {
                 double c = hc[3 * m + i];
@@ -1729,12 +1882,14 @@ int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const doub
         double M[3][3] = {{cs * cp - ct * sp * ss, -ss * cp - ct * sp * cs, st * sp},
                           {cs * sp + ct * cp * ss, -ss * sp + ct * cp * cs, -st * cp},
                           {ss * st, cs * st, ct}};
-        for (int a = 0; a < ns; a++)
+        for (int a = 0; a < ns; a++) {
             for (int i = 0; i < 3; i++) {
                 double d = M[i][0] * H.ref[a][0] + M[i][1] * H.ref[a][1] + M[i][2] * H.ref[a][2];
                 double rp = hc[3 * m + i] + d;
-                hp[(3 * m * ns) + 3 * a + i] = rp - std::floor(rp / H.full[i]) * H.full[i];
+                hp[(3 * m * ns) + 3 * a + i] = H.slanted ? rp : rp - std::floor(rp / H.full[i]) * H.full[i];
             }
+            if (H.slanted) slant_pbc_h(&hp[(3 * m * ns) + 3 * a]);
+        }
     }
     const long long off3 = (long long)rep * g->maxN * 3, offp = (long long)rep * g->maxN * g->ns * 3;
     if (n) {

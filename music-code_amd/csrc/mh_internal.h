@@ -82,6 +82,7 @@ struct mh_ctx {
     bool slanted = false;
     double U[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, S[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     double eff[3] = {0, 0, 0};
+    double ang_deg[3] = {90, 90, 90};  // cell angles alpha, beta, gamma
     mh::DBuf xyz_s; // double [natoms][3] slanted atom coordinates S a
     mh::DBuf xyz;   // double [natoms][3]
     mh::DBuf type;  // int [natoms]

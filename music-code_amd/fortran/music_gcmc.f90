@@ -99,7 +99,6 @@ program music_gcmc
      end if
   end do
   call read_system(ctrl, sy)
-  if (sy%slanted) call fail('slanted cells are not offered by music_gcmc (music_mapmaker builds their maps)')
 
   ! ---- General Information: description, iterations, print interval,
   ! crash and config intervals, first simulation number, seed
@@ -284,6 +283,9 @@ program music_gcmc
      call check(mh_framework_upload(ctx, 1_c_int, sy%xyz, sy%typ, sy%q, sy%fund, sy%full), &
           'mh_framework_upload')
   end if
+  ! a slanted cell: maps fold into it, insertion follows its lattice vectors
+  ! (the library refuses sorbate Ewald sums and biased insertion there)
+  if (sy%slanted) call check(mh_framework_set_cell(ctx, sy%angles, sy%eff), 'mh_framework_set_cell')
   allocate(maps(2 * MH_MAX_SITES), mapnames(2 * MH_MAX_SITES))
   nmaps = 0
   do a = 1, nsite

@@ -105,9 +105,18 @@ int mo_coul_pair(double hicut, double locut, double shield, double r2, double q1
     return 1;
 }
 
+static inline int slanted(const mo_gcmc_sys *s)
+{
+    return s->cell && !s->cell->ortho;
+}
+
 /* simcell_pbcarray (simcell.F90:521-547) */
 static void pbc(const mo_gcmc_sys *s, const double *rp, double *r)
 {
+    if (slanted(s)) {
+        for (int a = 0; a < s->ns; a++) mo_cell_pbc(s->cell, rp + 3 * a, r + 3 * a);
+        return;
+    }
     for (int a = 0; a < s->ns; a++)
         for (int i = 0; i < 3; i++) {
             double c = rp[3 * a + i];
@@ -206,22 +215,26 @@ static int mol_energy(const mo_gcmc_sys *s, const mo_gcmc_state *st, const doubl
                       double *umap, double *upair)
 {
     double um = 0.0;
+    const int sl = slanted(s);
+    /* pmap_int / interpolate_int (interpolate.F90:663-679): slanted cells
+     * interpolate without the periodic wrap of the upper corner */
+    int (*herm)(const int *, const double *, const int32_t *, const float *, long long, const double *,
+                double *) = sl ? mo_pothermite_nowrap : mo_pothermite;
     for (int a = 0; a < s->ns; a++) {
         double c[3];
-        mo_maptouc(s->fund_ell, r + 3 * a, c);
+        if (sl) mo_cell_maptouc(s->cell, r + 3 * a, c);
+        else mo_maptouc(s->fund_ell, r + 3 * a, c);
         int pm = s->site_pmap[a];
         if (pm >= 0) {
             double p;
-            if (!mo_pothermite(s->map_npnts[pm], s->map_step[pm], s->map_ptr[pm], s->map_vals[pm],
-                               s->map_nf[pm], c, &p))
+            if (!herm(s->map_npnts[pm], s->map_step[pm], s->map_ptr[pm], s->map_vals[pm], s->map_nf[pm], c, &p))
                 return 0;
             um = um + p;
         }
         int em = s->site_emap[a];
         if (em >= 0) {
             double p;
-            if (!mo_pothermite(s->map_npnts[em], s->map_step[em], s->map_ptr[em], s->map_vals[em],
-                               s->map_nf[em], c, &p))
+            if (!herm(s->map_npnts[em], s->map_step[em], s->map_ptr[em], s->map_vals[em], s->map_nf[em], c, &p))
                 return 0;
             um = um + p * s->sq[a];
         }
@@ -233,7 +246,8 @@ static int mol_energy(const mo_gcmc_sys *s, const mo_gcmc_state *st, const doubl
         for (int a = 0; a < s->ns; a++)
             for (int b = 0; b < s->ns; b++) {
                 double sep[3];
-                mo_minimage(s->full_ell, r + 3 * a, ro + 3 * b, sep);
+                if (sl) mo_cell_minimage(s->cell, r + 3 * a, ro + 3 * b, sep);
+                else mo_minimage(s->full_ell, r + 3 * a, ro + 3 * b, sep);
                 double r2 = 0.0;
                 r2 = r2 + sep[0] * sep[0];
                 r2 = r2 + sep[1] * sep[1];
@@ -445,7 +459,10 @@ int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
                 double com[3], ang[3] = {0.0, 0.0, 0.0};
                 double biasfactor = 1.0;
                 if (mtype == MO_MV_INSERT) {
-                    for (int i = 0; i < 3; i++) com[i] = s->full_ell[i] * mo_rranf(&st->rng);
+                    /* simcell_uniformRandPt (simcell.F90:274-295) */
+                    if (slanted(s)) mo_cell_uniform(s->cell, &st->rng, com);
+                    else
+                        for (int i = 0; i < 3; i++) com[i] = s->full_ell[i] * mo_rranf(&st->rng);
                 } else {
                     /* rigidmoves_comFromBmap: a cubelet by its weight, a point
                      * in it (bmap_getbiaspt), a unit cell of the simulation cell */

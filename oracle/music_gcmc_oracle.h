@@ -69,6 +69,10 @@ typedef struct {
     /* adjustable displacements (rigidmoves.F90:2586-2617): the state's delta
      * is used and updated after every attempt of the move */
     int adjust_trans, adjust_rot;
+    /* slanted simulation cell (NULL or orthorhombic: full_ell / fund_ell):
+     * insertion by simcell_uniformRandPt, pbc, minimum image and maptouc in
+     * the slanted basis, Hermite without the corner wrap */
+    const mo_cell *cell;
 } mo_gcmc_sys;
 
 typedef struct {

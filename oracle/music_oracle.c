@@ -650,6 +650,22 @@ void mo_cell_uniform(const mo_cell *c, mo_rng *rng, double out[3])
     for (int i = 0; i < 3; i++) out[i] = (v[0][i] + v[1][i]) + v[2][i];
 }
 
+/* simcell_getvolume -> fundcell_genvolume: vector_tripledotprod (vector.F90:
+ * 405-429), the cross product l1 x l2 then its dot product with l3.
+ * Pinned: the reference build's simcell_getvolume on the FCELL cells. */
+double mo_cell_volume(const mo_cell *c)
+{
+    const double *l1 = c->lvec[0], *l2 = c->lvec[1], *l3 = c->lvec[2];
+    double x = l1[1] * l2[2] - l1[2] * l2[1];
+    double y = -(l1[0] * l2[2] - l1[2] * l2[0]);
+    double z = l1[0] * l2[1] - l1[1] * l2[0];
+    double v = 0.0;
+    v = v + x * l3[0];
+    v = v + y * l3[1];
+    v = v + z * l3[2];
+    return v;
+}
+
 /* -------------------------------------------------------- compaction */
 /* pmap.F90:509-656.  Grid node (i,j,k) (1-based x,y,z) has source index
  * m = (i-1) + nx*((j-1) + ny*(k-1)) (pmap_addpoint assigns ptr = running

@@ -142,6 +142,9 @@ void mo_build_nodes(int kind, const mo_framework *fw, const int npnts[3], const 
 int mo_node_at(int kind, const mo_framework *fw, const double p[3], double out[8]);
 /* simcell_uniformRandPt (simcell.F90:268-295) */
 void mo_cell_uniform(const mo_cell *c, mo_rng *rng, double out[3]);
+/* simcell_getvolume (simcell.F90:300-304, fundcell_genvolume fundcell.F90:
+ * 159-165): (l1 x l2) . l3 of the simulation cell's lattice vectors */
+double mo_cell_volume(const mo_cell *c);
 
 /* ---- compaction (pmap.F90:509-656) ----
  * r: FP32 potential per node in grid order (x fastest), cutoff in kcal/mol.

@@ -66,6 +66,8 @@ def lib(native=False):
     L.mo_node_at.argtypes = [_i, _vp, _vp, _vp]
     L.mo_node_at.restype = _i
     L.mo_cell_uniform.argtypes = [_vp, _vp, _vp]
+    L.mo_cell_volume.argtypes = [_vp]
+    L.mo_cell_volume.restype = _d
     if not native:
         _LIB = L
     return L
@@ -167,6 +169,11 @@ def node_at(kind, fw, points):
         ok[i] = bool(lib().mo_node_at(kind, ctypes.byref(st), _p(pts[i]), _p(o)))
         out[i] = o
     return ok, out
+
+
+def cell_volume(c):
+    """Simulation cell volume (simcell_getvolume)."""
+    return lib().mo_cell_volume(ctypes.byref(c))
 
 
 def cell_uniform(c, seed, n):
@@ -565,6 +572,8 @@ class Gcmc:
         st.nk = n
         st.kv, st.wk, st.Sre, st.Sim = [a.ctypes.data for a in (self.kv, self.wk, self.Sre, self.Sim)]
         vol = float(np.prod(spec["full_ell"]))
+        if self.cell is not None and not self.cell.ortho:
+            vol = cell_volume(self.cell)
         st.B = adams_B(fugacity_kPa, vol, spec["T"])
         L.mo_rng_init(ctypes.byref(st.rng), int(seed))
         self.trace = (GcmcTrace * max(trace, 1))()

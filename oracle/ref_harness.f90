@@ -26,7 +26,7 @@
 !                                           default-real literals, as in
 !                                           rigidmoves_adjustdeltatrans
 !   FCELL   (see read order below)       -> slant / unslant matrices, lattice
-!                                           vectors, minimum images, maptouc,
+!                                           vectors, volume, minimum images, maptouc,
 !                                           pbc and uniform random points of a
 !                                           (possibly slanted) cell
 !   LJNODEX / HERMITEX                   -> LJNODE / HERMITE in a cell given
@@ -55,7 +55,7 @@ program ref_harness
       molecule_getmoi
   Use fundcell, Only: Fundamental_Cell, fundcell_init, fundcell_construct
   Use vector, Only: IntVecType
-  Use simcell, Only: simcell_maptouc, simcell_pbc, simcell_uniformRandPt
+  Use simcell, Only: simcell_maptouc, simcell_pbc, simcell_uniformRandPt, simcell_getvolume
   Implicit None
 
   Character(len=32) :: job
@@ -504,6 +504,7 @@ Contains
     Write(*,'(A,9ES26.17E3)') 'U', ((cell%fullC%unslantmatrix%comp(i,j), j = 1, 3), i = 1, 3)
     Write(*,'(A,9ES26.17E3)') 'S', ((cell%fullC%slantmatrix%comp(i,j), j = 1, 3), i = 1, 3)
     Write(*,'(A,9ES26.17E3)') 'L', (cell%fullC%lvec(i)%comp, i = 1, 3)
+    Write(*,'(A,ES26.17E3)') 'VOL', simcell_getvolume(cell)
     Read(*,*) n
     Do i = 1, n
       Read(*,*) v

@@ -367,6 +367,8 @@ def gen_fcell():
                 res["ortho"] = f[1] == "T"
             elif f[0] in ("U", "S", "L"):
                 res[f[0]] = [float(x) for x in f[1:]]
+            elif f[0] == "VOL":
+                res["vol"] = float(f[1])
             elif f[0] == "SEP":
                 res["sep"].append([float(x) for x in f[1:]])
             elif f[0] in ("UC", "PBC", "RAND"):

@@ -4,7 +4,8 @@ lj / pmap modules (tests/test_oracle_ref_cells.py): the slanted single-fold
 minimum image in the node kernel (fundcell_multMinImage), the grid over the
 effective box, compaction, and interpolation folding into the slanted unit
 cell without the corner wrap; both map precisions run the FP64 node kernel
-for slanted cells; GCMC batches refuse them."""
+for slanted cells; GCMC batches in slanted cells against the oracle chain
+(without sorbate Ewald sums, which they refuse)."""
 import numpy as np
 import pytest
 
@@ -18,10 +19,9 @@ CELLS = [("monoclinic", [14.0, 13.0, 15.0], [90.0, 104.0, 90.0], [17.6, 13.0, 14
          ("triclinic", [12.5, 13.5, 12.0], [84.0, 97.0, 108.0], [16.8, 13.0, 12.0], (1, 1, 1))]
 
 
-def _system(hip, ell, ang, eff, nc, seed):
+def _system(hip, ell, ang, eff, nc, seed, n=90):
     rng = np.random.default_rng(seed)
     c = O.cell(ell, ang, eff, nc)
-    n = 90
     # atoms uniform in the slanted cell: fractional coordinates times the lattice vectors
     U, S, lv = O.cell_matrices(c)
     frac = rng.uniform(0, 1, (n, 3))
@@ -97,11 +97,81 @@ def test_slanted_ewald_map_matches_oracle(hip):
     m.close()
 
 
-def test_gcmc_refuses_slanted_cells(hip):
+def _gcmc_spec(hip, name, ell, ang, eff, nc, seed):
+    """A 2-site rigid sorbate (sites 1.1 A apart, one LJ map) in a slanted
+    framework cell, sorbate-sorbate LJ within 5 A and plain Coulomb."""
+    from musichip import inputs
+    # a sparse framework (30 atoms): room for the sorbate
+    c, xyz, typ, q, tab, full = _system(hip, ell, ang, eff, nc, seed, n=30)
+    m = hip.build_map(H.MH_MAP_LJ, H.MH_PREC_FP64, 0.3, 100.0)
+    A, B = inputs.lj_AB(3.3, 36.0)
+    spec = dict(full_ell=full, fund_ell=np.array(ell, dtype=float), ref=[[0.0, 0.0, -0.55], [0.0, 0.0, 0.55]],
+                stype=[0, 0], sq=[0.4, -0.4], nst=1, lj=([A], [B], [5.0 ** 2], [0.5 ** 2], [1]),
+                coul=dict(on=[1], hicut=5.0, locut=0.5, shield=1.0), T=300.0,
+                moves=[("RINSERT", 1.0), ("RDELETE", 1.0), ("RTRANSLATE", 1.0), ("RROTATE", 1.0)],
+                dtrans=0.5, drot=0.4, cell=c)
+    return spec, {0: (m, None), 1: (m, None)}, m
+
+
+@pytest.mark.parametrize("name,ell,ang,eff,nc", CELLS)
+def test_gcmc_trajectory_in_slanted_cell_matches_oracle(hip, name, ell, ang, eff, nc):
+    """Insertion by the lattice vectors (simcell_uniformRandPt), slanted pbc
+    of the sites, slanted minimum image of sorbate pairs and map lookups
+    folded into the slanted unit cell: the FP64 move sequence equals the
+    oracle's, whose slanted pieces match the reference build bit for bit."""
+    from musichip import gcmc as G
+    from test_gpu_gcmc import _oracle_for
+    print("case: GCMC of a 2-site sorbate in the %s cell, FP64 trajectory parity" % name, flush=True)
+    spec, maps, m = _gcmc_spec(hip, name, ell, ang, eff, nc, 41)
+    fugs, seeds, nsteps = [1.0e5, 1.0e6], [5151, 5152], 4000
+    ref = []
+    for r in range(2):
+        g = _oracle_for(spec, maps, fugs[r], seeds[r])
+        g.run(nsteps)
+        ref.append((g.trace_array(nsteps), g.st.N, g.st.energy))
+    b = G.GcmcBatch(hip, spec, fugs, seeds, maxN=256, maps=maps)
+    b.set_trace(nsteps)
+    b.run(nsteps)
+    s = b.state()
+    for r in range(2):
+        to, No, eo = ref[r]
+        tg = b.trace(r)
+        assert np.array_equal(tg[:, :3].astype(int), to[:, :3].astype(int)), r
+        scale = np.maximum(np.abs(to[:, 3]), 1.0)
+        assert np.max(np.abs(tg[:, 3] - to[:, 3]) / scale) < 1e-10
+        assert No == s["N"][r]
+        assert abs(eo - s["energy"][r]) < 1e-9 * max(1.0, abs(eo))
+    assert max(s["N"]) >= 5, list(s["N"])
+    tot = b.total_energy()
+    assert np.allclose(tot, s["energy"], rtol=1e-10, atol=1e-9)
+    # FP32 kernel: per-trial energies of the same configurations within 1e-5
+    b32 = G.GcmcBatch(hip, spec, fugs, seeds, maxN=256, maps=maps, precision=H.MH_PREC_FP32)
+    rng = np.random.default_rng(3)
+    U, S_, lv = O.cell_matrices(spec["cell"])
+    n = 0
+    for r in range(2):
+        com, ang_, _ = b.config(r)
+        b32.set_config(r, com, ang_)
+        tc = rng.uniform(0, 1, (2000, 3)) @ lv
+        ta = np.stack([rng.uniform(-np.pi, np.pi, 2000), np.arccos(rng.uniform(-1, 1, 2000)),
+                       rng.uniform(-np.pi, np.pi, 2000)], 1)
+        ok64, e64 = b.trial_energy(r, tc, ta)
+        ok32, e32 = b32.trial_energy(r, tc, ta)
+        assert np.count_nonzero(ok64 != ok32) <= 4
+        both = ok64 & ok32
+        n += int(both.sum())
+        assert np.max(np.abs(e32[both] - e64[both]) / np.maximum(np.abs(e64[both]), 1.0)) <= 1e-5
+    assert n > 200
+    b32.close()
+    b.close()
+    m.close()
+
+
+def test_gcmc_refuses_ewald_and_bias_in_slanted_cells(hip):
     from musichip import gcmc as G
     name, ell, ang, eff, nc = CELLS[0]
-    _system(hip, ell, ang, eff, nc, 33)
-    spec = dict(full_ell=ell, fund_ell=ell, ref=[[0, 0, 0]], stype=[0], sq=[0.0], nst=1,
-                lj=([1e6], [1e3], [25.0], [0.25], [1]), T=300.0, moves=[("RINSERT", 1.0), ("RDELETE", 1.0)])
+    spec, maps, m = _gcmc_spec(hip, name, ell, ang, eff, nc, 33)
+    spec = dict(spec, coul=None, ewald=dict(kappa=0.3, kmax=5, rc=5.0))
     with pytest.raises(H.MusicHipError, match="slanted"):
-        G.GcmcBatch(hip, spec, [100.0], [1], maxN=16)
+        G.GcmcBatch(hip, spec, [100.0], [1], maxN=16, maps=maps)
+    m.close()

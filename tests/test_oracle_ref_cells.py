@@ -25,6 +25,7 @@ def test_cell_matrices_and_lattice_vectors(golden):
         assert np.array_equal(U.ravel(), np.array(g["U"])), g["angles"]
         assert np.array_equal(S.ravel(), np.array(g["S"])), g["angles"]
         assert np.array_equal(L.ravel(), np.array(g["L"])), g["angles"]
+        assert O.cell_volume(c) == g["vol"], g["angles"]
     # the monoclinic / triclinic matrices are not the orthorhombic identity
     assert not golden["fcell"][1]["ortho"] and abs(golden["fcell"][1]["U"][2] + 0.1994) < 1e-3
 
