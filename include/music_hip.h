@@ -225,6 +225,7 @@ int mh_bmap_search(const mh_bmap *b, int n, const double *keys, int64_t *index);
 enum { MH_MV_INSERT = 0, MH_MV_DELETE = 1, MH_MV_TRANSLATE = 2, MH_MV_ROTATE = 3, MH_MV_BINSERT = 4,
        MH_MV_BDELETE = 5 };
 #define MH_MAX_SITES 8
+#define MH_MAX_SPECIES 4
 
 typedef struct {
     int nsites;                          /* 1..8 */
@@ -273,6 +274,23 @@ typedef struct mh_gcmc mh_gcmc;
  * random_init); maxN = molecule capacity per replica. */
 int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double *fugacity_kPa,
                    const int32_t *seeds, int maxN, mh_gcmc **out);
+/* A mixture of nspc (1..MH_MAX_SPECIES) sorbates (gcmc_dosim, gcmc.F90:
+ * 224-258: every step draws a species with equal probability, then a move
+ * of its move set; its molecules interact with those of every species).
+ * spc[s] gives species s's model, maps, move set, displacements, bias map
+ * and max_nrg; the pair tables (site types of all species index them),
+ * Coulomb / Ewald settings, temperature and precision are spc[0]'s.
+ * fugacity_kPa[nrep][nspc]; maxN = molecule capacity per species and
+ * replica.  mh_gcmc_create is this call with nspc = 1.  Traces record
+ * move = 16 species + move type and N of the moved species. */
+int mh_gcmc_create_mix(mh_ctx *ctx, const mh_gcmc_params *spc, int nspc, int nrep, const double *fugacity_kPa,
+                       const int32_t *seeds, int maxN, mh_gcmc **out);
+/* Species addressed by the per-species calls that follow (mh_gcmc_get_state's
+ * N / sumN / att / acc, mh_gcmc_get_moments, mh_gcmc_get_deltas /
+ * set_deltas, mh_gcmc_set_counts, mh_gcmc_set_config / get_config,
+ * mh_trial_energy); 0 after creation.  Energies, step counts and RNG states
+ * are per replica. */
+int mh_gcmc_species(mh_gcmc *g, int spc);
 int mh_gcmc_destroy(mh_gcmc *g);
 /* Threads per replica (64..1024, power of 2); chosen by the autotuner. */
 int mh_gcmc_set_threads(mh_gcmc *g, int threads);

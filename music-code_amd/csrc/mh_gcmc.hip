@@ -41,13 +41,29 @@ constexpr double kTwoPiR = 2.0 * 3.1415927;  // defaults.F90:35
 constexpr double kTokcalG = 332.064;
 
 
-struct GDev {
+constexpr int GSPC = MH_MAX_SPECIES;  // max species of a mixture
+
+// One sorbate species: its rigid model, maps and move set.
+struct SpcDev {
     int ns;
     double ref[GS][3];
-    int st[GS];
+    int st[GS];       // site types (index into the common pair tables)
     double q[GS];
     int has_p[GS], has_e[GS];
     MapRef pm[GS], em[GS];
+    int nmoves;
+    int mtype[4];
+    double cumwt[4];
+    double insdel, dtrans, drot, maxnrg, molconst;
+    int adj_trans, adj_rot;  // adjustable displacements (rigidmoves_adjustdelta*)
+    double inv_ns, inv_nss;  // 1/ns, 1/ns^2 for the pair-item decomposition
+    BiasRef bias;            // BINSERT / BDELETE
+};
+
+// A molecular system: the SpcDev base is the species of the move in
+// progress (a copy of sp[s] in the kernel's LDS); the rest is shared by
+// the species (cell, pair tables, Ewald, temperature).
+struct GDev : SpcDev {
     int nst;
     double A[64], B[64], h2[64], l2[64];
     float Af[64], Bf[64], h2f[64], l2f[64];
@@ -61,15 +77,13 @@ struct GDev {
     const double *wk;      // [nk]
     const int8_t *kidx;    // [nk][4] (mx, my, mz, pad)
     double full[3], fund[3];
-    int nmoves;
-    int mtype[4];
-    double cumwt[4];
-    double insdel, dtrans, drot, maxnrg, rti, molconst;
-    int adj_trans, adj_rot;  // adjustable displacements (rigidmoves_adjustdelta*)
-    double inv_ns, inv_nss;  // 1/ns, 1/ns^2 for the pair-item decomposition
-    BiasRef bias;            // BINSERT / BDELETE
+    double rti;
     int ncell[3];            // unit cells per axis of the simulation cell
     int prec, maxN;
+    // species: molecules of species s occupy slots [s maxN, (s + 1) maxN) of
+    // a replica, each slot nsmax sites (the batch's largest molecule)
+    int nspc, nsmax;
+    SpcDev sp[GSPC];
     // e^{i (kx x + ky y)} table per (set, site) in dynamic LDS at byte
     // offset exy_off: [2][ns][kmax + 1][2 kmax + 1] (double2, float2 in FP32)
     int use_exy, exy_off;
@@ -91,13 +105,15 @@ __device__ __forceinline__ double adjust_delta(double d, double ratio) {
     return d;
 }
 
+// per replica; per-species quantities are [rep][GSPC](...) and a replica's
+// molecules [rep][nspc maxN] slots (species-major) of nsmax sites
 struct Rep {
-    int *N, *err;
+    int *N, *err;             // N: [rep][GSPC]
     int32_t *ix, *iy, *first, *iseed;
-    double *B, *energy, *sumN, *sumN2;
-    long long *nsteps, *att, *acc;
-    double *com, *ang, *pos;  // [rep][maxN][3], [rep][maxN][3], [rep][maxN][ns][3]
-    double *delta;            // [rep][2] translate / rotate displacement in use
+    double *B, *energy, *sumN, *sumN2;  // B, sumN, sumN2: [rep][GSPC]
+    long long *nsteps, *att, *acc;      // att, acc: [rep][GSPC][4]
+    double *com, *ang, *pos;  // [rep][slots][3], [rep][slots][3], [rep][slots][nsmax][3]
+    double *delta;            // [rep][GSPC][2] translate / rotate displacement in use
     double2 *S, *dS;          // [rep][nk]
     int *tr_move, *tr_acc, *tr_N;
     double *tr_dE;
@@ -278,12 +294,12 @@ __device__ __forceinline__ bool corner_term(const MapRef &m, const double *r, in
 
 // map energy of one site by one lane (totals and trial energies off the hot
 // loop): the 8 corner terms in corner order
-__device__ bool site_map_energy(const GDev &P, int a, const double *r, double &e) {
+__device__ bool site_map_energy(const GDev &P, const SpcDev &Q, int a, const double *r, double &e) {
     e = 0.0;
     double pm = 0.0, em = 0.0;
     for (int w = 0; w < 2; w++) {
-        if (!(w == 0 ? P.has_p[a] : P.has_e[a])) continue;
-        const MapRef &m = w == 0 ? P.pm[a] : P.em[a];
+        if (!(w == 0 ? Q.has_p[a] : Q.has_e[a])) continue;
+        const MapRef &m = w == 0 ? Q.pm[a] : Q.em[a];
         double s = 0.0;
         for (int c = 0; c < 8; c++) {
             double part;
@@ -292,8 +308,8 @@ __device__ bool site_map_energy(const GDev &P, int a, const double *r, double &e
         }
         (w == 0 ? pm : em) = s;
     }
-    if (P.has_p[a]) e = __dadd_rn(e, pm);
-    if (P.has_e[a]) e = __dadd_rn(e, __dmul_rn(em, P.q[a]));
+    if (Q.has_p[a]) e = __dadd_rn(e, pm);
+    if (Q.has_e[a]) e = __dadd_rn(e, __dmul_rn(em, Q.q[a]));
     return true;
 }
 
@@ -348,6 +364,8 @@ __device__ __forceinline__ bool pair_energy32(const GDev &P, int t, float qq, fl
 
 
 struct Shared {
+    int N[GSPC];                         // molecules per species (species of the move: its count)
+    int spc;                             // species of the move in progress
     double posA[GS * 3], posB[GS * 3];   // trial (A) and old (B) site coordinates
     double sc[3][2];                     // sin, cos of the trial Euler angles
     double mapv[2][GS];
@@ -433,8 +451,9 @@ __device__ __forceinline__ double2 k_delta(const GDev &P, const Shared &sh, int 
     }
 }
 
-// Energy terms of a move: set A (+) and set B (-) of one molecule against
-// every other molecule (skip = its slot) and the reciprocal-space delta,
+// Energy terms of a move: set A (+) and set B (-) of one molecule of species
+// sh.spc against every other molecule of every species (skip = its slot in
+// its species, N = that species' count) and the reciprocal-space delta,
 // whose dS(k) is left in K.d (registers) or dSg (KPL == 0).
 // out: map A, pairs A, map B, pairs B, reciprocal delta, ok(A).
 template <int NT, int PREC, int KPL, bool SL = false>
@@ -510,47 +529,56 @@ __device__ __forceinline__ void move_energy(const GDev &P, Shared &sh, const dou
         }
         __syncthreads();
     }
-    // pairs: items (other molecule j, its site b, trial site a)
+    // pairs: per species s2 the items (molecule j of s2, its site b, trial site a)
     double eA = 0.0, eB = 0.0, failA = 0.0;
-    const int nss = ns * ns;
-    const int items = N * nss;
     float eAf = 0.0f, eBf = 0.0f;
     const float kap32 = (float)P.kappa, rc32 = (float)P.rc2;
-    for (int w = tid; w < items; w += NT) {
-        const int j = (int)(((double)w + 0.5) * P.inv_nss);
-        if (j == skip) continue;
-        const int r = w - j * nss;
-        const int b = (int)(((double)r + 0.5) * P.inv_ns);
-        const int a = r - b * ns;
-        const double *pj = pos + ((long long)j * ns + b) * 3;
-        const double xb = pj[0], yb = pj[1], zb = pj[2];
-        const int t = P.st[a] * P.nst + P.st[b];
-        const double pb[3] = {xb, yb, zb};
-        if (PREC == MH_PREC_FP64) {
-            if (hasA) {
-                double sp[3];
-                min_image(P, &sh.posA[3 * a], pb, sp, SL);
-                if (!pair_energy64(P, t, P.q[a], P.q[b], sp[0], sp[1], sp[2], eA)) failA = 1.0;
-            }
-            if (hasB) {
-                double sp[3];
-                min_image(P, &sh.posB[3 * a], pb, sp, SL);
-                pair_energy64(P, t, P.q[a], P.q[b], sp[0], sp[1], sp[2], eB);
-            }
-        } else {
-            const float qq = P.ewald ? (float)(P.q[a] * P.q[b] * kTokcalG) : 0.0f;
-            if (hasA) {
-                double d[3];
-                min_image(P, &sh.posA[3 * a], pb, d, SL);
-                if (!pair_energy32(P, t, qq, kap32, rc32, (float)d[0], (float)d[1], (float)d[2], eAf)) failA = 1.0;
-                if (P.coul && !coul_energy64(P, t, P.q[a], P.q[b], d[0] * d[0] + d[1] * d[1] + d[2] * d[2], eA))
-                    failA = 1.0;
-            }
-            if (hasB) {
-                double d[3];
-                min_image(P, &sh.posB[3 * a], pb, d, SL);
-                pair_energy32(P, t, qq, kap32, rc32, (float)d[0], (float)d[1], (float)d[2], eBf);
-                if (P.coul) coul_energy64(P, t, P.q[a], P.q[b], d[0] * d[0] + d[1] * d[1] + d[2] * d[2], eB);
+    for (int s2 = 0; s2 < P.nspc; s2++) {
+        const bool own = s2 == sh.spc;
+        const SpcDev &O = (P.nspc == 1) ? static_cast<const SpcDev &>(P) : P.sp[s2];
+        const int ns2 = O.ns;
+        const int n2 = own ? N : sh.N[s2];
+        const int nss = ns2 * ns;
+        const int items = n2 * nss;
+        const double inv_nss = own ? P.inv_nss : 1.0 / (double)nss;
+        const double *pos2 = pos + (long long)s2 * P.maxN * P.nsmax * 3;
+        for (int w = tid; w < items; w += NT) {
+            const int j = (int)(((double)w + 0.5) * inv_nss);
+            if (own && j == skip) continue;
+            const int r = w - j * nss;
+            const int b = (int)(((double)r + 0.5) * P.inv_ns);
+            const int a = r - b * ns;
+            const double *pj = pos2 + ((long long)j * P.nsmax + b) * 3;
+            const double xb = pj[0], yb = pj[1], zb = pj[2];
+            const int t = P.st[a] * P.nst + O.st[b];
+            const double qb = O.q[b];
+            const double pb[3] = {xb, yb, zb};
+            if (PREC == MH_PREC_FP64) {
+                if (hasA) {
+                    double sp[3];
+                    min_image(P, &sh.posA[3 * a], pb, sp, SL);
+                    if (!pair_energy64(P, t, P.q[a], qb, sp[0], sp[1], sp[2], eA)) failA = 1.0;
+                }
+                if (hasB) {
+                    double sp[3];
+                    min_image(P, &sh.posB[3 * a], pb, sp, SL);
+                    pair_energy64(P, t, P.q[a], qb, sp[0], sp[1], sp[2], eB);
+                }
+            } else {
+                const float qq = P.ewald ? (float)(P.q[a] * qb * kTokcalG) : 0.0f;
+                if (hasA) {
+                    double d[3];
+                    min_image(P, &sh.posA[3 * a], pb, d, SL);
+                    if (!pair_energy32(P, t, qq, kap32, rc32, (float)d[0], (float)d[1], (float)d[2], eAf)) failA = 1.0;
+                    if (P.coul && !coul_energy64(P, t, P.q[a], qb, d[0] * d[0] + d[1] * d[1] + d[2] * d[2], eA))
+                        failA = 1.0;
+                }
+                if (hasB) {
+                    double d[3];
+                    min_image(P, &sh.posB[3 * a], pb, d, SL);
+                    pair_energy32(P, t, qq, kap32, rc32, (float)d[0], (float)d[1], (float)d[2], eBf);
+                    if (P.coul) coul_energy64(P, t, P.q[a], qb, d[0] * d[0] + d[1] * d[1] + d[2] * d[2], eB);
+                }
             }
         }
     }
@@ -721,33 +749,50 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
         reinterpret_cast<int *>(&P)[i] = reinterpret_cast<const int *>(Pg)[i];
     __syncthreads();
     if (R.err[rep]) return;
-    const int ns = P.ns;
-    Rng rng{R.ix[rep], R.iy[rep], R.first[rep], R.iseed[rep]};
-    int N = R.N[rep];
-    double energy = R.energy[rep], sumN = R.sumN[rep], sumN2 = R.sumN2[rep];
-    long long nst = R.nsteps[rep];
-    long long att[4], acc[4];
-    for (int i = 0; i < 4; i++) {
-        att[i] = R.att[4 * rep + i];
-        acc[i] = R.acc[4 * rep + i];
+    // per-species chain state in LDS (the species of a step: a copy in
+    // registers, written back at the end of the step)
+    __shared__ double sB[GSPC], sDel[GSPC][2], sSum[GSPC], sSum2[GSPC];
+    __shared__ long long sAtt[GSPC][4], sAcc[GSPC][4];
+    const int nspc = P.nspc;
+    if (tid < GSPC) {
+        const int o = GSPC * rep + tid;
+        const bool live = tid < nspc;
+        sh.N[tid] = live ? R.N[o] : 0;
+        sB[tid] = live ? R.B[o] : 0.0;
+        sSum[tid] = live ? R.sumN[o] : 0.0;
+        sSum2[tid] = live ? R.sumN2[o] : 0.0;
+        sDel[tid][0] = live ? R.delta[2 * o] : 0.0;
+        sDel[tid][1] = live ? R.delta[2 * o + 1] : 0.0;
+        for (int i = 0; i < 4; i++) {
+            sAtt[tid][i] = live ? R.att[4 * o + i] : 0;
+            sAcc[tid][i] = live ? R.acc[4 * o + i] : 0;
+        }
     }
-    double dtr = R.delta[2 * rep], drt = R.delta[2 * rep + 1];
-    const double B = R.B[rep];
-    double *gcom = R.com + (long long)rep * P.maxN * 3;
-    double *gang = R.ang + (long long)rep * P.maxN * 3;
-    double *gpos = R.pos + (long long)rep * P.maxN * ns * 3;
+    if (tid == 0) sh.spc = 0;  // the uploaded model is species 0's
+    int ns = P.ns, cur = 0;
+    Rng rng{R.ix[rep], R.iy[rep], R.first[rep], R.iseed[rep]};
+    double energy = R.energy[rep];
+    long long nst = R.nsteps[rep];
+    const long long slots = (long long)nspc * P.maxN, ms3 = 3LL * P.nsmax;
+    double *gcom = R.com + (long long)rep * slots * 3;
+    double *gang = R.ang + (long long)rep * slots * 3;
+    double *gpos = R.pos + (long long)rep * slots * ms3;
     double2 *S = R.S + (long long)rep * R.kstride;
     double2 *dS = R.dS + (long long)rep * R.kstride;
     double *pos, *com, *ang;
     void *exy = reinterpret_cast<char *>(dyn) + P.exy_off;
+    __syncthreads();
     if constexpr (LDSMOL) {
         pos = dyn;
-        com = dyn + (long long)P.maxN * ns * 3;
-        ang = com + (long long)P.maxN * 3;
-        for (int i = tid; i < N * ns * 3; i += NT) pos[i] = gpos[i];
-        for (int i = tid; i < N * 3; i += NT) {
-            com[i] = gcom[i];
-            ang[i] = gang[i];
+        com = dyn + slots * ms3;
+        ang = com + slots * 3;
+        for (int s2 = 0; s2 < nspc; s2++) {
+            const long long n2 = sh.N[s2], o3 = (long long)s2 * P.maxN * 3, op = (long long)s2 * P.maxN * ms3;
+            for (long long i = tid; i < n2 * ms3; i += NT) pos[op + i] = gpos[op + i];
+            for (long long i = tid; i < n2 * 3; i += NT) {
+                com[o3 + i] = gcom[o3 + i];
+                ang[o3 + i] = gang[o3 + i];
+            }
         }
     } else {
         pos = gpos;
@@ -775,7 +820,27 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
     __syncthreads();
     int err = 0, done = 0;
     for (int it = 0; it < nrun; it++) {
-        (void)rranf(rng);  // species (nspc = 1), gcmc.F90:242
+        // gcmc_dosim: a species with equal probability (gcmc.F90:240-242)
+        const int s = (int)(rranf(rng) * (double)nspc);
+        if (s != cur) {  // its model into the LDS view
+            __syncthreads();
+            for (int i = tid; i < (int)(sizeof(SpcDev) / 4); i += NT)
+                reinterpret_cast<int *>(static_cast<SpcDev *>(&P))[i] = reinterpret_cast<const int *>(&P.sp[s])[i];
+            if (tid == 0) sh.spc = s;
+            __syncthreads();
+            cur = s;
+            ns = P.ns;
+        }
+        int N = sh.N[s];
+        long long att[4], acc[4];
+        for (int i = 0; i < 4; i++) {
+            att[i] = sAtt[s][i];
+            acc[i] = sAcc[s][i];
+        }
+        double dtr = sDel[s][0], drt = sDel[s][1];
+        const double B = sB[s];
+        double *comS = com + (long long)s * P.maxN * 3, *angS = ang + (long long)s * P.maxN * 3;
+        double *posS = pos + (long long)s * P.maxN * ms3;
         const double prob = rranf(rng);
         int mv = 0;
         for (int i = 0; i < P.nmoves; i++) {
@@ -843,8 +908,8 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                 mol = (int)(rranf(rng) * N);
                 att[mbase]++;
                 for (int i = 0; i < 3; i++) {
-                    tcom[i] = com[3 * mol + i];
-                    tang[i] = ang[3 * mol + i];
+                    tcom[i] = comS[3 * mol + i];
+                    tang[i] = angS[3 * mol + i];
                 }
                 if (mbase == MH_MV_DELETE && biased) {
                     // rigidmoves_bdelete: bmap_getcellwt of the centre of mass
@@ -889,9 +954,9 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
             // ---- site coordinates: set B = the molecule as stored; set A =
             // the trial (or, for a deletion, the stored molecule)
             const int n3 = 3 * ns;
-            if (hasB && tid >= 64 - n3 && tid < 64) sh.posB[tid - (64 - n3)] = pos[(long long)mol * n3 + tid - (64 - n3)];
+            if (hasB && tid >= 64 - n3 && tid < 64) sh.posB[tid - (64 - n3)] = posS[mol * ms3 + tid - (64 - n3)];
             if (mbase == MH_MV_DELETE) {
-                if (tid < n3) sh.posA[tid] = pos[(long long)mol * n3 + tid];
+                if (tid < n3) sh.posA[tid] = posS[mol * ms3 + tid];
             } else if (SL) {
                 if (tid == 0) site_coords(P, tcom, tang, sh.posA);
             } else if (ns == 1) {
@@ -935,10 +1000,10 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                     accepted = boltz > rranf(rng);
                     if (accepted) {
                         if (tid < 3) {
-                            com[3 * N + tid] = tcom[tid];
-                            ang[3 * N + tid] = tang[tid];
+                            comS[3 * N + tid] = tcom[tid];
+                            angS[3 * N + tid] = tang[tid];
                         }
-                        if (tid < n3) pos[(long long)N * n3 + tid] = sh.posA[tid];
+                        if (tid < n3) posS[N * ms3 + tid] = sh.posA[tid];
                         if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                         N++;
                         energy += U;
@@ -964,10 +1029,10 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                     const int last = N - 1;
                     // every lane finished reading the molecules (block_sum barrier)
                     if (tid < 3) {
-                        com[3 * mol + tid] = com[3 * last + tid];
-                        ang[3 * mol + tid] = ang[3 * last + tid];
+                        comS[3 * mol + tid] = comS[3 * last + tid];
+                        angS[3 * mol + tid] = angS[3 * last + tid];
                     }
-                    if (tid < n3) pos[(long long)mol * n3 + tid] = pos[(long long)last * n3 + tid];
+                    if (tid < n3) posS[mol * ms3 + tid] = posS[last * ms3 + tid];
                     if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                     N--;
                     energy -= U;
@@ -987,10 +1052,10 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                     accepted = boltz > rranf(rng);
                     if (accepted) {
                         if (tid < 3) {
-                            com[3 * mol + tid] = tcom[tid];
-                            ang[3 * mol + tid] = tang[tid];
+                            comS[3 * mol + tid] = tcom[tid];
+                            angS[3 * mol + tid] = tang[tid];
                         }
-                        if (tid < n3) pos[(long long)mol * n3 + tid] = sh.posA[tid];
+                        if (tid < n3) posS[mol * ms3 + tid] = sh.posA[tid];
                         if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                         energy += dU;
                         acc[mtype]++;
@@ -1008,24 +1073,55 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
             long long t = nst - R.tr_base;
             if (t >= 0 && t < R.tr_cap) {
                 long long o = (long long)rep * R.tr_cap + t;
-                R.tr_move[o] = mtype;
+                R.tr_move[o] = 16 * s + mtype;
                 R.tr_acc[o] = accepted;
                 R.tr_N[o] = N;
                 R.tr_dE[o] = de;
             }
         }
-        sumN += N;
-        sumN2 += (double)N * N;
+        // the species' counters back to LDS; gcmcmoves_updatenmolecs: every
+        // species' statistics after every step
+        if (tid == 0) {
+            sh.N[s] = N;
+            for (int i = 0; i < 4; i++) {
+                sAtt[s][i] = att[i];
+                sAcc[s][i] = acc[i];
+            }
+            sDel[s][0] = dtr;
+            sDel[s][1] = drt;
+            for (int i = 0; i < nspc; i++) {
+                const double n = (double)sh.N[i];
+                sSum[i] += n;
+                sSum2[i] += n * n;
+            }
+        }
         nst++;
         done++;
         __syncthreads();
     }
-    // write the replica back
+    // write the replica back (a step that stopped on an error leaves its
+    // species' counters as they were before it)
+    __syncthreads();
     if constexpr (LDSMOL) {
-        for (int i = tid; i < N * ns * 3; i += NT) gpos[i] = pos[i];
-        for (int i = tid; i < N * 3; i += NT) {
-            gcom[i] = com[i];
-            gang[i] = ang[i];
+        for (int s2 = 0; s2 < nspc; s2++) {
+            const long long n2 = sh.N[s2], o3 = (long long)s2 * P.maxN * 3, op = (long long)s2 * P.maxN * ms3;
+            for (long long i = tid; i < n2 * ms3; i += NT) gpos[op + i] = pos[op + i];
+            for (long long i = tid; i < n2 * 3; i += NT) {
+                gcom[o3 + i] = com[o3 + i];
+                gang[o3 + i] = ang[o3 + i];
+            }
+        }
+    }
+    if (tid < nspc) {
+        const int o = GSPC * rep + tid;
+        R.N[o] = sh.N[tid];
+        R.sumN[o] = sSum[tid];
+        R.sumN2[o] = sSum2[tid];
+        R.delta[2 * o] = sDel[tid][0];
+        R.delta[2 * o + 1] = sDel[tid][1];
+        for (int i = 0; i < 4; i++) {
+            R.att[4 * o + i] = sAtt[tid][i];
+            R.acc[4 * o + i] = sAcc[tid][i];
         }
     }
     if constexpr (KPL > 0) {
@@ -1042,23 +1138,14 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
         R.iy[rep] = rng.iy;
         R.first[rep] = rng.first;
         R.iseed[rep] = rng.iseed;
-        R.N[rep] = N;
         R.energy[rep] = energy;
-        R.sumN[rep] = sumN;
-        R.sumN2[rep] = sumN2;
         R.nsteps[rep] = nst;
-        for (int i = 0; i < 4; i++) {
-            R.att[4 * rep + i] = att[i];
-            R.acc[4 * rep + i] = acc[i];
-        }
-        R.delta[2 * rep] = dtr;
-        R.delta[2 * rep + 1] = drt;
     }
 }
 
 // Insertion energies of trial molecules (one workgroup per trial)
 template <int NT, int PREC>
-__global__ void __launch_bounds__(NT) k_trial(const GDev *__restrict__ Ptab, Rep R, int rep, int n,
+__global__ void __launch_bounds__(NT) k_trial(const GDev *__restrict__ Ptab, Rep R, int rep, int spc, int n,
                                                const double *__restrict__ tcom, const double *__restrict__ tang,
                                                double2 *__restrict__ dSbuf, double *__restrict__ e,
                                                uint8_t *__restrict__ okout) {
@@ -1069,10 +1156,16 @@ __global__ void __launch_bounds__(NT) k_trial(const GDev *__restrict__ Ptab, Rep
     for (int i = tid; i < (int)(sizeof(GDev) / 4); i += NT)
         reinterpret_cast<int *>(&P)[i] = reinterpret_cast<const int *>(Pg)[i];
     __syncthreads();
+    // the trial molecule's species as the view
+    for (int i = tid; i < (int)(sizeof(SpcDev) / 4); i += NT)
+        reinterpret_cast<int *>(static_cast<SpcDev *>(&P))[i] = reinterpret_cast<const int *>(&Pg->sp[spc])[i];
+    if (tid < GSPC) sh.N[tid] = tid < P.nspc ? R.N[GSPC * rep + tid] : 0;
+    if (tid == 0) sh.spc = spc;
+    __syncthreads();
     const int t = blockIdx.x;
     if (t >= n) return;
-    const int N = R.N[rep];
-    const double *pos = R.pos + (long long)rep * P.maxN * P.ns * 3;
+    const int N = sh.N[spc];
+    const double *pos = R.pos + (long long)rep * P.nspc * P.maxN * P.nsmax * 3;
     const double2 *S = R.S + (long long)rep * R.kstride;
     if (tid == 0) site_coords(P, tcom + 3 * t, tang + 3 * t, sh.posA);
     __syncthreads();
@@ -1099,27 +1192,32 @@ __global__ void k_sk_recompute(const GDev *__restrict__ Ptab, Rep R) {
     const GDev &P = Ptab[R.sys ? R.sys[rep] : 0];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P.nk) return;
-    const int N = R.N[rep];
-    const double *pos = R.pos + (long long)rep * P.maxN * P.ns * 3;
+    const double *pos = R.pos + (long long)rep * P.nspc * P.maxN * P.nsmax * 3;
     const int8_t *ki = P.kidx + 4 * k;
     double kv[3] = {((double)ki[0] / P.full[0]) * kTwoPiR, ((double)ki[1] / P.full[1]) * kTwoPiR,
                     ((double)ki[2] / P.full[2]) * kTwoPiR};
     double re = 0.0, im = 0.0;
-    for (int m = 0; m < N; m++)
-        for (int a = 0; a < P.ns; a++) {
-            if (P.q[a] == 0.0) continue;
-            const double *r = pos + ((long long)m * P.ns + a) * 3;
-            double arg = kv[0] * r[0] + kv[1] * r[1] + kv[2] * r[2];
-            double s_, c_;
-            sincos(arg, &s_, &c_);
-            re += P.q[a] * c_;
-            im += P.q[a] * s_;
-        }
+    for (int s = 0; s < P.nspc; s++) {
+        const SpcDev &Q = P.sp[s];
+        const int N = R.N[GSPC * rep + s];
+        const double *ps = pos + (long long)s * P.maxN * P.nsmax * 3;
+        for (int m = 0; m < N; m++)
+            for (int a = 0; a < Q.ns; a++) {
+                if (Q.q[a] == 0.0) continue;
+                const double *r = ps + ((long long)m * P.nsmax + a) * 3;
+                double arg = kv[0] * r[0] + kv[1] * r[1] + kv[2] * r[2];
+                double s_, c_;
+                sincos(arg, &s_, &c_);
+                re += Q.q[a] * c_;
+                im += Q.q[a] * s_;
+            }
+    }
     R.S[(long long)rep * R.kstride + k] = make_double2(re, im);
 }
 
-// total energy per replica: sum over molecules m of (maps + pairs with j < m)
-// + reciprocal + N * molconst; one workgroup per replica
+// total energy per replica: sum over molecules of (maps + pairs with the
+// molecules of earlier species and the earlier molecules of its species)
+// + reciprocal + sum of N_s molconst_s; one workgroup per replica
 template <int NT>
 __global__ void __launch_bounds__(NT) k_total(const GDev *__restrict__ Ptab, Rep R, double *__restrict__ out) {
     __shared__ GDev P;
@@ -1134,35 +1232,50 @@ __global__ void __launch_bounds__(NT) k_total(const GDev *__restrict__ Ptab, Rep
         reinterpret_cast<int *>(&P)[i] = reinterpret_cast<const int *>(Pg)[i];
     __syncthreads();
     const int rep = blockIdx.x;
-    const int N = R.N[rep], ns = P.ns;
-    const double *pos = R.pos + (long long)rep * P.maxN * ns * 3;
-    double e = 0.0, bad = 0.0;
-    for (int m = tid; m < N; m += NT) {
-        const double *pm = pos + (long long)m * ns * 3;
-        for (int a = 0; a < ns; a++) {
+    const long long ms3 = 3LL * P.nsmax, per = (long long)P.maxN * ms3;
+    const double *pos = R.pos + (long long)rep * P.nspc * per;
+    int Ns[GSPC], ntot = 0;
+    for (int s = 0; s < GSPC; s++) {
+        Ns[s] = s < P.nspc ? R.N[GSPC * rep + s] : 0;
+        ntot += Ns[s];
+    }
+    double e = 0.0, bad = 0.0, nconst = 0.0;
+    for (int idx = tid; idx < ntot; idx += NT) {
+        int s = 0, m = idx;
+        while (m >= Ns[s]) m -= Ns[s++];
+        const SpcDev &Q = P.sp[s];
+        const double *pm = pos + s * per + m * ms3;
+        for (int a = 0; a < Q.ns; a++) {
             double em;
-            if (!site_map_energy(P, a, pm + 3 * a, em)) bad = 1.0;
+            if (!site_map_energy(P, Q, a, pm + 3 * a, em)) bad = 1.0;
             e += em;
         }
-        for (int j = 0; j < m; j++) {
-            const double *pj = pos + (long long)j * ns * 3;
-            for (int a = 0; a < ns; a++)
-                for (int b = 0; b < ns; b++) {
-                    double sp[3];
-                    min_image(P, pm + 3 * a, pj + 3 * b, sp, P.slanted != 0);
-                    if (!pair_energy64(P, P.st[a] * P.nst + P.st[b], P.q[a], P.q[b], sp[0], sp[1], sp[2], e))
-                        bad = 1.0;
-                }
+        for (int s2 = 0; s2 <= s; s2++) {
+            const SpcDev &O = P.sp[s2];
+            const int nj = s2 < s ? Ns[s2] : m;
+            for (int j = 0; j < nj; j++) {
+                const double *pj = pos + s2 * per + j * ms3;
+                for (int a = 0; a < Q.ns; a++)
+                    for (int b = 0; b < O.ns; b++) {
+                        double sp[3];
+                        min_image(P, pm + 3 * a, pj + 3 * b, sp, P.slanted != 0);
+                        if (!pair_energy64(P, Q.st[a] * P.nst + O.st[b], Q.q[a], O.q[b], sp[0], sp[1], sp[2], e))
+                            bad = 1.0;
+                    }
+            }
         }
+        nconst += Q.molconst;
     }
-    if (P.ewald)
+    if (P.ewald) {
+        e += nconst;
         for (int k = tid; k < P.nk; k += NT) {
             double2 s = R.S[(long long)rep * R.kstride + k];
             e += P.wk[k] * (s.x * s.x + s.y * s.y);
         }
+    }
     double v[2] = {e, bad};
     block_sum<NT, 2>(v, red);
-    if (tid == 0) out[rep] = (v[1] != 0.0) ? NAN : v[0] + (P.ewald ? N * P.molconst : 0.0);
+    if (tid == 0) out[rep] = (v[1] != 0.0) ? NAN : v[0];
 }
 
 // uncompacted copy of a map: out[n][8] = vals[ptr[n] - 1][8], V = NaN where
@@ -1201,6 +1314,8 @@ struct SysHost {
 struct mh_gcmc {
     mh_ctx *ctx = nullptr;
     int nrep = 0, maxN = 0, prec = 0, threads = 256;
+    int nspc = 1;           // species of the (single-system) batch
+    int sel = 0;            // species the per-species calls address
     int ns = 1;             // max sites over the systems (LDS layout)
     int nk = 0;             // max k vectors over the systems
     long long kstride = 0;  // S(k) entries per replica
@@ -1368,12 +1483,12 @@ static int plan(mh_gcmc *g) {
     int ns = 1, nk = 0;
     for (auto &s : g->sys)
         if (s.used) {
-            ns = std::max(ns, s.H.ns);
+            ns = std::max(ns, s.H.nsmax);
             nk = std::max(nk, s.H.nk);
         }
     g->ns = std::max(g->ns, ns);
     g->nk = nk;
-    const long long need = (long long)g->maxN * (3 * g->ns + 6) * (long long)sizeof(double);
+    const long long need = (long long)g->nspc * g->maxN * (3 * g->ns + 6) * (long long)sizeof(double);
     g->ldsmol = need <= budget;
     g->lds_bytes = g->ldsmol ? (size_t)need : 0;
     g->kpl = g->ldsmol ? pick_kpl(nk, g->threads) : 0;
@@ -1384,11 +1499,12 @@ static int plan(mh_gcmc *g) {
         GDev &H = g->sys[i].H;
         if (g->sys[i].used) {
             const long long K = H.kmax;
-            const long long exy = H.ewald ? 2LL * H.ns * (K + 1) * (2 * K + 1) * ent : 0;
+            const long long exy = H.ewald ? 2LL * H.nsmax * (K + 1) * (2 * K + 1) * ent : 0;
             H.use_exy = (!g->exy_off && g->kpl > 0 && exy > 0 && off + exy <= budget) ? 1 : 0;
             H.exy_off = H.use_exy ? (int)off : 0;
             if (H.use_exy) extra = std::max(extra, exy);
             tab[i] = H;
+            tab[i].nsmax = g->ns;  // slot layout of the batch
         } else {
             std::memset(&tab[i], 0, sizeof(GDev));
         }
@@ -1583,21 +1699,24 @@ static int build_system(mh_ctx *ctx, const mh_gcmc_params *p, int maxN, const do
     }
     H.wk = S.wk.as<double>();
     H.kidx = S.kidx.as<int8_t>();
+    H.nspc = 1;
+    H.nsmax = H.ns;
+    H.sp[0] = static_cast<const SpcDev &>(H);
     S.used = true;
     return MH_OK;
 }
 
 // replica state buffers for nrep replicas
 static int alloc_replicas(mh_gcmc *g) {
-    const long long R = g->nrep, maxN = g->maxN;
+    const long long R = g->nrep, slots = (long long)g->nspc * g->maxN, Q = (long long)GSPC * g->nrep;
     int rc;
-    if ((rc = g->N.alloc(4 * R)) || (rc = g->err.alloc(4 * R)) || (rc = g->ix.alloc(4 * R)) ||
+    if ((rc = g->N.alloc(4 * Q)) || (rc = g->err.alloc(4 * R)) || (rc = g->ix.alloc(4 * R)) ||
         (rc = g->iy.alloc(4 * R)) || (rc = g->first.alloc(4 * R)) || (rc = g->iseed.alloc(4 * R)) ||
-        (rc = g->B.alloc(8 * R)) || (rc = g->energy.alloc(8 * R)) || (rc = g->sumN.alloc(8 * R)) ||
-        (rc = g->sumN2.alloc(8 * R)) || (rc = g->nsteps.alloc(8 * R)) || (rc = g->att.alloc(32 * R)) ||
-        (rc = g->acc.alloc(32 * R)) || (rc = g->delta.alloc(16 * R)) ||
-        (rc = g->com.alloc(sizeof(double) * 3 * R * maxN)) || (rc = g->ang.alloc(sizeof(double) * 3 * R * maxN)) ||
-        (rc = g->pos.alloc(sizeof(double) * 3 * R * maxN * g->ns)) ||
+        (rc = g->B.alloc(8 * Q)) || (rc = g->energy.alloc(8 * R)) || (rc = g->sumN.alloc(8 * Q)) ||
+        (rc = g->sumN2.alloc(8 * Q)) || (rc = g->nsteps.alloc(8 * R)) || (rc = g->att.alloc(32 * Q)) ||
+        (rc = g->acc.alloc(32 * Q)) || (rc = g->delta.alloc(16 * Q)) ||
+        (rc = g->com.alloc(sizeof(double) * 3 * R * slots)) || (rc = g->ang.alloc(sizeof(double) * 3 * R * slots)) ||
+        (rc = g->pos.alloc(sizeof(double) * 3 * R * slots * g->ns)) ||
         (rc = g->S.alloc(sizeof(double2) * (R * g->kstride + 1))) ||
         (rc = g->dS.alloc(sizeof(double2) * (R * g->kstride + 1))) || (rc = g->tr_move.alloc(4)) ||
         (rc = g->tr_acc.alloc(4)) || (rc = g->tr_N.alloc(4)) || (rc = g->tr_dE.alloc(8)))
@@ -1608,32 +1727,44 @@ static int alloc_replicas(mh_gcmc *g) {
     return MH_OK;
 }
 
-// fresh chain state of replicas [r0, r0 + n): N = 0, RNG from seeds, B from
-// the fugacity and the system volume (gcmc_genfuglist, gcmc.F90:303-319)
+// fresh chain state of replicas reps: no molecules, RNG from seeds, B per
+// species from its fugacity fug[i][s] and the system volume
+// (gcmc_genfuglist, gcmc.F90:303-319), the species' displacements
 static int init_replicas(mh_gcmc *g, const std::vector<int> &reps, const double *fug, const int32_t *seeds,
-                         double volume, double T, double dtrans, double drot) {
+                         double volume, double T, const GDev &H) {
     const double sivolume = volume * 1.0e-30 * 6.0221367e23;
+    const int nspc = H.nspc;
     for (size_t i = 0; i < reps.size(); i++) {
-        MH_CHECK(fug[i] > 0.0, MH_EINVAL, "gcmc: fugacity must be > 0");
+        for (int sp = 0; sp < nspc; sp++)
+            MH_CHECK(fug[i * nspc + sp] > 0.0, MH_EINVAL, "gcmc: fugacity must be > 0");
         const int r = reps[i];
         const int32_t zero = 0, one = 1;
-        const double hB = std::log(fug[i] * 1.0e3 * sivolume / (8.314 * T)) - std::log(1.0), zd = 0.0;
-        const long long zl4[4] = {0, 0, 0, 0}, zl = 0;
-        MH_HIP(hipMemcpy(g->N.as<int>() + r, &zero, 4, hipMemcpyHostToDevice));
+        const double zd = 0.0;
+        const long long zl = 0;
+        int zN[GSPC] = {0, 0, 0, 0};
+        double hB[GSPC] = {0, 0, 0, 0}, zD[GSPC] = {0, 0, 0, 0}, d2[2 * GSPC];
+        long long zl4[4 * GSPC];
+        for (int k = 0; k < 4 * GSPC; k++) zl4[k] = 0;
+        for (int sp = 0; sp < GSPC; sp++) {
+            if (sp < nspc)
+                hB[sp] = std::log(fug[i * nspc + sp] * 1.0e3 * sivolume / (8.314 * T)) - std::log(1.0);
+            d2[2 * sp] = sp < nspc ? H.sp[sp].dtrans : 0.0;
+            d2[2 * sp + 1] = sp < nspc ? H.sp[sp].drot : 0.0;
+        }
+        MH_HIP(hipMemcpy(g->N.as<int>() + GSPC * r, zN, 4 * GSPC, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->err.as<int>() + r, &zero, 4, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->ix.as<int32_t>() + r, &zero, 4, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->iy.as<int32_t>() + r, &zero, 4, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->first.as<int32_t>() + r, &one, 4, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->iseed.as<int32_t>() + r, &seeds[i], 4, hipMemcpyHostToDevice));
-        MH_HIP(hipMemcpy(g->B.as<double>() + r, &hB, 8, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->B.as<double>() + GSPC * r, hB, 8 * GSPC, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->energy.as<double>() + r, &zd, 8, hipMemcpyHostToDevice));
-        MH_HIP(hipMemcpy(g->sumN.as<double>() + r, &zd, 8, hipMemcpyHostToDevice));
-        MH_HIP(hipMemcpy(g->sumN2.as<double>() + r, &zd, 8, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->sumN.as<double>() + GSPC * r, zD, 8 * GSPC, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->sumN2.as<double>() + GSPC * r, zD, 8 * GSPC, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->nsteps.as<long long>() + r, &zl, 8, hipMemcpyHostToDevice));
-        MH_HIP(hipMemcpy(g->att.as<long long>() + 4 * r, zl4, 32, hipMemcpyHostToDevice));
-        MH_HIP(hipMemcpy(g->acc.as<long long>() + 4 * r, zl4, 32, hipMemcpyHostToDevice));
-        const double d2[2] = {dtrans, drot};
-        MH_HIP(hipMemcpy(g->delta.as<double>() + 2 * r, d2, 16, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->att.as<long long>() + 4 * GSPC * r, zl4, 32 * GSPC, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->acc.as<long long>() + 4 * GSPC * r, zl4, 32 * GSPC, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->delta.as<double>() + 2 * GSPC * r, d2, 16 * GSPC, hipMemcpyHostToDevice));
         if (g->kstride)
             MH_HIP(hipMemset(g->S.as<double2>() + (long long)r * g->kstride, 0, sizeof(double2) * g->kstride));
     }
@@ -1642,39 +1773,80 @@ static int init_replicas(mh_gcmc *g, const std::vector<int> &reps, const double 
 
 extern "C" {
 
-int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double *fugacity_kPa,
-                   const int32_t *seeds, int maxN, mh_gcmc **out) {
-    MH_CHECK(ctx && p && fugacity_kPa && seeds && out, MH_EINVAL, "mh_gcmc_create: bad arguments");
+int mh_gcmc_create_mix(mh_ctx *ctx, const mh_gcmc_params *spc, int nspc, int nrep, const double *fugacity_kPa,
+                       const int32_t *seeds, int maxN, mh_gcmc **out) {
+    MH_CHECK(ctx && spc && fugacity_kPa && seeds && out, MH_EINVAL, "mh_gcmc_create: bad arguments");
+    MH_CHECK(nspc >= 1 && nspc <= GSPC, MH_EINVAL, "mh_gcmc_create: nspc out of range [1,4]");
     MH_CHECK(nrep >= 1 && nrep <= 65535, MH_EINVAL, "mh_gcmc_create: nrep out of range");
     MH_CHECK(maxN >= 1 && maxN <= (1 << 20), MH_EINVAL, "mh_gcmc_create: maxN out of range");
+    const mh_gcmc_params *p = &spc[0];
     MH_CHECK(p->precision == MH_PREC_FP64 || p->precision == MH_PREC_FP32, MH_EINVAL,
              "mh_gcmc_create: bad precision");
-    for (int a = 0; a < p->nsites && a < MH_MAX_SITES; a++)
-        MH_CHECK((!p->site_pmap[a] || p->site_pmap[a]->slanted == ctx->slanted) &&
-                     (!p->site_emap[a] || p->site_emap[a]->slanted == ctx->slanted),
-                 MH_EINVAL, "mh_gcmc_create: the maps were built for another cell shape than the context's");
+    for (int sp = 0; sp < nspc; sp++)
+        for (int a = 0; a < spc[sp].nsites && a < MH_MAX_SITES; a++)
+            MH_CHECK((!spc[sp].site_pmap[a] || spc[sp].site_pmap[a]->slanted == ctx->slanted) &&
+                         (!spc[sp].site_emap[a] || spc[sp].site_emap[a]->slanted == ctx->slanted),
+                     MH_EINVAL, "mh_gcmc_create: the maps were built for another cell shape than the context's");
     MH_HIP(hipSetDevice(ctx->device));
     std::unique_ptr<mh_gcmc> g(new mh_gcmc());
     g->ctx = ctx;
     g->nrep = nrep;
     g->maxN = maxN;
+    g->nspc = nspc;
     g->prec = p->precision;
     g->sys.resize(1);
     int rc;
     if ((rc = build_system(ctx, p, maxN, ctx->fund_ell, ctx->full_ell, g->sys[0], "mh_gcmc_create", true))) return rc;
-    g->ns = p->nsites;
-    g->slanted = g->sys[0].H.slanted != 0;
-    g->nk = g->sys[0].H.nk;
+    GDev &H = g->sys[0].H;
+    // species 1..: their model and move set with species 0's shared fields
+    // (site types index species 0's pair tables)
+    for (int sp = 1; sp < nspc; sp++) {
+        mh_gcmc_params q = *p;
+        const mh_gcmc_params &o = spc[sp];
+        q.nsites = o.nsites;
+        std::memcpy(q.ref, o.ref, sizeof q.ref);
+        std::memcpy(q.site_type, o.site_type, sizeof q.site_type);
+        std::memcpy(q.site_q, o.site_q, sizeof q.site_q);
+        std::memcpy(q.site_pmap, o.site_pmap, sizeof q.site_pmap);
+        std::memcpy(q.site_emap, o.site_emap, sizeof q.site_emap);
+        q.nmoves = o.nmoves;
+        std::memcpy(q.move_type, o.move_type, sizeof q.move_type);
+        std::memcpy(q.move_weight, o.move_weight, sizeof q.move_weight);
+        q.delta_trans = o.delta_trans;
+        q.delta_rot = o.delta_rot;
+        q.max_nrg = o.max_nrg;
+        q.bias = o.bias;
+        q.adjust_trans = o.adjust_trans;
+        q.adjust_rot = o.adjust_rot;
+        SysHost T;
+        if ((rc = build_system(ctx, &q, maxN, ctx->fund_ell, ctx->full_ell, T, "mh_gcmc_create", true))) return rc;
+        H.sp[sp] = static_cast<const SpcDev &>(T.H);
+        H.nsmax = std::max(H.nsmax, T.H.ns);
+    }
+    H.nspc = nspc;
+    g->ns = H.nsmax;
+    g->slanted = H.slanted != 0;
+    g->nk = H.nk;
     g->kstride = g->nk;
     if ((rc = g->dP.alloc(sizeof(GDev)))) return rc;
     if ((rc = alloc_replicas(g.get()))) return rc;
     std::vector<int> reps(nrep);
     for (int i = 0; i < nrep; i++) reps[i] = i;
-    if ((rc = init_replicas(g.get(), reps, fugacity_kPa, seeds, g->sys[0].volume, p->temperature, p->delta_trans,
-                                p->delta_rot))) return rc;
+    if ((rc = init_replicas(g.get(), reps, fugacity_kPa, seeds, g->sys[0].volume, p->temperature, H))) return rc;
     if ((rc = plan(g.get()))) return rc;
     ctx_retain(ctx);
     *out = g.release();
+    return MH_OK;
+}
+
+int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double *fugacity_kPa,
+                   const int32_t *seeds, int maxN, mh_gcmc **out) {
+    return mh_gcmc_create_mix(ctx, p, 1, nrep, fugacity_kPa, seeds, maxN, out);
+}
+
+int mh_gcmc_species(mh_gcmc *g, int spc) {
+    MH_CHECK(g && spc >= 0 && spc < g->nspc, MH_EINVAL, "mh_gcmc_species: no such species");
+    g->sel = spc;
     return MH_OK;
 }
 
@@ -1770,7 +1942,7 @@ int mh_pool_admit(mh_gcmc *g, const mh_gcmc_params *p, const double fund_ell[3],
         S = SysHost();
         return rc;
     }
-    if ((rc = init_replicas(g, reps, fugacity_kPa, seeds, S.volume, p->temperature, p->delta_trans, p->delta_rot))) {
+    if ((rc = init_replicas(g, reps, fugacity_kPa, seeds, S.volume, p->temperature, S.H))) {
         S = SysHost();
         return rc;
     }
@@ -1813,12 +1985,12 @@ int mh_pool_status(mh_gcmc *g, int ticket, int *nrep, int *slots, int *remaining
         if (slots) slots[i] = r;
         if (remaining) MH_HIP(hipMemcpy(remaining + i, g->budget.as<int>() + r, 4, hipMemcpyDeviceToHost));
         if (err) MH_HIP(hipMemcpy(err + i, g->err.as<int>() + r, 4, hipMemcpyDeviceToHost));
-        if (N) MH_HIP(hipMemcpy(N + i, g->N.as<int>() + r, 4, hipMemcpyDeviceToHost));
+        if (N) MH_HIP(hipMemcpy(N + i, g->N.as<int>() + GSPC * r, 4, hipMemcpyDeviceToHost));
         if (energy) MH_HIP(hipMemcpy(energy + i, g->energy.as<double>() + r, 8, hipMemcpyDeviceToHost));
-        if (sumN) MH_HIP(hipMemcpy(sumN + i, g->sumN.as<double>() + r, 8, hipMemcpyDeviceToHost));
+        if (sumN) MH_HIP(hipMemcpy(sumN + i, g->sumN.as<double>() + GSPC * r, 8, hipMemcpyDeviceToHost));
         if (nsteps) MH_HIP(hipMemcpy(nsteps + i, g->nsteps.as<long long>() + r, 8, hipMemcpyDeviceToHost));
-        if (att) MH_HIP(hipMemcpy(att + 4 * i, g->att.as<long long>() + 4 * r, 32, hipMemcpyDeviceToHost));
-        if (acc) MH_HIP(hipMemcpy(acc + 4 * i, g->acc.as<long long>() + 4 * r, 32, hipMemcpyDeviceToHost));
+        if (att) MH_HIP(hipMemcpy(att + 4 * i, g->att.as<long long>() + 4 * GSPC * r, 32, hipMemcpyDeviceToHost));
+        if (acc) MH_HIP(hipMemcpy(acc + 4 * i, g->acc.as<long long>() + 4 * GSPC * r, 32, hipMemcpyDeviceToHost));
     }
     return MH_OK;
 }
@@ -1843,8 +2015,9 @@ int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const doub
     MH_CHECK(n >= 0 && n <= g->maxN && (n == 0 || com), MH_EINVAL, "mh_gcmc_set_config: bad n");
     MH_HIP(hipSetDevice(g->ctx->device));
     MH_CHECK(!g->pool || g->slot_sys[rep] >= 0, MH_EINVAL, "mh_gcmc_set_config: free slot");
-    const GDev &H = g->sys_of(rep);
-    const int ns = H.ns;
+    const GDev &G0 = g->sys_of(rep);
+    const SpcDev &Q = G0.sp[g->sel];
+    const int ns = Q.ns;
     std::vector<double> hc(3 * (size_t)n), ha(3 * (size_t)n, 0.0), hp(3 * (size_t)n * ns);
     for (int m = 0; m < n; m++) {
         for (int i = 0; i < 3; i++) {
@@ -1856,16 +2029,16 @@ int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const doub
             double v[3];
             for (int i = 0; i < 3; i++) {
                 double t = 0.0;
-                for (int c = 0; c < 3; c++) t = t + H.sc.S[3 * i + c] * r[c];
-                v[i] = t - std::floor(t / H.full[i]) * H.full[i];
+                for (int c = 0; c < 3; c++) t = t + G0.sc.S[3 * i + c] * r[c];
+                v[i] = t - std::floor(t / G0.full[i]) * G0.full[i];
             }
             for (int i = 0; i < 3; i++) {
                 double t = 0.0;
-                for (int c = 0; c < 3; c++) t = t + H.sc.U[3 * i + c] * v[c];
+                for (int c = 0; c < 3; c++) t = t + G0.sc.U[3 * i + c] * v[c];
                 r[i] = t;
             }
         };
-        if (ns == 1 && H.slanted) {
+        if (ns == 1 && G0.slanted) {
             for (int i = 0; i < 3; i++) hp[3 * m + i] = hc[3 * m + i];
             slant_pbc_h(&hp[3 * m]);
             continue;
@@ -1873,7 +2046,7 @@ int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const doub
         if (ns == 1) {
             for (int i = 0; i < 3; i++) {
                 double c = hc[3 * m + i];
-                hp[3 * m + i] = c - std::floor(c / H.full[i]) * H.full[i];
+                hp[3 * m + i] = c - std::floor(c / G0.full[i]) * G0.full[i];
             }
             continue;
         }
@@ -1884,20 +2057,23 @@ int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const doub
                           {ss * st, cs * st, ct}};
         for (int a = 0; a < ns; a++) {
             for (int i = 0; i < 3; i++) {
-                double d = M[i][0] * H.ref[a][0] + M[i][1] * H.ref[a][1] + M[i][2] * H.ref[a][2];
+                double d = M[i][0] * Q.ref[a][0] + M[i][1] * Q.ref[a][1] + M[i][2] * Q.ref[a][2];
                 double rp = hc[3 * m + i] + d;
-                hp[(3 * m * ns) + 3 * a + i] = H.slanted ? rp : rp - std::floor(rp / H.full[i]) * H.full[i];
+                hp[(3 * m * ns) + 3 * a + i] = G0.slanted ? rp : rp - std::floor(rp / G0.full[i]) * G0.full[i];
             }
-            if (H.slanted) slant_pbc_h(&hp[(3 * m * ns) + 3 * a]);
+            if (G0.slanted) slant_pbc_h(&hp[(3 * m * ns) + 3 * a]);
         }
     }
-    const long long off3 = (long long)rep * g->maxN * 3, offp = (long long)rep * g->maxN * g->ns * 3;
+    // species g->sel's slots: [sel maxN, (sel + 1) maxN) of the replica, g->ns sites each
+    const long long slot0 = (long long)rep * g->nspc * g->maxN + (long long)g->sel * g->maxN;
+    const long long off3 = slot0 * 3, offp = slot0 * g->ns * 3;
     if (n) {
         MH_HIP(hipMemcpy(g->com.as<double>() + off3, hc.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->ang.as<double>() + off3, ha.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-        MH_HIP(hipMemcpy(g->pos.as<double>() + offp, hp.data(), sizeof(double) * 3 * n * ns, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy2D(g->pos.as<double>() + offp, sizeof(double) * 3 * g->ns, hp.data(),
+                           sizeof(double) * 3 * ns, sizeof(double) * 3 * ns, n, hipMemcpyHostToDevice));
     }
-    MH_HIP(hipMemcpy(g->N.as<int>() + rep, &n, sizeof(int), hipMemcpyHostToDevice));
+    MH_HIP(hipMemcpy(g->N.as<int>() + GSPC * rep + g->sel, &n, sizeof(int), hipMemcpyHostToDevice));
     // structure factor and energy of every replica from their coordinates
     std::vector<double> e(g->nrep);
     int rc = mh_gcmc_total_energy(g, e.data(), 1);
@@ -1971,17 +2147,29 @@ int mh_gcmc_run_cycles(mh_gcmc *g, int nsteps, int ncycles) {
     return MH_OK;
 }
 
+// the selected species' entries (elem bytes each, per species) of a
+// per-species device array [rep][GSPC][...]
+static int spc_gather(mh_gcmc *g, const DBuf &b, size_t elem, void *out) {
+    const size_t R = g->nrep;
+    std::vector<char> h(R * GSPC * elem);
+    MH_HIP(hipMemcpy(h.data(), b.p, h.size(), hipMemcpyDeviceToHost));
+    for (size_t r = 0; r < R; r++)
+        std::memcpy((char *)out + r * elem, h.data() + (r * GSPC + g->sel) * elem, elem);
+    return MH_OK;
+}
+
 int mh_gcmc_get_state(mh_gcmc *g, int32_t *N, double *energy, double *sumN, int64_t *nsteps, int64_t *att,
                       int64_t *acc, int32_t *rng_ix, int32_t *rng_iy) {
     MH_CHECK(g, MH_EINVAL, "mh_gcmc_get_state: g is NULL");
     MH_HIP(hipSetDevice(g->ctx->device));
     const size_t R = g->nrep;
-    if (N) MH_HIP(hipMemcpy(N, g->N.p, 4 * R, hipMemcpyDeviceToHost));
+    int rc;
+    if (N && (rc = spc_gather(g, g->N, 4, N))) return rc;
     if (energy) MH_HIP(hipMemcpy(energy, g->energy.p, 8 * R, hipMemcpyDeviceToHost));
-    if (sumN) MH_HIP(hipMemcpy(sumN, g->sumN.p, 8 * R, hipMemcpyDeviceToHost));
+    if (sumN && (rc = spc_gather(g, g->sumN, 8, sumN))) return rc;
     if (nsteps) MH_HIP(hipMemcpy(nsteps, g->nsteps.p, 8 * R, hipMemcpyDeviceToHost));
-    if (att) MH_HIP(hipMemcpy(att, g->att.p, 32 * R, hipMemcpyDeviceToHost));
-    if (acc) MH_HIP(hipMemcpy(acc, g->acc.p, 32 * R, hipMemcpyDeviceToHost));
+    if (att && (rc = spc_gather(g, g->att, 32, att))) return rc;
+    if (acc && (rc = spc_gather(g, g->acc, 32, acc))) return rc;
     if (rng_ix) MH_HIP(hipMemcpy(rng_ix, g->ix.p, 4 * R, hipMemcpyDeviceToHost));
     if (rng_iy) MH_HIP(hipMemcpy(rng_iy, g->iy.p, 4 * R, hipMemcpyDeviceToHost));
     return MH_OK;
@@ -1990,9 +2178,9 @@ int mh_gcmc_get_state(mh_gcmc *g, int32_t *N, double *energy, double *sumN, int6
 int mh_gcmc_get_moments(mh_gcmc *g, double *sumN, double *sumN2) {
     MH_CHECK(g, MH_EINVAL, "mh_gcmc_get_moments: g is NULL");
     MH_HIP(hipSetDevice(g->ctx->device));
-    const size_t R = g->nrep;
-    if (sumN) MH_HIP(hipMemcpy(sumN, g->sumN.p, 8 * R, hipMemcpyDeviceToHost));
-    if (sumN2) MH_HIP(hipMemcpy(sumN2, g->sumN2.p, 8 * R, hipMemcpyDeviceToHost));
+    int rc;
+    if (sumN && (rc = spc_gather(g, g->sumN, 8, sumN))) return rc;
+    if (sumN2 && (rc = spc_gather(g, g->sumN2, 8, sumN2))) return rc;
     return MH_OK;
 }
 
@@ -2010,8 +2198,7 @@ int mh_gcmc_get_deltas(mh_gcmc *g, double *delta) {
     MH_CHECK(g && delta, MH_EINVAL, "mh_gcmc_get_deltas: bad arguments");
     MH_HIP(hipSetDevice(g->ctx->device));
     MH_HIP(hipStreamSynchronize(g->ctx->stream));
-    MH_HIP(hipMemcpy(delta, g->delta.p, 16 * (size_t)g->nrep, hipMemcpyDeviceToHost));
-    return MH_OK;
+    return spc_gather(g, g->delta, 16, delta);
 }
 
 int mh_gcmc_set_deltas(mh_gcmc *g, int rep, double delta_trans, double delta_rot) {
@@ -2020,7 +2207,7 @@ int mh_gcmc_set_deltas(mh_gcmc *g, int rep, double delta_trans, double delta_rot
     MH_HIP(hipSetDevice(g->ctx->device));
     MH_HIP(hipStreamSynchronize(g->ctx->stream));
     const double d2[2] = {delta_trans, delta_rot};
-    MH_HIP(hipMemcpy(g->delta.as<double>() + 2 * rep, d2, 16, hipMemcpyHostToDevice));
+    MH_HIP(hipMemcpy(g->delta.as<double>() + 2 * (GSPC * rep + g->sel), d2, 16, hipMemcpyHostToDevice));
     return MH_OK;
 }
 
@@ -2033,8 +2220,8 @@ int mh_gcmc_set_counts(mh_gcmc *g, int rep, const int64_t *att, const int64_t *a
     MH_HIP(hipStreamSynchronize(g->ctx->stream));
     long long a[4], c[4];
     for (int i = 0; i < 4; ++i) a[i] = att[i], c[i] = acc[i];
-    MH_HIP(hipMemcpy(g->att.as<long long>() + 4 * rep, a, 32, hipMemcpyHostToDevice));
-    MH_HIP(hipMemcpy(g->acc.as<long long>() + 4 * rep, c, 32, hipMemcpyHostToDevice));
+    MH_HIP(hipMemcpy(g->att.as<long long>() + 4 * (GSPC * rep + g->sel), a, 32, hipMemcpyHostToDevice));
+    MH_HIP(hipMemcpy(g->acc.as<long long>() + 4 * (GSPC * rep + g->sel), c, 32, hipMemcpyHostToDevice));
     return MH_OK;
 }
 
@@ -2042,14 +2229,16 @@ int mh_gcmc_get_config(mh_gcmc *g, int rep, int *n, double *com, double *ang, do
     MH_CHECK(g && rep >= 0 && rep < g->nrep && n, MH_EINVAL, "mh_gcmc_get_config: bad arguments");
     MH_HIP(hipSetDevice(g->ctx->device));
     int hn = 0;
-    MH_HIP(hipMemcpy(&hn, g->N.as<int>() + rep, 4, hipMemcpyDeviceToHost));
+    MH_HIP(hipMemcpy(&hn, g->N.as<int>() + GSPC * rep + g->sel, 4, hipMemcpyDeviceToHost));
     *n = hn;
-    const long long off3 = (long long)rep * g->maxN * 3, offp = (long long)rep * g->maxN * g->ns * 3;
+    const long long slot0 = (long long)rep * g->nspc * g->maxN + (long long)g->sel * g->maxN;
+    const long long off3 = slot0 * 3, offp = slot0 * g->ns * 3;
+    const int ns = g->sys_of(rep).sp[g->sel].ns;
     if (hn && com) MH_HIP(hipMemcpy(com, g->com.as<double>() + off3, sizeof(double) * 3 * hn, hipMemcpyDeviceToHost));
     if (hn && ang) MH_HIP(hipMemcpy(ang, g->ang.as<double>() + off3, sizeof(double) * 3 * hn, hipMemcpyDeviceToHost));
     if (hn && r)
-        MH_HIP(hipMemcpy(r, g->pos.as<double>() + offp, sizeof(double) * 3 * hn * g->sys_of(rep).ns,
-                         hipMemcpyDeviceToHost));
+        MH_HIP(hipMemcpy2D(r, sizeof(double) * 3 * ns, g->pos.as<double>() + offp, sizeof(double) * 3 * g->ns,
+                           sizeof(double) * 3 * ns, hn, hipMemcpyDeviceToHost));
     return MH_OK;
 }
 
@@ -2131,11 +2320,11 @@ int mh_trial_energy(mh_gcmc *g, int rep, int n, const double *com, const double 
     Rep r = g->rep();
     if (g->prec == MH_PREC_FP32)
         hipLaunchKernelGGL((k_trial<256, MH_PREC_FP32>), dim3(n), dim3(256), 0, g->ctx->stream, g->dP.as<GDev>(), r,
-                           rep, n, dc.as<double>(), da.as<double>(), dsb.as<double2>(), de.as<double>(),
+                           rep, g->sel, n, dc.as<double>(), da.as<double>(), dsb.as<double2>(), de.as<double>(),
                            dok.as<uint8_t>());
     else
         hipLaunchKernelGGL((k_trial<256, MH_PREC_FP64>), dim3(n), dim3(256), 0, g->ctx->stream, g->dP.as<GDev>(), r,
-                           rep, n, dc.as<double>(), da.as<double>(), dsb.as<double2>(), de.as<double>(),
+                           rep, g->sel, n, dc.as<double>(), da.as<double>(), dsb.as<double2>(), de.as<double>(),
                            dok.as<uint8_t>());
     MH_HIP(hipGetLastError());
     MH_HIP(hipStreamSynchronize(g->ctx->stream));

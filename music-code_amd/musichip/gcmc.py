@@ -38,6 +38,8 @@ def _sig():
     P = ctypes.POINTER
     sig = {
         "mh_gcmc_create": [_vp, _vp, _i, _vp, _vp, _i, P(_vp)],
+        "mh_gcmc_create_mix": [_vp, _vp, _i, _i, _vp, _vp, _i, P(_vp)],
+        "mh_gcmc_species": [_vp, _i],
         "mh_gcmc_destroy": [_vp],
         "mh_gcmc_set_threads": [_vp, _i],
         "mh_gcmc_set_config": [_vp, _i, _i, _vp, _vp],
@@ -248,6 +250,47 @@ class GcmcBatch:
             self.close()
         except Exception:
             pass
+
+
+SHARED_KEYS = ("nst", "lj", "coul", "ewald", "T", "full_ell", "fund_ell")
+
+
+class GcmcMix(GcmcBatch):
+    """Replicas of a mixture (gcmc_dosim's species draw every step,
+    gcmc.F90:240-242): specs[i] and maps[i] per species, the pair tables,
+    Coulomb / Ewald settings and temperature from specs[0] (site types
+    index its tables); fugacities [nrep][nspc].  The per-species calls
+    (state, config, set_config, deltas, counts, trial_energy) address the
+    species chosen with select()."""
+
+    def __init__(self, ctx, specs, fugacities, seeds, maxN=1024, maps=None,
+                 precision=H.MH_PREC_FP64, threads=None):
+        L = _sig()
+        self.ctx = ctx
+        self.nspc = len(specs)
+        shared = {k: specs[0][k] for k in SHARED_KEYS if k in specs[0]}
+        maps = maps or [None] * self.nspc
+        plist = [make_params(dict(sp, **shared), maps[i], precision) for i, sp in enumerate(specs)]
+        self.params = (GcmcParams * self.nspc)(*plist)
+        self.ns_list = [p.nsites for p in plist]
+        f = np.ascontiguousarray(fugacities, dtype=np.float64).reshape(-1, self.nspc)
+        self.nrep = len(f)
+        self.ns = self.ns_list[0]
+        self.maxN = maxN
+        s = np.ascontiguousarray(seeds, dtype=np.int32)
+        h = _vp()
+        H.check(L.mh_gcmc_create_mix(ctx.h, self.params, self.nspc, self.nrep, _p(f), _p(s), maxN,
+                                     ctypes.byref(h)), "mh_gcmc_create_mix")
+        self.h = h
+        H._LIVE["gcmc"].add(self)
+        self.maps = maps
+        self.bias_map = None
+        if threads:
+            self.set_threads(threads)
+
+    def select(self, spc):
+        H.check(_sig().mh_gcmc_species(self.h, int(spc)), "mh_gcmc_species")
+        self.ns = self.ns_list[spc]
 
 
 class Pool(GcmcBatch):

@@ -110,14 +110,14 @@ static inline int slanted(const mo_gcmc_sys *s)
     return s->cell && !s->cell->ortho;
 }
 
-/* simcell_pbcarray (simcell.F90:521-547) */
-static void pbc(const mo_gcmc_sys *s, const double *rp, double *r)
+/* simcell_pbcarray (simcell.F90:521-547) of ns sites in the cell of s */
+static void pbc(const mo_gcmc_sys *s, int ns, const double *rp, double *r)
 {
     if (slanted(s)) {
-        for (int a = 0; a < s->ns; a++) mo_cell_pbc(s->cell, rp + 3 * a, r + 3 * a);
+        for (int a = 0; a < ns; a++) mo_cell_pbc(s->cell, rp + 3 * a, r + 3 * a);
         return;
     }
-    for (int a = 0; a < s->ns; a++)
+    for (int a = 0; a < ns; a++)
         for (int i = 0; i < 3; i++) {
             double c = rp[3 * a + i];
             double idx = floor(c / s->full_ell[i]);
@@ -154,7 +154,8 @@ void mo_gcmc_kvecs(const mo_gcmc_sys *s, int *nk, double *kv, double *w)
     *nk = n;
 }
 
-/* structure factor contribution of one molecule: sre/sim[k] = sum_a q_a e^{ik.r_a} */
+/* structure factor contribution of one molecule of species s (k vectors of
+ * the global state st): sre/sim[k] = sum_a q_a e^{ik.r_a} */
 static void mol_sk(const mo_gcmc_sys *s, const mo_gcmc_state *st, const double *r, double *sre, double *sim)
 {
     for (int k = 0; k < st->nk; k++) {
@@ -209,21 +210,26 @@ double mo_gcmc_mol_const(const mo_gcmc_sys *s)
 }
 
 /* ---------------------------------------------------- molecule energy */
-/* Energy of a molecule with wrapped site coordinates r against the
- * framework maps and every molecule except 'skip'.  Returns 0 on failure. */
-static int mol_energy(const mo_gcmc_sys *s, const mo_gcmc_state *st, const double *r, int skip,
-                      double *umap, double *upair)
+/* Energy of a molecule of species sp (sys[sp]) with wrapped site coordinates
+ * r against the framework maps of its species and every molecule of every
+ * species except molecule 'skip' of species sp.  The pair tables, cell and
+ * Ewald / Coulomb settings are those of sys[0] (shared by the species; site
+ * types index the common table).  Returns 0 on failure. */
+static int mol_energy_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *const *sts, int sp,
+                          const double *r, int skip, double *umap, double *upair)
 {
+    const mo_gcmc_sys *s = sys[sp];
+    const mo_gcmc_sys *g = sys[0];
     double um = 0.0;
-    const int sl = slanted(s);
+    const int sl = slanted(g);
     /* pmap_int / interpolate_int (interpolate.F90:663-679): slanted cells
      * interpolate without the periodic wrap of the upper corner */
     int (*herm)(const int *, const double *, const int32_t *, const float *, long long, const double *,
                 double *) = sl ? mo_pothermite_nowrap : mo_pothermite;
     for (int a = 0; a < s->ns; a++) {
         double c[3];
-        if (sl) mo_cell_maptouc(s->cell, r + 3 * a, c);
-        else mo_maptouc(s->fund_ell, r + 3 * a, c);
+        if (sl) mo_cell_maptouc(g->cell, r + 3 * a, c);
+        else mo_maptouc(g->fund_ell, r + 3 * a, c);
         int pm = s->site_pmap[a];
         if (pm >= 0) {
             double p;
@@ -240,53 +246,59 @@ static int mol_energy(const mo_gcmc_sys *s, const mo_gcmc_state *st, const doubl
         }
     }
     double up = 0.0;
-    for (int m = 0; m < st->N; m++) {
-        if (m == skip) continue;
-        const double *ro = st->r + 3 * s->ns * m;
-        for (int a = 0; a < s->ns; a++)
-            for (int b = 0; b < s->ns; b++) {
-                double sep[3];
-                if (sl) mo_cell_minimage(s->cell, r + 3 * a, ro + 3 * b, sep);
-                else mo_minimage(s->full_ell, r + 3 * a, ro + 3 * b, sep);
-                double r2 = 0.0;
-                r2 = r2 + sep[0] * sep[0];
-                r2 = r2 + sep[1] * sep[1];
-                r2 = r2 + sep[2] * sep[2];
-                int ta = s->stype[a], tb = s->stype[b];
-                int t = ta * s->nst + tb;
-                if (s->lj_on[t]) {
-                    if (r2 <= s->lj_h2[t]) {
-                        if (r2 < s->lj_l2[t]) return 0;
-                        double r2i = 1.0 / r2;
-                        double r6i = r2i * r2i * r2i;
-                        double r12i = r6i * r6i;
-                        up = up + (s->lj_A[t] * r12i - s->lj_B[t] * r6i);
+    for (int s2 = 0; s2 < nspc; s2++) {
+        const mo_gcmc_sys *o = sys[s2];
+        const mo_gcmc_state *st = sts[s2];
+        for (int m = 0; m < st->N; m++) {
+            if (s2 == sp && m == skip) continue;
+            const double *ro = st->r + 3 * o->ns * m;
+            for (int a = 0; a < s->ns; a++)
+                for (int b = 0; b < o->ns; b++) {
+                    double sep[3];
+                    if (sl) mo_cell_minimage(g->cell, r + 3 * a, ro + 3 * b, sep);
+                    else mo_minimage(g->full_ell, r + 3 * a, ro + 3 * b, sep);
+                    double r2 = 0.0;
+                    r2 = r2 + sep[0] * sep[0];
+                    r2 = r2 + sep[1] * sep[1];
+                    r2 = r2 + sep[2] * sep[2];
+                    int ta = s->stype[a], tb = o->stype[b];
+                    int t = ta * g->nst + tb;
+                    if (g->lj_on[t]) {
+                        if (r2 <= g->lj_h2[t]) {
+                            if (r2 < g->lj_l2[t]) return 0;
+                            double r2i = 1.0 / r2;
+                            double r6i = r2i * r2i * r2i;
+                            double r12i = r6i * r6i;
+                            up = up + (g->lj_A[t] * r12i - g->lj_B[t] * r6i);
+                        }
+                    }
+                    if (g->ewald && r2 < g->rc2_real && s->sq[a] != 0.0 && o->sq[b] != 0.0) {
+                        double rr = sqrt(r2);
+                        up = up + s->sq[a] * o->sq[b] * erfc_as(g->kappa * rr) / rr * MO_TOKCAL;
+                    }
+                    if (g->coul && g->coul_on[t]) {
+                        double e;
+                        if (!mo_coul_pair(g->coul_hicut, g->coul_locut, g->coul_shield, r2, s->sq[a], o->sq[b], &e))
+                            return 0;
+                        up = up + e;
                     }
                 }
-                if (s->ewald && r2 < s->rc2_real && s->sq[a] != 0.0 && s->sq[b] != 0.0) {
-                    double rr = sqrt(r2);
-                    up = up + s->sq[a] * s->sq[b] * erfc_as(s->kappa * rr) / rr * MO_TOKCAL;
-                }
-                if (s->coul && s->coul_on[t]) {
-                    double e;
-                    if (!mo_coul_pair(s->coul_hicut, s->coul_locut, s->coul_shield, r2, s->sq[a], s->sq[b], &e))
-                        return 0;
-                    up = up + e;
-                }
-            }
+        }
     }
     *umap = um;
     *upair = up;
     return 1;
 }
 
-static void record(mo_gcmc_state *st, int step, int mtype, int acc, double de)
+/* trace of step 'step' in the global state g: move = 16 species + move
+ * type, N = molecules of the moved species after the step */
+static void record(mo_gcmc_state *g, int step, int spc, int mtype, int acc, int N, double de)
 {
-    if (!st->trace || step >= st->trace_cap) return;
-    mo_gcmc_trace *t = st->trace + step;
-    t->move = mtype;
+    if (!g->trace || step >= g->trace_cap) return;
+    mo_gcmc_trace *t = g->trace + step;
+    t->move = 16 * spc + mtype;
     t->accepted = acc;
-    t->N = st->N;
+    t->N = N;
     t->dE = de;
 }
 
@@ -424,24 +436,29 @@ static double adjust_delta(double d, double ratio)
     return d;
 }
 
-int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
+int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *const *sts, long long nsteps)
 {
-    const int ns = s->ns;
-    double rnew[24], rpnew[24], sre_new[1], sim_new[1];
-    (void)sre_new;
-    (void)sim_new;
+    const mo_gcmc_sys *g = sys[0];
+    mo_gcmc_state *G = sts[0];
+    double rnew[3 * MO_MAXS], rpnew[3 * MO_MAXS];
     double *ak = NULL, *bk = NULL, *ck = NULL, *dk = NULL;
-    if (s->ewald) {
-        ak = malloc(sizeof(double) * st->nk);
-        bk = malloc(sizeof(double) * st->nk);
-        ck = malloc(sizeof(double) * st->nk);
-        dk = malloc(sizeof(double) * st->nk);
+    if (g->ewald) {
+        ak = malloc(sizeof(double) * G->nk);
+        bk = malloc(sizeof(double) * G->nk);
+        ck = malloc(sizeof(double) * G->nk);
+        dk = malloc(sizeof(double) * G->nk);
     }
-    const double mconst = mo_gcmc_mol_const(s);
+    double mconst_s[MO_MAXSPC];
+    for (int i = 0; i < nspc; i++) mconst_s[i] = mo_gcmc_mol_const(sys[i]);
     for (long long it = 0; it < nsteps; it++) {
-        int step = (int)st->nsteps;
-        (void)(int)(mo_rranf(&st->rng) * 1.0); /* species draw, nspc = 1 (gcmc.F90:242) */
-        double prob = mo_rranf(&st->rng);
+        int step = (int)G->nsteps;
+        /* gcmc_dosim: a species with equal probability (gcmc.F90:240-242) */
+        const int sp = (int)(mo_rranf(&G->rng) * (double)nspc);
+        const mo_gcmc_sys *s = sys[sp];
+        mo_gcmc_state *st = sts[sp];
+        const int ns = s->ns;
+        const double mconst = mconst_s[sp];
+        double prob = mo_rranf(&G->rng);
         int mv = 0;
         for (int i = 0; i < s->nmoves; i++) {
             if (prob > s->cumwt[i]) mv++;
@@ -460,74 +477,74 @@ int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
                 double biasfactor = 1.0;
                 if (mtype == MO_MV_INSERT) {
                     /* simcell_uniformRandPt (simcell.F90:274-295) */
-                    if (slanted(s)) mo_cell_uniform(s->cell, &st->rng, com);
+                    if (slanted(g)) mo_cell_uniform(g->cell, &G->rng, com);
                     else
-                        for (int i = 0; i < 3; i++) com[i] = s->full_ell[i] * mo_rranf(&st->rng);
+                        for (int i = 0; i < 3; i++) com[i] = g->full_ell[i] * mo_rranf(&G->rng);
                 } else {
                     /* rigidmoves_comFromBmap: a cubelet by its weight, a point
                      * in it (bmap_getbiaspt), a unit cell of the simulation cell */
                     const mo_bmap *b = s->bias;
-                    long long t = mo_bmap_index(b, mo_rranf(&st->rng));
+                    long long t = mo_bmap_index(b, mo_rranf(&G->rng));
                     long long c = b->cube[t];
                     long long ci[3] = {c / ((long long)b->ncub[1] * b->ncub[2]), (c / b->ncub[2]) % b->ncub[1],
                                        c % b->ncub[2]};
                     double wt = b->wt[t] / b->cum_wt;
                     biasfactor = 1.0 / wt / (double)b->ncubelets;
                     for (int i = 0; i < 3; i++) com[i] = (double)ci[i] * b->step[i];
-                    for (int i = 0; i < 3; i++) com[i] = com[i] + mo_rranf(&st->rng) * b->step[i];
+                    for (int i = 0; i < 3; i++) com[i] = com[i] + mo_rranf(&G->rng) * b->step[i];
                     for (int i = 0; i < 3; i++)
                         if (com[i] >= b->eff[i]) com[i] = (double)0.9999990f * b->eff[i];
                     for (int i = 0; i < 3; i++) {
-                        int nc = (int)lround(s->full_ell[i] / s->fund_ell[i]);
-                        int icell = (int)(mo_rranf(&st->rng) * nc);
-                        com[i] = com[i] + s->fund_ell[i] * icell;
+                        int nc = (int)lround(g->full_ell[i] / g->fund_ell[i]);
+                        int icell = (int)(mo_rranf(&G->rng) * nc);
+                        com[i] = com[i] + g->fund_ell[i] * icell;
                     }
                 }
                 if (ns != 1) {
-                    double costheta = 1.0 - mo_rranf(&st->rng) * 2.0;
+                    double costheta = 1.0 - mo_rranf(&G->rng) * 2.0;
                     ang[1] = acos(costheta);
-                    ang[0] = MO_PI - mo_rranf(&st->rng) * MO_TWOPI;
-                    ang[2] = MO_PI - mo_rranf(&st->rng) * MO_TWOPI;
+                    ang[0] = MO_PI - mo_rranf(&G->rng) * MO_TWOPI;
+                    ang[2] = MO_PI - mo_rranf(&G->rng) * MO_TWOPI;
                 }
                 toxyz(s, com, ang, rpnew);
-                pbc(s, rpnew, rnew);
+                pbc(g, ns, rpnew, rnew);
                 double um, up;
-                int ok = mol_energy(s, st, rnew, -1, &um, &up);
+                int ok = mol_energy_mix(sys, nspc, sts, sp, rnew, -1, &um, &up);
                 double U = 0.0;
                 if (ok) {
                     U = um + up;
-                    if (s->ewald) {
-                        mol_sk(s, st, rnew, ak, bk);
-                        U = U + drecip(st, ak, bk, 1.0, NULL, NULL, 0.0) + mconst;
+                    if (g->ewald) {
+                        mol_sk(s, G, rnew, ak, bk);
+                        U = U + drecip(G, ak, bk, 1.0, NULL, NULL, 0.0) + mconst;
                     }
                     if (U > s->max_nrg) ok = 0;
                 }
                 if (ok) {
                     double bias = biasfactor / s->insdelratio;
-                    accepted = attempt(s, st, U, bias, 1, st->B, 1, 1.0 / (double)(float)(N + 1));
+                    accepted = attempt(s, G, U, bias, 1, st->B, 1, 1.0 / (double)(float)(N + 1));
                     if (accepted) {
                         memcpy(st->com + 3 * N, com, sizeof com);
                         memcpy(st->ang + 3 * N, ang, sizeof ang);
                         memcpy(st->rp + 3 * ns * N, rpnew, sizeof(double) * 3 * ns);
                         memcpy(st->r + 3 * ns * N, rnew, sizeof(double) * 3 * ns);
-                        if (s->ewald)
-                            for (int k = 0; k < st->nk; k++) {
-                                st->Sre[k] += ak[k];
-                                st->Sim[k] += bk[k];
+                        if (g->ewald)
+                            for (int k = 0; k < G->nk; k++) {
+                                G->Sre[k] += ak[k];
+                                G->Sim[k] += bk[k];
                             }
                         st->N = N + 1;
-                        st->energy += U;
+                        G->energy += U;
                         st->acc[0]++;
                         de = U;
                     }
                 }
             }
-            record(st, step, mtype, accepted, de);
+            record(G, step, sp, mtype, accepted, st->N, de);
         } else if (mtype == MO_MV_DELETE || mtype == MO_MV_BDELETE) {
             int accepted = 0;
             double de = 0.0;
             if (N != 0) {
-                int mol = (int)(mo_rranf(&st->rng) * N);
+                int mol = (int)(mo_rranf(&G->rng) * N);
                 st->att[1]++;
                 double biasfactor = 1.0;
                 if (mtype == MO_MV_BDELETE) {
@@ -536,7 +553,7 @@ int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
                      * outside the bias map stops the reference */
                     const mo_bmap *b = s->bias;
                     double uc[3];
-                    mo_maptouc(s->fund_ell, st->com + 3 * mol, uc);
+                    mo_maptouc(g->fund_ell, st->com + 3 * mol, uc);
                     long long ci[3];
                     for (int i = 0; i < 3; i++) {
                         ci[i] = (long long)(uc[i] / b->step[i]);
@@ -549,57 +566,57 @@ int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
                 }
                 const double *r = st->r + 3 * ns * mol;
                 double um, up;
-                int ok = mol_energy(s, st, r, mol, &um, &up);
+                int ok = mol_energy_mix(sys, nspc, sts, sp, r, mol, &um, &up);
                 double U = um + up;
-                if (ok && s->ewald) {
-                    mol_sk(s, st, r, ak, bk);
-                    U = U - drecip(st, ak, bk, -1.0, NULL, NULL, 0.0) + mconst;
+                if (ok && g->ewald) {
+                    mol_sk(s, G, r, ak, bk);
+                    U = U - drecip(G, ak, bk, -1.0, NULL, NULL, 0.0) + mconst;
                 }
                 if (!ok) return -1; /* mcmoves.F90:599-603: an existing molecule must evaluate */
                 double bias = biasfactor * s->insdelratio;
-                accepted = attempt(s, st, -U, bias, 1, -st->B, 1, 1.0 * N);
+                accepted = attempt(s, G, -U, bias, 1, -st->B, 1, 1.0 * N);
                 if (accepted) {
                     int last = N - 1;
                     memmove(st->com + 3 * mol, st->com + 3 * last, sizeof(double) * 3);
                     memmove(st->ang + 3 * mol, st->ang + 3 * last, sizeof(double) * 3);
                     memmove(st->rp + 3 * ns * mol, st->rp + 3 * ns * last, sizeof(double) * 3 * ns);
                     memmove(st->r + 3 * ns * mol, st->r + 3 * ns * last, sizeof(double) * 3 * ns);
-                    if (s->ewald)
-                        for (int k = 0; k < st->nk; k++) {
-                            st->Sre[k] -= ak[k];
-                            st->Sim[k] -= bk[k];
+                    if (g->ewald)
+                        for (int k = 0; k < G->nk; k++) {
+                            G->Sre[k] -= ak[k];
+                            G->Sim[k] -= bk[k];
                         }
                     st->N = N - 1;
-                    st->energy -= U;
+                    G->energy -= U;
                     st->acc[1]++;
                     de = -U;
                 }
             }
-            record(st, step, mtype, accepted, de);
+            record(G, step, sp, mtype, accepted, st->N, de);
         } else {
             /* TRANSLATE / ROTATE */
             int accepted = 0;
             double de = 0.0;
             int idx = (mtype == MO_MV_TRANSLATE) ? 2 : 3;
             if (N != 0) {
-                int mol = (int)(mo_rranf(&st->rng) * N);
+                int mol = (int)(mo_rranf(&G->rng) * N);
                 st->att[idx]++;
                 const double *rold = st->r + 3 * ns * mol;
                 double um0, up0;
-                if (!mol_energy(s, st, rold, mol, &um0, &up0)) return -2;
+                if (!mol_energy_mix(sys, nspc, sts, sp, rold, mol, &um0, &up0)) return -2;
                 double com[3], ang[3];
                 memcpy(com, st->com + 3 * mol, sizeof com);
                 memcpy(ang, st->ang + 3 * mol, sizeof ang);
                 if (mtype == MO_MV_TRANSLATE) {
                     double dv[3];
                     const double dtr = s->adjust_trans ? st->delta[0] : s->dtrans;
-                    for (int i = 0; i < 3; i++) dv[i] = ((mo_rranf(&st->rng) * 2.0) - 1.0) * dtr;
+                    for (int i = 0; i < 3; i++) dv[i] = ((mo_rranf(&G->rng) * 2.0) - 1.0) * dtr;
                     for (int i = 0; i < 3; i++) com[i] = com[i] + dv[i];
                 } else {
                     const double drt = s->adjust_rot ? st->delta[1] : s->drot;
-                    double dphi = ((mo_rranf(&st->rng) * 2.0) - 1.0) * drt;
-                    double dpsi = ((mo_rranf(&st->rng) * 2.0) - 1.0) * drt;
-                    double dcos = ((mo_rranf(&st->rng) * 2.0) - 1.0) * drt;
+                    double dphi = ((mo_rranf(&G->rng) * 2.0) - 1.0) * drt;
+                    double dpsi = ((mo_rranf(&G->rng) * 2.0) - 1.0) * drt;
+                    double dcos = ((mo_rranf(&G->rng) * 2.0) - 1.0) * drt;
                     if (ns != 1) {
                         double phi = ang[0] + dphi;
                         double psi = ang[2] + dpsi;
@@ -611,33 +628,33 @@ int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
                     }
                 }
                 toxyz(s, com, ang, rpnew);
-                pbc(s, rpnew, rnew);
+                pbc(g, ns, rpnew, rnew);
                 double um1, up1;
-                int ok = mol_energy(s, st, rnew, mol, &um1, &up1);
+                int ok = mol_energy_mix(sys, nspc, sts, sp, rnew, mol, &um1, &up1);
                 double dU = 0.0;
                 if (ok) {
                     double U1 = um1 + up1, U0 = um0 + up0;
                     if (U1 > s->max_nrg) ok = 0;
                     dU = U1 - U0;
-                    if (ok && s->ewald) {
-                        mol_sk(s, st, rold, ck, dk);
-                        mol_sk(s, st, rnew, ak, bk);
-                        dU = dU + drecip(st, ak, bk, 1.0, ck, dk, -1.0);
+                    if (ok && g->ewald) {
+                        mol_sk(s, G, rold, ck, dk);
+                        mol_sk(s, G, rnew, ak, bk);
+                        dU = dU + drecip(G, ak, bk, 1.0, ck, dk, -1.0);
                     }
                 }
                 if (ok) {
-                    accepted = attempt(s, st, dU, 1.0, 0, 0.0, 0, 1.0);
+                    accepted = attempt(s, G, dU, 1.0, 0, 0.0, 0, 1.0);
                     if (accepted) {
                         memcpy(st->com + 3 * mol, com, sizeof com);
                         memcpy(st->ang + 3 * mol, ang, sizeof ang);
                         memcpy(st->rp + 3 * ns * mol, rpnew, sizeof(double) * 3 * ns);
                         memcpy(st->r + 3 * ns * mol, rnew, sizeof(double) * 3 * ns);
-                        if (s->ewald)
-                            for (int k = 0; k < st->nk; k++) {
-                                st->Sre[k] += ak[k] - ck[k];
-                                st->Sim[k] += bk[k] - dk[k];
+                        if (g->ewald)
+                            for (int k = 0; k < G->nk; k++) {
+                                G->Sre[k] += ak[k] - ck[k];
+                                G->Sim[k] += bk[k] - dk[k];
                             }
-                        st->energy += dU;
+                        G->energy += dU;
                         st->acc[idx]++;
                         de = dU;
                     }
@@ -648,10 +665,11 @@ int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
                 else if (mtype != MO_MV_TRANSLATE && s->adjust_rot)
                     st->delta[1] = adjust_delta(st->delta[1], (double)st->acc[3] / (double)st->att[3]);
             }
-            record(st, step, mtype, accepted, de);
+            record(G, step, sp, mtype, accepted, st->N, de);
         }
-        st->sumN += st->N;
-        st->nsteps++;
+        /* gcmcmoves_updatenmolecs: every species after every step */
+        for (int i = 0; i < nspc; i++) sts[i]->sumN += sts[i]->N;
+        G->nsteps++;
     }
     free(ak);
     free(bk);
@@ -660,49 +678,92 @@ int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
     return 0;
 }
 
-/* full-system energy (checks the running total, interact_checknrgs) */
-double mo_gcmc_total_energy(const mo_gcmc_sys *s, const mo_gcmc_state *st)
+/* full-system energy (checks the running total, interact_checknrgs): map
+ * and pair terms of every molecule, each pair once (molecules of earlier
+ * species and earlier molecules of the same species), reciprocal sum and
+ * the per-molecule Ewald constants */
+double mo_gcmc_total_energy_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *const *sts)
 {
     double e = 0.0;
-    mo_gcmc_state tmp = *st;
-    /* map terms and pair terms, each pair once */
-    for (int m = 0; m < st->N; m++) {
-        tmp.N = m; /* molecules 0..m-1 only */
-        double um, up;
-        if (!mol_energy(s, &tmp, st->r + 3 * s->ns * m, -1, &um, &up)) return NAN; /* a failed evaluation */
-        e += um + up;
+    int keepN[MO_MAXSPC];
+    mo_gcmc_state tmp[MO_MAXSPC];
+    mo_gcmc_state *tp[MO_MAXSPC];
+    for (int i = 0; i < nspc; i++) {
+        keepN[i] = sts[i]->N;
+        tmp[i] = *sts[i];
+        tp[i] = &tmp[i];
     }
-    if (s->ewald) {
+    for (int sp = 0; sp < nspc; sp++) {
+        for (int i = 0; i < nspc; i++) tmp[i].N = i < sp ? keepN[i] : 0;
+        for (int m = 0; m < keepN[sp]; m++) {
+            tmp[sp].N = m; /* molecules 0..m-1 of this species */
+            double um, up;
+            if (!mol_energy_mix(sys, nspc, tp, sp, sts[sp]->r + 3 * sys[sp]->ns * m, -1, &um, &up))
+                return NAN; /* a failed evaluation */
+            e += um + up;
+        }
+    }
+    if (sys[0]->ewald) {
+        const mo_gcmc_state *G = sts[0];
         double er = 0.0;
-        for (int k = 0; k < st->nk; k++) er += st->wk[k] * (st->Sre[k] * st->Sre[k] + st->Sim[k] * st->Sim[k]);
-        e += er + st->N * mo_gcmc_mol_const(s);
+        for (int k = 0; k < G->nk; k++) er += G->wk[k] * (G->Sre[k] * G->Sre[k] + G->Sim[k] * G->Sim[k]);
+        e += er;
+        for (int i = 0; i < nspc; i++) e += sts[i]->N * mo_gcmc_mol_const(sys[i]);
     }
     return e;
 }
 
-void mo_gcmc_recompute_sk(const mo_gcmc_sys *s, mo_gcmc_state *st)
+void mo_gcmc_recompute_sk_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *const *sts)
 {
-    for (int k = 0; k < st->nk; k++) st->Sre[k] = st->Sim[k] = 0.0;
-    if (!s->ewald) return;
-    double *a = malloc(sizeof(double) * st->nk), *b = malloc(sizeof(double) * st->nk);
-    for (int m = 0; m < st->N; m++) {
-        mol_sk(s, st, st->r + 3 * s->ns * m, a, b);
-        for (int k = 0; k < st->nk; k++) {
-            st->Sre[k] += a[k];
-            st->Sim[k] += b[k];
+    mo_gcmc_state *G = sts[0];
+    for (int k = 0; k < G->nk; k++) G->Sre[k] = G->Sim[k] = 0.0;
+    if (!sys[0]->ewald) return;
+    double *a = malloc(sizeof(double) * G->nk), *b = malloc(sizeof(double) * G->nk);
+    for (int sp = 0; sp < nspc; sp++)
+        for (int m = 0; m < sts[sp]->N; m++) {
+            mol_sk(sys[sp], G, sts[sp]->r + 3 * sys[sp]->ns * m, a, b);
+            for (int k = 0; k < G->nk; k++) {
+                G->Sre[k] += a[k];
+                G->Sim[k] += b[k];
+            }
         }
-    }
     free(a);
     free(b);
 }
 
-/* place molecule m at com/ang (for seeded initial loadings) */
-void mo_gcmc_set_mol(const mo_gcmc_sys *s, mo_gcmc_state *st, int m, const double com[3], const double ang[3])
+/* place molecule m of species s (sys[s], sts[s]) at com/ang (seeded
+ * loadings); the cell is sys[0]'s */
+void mo_gcmc_set_mol_mix(const mo_gcmc_sys *const *sys, mo_gcmc_state *const *sts, int sp, int m,
+                         const double com[3], const double ang[3])
 {
+    const mo_gcmc_sys *s = sys[sp];
+    mo_gcmc_state *st = sts[sp];
     memcpy(st->com + 3 * m, com, sizeof(double) * 3);
     memcpy(st->ang + 3 * m, ang, sizeof(double) * 3);
     toxyz(s, com, ang, st->rp + 3 * s->ns * m);
-    pbc(s, st->rp + 3 * s->ns * m, st->r + 3 * s->ns * m);
+    pbc(sys[0], s->ns, st->rp + 3 * s->ns * m, st->r + 3 * s->ns * m);
+}
+
+/* one species: the mixture routines with nspc = 1 */
+int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps)
+{
+    return mo_gcmc_run_mix(&s, 1, &st, nsteps);
+}
+
+double mo_gcmc_total_energy(const mo_gcmc_sys *s, const mo_gcmc_state *st)
+{
+    mo_gcmc_state *p = (mo_gcmc_state *)st;
+    return mo_gcmc_total_energy_mix(&s, 1, &p);
+}
+
+void mo_gcmc_recompute_sk(const mo_gcmc_sys *s, mo_gcmc_state *st)
+{
+    mo_gcmc_recompute_sk_mix(&s, 1, &st);
+}
+
+void mo_gcmc_set_mol(const mo_gcmc_sys *s, mo_gcmc_state *st, int m, const double com[3], const double ang[3])
+{
+    mo_gcmc_set_mol_mix(&s, &st, 0, m, com, ang);
 }
 
 /* gcmc_genfuglist (gcmc.F90:303-319): B = ln(f 1e3 V 1e-30 N_A / (R T)) - ln(Zig) */

@@ -35,6 +35,7 @@ void mo_bmap_free(mo_bmap *b);
 long long mo_bmap_index(const mo_bmap *b, double key);
 #define MO_MAXS 8
 #define MO_MAXMAPS 8
+#define MO_MAXSPC 4
 
 typedef struct {
     double full_ell[3], fund_ell[3];
@@ -95,6 +96,17 @@ typedef struct {
     double delta[2]; /* translate / rotate displacement of adjustable moves */
 } mo_gcmc_state;
 
+/* Mixtures (gcmc_dosim, gcmc.F90:224-258): sys[i] / sts[i] for species i;
+ * the cell, pair tables (site types index a common table), Coulomb / Ewald
+ * settings and temperature are sys[0]'s; sts[0] also holds the RNG, the
+ * energy, S(k), the step count and the trace (move = 16 species + type, N =
+ * molecules of the moved species); sts[i] holds species i's molecules,
+ * counters, displacements and B. */
+int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *const *sts, long long nsteps);
+double mo_gcmc_total_energy_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *const *sts);
+void mo_gcmc_recompute_sk_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *const *sts);
+void mo_gcmc_set_mol_mix(const mo_gcmc_sys *const *sys, mo_gcmc_state *const *sts, int sp, int m,
+                         const double com[3], const double ang[3]);
 void mo_gcmc_kvecs(const mo_gcmc_sys *s, int *nk, double *kv, double *w);
 double mo_gcmc_mol_const(const mo_gcmc_sys *s);
 int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps);

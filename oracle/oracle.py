@@ -424,6 +424,12 @@ def _gcmc_lib():
         L.mo_gcmc_total_energy.restype = _d
         L.mo_gcmc_recompute_sk.argtypes = [_vp, _vp]
         L.mo_gcmc_set_mol.argtypes = [_vp, _vp, _i, _vp, _vp]
+        L.mo_gcmc_run_mix.argtypes = [_vp, _i, _vp, _ll]
+        L.mo_gcmc_run_mix.restype = _i
+        L.mo_gcmc_total_energy_mix.argtypes = [_vp, _i, _vp]
+        L.mo_gcmc_total_energy_mix.restype = _d
+        L.mo_gcmc_recompute_sk_mix.argtypes = [_vp, _i, _vp]
+        L.mo_gcmc_set_mol_mix.argtypes = [_vp, _vp, _i, _i, _vp, _vp]
         L.mo_adams_B.argtypes = [_d, _d, _d, _d]
         L.mo_adams_B.restype = _d
         L.mo_rng_init.argtypes = [_vp, ctypes.c_int32]
@@ -607,6 +613,55 @@ class Gcmc:
     def trace_array(self, n=None):
         n = self.st.trace_cap if n is None else n
         return np.array([(t.move, t.accepted, t.N, t.dE) for t in self.trace[:n]])
+
+
+GLOBAL_KEYS = ("full_ell", "fund_ell", "nst", "lj", "coul", "ewald", "T", "cell")
+
+
+class GcmcMix:
+    """A GCMC chain of several species on the CPU oracle (mo_gcmc_run_mix):
+    specs[i] as for Gcmc; the cell, pair tables (nst site types shared by
+    the species), Coulomb / Ewald and T are specs[0]'s.  fugacities[i] per
+    species.  The trace (RNG, energy, S(k)) lives with species 0."""
+
+    def __init__(self, specs, fugacities, seed, maxN=2048, trace=0):
+        self.nspc = len(specs)
+        glob = {k: specs[0][k] for k in GLOBAL_KEYS if k in specs[0]}
+        self.ch = [Gcmc(dict(sp, **glob), fugacities[i], seed if i == 0 else 1, maxN=maxN,
+                        trace=trace if i == 0 else 0) for i, sp in enumerate(specs)]
+        self.sysp = (_vp * self.nspc)(*[ctypes.cast(ctypes.byref(c.sys), _vp) for c in self.ch])
+        self.stp = (_vp * self.nspc)(*[ctypes.cast(ctypes.byref(c.st), _vp) for c in self.ch])
+
+    def set_config(self, spc_coms, spc_angs=None):
+        """Molecules per species: lists of com [n,3] and angles [n,3]."""
+        L = _gcmc_lib()
+        for i in range(self.nspc):
+            com = np.asarray(spc_coms[i], dtype=np.float64).reshape(-1, 3)
+            ang = np.zeros_like(com) if spc_angs is None else np.asarray(spc_angs[i], dtype=np.float64).reshape(-1, 3)
+            for m in range(len(com)):
+                L.mo_gcmc_set_mol_mix(self.sysp, self.stp, i, m, np.ascontiguousarray(com[m]).ctypes.data,
+                                      np.ascontiguousarray(ang[m]).ctypes.data)
+            self.ch[i].st.N = len(com)
+        L.mo_gcmc_recompute_sk_mix(self.sysp, self.nspc, self.stp)
+        self.ch[0].st.energy = self.total_energy()
+
+    def run(self, nsteps):
+        rc = _gcmc_lib().mo_gcmc_run_mix(self.sysp, self.nspc, self.stp, nsteps)
+        if rc:
+            raise RuntimeError("mo_gcmc_run_mix failed %d" % rc)
+
+    def total_energy(self):
+        return _gcmc_lib().mo_gcmc_total_energy_mix(self.sysp, self.nspc, self.stp)
+
+    @property
+    def energy(self):
+        return self.ch[0].st.energy
+
+    def N(self):
+        return [c.st.N for c in self.ch]
+
+    def trace_array(self, n=None):
+        return self.ch[0].trace_array(n)
 
 
 # ---------------------------------------------------------------- HMC
