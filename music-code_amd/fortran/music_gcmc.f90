@@ -1,6 +1,7 @@
 ! music_gcmc -- GCMC driver over libmusic_hip (usage: music_gcmc ctrlfile
 ! [fp32] [chain] [maxn=N] [check]), the drop-in for the reference's
-! drivers/music_gcmc.F90:79-257 for one rigid species with the rigid
+! drivers/music_gcmc.F90:79-257 for rigid species (one, or a mixture of up
+! to four: every step draws a species, gcmc.F90:240-242) with the rigid
 ! insert / delete / translate / rotate moves and the biased insert / delete
 ! (BINSERT with a bias pmap and temperature, BDELETE; rigidmoves_initInsBias,
 ! rigidmoves.F90:593-675).  It reads the ctrlfile's
@@ -63,7 +64,7 @@ program music_gcmc
   integer :: stypes(MH_MAX_SITES)
   character(len=LL) :: pmapf(MH_MAX_SITES), emapf(MH_MAX_SITES), biasf
   real(c_double) :: bias_T
-  type(c_ptr) :: bmap
+  type(c_ptr) :: bmaps(MH_MAX_SPECIES)
   type(c_ptr) :: ctx, g
   type(c_ptr), allocatable :: maps(:)
   character(len=LL), allocatable :: mapnames(:)
@@ -71,6 +72,16 @@ program music_gcmc
   real(c_double) :: ref(3, MH_MAX_SITES)
   integer(8) :: t0, t1, rate
   real(c_double) :: secs
+  ! the species of a mixture; the scalars above (spname, P, press, ...) hold
+  ! the species 'cur' being read or reported (use_species / save_species)
+  integer :: cur
+  type(mh_gcmc_params), target :: PS(MH_MAX_SPECIES)
+  character(len=NL) :: spnames(MH_MAX_SPECIES), mvnames(4, MH_MAX_SPECIES)
+  integer :: isps(MH_MAX_SPECIES), nmvs(MH_MAX_SPECIES), nsites(MH_MAX_SPECIES)
+  real(c_double), allocatable :: pressm(:,:)
+  character(len=LL) :: pmapfs(MH_MAX_SITES, MH_MAX_SPECIES), emapfs(MH_MAX_SITES, MH_MAX_SPECIES)
+  character(len=LL) :: biasfs(MH_MAX_SPECIES), srcfiles(MH_MAX_SPECIES)
+  real(c_double) :: bias_Ts(MH_MAX_SPECIES)
 
   call get_command_argument(1, ctrl)
   if (len_trim(ctrl) == 0) then
@@ -135,8 +146,12 @@ program music_gcmc
   read(strip(sy%t%line(i + 3)), *) npts
   read(strip(sy%t%line(i + 4)), *) bsize
   read(strip(sy%t%line(i + 5)), *) nspc
-  if (nspc /= 1) call fail('one GCMC species per run is offered')
+  if (nspc < 1 .or. nspc > MH_MAX_SPECIES) call fail('1 to 4 GCMC species are offered')
+  allocate(press(npts), pressm(npts, nspc))
   i = i + 6
+  nst = 0
+  srcfile = ''
+  do cur = 1, nspc
   do while (i <= sy%t%n)
      line = strip(sy%t%line(i))
      if (len_trim(line) > 0 .and. verify(trim(line), '-') /= 0) exit
@@ -144,8 +159,10 @@ program music_gcmc
   end do
   spname = word(strip(sy%t%line(i)), 1)
   isp = mol_index(sy, spname)
+  do k = 1, cur - 1
+     if (trim(spnames(k)) == trim(spname)) call fail('species ' // trim(spname) // ' listed twice')
+  end do
   i = i + 1
-  allocate(press(npts))
   call read_pressures(i)
   i = i + 1          ! sitemap line
   read(strip(sy%t%line(i + 1)), *) nmv
@@ -154,6 +171,8 @@ program music_gcmc
   i = i + 3
   P%delta_trans = 0.0d0
   P%delta_rot = 0.0d0
+  P%adjust_trans = 0
+  P%adjust_rot = 0
   biasf = ''
   bias_T = 0.0d0
   do k = 1, nmv
@@ -192,14 +211,16 @@ program music_gcmc
   P%nmoves = nmv
   P%temperature = T
   P%max_nrg = 1.0d10
+  if (any(P%move_type(1:nmv) == MH_MV_BINSERT) .neqv. any(P%move_type(1:nmv) == MH_MV_BDELETE)) &
+       call fail('BINSERT and BDELETE come as a pair (moves.F90:827-840)')
 
-  ! ---- the species: sites, body-frame coordinates, types, charges
+  ! ---- the species: sites, body-frame coordinates, types (one table over
+  ! the site types of every species), charges
   associate (m => sy%mols(isp))
     nsite = m%natoms
     if (nsite > MH_MAX_SITES) call fail('at most 8 sites per molecule')
     call principal_axes(m, sy%atoms, ref(:, 1:nsite))
     P%nsites = nsite
-    nst = 0
     do a = 1, nsite
        P%ref(:, a) = ref(:, a)
        P%site_q(a) = m%q(a)
@@ -208,6 +229,7 @@ program music_gcmc
           if (stypes(j) == m%atype(a)) k = j
        end do
        if (k == 0) then
+          if (nst == MH_MAX_SITES) call fail('at most 8 site types over the species')
           nst = nst + 1
           stypes(nst) = m%atype(a)
           k = nst
@@ -215,29 +237,42 @@ program music_gcmc
        P%site_type(a) = k - 1
     end do
   end associate
-  P%ntypes = nst
-  P%ewald = 0
-  P%lj_on = 0_c_int8_t
-  P%lj_A = 0.0d0
-  P%lj_B = 0.0d0
-  P%lj_hicut2 = 0.0d0
-  P%lj_locut2 = 0.0d0
   do a = 1, MH_MAX_SITES
      P%site_pmap(a) = c_null_ptr
      P%site_emap(a) = c_null_ptr
   end do
   pmapf = ''
   emapf = ''
+  call save_species(cur)
+  end do
+  ! pair tables and Coulomb / Ewald settings (species 1 carries them for
+  ! the mixture), species-framework maps per species
+  PS(1)%ntypes = nst
+  PS(1)%ewald = 0
+  PS(1)%coul = 0
+  PS(1)%lj_on = 0_c_int8_t
+  PS(1)%coul_on = 0_c_int8_t
+  PS(1)%lj_A = 0.0d0
+  PS(1)%lj_B = 0.0d0
+  PS(1)%lj_hicut2 = 0.0d0
+  PS(1)%lj_locut2 = 0.0d0
   call read_species_file()
   nunit = sy%nxyz(1) * sy%nxyz(2) * sy%nxyz(3)
-  call read_config_source()
+  do cur = 1, nspc
+     call use_species(cur)
+     call read_config_source()
+     call save_species(cur)
+  end do
+  call use_species(1)
 
   write(*, '(a)') 'Welcome to MuSiC (GCMC implementation, libmusic_hip)'
   write(*, '(2a)') 'Reading from control file ', trim(ctrl)
   write(*, '(a, i0, a, i0, a, i0, a, i0, a, i0)') '  iterations ', niter, ' x ', nmicro, ', print every ', iprint, &
        ', block ', bsize, ', seed ', iseed
-  write(*, '(a, i0, a, f9.3, a)') '  species ' // trim(spname) // ': ', nsite, ' sites, T = ', T, ' K'
   write(*, '(a, 3f10.4, a, 3i3)') '  simulation cell (A):', sy%full, ', unit cells', sy%nxyz
+  do cur = 1, nspc
+  call use_species(cur)
+  write(*, '(a, i0, a, f9.3, a)') '  species ' // trim(spname) // ': ', nsite, ' sites, T = ', T, ' K'
   do k = 1, npts
      write(*, '(a, i0, a, es24.16)') '  state point ', k, ': fugacity (kPa) ', press(k)
   end do
@@ -249,6 +284,10 @@ program music_gcmc
      write(*, '(a, i0, a, 3es24.16, a, i0, a, f8.4, 4a)') '  site ', a, ':', P%ref(:, a), ' type ', &
           P%site_type(a), ' q ', P%site_q(a), ' pmap ', trim(basename(pmapf(a))), ' emap ', trim(basename(emapf(a)))
   end do
+  if (len_trim(biasf) > 0) write(*, '(a, a, a, f10.3)') '  bias: ', trim(biasf), ' T ', bias_T
+  if (len_trim(srcfile) > 0) write(*, '(2a)') '  start configuration: ', trim(srcfile)
+  end do
+  call use_species(1)
   do k = 1, nst * nst
      if (P%lj_on(k) /= 0) write(*, '(a, i0, a, 4es24.16)') '  lj ', k - 1, ':', P%lj_A(k), P%lj_B(k), &
           P%lj_hicut2(k), P%lj_locut2(k)
@@ -257,16 +296,13 @@ program music_gcmc
        P%ewald_kmax, ' rcut ', P%ewald_rcut
   if (P%coul /= 0) write(*, '(a, es24.16, a, es24.16, a, es24.16)') '  plain coulomb: hicut ', P%coul_hicut, &
        ' locut ', P%coul_locut, ' shield ', P%coul_shield
-  if (len_trim(biasf) > 0) write(*, '(a, a, a, f10.3)') '  bias: ', trim(biasf), ' T ', bias_T
-  if (len_trim(srcfile) > 0) write(*, '(2a)') '  start configuration: ', trim(srcfile)
   if (dfon) then
+     if (nspc > 1) call fail('datafile: configuration files are written for one GCMC species')
      write(*, '(a, i0, 2a)') '  configuration files every ', iconfig, ' iterations: ', trim(confname)
      if (dfc%pair_nrg .and. (sy%nmol > 1 .or. P%ewald /= 0 .or. P%coul /= 0 .or. any(P%site_q(1:nsite) /= 0.0d0))) &
           call fail('datafile: pair_energy is stored for a single neutral species without framework')
   end if
   write(*, '(a, i0, 2a)') '  restart files every ', icrash, ' iterations: ', trim(resname)
-  if (any(P%move_type(1:nmv) == MH_MV_BINSERT) .neqv. any(P%move_type(1:nmv) == MH_MV_BDELETE)) &
-       call fail('BINSERT and BDELETE come as a pair (moves.F90:827-840)')
   flush(6)
   if (check_only) go to 900
 
@@ -286,17 +322,22 @@ program music_gcmc
   ! a slanted cell: maps fold into it, insertion follows its lattice vectors
   ! (the library refuses sorbate Ewald sums and biased insertion there)
   if (sy%slanted) call check(mh_framework_set_cell(ctx, sy%angles, sy%eff), 'mh_framework_set_cell')
-  allocate(maps(2 * MH_MAX_SITES), mapnames(2 * MH_MAX_SITES))
+  allocate(maps(3 * MH_MAX_SITES * MH_MAX_SPECIES), mapnames(3 * MH_MAX_SITES * MH_MAX_SPECIES))
   nmaps = 0
-  do a = 1, nsite
-     if (len_trim(pmapf(a)) > 0) P%site_pmap(a) = load_map(pmapf(a))
-     if (len_trim(emapf(a)) > 0) P%site_emap(a) = load_map(emapf(a))
+  do cur = 1, nspc
+     call use_species(cur)
+     do a = 1, nsite
+        if (len_trim(pmapf(a)) > 0) P%site_pmap(a) = load_map(pmapf(a))
+        if (len_trim(emapf(a)) > 0) P%site_emap(a) = load_map(emapf(a))
+     end do
+     bmaps(cur) = c_null_ptr
+     if (len_trim(biasf) > 0) then
+        call check(mh_bmap_create(ctx, load_map(biasf), bias_T, bmaps(cur)), 'mh_bmap_create ' // trim(biasf))
+        P%bias = bmaps(cur)
+     end if
+     call save_species(cur)
   end do
-  bmap = c_null_ptr
-  if (len_trim(biasf) > 0) then
-     call check(mh_bmap_create(ctx, load_map(biasf), bias_T, bmap), 'mh_bmap_create ' // trim(biasf))
-     P%bias = bmap
-  end if
+  call use_species(1)
   np = 1
   call system_clock(t0, rate)
   if (chain) then
@@ -310,7 +351,9 @@ program music_gcmc
   secs = real(t1 - t0, c_double) / real(rate, c_double)
   write(*, '(a, f10.3, a, es10.3, a)') 'Main Program   Time: ', secs, ' sec   (', &
        real(npts, c_double) * niter * nmicro / max(secs, 1.0d-9), ' steps/s over all state points)'
-  if (c_associated(bmap)) call check(mh_bmap_destroy(bmap), 'mh_bmap_destroy')
+  do cur = 1, nspc
+     if (c_associated(bmaps(cur))) call check(mh_bmap_destroy(bmaps(cur)), 'mh_bmap_destroy')
+  end do
   do k = 1, nmaps
      call check(mh_map_destroy(maps(k)), 'mh_map_destroy')
   end do
@@ -324,9 +367,9 @@ contains
   subroutine run_points(q0, q1, carry)
     integer, intent(in) :: q0, q1
     logical, intent(in) :: carry
-    integer :: nr, r, e, nev, it, n
+    integer :: nr, r, e, nev, it, n, q
     integer(c_int32_t), allocatable :: seeds(:), Ncur(:), ix(:), iy(:)
-    real(c_double), allocatable :: energy(:), sN(:), sN2(:), snap(:,:), etot(:)
+    real(c_double), allocatable :: energy(:), sN(:), sN2(:), snap(:,:), snap2(:,:), etot(:), fug(:)
     integer(c_int64_t), allocatable :: nsteps(:), att(:,:), acc(:,:)
     integer(8), allocatable :: ev(:), blk(:)
     integer(8) :: s, sprev, total
@@ -335,22 +378,38 @@ contains
     type(c_ptr) :: gnew
     nr = q1 - q0 + 1
     allocate(seeds(nr), Ncur(nr), ix(nr), iy(nr), energy(nr), sN(nr), sN2(nr), etot(nr), nsteps(nr))
-    allocate(att(4, nr), acc(4, nr))
+    allocate(att(4, nr), acc(4, nr), fug(nr * nspc))
     do r = 1, nr
        seeds(r) = int(iseed + (q0 + r - 1) - 1, c_int32_t)
+       do q = 1, nspc
+          fug((r - 1) * nspc + q) = pressm(q0 + r - 1, q)
+       end do
     end do
     if (chain) seeds(1) = int(iseed, c_int32_t)
-    call check(mh_gcmc_create(ctx, P, int(nr, c_int), press(q0:q1), seeds, int(maxn, c_int), gnew), &
+    call check(mh_gcmc_create_mix(ctx, PS, int(nspc, c_int), int(nr, c_int), fug, seeds, int(maxn, c_int), gnew), &
          'mh_gcmc_create')
-    if (len_trim(srcfile) > 0 .and. .not. carry) call start_from_restart(gnew, nr)
-    if (carry) then
-       allocate(com(3, maxn), ang(3, maxn), rr(3, maxn * nsite))
-       call check(mh_gcmc_get_config(g, 0_c_int, ncarry, com, ang, rr), 'mh_gcmc_get_config')
+    if (.not. carry) then
+       do q = 1, nspc
+          call use_species(q)
+          call check(mh_gcmc_species(gnew, int(q - 1, c_int)), 'mh_gcmc_species')
+          if (len_trim(srcfile) > 0) call start_from_restart(gnew, nr, q)
+       end do
+    else
+       ! every species' configuration, then the replica's random-number state
+       do q = 1, nspc
+          call use_species(q)
+          allocate(com(3, maxn), ang(3, maxn), rr(3, maxn * nsite))
+          call check(mh_gcmc_species(g, int(q - 1, c_int)), 'mh_gcmc_species')
+          call check(mh_gcmc_species(gnew, int(q - 1, c_int)), 'mh_gcmc_species')
+          call check(mh_gcmc_get_config(g, 0_c_int, ncarry, com, ang, rr), 'mh_gcmc_get_config')
+          call check(mh_gcmc_set_config(gnew, 0_c_int, ncarry, com, ang), 'mh_gcmc_set_config')
+          deallocate(com, ang, rr)
+       end do
        call check(mh_gcmc_get_state(g, Ncur, energy, sN, nsteps, att, acc, ix, iy), 'mh_gcmc_get_state')
-       call check(mh_gcmc_set_config(gnew, 0_c_int, ncarry, com, ang), 'mh_gcmc_set_config')
        call check(mh_gcmc_set_rng(gnew, 0_c_int, ix(1), iy(1)), 'mh_gcmc_set_rng')
        call check(mh_gcmc_destroy(g), 'mh_gcmc_destroy')
     end if
+    call use_species(1)
     g = gnew
 
     do r = 1, nr
@@ -358,9 +417,11 @@ contains
        write(*, '(a, i0, a, i0, 3x, a, i0, a)') 'Beginning GCMC simulation number ', q0 + r - 1, ' of ', npts, &
             '(', niter, ' steps)'
        write(*, '(a)') '------------------------------------------------------------'
-       write(*, '(a)') '    Species Name: ' // trim(spname)
-       write(*, '(a)') '      Press.(kPa)   Fugacity'
-       write(*, '(4x, 2f11.3)') press(q0 + r - 1), press(q0 + r - 1)
+       do q = 1, nspc
+          write(*, '(a)') '    Species Name: ' // trim(spnames(q))
+          write(*, '(a)') '      Press.(kPa)   Fugacity'
+          write(*, '(4x, 2f11.3)') pressm(q0 + r - 1, q), pressm(q0 + r - 1, q)
+       end do
     end do
     ! events: every print step and the start of its statistics block
     ! (a block holds the last ((s - 1) mod bsize) + 1 steps)
@@ -396,7 +457,7 @@ contains
        end do
     end if
     total = int(niter, 8) * nmicro
-    allocate(snap(nr, 0:0))
+    allocate(snap(nr, nspc))
     sprev = 0
     it = 0
     e = 1
@@ -410,8 +471,8 @@ contains
        sprev = s
        do k = 1, n
           if (int(k * iprint, 8) * nmicro == s) then
-             if (blk(k) == 0) snap(:, 0) = 0.0d0
-             call show_stats(k * iprint, s, s - blk(k), snap(:, 0))
+             if (blk(k) == 0) snap = 0.0d0
+             call show_stats(k * iprint, s, s - blk(k), snap)
           end if
        end do
        if (dfon .and. iconfig > 0 .and. s > 0) then
@@ -426,21 +487,30 @@ contains
              do r = 1, nr
                 call write_restart(restart_name(q0 + r - 1), s / nmicro, r - 1)
              end do
+             call use_species(1)
           end if
        end if
        ! a block start (after the print: the next block may start at this
        ! print step): keep sum N for the block average
        do k = 1, n
           if (blk(k) == s .and. s > 0) then
-             call check(mh_gcmc_get_moments(g, sN, sN2), 'mh_gcmc_get_moments')
-             snap(:, 0) = sN
+             do q = 1, nspc
+                call check(mh_gcmc_species(g, int(q - 1, c_int)), 'mh_gcmc_species')
+                call check(mh_gcmc_get_moments(g, sN, sN2), 'mh_gcmc_get_moments')
+                snap(:, q) = sN
+             end do
           end if
        end do
        if (s >= total) exit
     end do
     ! end of the points: recomputed against stored energies
-    call check(mh_gcmc_get_state(g, Ncur, energy, sN, nsteps, att, acc, ix, iy), 'mh_gcmc_get_state')
     call check(mh_gcmc_total_energy(g, etot, 0_c_int), 'mh_gcmc_total_energy')
+    allocate(snap2(nr, nspc))
+    do q = 1, nspc
+       call check(mh_gcmc_species(g, int(q - 1, c_int)), 'mh_gcmc_species')
+       call check(mh_gcmc_get_state(g, Ncur, energy, sN, nsteps, att, acc, ix, iy), 'mh_gcmc_get_state')
+       snap2(:, q) = sN
+    end do
     do r = 1, nr
        write(*, '(a)') ' ________ ACTUAL ENERGIES    _____________'
        write(*, '(a, f16.3)') '    Energy: ', etot(r) * 4.184d0
@@ -449,8 +519,10 @@ contains
        write(*, '(a)') '------------------------------------------------------------'
        write(*, '(a, i0, 4x, a, es10.3)') 'Ending GCMC simulation number ', q0 + r - 1, 'Nrg dev check: ', &
             abs(etot(r) - energy(r)) * 4.184d0
-       write(*, '(2x, a, a, f10.1, 4x, a, f8.2)') trim(spname), ':  P (kPa):', press(q0 + r - 1), &
-            'loading (molec/uc):', sN(r) / real(max(nsteps(r), 1_c_int64_t), c_double) / nunit
+       do q = 1, nspc
+          write(*, '(2x, a, a, f10.1, 4x, a, f8.2)') trim(spnames(q)), ':  P (kPa):', pressm(q0 + r - 1, q), &
+               'loading (molec/uc):', snap2(r, q) / real(max(nsteps(r), 1_c_int64_t), c_double) / nunit
+       end do
        write(*, '(a)') '------------------------------------------------------------'
     end do
     if (allocated(dfu)) then
@@ -522,52 +594,63 @@ contains
     end do
   end subroutine run_steps
 
-  ! the reference's statistics block (gcmc_displaystats) per replica
+  ! the reference's statistics block (gcmc_displaystats) per replica: per
+  ! species its fugacity, the acceptance of its moves and its number of
+  ! molecules (gcmcmoves_displaystats)
   subroutine show_stats(it, s, nblk, sblk)
     integer, intent(in) :: it
     integer(8), intent(in) :: s, nblk
-    real(c_double), intent(in) :: sblk(:)
-    integer :: r, nr, k, mt
-    integer(c_int32_t), allocatable :: Nc(:), ix(:), iy(:)
-    real(c_double), allocatable :: en(:), sN(:), sN2(:), dl(:,:)
-    integer(c_int64_t), allocatable :: nst(:), att(:,:), acc(:,:)
+    real(c_double), intent(in) :: sblk(:,:)
+    integer :: r, nr, k, mt, q
+    integer(c_int32_t), allocatable :: Nc(:,:), ix(:), iy(:)
+    real(c_double), allocatable :: en(:), sN(:,:), sN2(:,:), dl(:,:,:)
+    integer(c_int64_t), allocatable :: nst(:), att(:,:,:), acc(:,:,:)
     real(c_double) :: cum, dev, blkavg, ratio
     character(len=32) :: lab
-    nr = size(sblk)
-    allocate(Nc(nr), ix(nr), iy(nr), en(nr), sN(nr), sN2(nr), nst(nr), att(4, nr), acc(4, nr), dl(2, nr))
-    call check(mh_gcmc_get_deltas(g, dl), 'mh_gcmc_get_deltas')
-    call check(mh_gcmc_get_state(g, Nc, en, sN, nst, att, acc, ix, iy), 'mh_gcmc_get_state')
-    call check(mh_gcmc_get_moments(g, sN, sN2), 'mh_gcmc_get_moments')
+    nr = size(sblk, 1)
+    allocate(Nc(nr, nspc), ix(nr), iy(nr), en(nr), sN(nr, nspc), sN2(nr, nspc), nst(nr))
+    allocate(att(4, nr, nspc), acc(4, nr, nspc), dl(2, nr, nspc))
+    do q = 1, nspc
+       call check(mh_gcmc_species(g, int(q - 1, c_int)), 'mh_gcmc_species')
+       call check(mh_gcmc_get_deltas(g, dl(:, :, q)), 'mh_gcmc_get_deltas')
+       call check(mh_gcmc_get_state(g, Nc(:, q), en, sN(:, q), nst, att(:, :, q), acc(:, :, q), ix, iy), &
+            'mh_gcmc_get_state')
+       call check(mh_gcmc_get_moments(g, sN(:, q), sN2(:, q)), 'mh_gcmc_get_moments')
+    end do
     do r = 1, nr
        write(*, '(2x, a, i0, a, i0, a, i0)') 'Macro-iteration number ', it, ' (', s, ' micro) of simulation ', &
             merge(np, r, chain)
        write(*, '(2x, a)') '------------------------------------------------------------'
        write(*, '(2x, a)') 'The GCMC Stats:'
-       write(*, '(4x, a, f10.1, a)') 'Species Name: ' // trim(spname) // '    Pressure:', &
-            press(merge(np, r, chain)), ' kPa'
-       do k = 1, nmv
-          mt = mod(P%move_type(k), 4) + 1  ! BINSERT / BDELETE count as insert / delete
-          ratio = 0.0d0
-          if (att(mt, r) > 0) ratio = real(acc(mt, r), c_double) / real(att(mt, r), c_double)
-          lab = trim(basic_tag(P%move_type(k))) // ' (' // trim(mvname(k)) // ')'
-          if (P%move_type(k) == MH_MV_TRANSLATE) then
-             write(*, '(6x, a26, a, f5.3, 3x, a, i0, a, i0, a, f6.3)') lab, 'Acc. Ratio: ', ratio, '(', acc(mt, r), &
-                  ' of ', att(mt, r), ')  Max disp: ', dl(1, r)
-          else if (P%move_type(k) == MH_MV_ROTATE) then
-             write(*, '(6x, a26, a, f5.3, 3x, a, i0, a, i0, a, f6.3)') lab, 'Acc. Ratio: ', ratio, '(', acc(mt, r), &
-                  ' of ', att(mt, r), ')  Max disp: ', dl(2, r)
-          else
-             write(*, '(6x, a26, a, f5.3, 3x, a, i0, a, i0, a)') lab, 'Acc. Ratio: ', ratio, '(', acc(mt, r), &
-                  ' of ', att(mt, r), ')'
-          end if
+       do q = 1, nspc
+          call use_species(q)
+          write(*, '(4x, a, f10.1, a)') 'Species Name: ' // trim(spname) // '    Pressure:', &
+               press(merge(np, r, chain)), ' kPa'
+          do k = 1, nmv
+             mt = mod(P%move_type(k), 4) + 1  ! BINSERT / BDELETE count as insert / delete
+             ratio = 0.0d0
+             if (att(mt, r, q) > 0) ratio = real(acc(mt, r, q), c_double) / real(att(mt, r, q), c_double)
+             lab = trim(basic_tag(P%move_type(k))) // ' (' // trim(mvname(k)) // ')'
+             if (P%move_type(k) == MH_MV_TRANSLATE) then
+                write(*, '(6x, a26, a, f5.3, 3x, a, i0, a, i0, a, f6.3)') lab, 'Acc. Ratio: ', ratio, '(', &
+                     acc(mt, r, q), ' of ', att(mt, r, q), ')  Max disp: ', dl(1, r, q)
+             else if (P%move_type(k) == MH_MV_ROTATE) then
+                write(*, '(6x, a26, a, f5.3, 3x, a, i0, a, i0, a, f6.3)') lab, 'Acc. Ratio: ', ratio, '(', &
+                     acc(mt, r, q), ' of ', att(mt, r, q), ')  Max disp: ', dl(2, r, q)
+             else
+                write(*, '(6x, a26, a, f5.3, 3x, a, i0, a, i0, a)') lab, 'Acc. Ratio: ', ratio, '(', &
+                     acc(mt, r, q), ' of ', att(mt, r, q), ')'
+             end if
+          end do
+          cum = sN(r, q) / real(s, c_double)
+          dev = sN2(r, q) / real(s, c_double) - cum * cum
+          blkavg = (sN(r, q) - sblk(r, q)) / real(nblk, c_double)
+          write(*, '(6x, a, 4f7.2)') 'Number of Molecules (ins, blk, cum, stdd)', real(Nc(r, q), c_double), &
+               blkavg, cum, sqrt(max(dev, 0.0d0))
        end do
-       cum = sN(r) / real(s, c_double)
-       dev = sN2(r) / real(s, c_double) - cum * cum
-       blkavg = (sN(r) - sblk(r)) / real(nblk, c_double)
-       write(*, '(6x, a, 4f7.2)') 'Number of Molecules (ins, blk, cum, stdd)', real(Nc(r), c_double), blkavg, &
-            cum, sqrt(max(dev, 0.0d0))
        write(*, '(a)') ' '
     end do
+    call use_species(1)
   end subroutine show_stats
 
   function restart_name(simno) result(o)
@@ -607,64 +690,73 @@ contains
     end do
   end subroutine toxyz
 
-  ! config_writerestartfile (config.F90:2357-2430) for replica rep, and the
-  ! replica's random-number state beside it
+  ! config_writerestartfile (config.F90:2357-2430) for replica rep, one
+  ! block per GCMC species, and the replica's random-number state beside it
+  ! (then per species: molecules, displacements, attempt / acceptance counts)
   subroutine write_restart(path, iter, rep)
     character(len=*), intent(in) :: path
     integer(8), intent(in) :: iter
     integer, intent(in) :: rep
-    integer :: u, n, m, b, dof
+    integer :: u, n, m, b, dof, q
     integer(c_int) :: nmol
     real(c_double), allocatable :: com(:,:), ang(:,:), rr(:,:), rp(:,:), dl(:,:)
     integer(c_int32_t), allocatable :: Nc(:), ix(:), iy(:)
     real(c_double), allocatable :: en(:), sN(:)
     integer(c_int64_t), allocatable :: nst(:), att(:,:), acc(:,:)
     character(len=48) :: nm
-    allocate(com(3, maxn), ang(3, maxn), rr(3, maxn * nsite), rp(3, nsite))
-    call check(mh_gcmc_get_config(g, int(rep, c_int), nmol, com, ang, rr), 'mh_gcmc_get_config')
-    n = nmol
-    dof = sy%mols(isp)%dof
-    if (dof == 0) dof = merge(3, 6, nsite == 1)
-    nm = spname
+    allocate(com(3, maxn), ang(3, maxn), rr(3, maxn * MH_MAX_SITES), rp(3, MH_MAX_SITES))
+    allocate(Nc(nrep_now()), ix(nrep_now()), iy(nrep_now()), en(nrep_now()), sN(nrep_now()), nst(nrep_now()))
+    allocate(att(4, nrep_now()), acc(4, nrep_now()), dl(2, nrep_now()))
     open(newunit=u, file=trim(path), status='replace', action='write')
     write(u, '(1x, a)') trim(desc)
     write(u, '(i12)') iter
     write(u, '(2x, es22.15e3)') 0.0d0
-    write(u, '(2a)') '_MOLECULE_NAME_: ', nm
-    write(u, '(i20, a)') n, ' # Nmoles'
-    write(u, '(i20, a)') nsite, ' # Natoms'
-    write(u, '(a)') ' _Principal Coordinates Only_'
-    do m = 1, n
-       call toxyz(com(:, m), ang(:, m), rp)
-       do b = 1, nsite
-          write(u, '(a)') trim(vec13(rp(:, b)))
+    do q = 1, nspc
+       call use_species(q)
+       call check(mh_gcmc_species(g, int(q - 1, c_int)), 'mh_gcmc_species')
+       call check(mh_gcmc_get_config(g, int(rep, c_int), nmol, com, ang, rr), 'mh_gcmc_get_config')
+       n = nmol
+       dof = sy%mols(isp)%dof
+       if (dof == 0) dof = merge(3, 6, nsite == 1)
+       nm = spname
+       write(u, '(2a)') '_MOLECULE_NAME_: ', nm
+       write(u, '(i20, a)') n, ' # Nmoles'
+       write(u, '(i20, a)') nsite, ' # Natoms'
+       write(u, '(a)') ' _Principal Coordinates Only_'
+       do m = 1, n
+          call toxyz(com(:, m), ang(:, m), rp(:, 1:nsite))
+          do b = 1, nsite
+             write(u, '(a)') trim(vec13(rp(:, b)))
+          end do
+       end do
+       write(u, '(a)') ' _Velocities_'
+       do m = 1, n * nsite
+          write(u, '(a)') trim(vec13([0.0d0, 0.0d0, 0.0d0]))
+       end do
+       write(u, '(a)') ' _Generalized Coordinates_'
+       do m = 1, n
+          write(u, '(a)') 'FULLY_RIGID'
+          write(u, '(a, 3f13.5, i5)') trim(vec13(com(:, m))), ang(2, m), ang(1, m), ang(3, m), dof
        end do
     end do
-    write(u, '(a)') ' _Velocities_'
-    do m = 1, n * nsite
-       write(u, '(a)') trim(vec13([0.0d0, 0.0d0, 0.0d0]))
-    end do
-    write(u, '(a)') ' _Generalized Coordinates_'
-    do m = 1, n
-       write(u, '(a)') 'FULLY_RIGID'
-       write(u, '(a, 3f13.5, i5)') trim(vec13(com(:, m))), ang(2, m), ang(1, m), ang(3, m), dof
-    end do
     close(u)
-    allocate(Nc(nrep_now()), ix(nrep_now()), iy(nrep_now()), en(nrep_now()), sN(nrep_now()), nst(nrep_now()))
-    allocate(att(4, nrep_now()), acc(4, nrep_now()))
-    call check(mh_gcmc_get_state(g, Nc, en, sN, nst, att, acc, ix, iy), 'mh_gcmc_get_state')
     open(newunit=u, file=trim(path) // '.state', status='replace', action='write')
-    write(u, '(2i12)') ix(rep + 1), iy(rep + 1)
-    write(u, '(i12)') n
-    do m = 1, n
-       write(u, '(6es26.17e3)') com(:, m), ang(:, m)
+    do q = 1, nspc
+       call check(mh_gcmc_species(g, int(q - 1, c_int)), 'mh_gcmc_species')
+       call check(mh_gcmc_get_config(g, int(rep, c_int), nmol, com, ang, rr), 'mh_gcmc_get_config')
+       call check(mh_gcmc_get_state(g, Nc, en, sN, nst, att, acc, ix, iy), 'mh_gcmc_get_state')
+       call check(mh_gcmc_get_deltas(g, dl), 'mh_gcmc_get_deltas')
+       if (q == 1) write(u, '(2i12)') ix(rep + 1), iy(rep + 1)
+       write(u, '(i12)') nmol
+       do m = 1, nmol
+          write(u, '(6es26.17e3)') com(:, m), ang(:, m)
+       end do
+       write(u, '(2es26.17e3)') dl(:, rep + 1)
+       ! attempt / acceptance counts: adjustable displacements follow their ratio
+       write(u, '(8i21)') att(:, rep + 1), acc(:, rep + 1)
     end do
-    allocate(dl(2, nrep_now()))
-    call check(mh_gcmc_get_deltas(g, dl), 'mh_gcmc_get_deltas')
-    write(u, '(2es26.17e3)') dl(:, rep + 1)
-    ! attempt / acceptance counts: adjustable displacements follow their ratio
-    write(u, '(8i21)') att(:, rep + 1), acc(:, rep + 1)
     close(u)
+    call use_species(1)
   end subroutine write_restart
 
   integer function nrep_now()
@@ -700,14 +792,16 @@ contains
     end do
   end subroutine read_config_source
 
-  ! config_readrestartfile (config.F90:2114-2247): the species' molecules
-  ! from their generalized coordinates (centre of mass, theta, phi, psi) into
-  ! every replica of batch gb; with 'resume' the saved random-number state too
-  subroutine start_from_restart(gb, nr)
+  ! config_readrestartfile (config.F90:2114-2247): the molecules of GCMC
+  ! species q (the current species, selected in gb) from their generalized
+  ! coordinates (centre of mass, theta, phi, psi) into every replica of batch
+  ! gb; with 'resume' the saved random-number state and the species' block
+  ! of the state file too
+  subroutine start_from_restart(gb, nr, q)
     type(c_ptr), intent(in) :: gb
-    integer, intent(in) :: nr
+    integer, intent(in) :: nr, q
     type(text_file) :: rf
-    integer :: ii, nm, na, m, r, u, ios
+    integer :: ii, nm, na, m, r, u, ios, k, mk
     real(c_double), allocatable :: com(:,:), ang(:,:)
     real(c_double) :: v(7), dres(2)
     integer(c_int32_t) :: rx, ry
@@ -746,18 +840,27 @@ contains
        if (.not. exists(trim(rng))) call fail('resume: no saved state ' // trim(rng))
        open(newunit=u, file=trim(rng), status='old', action='read')
        read(u, *) rx, ry
-       read(u, *) m
-       if (m /= nm) call fail('resume: the saved state does not belong to the restart file')
-       do m = 1, nm
-          read(u, *) com(:, m), ang(:, m)
+       ! the blocks of the species before q, then q's
+       do k = 1, q
+          read(u, *, iostat=ios) mk
+          if (ios /= 0) call fail('resume: the saved state holds fewer species than the ctrlfile')
+          if (k == q .and. mk /= nm) call fail('resume: the saved state does not belong to the restart file')
+          do m = 1, mk
+             if (k == q) then
+                read(u, *) com(:, m), ang(:, m)
+             else
+                read(u, *) v(1:6)
+             end if
+          end do
+          read(u, *, iostat=ios) dres
+          if (ios /= 0) dres = [P%delta_trans, P%delta_rot]
+          have_cnt = .false.
+          if (ios == 0) then
+             read(u, *, iostat=ios) cnt
+             have_cnt = ios == 0
+          end if
+          if (k < q .and. .not. have_cnt) call fail('resume: incomplete species block in ' // trim(rng))
        end do
-       read(u, *, iostat=ios) dres
-       if (ios /= 0) dres = [P%delta_trans, P%delta_rot]
-       have_cnt = .false.
-       if (ios == 0) then
-          read(u, *, iostat=ios) cnt
-          have_cnt = ios == 0
-       end if
        close(u)
     end if
     do r = 1, nr
@@ -910,89 +1013,142 @@ contains
     if (adj /= 0 .and. adj /= 1) call fail('adjust delta flag must be 0 or 1')
   end subroutine read_delta
 
-  ! species-species file entries of the GCMC species
+  ! species-species file entries of the GCMC species (ssdriver_init,
+  ! ssdriver.F90): between two GCMC species (a species with itself, or two of
+  ! a mixture) the LJ and Coulomb / Ewald models fill the pair tables over
+  ! their site types; between a GCMC species and another molecule, the maps
+  ! of its sites.  Missing or OFF lines leave a pair without interaction.
   subroutine read_species_file()
     type(text_file) :: ss
-    character(len=LL) :: s, kind, tok, other
-    integer :: l, wv, k1, k2, b, c
-    logical :: self, any_lj
+    character(len=LL) :: s, kind, tok
+    integer :: l, wv, k1, k2, b, c, s1, s2, ncl
+    integer :: ljon(MH_MAX_SPECIES, MH_MAX_SPECIES), ckind(MH_MAX_SPECIES, MH_MAX_SPECIES)
+    real(c_double) :: ek, er
+    integer(c_int) :: kmax
+    logical :: ewset
     call read_text(find_file(sy%ssfile, 'NONE', '', sy%base), ss)
-    any_lj = .false.
+    ljon = 0
+    ckind = 0
+    ewset = .false.
+    call use_species(1)
     do l = 1, ss%n
        s = strip(ss%line(l))
        if (nwords(s) < 3) cycle
        kind = upper(word(s, 3))
        if (trim(kind) /= 'NCOUL' .and. trim(kind) /= 'COUL') cycle
-       if (trim(word(s, 1)) == trim(spname)) then
-          other = word(s, 2)
-       else if (trim(word(s, 2)) == trim(spname)) then
-          other = word(s, 1)
-       else
-          cycle
-       end if
-       self = trim(other) == trim(spname)
+       s1 = species_of(word(s, 1))
+       s2 = species_of(word(s, 2))
+       if (s1 == 0 .or. s2 == 0) cycle          ! species-framework: maps, below
        if (trim(upper(word(s, 4))) == 'OFF') cycle
-       if (self) then
-          if (trim(kind) == 'NCOUL') then
-             if (index(upper(s), 'LJ') == 0) call fail('species-species NCOUL model must be LJ: ' // trim(s))
-             ! ssbasic_init options (ssbasic.F90:125-177): FAST is what the
-             ! kernel does; a centre-of-mass cutoff (COMCUT@, COMTRUNC@),
-             ! SMOOTH@ and NEIGHBOR change the energy and are not offered
-             do wv = 4, nwords(s)
-                tok = upper(word(s, wv))
-                if (trim(tok) == 'BASIC' .or. trim(tok) == 'LJ' .or. trim(tok) == 'FAST') cycle
-                call fail('species-species option ' // trim(word(s, wv)) // ' is not offered (BASIC LJ FAST): ' &
-                     // trim(s))
+       if (trim(kind) == 'NCOUL') then
+          if (index(upper(s), 'LJ') == 0) call fail('species-species NCOUL model must be LJ: ' // trim(s))
+          ! ssbasic_init options (ssbasic.F90:125-177): FAST is what the
+          ! kernel does; a centre-of-mass cutoff (COMCUT@, COMTRUNC@),
+          ! SMOOTH@ and NEIGHBOR change the energy and are not offered
+          do wv = 4, nwords(s)
+             tok = upper(word(s, wv))
+             if (trim(tok) == 'BASIC' .or. trim(tok) == 'LJ' .or. trim(tok) == 'FAST') cycle
+             call fail('species-species option ' // trim(word(s, wv)) // ' is not offered (BASIC LJ FAST): ' &
+                  // trim(s))
+          end do
+          ljon(s1, s2) = 1
+          ljon(s2, s1) = 1
+          do b = 1, nst
+             do c = 1, nst
+                if (.not. pair_of(s1, s2, b, c)) cycle
+                k1 = pair_index(sy, sy%atoms(stypes(b) + 1)%name, sy%atoms(stypes(c) + 1)%name)
+                if (k1 == 0) call fail('no atom-atom LJ entry for ' // trim(sy%atoms(stypes(b) + 1)%name) // &
+                     ' ' // trim(sy%atoms(stypes(c) + 1)%name))
+                k2 = (b - 1) * nst + c
+                if (.not. sy%pairs(k1)%ok) cycle
+                P%lj_A(k2) = sy%pairs(k1)%A
+                P%lj_B(k2) = sy%pairs(k1)%B
+                P%lj_hicut2(k2) = sy%pairs(k1)%hicut**2
+                P%lj_locut2(k2) = sy%pairs(k1)%locut**2
+                P%lj_on(k2) = 1_c_int8_t
              end do
-             any_lj = .true.
-             do b = 1, nst
-                do c = 1, nst
-                   k1 = pair_index(sy, sy%atoms(stypes(b) + 1)%name, sy%atoms(stypes(c) + 1)%name)
-                   if (k1 == 0) call fail('no atom-atom LJ entry for ' // trim(sy%atoms(stypes(b) + 1)%name) // &
-                        ' ' // trim(sy%atoms(stypes(c) + 1)%name))
-                   k2 = (b - 1) * nst + c
-                   if (.not. sy%pairs(k1)%ok) cycle
-                   P%lj_A(k2) = sy%pairs(k1)%A
-                   P%lj_B(k2) = sy%pairs(k1)%B
-                   P%lj_hicut2(k2) = sy%pairs(k1)%hicut**2
-                   P%lj_locut2(k2) = sy%pairs(k1)%locut**2
-                   P%lj_on(k2) = 1_c_int8_t
-                end do
-             end do
-          else if (trim(upper(word(s, 4))) == 'BASIC' .and. index(upper(s), 'WFCOUL') == 0) then
-             ! plain Coulomb from the atom-atom COUL lines (coulomb_snglint);
-             ! the ssbasic options other than FAST are refused as for NCOUL
-             do wv = 5, nwords(s)
-                tok = upper(word(s, wv))
-                if (trim(tok) == 'FAST') cycle
-                call fail('species-species option ' // trim(word(s, wv)) // ' is not offered (BASIC FAST): ' &
-                     // trim(s))
-             end do
-             call plain_coulomb()
-          else
-             if (index(upper(s), 'EWALD') == 0) &
-                  call fail('species-species COUL model must be SUM EWALD, BASIC (plain COUL) or OFF: ' // trim(s))
-             kappa_param = 0.0d0
-             rcut = 0.0d0
-             P%ewald_kmax = 0
-             do wv = 4, nwords(s)
-                tok = upper(word(s, wv))
-                if (index(tok, 'KAPPA@') == 1) read(tok(7:), *) kappa_param
-                if (index(tok, 'KMAX@') == 1) read(tok(6:), *) P%ewald_kmax
-                if (index(tok, 'RCUT@') == 1) read(tok(6:), *) rcut
-             end do
-             if (kappa_param <= 0.0d0 .or. P%ewald_kmax <= 0) call fail('Ewald needs KAPPA@ and KMAX@: ' // trim(s))
-             ! kappa in 1/A from the reference's kappa parameter (ewald.F90:
-             ! kappa = KAPPA / shortest cell edge); without RCUT@ every
-             ! minimum-image pair enters the real-space sum, as the ewald
-             ! module's default rcut = -1 (ewald.F90:111-114, 812): a cutoff
-             ! beyond the longest minimum-image distance
-             P%ewald = 1
-             P%ewald_kappa = kappa_param / minval(sy%full)
-             if (rcut <= 0.0d0) rcut = 2.0d0 * maxval(sy%full)
-             P%ewald_rcut = rcut
-          end if
+          end do
+       else if (trim(upper(word(s, 4))) == 'BASIC' .and. index(upper(s), 'WFCOUL') == 0) then
+          ! plain Coulomb from the atom-atom COUL lines (coulomb_snglint);
+          ! the ssbasic options other than FAST are refused as for NCOUL
+          do wv = 5, nwords(s)
+             tok = upper(word(s, wv))
+             if (trim(tok) == 'FAST') cycle
+             call fail('species-species option ' // trim(word(s, wv)) // ' is not offered (BASIC FAST): ' &
+                  // trim(s))
+          end do
+          ckind(s1, s2) = 1
+          ckind(s2, s1) = 1
+          call plain_coulomb(s1, s2)
        else
+          if (index(upper(s), 'EWALD') == 0) &
+               call fail('species-species COUL model must be SUM EWALD, BASIC (plain COUL) or OFF: ' // trim(s))
+          kappa_param = 0.0d0
+          rcut = 0.0d0
+          kmax = 0
+          do wv = 4, nwords(s)
+             tok = upper(word(s, wv))
+             if (index(tok, 'KAPPA@') == 1) read(tok(7:), *) kappa_param
+             if (index(tok, 'KMAX@') == 1) read(tok(6:), *) kmax
+             if (index(tok, 'RCUT@') == 1) read(tok(6:), *) rcut
+          end do
+          if (kappa_param <= 0.0d0 .or. kmax <= 0) call fail('Ewald needs KAPPA@ and KMAX@: ' // trim(s))
+          ! kappa in 1/A from the reference's kappa parameter (ewald.F90:
+          ! kappa = KAPPA / shortest cell edge); without RCUT@ every
+          ! minimum-image pair enters the real-space sum, as the ewald
+          ! module's default rcut = -1 (ewald.F90:111-114, 812): a cutoff
+          ! beyond the longest minimum-image distance
+          ek = kappa_param / minval(sy%full)
+          if (rcut <= 0.0d0) rcut = 2.0d0 * maxval(sy%full)
+          er = rcut
+          ! one Ewald sum covers every charged species pair of the mixture
+          if (ewset .and. (ek /= P%ewald_kappa .or. kmax /= P%ewald_kmax .or. er /= P%ewald_rcut)) &
+               call fail('Ewald: one KAPPA@ / KMAX@ / RCUT@ for all species pairs is offered: ' // trim(s))
+          ewset = .true.
+          ckind(s1, s2) = 2
+          ckind(s2, s1) = 2
+          P%ewald = 1
+          P%ewald_kappa = ek
+          P%ewald_kmax = kmax
+          P%ewald_rcut = er
+       end if
+    end do
+    ! the tables are per site-type pair: species pairs with different models
+    ! must not share a site-type pair
+    do s1 = 1, nspc
+       do s2 = 1, nspc
+          do b = 1, nst
+             do c = 1, nst
+                if (.not. pair_of(s1, s2, b, c)) cycle
+                k2 = (b - 1) * nst + c
+                if ((ljon(s1, s2) == 0 .and. P%lj_on(k2) /= 0) .or. &
+                     (ckind(s1, s2) /= 1 .and. P%coul_on(k2) /= 0)) &
+                     call fail('species ' // trim(spnames(s1)) // ' and ' // trim(spnames(s2)) // &
+                     ' share site types with species pairs of another model')
+             end do
+          end do
+          ! Ewald sums every charge: charged species pairs are all EWALD
+          ncl = 0
+          if (any(PS(s1)%site_q(1:nsites(s1)) /= 0.0d0) .and. any(PS(s2)%site_q(1:nsites(s2)) /= 0.0d0)) ncl = 1
+          if (ewset .and. ncl == 1 .and. ckind(s1, s2) /= 2) &
+               call fail('Ewald: every pair of charged species needs the COUL SUM EWALD line (' // &
+               trim(spnames(s1)) // ', ' // trim(spnames(s2)) // ')')
+       end do
+    end do
+    call save_species(1)
+    ! species-framework maps
+    do cur = 1, nspc
+       call use_species(cur)
+       do l = 1, ss%n
+          s = strip(ss%line(l))
+          if (nwords(s) < 3) cycle
+          kind = upper(word(s, 3))
+          if (trim(kind) /= 'NCOUL' .and. trim(kind) /= 'COUL') cycle
+          s1 = species_of(word(s, 1))
+          s2 = species_of(word(s, 2))
+          if (s1 /= 0 .and. s2 /= 0) cycle
+          if (s1 /= cur .and. s2 /= cur) cycle
+          if (trim(upper(word(s, 4))) == 'OFF') cycle
           if (index(upper(s), 'MAP') == 0) &
                call fail('species-framework interactions come from maps (PMAP@ / EMAP@): ' // trim(s))
           do wv = 4, nwords(s)
@@ -1002,39 +1158,95 @@ contains
              if (k1 > 0) call assign_map(tok(1:k1 - 1), tok(k1 + 6:), pmapf)
              if (k2 > 0) call assign_map(tok(1:k2 - 1), tok(k2 + 6:), emapf)
           end do
-       end if
+       end do
+       call save_species(cur)
     end do
+    call use_species(1)
   end subroutine read_species_file
 
-  ! plain Coulomb between the sorbate's site types: per type pair the
-  ! atom_atom COUL line (on unless OFF); one hicut and shield constant for all
-  ! pairs (the kernel's form), locut LOW_DIST_CUTOFF
-  subroutine plain_coulomb()
+  ! index (1..nspc) of a GCMC species by name, 0 for another molecule
+  integer function species_of(nm)
+    character(len=*), intent(in) :: nm
+    integer :: q
+    species_of = 0
+    do q = 1, nspc
+       if (trim(spnames(q)) == trim(nm)) species_of = q
+    end do
+  end function species_of
+
+  ! site-type pair (b, c) of the table joins a site of species s1 with one of
+  ! species s2
+  logical function pair_of(s1, s2, b, c)
+    integer, intent(in) :: s1, s2, b, c
+    pair_of = any(PS(s1)%site_type(1:nsites(s1)) == b - 1) .and. any(PS(s2)%site_type(1:nsites(s2)) == c - 1)
+    pair_of = pair_of .or. (any(PS(s2)%site_type(1:nsites(s2)) == b - 1) .and. &
+         any(PS(s1)%site_type(1:nsites(s1)) == c - 1))
+  end function pair_of
+
+  ! plain Coulomb between the site types of species s1 and s2: per type pair
+  ! the atom_atom COUL line (on unless OFF); one hicut and shield constant for
+  ! all pairs (the kernel's form), locut LOW_DIST_CUTOFF
+  subroutine plain_coulomb(s1, s2)
+    integer, intent(in) :: s1, s2
     integer :: b, c
-    logical :: found, off, first
+    logical :: found, off
     real(c_double) :: hc, sh
     character(len=LL) :: aa
     aa = find_file(sy%aafile, 'NONE', '', sy%base)
-    P%coul = 1
     P%coul_locut = LOW_DIST_CUTOFF
-    first = .true.
     do b = 1, nst
        do c = 1, nst
+          if (.not. pair_of(s1, s2, b, c)) cycle
           call read_coul_pair(aa, sy%atoms(stypes(b) + 1)%name, sy%atoms(stypes(c) + 1)%name, found, off, hc, sh)
           if (.not. found) call fail('no atom-atom COUL entry for ' // trim(sy%atoms(stypes(b) + 1)%name) // &
                ' ' // trim(sy%atoms(stypes(c) + 1)%name))
           if (off) cycle
           P%coul_on((b - 1) * nst + c) = 1_c_int8_t
-          if (first) then
+          if (P%coul == 0) then
              P%coul_hicut = hc
              P%coul_shield = sh
-             first = .false.
+             P%coul = 1
           else if (hc /= P%coul_hicut .or. sh /= P%coul_shield) then
              call fail('plain Coulomb: one HICUT and SHIELDCONST for all site pairs is offered')
           end if
        end do
     end do
   end subroutine plain_coulomb
+
+  ! the scalars of the species being read or reported (spname, isp, nsite,
+  ! move names, pressures, model, map and bias files, start configuration)
+  ! from / into the per-species arrays
+  subroutine use_species(q)
+    integer, intent(in) :: q
+    spname = spnames(q)
+    isp = isps(q)
+    nsite = nsites(q)
+    nmv = nmvs(q)
+    mvname = mvnames(:, q)
+    press = pressm(:, q)
+    P = PS(q)
+    pmapf = pmapfs(:, q)
+    emapf = emapfs(:, q)
+    biasf = biasfs(q)
+    bias_T = bias_Ts(q)
+    srcfile = srcfiles(q)
+  end subroutine use_species
+
+  subroutine save_species(q)
+    integer, intent(in) :: q
+    spnames(q) = spname
+    isps(q) = isp
+    nsites(q) = nsite
+    nmvs(q) = nmv
+    mvnames(:, q) = mvname
+    pressm(:, q) = press
+    PS(q) = P
+    pmapfs(:, q) = pmapf
+    emapfs(:, q) = emapf
+    biasfs(q) = biasf
+    bias_Ts(q) = bias_T
+    srcfiles(q) = srcfile
+  end subroutine save_species
 
   ! map file of the sites of one atom type ('all': every site)
   subroutine assign_map(aname, fname, dst)

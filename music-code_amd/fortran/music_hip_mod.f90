@@ -21,7 +21,7 @@ module music_hip_mod
   integer(c_int), parameter, public :: MH_EWALD_PARITY = 0, MH_EWALD_PHYSICS = 1
   integer(c_int), parameter, public :: MH_MV_INSERT = 0, MH_MV_DELETE = 1, MH_MV_TRANSLATE = 2, &
        MH_MV_ROTATE = 3, MH_MV_BINSERT = 4, MH_MV_BDELETE = 5
-  integer, parameter, public :: MH_MAX_SITES = 8
+  integer, parameter, public :: MH_MAX_SITES = 8, MH_MAX_SPECIES = 4
 
   ! mh_gcmc_params (field order and types as in the C struct)
   type, bind(C), public :: mh_gcmc_params
@@ -56,7 +56,8 @@ module music_hip_mod
   public :: mh_ctx_create, mh_ctx_destroy, mh_framework_upload, mh_framework_set_cell, mh_ff_set_lj, mh_ff_set_buck
   public :: mh_ewald_set, mh_map_build, mh_map_destroy, mh_map_info, mh_map_get, mh_map_write
   public :: mh_map_load, mh_map_interp
-  public :: mh_gcmc_create, mh_gcmc_destroy, mh_gcmc_set_threads, mh_gcmc_autotune
+  public :: mh_gcmc_create, mh_gcmc_create_mix, mh_gcmc_species, mh_gcmc_destroy, mh_gcmc_set_threads, &
+       mh_gcmc_autotune
   public :: mh_gcmc_set_config, mh_gcmc_run, mh_gcmc_run_cycles, mh_gcmc_get_state, mh_gcmc_set_rng, &
             mh_gcmc_get_moments, mh_gcmc_get_deltas, mh_gcmc_set_deltas, mh_gcmc_set_counts
   public :: mh_gcmc_get_config, mh_gcmc_total_energy, mh_trial_energy
@@ -206,6 +207,25 @@ module music_hip_mod
        type(c_ptr), intent(out) :: g
        integer(c_int) :: rc
      end function mh_gcmc_create
+
+     function mh_gcmc_create_mix(ctx, params, nspc, nrep, fugacity_kPa, seeds, maxN, g) &
+          bind(C, name="mh_gcmc_create_mix") result(rc)
+       import :: c_int, c_ptr, c_double, c_int32_t, mh_gcmc_params
+       type(c_ptr), value :: ctx
+       type(mh_gcmc_params), intent(in) :: params(*)
+       integer(c_int), value :: nspc, nrep, maxN
+       real(c_double), intent(in) :: fugacity_kPa(*)
+       integer(c_int32_t), intent(in) :: seeds(*)
+       type(c_ptr), intent(out) :: g
+       integer(c_int) :: rc
+     end function mh_gcmc_create_mix
+
+     function mh_gcmc_species(g, spc) bind(C, name="mh_gcmc_species") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: g
+       integer(c_int), value :: spc
+       integer(c_int) :: rc
+     end function mh_gcmc_species
 
      function mh_gcmc_destroy(g) bind(C, name="mh_gcmc_destroy") result(rc)
        import :: c_int, c_ptr

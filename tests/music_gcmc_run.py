@@ -71,11 +71,16 @@ def parse(text):
 
 
 def parse_check(text):
-    """Parameters printed by 'music_gcmc ctrlfile check'."""
-    d = dict(press=[], sites=[], lj={}, moves=[], ewald=None, bias=None)
+    """Parameters printed by 'music_gcmc ctrlfile check'; for a mixture
+    d["species"] holds each species' name, pressures, sites, moves and
+    displacements (the flat keys then hold the last species')."""
+    d = dict(press=[], sites=[], lj={}, moves=[], ewald=None, bias=None, species=[])
     for line in text.splitlines():
         s = line.split()
-        if line.startswith("  state point"):
+        if line.startswith("  species "):
+            d["press"], d["sites"], d["moves"] = [], [], []
+            d["species"].append(dict(name=s[1].rstrip(":"), press=d["press"], sites=d["sites"], moves=d["moves"]))
+        elif line.startswith("  state point"):
             d["press"].append(float(s[-1]))
         elif line.startswith("  site "):
             d["sites"].append(dict(ref=[float(v) for v in s[2:5]], type=int(s[6]), q=float(s[8]),
@@ -86,6 +91,8 @@ def parse_check(text):
             d["moves"].append((s[2], float(s[-1])))
         elif line.startswith("  delta translate"):
             d["delta"] = (float(s[-2]), float(s[-1]))
+            if d["species"]:
+                d["species"][-1]["delta"] = d["delta"]
         elif line.startswith("  bias:"):
             d["bias"] = (s[1], float(s[3]))
         elif line.startswith("  ewald:"):
@@ -139,6 +146,85 @@ def bent_spec():
                 lj=(A, B, h2, l2, [1] * 4), T=300.0, ewald=dict(kappa=kappa, kmax=5, rc=40.0),
                 moves=[("RINSERT", 1.0), ("RDELETE", 1.0), ("RTRANSLATE", 1.0), ("RROTATE", 1.0)],
                 dtrans=0.3, drot=0.3)
+
+
+_SPC = re.compile(r"^\s+Species Name: (\S+)\s+Pressure:\s+(\S+) kPa")
+_LOADS = re.compile(r"^\s+(\S+):\s+P \(kPa\):\s+(\S+)\s+loading \(molec/uc\):\s+(\S+)")
+
+
+def parse_mix(text):
+    """A mixture run's output -> (blocks, ends): blocks[i] = dict(it, step,
+    sim, spc {name: dict(P, moves, N, blk, cum, std)}); ends[sim] =
+    dict(actual, stored, dev, loading {name: (P, loading)})."""
+    blocks, ends = [], {}
+    cur = sp = None
+    energies = []
+    last = None
+    for line in text.splitlines():
+        m = _MACRO.search(line)
+        if m:
+            cur = dict(it=int(m.group(1)), step=int(m.group(2)), sim=int(m.group(3)), spc={})
+            blocks.append(cur)
+            continue
+        m = _SPC.match(line)
+        if m and cur is not None:
+            sp = dict(P=float(m.group(2)), moves={})
+            cur["spc"][m.group(1)] = sp
+            continue
+        m = _MOVE.match(line)
+        if m and sp is not None:
+            sp["moves"][m.group(1)] = (int(m.group(4)), int(m.group(5)))
+            continue
+        m = _NMOL.search(line)
+        if m and sp is not None:
+            sp["N"], sp["blk"], sp["cum"], sp["std"] = [float(v) for v in m.groups()]
+            continue
+        m = _ENERGY.match(line)
+        if m:
+            energies.append(float(m.group(1)))
+            continue
+        m = _END.search(line)
+        if m:
+            last = int(m.group(1))
+            ends[last] = dict(actual=energies[-2], stored=energies[-1], dev=float(m.group(2)), loading={})
+            energies = []
+            continue
+        m = _LOADS.match(line)
+        if m and last is not None:
+            ends[last]["loading"][m.group(1)] = (float(m.group(2)), float(m.group(3)))
+    return blocks, ends
+
+
+def neutral_spec():
+    """Neutral.mol: one Patom site at the origin, q = 0; translate 0.5."""
+    return dict(ref=[[0.0, 0.0, 0.0]], sq=[0.0], moves=[("RINSERT", 2.0), ("RDELETE", 2.0), ("RTRANSLATE", 1.0)],
+                dtrans=0.5)
+
+
+def mix_specs():
+    """mix.ctr: the bent sorbate (types Ow 0, Hw 1) and the neutral one-site
+    sorbate (Patom 2) with one 3 x 3 LJ table: the self pairs of the fixture
+    and Lorentz-Berthelot cross pairs (hicut 9, locut 0.3), Ewald between
+    bent sorbates as in bent.ctr."""
+    bent = bent_spec()
+    self_ = {0: (3.166, 78.2, 0.5), 1: (1.2, 10.0, 0.3), 2: (3.0, 60.0, 0.5)}
+    A, B, h2, l2 = [], [], [], []
+    for i in range(3):
+        for j in range(3):
+            if i == j:
+                sig, eps, lo = self_[i]
+            else:
+                sig = (self_[i][0] + self_[j][0]) / 2
+                eps = np.sqrt(self_[i][1] * self_[j][1])
+                lo = 0.3
+            a, b = inputs.lj_AB(sig, eps)
+            A.append(a)
+            B.append(b)
+            h2.append(81.0)
+            l2.append(lo * lo)
+    bent = dict(bent, nst=3, lj=(A, B, h2, l2, [1] * 9))
+    neutral = dict(neutral_spec(), stype=[2])
+    return [bent, neutral]
 
 
 def read_restart_rp(path):

@@ -12,7 +12,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from music_gcmc_run import FIX, GCMC, GOLD, bent_spec, parse_check, probe_spec, spec_test007
+from music_gcmc_run import FIX, GCMC, GOLD, bent_spec, mix_specs, parse_check, probe_spec, spec_test007
 
 pytestmark = pytest.mark.skipif(not os.path.exists(GCMC),
                                 reason="music_gcmc not built (make -C music-code_amd fortran)")
@@ -160,3 +160,58 @@ def test_ewald_default_has_no_real_space_cutoff():
     to the kernel exceeds every minimum-image distance of the 20 A cell."""
     d = _check(os.path.join(FIX, "bent.ctr"))
     assert d["ewald"]["rc"] > 20.0 * np.sqrt(3.0) / 2.0
+
+
+def test_mixture_parameters():
+    """mix.ctr (the bent sorbate and a one-site LJ sorbate, gcmc_init reading
+    two species blocks): per species its pressures, sites, moves and
+    displacements; one LJ table over the site types of both species (Ow 0,
+    Hw 1, Patom 2) whose cross pairs are the Lorentz-Berthelot values of the
+    fixture's LBMIX lines; Ewald from the Bent-Bent line."""
+    d = _check(os.path.join(FIX, "mix.ctr"))
+    assert [s["name"] for s in d["species"]] == ["Bent", "Neutral"]
+    assert d["species"][0]["press"] == [5000.0, 8000.0] and d["species"][1]["press"] == [2000.0, 1000.0]
+    specs = mix_specs()
+    for s, spec in zip(d["species"], specs):
+        sub = dict(d, sites=s["sites"], moves=s["moves"], delta=s["delta"], press=s["press"])
+        _same_spec(sub, dict(spec, ewald=specs[0]["ewald"], lj=specs[0]["lj"], nst=3), 2)
+    assert [x["type"] for x in d["species"][1]["sites"]] == [2]
+
+
+def _mix_tree(tmp_path, ss_text=None, mol=None):
+    shutil.copytree(FIX, tmp_path / "fx", copy_function=shutil.copyfile)
+    shutil.copytree(os.path.join(GOLD, "potentials"), tmp_path / "potentials", copy_function=shutil.copyfile)
+    if ss_text is not None:
+        (tmp_path / "fx" / "spc_spc_mix").write_text(ss_text)
+    if mol is not None:
+        (tmp_path / "fx" / "Neutral.mol").write_text(mol)
+    return subprocess.run([GCMC, str(tmp_path / "fx" / "mix.ctr"), "check"], capture_output=True, text=True,
+                          timeout=60)
+
+
+def test_mixture_species_pair_rules(tmp_path):
+    """A species pair without its line has no interaction (the cross LJ
+    entries stay off); a charged second species needs the COUL SUM EWALD
+    lines of all its pairs with charged species, since one Ewald sum holds
+    every charge; Ewald lines with different parameters stop the driver."""
+    base = (tmp_path / "a")
+    base.mkdir()
+    out = _mix_tree(base, ss_text="Bent Bent NCOUL BASIC LJ FAST\nBent Bent COUL SUM FAST EWALD KAPPA@6.0 KMAX@5\n"
+                                  "Neutral Neutral NCOUL BASIC LJ FAST\n")
+    assert out.returncode == 0, out.stdout[-1500:]
+    d = parse_check(out.stdout)
+    assert sorted(d["lj"]) == [0, 1, 3, 4, 8]
+    charged = open(os.path.join(FIX, "Neutral.mol")).read().replace("Patom  0.00", "Patom  0.50")
+    (tmp_path / "b").mkdir()
+    out = _mix_tree(tmp_path / "b", mol=charged)
+    assert out.returncode != 0 and "COUL SUM EWALD" in out.stdout + out.stderr
+    (tmp_path / "c").mkdir()
+    txt = open(os.path.join(FIX, "spc_spc_mix")).read().replace(
+        "Bent     Neutral  COUL  OFF", "Bent     Neutral  COUL  SUM FAST EWALD KAPPA@5.0 KMAX@5")
+    out = _mix_tree(tmp_path / "c", ss_text=txt, mol=charged)
+    assert out.returncode != 0 and "one KAPPA@" in out.stdout + out.stderr
+    (tmp_path / "d").mkdir()
+    txt = txt.replace("KAPPA@5.0", "KAPPA@6.0").replace("Neutral  Neutral  COUL  OFF",
+                                                         "Neutral  Neutral  COUL  SUM FAST EWALD KAPPA@6.0 KMAX@5")
+    out = _mix_tree(tmp_path / "d", ss_text=txt, mol=charged)
+    assert out.returncode == 0, out.stdout[-1500:]

@@ -423,3 +423,103 @@ def test_music_gcmc_adjustable_displacements_resume(hip):
         assert disp2[-len(disp) // 2:] == disp[-len(disp) // 2:]
         got = open(os.path.join(td, "temp2.res.1"), "rb").read().split(b"\n")
         assert got[:1] + got[2:] == full[:1] + full[2:]
+
+
+def _oracle_mix_blocks(g, names, steps, every=500, block=250):
+    """Per print step and species of a GcmcMix chain: att/acc, N, block and
+    cumulative means of N (the species' sums over the common step count)."""
+    out = []
+    st0 = g.ch[0].st
+    base = st0.nsteps
+    sum0 = [c.st.sumN for c in g.ch]
+    done = 0
+    while done < steps:
+        g.run(every - block)
+        s0 = [c.st.sumN for c in g.ch]
+        g.run(block)
+        done += every
+        out.append({n: dict(att=[c.st.att[i] for i in range(4)], acc=[c.st.acc[i] for i in range(4)], N=c.st.N,
+                            blk=(c.st.sumN - s0[k]) / block, cum=(c.st.sumN - sum0[k]) / (st0.nsteps - base))
+                    for k, (n, c) in enumerate(zip(names, g.ch))})
+    return out
+
+
+def _compare_mix(blocks, oracle, tags=("INSERT", "DELETE", "TRANSLATE", "ROTATE")):
+    assert len(blocks) == len(oracle)
+    for b, o in zip(blocks, oracle):
+        assert sorted(b["spc"]) == sorted(o)
+        for n, ob in o.items():
+            bs = b["spc"][n]
+            for i, t in enumerate(tags):
+                if t in bs["moves"]:
+                    assert bs["moves"][t] == (ob["acc"][i], ob["att"][i]), (b["step"], n, t)
+            assert bs["N"] == ob["N"], (b["step"], n)
+            assert abs(bs["blk"] - ob["blk"]) <= 0.0051 and abs(bs["cum"] - ob["cum"]) <= 0.0051
+
+
+def test_music_gcmc_mixture(hip):
+    """mix.ctr (the bent sorbate with Ewald and a one-site LJ sorbate, two
+    state points): the batched run against oracle mixture chains
+    (mo_gcmc_run_mix) per point, species by species and move for move at
+    every printed step; the chained run against one oracle chain whose
+    fugacities change between the points; a run split at a restart and
+    resumed ends in the unbroken run's restart file."""
+    print("case: music_gcmc gcmc-maps/mix.ctr batched, chain, restart, FP64", flush=True)
+    names = ["Bent", "Neutral"]
+    fug = [[5000.0, 2000.0], [8000.0, 1000.0]]
+    specs = MG.mix_specs()
+    with tempfile.TemporaryDirectory() as td:
+        ctrl = os.path.join(MG.FIX, "mix.ctr")
+        blocks, ends = MG.parse_mix(MG.run([ctrl], td))
+        for p in range(2):
+            g = O.GcmcMix(specs, fug[p], 424242 + p)
+            _compare_mix([b for b in blocks if b["sim"] == p + 1], _oracle_mix_blocks(g, names, 2000))
+            assert abs(ends[p + 1]["stored"] - g.energy * 4.184) < 1.5e-3
+            assert ends[p + 1]["dev"] < 1e-8
+            assert [ends[p + 1]["loading"][n][0] for n in names] == fug[p]
+        assert min(b["spc"][n]["N"] for b in blocks[-2:] for n in names) > 0
+        # chained: the configuration of both species and the random stream
+        # carry over, the statistics restart
+        cblocks, _ = MG.parse_mix(MG.run([ctrl, "chain"], td))
+        g = O.GcmcMix(specs, fug[0], 424242)
+        for p in range(2):
+            if p:
+                for k, c in enumerate(g.ch):
+                    c.st.B = O.adams_B(fug[p][k], 8000.0, 300.0)
+                    for i in range(4):
+                        c.st.att[i] = 0
+                        c.st.acc[i] = 0
+                    c.st.sumN = 0.0
+                g.ch[0].st.nsteps = 0
+            _compare_mix([b for b in cblocks if b["sim"] == p + 1], _oracle_mix_blocks(g, names, 2000))
+    # restart files: one block per species; split at 1000 and resumed
+    with tempfile.TemporaryDirectory() as td:
+        txt = open(os.path.join(MG.FIX, "mix.ctr")).read()
+        ic = "100000               # No. of steps between writes to crash file"
+        assert ic in txt
+        full = txt.replace(ic, "2000                 # No. of steps between writes to crash file")
+        import shutil
+        shutil.copytree(MG.FIX, os.path.join(td, "f"), copy_function=shutil.copyfile)
+        shutil.copytree(os.path.join(GOLD, "potentials"), os.path.join(td, "potentials"),
+                        copy_function=shutil.copyfile)
+        open(os.path.join(td, "f", "full.ctr"), "w").write(full)
+        MG.run([os.path.join(td, "f", "full.ctr")], td)
+        want = open(os.path.join(td, "temp.res.1")).read().split("\n")
+        rp = MG.read_restart_rp(os.path.join(td, "temp.res.1"))
+        assert sorted(rp) == names and rp["Bent"].shape[1] == 3 and rp["Neutral"].shape[1] == 1
+        half = full.replace("2000                 # No. of iterations", "1000                 # No. of iterations")
+        half = half.replace("2000                 # No. of steps between writes to crash file",
+                            "1000                 # No. of steps between writes to crash file")
+        assert half != full
+        open(os.path.join(td, "f", "half.ctr"), "w").write(half)
+        MG.run([os.path.join(td, "f", "half.ctr")], td)
+        src = "------ Configuration Initialization -------------------------------------\n" \
+              "Bent\nRESTARTFILE ../temp.res.1\nNeutral\nRESTARTFILE ../temp.res.1\n"
+        res = half.replace("temp.res             # Restart File to write to",
+                           "temp2.res            # Restart File to write to").rstrip("\n") + "\n" + src
+        open(os.path.join(td, "f", "res.ctr"), "w").write(res)
+        out = MG.run([os.path.join(td, "f", "res.ctr"), "resume"], td)
+        assert out.count("started from") == 2
+        got = open(os.path.join(td, "temp2.res.1")).read().split("\n")
+        assert got[1].strip() == "1000" and want[1].strip() == "2000"
+        assert got[:1] + got[2:] == want[:1] + want[2:]
