@@ -265,6 +265,16 @@ typedef struct {
      * (rigidmoves_adjustdeltatrans / _adjustdeltarot, rigidmoves.F90:2586-2617,
      * called from mcmoves_perturb, mcmoves.F90:460-461).  Per replica. */
     int adjust_trans, adjust_rot;
+    /* Centre-of-mass cutoffs of the sorbate pair terms (the species-species
+     * options COMCUT@r / COMTRUNC@r of ssbasic_init, ssbasic.F90:139-160,
+     * applied in ssbasic_mmint, ssbasic.F90:745-756): radius (A) for the pairs
+     * of this species' molecules with those of species s2; a molecule pair
+     * whose geometric centres (the mean of the unfolded site positions,
+     * config_getMolecCTR, config.F90:1788-1802) are further apart, minimum
+     * image, has no LJ (lj_comcut) or plain-Coulomb (coul_comcut) term; the
+     * Ewald real-space term is not affected.  0 = no cutoff.  The radii of a
+     * mixture must be symmetric (species s to s2 = s2 to s). */
+    double lj_comcut[MH_MAX_SPECIES], coul_comcut[MH_MAX_SPECIES];
 } mh_gcmc_params;
 
 typedef struct mh_gcmc mh_gcmc;

@@ -74,6 +74,12 @@ struct GDev : SpcDev {
     int coul;
     int con[64];
     double cfac, ch2, cshift, cl2;
+    // centre-of-mass cutoffs (ssbasic COMCUT@ / COMTRUNC@, ssbasic.F90:
+    // 745-756), squared, per species pair: a molecule pair whose geometric
+    // centres are further apart has no LJ (cc_lj) or plain-Coulomb (cc_cl)
+    // term; 0 = no cutoff.  comcut: any of them set.
+    int comcut;
+    double cc_lj[GSPC][GSPC], cc_cl[GSPC][GSPC];
     const double *wk;      // [nk]
     const int8_t *kidx;    // [nk][4] (mx, my, mz, pad)
     double full[3], fund[3];
@@ -113,6 +119,7 @@ struct Rep {
     double *B, *energy, *sumN, *sumN2;  // B, sumN, sumN2: [rep][GSPC]
     long long *nsteps, *att, *acc;      // att, acc: [rep][GSPC][4]
     double *com, *ang, *pos;  // [rep][slots][3], [rep][slots][3], [rep][slots][nsmax][3]
+    double *ctr;              // [rep][slots][3] geometric centres (centre-of-mass cutoffs)
     double *delta;            // [rep][GSPC][2] translate / rotate displacement in use
     double2 *S, *dS;          // [rep][nk]
     int *tr_move, *tr_acc, *tr_N;
@@ -147,13 +154,27 @@ __device__ __forceinline__ void euler_row(int i, double sp, double cp, double st
     }
 }
 
-// coordinate i of site a: rigidcoords_toxyz (rigidcoords.F90:254-276) + pbc
-__device__ __forceinline__ double site_coord(const GDev &P, int a, int i, const double com[3], const double m[3]) {
+// coordinate i of site a, unfolded: rigidcoords_toxyz (rigidcoords.F90:
+// 254-276); site_coord adds the pbc
+__device__ __forceinline__ double site_coord_unf(const GDev &P, int a, int i, const double com[3],
+                                                 const double m[3]) {
     double d = 0.0;
     d = __dadd_rn(d, __dmul_rn(m[0], P.ref[a][0]));
     d = __dadd_rn(d, __dmul_rn(m[1], P.ref[a][1]));
     d = __dadd_rn(d, __dmul_rn(m[2], P.ref[a][2]));
-    return wrap_full(__dadd_rn(com[i], d), P.full[i]);
+    return __dadd_rn(com[i], d);
+}
+
+__device__ __forceinline__ double site_coord(const GDev &P, int a, int i, const double com[3], const double m[3]) {
+    return wrap_full(site_coord_unf(P, a, i, com, m), P.full[i]);
+}
+
+// config_getMolecCTR (config.F90:1788-1802): coordinate i of the mean of the
+// ns unfolded site positions rp[3 a + i], summed in site order
+__device__ __forceinline__ double mol_centre(const double *rp, int ns, int i) {
+    double c = 0.0;
+    for (int a = 0; a < ns; a++) c = __dadd_rn(c, rp[3 * a + i]);
+    return __ddiv_rn(c, (double)ns);
 }
 
 // simcell_pbc of one position in a slanted cell (fundcell.F90:640-709 basis):
@@ -180,8 +201,10 @@ __device__ __forceinline__ void min_image(const GDev &P, const double *a, const 
     mat_vec3(P.sc.U, d, sep);
 }
 
-// all site coordinates of one molecule by one lane (host-side paths)
-__device__ void site_coords(const GDev &P, const double com[3], const double ang[3], double *out) {
+// all site coordinates of one molecule by one lane (host-side paths); unf
+// (optional) receives them unfolded
+__device__ void site_coords(const GDev &P, const double com[3], const double ang[3], double *out,
+                            double *unf = nullptr) {
     if (P.slanted) {
         if (P.ns == 1) {
             for (int i = 0; i < 3; i++) out[i] = com[i];
@@ -202,11 +225,16 @@ __device__ void site_coords(const GDev &P, const double com[3], const double ang
                 }
             }
         }
+        if (unf)
+            for (int i = 0; i < 3 * P.ns; i++) unf[i] = out[i];
         for (int a = 0; a < P.ns; a++) slant_pbc(P, out + 3 * a);
         return;
     }
     if (P.ns == 1) {
-        for (int i = 0; i < 3; i++) out[i] = wrap_full(com[i], P.full[i]);
+        for (int i = 0; i < 3; i++) {
+            out[i] = wrap_full(com[i], P.full[i]);
+            if (unf) unf[i] = com[i];
+        }
         return;
     }
     double sp, cp, st, ct, ss, cs;
@@ -216,7 +244,11 @@ __device__ void site_coords(const GDev &P, const double com[3], const double ang
     for (int i = 0; i < 3; i++) {
         double m[3];
         euler_row(i, sp, cp, st, ct, ss, cs, m);
-        for (int a = 0; a < P.ns; a++) out[3 * a + i] = site_coord(P, a, i, com, m);
+        for (int a = 0; a < P.ns; a++) {
+            const double u = site_coord_unf(P, a, i, com, m);
+            out[3 * a + i] = wrap_full(u, P.full[i]);
+            if (unf) unf[3 * a + i] = u;
+        }
     }
 }
 
@@ -317,8 +349,9 @@ __device__ bool site_map_energy(const GDev &P, const SpcDev &Q, int a, const dou
 // (331.5 shield q1 q2)(1/r - 1/hicut) within hicut; false below locut.  FP64
 // in both kernel precisions: without a cutoff thousands of +-1/r terms
 // cancel, and FP32 pair terms would leave ~1e-5 of the net.
-__device__ __forceinline__ bool coul_energy64(const GDev &P, int t, double qa, double qb, double r2, double &e) {
-    if (P.con[t] && !(P.ch2 > 0.0 && r2 > P.ch2)) {
+__device__ __forceinline__ bool coul_energy64(const GDev &P, int t, double qa, double qb, double r2, double &e,
+                                              bool pair_on = true) {
+    if (pair_on && P.con[t] && !(P.ch2 > 0.0 && r2 > P.ch2)) {
         if (r2 < P.cl2) return false;
         const double r = sqrt(r2);
         e = __dadd_rn(e, __dmul_rn(__dmul_rn(__dmul_rn(P.cfac, qa), qb), __dsub_rn(1.0 / r, P.cshift)));
@@ -328,10 +361,11 @@ __device__ __forceinline__ bool coul_energy64(const GDev &P, int t, double qa, d
 
 // pair energy of a site pair (lj_snglint without derivatives + Ewald real
 // space + plain Coulomb).  Returns false on an LJ or Coulomb locut violation.
+// lj_on / cl_on: the molecule pair is within its centre-of-mass cutoffs.
 __device__ __forceinline__ bool pair_energy64(const GDev &P, int t, double qa, double qb, double sx, double sy,
-                                              double sz, double &e) {
+                                              double sz, double &e, bool lj_on = true, bool cl_on = true) {
     double r2 = __dadd_rn(__dadd_rn(__dmul_rn(sx, sx), __dmul_rn(sy, sy)), __dmul_rn(sz, sz));
-    if (P.on[t] && !(r2 > P.h2[t])) {
+    if (lj_on && P.on[t] && !(r2 > P.h2[t])) {
         if (r2 < P.l2[t]) return false;
         double r2i = 1.0 / r2;
         double r6i = __dmul_rn(__dmul_rn(r2i, r2i), r2i);
@@ -342,14 +376,14 @@ __device__ __forceinline__ bool pair_energy64(const GDev &P, int t, double qa, d
         double rr = sqrt(r2);
         e = __dadd_rn(e, __dmul_rn(__ddiv_rn(__dmul_rn(__dmul_rn(qa, qb), erfc_as64(P.kappa * rr)), rr), kTokcalG));
     }
-    return P.coul ? coul_energy64(P, t, qa, qb, r2, e) : true;
+    return P.coul ? coul_energy64(P, t, qa, qb, r2, e, cl_on) : true;
 }
 
 __device__ __forceinline__ bool pair_energy32(const GDev &P, int t, float qq, float kappa, float rc2, float sx,
-                                              float sy, float sz, float &e) {
+                                              float sy, float sz, float &e, bool lj_on = true) {
 #pragma clang fp contract(fast)
     const float r2 = sx * sx + sy * sy + sz * sz;
-    if (P.on[t] && !(r2 > P.h2f[t])) {
+    if (lj_on && P.on[t] && !(r2 > P.h2f[t])) {
         if (r2 < P.l2f[t]) return false;
         const float r2i = rcp_fast(r2);
         const float r6i = r2i * r2i * r2i;
@@ -367,6 +401,8 @@ struct Shared {
     int N[GSPC];                         // molecules per species (species of the move: its count)
     int spc;                             // species of the move in progress
     double posA[GS * 3], posB[GS * 3];   // trial (A) and old (B) site coordinates
+    double rpA[GS * 3];                  // trial sites unfolded (centre-of-mass cutoffs)
+    double ctr[2][3];                    // geometric centres of sets A and B
     double sc[3][2];                     // sin, cos of the trial Euler angles
     double mapv[2][GS];
     int mapok[2][GS];
@@ -456,13 +492,22 @@ __device__ __forceinline__ double2 k_delta(const GDev &P, const Shared &sh, int 
 // its species, N = that species' count) and the reciprocal-space delta,
 // whose dS(k) is left in K.d (registers) or dSg (KPL == 0).
 // out: map A, pairs A, map B, pairs B, reciprocal delta, ok(A).
+// Centre-of-mass cutoffs (P.comcut): ctrg = the replica's molecule centres
+// [slots][3]; set A's centre is cA (a stored molecule) or the mean of
+// sh.rpA (a trial), set B's is cB.
 template <int NT, int PREC, int KPL, bool SL = false>
 __device__ __forceinline__ void move_energy(const GDev &P, Shared &sh, const double *__restrict__ pos, int N,
                                             int skip, bool hasA, bool hasB, double asign, KReg<KPL> &K,
                                             const double2 *__restrict__ Sg, double2 *__restrict__ dSg,
-                                            void *exy, double out[6]) {
+                                            void *exy, double out[6], const double *__restrict__ ctrg,
+                                            const double *cA, const double *cB) {
     const int tid = threadIdx.x;
     const int ns = P.ns;
+    if (P.comcut && tid >= NT - 6) {  // read after the barrier below
+        const int set = (tid - (NT - 6)) / 3, i = (tid - (NT - 6)) % 3;
+        if (set == 0 && hasA) sh.ctr[0][i] = cA ? cA[i] : mol_centre(sh.rpA, ns, i);
+        if (set == 1 && hasB) sh.ctr[1][i] = cB[i];
+    }
     // maps: item l = (set * ns + site) * 8 + corner; the 8 corner lanes of a
     // site are an aligned lane group
     for (int l = tid; l < 16 * ns; l += NT) {
@@ -542,6 +587,8 @@ __device__ __forceinline__ void move_energy(const GDev &P, Shared &sh, const dou
         const int items = n2 * nss;
         const double inv_nss = own ? P.inv_nss : 1.0 / (double)nss;
         const double *pos2 = pos + (long long)s2 * P.maxN * P.nsmax * 3;
+        const double *ctr2 = ctrg + (long long)s2 * P.maxN * 3;
+        const double ccl = P.cc_lj[sh.spc][s2], ccc = P.cc_cl[sh.spc][s2];
         for (int w = tid; w < items; w += NT) {
             const int j = (int)(((double)w + 0.5) * inv_nss);
             if (own && j == skip) continue;
@@ -553,31 +600,54 @@ __device__ __forceinline__ void move_energy(const GDev &P, Shared &sh, const dou
             const int t = P.st[a] * P.nst + O.st[b];
             const double qb = O.q[b];
             const double pb[3] = {xb, yb, zb};
+            // ssbasic_mmint's centre-to-centre test per set (ssbasic.F90:
+            // 745-756): beyond the radius the pair's term is skipped
+            bool ljA = true, clA = true, ljB = true, clB = true;
+            if (P.comcut) {
+                const double *cj = ctr2 + 3LL * j;
+                for (int set = 0; set < 2; set++) {
+                    if (!(set == 0 ? hasA : hasB)) continue;
+                    double sc[3];
+                    min_image(P, sh.ctr[set], cj, sc, SL);
+                    const double d2 =
+                        __dadd_rn(__dadd_rn(__dmul_rn(sc[0], sc[0]), __dmul_rn(sc[1], sc[1])), __dmul_rn(sc[2], sc[2]));
+                    const bool lo = !(ccl > 0.0 && d2 > ccl), co = !(ccc > 0.0 && d2 > ccc);
+                    if (set == 0) {
+                        ljA = lo;
+                        clA = co;
+                    } else {
+                        ljB = lo;
+                        clB = co;
+                    }
+                }
+            }
             if (PREC == MH_PREC_FP64) {
                 if (hasA) {
                     double sp[3];
                     min_image(P, &sh.posA[3 * a], pb, sp, SL);
-                    if (!pair_energy64(P, t, P.q[a], qb, sp[0], sp[1], sp[2], eA)) failA = 1.0;
+                    if (!pair_energy64(P, t, P.q[a], qb, sp[0], sp[1], sp[2], eA, ljA, clA)) failA = 1.0;
                 }
                 if (hasB) {
                     double sp[3];
                     min_image(P, &sh.posB[3 * a], pb, sp, SL);
-                    pair_energy64(P, t, P.q[a], qb, sp[0], sp[1], sp[2], eB);
+                    pair_energy64(P, t, P.q[a], qb, sp[0], sp[1], sp[2], eB, ljB, clB);
                 }
             } else {
                 const float qq = P.ewald ? (float)(P.q[a] * qb * kTokcalG) : 0.0f;
                 if (hasA) {
                     double d[3];
                     min_image(P, &sh.posA[3 * a], pb, d, SL);
-                    if (!pair_energy32(P, t, qq, kap32, rc32, (float)d[0], (float)d[1], (float)d[2], eAf)) failA = 1.0;
-                    if (P.coul && !coul_energy64(P, t, P.q[a], qb, d[0] * d[0] + d[1] * d[1] + d[2] * d[2], eA))
+                    if (!pair_energy32(P, t, qq, kap32, rc32, (float)d[0], (float)d[1], (float)d[2], eAf, ljA))
+                        failA = 1.0;
+                    if (P.coul &&
+                        !coul_energy64(P, t, P.q[a], qb, d[0] * d[0] + d[1] * d[1] + d[2] * d[2], eA, clA))
                         failA = 1.0;
                 }
                 if (hasB) {
                     double d[3];
                     min_image(P, &sh.posB[3 * a], pb, d, SL);
-                    pair_energy32(P, t, qq, kap32, rc32, (float)d[0], (float)d[1], (float)d[2], eBf);
-                    if (P.coul) coul_energy64(P, t, P.q[a], qb, d[0] * d[0] + d[1] * d[1] + d[2] * d[2], eB);
+                    pair_energy32(P, t, qq, kap32, rc32, (float)d[0], (float)d[1], (float)d[2], eBf, ljB);
+                    if (P.coul) coul_energy64(P, t, P.q[a], qb, d[0] * d[0] + d[1] * d[1] + d[2] * d[2], eB, clB);
                 }
             }
         }
@@ -777,6 +847,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
     double *gcom = R.com + (long long)rep * slots * 3;
     double *gang = R.ang + (long long)rep * slots * 3;
     double *gpos = R.pos + (long long)rep * slots * ms3;
+    double *gctr = R.ctr + (long long)rep * slots * 3;  // centres stay in HBM
     double2 *S = R.S + (long long)rep * R.kstride;
     double2 *dS = R.dS + (long long)rep * R.kstride;
     double *pos, *com, *ang;
@@ -841,6 +912,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
         const double B = sB[s];
         double *comS = com + (long long)s * P.maxN * 3, *angS = ang + (long long)s * P.maxN * 3;
         double *posS = pos + (long long)s * P.maxN * ms3;
+        double *ctrS = gctr + (long long)s * P.maxN * 3;
         const double prob = rranf(rng);
         int mv = 0;
         for (int i = 0; i < P.nmoves; i++) {
@@ -958,9 +1030,12 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
             if (mbase == MH_MV_DELETE) {
                 if (tid < n3) sh.posA[tid] = posS[mol * ms3 + tid];
             } else if (SL) {
-                if (tid == 0) site_coords(P, tcom, tang, sh.posA);
+                if (tid == 0) site_coords(P, tcom, tang, sh.posA, sh.rpA);
             } else if (ns == 1) {
-                if (tid < 3) sh.posA[tid] = wrap_full(tcom[tid], P.full[tid]);
+                if (tid < 3) {
+                    sh.posA[tid] = wrap_full(tcom[tid], P.full[tid]);
+                    sh.rpA[tid] = tcom[tid];
+                }
             } else {
                 if (tid < 3) {
                     double s_, c_;
@@ -973,16 +1048,20 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                     const int a = tid / 3, i = tid - 3 * a;
                     double m[3];
                     euler_row(i, sh.sc[0][0], sh.sc[0][1], sh.sc[1][0], sh.sc[1][1], sh.sc[2][0], sh.sc[2][1], m);
-                    sh.posA[tid] = site_coord(P, a, i, tcom, m);
+                    const double u = site_coord_unf(P, a, i, tcom, m);
+                    sh.posA[tid] = wrap_full(u, P.full[i]);
+                    sh.rpA[tid] = u;
                 }
             }
             __syncthreads();
             double o[6];
             if (mbase == MH_MV_DELETE) {
                 // the molecule's energy in the current system: its sites as set A
-                move_energy<NT, PREC, KPL, SL>(P, sh, pos, N, mol, true, false, -1.0, K, S, dS, exy, o);
+                move_energy<NT, PREC, KPL, SL>(P, sh, pos, N, mol, true, false, -1.0, K, S, dS, exy, o, gctr,
+                                               ctrS + 3 * mol, nullptr);
             } else {
-                move_energy<NT, PREC, KPL, SL>(P, sh, pos, N, mol, hasA, hasB, 1.0, K, S, dS, exy, o);
+                move_energy<NT, PREC, KPL, SL>(P, sh, pos, N, mol, hasA, hasB, 1.0, K, S, dS, exy, o, gctr, nullptr,
+                                               hasB ? ctrS + 3 * mol : nullptr);
             }
             const double mA = o[0], pA = o[1], mB = o[2], pB = o[3], eK = o[4];
             bool ok = o[5] != 0.0;
@@ -1004,6 +1083,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                             angS[3 * N + tid] = tang[tid];
                         }
                         if (tid < n3) posS[N * ms3 + tid] = sh.posA[tid];
+                        if (P.comcut && tid < 3) ctrS[3 * N + tid] = sh.ctr[0][tid];
                         if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                         N++;
                         energy += U;
@@ -1033,6 +1113,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                         angS[3 * mol + tid] = angS[3 * last + tid];
                     }
                     if (tid < n3) posS[mol * ms3 + tid] = posS[last * ms3 + tid];
+                    if (P.comcut && tid < 3) ctrS[3 * mol + tid] = ctrS[3 * last + tid];
                     if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                     N--;
                     energy -= U;
@@ -1056,6 +1137,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                             angS[3 * mol + tid] = tang[tid];
                         }
                         if (tid < n3) posS[mol * ms3 + tid] = sh.posA[tid];
+                        if (P.comcut && tid < 3) ctrS[3 * mol + tid] = sh.ctr[0][tid];
                         if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                         energy += dU;
                         acc[mtype]++;
@@ -1167,15 +1249,17 @@ __global__ void __launch_bounds__(NT) k_trial(const GDev *__restrict__ Ptab, Rep
     const int N = sh.N[spc];
     const double *pos = R.pos + (long long)rep * P.nspc * P.maxN * P.nsmax * 3;
     const double2 *S = R.S + (long long)rep * R.kstride;
-    if (tid == 0) site_coords(P, tcom + 3 * t, tang + 3 * t, sh.posA);
+    const double *ctrg = R.ctr + (long long)rep * P.nspc * P.maxN * 3;
+    if (tid == 0) site_coords(P, tcom + 3 * t, tang + 3 * t, sh.posA, sh.rpA);
     __syncthreads();
     double o[6];
     KReg<0> K;
     if (P.slanted)
         move_energy<NT, PREC, 0, true>(P, sh, pos, N, -1, true, false, 1.0, K, S, dSbuf + (long long)t * P.nk,
-                                       nullptr, o);
+                                       nullptr, o, ctrg, nullptr, nullptr);
     else
-        move_energy<NT, PREC, 0>(P, sh, pos, N, -1, true, false, 1.0, K, S, dSbuf + (long long)t * P.nk, nullptr, o);
+        move_energy<NT, PREC, 0>(P, sh, pos, N, -1, true, false, 1.0, K, S, dSbuf + (long long)t * P.nk, nullptr, o,
+                                 ctrg, nullptr, nullptr);
     if (tid == 0) {
         double U = __dadd_rn(o[0], o[1]);
         if (P.ewald) U = U + o[4] + P.molconst;
@@ -1234,6 +1318,7 @@ __global__ void __launch_bounds__(NT) k_total(const GDev *__restrict__ Ptab, Rep
     const int rep = blockIdx.x;
     const long long ms3 = 3LL * P.nsmax, per = (long long)P.maxN * ms3;
     const double *pos = R.pos + (long long)rep * P.nspc * per;
+    const double *ctr = R.ctr + (long long)rep * P.nspc * P.maxN * 3;
     int Ns[GSPC], ntot = 0;
     for (int s = 0; s < GSPC; s++) {
         Ns[s] = s < P.nspc ? R.N[GSPC * rep + s] : 0;
@@ -1255,11 +1340,22 @@ __global__ void __launch_bounds__(NT) k_total(const GDev *__restrict__ Ptab, Rep
             const int nj = s2 < s ? Ns[s2] : m;
             for (int j = 0; j < nj; j++) {
                 const double *pj = pos + s2 * per + j * ms3;
+                bool ljo = true, clo = true;
+                if (P.comcut) {
+                    double sc[3];
+                    min_image(P, ctr + 3 * ((long long)s * P.maxN + m), ctr + 3 * ((long long)s2 * P.maxN + j), sc,
+                              P.slanted != 0);
+                    const double d2 =
+                        __dadd_rn(__dadd_rn(__dmul_rn(sc[0], sc[0]), __dmul_rn(sc[1], sc[1])), __dmul_rn(sc[2], sc[2]));
+                    ljo = !(P.cc_lj[s][s2] > 0.0 && d2 > P.cc_lj[s][s2]);
+                    clo = !(P.cc_cl[s][s2] > 0.0 && d2 > P.cc_cl[s][s2]);
+                }
                 for (int a = 0; a < Q.ns; a++)
                     for (int b = 0; b < O.ns; b++) {
                         double sp[3];
                         min_image(P, pm + 3 * a, pj + 3 * b, sp, P.slanted != 0);
-                        if (!pair_energy64(P, Q.st[a] * P.nst + O.st[b], Q.q[a], O.q[b], sp[0], sp[1], sp[2], e))
+                        if (!pair_energy64(P, Q.st[a] * P.nst + O.st[b], Q.q[a], O.q[b], sp[0], sp[1], sp[2], e, ljo,
+                                           clo))
                             bad = 1.0;
                     }
             }
@@ -1330,7 +1426,7 @@ struct mh_gcmc {
     DBuf dP;                    // GDev[sys.size()]
     DBuf sysidx, budget;        // pool only
     DBuf N, err, ix, iy, first, iseed, B, energy, sumN, sumN2, nsteps, att, acc;
-    DBuf com, ang, pos, S, dS, delta;
+    DBuf com, ang, pos, ctr, S, dS, delta;
     DBuf tr_move, tr_acc, tr_N, tr_dE;
     int tr_cap = 0;
     long long tr_base = 0;
@@ -1364,6 +1460,7 @@ struct mh_gcmc {
         r.com = com.as<double>();
         r.ang = ang.as<double>();
         r.pos = pos.as<double>();
+        r.ctr = ctr.as<double>();
         r.delta = delta.as<double>();
         r.S = S.as<double2>();
         r.dS = dS.as<double2>();
@@ -1578,6 +1675,15 @@ static int build_system(mh_ctx *ctx, const mh_gcmc_params *p, int maxN, const do
         H.cshift = p->coul_hicut > 0.0 ? 1.0 / p->coul_hicut : 0.0;
         H.cl2 = p->coul_locut * p->coul_locut;
     }
+    // centre-of-mass cutoffs of species 0 towards each species (the rows of
+    // the other species of a mixture are filled by mh_gcmc_create_mix)
+    for (int s2 = 0; s2 < GSPC; s2++) {
+        MH_CHECK(p->lj_comcut[s2] >= 0.0 && p->coul_comcut[s2] >= 0.0, MH_EINVAL,
+                 w + ": centre-of-mass cutoffs must be >= 0");
+        H.cc_lj[0][s2] = p->lj_comcut[s2] * p->lj_comcut[s2];
+        H.cc_cl[0][s2] = p->coul_comcut[s2] * p->coul_comcut[s2];
+        if (p->lj_comcut[s2] > 0.0 || p->coul_comcut[s2] > 0.0) H.comcut = 1;
+    }
     H.ewald = p->ewald ? 1 : 0;
     H.kmax = p->ewald ? p->ewald_kmax : 0;
     H.kappa = p->ewald_kappa;
@@ -1717,6 +1823,7 @@ static int alloc_replicas(mh_gcmc *g) {
         (rc = g->acc.alloc(32 * Q)) || (rc = g->delta.alloc(16 * Q)) ||
         (rc = g->com.alloc(sizeof(double) * 3 * R * slots)) || (rc = g->ang.alloc(sizeof(double) * 3 * R * slots)) ||
         (rc = g->pos.alloc(sizeof(double) * 3 * R * slots * g->ns)) ||
+        (rc = g->ctr.alloc(sizeof(double) * 3 * R * slots)) ||
         (rc = g->S.alloc(sizeof(double2) * (R * g->kstride + 1))) ||
         (rc = g->dS.alloc(sizeof(double2) * (R * g->kstride + 1))) || (rc = g->tr_move.alloc(4)) ||
         (rc = g->tr_acc.alloc(4)) || (rc = g->tr_N.alloc(4)) || (rc = g->tr_dE.alloc(8)))
@@ -1822,7 +1929,18 @@ int mh_gcmc_create_mix(mh_ctx *ctx, const mh_gcmc_params *spc, int nspc, int nre
         if ((rc = build_system(ctx, &q, maxN, ctx->fund_ell, ctx->full_ell, T, "mh_gcmc_create", true))) return rc;
         H.sp[sp] = static_cast<const SpcDev &>(T.H);
         H.nsmax = std::max(H.nsmax, T.H.ns);
+        for (int s2 = 0; s2 < GSPC; s2++) {
+            MH_CHECK(o.lj_comcut[s2] >= 0.0 && o.coul_comcut[s2] >= 0.0, MH_EINVAL,
+                     "mh_gcmc_create: centre-of-mass cutoffs must be >= 0");
+            H.cc_lj[sp][s2] = o.lj_comcut[s2] * o.lj_comcut[s2];
+            H.cc_cl[sp][s2] = o.coul_comcut[s2] * o.coul_comcut[s2];
+            if (o.lj_comcut[s2] > 0.0 || o.coul_comcut[s2] > 0.0) H.comcut = 1;
+        }
     }
+    for (int a = 0; a < nspc; a++)
+        for (int b = 0; b < nspc; b++)
+            MH_CHECK(H.cc_lj[a][b] == H.cc_lj[b][a] && H.cc_cl[a][b] == H.cc_cl[b][a], MH_EINVAL,
+                     "mh_gcmc_create: centre-of-mass cutoffs of a species pair differ between its two species");
     H.nspc = nspc;
     g->ns = H.nsmax;
     g->slanted = H.slanted != 0;
@@ -2018,11 +2136,12 @@ int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const doub
     const GDev &G0 = g->sys_of(rep);
     const SpcDev &Q = G0.sp[g->sel];
     const int ns = Q.ns;
-    std::vector<double> hc(3 * (size_t)n), ha(3 * (size_t)n, 0.0), hp(3 * (size_t)n * ns);
+    std::vector<double> hc(3 * (size_t)n), ha(3 * (size_t)n, 0.0), hp(3 * (size_t)n * ns), hctr(3 * (size_t)n);
     for (int m = 0; m < n; m++) {
         for (int i = 0; i < 3; i++) {
             hc[3 * m + i] = com[3 * m + i];
             if (ang) ha[3 * m + i] = ang[3 * m + i];
+            hctr[3 * m + i] = hc[3 * m + i];  // one site: its unfolded position
         }
         // host restatement of site_coords (rigidcoords_toxyz + simcell_pbc)
         auto slant_pbc_h = [&](double *r) {
@@ -2055,14 +2174,18 @@ int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const doub
         double M[3][3] = {{cs * cp - ct * sp * ss, -ss * cp - ct * sp * cs, st * sp},
                           {cs * sp + ct * cp * ss, -ss * sp + ct * cp * cs, -st * cp},
                           {ss * st, cs * st, ct}};
+        double csum[3] = {0.0, 0.0, 0.0};
         for (int a = 0; a < ns; a++) {
             for (int i = 0; i < 3; i++) {
                 double d = M[i][0] * Q.ref[a][0] + M[i][1] * Q.ref[a][1] + M[i][2] * Q.ref[a][2];
                 double rp = hc[3 * m + i] + d;
+                csum[i] = csum[i] + rp;
                 hp[(3 * m * ns) + 3 * a + i] = G0.slanted ? rp : rp - std::floor(rp / G0.full[i]) * G0.full[i];
             }
             if (G0.slanted) slant_pbc_h(&hp[(3 * m * ns) + 3 * a]);
         }
+        // config_getMolecCTR: the mean of the unfolded site positions
+        for (int i = 0; i < 3; i++) hctr[3 * m + i] = csum[i] / (double)ns;
     }
     // species g->sel's slots: [sel maxN, (sel + 1) maxN) of the replica, g->ns sites each
     const long long slot0 = (long long)rep * g->nspc * g->maxN + (long long)g->sel * g->maxN;
@@ -2070,6 +2193,7 @@ int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const doub
     if (n) {
         MH_HIP(hipMemcpy(g->com.as<double>() + off3, hc.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy(g->ang.as<double>() + off3, ha.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        MH_HIP(hipMemcpy(g->ctr.as<double>() + off3, hctr.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
         MH_HIP(hipMemcpy2D(g->pos.as<double>() + offp, sizeof(double) * 3 * g->ns, hp.data(),
                            sizeof(double) * 3 * ns, sizeof(double) * 3 * ns, n, hipMemcpyHostToDevice));
     }
@@ -2339,7 +2463,7 @@ int mh_gcmc_autotune(mh_gcmc *g, int probe_steps, int *threads_out, int *exy_out
     hipStream_t st = g->ctx->stream;
     // snapshot of every buffer a run modifies
     std::vector<DBuf *> bufs = {&g->N, &g->err, &g->ix, &g->iy, &g->first, &g->iseed, &g->energy, &g->sumN,
-                                &g->sumN2, &g->nsteps, &g->att, &g->acc, &g->com, &g->ang, &g->pos, &g->S,
+                                &g->sumN2, &g->nsteps, &g->att, &g->acc, &g->com, &g->ang, &g->pos, &g->ctr, &g->S,
                                 &g->delta};
     if (g->pool) bufs.push_back(&g->budget);
     const int nb = (int)bufs.size();

@@ -296,6 +296,13 @@ program music_gcmc
        P%ewald_kmax, ' rcut ', P%ewald_rcut
   if (P%coul /= 0) write(*, '(a, es24.16, a, es24.16, a, es24.16)') '  plain coulomb: hicut ', P%coul_hicut, &
        ' locut ', P%coul_locut, ' shield ', P%coul_shield
+  do k = 1, nspc
+     do j = k, nspc
+        if (PS(k)%lj_comcut(j) > 0.0d0 .or. PS(k)%coul_comcut(j) > 0.0d0) &
+             write(*, '(a, 2(1x, a), a, 2es24.16)') '  centre cutoff', trim(spnames(k)), trim(spnames(j)), &
+             ': lj / coul', PS(k)%lj_comcut(j), PS(k)%coul_comcut(j)
+     end do
+  end do
   if (dfon) then
      if (nspc > 1) call fail('datafile: configuration files are written for one GCMC species')
      write(*, '(a, i0, 2a)') '  configuration files every ', iconfig, ' iterations: ', trim(confname)
@@ -1023,12 +1030,15 @@ contains
     character(len=LL) :: s, kind, tok
     integer :: l, wv, k1, k2, b, c, s1, s2, ncl
     integer :: ljon(MH_MAX_SPECIES, MH_MAX_SPECIES), ckind(MH_MAX_SPECIES, MH_MAX_SPECIES)
+    real(c_double) :: ljcc(MH_MAX_SPECIES, MH_MAX_SPECIES), clcc(MH_MAX_SPECIES, MH_MAX_SPECIES), rcc
     real(c_double) :: ek, er
     integer(c_int) :: kmax
     logical :: ewset
     call read_text(find_file(sy%ssfile, 'NONE', '', sy%base), ss)
     ljon = 0
     ckind = 0
+    ljcc = 0.0d0
+    clcc = 0.0d0
     ewset = .false.
     call use_species(1)
     do l = 1, ss%n
@@ -1043,13 +1053,18 @@ contains
        if (trim(kind) == 'NCOUL') then
           if (index(upper(s), 'LJ') == 0) call fail('species-species NCOUL model must be LJ: ' // trim(s))
           ! ssbasic_init options (ssbasic.F90:125-177): FAST is what the
-          ! kernel does; a centre-of-mass cutoff (COMCUT@, COMTRUNC@),
+          ! kernel does; COMCUT@r / COMTRUNC@r set the centre-of-mass cutoff;
           ! SMOOTH@ and NEIGHBOR change the energy and are not offered
           do wv = 4, nwords(s)
              tok = upper(word(s, wv))
              if (trim(tok) == 'BASIC' .or. trim(tok) == 'LJ' .or. trim(tok) == 'FAST') cycle
-             call fail('species-species option ' // trim(word(s, wv)) // ' is not offered (BASIC LJ FAST): ' &
-                  // trim(s))
+             if (centre_cutoff(tok, rcc)) then
+                ljcc(s1, s2) = rcc
+                ljcc(s2, s1) = rcc
+                cycle
+             end if
+             call fail('species-species option ' // trim(word(s, wv)) // ' is not offered (BASIC LJ FAST ' // &
+                  'COMCUT@ COMTRUNC@): ' // trim(s))
           end do
           ljon(s1, s2) = 1
           ljon(s2, s1) = 1
@@ -1074,8 +1089,13 @@ contains
           do wv = 5, nwords(s)
              tok = upper(word(s, wv))
              if (trim(tok) == 'FAST') cycle
-             call fail('species-species option ' // trim(word(s, wv)) // ' is not offered (BASIC FAST): ' &
-                  // trim(s))
+             if (centre_cutoff(tok, rcc)) then
+                clcc(s1, s2) = rcc
+                clcc(s2, s1) = rcc
+                cycle
+             end if
+             call fail('species-species option ' // trim(word(s, wv)) // ' is not offered (BASIC FAST ' // &
+                  'COMCUT@ COMTRUNC@): ' // trim(s))
           end do
           ckind(s1, s2) = 1
           ckind(s2, s1) = 1
@@ -1136,6 +1156,10 @@ contains
        end do
     end do
     call save_species(1)
+    do s1 = 1, nspc
+       PS(s1)%lj_comcut(1:nspc) = ljcc(s1, 1:nspc)
+       PS(s1)%coul_comcut(1:nspc) = clcc(s1, 1:nspc)
+    end do
     ! species-framework maps
     do cur = 1, nspc
        call use_species(cur)
@@ -1163,6 +1187,50 @@ contains
     end do
     call use_species(1)
   end subroutine read_species_file
+
+  ! ssbasic_init's centre-of-mass cutoff options (ssbasic.F90:139-160):
+  ! COMCUT@r (r at most half the simulation cell's smallest width, else the
+  ! reference stops) and COMTRUNC@r (no such check)
+  logical function centre_cutoff(tok, r)
+    character(len=*), intent(in) :: tok
+    real(c_double), intent(out) :: r
+    integer :: at, ios
+    centre_cutoff = .false.
+    r = 0.0d0
+    at = index(tok, '@')
+    if (at == 0) return
+    if (tok(1:at - 1) /= 'COMCUT' .and. tok(1:at - 1) /= 'COMTRUNC') return
+    read(tok(at + 1:), *, iostat=ios) r
+    if (ios /= 0 .or. r <= 0.0d0) call fail('centre-of-mass cutoff needs a radius > 0: ' // trim(tok))
+    if (tok(1:at - 1) == 'COMCUT' .and. r > cell_min_width() / 2.0d0) &
+         call fail('mol-mol cut-off ' // trim(tok) // ' must be less than half the minimum width of the ' // &
+         'simulation cell (ssbasic.F90:153-160)')
+    centre_cutoff = .true.
+  end function centre_cutoff
+
+  ! fundcell_genwidths (fundcell.F90:229-247) for the simulation cell (the
+  ! fundamental cell's lattice vectors times the unit cells per axis): each
+  ! lattice vector projected on the normal of the other two
+  real(c_double) function cell_min_width()
+    real(c_double) :: l(3, 3), cp(3), w
+    integer :: i, j, k
+    do i = 1, 3
+       l(:, i) = sy%lvec(:, i) * real(sy%nxyz(i), c_double)
+       if (.not. sy%slanted .or. all(sy%lvec(:, i) == 0.0d0)) then
+          l(:, i) = 0.0d0
+          l(i, i) = sy%full(i)
+       end if
+    end do
+    cell_min_width = huge(1.0d0)
+    do i = 1, 3
+       j = mod(i, 3) + 1
+       k = mod(i + 1, 3) + 1
+       cp = [l(2, j) * l(3, k) - l(3, j) * l(2, k), l(3, j) * l(1, k) - l(1, j) * l(3, k), &
+             l(1, j) * l(2, k) - l(2, j) * l(1, k)]
+       w = dot_product(l(:, i), cp) / sqrt(dot_product(cp, cp))
+       cell_min_width = min(cell_min_width, w)
+    end do
+  end function cell_min_width
 
   ! index (1..nspc) of a GCMC species by name, 0 for another molecule
   integer function species_of(nm)

@@ -50,6 +50,9 @@ module music_hip_mod
      integer(c_int8_t) :: coul_on(64) = 0_c_int8_t
      real(c_double) :: coul_hicut = 0.0d0, coul_locut = 1.0d-3, coul_shield = 1.0d0
      integer(c_int) :: adjust_trans = 0, adjust_rot = 0
+     ! centre-of-mass cutoffs (A) of the LJ / plain-Coulomb pair terms with
+     ! each partner species (COMCUT@r), 0 = none
+     real(c_double) :: lj_comcut(MH_MAX_SPECIES) = 0.0d0, coul_comcut(MH_MAX_SPECIES) = 0.0d0
   end type mh_gcmc_params
 
   public :: mh_error_message

@@ -28,7 +28,8 @@ class GcmcParams(ctypes.Structure):
                 ("move_weight", _d * 4), ("delta_trans", _d), ("delta_rot", _d), ("max_nrg", _d),
                 ("precision", _i), ("bias", _vp), ("coul", _i), ("coul_on", ctypes.c_uint8 * 64),
                 ("coul_hicut", _d), ("coul_locut", _d), ("coul_shield", _d),
-                ("adjust_trans", _i), ("adjust_rot", _i)]
+                ("adjust_trans", _i), ("adjust_rot", _i),
+                ("lj_comcut", _d * 4), ("coul_comcut", _d * 4)]
 
 
 def _sig():
@@ -115,6 +116,14 @@ def make_params(spec, maps=None, precision=H.MH_PREC_FP64):
     p.delta_rot = spec.get("drot", 0.0)
     p.adjust_trans = int(spec.get("adjust_trans", 0))
     p.adjust_rot = int(spec.get("adjust_rot", 0))
+    # centre-of-mass cutoffs (ssbasic COMCUT@r): dict(lj=r, coul=r), a radius
+    # for every partner species or a list per partner species
+    cc = spec.get("comcut") or {}
+    for key, fld in (("lj", p.lj_comcut), ("coul", p.coul_comcut)):
+        v = cc.get(key, 0.0)
+        v = list(v) if np.ndim(v) else [v] * 4
+        for i, r in enumerate(v):
+            fld[i] = float(r or 0.0)
     p.max_nrg = spec.get("max_nrg", C.VERY_LRG_NRG)
     p.precision = precision
     bias = spec.get("bias_map")  # a hip.BiasMap for BINSERT / BDELETE

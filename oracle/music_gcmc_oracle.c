@@ -210,13 +210,27 @@ double mo_gcmc_mol_const(const mo_gcmc_sys *s)
 }
 
 /* ---------------------------------------------------- molecule energy */
+/* config_getMolecCTR (config.F90:1788-1802): the sum of the ns unfolded site
+ * positions in site order, divided by ns */
+static void mol_centre(int ns, const double *rp, double c[3])
+{
+    for (int i = 0; i < 3; i++) {
+        double t = 0.0;
+        for (int a = 0; a < ns; a++) t = t + rp[3 * a + i];
+        c[i] = t / (double)ns;
+    }
+}
+
 /* Energy of a molecule of species sp (sys[sp]) with wrapped site coordinates
- * r against the framework maps of its species and every molecule of every
- * species except molecule 'skip' of species sp.  The pair tables, cell and
- * Ewald / Coulomb settings are those of sys[0] (shared by the species; site
- * types index the common table).  Returns 0 on failure. */
+ * r (unfolded rp) against the framework maps of its species and every
+ * molecule of every species except molecule 'skip' of species sp.  The pair
+ * tables, cell and Ewald / Coulomb settings are those of sys[0] (shared by
+ * the species; site types index the common table); the centre-of-mass
+ * cutoffs are sys[sp]'s towards each species (ssbasic_mmint,
+ * ssbasic.F90:745-756: the molecule pair's LJ or Coulomb term is skipped
+ * when the centres are further apart).  Returns 0 on failure. */
 static int mol_energy_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *const *sts, int sp,
-                          const double *r, int skip, double *umap, double *upair)
+                          const double *r, const double *rp, int skip, double *umap, double *upair)
 {
     const mo_gcmc_sys *s = sys[sp];
     const mo_gcmc_sys *g = sys[0];
@@ -246,12 +260,28 @@ static int mol_energy_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state
         }
     }
     double up = 0.0;
+    double ctr[3];
+    mol_centre(s->ns, rp, ctr);
     for (int s2 = 0; s2 < nspc; s2++) {
         const mo_gcmc_sys *o = sys[s2];
         const mo_gcmc_state *st = sts[s2];
+        const double cl = s->lj_comcut2[s2], cc = s->coul_comcut2[s2];
         for (int m = 0; m < st->N; m++) {
             if (s2 == sp && m == skip) continue;
             const double *ro = st->r + 3 * o->ns * m;
+            int lj_pair = 1, coul_pair = 1;
+            if (cl > 0.0 || cc > 0.0) {
+                double c2[3], sep[3];
+                mol_centre(o->ns, st->rp + 3 * o->ns * m, c2);
+                if (sl) mo_cell_minimage(g->cell, ctr, c2, sep);
+                else mo_minimage(g->full_ell, ctr, c2, sep);
+                double d2 = 0.0;
+                d2 = d2 + sep[0] * sep[0];
+                d2 = d2 + sep[1] * sep[1];
+                d2 = d2 + sep[2] * sep[2];
+                if (cl > 0.0 && d2 > cl) lj_pair = 0;
+                if (cc > 0.0 && d2 > cc) coul_pair = 0;
+            }
             for (int a = 0; a < s->ns; a++)
                 for (int b = 0; b < o->ns; b++) {
                     double sep[3];
@@ -263,7 +293,7 @@ static int mol_energy_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state
                     r2 = r2 + sep[2] * sep[2];
                     int ta = s->stype[a], tb = o->stype[b];
                     int t = ta * g->nst + tb;
-                    if (g->lj_on[t]) {
+                    if (lj_pair && g->lj_on[t]) {
                         if (r2 <= g->lj_h2[t]) {
                             if (r2 < g->lj_l2[t]) return 0;
                             double r2i = 1.0 / r2;
@@ -276,7 +306,7 @@ static int mol_energy_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state
                         double rr = sqrt(r2);
                         up = up + s->sq[a] * o->sq[b] * erfc_as(g->kappa * rr) / rr * MO_TOKCAL;
                     }
-                    if (g->coul && g->coul_on[t]) {
+                    if (coul_pair && g->coul && g->coul_on[t]) {
                         double e;
                         if (!mo_coul_pair(g->coul_hicut, g->coul_locut, g->coul_shield, r2, s->sq[a], o->sq[b], &e))
                             return 0;
@@ -509,7 +539,7 @@ int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *cons
                 toxyz(s, com, ang, rpnew);
                 pbc(g, ns, rpnew, rnew);
                 double um, up;
-                int ok = mol_energy_mix(sys, nspc, sts, sp, rnew, -1, &um, &up);
+                int ok = mol_energy_mix(sys, nspc, sts, sp, rnew, rpnew, -1, &um, &up);
                 double U = 0.0;
                 if (ok) {
                     U = um + up;
@@ -566,7 +596,7 @@ int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *cons
                 }
                 const double *r = st->r + 3 * ns * mol;
                 double um, up;
-                int ok = mol_energy_mix(sys, nspc, sts, sp, r, mol, &um, &up);
+                int ok = mol_energy_mix(sys, nspc, sts, sp, r, st->rp + 3 * ns * mol, mol, &um, &up);
                 double U = um + up;
                 if (ok && g->ewald) {
                     mol_sk(s, G, r, ak, bk);
@@ -603,7 +633,7 @@ int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *cons
                 st->att[idx]++;
                 const double *rold = st->r + 3 * ns * mol;
                 double um0, up0;
-                if (!mol_energy_mix(sys, nspc, sts, sp, rold, mol, &um0, &up0)) return -2;
+                if (!mol_energy_mix(sys, nspc, sts, sp, rold, st->rp + 3 * ns * mol, mol, &um0, &up0)) return -2;
                 double com[3], ang[3];
                 memcpy(com, st->com + 3 * mol, sizeof com);
                 memcpy(ang, st->ang + 3 * mol, sizeof ang);
@@ -630,7 +660,7 @@ int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *cons
                 toxyz(s, com, ang, rpnew);
                 pbc(g, ns, rpnew, rnew);
                 double um1, up1;
-                int ok = mol_energy_mix(sys, nspc, sts, sp, rnew, mol, &um1, &up1);
+                int ok = mol_energy_mix(sys, nspc, sts, sp, rnew, rpnew, mol, &um1, &up1);
                 double dU = 0.0;
                 if (ok) {
                     double U1 = um1 + up1, U0 = um0 + up0;
@@ -698,7 +728,8 @@ double mo_gcmc_total_energy_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc
         for (int m = 0; m < keepN[sp]; m++) {
             tmp[sp].N = m; /* molecules 0..m-1 of this species */
             double um, up;
-            if (!mol_energy_mix(sys, nspc, tp, sp, sts[sp]->r + 3 * sys[sp]->ns * m, -1, &um, &up))
+            if (!mol_energy_mix(sys, nspc, tp, sp, sts[sp]->r + 3 * sys[sp]->ns * m, sts[sp]->rp + 3 * sys[sp]->ns * m,
+                                -1, &um, &up))
                 return NAN; /* a failed evaluation */
             e += um + up;
         }

@@ -74,6 +74,13 @@ typedef struct {
      * insertion by simcell_uniformRandPt, pbc, minimum image and maptouc in
      * the slanted basis, Hermite without the corner wrap */
     const mo_cell *cell;
+    /* centre-of-mass cutoffs of the sorbate pair terms (ssbasic COMCUT@ /
+     * COMTRUNC@, ssbasic.F90:139-160, 745-756), squared, of this species'
+     * molecules with those of species [s2]: a pair whose geometric centres
+     * (config_getMolecCTR, config.F90:1788-1802: the mean of the unfolded
+     * site positions) are further apart (simcell_minimage) has no LJ
+     * (lj_comcut2) or plain-Coulomb (coul_comcut2) term; 0 = no cutoff */
+    double lj_comcut2[MO_MAXSPC], coul_comcut2[MO_MAXSPC];
 } mo_gcmc_sys;
 
 typedef struct {

@@ -392,7 +392,8 @@ class GcmcSys(ctypes.Structure):
                 ("nmoves", _i), ("move_type", _i * 4), ("cumwt", _d * 4),
                 ("insdelratio", _d), ("dtrans", _d), ("drot", _d), ("max_nrg", _d), ("bias", _vp),
                 ("coul", _i), ("coul_on", _i * 64), ("coul_hicut", _d), ("coul_locut", _d),
-                ("coul_shield", _d), ("adjust_trans", _i), ("adjust_rot", _i), ("cell", _vp)]
+                ("coul_shield", _d), ("adjust_trans", _i), ("adjust_rot", _i), ("cell", _vp),
+                ("lj_comcut2", _d * 4), ("coul_comcut2", _d * 4)]
 
 
 class GcmcTrace(ctypes.Structure):
@@ -557,6 +558,14 @@ class Gcmc:
         s.bias = ctypes.cast(ctypes.byref(self.bias.b), _vp) if self.bias is not None else None
         self.cell = spec.get("cell")
         s.cell = ctypes.cast(ctypes.byref(self.cell), _vp) if self.cell is not None else None
+        # centre-of-mass cutoffs (ssbasic COMCUT@r): dict(lj=r, coul=r), each
+        # a radius for every species pair or a list per partner species
+        cc = spec.get("comcut") or {}
+        for key, fld in (("lj", s.lj_comcut2), ("coul", s.coul_comcut2)):
+            v = cc.get(key, 0.0)
+            v = list(v) if np.ndim(v) else [v] * 4
+            for i, r in enumerate(v):
+                fld[i] = float(r) ** 2 if r else 0.0
         self.sys = s
         st = GcmcState()
         st.maxN = maxN

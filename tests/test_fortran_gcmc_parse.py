@@ -137,13 +137,15 @@ def test_pressure_file(tmp_path):
     assert out.returncode != 0 and "differs from the ctrlfile" in out.stdout + out.stderr
 
 
-@pytest.mark.parametrize("line", ["Bent     Bent     NCOUL BASIC LJ FAST COMCUT@9.0",
+@pytest.mark.parametrize("line", ["Bent     Bent     NCOUL BASIC LJ FAST COMCUT@10.5",
                                   "Bent     Bent     NCOUL BASIC LJ FAST SMOOTH@8.0",
-                                  "Bent     Bent     NCOUL BASIC LJ FAST NEIGHBOR"])
+                                  "Bent     Bent     NCOUL BASIC LJ FAST NEIGHBOR",
+                                  "Bent     Bent     NCOUL BASIC LJ FAST COMCUT@abc"])
 def test_unsupported_species_options_fail(tmp_path, line):
-    """ssbasic options that change the energy (the centre-of-mass cutoff of
-    ssbasic_mmint, ssbasic.F90:139-160, 745-756; smoothing; neighbour lists)
-    stop the driver with a message instead of being ignored."""
+    """ssbasic options that change the energy and are not offered
+    (smoothing, neighbour lists), a COMCUT radius beyond half the 20 A
+    cell's width (the reference stops, ssbasic.F90:153-160) and a radius
+    that is no number stop the driver with a message."""
     shutil.copytree(FIX, tmp_path / "fx", copy_function=shutil.copyfile)
     ss = tmp_path / "fx" / "spc_spc_bent"
     txt = ss.read_text().replace("Bent     Bent     NCOUL BASIC LJ FAST", line)
@@ -151,7 +153,24 @@ def test_unsupported_species_options_fail(tmp_path, line):
     out = subprocess.run([GCMC, str(tmp_path / "fx" / "bent.ctr"), "check"], capture_output=True, text=True,
                          timeout=60)
     assert out.returncode != 0
-    assert "is not offered" in out.stdout + out.stderr
+    assert any(m in out.stdout + out.stderr for m in ("is not offered", "less than half", "radius > 0"))
+
+
+@pytest.mark.parametrize("tok,lj,coul", [("NCOUL BASIC LJ FAST COMCUT@8.0", 8.0, 0.0),
+                                         ("NCOUL BASIC LJ FAST COMTRUNC@10.5", 10.5, 0.0)])
+def test_centre_of_mass_cutoff_parameters(tmp_path, tok, lj, coul):
+    """COMCUT@r / COMTRUNC@r on the species-species LJ line (ssbasic_init,
+    ssbasic.F90:139-160): the radius handed to the library; COMTRUNC skips
+    the half-width check."""
+    shutil.copytree(FIX, tmp_path / "fx", copy_function=shutil.copyfile)
+    ss = tmp_path / "fx" / "spc_spc_bent"
+    ss.write_text(ss.read_text().replace("NCOUL BASIC LJ FAST", tok))
+    out = subprocess.run([GCMC, str(tmp_path / "fx" / "bent.ctr"), "check"], capture_output=True, text=True,
+                         timeout=60)
+    assert out.returncode == 0, out.stdout[-1500:]
+    line = [l for l in out.stdout.splitlines() if l.startswith("  centre cutoff")]
+    assert len(line) == 1 and line[0].split()[2:4] == ["Bent", "Bent:"]
+    assert [float(v) for v in line[0].split()[-2:]] == [lj, coul]
 
 
 def test_ewald_default_has_no_real_space_cutoff():
