@@ -445,33 +445,36 @@ def cpu_leg(pool, host, cfg, tabs, c1cpu, c3cpu, srv, seconds):
     npn = (cfg["fund_ell"] / cfg["gridspace"]).astype(int) + 1
     step = cfg["fund_ell"] / (npn - 1)
     probe, _ = pool.run(CB.map_nodes_work, map_cpu_items(cfg, tabs, npn, step, [64, 64, 64]))
-    per_node = [t / n for n, t in probe]
+    per_node = [t / n for n, t, _ in probe]
     nodes_each = [int(max(64, min(200000, seconds / 3.0 / max(pn, 1e-9)))) for pn in per_node]
     single, _ = pool.run(CB.map_nodes_work, map_cpu_items(cfg, tabs, npn, step, nodes_each))
-    one = sum(n for n, _ in single) / sum(t for _, t in single)
+    one = sum(n for n, _, _ in single) / sum(t for _, t, _ in single)
     agg_items = map_cpu_items(cfg, tabs, npn, step, nodes_each, parts=P)
     res, wall = pool.run(CB.map_nodes_work, agg_items)
-    agg = sum(n for n, _ in res) / wall
+    agg, nw = CB.aggregate_rate(res)
     log("CPU maps: one core %.3e pts/s, %d processes %.3e pts/s" % (one, P, agg))
     out = dict(value=agg, unit="map pts/s", cores=P, kind="port",
                sample="first %s nodes of the C / O pmaps and the emap of the charged-framework map build "
                       "(FP64 restatement, -O3 -march=native, one single-threaded process per work item, %d "
-                      "processes)" % (nodes_each, P),
+                      "processes; aggregate = sum over the %d busy processes of nodes / compute seconds)"
+                      % (nodes_each, P, nw),
                single_core_map_pts_per_s=one, host=host)
     # C1 / C3 GCMC: one state point per process
-    for tag, c, steps in (("c1", c1cpu, 20000), ("c3", c3cpu, 4000)):
+    for tag, c, steps in (("c1", c1cpu, 200000), ("c3", c3cpu, 4000)):
         if c is None:
             continue
         items = [(c["spec"], f, sd, cfg_, steps) for f, sd, cfg_ in zip(c["fug"], c["seeds"], c["cfgs"])]
         res, wall = pool.run(CB.gcmc_work, items)
-        att = sum(a for a, _, _ in res)
-        st = sum(n for _, n, _ in res)
-        tsum = sum(t for _, _, t in res)
+        att = sum(r[0] for r in res)
+        st = sum(r[1] for r in res)
+        tsum = sum(r[2] for r in res)
+        agg, nw = CB.aggregate_rate(res, unit=0, secs=2)
         out[tag + "_gcmc"] = dict(single_core_attempted_moves_per_s=att / tsum, single_core_steps_per_s=st / tsum,
-                                  aggregate_attempted_moves_per_s=att / wall, processes=min(P, len(items)),
-                                  sample="%d state points x %d steps" % (len(items), steps),
+                                  aggregate_attempted_moves_per_s=agg, processes=nw,
+                                  sample="%d state points x %d steps; aggregate = sum over the busy processes "
+                                         "of attempted moves / compute seconds" % (len(items), steps),
                                   seconds_per_step=tsum / st)
-        log("CPU %s GCMC: one core %.3e moves/s, aggregate %.3e" % (tag, att / tsum, att / wall))
+        log("CPU %s GCMC: one core %.3e moves/s, aggregate %.3e over %d processes" % (tag, att / tsum, agg, nw))
     # serving stream on the CPU, modelled: per request the map cost (nodes x
     # LJ pairs within 12.8 A x the measured seconds per pair of the C pmap
     # sample) plus its GCMC steps x the measured C1 seconds per step, served
@@ -484,14 +487,27 @@ def cpu_leg(pool, host, cfg, tabs, c1cpu, c3cpu, srv, seconds):
         svc = np.array([z["nodes"] * min(z["atoms"], z["atoms"] / z["volume"] * vol_c) * c_pair + z["steps"] * c_step
                         for z in srv["sizes"]])
         lat = CB.fifo_latencies(svc, np.asarray(srv["arrivals"]), P)
+        # the CPU's own operating point: Poisson arrivals at 70 % of the P
+        # processes' capacity (the device run's target utilisation)
+        cap = P / float(svc.mean())
+        rng = np.random.default_rng(4)
+        own = np.cumsum(rng.exponential(1.0 / (0.7 * cap), len(svc)))
+        lat_own = CB.fifo_latencies(svc, own, P)
         out["c4_fifo_model"] = dict(p50_latency_ms=float(np.percentile(lat, 50) * 1e3),
                                     p99_latency_ms=float(np.percentile(lat, 99) * 1e3),
+                                    capacity_requests_per_s=cap,
+                                    at_70pct_capacity=dict(arrival_rate_per_s=0.7 * cap,
+                                                           p50_latency_ms=float(np.percentile(lat_own, 50) * 1e3),
+                                                           p99_latency_ms=float(np.percentile(lat_own, 99) * 1e3)),
                                     mean_service_s=float(svc.mean()), processes=P,
                                     note="modelled from the measured per-pair (map) and per-step (C1 GCMC) "
                                          "costs of this host; same request sizes and arrival times as the "
                                          "device run")
-        log("CPU serving model: p50 %.0f ms, p99 %.0f ms" % (out["c4_fifo_model"]["p50_latency_ms"],
-                                                           out["c4_fifo_model"]["p99_latency_ms"]))
+        log("CPU serving model: device arrivals p50 %.0f ms, p99 %.0f ms; capacity %.2f requests/s, at 70%%: "
+            "p50 %.0f ms, p99 %.0f ms" % (out["c4_fifo_model"]["p50_latency_ms"],
+                                         out["c4_fifo_model"]["p99_latency_ms"], cap,
+                                         out["c4_fifo_model"]["at_70pct_capacity"]["p50_latency_ms"],
+                                         out["c4_fifo_model"]["at_70pct_capacity"]["p99_latency_ms"]))
     return out
 
 
@@ -532,6 +548,7 @@ def main():
         # the CPU worker processes are forked before anything touches the GPU
         from oracle import cpu_baseline as CB
         host = CB.host_info()
+        CB.build_native()  # once, for this host, before the workers start
         pool = CB.Pool(max(1, min(args.cpu_procs, host["cpus_available"])))
     import torch  # noqa: F401  (device memory/stream plumbing only)
     if args.copy_probe_mib:

@@ -6,7 +6,9 @@ them side by side for the aggregate.
 
 The worker pool is forked before the benchmark touches the GPU, so no
 worker inherits device state; work items carry their inputs (numpy arrays)
-and return (units of work, seconds)."""
+and return (units of work, seconds, worker pid): the aggregate rate sums,
+over the workers, each worker's units over its own compute seconds, so the
+transfer of the inputs to the workers is not counted."""
 import multiprocessing as mp
 import os
 import platform
@@ -33,6 +35,11 @@ def host_info():
     return dict(cpu_model=model, nproc=os.cpu_count(), cpus_available=avail)
 
 
+def build_native():
+    """Build (if stale) the -march=native oracle of this host."""
+    O.lib(native=True)
+
+
 def _native():
     L = O.lib(native=True)
     O._LIB = L
@@ -45,7 +52,7 @@ def map_nodes_work(args):
     _native()
     t = time.perf_counter()
     O.build_nodes(kind, fw, npn, step, m0, m1)
-    return m1 - m0, time.perf_counter() - t
+    return m1 - m0, time.perf_counter() - t, os.getpid()
 
 
 def gcmc_work(args):
@@ -61,7 +68,7 @@ def gcmc_work(args):
     t = time.perf_counter()
     g.run(steps)
     dt = time.perf_counter() - t
-    return sum(int(g.st.att[i]) for i in range(4)) - a0, steps, dt
+    return sum(int(g.st.att[i]) for i in range(4)) - a0, steps, dt, os.getpid()
 
 
 class Pool:
@@ -79,6 +86,17 @@ class Pool:
     def close(self):
         self.pool.close()
         self.pool.join()
+
+
+def aggregate_rate(res, unit=0, secs=-2):
+    """Sum over the worker processes of (units / compute seconds) of the
+    items each one ran (items: tuples with the units at index `unit`, the
+    seconds at `secs` and the pid last)."""
+    per = {}
+    for r in res:
+        u, t = per.get(r[-1], (0.0, 0.0))
+        per[r[-1]] = (u + r[unit], t + r[secs])
+    return sum(u / t for u, t in per.values() if t > 0), len(per)
 
 
 def fifo_latencies(service_s, arrivals, servers):

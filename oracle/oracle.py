@@ -23,16 +23,32 @@ def _p(a):
     return a.ctypes.data_as(_vp) if a is not None else None
 
 
+def _host_tag():
+    """Tag of the -march=native build: the CPU model and flags of this host."""
+    import hashlib
+    info = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith(("model name", "flags")):
+                info += line
+                if line.startswith("flags"):
+                    break
+    except OSError:
+        pass
+    return hashlib.sha1(info.encode()).hexdigest()[:10]
+
+
 def lib(native=False):
     """Load (building once if absent) liboracle.so."""
     global _LIB
     if _LIB is not None and not native:
         return _LIB
-    name = "liboracle_native.so" if native else "liboracle.so"
+    name = "liboracle_native_%s.so" % _host_tag() if native else "liboracle.so"
     path = os.path.join(HERE, name)
     srcs = [os.path.join(HERE, f) for f in os.listdir(HERE) if f.endswith((".c", ".h"))]
     if not os.path.exists(path) or os.path.getmtime(path) < max(os.path.getmtime(f) for f in srcs):
-        subprocess.check_call(["make", "-s", "-C", HERE, "native" if native else "all"])
+        cmd = ["make", "-s", "-C", HERE]
+        subprocess.check_call(cmd + (["native", "NATIVE_OUT=" + name] if native else ["all"]))
     L = ctypes.CDLL(path)
     L.mo_rranf_n.argtypes = [ctypes.c_int32, _i, _vp, _vp, _vp]
     L.mo_erfc.argtypes = [_d]
