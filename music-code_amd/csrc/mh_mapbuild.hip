@@ -3,9 +3,9 @@
 //
 // Layout in HBM:
 //   framework atoms        double [natoms][3] + int type + double q
-//   full map (FP32)        float  [nodes][8]   node m = x + nx*(y + ny*z)
+//   full map (FP32)        float  [nodes][8]   scan order s = (x*ny + y)*nz + z
 //   full map (FP64, opt.)  double [nodes][8]
-//   Ewald reciprocal part  double [nodes][8]   (separable 3-stage DFT)
+//   Ewald reciprocal part  double [nodes][8]   node m = x + nx*(y + ny*z) (separable 3-stage DFT)
 //   compact map            float  [nfsize][8]  32-byte records
 //   ptr                    int32  [nx][ny][nz] z fastest, 1-based
 //
@@ -346,7 +346,9 @@ __global__ void __launch_bounds__(NT) k_nodes64(GridP g, const PairTab *__restri
     const int nbx = (g.nx + BX - 1) / BX, nby = (g.ny + BY - 1) / BY;
     const int b = blockIdx.x;
     const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
-    const int x = bx * BX + (tid % BX), y = by * BY + ((tid / BX) % BY), z = bz * BZ + tid / (BX * BY);
+    // lanes run along z first: the node values are stored in the map's scan
+    // order (z fastest), four consecutive z of a column per 128-byte line
+    const int z = bz * BZ + (tid % BZ), x = bx * BX + ((tid / BZ) % BX), y = by * BY + tid / (BZ * BX);
     const bool valid = (x < g.nx) && (y < g.ny) && (z < g.nz);
     double px = (double)x * g.step[0], py = (double)y * g.step[1], pz = (double)z * g.step[2];
     if (g.slanted) {
@@ -435,7 +437,10 @@ __global__ void __launch_bounds__(NT) k_nodes64(GridP g, const PairTab *__restri
         __syncthreads();
     }
     if (!valid) return;
+    // m: config_makegrid order (x fastest) of the reciprocal-space table;
+    // s: the map's scan order (z fastest) of the node values
     const long long m = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
+    const long long s = ((long long)x * g.ny + y) * g.nz + z;
     double o[8];
     if (KIND == K_EWALD) {
         if (failed) {
@@ -450,11 +455,11 @@ __global__ void __launch_bounds__(NT) k_nodes64(GridP g, const PairTab *__restri
         o[0] = failed ? kVeryLrg : pot;
         for (int i = 0; i < 7; i++) o[i + 1] = h[i];
     }
-    float4 *d32 = reinterpret_cast<float4 *>(out32 + 8 * m);
+    float4 *d32 = reinterpret_cast<float4 *>(out32 + 8 * s);
     d32[0] = make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
     d32[1] = make_float4((float)o[4], (float)o[5], (float)o[6], (float)o[7]);
     if (out64) {
-        double2 *d64 = reinterpret_cast<double2 *>(out64 + 8 * m);
+        double2 *d64 = reinterpret_cast<double2 *>(out64 + 8 * s);
         for (int i = 0; i < 4; i++) d64[i] = make_double2(o[2 * i], o[2 * i + 1]);
     }
 }
@@ -527,7 +532,9 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
     const int nbx = (g.nx + BX - 1) / BX, nby = (g.ny + BY - 1) / BY;
     const int b = blockIdx.x;
     const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
-    const int x = bx * BX + (tid % BX), y = by * BY + ((tid / BX) % BY), z = bz * BZ + tid / (BX * BY);
+    // lanes run along z first: the node values are stored in the map's scan
+    // order (z fastest), four consecutive z of a column per 128-byte line
+    const int z = bz * BZ + (tid % BZ), x = bx * BX + ((tid / BZ) % BX), y = by * BY + tid / (BZ * BX);
     const bool valid = (x < g.nx) && (y < g.ny) && (z < g.nz);
     const double px = (double)x * g.step[0], py = (double)y * g.step[1], pz = (double)z * g.step[2];
     double lo[3] = {(double)(bx * BX) * g.step[0], (double)(by * BY) * g.step[1], (double)(bz * BZ) * g.step[2]};
@@ -722,7 +729,10 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
         __syncthreads();
     }
     if (!valid) return;
+    // m: config_makegrid order (x fastest) of the reciprocal-space table;
+    // s: the map's scan order (z fastest) of the node values
     const long long m = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
+    const long long s = ((long long)x * g.ny + y) * g.nz + z;
     double o[8];
     if (KIND == K_EWALD)
         for (int i = 0; i < 7; i++) S[i + 1] = s_acc[i][tid];
@@ -738,11 +748,11 @@ __global__ void __launch_bounds__(NT, 4) k_nodes32(GridP g, const PairTab *__res
         o[0] = failed ? kVeryLrg : S[0];
         for (int i = 1; i < 8; i++) o[i] = S[i];
     }
-    float4 *d32 = reinterpret_cast<float4 *>(out32 + 8 * m);
+    float4 *d32 = reinterpret_cast<float4 *>(out32 + 8 * s);
     d32[0] = make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
     d32[1] = make_float4((float)o[4], (float)o[5], (float)o[6], (float)o[7]);
     if (out64) {
-        double2 *d64 = reinterpret_cast<double2 *>(out64 + 8 * m);
+        double2 *d64 = reinterpret_cast<double2 *>(out64 + 8 * s);
         for (int i = 0; i < 4; i++) d64[i] = make_double2(o[2 * i], o[2 * i + 1]);
     }
 }
@@ -864,9 +874,7 @@ __global__ void k_below(long long n, int nx, int ny, int nz, const float *__rest
                         float cut, uint8_t *__restrict__ below) {
     long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    int k = (int)(s % nz), j = (int)((s / nz) % ny), i = (int)(s / ((long long)nz * ny));
-    long long m = (long long)i + (long long)nx * ((long long)j + (long long)ny * k);
-    below[s] = full[8 * m] < cut ? 1 : 0;
+    below[s] = full[8 * s] < cut ? 1 : 0;
 }
 
 __global__ void k_owner(long long n, int nx, int ny, int nz, const uint8_t *__restrict__ below,
@@ -985,9 +993,8 @@ __global__ void k_assign(long long n, int nx, int ny, int nz, const uint8_t *__r
     int k = (int)(s % nz), j = (int)((s / nz) % ny), i = (int)(s / ((long long)nz * ny));
     if (below[s]) {
         long long idx = base[s];
-        long long m = (long long)i + (long long)nx * ((long long)j + (long long)ny * k);
         ptr[s] = (int32_t)(idx + 1);
-        const float4 *src = reinterpret_cast<const float4 *>(full + 8 * m);
+        const float4 *src = reinterpret_cast<const float4 *>(full + 8 * s);
         float4 *dst = reinterpret_cast<float4 *>(cvals + 8 * idx);
         dst[0] = src[0];
         dst[1] = src[1];
@@ -1000,8 +1007,7 @@ __global__ void k_assign(long long n, int nx, int ny, int nz, const uint8_t *__r
                     if (!below[t] && owner[t] == s) {
                         idx++;
                         ptr[t] = (int32_t)(idx + 1);
-                        long long mt = (long long)a + (long long)nx * ((long long)bb + (long long)ny * c);
-                        const float4 *s2 = reinterpret_cast<const float4 *>(full + 8 * mt);
+                        const float4 *s2 = reinterpret_cast<const float4 *>(full + 8 * t);
                         float4 *d2 = reinterpret_cast<float4 *>(cvals + 8 * idx);
                         d2[0] = s2[0];
                         d2[1] = s2[1];

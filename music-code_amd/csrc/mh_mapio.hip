@@ -108,11 +108,27 @@ int mh_map_get_full(const mh_map *map, float *f32, double *f64) {
     MH_CHECK(map, MH_EINVAL, "mh_map_get_full: map is NULL");
     MH_CHECK(map->full32.p, MH_ESTATE, "mh_map_get_full: map was built without keep_full");
     MH_HIP(hipSetDevice(map->ctx->device));
-    const long long nodes = (long long)map->npnts[0] * map->npnts[1] * map->npnts[2];
-    if (f32) MH_HIP(hipMemcpy(f32, map->full32.p, sizeof(float) * 8 * nodes, hipMemcpyDeviceToHost));
+    const int nx = map->npnts[0], ny = map->npnts[1], nz = map->npnts[2];
+    const long long nodes = (long long)nx * ny * nz;
+    // the device keeps the nodes in the map's scan order (z fastest); the
+    // caller receives config_makegrid order (x fastest)
+    auto reorder = [&](auto *dst, const auto &src) {
+        for (long long s = 0; s < nodes; s++) {
+            const long long i = s / ((long long)ny * nz), j = (s / nz) % ny, k = s % nz;
+            const long long m = i + (long long)nx * (j + (long long)ny * k);
+            for (int c = 0; c < 8; c++) dst[8 * m + c] = src[8 * s + c];
+        }
+    };
+    if (f32) {
+        std::vector<float> h((size_t)(8 * nodes));
+        MH_HIP(hipMemcpy(h.data(), map->full32.p, sizeof(float) * 8 * nodes, hipMemcpyDeviceToHost));
+        reorder(f32, h);
+    }
     if (f64) {
         MH_CHECK(map->full64.p, MH_ESTATE, "mh_map_get_full: no FP64 node values kept");
-        MH_HIP(hipMemcpy(f64, map->full64.p, sizeof(double) * 8 * nodes, hipMemcpyDeviceToHost));
+        std::vector<double> h((size_t)(8 * nodes));
+        MH_HIP(hipMemcpy(h.data(), map->full64.p, sizeof(double) * 8 * nodes, hipMemcpyDeviceToHost));
+        reorder(f64, h);
     }
     return MH_OK;
 }
