@@ -55,6 +55,8 @@ def lib(native=False):
     L.mo_erfc.restype = _d
     L.mo_lj_node.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.mo_lj_node.restype = _i
+    L.mo_buck_node.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.mo_buck_node.restype = _i
     L.mo_ewald_sfactor.argtypes = [_i, _vp, _vp, _vp, _d, _i, _d, _i, _vp, _vp, _vp, _vp, _vp]
     L.mo_ewald_sfactor.restype = _i
     L.mo_ewald_node.argtypes = [_vp, _i, _vp, _vp, _d, _d, _i, _vp, _vp, _vp, _vp, _vp]
