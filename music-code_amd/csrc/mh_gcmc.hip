@@ -508,76 +508,12 @@ __device__ __forceinline__ void move_energy(const GDev &P, Shared &sh, const dou
         if (set == 0 && hasA) sh.ctr[0][i] = cA ? cA[i] : mol_centre(sh.rpA, ns, i);
         if (set == 1 && hasB) sh.ctr[1][i] = cB[i];
     }
-    // maps: item l = (set * ns + site) * 8 + corner; the 8 corner lanes of a
-    // site are an aligned lane group
-    for (int l = tid; l < 16 * ns; l += NT) {
-        const int set = l / (8 * ns), a = (l >> 3) % ns, corner = l & 7;
-        const bool present = set == 0 ? hasA : hasB;
-        const double *r = set == 0 ? &sh.posA[3 * a] : &sh.posB[3 * a];
-        double pp = 0.0, pe = 0.0;
-        int ok = 1;
-        if (present) {
-            const SlantCell *sc = SL ? &P.sc : nullptr;
-            if (P.has_p[a] && !corner_term<PREC>(P.pm[a], r, corner, pp, sc)) ok = 0;
-            if (P.has_e[a] && !corner_term<PREC>(P.em[a], r, corner, pe, sc)) ok = 0;
-        }
-        for (int off = 1; off < 8; off <<= 1) {
-            pp += __shfl_xor(pp, off);
-            pe += __shfl_xor(pe, off);
-            ok &= __shfl_xor(ok, off);
-        }
-        if (corner == 0 && present) {
-            double e = 0.0;
-            if (P.has_p[a]) e = __dadd_rn(e, pp);
-            if (P.has_e[a]) e = __dadd_rn(e, __dmul_rn(pe, P.q[a]));
-            sh.mapv[set][a] = e;
-            sh.mapok[set][a] = ok;
-        }
-    }
-    // e^{i m 2 pi x / L} tables: one lane per (set, site, axis), recurrence in m
-    if (P.ewald) {
-        const int e0 = NT >= 256 ? 128 : (NT == 128 ? 64 : 0);
-        const int l = tid - e0;
-        if (l >= 0 && l < 6 * ns) {
-            const int set = l / (3 * ns), a = (l / 3) % ns, ax = l % 3;
-            const bool present = set == 0 ? hasA : hasB;
-            if (present && P.q[a] != 0.0) {
-                const double *pp = set == 0 ? sh.posA : sh.posB;
-                const double th = (1.0 / P.full[ax]) * kTwoPiR * pp[3 * a + ax];
-                double s1, c1;
-                sincos(th, &s1, &c1);
-                double cr = 1.0, ci = 0.0;
-                for (int m = 0; m <= P.kmax; m++) {
-                    sh.E[set][a][ax][m] = make_double2(cr, ci);
-                    sh.Ef[set][a][ax][m] = make_float2((float)cr, (float)ci);
-                    const double nr = cr * c1 - ci * s1, ni = cr * s1 + ci * c1;
-                    cr = nr;
-                    ci = ni;
-                }
-            }
-        }
-    }
-    __syncthreads();
-    if (KPL > 0 && P.ewald && P.use_exy) {  // the table lives in the persistent kernel's LDS
-        const int W = 2 * P.kmax + 1, per = (P.kmax + 1) * W;
-        for (int i = tid; i < 2 * ns * per; i += NT) {
-            const int sa = i / per, r = i - sa * per;
-            const int set = sa / ns, a = sa - set * ns;
-            const int mx = r / W, my = r - mx * W - P.kmax;
-            if (!(set == 0 ? hasA : hasB) || P.q[a] == 0.0) continue;
-            const double2 ex = sh.E[set][a][0][mx];
-            double2 ey = sh.E[set][a][1][my < 0 ? -my : my];
-            if (my < 0) ey.y = -ey.y;
-            const double2 o = make_double2(ex.x * ey.x - ex.y * ey.y, ex.x * ey.y + ex.y * ey.x);
-            if (PREC == MH_PREC_FP64) reinterpret_cast<double2 *>(exy)[i] = o;
-            else reinterpret_cast<float2 *>(exy)[i] = make_float2((float)o.x, (float)o.y);
-        }
-        __syncthreads();
-    }
-    // pairs: per species s2 the items (molecule j of s2, its site b, trial site a)
+    // pairs: per species s2 the items (molecule j of s2, its site b, trial site a),
+    // item w of lane first + k stride
     double eA = 0.0, eB = 0.0, failA = 0.0;
     float eAf = 0.0f, eBf = 0.0f;
     const float kap32 = (float)P.kappa, rc32 = (float)P.rc2;
+    auto pairs = [&](int first, int stride) {
     for (int s2 = 0; s2 < P.nspc; s2++) {
         const bool own = s2 == sh.spc;
         const SpcDev &O = (P.nspc == 1) ? static_cast<const SpcDev &>(P) : P.sp[s2];
@@ -589,7 +525,7 @@ __device__ __forceinline__ void move_energy(const GDev &P, Shared &sh, const dou
         const double *pos2 = pos + (long long)s2 * P.maxN * P.nsmax * 3;
         const double *ctr2 = ctrg + (long long)s2 * P.maxN * 3;
         const double ccl = P.cc_lj[sh.spc][s2], ccc = P.cc_cl[sh.spc][s2];
-        for (int w = tid; w < items; w += NT) {
+        for (int w = first; w < items; w += stride) {
             const int j = (int)(((double)w + 0.5) * inv_nss);
             if (own && j == skip) continue;
             const int r = w - j * nss;
@@ -652,6 +588,83 @@ __device__ __forceinline__ void move_energy(const GDev &P, Shared &sh, const dou
             }
         }
     }
+    };
+    // maps: item l = (set * ns + site) * 8 + corner; the 8 corner lanes of a
+    // site are an aligned lane group
+    for (int l = tid; l < 16 * ns; l += NT) {
+        const int set = l / (8 * ns), a = (l >> 3) % ns, corner = l & 7;
+        const bool present = set == 0 ? hasA : hasB;
+        const double *r = set == 0 ? &sh.posA[3 * a] : &sh.posB[3 * a];
+        double pp = 0.0, pe = 0.0;
+        int ok = 1;
+        if (present) {
+            const SlantCell *sc = SL ? &P.sc : nullptr;
+            if (P.has_p[a] && !corner_term<PREC>(P.pm[a], r, corner, pp, sc)) ok = 0;
+            if (P.has_e[a] && !corner_term<PREC>(P.em[a], r, corner, pe, sc)) ok = 0;
+        }
+        for (int off = 1; off < 8; off <<= 1) {
+            pp += __shfl_xor(pp, off);
+            pe += __shfl_xor(pe, off);
+            ok &= __shfl_xor(ok, off);
+        }
+        if (corner == 0 && present) {
+            double e = 0.0;
+            if (P.has_p[a]) e = __dadd_rn(e, pp);
+            if (P.has_e[a]) e = __dadd_rn(e, __dmul_rn(pe, P.q[a]));
+            sh.mapv[set][a] = e;
+            sh.mapok[set][a] = ok;
+        }
+    }
+    // e^{i m 2 pi x / L} tables: one lane per (set, site, axis), recurrence in m
+    if (P.ewald) {
+        const int e0 = NT >= 256 ? 128 : (NT == 128 ? 64 : 0);
+        const int l = tid - e0;
+        if (l >= 0 && l < 6 * ns) {
+            const int set = l / (3 * ns), a = (l / 3) % ns, ax = l % 3;
+            const bool present = set == 0 ? hasA : hasB;
+            if (present && P.q[a] != 0.0) {
+                const double *pp = set == 0 ? sh.posA : sh.posB;
+                const double th = (1.0 / P.full[ax]) * kTwoPiR * pp[3 * a + ax];
+                double s1, c1;
+                sincos(th, &s1, &c1);
+                double cr = 1.0, ci = 0.0;
+                for (int m = 0; m <= P.kmax; m++) {
+                    sh.E[set][a][ax][m] = make_double2(cr, ci);
+                    sh.Ef[set][a][ax][m] = make_float2((float)cr, (float)ci);
+                    const double nr = cr * c1 - ci * s1, ni = cr * s1 + ci * c1;
+                    cr = nr;
+                    ci = ni;
+                }
+            }
+        }
+    }
+    // without centre cutoffs, and when the pair items fit the lanes above the
+    // first wave (which holds the map lanes) in one pass, those lanes take them
+    // before the barrier, while the map reads are in flight
+    int nitems = 0;
+    for (int s2 = 0; s2 < P.nspc; s2++)
+        nitems += (s2 == sh.spc ? N : sh.N[s2]) * ((P.nspc == 1) ? P.ns : P.sp[s2].ns) * ns;
+    const bool early = NT >= 128 && !P.comcut && nitems <= NT - 64;
+    if (early && tid >= 64) pairs(tid - 64, NT - 64);
+    __syncthreads();
+    if (KPL > 0 && P.ewald && P.use_exy) {  // the table lives in the persistent kernel's LDS
+        const int W = 2 * P.kmax + 1, per = (P.kmax + 1) * W;
+        for (int i = tid; i < 2 * ns * per; i += NT) {
+            const int sa = i / per, r = i - sa * per;
+            const int set = sa / ns, a = sa - set * ns;
+            const int mx = r / W, my = r - mx * W - P.kmax;
+            if (!(set == 0 ? hasA : hasB) || P.q[a] == 0.0) continue;
+            const double2 ex = sh.E[set][a][0][mx];
+            double2 ey = sh.E[set][a][1][my < 0 ? -my : my];
+            if (my < 0) ey.y = -ey.y;
+            const double2 o = make_double2(ex.x * ey.x - ex.y * ey.y, ex.x * ey.y + ex.y * ey.x);
+            if (PREC == MH_PREC_FP64) reinterpret_cast<double2 *>(exy)[i] = o;
+            else reinterpret_cast<float2 *>(exy)[i] = make_float2((float)o.x, (float)o.y);
+        }
+        __syncthreads();
+    }
+    if (!early) pairs(tid, NT);
+
     if (PREC != MH_PREC_FP64) {  // FP64 plain-Coulomb sums already in eA, eB
         eA += (double)eAf;
         eB += (double)eBf;
