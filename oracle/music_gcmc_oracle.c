@@ -466,6 +466,85 @@ static double adjust_delta(double d, double ratio)
     return d;
 }
 
+/* rigidmoves_comFromBmap (rigidmoves.F90:1938-1969): a cubelet by its
+ * weight, a point in it (bmap_getbiaspt), a unit cell of the simulation
+ * cell; *bias = 1 / (weight ncubelets) */
+static void bmap_com(const mo_gcmc_sys *s, const mo_gcmc_sys *g, mo_rng *rng, double com[3], double *bias)
+{
+    const mo_bmap *b = s->bias;
+    long long t = mo_bmap_index(b, mo_rranf(rng));
+    long long c = b->cube[t];
+    long long ci[3] = {c / ((long long)b->ncub[1] * b->ncub[2]), (c / b->ncub[2]) % b->ncub[1], c % b->ncub[2]};
+    double wt = b->wt[t] / b->cum_wt;
+    *bias = 1.0 / wt / (double)b->ncubelets;
+    for (int i = 0; i < 3; i++) com[i] = (double)ci[i] * b->step[i];
+    for (int i = 0; i < 3; i++) com[i] = com[i] + mo_rranf(rng) * b->step[i];
+    for (int i = 0; i < 3; i++)
+        if (com[i] >= b->eff[i]) com[i] = (double)0.9999990f * b->eff[i];
+    for (int i = 0; i < 3; i++) {
+        int nc = (int)lround(g->full_ell[i] / g->fund_ell[i]);
+        int icell = (int)(mo_rranf(rng) * nc);
+        com[i] = com[i] + g->fund_ell[i] * icell;
+    }
+}
+
+/* rigidmoves_reorient with newxyz (random Euler angles; none for one site) */
+static void random_angles(int ns, mo_rng *rng, double ang[3])
+{
+    ang[0] = ang[1] = ang[2] = 0.0;
+    if (ns != 1) {
+        double costheta = 1.0 - mo_rranf(rng) * 2.0;
+        ang[1] = acos(costheta);
+        ang[0] = MO_PI - mo_rranf(rng) * MO_TWOPI;
+        ang[2] = MO_PI - mo_rranf(rng) * MO_TWOPI;
+    }
+}
+
+/* cavitylist_getIndexVec (cavitylist.F90:536-546): Int(r / d) per axis of a
+ * folded position, cubes numbered with z fastest.  A coordinate that rounds
+ * onto the upper face is kept in the last cube. */
+static long long cav_cube(const mo_cavity *cv, const double *r)
+{
+    long long c = 0;
+    for (int i = 0; i < 3; i++) {
+        int k = (int)(r[i] / cv->d[i]);
+        if (k > cv->n[i] - 1) k = cv->n[i] - 1;
+        if (k < 0) k = 0;
+        c = c * cv->n[i] + k;
+    }
+    return c;
+}
+
+/* cavitylist_updateMolecInfo (cavitylist.F90:351-513) for the sites r of one
+ * molecule of species sp entering (sign 1) or leaving (-1) the cavity list.
+ * A cube holds a count of sites; it is free when the count is zero, which
+ * is the reference's test of its per-species molecule lists. */
+static void cav_move(const mo_gcmc_sys *const *sys, mo_gcmc_state *const *sts, int sp, const double *r, int sign)
+{
+    const mo_cavity *cv = sys[0]->cav;
+    if (!cv || !(cv->spc_mask >> sp & 1)) return;
+    for (int a = 0; a < sys[sp]->ns; a++) sts[0]->cube_cnt[cav_cube(cv, r + 3 * a)] += sign;
+}
+
+/* the cavity list of the current configuration (cavitylist_initSorbInfo,
+ * cavitylist.F90:282-340) */
+void mo_gcmc_cav_rebuild(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *const *sts)
+{
+    const mo_cavity *cv = sys[0]->cav;
+    if (!cv) return;
+    long long nc = (long long)cv->n[0] * cv->n[1] * cv->n[2];
+    for (long long c = 0; c < nc; c++) sts[0]->cube_cnt[c] = 0;
+    for (int sp = 0; sp < nspc; sp++)
+        for (int m = 0; m < sts[sp]->N; m++) cav_move(sys, sts, sp, sts[sp]->r + 3 * sys[sp]->ns * m, 1);
+}
+
+/* stats_update (stats.F90:152-195) of cav_prob(t) with the cavities found */
+static void cav_stats(mo_gcmc_state *st, int t, double v)
+{
+    st->cav_n[t]++;
+    st->cav_avg[t] = (st->cav_avg[t] * ((double)st->cav_n[t] - 1.0) + v) / ((double)st->cav_n[t] * 1.0);
+}
+
 int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *const *sts, long long nsteps)
 {
     const mo_gcmc_sys *g = sys[0];
@@ -480,6 +559,11 @@ int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *cons
     }
     double mconst_s[MO_MAXSPC];
     for (int i = 0; i < nspc; i++) mconst_s[i] = mo_gcmc_mol_const(sys[i]);
+    /* cavity-biased trials: centre, angles and bias per successful trial */
+    int tmax = 0;
+    for (int i = 0; i < nspc; i++)
+        if (sys[i]->cav_ntrials > tmax) tmax = sys[i]->cav_ntrials;
+    double *ctr_trial = tmax ? malloc(sizeof(double) * 7 * tmax) : NULL;
     for (long long it = 0; it < nsteps; it++) {
         int step = (int)G->nsteps;
         /* gcmc_dosim: a species with equal probability (gcmc.F90:240-242) */
@@ -505,41 +589,65 @@ int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *cons
             {
                 double com[3], ang[3] = {0.0, 0.0, 0.0};
                 double biasfactor = 1.0;
-                if (mtype == MO_MV_INSERT) {
+                int found = 1;
+                if (mtype == MO_MV_BINSERT && s->cav_ntrials > 0) {
+                    /* rigidmoves_getConfigFromCavBias (rigidmoves.F90:1987-2126):
+                     * ntrials centres (uniform, or from the bias map), each
+                     * with random angles; the trials whose sites all lie in
+                     * cavities; cav_prob(totmol) updated with their number;
+                     * one of them at random, bias x cav_prob / ntrials.
+                     * totmol counts the molecule being inserted
+                     * (config_getTotMoles after the slot is opened). */
+                    const mo_cavity *cv = g->cav;
+                    const int T = s->cav_ntrials;
+                    int nsucc = 0;
+                    for (int t = 0; t < T; t++) {
+                        double c[3], a[3], bf = 1.0;
+                        if (s->cav_map) bmap_com(s, g, &G->rng, c, &bf);
+                        else
+                            for (int i = 0; i < 3; i++) c[i] = g->full_ell[i] * mo_rranf(&G->rng);
+                        random_angles(ns, &G->rng, a);
+                        toxyz(s, c, a, rpnew);
+                        pbc(g, ns, rpnew, rnew);
+                        int cav = 1;
+                        for (int q = 0; q < ns && cav; q++) {
+                            long long cb = cav_cube(cv, rnew + 3 * q);
+                            cav = (!cv->tab || cv->tab[cb]) && G->cube_cnt[cb] == 0;
+                        }
+                        if (cav) {
+                            double *o = ctr_trial + 7 * nsucc++;
+                            memcpy(o, c, sizeof c);
+                            memcpy(o + 3, a, sizeof a);
+                            o[6] = bf;
+                        }
+                    }
+                    int totmol = 1;
+                    for (int i = 0; i < nspc; i++) totmol += sts[i]->N;
+                    if (totmol >= st->cav_cap) return -5; /* capacity of the cav_prob table */
+                    cav_stats(st, totmol, (double)nsucc);
+                    if (nsucc == 0) {
+                        found = 0;
+                    } else {
+                        int pick = (int)(mo_rranf(&G->rng) * nsucc);
+                        const double *o = ctr_trial + 7 * pick;
+                        memcpy(com, o, sizeof com);
+                        memcpy(ang, o + 3, sizeof ang);
+                        biasfactor = o[6] * (st->cav_avg[totmol] / (double)T);
+                    }
+                } else if (mtype == MO_MV_INSERT) {
                     /* simcell_uniformRandPt (simcell.F90:274-295) */
                     if (slanted(g)) mo_cell_uniform(g->cell, &G->rng, com);
                     else
                         for (int i = 0; i < 3; i++) com[i] = g->full_ell[i] * mo_rranf(&G->rng);
+                    random_angles(ns, &G->rng, ang);
                 } else {
-                    /* rigidmoves_comFromBmap: a cubelet by its weight, a point
-                     * in it (bmap_getbiaspt), a unit cell of the simulation cell */
-                    const mo_bmap *b = s->bias;
-                    long long t = mo_bmap_index(b, mo_rranf(&G->rng));
-                    long long c = b->cube[t];
-                    long long ci[3] = {c / ((long long)b->ncub[1] * b->ncub[2]), (c / b->ncub[2]) % b->ncub[1],
-                                       c % b->ncub[2]};
-                    double wt = b->wt[t] / b->cum_wt;
-                    biasfactor = 1.0 / wt / (double)b->ncubelets;
-                    for (int i = 0; i < 3; i++) com[i] = (double)ci[i] * b->step[i];
-                    for (int i = 0; i < 3; i++) com[i] = com[i] + mo_rranf(&G->rng) * b->step[i];
-                    for (int i = 0; i < 3; i++)
-                        if (com[i] >= b->eff[i]) com[i] = (double)0.9999990f * b->eff[i];
-                    for (int i = 0; i < 3; i++) {
-                        int nc = (int)lround(g->full_ell[i] / g->fund_ell[i]);
-                        int icell = (int)(mo_rranf(&G->rng) * nc);
-                        com[i] = com[i] + g->fund_ell[i] * icell;
-                    }
-                }
-                if (ns != 1) {
-                    double costheta = 1.0 - mo_rranf(&G->rng) * 2.0;
-                    ang[1] = acos(costheta);
-                    ang[0] = MO_PI - mo_rranf(&G->rng) * MO_TWOPI;
-                    ang[2] = MO_PI - mo_rranf(&G->rng) * MO_TWOPI;
+                    bmap_com(s, g, &G->rng, com, &biasfactor);
+                    random_angles(ns, &G->rng, ang);
                 }
                 toxyz(s, com, ang, rpnew);
                 pbc(g, ns, rpnew, rnew);
-                double um, up;
-                int ok = mol_energy_mix(sys, nspc, sts, sp, rnew, rpnew, -1, &um, &up);
+                double um = 0.0, up = 0.0;
+                int ok = found && mol_energy_mix(sys, nspc, sts, sp, rnew, rpnew, -1, &um, &up);
                 double U = 0.0;
                 if (ok) {
                     U = um + up;
@@ -563,6 +671,7 @@ int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *cons
                                 G->Sim[k] += bk[k];
                             }
                         st->N = N + 1;
+                        cav_move(sys, sts, sp, rnew, 1);
                         G->energy += U;
                         st->acc[0]++;
                         de = U;
@@ -577,7 +686,7 @@ int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *cons
                 int mol = (int)(mo_rranf(&G->rng) * N);
                 st->att[1]++;
                 double biasfactor = 1.0;
-                if (mtype == MO_MV_BDELETE) {
+                if (mtype == MO_MV_BDELETE && !(s->cav_ntrials > 0 && !s->cav_map)) {
                     /* rigidmoves_bdelete: bmap_getcellwt of the centre of mass
                      * mapped into the unit cell (simcell_maptouc); a cubelet
                      * outside the bias map stops the reference */
@@ -594,6 +703,14 @@ int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *cons
                     double wt = b->wt[t - 1] / b->cum_wt;
                     biasfactor = (double)b->ncubelets * wt;
                 }
+                if (mtype == MO_MV_BDELETE && s->cav_ntrials > 0) {
+                    /* rigidmoves_bdelete (rigidmoves.F90:1632-1643): divided by
+                     * cav_prob(totmol) / ntrials at the current loading */
+                    int totmol = 0;
+                    for (int i = 0; i < nspc; i++) totmol += sts[i]->N;
+                    if (totmol >= st->cav_cap) return -5;
+                    biasfactor = biasfactor / (st->cav_avg[totmol] / (double)s->cav_ntrials);
+                }
                 const double *r = st->r + 3 * ns * mol;
                 double um, up;
                 int ok = mol_energy_mix(sys, nspc, sts, sp, r, st->rp + 3 * ns * mol, mol, &um, &up);
@@ -607,6 +724,7 @@ int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *cons
                 accepted = attempt(s, G, -U, bias, 1, -st->B, 1, 1.0 * N);
                 if (accepted) {
                     int last = N - 1;
+                    cav_move(sys, sts, sp, st->r + 3 * ns * mol, -1);
                     memmove(st->com + 3 * mol, st->com + 3 * last, sizeof(double) * 3);
                     memmove(st->ang + 3 * mol, st->ang + 3 * last, sizeof(double) * 3);
                     memmove(st->rp + 3 * ns * mol, st->rp + 3 * ns * last, sizeof(double) * 3 * ns);
@@ -675,6 +793,8 @@ int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *cons
                 if (ok) {
                     accepted = attempt(s, G, dU, 1.0, 0, 0.0, 0, 1.0);
                     if (accepted) {
+                        cav_move(sys, sts, sp, st->r + 3 * ns * mol, -1);
+                        cav_move(sys, sts, sp, rnew, 1);
                         memcpy(st->com + 3 * mol, com, sizeof com);
                         memcpy(st->ang + 3 * mol, ang, sizeof ang);
                         memcpy(st->rp + 3 * ns * mol, rpnew, sizeof(double) * 3 * ns);
@@ -705,6 +825,7 @@ int mo_gcmc_run_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *cons
     free(bk);
     free(ck);
     free(dk);
+    free(ctr_trial);
     return 0;
 }
 

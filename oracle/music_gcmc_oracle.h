@@ -37,6 +37,19 @@ long long mo_bmap_index(const mo_bmap *b, double key);
 #define MO_MAXMAPS 8
 #define MO_MAXSPC 4
 
+/* Cavity list of cavity-biased insertion (cavitylist.F90): a grid of
+ * n[0] x n[1] x n[2] cubes over the simulation cell (cavitylist_init,
+ * :155-275); a cube is a cavity when it is tabulated (tab, NULL = all:
+ * cavitylist_initArrUniform / _initArrPmap, :744-978) and no site of a
+ * molecule of the species in spc_mask lies in it
+ * (cavitylist_checkcavityfromIndex, :1137-1157). */
+typedef struct {
+    int n[3];
+    double d[3];          /* cube edges: full_ell / n */
+    const int8_t *tab;    /* [n0][n1][n2] (z fastest) 1 = tabulated, or NULL */
+    int spc_mask;         /* bit s: molecules of species s occupy cubes */
+} mo_cavity;
+
 typedef struct {
     double full_ell[3], fund_ell[3];
     int ns;
@@ -81,6 +94,13 @@ typedef struct {
      * site positions) are further apart (simcell_minimage) has no LJ
      * (lj_comcut2) or plain-Coulomb (coul_comcut2) term; 0 = no cutoff */
     double lj_comcut2[MO_MAXSPC], coul_comcut2[MO_MAXSPC];
+    /* cavity bias (rigidmoves_getConfigFromCavBias, rigidmoves.F90:1987-
+     * 2126): the cavity list (sys[0]'s is the mixture's); this species'
+     * BINSERT / BDELETE are cavity biased when cav_ntrials > 0, with centres
+     * uniform in the cell (CAVITY_BIAS) or from its bias map when cav_map
+     * (MAP_CAVITY_BIAS) */
+    const mo_cavity *cav;
+    int cav_ntrials, cav_map;
 } mo_gcmc_sys;
 
 typedef struct {
@@ -101,6 +121,15 @@ typedef struct {
     mo_gcmc_trace *trace;
     int trace_cap;
     double delta[2]; /* translate / rotate displacement of adjustable moves */
+    /* cavity bias: sites per cube of the cavity list (sts[0]'s), and this
+     * species' running average of the cavities found per insertion at total
+     * loading t (cav_prob(t), stats_update, stats.F90:152-195; each entry
+     * started with one update of ntrials / 2, rigidmoves.F90:2748-2760),
+     * t < cav_cap */
+    int *cube_cnt;
+    double *cav_avg;
+    long long *cav_n;
+    int cav_cap;
 } mo_gcmc_state;
 
 /* Mixtures (gcmc_dosim, gcmc.F90:224-258): sys[i] / sts[i] for species i;
@@ -114,6 +143,7 @@ double mo_gcmc_total_energy_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc
 void mo_gcmc_recompute_sk_mix(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *const *sts);
 void mo_gcmc_set_mol_mix(const mo_gcmc_sys *const *sys, mo_gcmc_state *const *sts, int sp, int m,
                          const double com[3], const double ang[3]);
+void mo_gcmc_cav_rebuild(const mo_gcmc_sys *const *sys, int nspc, mo_gcmc_state *const *sts);
 void mo_gcmc_kvecs(const mo_gcmc_sys *s, int *nk, double *kv, double *w);
 double mo_gcmc_mol_const(const mo_gcmc_sys *s);
 int mo_gcmc_run(const mo_gcmc_sys *s, mo_gcmc_state *st, long long nsteps);

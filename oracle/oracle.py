@@ -411,7 +411,12 @@ class GcmcSys(ctypes.Structure):
                 ("insdelratio", _d), ("dtrans", _d), ("drot", _d), ("max_nrg", _d), ("bias", _vp),
                 ("coul", _i), ("coul_on", _i * 64), ("coul_hicut", _d), ("coul_locut", _d),
                 ("coul_shield", _d), ("adjust_trans", _i), ("adjust_rot", _i), ("cell", _vp),
-                ("lj_comcut2", _d * 4), ("coul_comcut2", _d * 4)]
+                ("lj_comcut2", _d * 4), ("coul_comcut2", _d * 4), ("cav", _vp), ("cav_ntrials", _i),
+                ("cav_map", _i)]
+
+
+class MoCavity(ctypes.Structure):
+    _fields_ = [("n", _i * 3), ("d", _d * 3), ("tab", _vp), ("spc_mask", _i)]
 
 
 class GcmcTrace(ctypes.Structure):
@@ -428,7 +433,7 @@ class GcmcState(ctypes.Structure):
                 ("nk", _i), ("kv", _vp), ("wk", _vp), ("Sre", _vp), ("Sim", _vp),
                 ("energy", _d), ("B", _d), ("rng", _Rng), ("att", _ll * 4), ("acc", _ll * 4),
                 ("sumN", _d), ("nsteps", _ll), ("trace", _vp), ("trace_cap", _i),
-                ("delta", _d * 2)]
+                ("delta", _d * 2), ("cube_cnt", _vp), ("cav_avg", _vp), ("cav_n", _vp), ("cav_cap", _i)]
 
 
 def _gcmc_lib():
@@ -449,6 +454,7 @@ def _gcmc_lib():
         L.mo_gcmc_total_energy_mix.restype = _d
         L.mo_gcmc_recompute_sk_mix.argtypes = [_vp, _i, _vp]
         L.mo_gcmc_set_mol_mix.argtypes = [_vp, _vp, _i, _i, _vp, _vp]
+        L.mo_gcmc_cav_rebuild.argtypes = [_vp, _i, _vp]
         L.mo_adams_B.argtypes = [_d, _d, _d, _d]
         L.mo_adams_B.restype = _d
         L.mo_rng_init.argtypes = [_vp, ctypes.c_int32]
@@ -508,7 +514,11 @@ class Gcmc:
     maps: list of (npnts, step, ptr, vals8) and site_pmap/site_emap,
     ewald (kappa, kmax, rc) or coul (on [nst*nst], hicut, locut, shield), T,
     moves [(type, weight)], dtrans, drot, and for BINSERT / BDELETE bias (a
-    Bmap); a slanted simulation cell as 'cell' (from cell())."""
+    Bmap); a slanted simulation cell as 'cell' (from cell()).  Cavity bias:
+    'cavity' = dict(n=(nx, ny, nz), tab=None or int8 [nx, ny, nz] (1 =
+    tabulated), mask=species bits that occupy cubes), and per species
+    'cav_ntrials' (> 0: its BINSERT / BDELETE are cavity biased) and
+    'cav_map' (MAP_CAVITY_BIAS: centres from its bias map)."""
 
     def __init__(self, spec, fugacity_kPa, seed, maxN=2048, trace=0):
         L = _gcmc_lib()
@@ -584,6 +594,23 @@ class Gcmc:
             v = list(v) if np.ndim(v) else [v] * 4
             for i, r in enumerate(v):
                 fld[i] = float(r) ** 2 if r else 0.0
+        cv = spec.get("cavity")
+        self.cav = None
+        if cv is not None:
+            c = MoCavity()
+            for i in range(3):
+                c.n[i] = int(cv["n"][i])
+                c.d[i] = float(spec["full_ell"][i]) / int(cv["n"][i])
+            tab = cv.get("tab")
+            if tab is not None:
+                tab = np.ascontiguousarray(tab, dtype=np.int8).reshape(-1)
+                self.keep.append(tab)
+                c.tab = tab.ctypes.data
+            c.spc_mask = int(cv.get("mask", 1))
+            self.cav = c
+            s.cav = ctypes.cast(ctypes.byref(c), _vp)
+        s.cav_ntrials = int(spec.get("cav_ntrials", 0))
+        s.cav_map = int(spec.get("cav_map", 0))
         self.sys = s
         st = GcmcState()
         st.maxN = maxN
@@ -609,6 +636,14 @@ class Gcmc:
             vol = cell_volume(self.cell)
         st.B = adams_B(fugacity_kPa, vol, spec["T"])
         L.mo_rng_init(ctypes.byref(st.rng), int(seed))
+        ncube = int(np.prod(cv["n"])) if cv is not None else 1
+        self.cube_cnt = np.zeros(ncube, dtype=np.int32)
+        # cav_prob(t) of rigidmoves_equateInsBias: one update of ntrials / 2
+        self.cav_cap = 4 * maxN + 2
+        self.cav_avg = np.full(self.cav_cap, s.cav_ntrials / 2.0)
+        self.cav_n = np.ones(self.cav_cap, dtype=np.int64)
+        st.cube_cnt, st.cav_avg, st.cav_n = [a.ctypes.data for a in (self.cube_cnt, self.cav_avg, self.cav_n)]
+        st.cav_cap = self.cav_cap
         self.trace = (GcmcTrace * max(trace, 1))()
         st.trace = ctypes.cast(self.trace, _vp) if trace else None
         st.trace_cap = trace
@@ -624,6 +659,9 @@ class Gcmc:
                               np.ascontiguousarray(ang[m]).ctypes.data)
         self.st.N = len(com)
         L.mo_gcmc_recompute_sk(ctypes.byref(self.sys), ctypes.byref(self.st))
+        sp = (_vp * 1)(ctypes.cast(ctypes.byref(self.sys), _vp))
+        st = (_vp * 1)(ctypes.cast(ctypes.byref(self.st), _vp))
+        L.mo_gcmc_cav_rebuild(sp, 1, st)
         self.st.energy = self.total_energy()
 
     def run(self, nsteps):
@@ -642,7 +680,7 @@ class Gcmc:
         return np.array([(t.move, t.accepted, t.N, t.dE) for t in self.trace[:n]])
 
 
-GLOBAL_KEYS = ("full_ell", "fund_ell", "nst", "lj", "coul", "ewald", "T", "cell")
+GLOBAL_KEYS = ("full_ell", "fund_ell", "nst", "lj", "coul", "ewald", "T", "cell", "cavity")
 
 
 class GcmcMix:
@@ -670,6 +708,7 @@ class GcmcMix:
                                       np.ascontiguousarray(ang[m]).ctypes.data)
             self.ch[i].st.N = len(com)
         L.mo_gcmc_recompute_sk_mix(self.sysp, self.nspc, self.stp)
+        L.mo_gcmc_cav_rebuild(self.sysp, self.nspc, self.stp)
         self.ch[0].st.energy = self.total_energy()
 
     def run(self, nsteps):
