@@ -295,6 +295,46 @@ int mh_gcmc_create(mh_ctx *ctx, const mh_gcmc_params *p, int nrep, const double 
  * move = 16 species + move type and N of the moved species. */
 int mh_gcmc_create_mix(mh_ctx *ctx, const mh_gcmc_params *spc, int nspc, int nrep, const double *fugacity_kPa,
                        const int32_t *seeds, int maxN, mh_gcmc **out);
+
+/* Cavity-biased insertion and deletion (BINSERT line "CAVITY" = CAVITY_BIAS
+ * or "bmap, CAVITY" = MAP_CAVITY_BIAS; rigidmoves_initInsBias,
+ * rigidmoves.F90:620-670).  The cavity list (cavitylist.F90, control file
+ * section "Cavity Bias Info", cavitylist_readCtrlFile :1022-1082) is a grid
+ * of n[0] x n[1] x n[2] cubes over the simulation cell (edge ell / n); a
+ * cube is a cavity when it is tabulated (tab[n0][n1][n2], z fastest, 1 =
+ * tabulated; NULL = every cube, cavitylist_initArrUniform) and no site of a
+ * molecule of the species in species_mask (bit s) lies in it.  A cavity-
+ * biased insertion of species s (rigidmoves_getConfigFromCavBias, :1987-
+ * 2126) draws ntrials[s] centres, uniform in the cell or, when
+ * map_bias[s], from the species' bias map (params->bias), each with random
+ * angles; the trials whose sites all lie in cavities are counted into the
+ * running average cav_prob(totmol) (totmol = molecules of all species with
+ * the new one; every entry starts with one update of ntrials / 2,
+ * rigidmoves_equateInsBias :2748-2760); with none the insertion fails;
+ * otherwise one of them at random enters the acceptance with its map bias x
+ * cav_prob / ntrials.  The species' BDELETE divides its bias by cav_prob /
+ * ntrials at the current total (rigidmoves_bdelete :1632-1643).
+ * ntrials[s] = 0: species s moves without cavity bias.  ntrials <= 1024;
+ * launches use at least ntrials threads.  Orthorhombic cells, not for slot
+ * pools.  Cube counts follow every accepted move and mh_gcmc_set_config. */
+typedef struct {
+    int n[3];
+    const uint8_t *tab;
+    int species_mask;
+    int ntrials[MH_MAX_SPECIES];
+    int map_bias[MH_MAX_SPECIES];
+} mh_cavity;
+int mh_gcmc_set_cavity(mh_gcmc *g, const mh_cavity *c);
+/* cav_prob(t) of the selected species for t < cap (running average, update
+ * count) and the site count of every cube, of replica rep (any pointer may
+ * be NULL). */
+int mh_gcmc_get_cavity(mh_gcmc *g, int rep, int cap, double *avg, int64_t *n, int32_t *cubes);
+/* cavitylist_initArrPmap (cavitylist.F90:790-978): the tabulation from a
+ * pmap: the cube (edge full_ell / n) holding the centre of each map cubelet
+ * whose lower-corner node is tabulated with V < ucutoff (kcal/mol), marked
+ * in each of the ncell unit cells of the simulation cell. */
+int mh_cavity_tab(const mh_map *pmap, double ucutoff, const int n[3], const int ncell[3], const double full_ell[3],
+                  uint8_t *tab);
 /* Species addressed by the per-species calls that follow (mh_gcmc_get_state's
  * N / sumN / att / acc, mh_gcmc_get_moments, mh_gcmc_get_deltas /
  * set_deltas, mh_gcmc_set_counts, mh_gcmc_set_config / get_config,

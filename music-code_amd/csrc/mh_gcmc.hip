@@ -58,6 +58,9 @@ struct SpcDev {
     int adj_trans, adj_rot;  // adjustable displacements (rigidmoves_adjustdelta*)
     double inv_ns, inv_nss;  // 1/ns, 1/ns^2 for the pair-item decomposition
     BiasRef bias;            // BINSERT / BDELETE
+    // cavity-biased BINSERT / BDELETE (CAVITY_BIAS, MAP_CAVITY_BIAS): trials
+    // per insertion (0: not cavity biased), centres from the bias map
+    int cav_T, cav_map;
 };
 
 // A molecular system: the SpcDev base is the species of the move in
@@ -98,6 +101,12 @@ struct GDev : SpcDev {
     int slanted;
     SlantCell sc;
     double lvec[3][3];
+    // cavity list (cavitylist.F90): cav_n cubes per axis of edge cav_d over
+    // the simulation cell, tabulated cubes (NULL: all), species whose sites
+    // occupy cubes (bit s)
+    int cav_on, cav_n[3], cav_mask;
+    double cav_d[3];
+    const uint8_t *cav_tab;
 };
 
 // rigidmoves_adjustdeltatrans / _adjustdeltarot (rigidmoves.F90:2586-2617):
@@ -131,6 +140,13 @@ struct Rep {
     const int *sys;
     int *budget;
     long long kstride;        // S(k) entries per replica
+    // cavity bias: sites per cube [rep][cube_stride]; cav_prob running
+    // averages and their update counts [rep][GSPC][cav_cap]
+    int *cube;
+    long long cube_stride;
+    double *cav_avg;
+    long long *cav_cnt;
+    int cav_cap;
 };
 
 
@@ -409,6 +425,9 @@ struct Shared {
     double2 E[2][GS][3][GKMAX + 1];      // per position set, site, axis: e^{i m 2 pi x / L}
     float2 Ef[2][GS][3][GKMAX + 1];      // FP32 copy
     double red[16 * 6];
+    unsigned long long cavbits[16];      // cavity trials that found a cavity, per wave
+    double cavsel[7];                    // the chosen trial: centre, angles, bias
+    double cavpc;                        // cav_prob(totmol) after its update
 };
 
 // k vectors owned by one lane, kept in registers for the launch
@@ -812,6 +831,149 @@ __device__ __forceinline__ void sk_accept(const GDev &P, KReg<KPL> &K, double2 *
 // ([maxN][ns][3] sites, [maxN][3] com, [maxN][3] angles, FP64).
 // KPLX >= 0: k vectors per lane in registers (orthorhombic cell); KPLX < 0:
 // slanted cell (no sorbate Ewald sum).
+// rigidmoves_comFromBmap (rigidmoves.F90:1938-1969): a cubelet by weight, a
+// point in it (bmap_getbiaspt), a unit cell of the simulation cell; biasf =
+// 1 / (weight ncubelets)
+__device__ __forceinline__ void bmap_com(const GDev &P, Rng &rng, double c[3], double &biasf) {
+    const BiasRef &Bm = P.bias;
+    const long long t = bias_index(Bm, rranf(rng));
+    const long long cb = Bm.cube[t];
+    const long long ci[3] = {cb / ((long long)Bm.ncub[1] * Bm.ncub[2]), (cb / Bm.ncub[2]) % Bm.ncub[1],
+                             cb % Bm.ncub[2]};
+    biasf = __ddiv_rn(__ddiv_rn(1.0, Bm.wn[t]), Bm.ncubelets);
+    for (int i = 0; i < 3; i++) c[i] = __dmul_rn((double)ci[i], Bm.step[i]);
+    for (int i = 0; i < 3; i++) c[i] = __dadd_rn(c[i], __dmul_rn(rranf(rng), Bm.step[i]));
+    for (int i = 0; i < 3; i++)
+        if (c[i] >= Bm.eff[i]) c[i] = __dmul_rn((double)0.9999990f, Bm.eff[i]);
+    for (int i = 0; i < 3; i++) {
+        const int icell = (int)(rranf(rng) * (double)P.ncell[i]);
+        c[i] = __dadd_rn(c[i], __dmul_rn(P.fund[i], (double)icell));
+    }
+}
+
+// cavitylist_getIndexVec (cavitylist.F90:536-546): Int(r / d) per axis of a
+// folded position, z fastest; a coordinate rounding onto the upper face
+// stays in the last cube
+__device__ __forceinline__ long long cav_cube(const GDev &P, const double *r) {
+    long long c = 0;
+    for (int i = 0; i < 3; i++) {
+        int k = (int)__ddiv_rn(r[i], P.cav_d[i]);
+        k = min(max(k, 0), P.cav_n[i] - 1);
+        c = c * P.cav_n[i] + k;
+    }
+    return c;
+}
+
+// rigidmoves_getConfigFromCavBias (rigidmoves.F90:1987-2126): cav_T trial
+// centres (uniform, or from the bias map) with random angles, drawn in
+// order by every lane; lane t checks trial t (every site in a tabulated
+// cube holding no site of a listed species); cav_prob(totmol) updated with
+// the number found (stats_update, by lane 0, which alone reads and writes
+// the table); one of the successes at random.  Returns 0 with the trial in
+// com / ang and biasf = its map bias x cav_prob / cav_T, 1 when no trial
+// found a cavity (the insertion fails), 5 when the total loading exceeds
+// the cav_prob table.  Requires cav_T <= NT.
+template <int NT>
+__device__ int cavity_trials(const GDev &P, Shared &sh, const Rep &R, int rep, int s, Rng &rng, double com[3],
+                             double ang[3], double &biasf) {
+    const int tid = threadIdx.x, T = P.cav_T, ns = P.ns;
+    double my[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0};
+    double mycos = 1.0;
+    for (int t = 0; t < T; t++) {
+        double c[3], a[3] = {0.0, 0.0, 0.0}, bf = 1.0, ct = 1.0;
+        if (P.cav_map) bmap_com(P, rng, c, bf);
+        else
+            for (int i = 0; i < 3; i++) c[i] = __dmul_rn(P.full[i], rranf(rng));
+        if (ns != 1) {  // rigidmoves_reorient with newxyz
+            ct = __dsub_rn(1.0, __dmul_rn(rranf(rng), 2.0));
+            a[0] = __dsub_rn(3.1415927, __dmul_rn(rranf(rng), kTwoPiR));
+            a[2] = __dsub_rn(3.1415927, __dmul_rn(rranf(rng), kTwoPiR));
+        }
+        if (t == tid) {
+            for (int i = 0; i < 3; i++) {
+                my[i] = c[i];
+                my[3 + i] = a[i];
+            }
+            my[6] = bf;
+            mycos = ct;
+        }
+    }
+    bool found = false;
+    if (tid < T) {
+        if (ns != 1) my[4] = acos(mycos);
+        double sc[3][2];
+        for (int k = 0; k < 3; k++) sincos(my[3 + k], &sc[k][0], &sc[k][1]);
+        const int *cube = R.cube + (long long)rep * R.cube_stride;
+        found = true;
+        for (int a = 0; a < ns && found; a++) {
+            double r[3];
+            for (int i = 0; i < 3; i++) {
+                if (ns == 1) {
+                    r[i] = wrap_full(my[i], P.full[i]);
+                } else {
+                    double m[3];
+                    euler_row(i, sc[0][0], sc[0][1], sc[1][0], sc[1][1], sc[2][0], sc[2][1], m);
+                    r[i] = wrap_full(site_coord_unf(P, a, i, my, m), P.full[i]);
+                }
+            }
+            const long long cb = cav_cube(P, r);
+            found = (!P.cav_tab || P.cav_tab[cb]) &&
+                    __hip_atomic_load(cube + cb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+        }
+    }
+    const unsigned long long bal = __ballot(found);
+    if ((tid & 63) == 0) sh.cavbits[tid >> 6] = bal;
+    __syncthreads();
+    int nsucc = 0;
+    for (int w = 0; w < NT / 64; w++) nsucc += __popcll(sh.cavbits[w]);
+    // config_getTotMoles with the slot of the new molecule opened
+    int totmol = 1;
+    for (int i = 0; i < P.nspc; i++) totmol += sh.N[i];
+    if (totmol >= R.cav_cap) return 5;
+    if (tid == 0) {
+        const long long o = ((long long)rep * GSPC + s) * R.cav_cap + totmol;
+        const long long n1 = R.cav_cnt[o] + 1;
+        const double avg = __ddiv_rn(__dadd_rn(__dmul_rn(R.cav_avg[o], __dsub_rn((double)n1, 1.0)), (double)nsucc),
+                                     __dmul_rn((double)n1, 1.0));
+        R.cav_avg[o] = avg;
+        R.cav_cnt[o] = n1;
+        sh.cavpc = avg;
+    }
+    __syncthreads();
+    if (nsucc == 0) return 1;
+    // the pick-th success in trial order
+    const int pick = (int)(rranf(rng) * (double)nsucc);
+    int tsel = 0, left = pick;
+    for (int w = 0; w < NT / 64; w++) {
+        unsigned long long b = sh.cavbits[w];
+        const int c = __popcll(b);
+        if (left < c) {
+            for (int j = 0; j < left; j++) b &= b - 1;
+            tsel = 64 * w + __builtin_ctzll(b);
+            break;
+        }
+        left -= c;
+    }
+    if (tid == tsel)
+        for (int i = 0; i < 7; i++) sh.cavsel[i] = my[i];
+    __syncthreads();
+    for (int i = 0; i < 3; i++) {
+        com[i] = sh.cavsel[i];
+        ang[i] = sh.cavsel[3 + i];
+    }
+    biasf = __dmul_rn(sh.cavsel[6], __ddiv_rn(sh.cavpc, (double)T));
+    return 0;
+}
+
+// molecules entering (sign 1) or leaving (-1) the cavity list: its folded
+// sites r[3 ns] (cavitylist_updateMolecInfo, cavitylist.F90:351-513, as
+// site counts per cube)
+__device__ __forceinline__ void cav_sites(const GDev &P, const Rep &R, int rep, int s, const double *r, int sign) {
+    const int tid = threadIdx.x;
+    if (P.cav_on && (P.cav_mask >> s & 1) && tid < P.ns)
+        atomicAdd(R.cube + (long long)rep * R.cube_stride + cav_cube(P, r + 3 * tid), sign);
+}
+
 template <int NT, int PREC, int KPLX, bool LDSMOL>
 __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep R, int nsteps) {
     constexpr int KPL = KPLX > 0 ? KPLX : 0;
@@ -943,7 +1105,19 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
         int mol = -1;
         // ---- trial generation (identical in every lane)
         double tcom[3], tang[3] = {0.0, 0.0, 0.0};
-        if (mbase == MH_MV_INSERT) {
+        if (mbase == MH_MV_INSERT && biased && P.cav_T > 0) {
+            att[0]++;
+            if (N >= P.maxN) {
+                err = 2;  // capacity
+                break;
+            }
+            const int c = cavity_trials<NT>(P, sh, R, rep, s, rng, tcom, tang, biasf);
+            if (c == 5) {
+                err = 5;
+                break;
+            }
+            doit = c == 0;
+        } else if (mbase == MH_MV_INSERT) {
             att[0]++;
             if (N >= P.maxN) {
                 err = 2;  // capacity
@@ -963,22 +1137,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                     for (int i = 0; i < 3; i++) tcom[i] = __dmul_rn(P.full[i], rranf(rng));
                 }
             } else {
-                // rigidmoves_comFromBmap: a cubelet by weight, a point in it
-                // (bmap_getbiaspt), a unit cell of the simulation cell
-                const BiasRef &Bm = P.bias;
-                const long long t = bias_index(Bm, rranf(rng));
-                const long long c = Bm.cube[t];
-                const long long ci[3] = {c / ((long long)Bm.ncub[1] * Bm.ncub[2]), (c / Bm.ncub[2]) % Bm.ncub[1],
-                                         c % Bm.ncub[2]};
-                biasf = __ddiv_rn(__ddiv_rn(1.0, Bm.wn[t]), Bm.ncubelets);
-                for (int i = 0; i < 3; i++) tcom[i] = __dmul_rn((double)ci[i], Bm.step[i]);
-                for (int i = 0; i < 3; i++) tcom[i] = __dadd_rn(tcom[i], __dmul_rn(rranf(rng), Bm.step[i]));
-                for (int i = 0; i < 3; i++)
-                    if (tcom[i] >= Bm.eff[i]) tcom[i] = __dmul_rn((double)0.9999990f, Bm.eff[i]);
-                for (int i = 0; i < 3; i++) {
-                    const int icell = (int)(rranf(rng) * (double)P.ncell[i]);
-                    tcom[i] = __dadd_rn(tcom[i], __dmul_rn(P.fund[i], (double)icell));
-                }
+                bmap_com(P, rng, tcom, biasf);
             }
             if (ns != 1) {
                 double costheta = __dsub_rn(1.0, __dmul_rn(rranf(rng), 2.0));
@@ -996,7 +1155,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                     tcom[i] = comS[3 * mol + i];
                     tang[i] = angS[3 * mol + i];
                 }
-                if (mbase == MH_MV_DELETE && biased) {
+                if (mbase == MH_MV_DELETE && biased && !(P.cav_T > 0 && !P.cav_map)) {
                     // rigidmoves_bdelete: bmap_getcellwt of the centre of mass
                     // mapped into the unit cell (simcell_maptouc)
                     const BiasRef &Bm = P.bias;
@@ -1011,6 +1170,21 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                         break;
                     }
                     biasf = __dmul_rn(Bm.ncubelets, Bm.wn[t - 1]);
+                }
+                if (mbase == MH_MV_DELETE && biased && P.cav_T > 0) {
+                    // rigidmoves_bdelete (rigidmoves.F90:1632-1643): divided
+                    // by cav_prob(totmol) / cav_T at the current loading; the
+                    // table is lane 0's
+                    int totmol = 0;
+                    for (int i = 0; i < nspc; i++) totmol += sh.N[i];
+                    if (totmol >= R.cav_cap) {
+                        err = 5;
+                        break;
+                    }
+                    if (tid == 0) sh.cavpc = R.cav_avg[((long long)rep * GSPC + s) * R.cav_cap + totmol];
+                    __syncthreads();
+                    biasf = __ddiv_rn(biasf, __ddiv_rn(sh.cavpc, (double)P.cav_T));
+                    __syncthreads();
                 }
                 if (mtype == MH_MV_TRANSLATE) {
                     for (int i = 0; i < 3; i++) {
@@ -1097,6 +1271,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                         }
                         if (tid < n3) posS[N * ms3 + tid] = sh.posA[tid];
                         if (P.comcut && tid < 3) ctrS[3 * N + tid] = sh.ctr[0][tid];
+                        cav_sites(P, R, rep, s, sh.posA, 1);
                         if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                         N++;
                         energy += U;
@@ -1127,6 +1302,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                     }
                     if (tid < n3) posS[mol * ms3 + tid] = posS[last * ms3 + tid];
                     if (P.comcut && tid < 3) ctrS[3 * mol + tid] = ctrS[3 * last + tid];
+                    cav_sites(P, R, rep, s, sh.posA, -1);
                     if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                     N--;
                     energy -= U;
@@ -1151,6 +1327,8 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                         }
                         if (tid < n3) posS[mol * ms3 + tid] = sh.posA[tid];
                         if (P.comcut && tid < 3) ctrS[3 * mol + tid] = sh.ctr[0][tid];
+                        cav_sites(P, R, rep, s, sh.posB, -1);
+                        cav_sites(P, R, rep, s, sh.posA, 1);
                         if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                         energy += dU;
                         acc[mtype]++;
@@ -1164,6 +1342,9 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                     drt = adjust_delta(drt, (double)acc[3] / (double)att[3]);
             }
         }
+        // cube counts changed by atomics at the device level: complete before
+        // the next step's trials read them
+        if (P.cav_on && accepted) __threadfence();
         if (tid == 0 && R.tr_cap > 0) {
             long long t = nst - R.tr_base;
             if (t >= 0 && t < R.tr_cap) {
@@ -1420,6 +1601,32 @@ struct SysHost {
     std::vector<int> slots;  // replicas (slots) of this system, admit order
 };
 
+// cavity list of the stored configurations (cavitylist_initSorbInfo,
+// cavitylist.F90:282-340): cleared, then every site of the listed species
+// counted in its cube (two launches: the clear completes first)
+__global__ void k_cav_clear(Rep R) {
+    int *c = R.cube + (long long)blockIdx.y * R.cube_stride;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < R.cube_stride;
+         i += (long long)gridDim.x * blockDim.x)
+        c[i] = 0;
+}
+
+__global__ void k_cav_fill(const GDev *__restrict__ Ptab, Rep R) {
+    const GDev &P = Ptab[0];
+    const int rep = blockIdx.x;
+    const long long slots = (long long)P.nspc * P.maxN, ms3 = 3LL * P.nsmax;
+    const double *gpos = R.pos + (long long)rep * slots * ms3;
+    for (int s = 0; s < P.nspc; s++) {
+        if (!(P.cav_mask >> s & 1)) continue;
+        const int ns = P.sp[s].ns, n = R.N[GSPC * rep + s];
+        for (int i = threadIdx.x; i < n * ns; i += blockDim.x) {
+            const int m = i / ns, a = i - m * ns;
+            const double *r = gpos + ((long long)s * P.maxN + m) * ms3 + 3 * a;
+            atomicAdd(R.cube + (long long)rep * R.cube_stride + cav_cube(P, r), 1);
+        }
+    }
+}
+
 struct mh_gcmc {
     mh_ctx *ctx = nullptr;
     int nrep = 0, maxN = 0, prec = 0, threads = 256;
@@ -1441,6 +1648,11 @@ struct mh_gcmc {
     DBuf N, err, ix, iy, first, iseed, B, energy, sumN, sumN2, nsteps, att, acc;
     DBuf com, ang, pos, ctr, S, dS, delta;
     DBuf tr_move, tr_acc, tr_N, tr_dE;
+    // cavity bias (mh_gcmc_set_cavity): cube counts, cav_prob tables, the
+    // tabulation; launches use at least cav_tmax threads (one trial per lane)
+    DBuf cube, cav_avg, cav_cnt, cav_tab;
+    long long ncube = 0;
+    int cav_cap = 0, cav_tmax = 0;
     int tr_cap = 0;
     long long tr_base = 0;
     // hipGraph of one cycle's launch (mh_gcmc_run_cycles) and what it baked in
@@ -1486,6 +1698,11 @@ struct mh_gcmc {
         r.sys = pool ? sysidx.as<int>() : nullptr;
         r.budget = pool ? budget.as<int>() : nullptr;
         r.kstride = kstride;
+        r.cube = cube.as<int>();
+        r.cube_stride = ncube;
+        r.cav_avg = cav_avg.as<double>();
+        r.cav_cnt = cav_cnt.as<long long>();
+        r.cav_cap = cav_cap;
         return r;
     }
     const GDev &sys_of(int rep) const { return sys[pool ? slot_sys[rep] : 0].H; }
@@ -1621,6 +1838,20 @@ static int plan(mh_gcmc *g) {
     }
     if (extra) g->lds_bytes = (size_t)(off + extra);
     MH_HIP(hipMemcpy(g->dP.p, tab.data(), sizeof(GDev) * tab.size(), hipMemcpyHostToDevice));
+    return MH_OK;
+}
+
+// BINSERT / BDELETE draw from the species' bias map unless they are
+// cavity biased without one (CAVITY_BIAS, set by mh_gcmc_set_cavity)
+static int moves_ready(const GDev &H, int nspc, const char *who) {
+    for (int s = 0; s < nspc; s++) {
+        const SpcDev &Q = H.sp[s];
+        for (int k = 0; k < Q.nmoves; k++) {
+            if (Q.mtype[k] < MH_MV_BINSERT || Q.bias.cube || (Q.cav_T > 0 && !Q.cav_map)) continue;
+            set_error(std::string(who) + ": BINSERT / BDELETE need a bias map (or CAVITY_BIAS)");
+            return MH_EINVAL;
+        }
+    }
     return MH_OK;
 }
 
@@ -1775,7 +2006,6 @@ static int build_system(mh_ctx *ctx, const mh_gcmc_params *p, int maxN, const do
     for (int i = 0; i < p->nmoves; i++) {
         const int mt = p->move_type[i];
         MH_CHECK(mt >= 0 && mt <= MH_MV_BDELETE, MH_EINVAL, w + ": bad move type");
-        MH_CHECK(mt < MH_MV_BINSERT || p->bias, MH_EINVAL, w + ": BINSERT / BDELETE need a bias map");
         H.mtype[i] = mt;
         double wn = p->move_weight[i] / wsum;
         c = c + wn;
@@ -1995,10 +2225,21 @@ int mh_gcmc_set_threads(mh_gcmc *g, int threads) {
     MH_CHECK(g, MH_EINVAL, "mh_gcmc_set_threads: g is NULL");
     MH_CHECK(threads == 64 || threads == 128 || threads == 256 || threads == 512 || threads == 1024,
              MH_EINVAL, "mh_gcmc_set_threads: threads must be 64, 128, 256, 512 or 1024");
+    MH_CHECK(threads >= g->cav_tmax, MH_EINVAL, "mh_gcmc_set_threads: fewer threads than cavity trials");
     MH_HIP(hipSetDevice(g->ctx->device));
     MH_HIP(hipStreamSynchronize(g->ctx->stream));
     g->threads = threads;
     return plan(g);
+}
+
+static int cav_rebuild(mh_gcmc *g) {
+    Rep r = g->rep();
+    const int bx = (int)std::min<long long>((g->ncube + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_cav_clear, dim3(bx, g->nrep), dim3(256), 0, g->ctx->stream, r);
+    hipLaunchKernelGGL(k_cav_fill, dim3(g->nrep), dim3(256), 0, g->ctx->stream, g->dP.as<GDev>(), r);
+    MH_HIP(hipGetLastError());
+    MH_HIP(hipStreamSynchronize(g->ctx->stream));
+    return MH_OK;
 }
 
 /* ---------------------------------------------------------- slot pools */
@@ -2065,6 +2306,7 @@ int mh_pool_admit(mh_gcmc *g, const mh_gcmc_params *p, const double fund_ell[3],
     mh_gcmc_params q = *p;
     q.precision = g->prec;
     int rc = build_system(g->ctx, &q, g->maxN, fund_ell, full_ell, S, "mh_pool_admit", false);
+    if (!rc) rc = moves_ready(S.H, 1, "mh_pool_admit");
     if (!rc && S.H.nk > g->kstride) {
         set_error("mh_pool_admit: more k vectors than the pool's kmax_cap");
         rc = MH_ECAPACITY;
@@ -2211,15 +2453,151 @@ int mh_gcmc_set_config(mh_gcmc *g, int rep, int n, const double *com, const doub
                            sizeof(double) * 3 * ns, sizeof(double) * 3 * ns, n, hipMemcpyHostToDevice));
     }
     MH_HIP(hipMemcpy(g->N.as<int>() + GSPC * rep + g->sel, &n, sizeof(int), hipMemcpyHostToDevice));
+    if (g->ncube) {
+        int rc = cav_rebuild(g);
+        if (rc) return rc;
+    }
     // structure factor and energy of every replica from their coordinates
     std::vector<double> e(g->nrep);
     int rc = mh_gcmc_total_energy(g, e.data(), 1);
     return rc;
 }
 
+/* ---------------------------------------------------------- cavity bias */
+
+int mh_gcmc_set_cavity(mh_gcmc *g, const mh_cavity *c) {
+    MH_CHECK(g && c, MH_EINVAL, "mh_gcmc_set_cavity: bad arguments");
+    MH_CHECK(!g->pool, MH_EINVAL, "mh_gcmc_set_cavity: not for slot pools");
+    MH_CHECK(!g->slanted, MH_EINVAL, "mh_gcmc_set_cavity: the cube grid needs an orthorhombic cell");
+    MH_CHECK(!g->ncube, MH_EINVAL, "mh_gcmc_set_cavity: the batch already has a cavity list");
+    GDev &H = g->sys[0].H;
+    long long nc = 1;
+    for (int i = 0; i < 3; i++) {
+        MH_CHECK(c->n[i] >= 1 && c->n[i] <= 4096, MH_EINVAL, "mh_gcmc_set_cavity: cubes per axis out of [1, 4096]");
+        nc *= c->n[i];
+    }
+    MH_CHECK(nc <= (1LL << 28), MH_EINVAL, "mh_gcmc_set_cavity: more than 2^28 cubes");
+    MH_CHECK(c->species_mask >= 0 && c->species_mask < (1 << g->nspc), MH_EINVAL,
+             "mh_gcmc_set_cavity: species_mask names a species the batch does not have");
+    int tmax = 0;
+    for (int s = 0; s < g->nspc; s++) {
+        const int T = c->ntrials[s];
+        MH_CHECK(T >= 0 && T <= 1024, MH_EINVAL, "mh_gcmc_set_cavity: ntrials out of [0, 1024]");
+        if (T == 0) continue;
+        bool bins = false;
+        for (int k = 0; k < H.sp[s].nmoves; k++) bins = bins || H.sp[s].mtype[k] == MH_MV_BINSERT;
+        MH_CHECK(bins, MH_EINVAL, "mh_gcmc_set_cavity: a cavity-biased species has no BINSERT move");
+        MH_CHECK(!c->map_bias[s] || H.sp[s].bias.cube, MH_EINVAL,
+                 "mh_gcmc_set_cavity: MAP_CAVITY_BIAS needs the species' bias map");
+        tmax = std::max(tmax, T);
+    }
+    MH_CHECK(tmax > 0, MH_EINVAL, "mh_gcmc_set_cavity: no species is cavity biased");
+    MH_HIP(hipSetDevice(g->ctx->device));
+    MH_HIP(hipStreamSynchronize(g->ctx->stream));
+    const int cap = g->nspc * g->maxN + 2;
+    const long long nt = (long long)g->nrep * GSPC * cap;
+    int rc;
+    if ((rc = g->cube.alloc(sizeof(int) * nc * g->nrep)) || (rc = g->cav_avg.alloc(sizeof(double) * nt)) ||
+        (rc = g->cav_cnt.alloc(sizeof(long long) * nt)) || (c->tab && (rc = g->cav_tab.alloc(nc))))
+        return rc;
+    if (c->tab) MH_HIP(hipMemcpy(g->cav_tab.p, c->tab, nc, hipMemcpyHostToDevice));
+    // rigidmoves_equateInsBias (rigidmoves.F90:2748-2760): every entry
+    // starts with one update of ntrials / 2
+    std::vector<double> avg(nt, 0.0);
+    std::vector<long long> cnt(nt, 1);
+    for (int r = 0; r < g->nrep; r++)
+        for (int s = 0; s < g->nspc; s++)
+            for (int t = 0; t < cap; t++) avg[((size_t)r * GSPC + s) * cap + t] = c->ntrials[s] / 2.0;
+    MH_HIP(hipMemcpy(g->cav_avg.p, avg.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
+    MH_HIP(hipMemcpy(g->cav_cnt.p, cnt.data(), sizeof(long long) * nt, hipMemcpyHostToDevice));
+    H.cav_on = 1;
+    H.cav_mask = c->species_mask;
+    H.cav_tab = c->tab ? g->cav_tab.as<uint8_t>() : nullptr;
+    for (int i = 0; i < 3; i++) {
+        H.cav_n[i] = c->n[i];
+        H.cav_d[i] = H.full[i] / (double)c->n[i];  // cavitylist_init: ell / n
+    }
+    for (int s = 0; s < g->nspc; s++) {
+        H.sp[s].cav_T = c->ntrials[s];
+        H.sp[s].cav_map = c->map_bias[s] ? 1 : 0;
+    }
+    H.cav_T = H.sp[0].cav_T;  // the base view is species 0's
+    H.cav_map = H.sp[0].cav_map;
+    g->ncube = nc;
+    g->cav_cap = cap;
+    g->cav_tmax = tmax;
+    if (g->threads < tmax) g->threads = tmax <= 64 ? 64 : tmax <= 128 ? 128 : tmax <= 256 ? 256 : tmax <= 512 ? 512 : 1024;
+    if ((rc = plan(g))) return rc;
+    return cav_rebuild(g);
+}
+
+int mh_gcmc_get_cavity(mh_gcmc *g, int rep, int cap, double *avg, int64_t *n, int32_t *cubes) {
+    MH_CHECK(g && rep >= 0 && rep < g->nrep && cap >= 0, MH_EINVAL, "mh_gcmc_get_cavity: bad arguments");
+    MH_CHECK(g->ncube, MH_EINVAL, "mh_gcmc_get_cavity: the batch has no cavity list");
+    MH_HIP(hipSetDevice(g->ctx->device));
+    MH_HIP(hipStreamSynchronize(g->ctx->stream));
+    const int k = std::min(cap, g->cav_cap);
+    const size_t o = ((size_t)rep * GSPC + g->sel) * g->cav_cap;
+    if (avg && k) MH_HIP(hipMemcpy(avg, g->cav_avg.as<double>() + o, sizeof(double) * k, hipMemcpyDeviceToHost));
+    if (n && k) MH_HIP(hipMemcpy(n, g->cav_cnt.as<long long>() + o, sizeof(long long) * k, hipMemcpyDeviceToHost));
+    if (cubes)
+        MH_HIP(hipMemcpy(cubes, g->cube.as<int>() + (size_t)rep * g->ncube, sizeof(int) * g->ncube,
+                         hipMemcpyDeviceToHost));
+    return MH_OK;
+}
+
+int mh_cavity_tab(const mh_map *pmap, double ucutoff, const int n[3], const int ncell[3], const double full_ell[3],
+                  uint8_t *tab) {
+    MH_CHECK(pmap && n && ncell && full_ell && tab, MH_EINVAL, "mh_cavity_tab: bad arguments");
+    int np[3];
+    int64_t nf = 0, counts[4];
+    double step[3];
+    int rc = mh_map_info(pmap, np, &nf, counts, step);
+    if (rc) return rc;
+    for (int i = 0; i < 3; i++) {
+        MH_CHECK(n[i] >= 1 && ncell[i] >= 1 && n[i] % ncell[i] == 0, MH_EINVAL,
+                 "mh_cavity_tab: cubes per axis must be a multiple of the unit cells (cavitylist.F90:233-241)");
+        MH_CHECK(n[i] <= np[i], MH_EINVAL, "mh_cavity_tab: more cubes than map nodes on an axis");
+    }
+    const size_t nodes = (size_t)np[0] * np[1] * np[2];
+    std::vector<int32_t> ptr(nodes);
+    std::vector<float> v8(8 * (size_t)nf);
+    if ((rc = mh_map_get(pmap, ptr.data(), v8.data()))) return rc;
+    std::memset(tab, 0, (size_t)n[0] * n[1] * n[2]);
+    double d[3];
+    int per[3];
+    for (int i = 0; i < 3; i++) {
+        d[i] = full_ell[i] / n[i];
+        per[i] = n[i] / ncell[i];
+    }
+    // cavitylist_initArrPmap (cavitylist.F90:790-978): the cube of each map
+    // cubelet centre whose lower-corner node is tabulated below ucutoff,
+    // in every unit cell of the simulation cell
+    for (int i = 0; i < np[0] - 1; i++)
+        for (int j = 0; j < np[1] - 1; j++)
+            for (int k = 0; k < np[2] - 1; k++) {
+                const int32_t q = ptr[((size_t)i * np[1] + j) * np[2] + k];
+                if (q == 0 || !((double)v8[q - 1] < ucutoff)) continue;
+                const double x[3] = {i * step[0] + step[0] / 2, j * step[1] + step[1] / 2, k * step[2] + step[2] / 2};
+                int ic[3];
+                for (int a = 0; a < 3; a++) {
+                    ic[a] = (int)(x[a] / d[a]);
+                    MH_CHECK(ic[a] < per[a], MH_EINVAL, "mh_cavity_tab: a map cubelet lies beyond the unit cell");
+                }
+                for (int u = 0; u < ncell[0]; u++)
+                    for (int v = 0; v < ncell[1]; v++)
+                        for (int w = 0; w < ncell[2]; w++)
+                            tab[((size_t)(ic[0] + per[0] * u) * n[1] + (ic[1] + per[1] * v)) * n[2] +
+                                (ic[2] + per[2] * w)] = 1;
+            }
+    return MH_OK;
+}
+
 int mh_gcmc_run(mh_gcmc *g, int nsteps) {
     MH_CHECK(g && nsteps >= 0, MH_EINVAL, "mh_gcmc_run: bad arguments");
     if (nsteps == 0) return MH_OK;
+    if (!g->pool)
+        if (int rc = moves_ready(g->sys[0].H, g->nspc, "mh_gcmc_run")) return rc;
     MH_HIP(hipSetDevice(g->ctx->device));
     KTimer t(g->ctx, g->prec == MH_PREC_FP32 ? "gcmc32" : "gcmc64");
     launch_any(g, nsteps);
@@ -2232,6 +2610,7 @@ int mh_gcmc_run(mh_gcmc *g, int nsteps) {
         MH_CHECK(err[i] != 2, MH_ECAPACITY, "mh_gcmc_run: a replica reached maxN molecules");
         MH_CHECK(err[i] != 4, MH_ESTATE,
                  "mh_gcmc_run: BDELETE picked a molecule whose cubelet is not in the bias map");
+        MH_CHECK(err[i] != 5, MH_ECAPACITY, "mh_gcmc_run: the total loading reached the cavity-bias table size");
         MH_CHECK(err[i] == 0, MH_ESTATE, "mh_gcmc_run: an existing molecule failed to evaluate");
     }
     g->tr_base = g->tr_base;  // trace window stays anchored at the step set_trace saw
@@ -2241,6 +2620,8 @@ int mh_gcmc_run(mh_gcmc *g, int nsteps) {
 int mh_gcmc_run_cycles(mh_gcmc *g, int nsteps, int ncycles) {
     MH_CHECK(g && nsteps >= 0 && ncycles >= 0, MH_EINVAL, "mh_gcmc_run_cycles: bad arguments");
     if (nsteps == 0 || ncycles == 0) return MH_OK;
+    if (!g->pool)
+        if (int rc = moves_ready(g->sys[0].H, g->nspc, "mh_gcmc_run_cycles")) return rc;
     MH_HIP(hipSetDevice(g->ctx->device));
     mh_gcmc::GraphKey k;
     std::memset(&k, 0, sizeof k);
@@ -2279,6 +2660,7 @@ int mh_gcmc_run_cycles(mh_gcmc *g, int nsteps, int ncycles) {
         MH_CHECK(err[i] != 2, MH_ECAPACITY, "mh_gcmc_run_cycles: a replica reached maxN molecules");
         MH_CHECK(err[i] != 4, MH_ESTATE,
                  "mh_gcmc_run_cycles: BDELETE picked a molecule whose cubelet is not in the bias map");
+        MH_CHECK(err[i] != 5, MH_ECAPACITY, "mh_gcmc_run_cycles: the total loading reached the cavity-bias table size");
         MH_CHECK(err[i] == 0, MH_ESTATE, "mh_gcmc_run_cycles: an existing molecule failed to evaluate");
     }
     return MH_OK;
@@ -2472,6 +2854,8 @@ int mh_trial_energy(mh_gcmc *g, int rep, int n, const double *com, const double 
 
 int mh_gcmc_autotune(mh_gcmc *g, int probe_steps, int *threads_out, int *exy_out) {
     MH_CHECK(g && probe_steps > 0, MH_EINVAL, "mh_gcmc_autotune: bad arguments");
+    if (!g->pool)
+        if (int rc = moves_ready(g->sys[0].H, g->nspc, "mh_gcmc_autotune")) return rc;
     MH_HIP(hipSetDevice(g->ctx->device));
     hipStream_t st = g->ctx->stream;
     // snapshot of every buffer a run modifies
@@ -2479,6 +2863,11 @@ int mh_gcmc_autotune(mh_gcmc *g, int probe_steps, int *threads_out, int *exy_out
                                 &g->sumN2, &g->nsteps, &g->att, &g->acc, &g->com, &g->ang, &g->pos, &g->ctr, &g->S,
                                 &g->delta};
     if (g->pool) bufs.push_back(&g->budget);
+    if (g->ncube) {
+        bufs.push_back(&g->cube);
+        bufs.push_back(&g->cav_avg);
+        bufs.push_back(&g->cav_cnt);
+    }
     const int nb = (int)bufs.size();
     std::vector<DBuf> snap(nb);
     int rc;
@@ -2501,6 +2890,7 @@ int mh_gcmc_autotune(mh_gcmc *g, int probe_steps, int *threads_out, int *exy_out
     int best_t = g->threads;
     bool best_exy_off = g->exy_off;
     for (int t : {64, 128, 256, 512}) {
+        if (t < g->cav_tmax) continue;  // one cavity trial per lane
         for (int xo = 0; xo < 2; xo++) {
             g->threads = t;
             g->exy_off = xo != 0;

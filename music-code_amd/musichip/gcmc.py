@@ -32,6 +32,11 @@ class GcmcParams(ctypes.Structure):
                 ("lj_comcut", _d * 4), ("coul_comcut", _d * 4)]
 
 
+class Cavity(ctypes.Structure):
+    """mh_cavity: cube grid, tabulation, listed species, trials per species."""
+    _fields_ = [("n", _i * 3), ("tab", _vp), ("species_mask", _i), ("ntrials", _i * 4), ("map_bias", _i * 4)]
+
+
 def _sig():
     L = H.lib()
     if getattr(L, "_gcmc_sig", False):
@@ -64,6 +69,9 @@ def _sig():
         "mh_pool_wait": [_vp],
         "mh_pool_status": [_vp, _i, P(_i), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "mh_pool_release": [_vp, _i],
+        "mh_gcmc_set_cavity": [_vp, _vp],
+        "mh_gcmc_get_cavity": [_vp, _i, _i, _vp, _vp, _vp],
+        "mh_cavity_tab": [_vp, _d, _vp, _vp, _vp, _vp],
     }
     for n, a in sig.items():
         f = getattr(L, n)
@@ -249,6 +257,32 @@ class GcmcBatch:
         H.check(_sig().mh_trial_energy(self.h, rep, n, _p(com), _p(ang), _p(e), _p(ok)), "mh_trial_energy")
         return ok.astype(bool), e
 
+    def set_cavity(self, n, ntrials, map_bias=None, tab=None, species_mask=1):
+        """Cavity-biased BINSERT / BDELETE (mh_gcmc_set_cavity): n cubes per
+        axis, tab (uint8 [n0, n1, n2], None = all tabulated), ntrials and
+        map_bias per species, species_mask of the species occupying cubes."""
+        c = Cavity()
+        for i in range(3):
+            c.n[i] = int(n[i])
+        nt = list(ntrials) if np.ndim(ntrials) else [int(ntrials)]
+        mb = list(map_bias) if map_bias is not None and np.ndim(map_bias) else [int(map_bias or 0)] * len(nt)
+        for s_, (t, m) in enumerate(zip(nt, mb)):
+            c.ntrials[s_], c.map_bias[s_] = int(t), int(m)
+        self.cav_tab = None if tab is None else np.ascontiguousarray(tab, dtype=np.uint8).reshape(-1)
+        c.tab = _p(self.cav_tab)
+        c.species_mask = int(species_mask)
+        self.cav_n = tuple(int(v) for v in n)
+        H.check(_sig().mh_gcmc_set_cavity(self.h, ctypes.byref(c)), "mh_gcmc_set_cavity")
+
+    def cavity(self, rep, cap):
+        """(cav_prob averages [cap], update counts [cap], cube site counts) of
+        the selected species in replica rep."""
+        avg = np.zeros(cap)
+        n = np.zeros(cap, dtype=np.int64)
+        cubes = np.zeros(int(np.prod(self.cav_n)), dtype=np.int32)
+        H.check(_sig().mh_gcmc_get_cavity(self.h, rep, cap, _p(avg), _p(n), _p(cubes)), "mh_gcmc_get_cavity")
+        return avg, n, cubes
+
     def close(self):
         if self.h:
             _sig().mh_gcmc_destroy(self.h)
@@ -362,6 +396,17 @@ class Pool(GcmcBatch):
     def release(self, ticket):
         H.check(_sig().mh_pool_release(self.h, int(ticket)), "mh_pool_release")
         self.keep.pop(ticket, None)
+
+
+def cavity_tab(pmap, ucutoff, n, ncell, full_ell):
+    """mh_cavity_tab: the cavity list's tabulation from a pmap (cubes holding
+    a map cubelet centre below ucutoff kcal/mol), uint8 [n0, n1, n2]."""
+    tab = np.zeros(tuple(int(v) for v in n), dtype=np.uint8)
+    nn = np.ascontiguousarray(n, dtype=np.int32)
+    nc = np.ascontiguousarray(ncell, dtype=np.int32)
+    fe = np.ascontiguousarray(full_ell, dtype=np.float64)
+    H.check(_sig().mh_cavity_tab(pmap.h, float(ucutoff), _p(nn), _p(nc), _p(fe), _p(tab)), "mh_cavity_tab")
+    return tab
 
 
 def adams_B(fug_kPa, volume, T, Zig=1.0):

@@ -1,0 +1,127 @@
+"""Cavity-biased insertion / deletion on the device (mh_gcmc_set_cavity,
+CAVITY_BIAS / MAP_CAVITY_BIAS in mh_gcmc): the tabulation from a pmap
+(mh_cavity_tab) against a restatement of cavitylist_initArrPmap and its
+unit-cell periodicity, and FP64 trajectories move for move against oracle
+chains, with the cav_prob tables and the cube counts of every replica equal
+to the oracle's."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from musichip import hip as H
+from musichip import gcmc as G
+from musichip import synth
+import hmc_systems as HS
+from test_gpu_bias import _framework_maps, _oracle_map, _oracle_spec, _co2_spec
+
+pytestmark = pytest.mark.gpu
+
+
+def _tab_restated(m, ucutoff, n, ncell, full):
+    """cavitylist_initArrPmap: cube of each cubelet centre whose lower-corner
+    node is tabulated below ucutoff, in every unit cell."""
+    npn, step, ptr, vals = _oracle_map(m)
+    ptr = np.asarray(ptr).reshape(npn)
+    n = np.asarray(n)
+    per = n // np.asarray(ncell)
+    d = np.asarray(full) / n
+    tab = np.zeros(n, dtype=np.uint8)
+    for i in range(npn[0] - 1):
+        for j in range(npn[1] - 1):
+            for k in range(npn[2] - 1):
+                q = ptr[i, j, k]
+                if q == 0 or not float(vals[0][q - 1]) < ucutoff:
+                    continue
+                x = np.array([i, j, k]) * np.asarray(step) + np.asarray(step) / 2
+                ic = (x / d).astype(int)
+                for u in range(ncell[0]):
+                    for v in range(ncell[1]):
+                        for w in range(ncell[2]):
+                            tab[ic[0] + per[0] * u, ic[1] + per[1] * v, ic[2] + per[2] * w] = 1
+    return tab
+
+
+def test_cavity_tab_from_pmap(hip):
+    print("case: cavity tabulation from a pmap", flush=True)
+    m = _framework_maps(hip, {"P": (148.0, 3.73)})["P"]
+    L = np.asarray(HS.L16)
+    for ucut, n, ncell in ((0.0, (10, 10, 10), (1, 1, 1)), (-1.0, (16, 8, 12), (2, 1, 3))):
+        full = L * np.asarray(ncell)
+        tab = G.cavity_tab(m, ucut, n, ncell, full)
+        ref = _tab_restated(m, ucut, n, ncell, full)
+        assert np.array_equal(tab, ref)
+        assert 0 < tab.sum() < tab.size
+        per = np.asarray(n) // np.asarray(ncell)
+        if ncell[0] == 2:
+            assert np.array_equal(tab[: per[0]], tab[per[0]:])
+        if ncell[2] == 3:
+            assert np.array_equal(tab[:, :, : per[2]], tab[:, :, per[2]: 2 * per[2]])
+    m.close()
+
+
+def _oracle_cav(n, tab, mask=1):
+    return dict(n=n, tab=None if tab is None else tab.astype(np.int8), mask=mask)
+
+
+@pytest.mark.parametrize("system", ["probe", "co2"])
+def test_cavity_gcmc_fp64_matches_oracle(hip, system):
+    """CAVITY_BIAS (1-site probe, tabulation from the pmap, 20 trials) and
+    MAP_CAVITY_BIAS (3-site, Ewald, translate + rotate, uniform tabulation,
+    100 trials over two waves) replica by replica against oracle chains."""
+    print("case: cavity-biased GCMC FP64 vs oracle, %s" % system, flush=True)
+    n = (10, 10, 10)
+    if system == "probe":
+        maps = _framework_maps(hip, {"P": (148.0, 3.73)})
+        from musichip.inputs import lj_AB
+        A, B = lj_AB(3.73, 148.0)
+        spec = dict(full_ell=list(HS.L16), fund_ell=list(HS.L16), ref=[[0.0, 0.0, 0.0]], stype=[0], sq=[0.0], nst=1,
+                    lj=([A], [B], [HS.HICUT ** 2], [0.25], [1]), T=300.0,
+                    moves=[("BINSERT", 1.0), ("BDELETE", 1.0), ("RTRANSLATE", 1.0)], dtrans=0.25)
+        site_maps = [maps["P"]]
+        fug = [200.0, 2000.0, 20000.0, 2000.0]
+        tab = G.cavity_tab(maps["P"], 0.0, n, (1, 1, 1), HS.L16)
+        T, cmap, dbias, obias = 20, 0, None, None
+    else:
+        sorb = synth.co2_like()
+        maps = _framework_maps(hip, {t: sorb["lj"][t] for t in ("O", "C")})
+        spec = _co2_spec(None)
+        site_maps = [maps[t] for t in sorb["types"]]
+        fug = [100.0, 1000.0, 5000.0, 1000.0]
+        tab = None
+        T, cmap = 100, 1
+        bm = maps["C"]
+        dbias = H.BiasMap(bm, 300.0)
+        npn, step, ptr, vals = _oracle_map(bm)
+        obias = O.Bmap(npn, step, HS.L16, ptr, vals, 300.0)
+        spec["bias_map"] = dbias
+    seeds = [81, 82, 83, 84]
+    b = G.GcmcBatch(hip, spec, fug, seeds, maxN=256, maps={a: (m, None) for a, m in enumerate(site_maps)})
+    b.set_cavity(n, [T], [cmap], tab=tab, species_mask=1)
+    ospec = dict(_oracle_spec(spec, site_maps, obias), cavity=_oracle_cav(n, tab), cav_ntrials=T, cav_map=cmap)
+    chains = [O.Gcmc(ospec, f, s_, maxN=256) for f, s_ in zip(fug, seeds)]
+    for step_to in (1500, 4000, 8000):
+        b.run(step_to - int(b.state()["nsteps"][0]))
+        st = b.state()
+        for r, g in enumerate(chains):
+            g.run(step_to - g.st.nsteps)
+            assert list(st["att"][r]) == [g.st.att[i] for i in range(4)], (r, step_to)
+            assert list(st["acc"][r]) == [g.st.acc[i] for i in range(4)], (r, step_to)
+            assert st["N"][r] == g.st.N
+            assert st["energy"][r] == pytest.approx(g.st.energy, rel=1e-10, abs=1e-10)
+            # the device table holds loadings up to nspc maxN + 1
+            cap = 258
+            avg, cnt, cubes = b.cavity(r, cap)
+            assert np.array_equal(cnt, g.cav_n[:cap]), (r, step_to)
+            np.testing.assert_allclose(avg, g.cav_avg[:cap], rtol=1e-15, atol=0)
+            assert np.array_equal(cubes, g.cube_cnt), (r, step_to)
+    assert st["acc"][:, 0].sum() > 50 and st["acc"][:, 1].sum() > 20
+    # a configuration set from the host (half of replica 2's molecules)
+    # rebuilds the cube counts
+    c = b.config(2)
+    k = len(c[0]) // 2
+    b.set_config(0, c[0][:k], c[1][:k])
+    chains[0].set_config(c[0][:k], c[1][:k])
+    assert np.array_equal(b.cavity(0, 4)[2], chains[0].cube_cnt)
+    b.close()
+    if dbias is not None:
+        dbias.close()
