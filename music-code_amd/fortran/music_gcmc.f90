@@ -82,6 +82,12 @@ program music_gcmc
   character(len=LL) :: pmapfs(MH_MAX_SITES, MH_MAX_SPECIES), emapfs(MH_MAX_SITES, MH_MAX_SPECIES)
   character(len=LL) :: biasfs(MH_MAX_SPECIES), srcfiles(MH_MAX_SPECIES)
   real(c_double) :: bias_Ts(MH_MAX_SPECIES)
+  ! cavity bias: trials and map bias per species, the cavity list
+  integer :: cav_ntr(MH_MAX_SPECIES) = 0, cav_mb(MH_MAX_SPECIES) = 0
+  type(mh_cavity) :: cav
+  integer(c_int8_t), allocatable, target :: cavtab(:)
+  character(len=LL) :: cavpmap
+  real(c_double) :: cavucut
 
   call get_command_argument(1, ctrl)
   if (len_trim(ctrl) == 0) then
@@ -192,15 +198,32 @@ program music_gcmc
         call read_delta(i, P%delta_rot, P%adjust_rot)
         i = i + 1
      case ('BINSERT')
-        ! bias pmap file (MAP_BIAS), then the bias temperature; cavity bias
-        ! ('CAVITY', 'file, CAVITY') is not offered
+        ! rigidmoves_initInsBias (rigidmoves.F90:620-670): a bias pmap file
+        ! and the bias temperature (MAP_BIAS); 'CAVITY' and the number of
+        ! trials (CAVITY_BIAS); 'file, CAVITY', the temperature and the
+        ! number of trials (MAP_CAVITY_BIAS)
         P%move_type(k) = MH_MV_BINSERT
         line = strip(sy%t%line(i))
-        if (index(line, ',') > 0 .or. trim(upper(word(line, 1))) == 'CAVITY') &
-             call fail('cavity-biased insertion is not offered (MAP_BIAS only)')
-        biasf = word(line, 1)
-        read(word(strip(sy%t%line(i + 1)), 1), *) bias_T
-        i = i + 2
+        if (index(line, ',') == 0 .and. trim(upper(word(line, 1))) == 'CAVITY') then
+           read(word(strip(sy%t%line(i + 1)), 1), *) cav_ntr(cur)
+           cav_mb(cur) = 0
+           i = i + 2
+        else if (index(line, ',') > 0) then
+           if (trim(upper(word(line, 2))) /= 'CAVITY' .or. nwords(line) /= 2) &
+                call fail('BINSERT: put the bias map and CAVITY in that order (rigidmoves.F90:649-666)')
+           biasf = word(line, 1)
+           read(word(strip(sy%t%line(i + 1)), 1), *) bias_T
+           read(word(strip(sy%t%line(i + 2)), 1), *) cav_ntr(cur)
+           cav_mb(cur) = 1
+           i = i + 3
+        else
+           biasf = word(line, 1)
+           read(word(strip(sy%t%line(i + 1)), 1), *) bias_T
+           i = i + 2
+        end if
+        if (index(line, ',') > 0 .or. trim(upper(word(line, 1))) == 'CAVITY') then
+           if (cav_ntr(cur) < 1 .or. cav_ntr(cur) > 1024) call fail('BINSERT: 1 to 1024 cavity trials are offered')
+        end if
      case ('BDELETE')
         P%move_type(k) = MH_MV_BDELETE
      case default
@@ -264,6 +287,7 @@ program music_gcmc
      call save_species(cur)
   end do
   call use_species(1)
+  call read_cavity_section()
 
   write(*, '(a)') 'Welcome to MuSiC (GCMC implementation, libmusic_hip)'
   write(*, '(2a)') 'Reading from control file ', trim(ctrl)
@@ -285,8 +309,11 @@ program music_gcmc
           P%site_type(a), ' q ', P%site_q(a), ' pmap ', trim(basename(pmapf(a))), ' emap ', trim(basename(emapf(a)))
   end do
   if (len_trim(biasf) > 0) write(*, '(a, a, a, f10.3)') '  bias: ', trim(biasf), ' T ', bias_T
+  if (cav_ntr(cur) > 0) write(*, '(a, i0, a, i0)') '  cavity bias: trials ', cav_ntr(cur), ' map ', cav_mb(cur)
   if (len_trim(srcfile) > 0) write(*, '(2a)') '  start configuration: ', trim(srcfile)
   end do
+  if (cav%species_mask /= 0) write(*, '(a, 3(1x, i0), a, i0, 3a, es12.4)') '  cavity list: cubes', cav%n, &
+       ', species mask ', cav%species_mask, ', tabulation ', trim(cavpmap), ' below ', cavucut
   call use_species(1)
   do k = 1, nst * nst
      if (P%lj_on(k) /= 0) write(*, '(a, i0, a, 4es24.16)') '  lj ', k - 1, ':', P%lj_A(k), P%lj_B(k), &
@@ -345,6 +372,12 @@ program music_gcmc
      call save_species(cur)
   end do
   call use_species(1)
+  if (cav%species_mask /= 0 .and. trim(upper(cavpmap)) /= 'NULL') then
+     allocate(cavtab(cav%n(1) * cav%n(2) * cav%n(3)))
+     call check(mh_cavity_tab(load_map(cavpmap), cavucut, cav%n, sy%nxyz, sy%full, cavtab), &
+          'mh_cavity_tab ' // trim(cavpmap))
+     cav%tab = c_loc(cavtab)
+  end if
   np = 1
   call system_clock(t0, rate)
   if (chain) then
@@ -395,6 +428,7 @@ contains
     if (chain) seeds(1) = int(iseed, c_int32_t)
     call check(mh_gcmc_create_mix(ctx, PS, int(nspc, c_int), int(nr, c_int), fug, seeds, int(maxn, c_int), gnew), &
          'mh_gcmc_create')
+    if (cav%species_mask /= 0) call check(mh_gcmc_set_cavity(gnew, cav), 'mh_gcmc_set_cavity')
     if (.not. carry) then
        do q = 1, nspc
           call use_species(q)
@@ -1284,6 +1318,47 @@ contains
   ! the scalars of the species being read or reported (spname, isp, nsite,
   ! move names, pressures, model, map and bias files, start configuration)
   ! from / into the per-species arrays
+  ! the cavity list of cavity-biased insertion: cavitylist_init /
+  ! cavitylist_readCtrlFile (cavitylist.F90:155-275, 1022-1082): the number
+  ! of sorbates, their names, cubes per axis, the pmap of the tabulation
+  ! (NULL: every cube) and, for a pmap, the energy cutoff
+  subroutine read_cavity_section()
+    integer :: ii, ncs, k2, q, hit
+    character(len=NL) :: nm
+    if (all(cav_ntr(1:nspc) == 0)) return
+    if (sy%slanted) call fail('cavity bias needs an orthorhombic simulation cell')
+    ii = find_section(sy%t, 'Cavity Bias Info')
+    if (ii < 0) call fail('cavity bias needs the "Cavity Bias Info" section (cavitylist.F90:155-275)')
+    read(word(strip(sy%t%line(ii)), 1), *) ncs
+    if (ncs < 1) call fail('Cavity Bias Info: no sorbate listed')
+    cav%species_mask = 0
+    do k2 = 1, ncs
+       nm = word(strip(sy%t%line(ii + k2)), 1)
+       hit = 0
+       do q = 1, nspc
+          if (trim(spnames(q)) == trim(nm)) hit = q
+       end do
+       if (hit == 0) call fail('Cavity Bias Info: ' // trim(nm) // ' is not a species of the simulation')
+       if (btest(cav%species_mask, hit - 1)) call fail('Cavity Bias Info: ' // trim(nm) // ' listed twice')
+       cav%species_mask = ibset(cav%species_mask, hit - 1)
+    end do
+    read(strip(sy%t%line(ii + ncs + 1)), *) cav%n
+    do q = 1, 3
+       if (cav%n(q) < 1 .or. cav%n(q) > 4096) call fail('Cavity Bias Info: 1 to 4096 cubes per axis')
+       if (mod(cav%n(q), sy%nxyz(q)) /= 0) &
+            call fail('Cavity Bias Info: cubes per axis must be a multiple of the unit cells (cavitylist.F90:233-241)')
+    end do
+    cavpmap = word(strip(sy%t%line(ii + ncs + 2)), 1)
+    cavucut = 0.0d0
+    if (trim(upper(cavpmap)) /= 'NULL') read(word(strip(sy%t%line(ii + ncs + 3)), 1), *) cavucut
+    do q = 1, nspc
+       cav%ntrials(q) = int(cav_ntr(q), c_int)
+       cav%map_bias(q) = int(cav_mb(q), c_int)
+       if (cav_ntr(q) > 0 .and. .not. btest(cav%species_mask, q - 1)) &
+            call fail('Cavity Bias Info: a cavity-biased species must be listed (cavitylist.F90:363-368)')
+    end do
+  end subroutine read_cavity_section
+
   subroutine use_species(q)
     integer, intent(in) :: q
     spname = spnames(q)

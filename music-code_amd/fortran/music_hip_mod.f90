@@ -55,6 +55,17 @@ module music_hip_mod
      real(c_double) :: lj_comcut(MH_MAX_SPECIES) = 0.0d0, coul_comcut(MH_MAX_SPECIES) = 0.0d0
   end type mh_gcmc_params
 
+  ! cavity-biased insertion / deletion (mh_gcmc_set_cavity): cube grid,
+  ! tabulation (uint8 [n1][n2][n3], z fastest, c_null_ptr = every cube),
+  ! species occupying cubes (bit s), trials and map bias per species
+  type, bind(C), public :: mh_cavity
+     integer(c_int) :: n(3) = 1
+     type(c_ptr) :: tab = c_null_ptr
+     integer(c_int) :: species_mask = 0
+     integer(c_int) :: ntrials(MH_MAX_SPECIES) = 0
+     integer(c_int) :: map_bias(MH_MAX_SPECIES) = 0
+  end type mh_cavity
+
   public :: mh_error_message
   public :: mh_ctx_create, mh_ctx_destroy, mh_framework_upload, mh_framework_set_cell, mh_ff_set_lj, mh_ff_set_buck
   public :: mh_ewald_set, mh_map_build, mh_map_destroy, mh_map_info, mh_map_get, mh_map_write
@@ -65,6 +76,7 @@ module music_hip_mod
             mh_gcmc_get_moments, mh_gcmc_get_deltas, mh_gcmc_set_deltas, mh_gcmc_set_counts
   public :: mh_gcmc_get_config, mh_gcmc_total_energy, mh_trial_energy
   public :: mh_bmap_create, mh_bmap_destroy, mh_bmap_info
+  public :: mh_gcmc_set_cavity, mh_cavity_tab
 
   interface
      function c_mh_last_error() bind(C, name="mh_last_error") result(p)
@@ -337,6 +349,23 @@ module music_hip_mod
        integer(c_int), value :: refresh
        integer(c_int) :: rc
      end function mh_gcmc_total_energy
+
+     function mh_gcmc_set_cavity(g, c) bind(C, name="mh_gcmc_set_cavity") result(rc)
+       import :: c_int, c_ptr, mh_cavity
+       type(c_ptr), value :: g
+       type(mh_cavity), intent(in) :: c
+       integer(c_int) :: rc
+     end function mh_gcmc_set_cavity
+
+     function mh_cavity_tab(pmap, ucutoff, n, ncell, full_ell, tab) bind(C, name="mh_cavity_tab") result(rc)
+       import :: c_int, c_ptr, c_double, c_int8_t
+       type(c_ptr), value :: pmap
+       real(c_double), value :: ucutoff
+       integer(c_int), intent(in) :: n(3), ncell(3)
+       real(c_double), intent(in) :: full_ell(3)
+       integer(c_int8_t), intent(out) :: tab(*)
+       integer(c_int) :: rc
+     end function mh_cavity_tab
 
      function mh_bmap_create(ctx, pmap, bias_temperature, b) bind(C, name="mh_bmap_create") result(rc)
        import :: c_int, c_ptr, c_double

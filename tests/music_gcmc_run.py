@@ -95,6 +95,13 @@ def parse_check(text):
                 d["species"][-1]["delta"] = d["delta"]
         elif line.startswith("  bias:"):
             d["bias"] = (s[1], float(s[3]))
+        elif line.startswith("  cavity bias:"):
+            d["cavity"] = (int(s[3]), int(s[5]))
+            if d["species"]:
+                d["species"][-1]["cavity"] = d["cavity"]
+        elif line.startswith("  cavity list:"):
+            d["cavity_list"] = dict(n=[int(v.rstrip(",")) for v in s[3:6]], mask=int(s[8].rstrip(",")), tab=s[10],
+                                    ucut=float(s[12]))
         elif line.startswith("  ewald:"):
             d["ewald"] = dict(kappa=float(s[2]), kmax=int(s[4]), rc=float(s[6]))
         elif line.startswith("  iterations"):

@@ -71,6 +71,50 @@ def test_biased_moves_parameters():
     _same_spec(d, spec, 3)
 
 
+def _staged(tmp_path, text):
+    """An edited control file beside copies of the fixture inputs."""
+    shutil.copytree(FIX, tmp_path / "fx", copy_function=shutil.copyfile)
+    shutil.copytree(os.path.join(GOLD, "potentials"), tmp_path / "potentials", copy_function=shutil.copyfile)
+    f = tmp_path / "fx" / "edited.ctr"
+    f.write_text(text)
+    return f
+
+
+def test_cavity_bias_parameters(tmp_path):
+    """BINSERT 'bmap, CAVITY' (MAP_CAVITY_BIAS: bias file, temperature,
+    trials) and 'CAVITY' (CAVITY_BIAS: trials), and the Cavity Bias Info
+    section (rigidmoves_initInsBias, rigidmoves.F90:620-670;
+    cavitylist_readCtrlFile, cavitylist.F90:1022-1082)."""
+    d = _check(os.path.join(FIX, "gcmc_cavity.ctr"))
+    assert [m[0] for m in d["moves"]] == ["BINSERT", "BDELETE"]
+    assert d["bias"] == ("Lattice.LJ.map", 300.0)
+    assert d["cavity"] == (20, 1)
+    assert d["cavity_list"] == dict(n=[10, 10, 12], mask=1, tab="Lattice.LJ.map", ucut=0.0)
+    txt = open(os.path.join(FIX, "gcmc_cavity.ctr")).read()
+    plain = txt.replace("Lattice.LJ.map, CAVITY  # Bias Potential File, cavity bias\n300                # Bias temperature\n",
+                        "CAVITY             # cavity bias, no bias map\n")
+    plain = plain.replace("Lattice.LJ.map       # pmap of the tabulated cubes", "NULL                 # every cube")
+    plain = plain.replace("0.0                  # energy cutoff (kcal/mol) of the tabulation\n", "")
+    f = _staged(tmp_path, plain)
+    d = _check(str(f))
+    assert d["bias"] is None and d["cavity"] == (20, 0)
+    assert d["cavity_list"]["tab"] == "NULL"
+
+
+@pytest.mark.parametrize("edit, msg", [
+    (("Lattice.LJ.map, CAVITY", "CAVITY, Lattice.LJ.map"), "in that order"),
+    (("10, 10, 12           # cubes", "10, 10, 11           # cubes"), "multiple of the unit cells"),
+    (("Probe                # sorbate\n10, 10", "Lattice              # sorbate\n10, 10"), "not a species"),
+    (("------ Cavity Bias Info", "------ Cavity Bias Gone"), "Cavity Bias Info"),
+])
+def test_cavity_bias_errors(tmp_path, edit, msg):
+    txt = open(os.path.join(FIX, "gcmc_cavity.ctr")).read()
+    assert edit[0] in txt
+    f = _staged(tmp_path, txt.replace(edit[0], edit[1], 1))
+    r = subprocess.run([GCMC, str(f), "check"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and msg in (r.stdout + r.stderr), r.stdout + r.stderr
+
+
 def test_bent_sorbate_parameters():
     """Principal axes (tred2 / tql2 order in the driver and in the Python
     host, pinned to the reference in test_oracle_ref_moves.py), the LBMIX

@@ -18,9 +18,12 @@ pytestmark = pytest.mark.gpu
 
 
 def _tab_restated(m, ucutoff, n, ncell, full):
+    return tab_from_arrays(*_oracle_map(m), ucutoff, n, ncell, full)
+
+
+def tab_from_arrays(npn, step, ptr, vals, ucutoff, n, ncell, full):
     """cavitylist_initArrPmap: cube of each cubelet centre whose lower-corner
     node is tabulated below ucutoff, in every unit cell."""
-    npn, step, ptr, vals = _oracle_map(m)
     ptr = np.asarray(ptr).reshape(npn)
     n = np.asarray(n)
     per = n // np.asarray(ncell)

@@ -264,6 +264,31 @@ def test_music_gcmc_biased_moves(hip):
         assert sum(b["moves"]["INSERT"][0] for b in blocks if b["it"] == 3000) > 20
 
 
+def test_music_gcmc_cavity_bias(hip):
+    """gcmc_cavity.ctr: MAP_CAVITY_BIAS (bias map Lattice.LJ.map at 300 K, 20
+    trials, a 10 x 10 x 12 cube list tabulated from the map below 0
+    kcal/mol) through the driver, three state points, against oracle
+    chains with the oracle's bias map and cavity list, move for move."""
+    print("case: music_gcmc gcmc-maps/gcmc_cavity.ctr FP64", flush=True)
+    from test_gpu_cavity import tab_from_arrays
+    spec = dict(MG.probe_spec(), moves=[("BINSERT", 1.0), ("BDELETE", 1.0)])
+    press = [10.0, 100.0, 1000.0]
+    with tempfile.TemporaryDirectory() as td:
+        mlist = _fixture_maps(hip, td)
+        npn, step, ptr, vals = mlist[0]
+        bias = O.Bmap(npn, step, [20.0, 20.0, 12.0], ptr, vals, 300.0)
+        tab = tab_from_arrays(npn, step, ptr, vals, 0.0, (10, 10, 12), (1, 1, 2), [20.0, 20.0, 24.0])
+        assert 0 < tab.sum() < tab.size
+        cav = dict(n=(10, 10, 12), tab=tab.astype(np.int8), mask=1)
+        spec = dict(spec, maps=mlist, site_pmap=[0], site_emap=[1], bias=bias, cavity=cav, cav_ntrials=20, cav_map=1)
+        blocks, ends = MG.parse(MG.run([os.path.join(MG.FIX, "gcmc_cavity.ctr")], td))
+        for p in range(3):
+            g = O.Gcmc(spec, press[p], 1892134 + p)
+            _compare([b for b in blocks if b["sim"] == p + 1], _oracle_blocks(g, 3000))
+            assert abs(ends[p + 1]["stored"] - g.st.energy * 4.184) < 1.5e-3
+        assert sum(b["moves"]["INSERT"][0] for b in blocks if b["it"] == 3000) > 20
+
+
 def _t007_tree(td, **edits):
     """test007's inputs in td (ctrlfile edited line by line: key -> (old, new))."""
     import shutil
