@@ -128,3 +128,44 @@ def test_cavity_gcmc_fp64_matches_oracle(hip, system):
     b.close()
     if dbias is not None:
         dbias.close()
+
+
+def test_cavity_mixture_fp64_matches_oracle(hip):
+    """A CO2-like sorbate with CAVITY_BIAS (40 trials, no bias map) beside
+    methane with random moves, both in the cavity list: FP64 move sequences,
+    energies, cav_prob tables and cube counts against the oracle's mixture."""
+    print("case: cavity-biased CO2 + CH4 mixture, FP64 trajectory parity", flush=True)
+    from test_gpu_mix import _mixture, _oracle
+    specs, maps, keep = _mixture(hip, ewald=True)
+    specs[0] = dict(specs[0], moves=[("BINSERT", 1.0), ("BDELETE", 1.0), ("RTRANSLATE", 1.0), ("RROTATE", 1.0)])
+    n = (10, 10, 10)
+    ospecs = [dict(specs[0], cavity=_oracle_cav(n, None, mask=3), cav_ntrials=40, cav_map=0),
+              dict(specs[1], cavity=_oracle_cav(n, None, mask=3))]
+    fugs = [[3000.0, 3.0e4], [1.0e4, 1.0e5]]
+    seeds = [4301, 4302]
+    nsteps = 3000
+    b = G.GcmcMix(hip, specs, fugs, seeds, maxN=256, maps=maps)
+    b.set_cavity(n, [40, 0], [0, 0], tab=None, species_mask=3)
+    b.set_trace(nsteps)
+    b.run(nsteps)
+    cap = 2 * 256 + 2
+    for r in range(2):
+        g = _oracle(ospecs, maps, fugs[r], seeds[r], nsteps)
+        to = g.trace_array(nsteps)
+        tg = b.trace(r)
+        assert np.array_equal(tg[:, :3].astype(int), to[:, :3].astype(int)), r
+        scale = np.maximum(np.abs(to[:, 3]), 1.0)
+        assert np.max(np.abs(tg[:, 3] - to[:, 3]) / scale) < 1e-10
+        for s in range(2):
+            b.select(s)
+            st = b.state()
+            assert st["N"][r] == g.N()[s]
+            assert list(st["att"][r]) == list(g.ch[s].st.att) and list(st["acc"][r]) == list(g.ch[s].st.acc)
+        b.select(0)
+        avg, cnt, cubes = b.cavity(r, cap)
+        assert np.array_equal(cnt, g.ch[0].cav_n[:cap])
+        np.testing.assert_allclose(avg, g.ch[0].cav_avg[:cap], rtol=1e-15, atol=0)
+        assert np.array_equal(cubes, g.ch[0].cube_cnt)
+        assert cubes.sum() == 3 * g.N()[0] + g.N()[1]
+    assert b.state()["acc"][:, 0].sum() > 20
+    b.close()
