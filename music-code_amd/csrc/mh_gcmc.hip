@@ -974,7 +974,9 @@ __device__ __forceinline__ void cav_sites(const GDev &P, const Rep &R, int rep, 
         atomicAdd(R.cube + (long long)rep * R.cube_stride + cav_cube(P, r + 3 * tid), sign);
 }
 
-template <int NT, int PREC, int KPLX, bool LDSMOL>
+// CAV: the instantiation with cavity-biased moves (S(k) and molecules in
+// HBM); the others carry none of its code or registers
+template <int NT, int PREC, int KPLX, bool LDSMOL, bool CAV = false>
 __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep R, int nsteps) {
     constexpr int KPL = KPLX > 0 ? KPLX : 0;
     constexpr bool SL = KPLX < 0;
@@ -1105,7 +1107,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
         int mol = -1;
         // ---- trial generation (identical in every lane)
         double tcom[3], tang[3] = {0.0, 0.0, 0.0};
-        if (mbase == MH_MV_INSERT && biased && P.cav_T > 0) {
+        if (CAV && mbase == MH_MV_INSERT && biased && P.cav_T > 0) {
             att[0]++;
             if (N >= P.maxN) {
                 err = 2;  // capacity
@@ -1171,7 +1173,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                     }
                     biasf = __dmul_rn(Bm.ncubelets, Bm.wn[t - 1]);
                 }
-                if (mbase == MH_MV_DELETE && biased && P.cav_T > 0) {
+                if (CAV && mbase == MH_MV_DELETE && biased && P.cav_T > 0) {
                     // rigidmoves_bdelete (rigidmoves.F90:1632-1643): divided
                     // by cav_prob(totmol) / cav_T at the current loading; the
                     // table is lane 0's
@@ -1271,7 +1273,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                         }
                         if (tid < n3) posS[N * ms3 + tid] = sh.posA[tid];
                         if (P.comcut && tid < 3) ctrS[3 * N + tid] = sh.ctr[0][tid];
-                        cav_sites(P, R, rep, s, sh.posA, 1);
+                        if constexpr (CAV) cav_sites(P, R, rep, s, sh.posA, 1);
                         if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                         N++;
                         energy += U;
@@ -1302,7 +1304,7 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                     }
                     if (tid < n3) posS[mol * ms3 + tid] = posS[last * ms3 + tid];
                     if (P.comcut && tid < 3) ctrS[3 * mol + tid] = ctrS[3 * last + tid];
-                    cav_sites(P, R, rep, s, sh.posA, -1);
+                    if constexpr (CAV) cav_sites(P, R, rep, s, sh.posA, -1);
                     if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                     N--;
                     energy -= U;
@@ -1327,8 +1329,10 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
                         }
                         if (tid < n3) posS[mol * ms3 + tid] = sh.posA[tid];
                         if (P.comcut && tid < 3) ctrS[3 * mol + tid] = sh.ctr[0][tid];
-                        cav_sites(P, R, rep, s, sh.posB, -1);
-                        cav_sites(P, R, rep, s, sh.posA, 1);
+                        if constexpr (CAV) {
+                            cav_sites(P, R, rep, s, sh.posB, -1);
+                            cav_sites(P, R, rep, s, sh.posA, 1);
+                        }
                         if (P.ewald) sk_accept<NT, KPL>(P, K, S, dS);
                         energy += dU;
                         acc[mtype]++;
@@ -1344,7 +1348,8 @@ __global__ void __launch_bounds__(NT) k_gcmc(const GDev *__restrict__ Ptab, Rep 
         }
         // cube counts changed by atomics at the device level: complete before
         // the next step's trials read them
-        if (P.cav_on && accepted) __threadfence();
+        if constexpr (CAV)
+            if (P.cav_on && accepted) __threadfence();
         if (tid == 0 && R.tr_cap > 0) {
             long long t = nst - R.tr_base;
             if (t >= 0 && t < R.tr_cap) {
@@ -1746,6 +1751,11 @@ static void launch_lds(mh_gcmc *g, const GDev *P, const Rep &r, int nsteps) {
 
 template <int PREC, int NT>
 static void launch_nt(mh_gcmc *g, const GDev *P, const Rep &r, int nsteps) {
+    if (g->ncube) {  // cavity bias: its own instantiation (plan() keeps S(k) and molecules in HBM)
+        hipLaunchKernelGGL((k_gcmc<NT, PREC, 0, false, true>), dim3(g->nrep), dim3(NT), 0, g->ctx->stream, P, r,
+                           nsteps);
+        return;
+    }
     if (g->slanted) {  // slanted cell: its own instantiation (no k vectors)
         if (g->ldsmol) launch_lds<PREC, NT, -1>(g, P, r, nsteps);
         else hipLaunchKernelGGL((k_gcmc<NT, PREC, -1, false>), dim3(g->nrep), dim3(NT), 0, g->ctx->stream, P, r, nsteps);
@@ -1816,7 +1826,7 @@ static int plan(mh_gcmc *g) {
     g->ns = std::max(g->ns, ns);
     g->nk = nk;
     const long long need = (long long)g->nspc * g->maxN * (3 * g->ns + 6) * (long long)sizeof(double);
-    g->ldsmol = need <= budget;
+    g->ldsmol = !g->ncube && need <= budget;
     g->lds_bytes = g->ldsmol ? (size_t)need : 0;
     g->kpl = g->ldsmol ? pick_kpl(nk, g->threads) : 0;
     const long long off = (g->lds_bytes + 255) / 256 * 256, ent = (g->prec == MH_PREC_FP64) ? 16 : 8;
