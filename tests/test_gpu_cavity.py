@@ -169,3 +169,48 @@ def test_cavity_mixture_fp64_matches_oracle(hip):
         assert cubes.sum() == 3 * g.N()[0] + g.N()[1]
     assert b.state()["acc"][:, 0].sum() > 20
     b.close()
+
+
+def test_cavity_graph_replay_and_fp32(hip):
+    """Cycles replayed from one captured hipGraph equal eager launches for a
+    cavity-biased batch (cube counts, cav_prob tables, counters); the FP32
+    kernel's mean loading agrees with FP64's."""
+    print("case: cavity-biased GCMC, graph replay and FP32", flush=True)
+    maps = _framework_maps(hip, {"P": (148.0, 3.73)})
+    from musichip.inputs import lj_AB
+    A, B = lj_AB(3.73, 148.0)
+    spec = dict(full_ell=list(HS.L16), fund_ell=list(HS.L16), ref=[[0.0, 0.0, 0.0]], stype=[0], sq=[0.0], nst=1,
+                lj=([A], [B], [HS.HICUT ** 2], [0.25], [1]), T=300.0,
+                moves=[("BINSERT", 1.0), ("BDELETE", 1.0), ("RTRANSLATE", 1.0)], dtrans=0.25)
+    n = (10, 10, 10)
+    fug = [3000.0] * 32
+    seeds = list(range(900, 932))
+    mp = {0: (maps["P"], None)}
+    eager = G.GcmcBatch(hip, spec, fug, seeds, maxN=256, maps=mp)
+    graph = G.GcmcBatch(hip, spec, fug, seeds, maxN=256, maps=mp)
+    for b in (eager, graph):
+        b.set_cavity(n, [30], [0], tab=None, species_mask=1)
+    eager.run(2000)
+    graph.run_cycles(500, 4)
+    se, sg = eager.state(), graph.state()
+    for k in ("N", "att", "acc", "energy", "nsteps"):
+        assert np.array_equal(se[k], sg[k]), k
+    for r in range(0, 32, 7):
+        for a, b in zip(eager.cavity(r, 258), graph.cavity(r, 258)):
+            assert np.array_equal(a, b)
+    means = {}
+    for prec in (H.MH_PREC_FP64, H.MH_PREC_FP32):
+        b = G.GcmcBatch(hip, spec, fug, seeds, maxN=256, maps=mp, precision=prec)
+        b.set_cavity(n, [30], [0], tab=None, species_mask=1)
+        b.run(5000)
+        s0 = b.state()
+        b.run(40000)
+        s1 = b.state()
+        m = (s1["sumN"] - s0["sumN"]) / (s1["nsteps"] - s0["nsteps"])
+        means[prec] = (m.mean(), m.std(ddof=1) / np.sqrt(len(m)))
+        b.close()
+    (m64, e64), (m32, e32) = means[H.MH_PREC_FP64], means[H.MH_PREC_FP32]
+    assert m64 > 1.0
+    assert abs(m64 - m32) < 4 * np.hypot(e64, e32) + 0.01 * m64, means
+    eager.close()
+    graph.close()
