@@ -134,6 +134,32 @@ def c1_leg(H, steps):
     out["mean_loading"] = [float(x) for x in (s1["sumN"] - s0["sumN"]) / steps]
     cfgs = [b.config(r) for r in range(8)]
     b.close()
+    # the same system with cavity-biased insertion / deletion (CAVITY_BIAS,
+    # 20 trials per insertion): 1 A cubes tabulated from the methane map
+    # below 0 kcal/mol, in every unit cell of the simulation cell
+    full, fund = ms["spec"]["full_ell"], ms["spec"]["fund_ell"]
+    ncell = tuple(int(round(a / b_)) for a, b_ in zip(full, fund))
+    ncub = tuple(int(round(a)) // c * c for a, c in zip(full, ncell))
+    tab = G.cavity_tab(ms["map_objs"][0], 0.0, ncub, ncell, full)
+    cspec = dict(ms["spec"], moves=[("BINSERT", 1.0), ("BDELETE", 1.0), ("RTRANSLATE", 1.0)])
+    cb = G.GcmcBatch(ctx, cspec, fug, seeds, maxN=512, maps=ms["maps"], precision=H.MH_PREC_FP64)
+    cb.set_cavity(ncub, [20], [0], tab=tab, species_mask=1)
+    cb.run(steps // 4)
+    c0 = cb.state()
+    t = time.perf_counter()
+    cb.run(steps)
+    dt = time.perf_counter() - t
+    c1s = cb.state()
+    cb.close()
+    out["cavity_bias"] = dict(
+        workload="the same 8 state points with CAVITY_BIAS BINSERT / BDELETE (20 trials), translate 1:1:1, "
+                 "cubes %s tabulated from the map below 0 kcal/mol (%d of %d), %d steps after %d" %
+                 ("x".join(str(v) for v in ncub), int(tab.sum()), tab.size, steps, steps // 4),
+        fp64_batched_8pt_steps_per_s=8 * steps / dt,
+        fp64_batched_8pt_attempted_moves_per_s=int((c1s["att"] - c0["att"]).sum()) / dt,
+        mean_loading=[float(x) for x in (c1s["sumN"] - c0["sumN"]) / steps])
+    log("c1 cavity bias (20 trials): batched %.3e moves/s" %
+        out["cavity_bias"]["fp64_batched_8pt_attempted_moves_per_s"])
     tot, att = 0.0, 0
     for r in range(8):
         one = G.GcmcBatch(ctx, ms["spec"], fug[r:r + 1], seeds[r:r + 1], maxN=512, maps=ms["maps"],
