@@ -24,6 +24,16 @@
  *   rigidmoves_bdelete    rigidmoves.F90:1578-1654 (bias of the molecule's cubelet)
  *   bmap_cubeinit / bmap_getbiasindex / bmap_getbiaspt / bmap_getcellwt
  *                         bmap.F90:181-636
+ *   rigidmoves_getConfigFromCavBias  rigidmoves.F90:1987-2126 (cavity bias:
+ *                         ntrials x (3 or 7 centre draws, +3 if multi-site),
+ *                         then one selection draw when a trial succeeded)
+ *   cavitylist_getIndexVec / _checkcavityfromIndex / _updateMolecInfo
+ *                         cavitylist.F90:351-546, 1137-1157; stats_update
+ *                         stats.F90:152-195
+ *   Cavity bias is pinned by what the mathematics fixes (occupancy recounts,
+ *   the running average's closed form, the ideal-gas mean in a tabulated
+ *   quarter cell: tests/test_cavity_oracle.py); no reference output covers
+ *   it.
  * Molecule energy = map terms (interpolate_int per site, factor q for the
  * emap) + LJ pairs with every other molecule (lj_snglint, nderivs = 0) +
  * Ewald terms.  Failure (LJ locut, untabulated map corner, U > max_nrg)
