@@ -11,7 +11,8 @@ cube index puts them elsewhere), and the grand canonical mean of an ideal
 gas confined by the tabulation to a quarter of the cell: (1 - (3/4)^T)
 e^B / 4 only when the acceptance carries cav_prob / ntrials on insertion and
 its inverse on deletion (without it the mean is 4 times larger, inverted 4
-times smaller)."""
+times smaller), and MAP_CAVITY_BIAS for an ideal gas in a map field at the
+unbiased mean e^B <exp(-U/kT)>."""
 import numpy as np
 
 from oracle import oracle as O
@@ -136,3 +137,49 @@ def test_ideal_gas_in_a_quarter_of_the_cell():
     lam = (1.0 - 0.75 ** 10) * eB / 4
     assert lam > 20
     assert abs(m - lam) < 4 * se + 0.01 * lam, (m, se, lam)
+
+
+def _wave_map(n=25, amp=1.0):
+    """A tabulated map over the cell with V = amp cos(2 pi x / L) and its x
+    derivative (the other columns zero), in the map-file layout."""
+    npn = (n, n, n)
+    step = L / (n - 1)
+    nf = n ** 3
+    ptr = np.arange(1, nf + 1, dtype=np.int32)
+    x = (np.arange(nf) // (n * n)) * step[0]
+    vals = np.zeros((8, nf), dtype=np.float32)
+    vals[0] = amp * np.cos(2 * np.pi * x / L[0])
+    vals[1] = -amp * (2 * np.pi / L[0]) * np.sin(2 * np.pi * x / L[0])
+    return npn, step, ptr, vals
+
+
+def test_map_cavity_ideal_gas_mean():
+    """MAP_CAVITY_BIAS for non-interacting molecules in a map field: nearly
+    every insertion finds a cavity (small cubes, every cube tabulated), so
+    the mean loading is e^B <exp(-U/kT)> as for unbiased insertion — only
+    when the map bias of the chosen trial enters with cav_prob / ntrials
+    (dropping the map bias shifts the mean by the field's Boltzmann
+    weights)."""
+    npn, step, ptr, vals = _wave_map()
+    bias = O.Bmap(npn, step, L, ptr, vals, 300.0)
+    fug = 1.0e5
+    eB = np.exp(O.adams_B(fug, float(np.prod(L)), 300.0))
+    rng = np.random.default_rng(8)
+    ok, e = O.pothermite(npn, step, ptr, vals, rng.random((20000, 3)) * L, fund_ell=L)
+    assert ok.all()
+    lam = eB * np.mean(np.exp(-e / (8.314e-3 / 4.184 * 300.0)))
+    cav = dict(n=(40, 40, 40), tab=None, mask=1)
+    spec = dict(full_ell=L, fund_ell=L, ref=[[0.0, 0.0, 0.0]], stype=[0], sq=[0.0], nst=1,
+                lj=([0.0], [0.0], [0.0], [0.0], [0]), T=300.0, maps=[(npn, step, ptr, vals)], site_pmap=[0],
+                site_emap=[-1], bias=bias, cavity=cav, cav_ntrials=10, cav_map=1,
+                moves=[("BINSERT", 1.0), ("BDELETE", 1.0)])
+    means = []
+    for seed in range(6):
+        g = O.Gcmc(spec, fug, 300 + seed, maxN=512)
+        g.run(20000)
+        s0, n0 = g.st.sumN, g.st.nsteps
+        g.run(150000)
+        means.append((g.st.sumN - s0) / (g.st.nsteps - n0))
+    m, se = np.mean(means), np.std(means, ddof=1) / np.sqrt(len(means))
+    assert lam > 20
+    assert abs(m - lam) < 4 * se + 0.02 * lam, (m, se, lam)
